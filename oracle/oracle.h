@@ -1,0 +1,104 @@
+/*
+ * ORACLE -- test infrastructure only (see oracle_math.h).
+ * Declarations of the oracle's C entry points (bound from Python with ctypes).
+ */
+#ifndef B3ORACLE_H
+#define B3ORACLE_H
+
+#include "oracle_math.h"
+
+typedef struct b3oShapes
+{
+	const b3hCollidable* collidables;
+	const b3hConvexPolyhedronData* convexes;
+	const b3hFloat4* vertices;
+	const b3hFloat4* uniqueEdges;
+	const b3hGpuFace* faces;
+	const int* indices;
+} b3oShapes;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void b3o_compute_world_aabbs(int n, const b3hRigidBodyData* bodies, const b3hCollidable* cols,
+							 const b3hAabb* localAabbs, b3hAabb* worldAabbs);
+void b3o_integrate(int n, b3hRigidBodyData* bodies, float dt, float angDamp, const float* grav3);
+void b3o_sort_keys(int n, b3hSortData* kv);
+int b3o_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
+				   const float* cellSize3, const int* dims3, int maxBodiesPerCell,
+				   b3hInt4* pairsOut, int maxPairs, int* totalFound);
+int b3o_sap_axis(const b3hAabb* all, const int* smallMap, int nSmall);
+int b3o_sap_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
+				  int axisOverride, b3hInt4* pairsOut, int maxPairs, int* totalFound, int* axisOut);
+int b3o_brute_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
+					b3hInt4* pairsOut, int maxPairs);
+
+/* narrowphase: returns number of contacts; pairs[i].z receives the contact slot
+ * or -1.  sepOut (optional, nPairs) receives the SAT state/normal per pair. */
+int b3o_mpr_contacts(b3hInt4* pairs, int nPairs, const b3hRigidBodyData* bodies,
+					 const b3hCollidable* cols, const b3hConvexPolyhedronData* convexes,
+					 const b3hFloat4* vertices, b3hContact4Data* contacts, int capacity);
+int b3o_narrowphase(b3hInt4* pairs, int nPairs, const b3hRigidBodyData* bodies,
+					const b3hCollidable* cols, const b3hConvexPolyhedronData* convexes,
+					const b3hFloat4* vertices, const b3hFloat4* uniqueEdges, const b3hGpuFace* faces,
+					const int* indices, b3hContact4Data* contacts, int capacity,
+					int* hasSepOut, b3hFloat4* sepOut);
+
+/* solver front end + PGS (contacts in, bodies updated in place) */
+int b3o_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaData* inertias,
+					   int numContacts, const b3hContact4Data* contactsIn, const b3hSolverParams* params,
+					   int staticIdx, b3hContact4Data* sortedContactsOut, b3hContactConstraint4* constraintsOut,
+					   unsigned* cellCountsOut, unsigned* cellOffsetsOut, int* batchSizesOut,
+					   const float* initialImpulses, float* residualsOut);
+
+#ifdef __cplusplus
+}
+#endif
+
+/* ---- whole step (b3GpuRigidBodyPipeline::stepSimulation,
+ *      b3GpuRigidBodyPipeline.cpp:221-463) ---- */
+typedef struct b3oWorld
+{
+	int numBodies;
+	b3hRigidBodyData* bodies;
+	const b3hInertiaData* inertias;
+	const b3hCollidable* collidables;
+	const b3hConvexPolyhedronData* convexes;
+	const b3hFloat4* vertices;
+	const b3hFloat4* uniqueEdges;
+	const b3hGpuFace* faces;
+	const int* indices;
+	const b3hAabb* localAabbs;
+	const int* smallMap;
+	int nSmall;
+	const int* largeMap;
+	int nLarge;
+	int broadphase; /* 0 uniform grid, 1 SAP */
+	float cellSize[3];
+	int gridDims[3];
+	int maxBodiesPerCell;
+	int maxPairs;
+	int maxContacts;
+	b3hSolverParams solver;
+	float gravity[3];
+	int staticIdx;
+	/* caller-allocated scratch: numBodies AABBs, maxPairs pairs, maxContacts contacts */
+	b3hAabb* worldAabbs;
+	b3hInt4* pairs;
+	b3hContact4Data* contacts;
+	/* outputs */
+	int numPairs;
+	int numContacts;
+	float* residuals; /* optional, 2*iterations */
+} b3oWorld;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+int b3o_step(b3oWorld* w, float dt);
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,103 @@
+/*
+ * ORACLE -- test infrastructure only.  Scalar CPU restatement of the Bullet3
+ * GPU rigid-body step, used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker.  Nothing in the product links it.
+ *
+ * Float conventions (identical to the device code so results can be compared
+ * bit for bit): no FMA contraction (built with -ffp-contract=off), left-to-right
+ * evaluation, dot3 = (x*x' + y*y') + z*z', cross3 as in the OpenCL builtin,
+ * correctly rounded / and sqrtf, and the short polynomial sine/cosine below.
+ */
+#ifndef B3ORACLE_MATH_H
+#define B3ORACLE_MATH_H
+
+#include <math.h>
+#include <float.h>
+#include <stdint.h>
+#include <string.h>
+#include "../include/b3hip_types.h"
+
+typedef b3hFloat4 F4;
+
+static inline F4 f4(float x, float y, float z, float w) { F4 r; r.x = x; r.y = y; r.z = z; r.w = w; return r; }
+static inline F4 add4(F4 a, F4 b) { return f4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+static inline F4 sub4(F4 a, F4 b) { return f4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
+static inline F4 mul4(F4 a, float s) { return f4(a.x * s, a.y * s, a.z * s, a.w * s); }
+static inline F4 smul4(float s, F4 a) { return f4(s * a.x, s * a.y, s * a.z, s * a.w); }
+static inline F4 neg4(F4 a) { return f4(-a.x, -a.y, -a.z, -a.w); }
+static inline float dot3(F4 a, F4 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline float dot4(F4 a, F4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+static inline F4 cross3(F4 a, F4 b)
+{
+	return f4(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x, 0.f);
+}
+static inline F4 xyz0(F4 a) { return f4(a.x, a.y, a.z, 0.f); }
+
+/* b3QuatMul / qtMul: src/Bullet3Common/shared/b3Quat.h:43-51 */
+static inline F4 qmul(F4 a, F4 b)
+{
+	F4 c = cross3(a, b);
+	F4 r;
+	r.x = c.x + (a.w * b.x + b.w * a.x);
+	r.y = c.y + (a.w * b.y + b.w * a.y);
+	r.z = c.z + (a.w * b.z + b.w * a.z);
+	r.w = a.w * b.w - dot3(a, b);
+	return r;
+}
+static inline F4 qinv(F4 q) { return f4(-q.x, -q.y, -q.z, q.w); }
+/* b3QuatRotate / qtRotate: src/Bullet3Common/shared/b3Quat.h:68-75 */
+static inline F4 qrot(F4 q, F4 v)
+{
+	return qmul(qmul(q, xyz0(v)), qinv(q));
+}
+static inline F4 qinvrot(F4 q, F4 v) { return qrot(qinv(q), v); }
+/* b3TransformPoint / transform: src/Bullet3Common/shared/b3Quat.h:87-90 */
+static inline F4 xform(F4 p, F4 pos, F4 q) { return add4(qrot(q, p), pos); }
+
+/* normalize of a 3-vector (w forced to 0) */
+static inline F4 nrm3(F4 v)
+{
+	float l2 = dot3(v, v);
+	float k = 1.f / sqrtf(l2);
+	return f4(v.x * k, v.y * k, v.z * k, 0.f);
+}
+
+/* Sine and cosine for |x| <= pi/8 (the integration step clamps the half angle
+ * to that range): Taylor polynomials to x^9 / x^8, evaluated in Horner form.
+ * Truncation error < 1e-10 on the range, i.e. below float resolution. */
+static inline float sin_small(float x)
+{
+	float x2 = x * x;
+	float p = -1.f / 5040.f + x2 * (1.f / 362880.f);
+	p = 1.f / 120.f + x2 * p;
+	p = -1.f / 6.f + x2 * p;
+	return x + (x * x2) * p;
+}
+static inline float cos_small(float x)
+{
+	float x2 = x * x;
+	float p = -1.f / 720.f + x2 * (1.f / 40320.f);
+	p = 1.f / 24.f + x2 * p;
+	p = -0.5f + x2 * p;
+	return 1.f + x2 * p;
+}
+
+static inline int f2i(float f) { int i; memcpy(&i, &f, 4); return i; }
+static inline float i2f(int i) { float f; memcpy(&f, &i, 4); return f; }
+static inline unsigned f2u(float f) { unsigned i; memcpy(&i, &f, 4); return i; }
+
+/* mtMul1: matrix * column vector (solveContact.cl:141-148) */
+static inline F4 mtMul1(const b3hMat3x3& a, F4 b)
+{
+	return f4(dot3(a.m_row[0], b), dot3(a.m_row[1], b), dot3(a.m_row[2], b), 0.f);
+}
+/* mtMul3: row vector * matrix (b3Mat3x3.h:150-161) */
+static inline F4 mtMul3(F4 a, const b3hMat3x3& b)
+{
+	F4 colx = f4(b.m_row[0].x, b.m_row[1].x, b.m_row[2].x, 0);
+	F4 coly = f4(b.m_row[0].y, b.m_row[1].y, b.m_row[2].y, 0);
+	F4 colz = f4(b.m_row[0].z, b.m_row[1].z, b.m_row[2].z, 0);
+	return f4(dot3(a, colx), dot3(a, coly), dot3(a, colz), 0.f);
+}
+
+#endif

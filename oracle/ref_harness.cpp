@@ -1,0 +1,122 @@
+/*
+ * ORACLE -- test infrastructure only.
+ * Harness that exposes pieces of the reference's own C++ code (compiled from
+ * /root/reference by oracle/ref.mk into oracle/_ref/libb3ref.so) through a C
+ * interface, so tests can pin the oracle against them:
+ *   b3MprPenetration      src/Bullet3Collision/NarrowPhaseCollision/shared/b3MprPenetration.h:816-886
+ *   integrateSingleTransform  src/Bullet3Dynamics/shared/b3IntegrateTransforms.h:5-55
+ *   b3ComputeWorldAabb    src/Bullet3Collision/NarrowPhaseCollision/shared/b3UpdateAabbs.h:8-33
+ *   setConstraint4        src/Bullet3Dynamics/shared/b3ConvertConstraint4.h:62-148
+ *   b3ConvexUtility::initializePolyhedralFeatures  src/Bullet3Collision/NarrowPhaseCollision/b3ConvexUtility.cpp:26-248
+ * The headers are the reference's dual-compile ("shared") headers, built here
+ * as the C++ they are written to be.
+ */
+#include "Bullet3Common/b3Vector3.h"
+#include "Bullet3Common/b3Quaternion.h"
+#include "Bullet3Common/b3Matrix3x3.h"
+#include "Bullet3Collision/NarrowPhaseCollision/shared/b3RigidBodyData.h"
+#include "Bullet3Collision/NarrowPhaseCollision/shared/b3Collidable.h"
+#include "Bullet3Collision/NarrowPhaseCollision/shared/b3ConvexPolyhedronData.h"
+#include "Bullet3Collision/NarrowPhaseCollision/shared/b3MprPenetration.h"
+#include "Bullet3Collision/NarrowPhaseCollision/shared/b3UpdateAabbs.h"
+#include "Bullet3Collision/NarrowPhaseCollision/shared/b3Contact4Data.h"
+#include "Bullet3Dynamics/shared/b3IntegrateTransforms.h"
+#include "Bullet3Dynamics/shared/b3ConvertConstraint4.h"
+#include "Bullet3Collision/NarrowPhaseCollision/b3ConvexUtility.h"
+#include <string.h>
+
+extern "C" {
+
+/* per pair: res (0 contact, -1 none), depth, dir[4], pos[4] */
+int ref_mpr(const int* pairs4, int nPairs, const void* bodies, const void* cols, const void* convexes,
+			const void* verts, int* resOut, float* depthOut, float* dirOut, float* posOut)
+{
+	const b3RigidBodyData* b = (const b3RigidBodyData*)bodies;
+	const b3Collidable* c = (const b3Collidable*)cols;
+	const b3ConvexPolyhedronData* h = (const b3ConvexPolyhedronData*)convexes;
+	const b3Float4* v = (const b3Float4*)verts;
+	b3AlignedObjectArray<b3Float4> sep;
+	sep.resize(nPairs);
+	b3AlignedObjectArray<int> has;
+	has.resize(nPairs);
+	int count = 0;
+	for (int i = 0; i < nPairs; i++)
+	{
+		int a = pairs4[4 * i], bb = pairs4[4 * i + 1];
+		resOut[i] = -1;
+		if (b[a].m_invMass == 0 && b[bb].m_invMass == 0) continue;
+		if (c[b[a].m_collidableIdx].m_shapeType != SHAPE_CONVEX_HULL || c[b[bb].m_collidableIdx].m_shapeType != SHAPE_CONVEX_HULL) continue;
+		float depth = 0;
+		b3Float4 dir, pos;
+		int r = b3MprPenetration(i, a, bb, b, h, c, v, &sep[0], &has[0], &depth, &dir, &pos);
+		resOut[i] = r;
+		if (r == 0)
+		{
+			count++;
+			depthOut[i] = depth;
+			for (int k = 0; k < 4; k++)
+			{
+				dirOut[4 * i + k] = dir[k];
+				posOut[4 * i + k] = pos[k];
+			}
+		}
+	}
+	return count;
+}
+
+void ref_integrate(void* bodies, int n, float dt, float damp, const float* g3)
+{
+	b3RigidBodyData* b = (b3RigidBodyData*)bodies;
+	b3Vector3 g = b3MakeVector3(g3[0], g3[1], g3[2]);
+	for (int i = 0; i < n; i++) integrateSingleTransform(b, i, dt, damp, g);
+}
+
+void ref_world_aabbs(int n, const void* bodies, const void* cols, const void* local, void* world)
+{
+	for (int i = 0; i < n; i++)
+		b3ComputeWorldAabb(i, (const b3RigidBodyData*)bodies, (const b3Collidable*)cols, (const b3Aabb*)local, (b3Aabb*)world);
+}
+
+/* one contact -> one constraint (inverse inertias given as 3x4 row blocks) */
+void ref_set_constraint4(const void* bodyA, const void* invInertiaA, const void* bodyB, const void* invInertiaB,
+						 const void* contact, float dt, float drift, float coeff, void* out)
+{
+	const b3RigidBodyData* A = (const b3RigidBodyData*)bodyA;
+	const b3RigidBodyData* B = (const b3RigidBodyData*)bodyB;
+	b3ContactConstraint4_t cs;
+	memset(&cs, 0, sizeof(cs));
+	setConstraint4(A->m_pos, A->m_linVel, A->m_angVel, A->m_invMass, *(const b3Matrix3x3*)invInertiaA,
+				   B->m_pos, B->m_linVel, B->m_angVel, B->m_invMass, *(const b3Matrix3x3*)invInertiaB,
+				   (b3Contact4Data*)contact, dt, drift, coeff, &cs);
+	memcpy(out, &cs, sizeof(cs));
+}
+
+/* Hull features of a point cloud.  Writes counts[0..3] = numVertices,
+ * numFaces, numIndices, numUniqueEdges; arrays sized by the caller
+ * (vertices 4*cap, planes 4*cap, faceIndexCounts cap, indices cap, edges 4*cap). */
+int ref_hull(const float* pts3, int n, int cap, int* counts, float* vertsOut, float* planesOut, int* faceCountsOut,
+			 int* indicesOut, float* edgesOut)
+{
+	b3AlignedObjectArray<b3Vector3> v;
+	for (int i = 0; i < n; i++) v.push_back(b3MakeVector3(pts3[3 * i], pts3[3 * i + 1], pts3[3 * i + 2]));
+	b3ConvexUtility u;
+	u.initializePolyhedralFeatures(&v[0], n, true);
+	int nv = u.m_vertices.size(), nf = u.m_faces.size(), ne = u.m_uniqueEdges.size();
+	int ni = 0;
+	for (int f = 0; f < nf; f++) ni += u.m_faces[f].m_indices.size();
+	counts[0] = nv; counts[1] = nf; counts[2] = ni; counts[3] = ne;
+	if (nv > cap || nf > cap || ni > cap || ne > cap) return -1;
+	for (int i = 0; i < nv; i++)
+		for (int k = 0; k < 4; k++) vertsOut[4 * i + k] = k < 3 ? u.m_vertices[i][k] : 0.f;
+	int w = 0;
+	for (int f = 0; f < nf; f++)
+	{
+		for (int k = 0; k < 4; k++) planesOut[4 * f + k] = u.m_faces[f].m_plane[k];
+		faceCountsOut[f] = u.m_faces[f].m_indices.size();
+		for (int j = 0; j < u.m_faces[f].m_indices.size(); j++) indicesOut[w++] = u.m_faces[f].m_indices[j];
+	}
+	for (int i = 0; i < ne; i++)
+		for (int k = 0; k < 4; k++) edgesOut[4 * i + k] = k < 3 ? u.m_uniqueEdges[i][k] : 0.f;
+	return 0;
+}
+}
