@@ -1,0 +1,163 @@
+"""Benchmark: Bullet3 GPU rigid-body step on one MI355X.
+
+Workload (default): the single large pile -- 131072 dynamic convex bodies (one
+third unit boxes, 16- and 32-point hulls), a static ground box, uniform-grid
+broadphase with cell 2*sqrt(3) m, 10 PGS iterations, dt 1/60.  Warmup steps let
+the lattice collapse into a pile (and capture the step into a HIP graph); then
+exactly --steps steps are timed.  A "step" is one full stepSimulation: world
+AABBs, broadphase, narrowphase, solver set-up, 2x10 PGS sweeps, integration.
+
+Prints one JSON line (metric body-steps/s).  roofline: the stage with the
+largest share of the step, its algorithmic HBM bytes (records read once and
+written once, per the byte counts below) over its duration measured with HIP
+events on the pipeline's stream.  cpu_baseline: the scalar oracle (same
+algorithm, one core) stepped from the GPU's post-warmup state for ~10 s.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+STAGES = ["aabb", "broadphase", "narrowphase", "solver", "pgs+setup", "integrate"]
+
+
+def stage_bytes(n, p, c, k):
+    """algorithmic bytes per stage (record sizes: body 80, AABB 32, pair 16,
+    contact 112, constraint 176, inertia 48 per matrix)"""
+    return {
+        "aabb": 112 * n,                       # body 80 + local AABB 32 read (write 32 counted in broadphase input)
+        "broadphase": 32 * n + 16 * p,         # AABBs read, pairs written
+        "narrowphase": 80 * p + 112 * c,       # pair + 2 body poses read, contacts written
+        "pgs+setup": 556 * c + 2 * k * (192 * c + 128 * n),
+        "integrate": 160 * n,
+    }
+
+
+def run(args):
+    from bullet3_amd import capi, scenes
+    from bullet3_amd.world import World
+
+    if capi.device_count() < 1:
+        raise SystemExit("bench.py needs a HIP device")
+    if args.config == "pile":
+        sc = scenes.convex_pile(args.nx, args.ny, args.nz)
+    else:
+        sc = scenes.cube_grid(10, 10, 10)
+    n_dyn = int((sc.mass > 0).sum())
+    t0 = time.time()
+    w = World(sc, broadphase="grid")
+    sp = w.solver_params()
+    sp.m_numIterations = args.iters
+    if args.friction is not None:
+        sp.m_frictionCoeff = args.friction
+    w.set_solver(sp)
+    setup_s = time.time() - t0
+    dt = 1.0 / 60.0
+    for _ in range(args.warmup):
+        w.step_graph(dt, 1)
+    w.sync()
+    # timed region: exactly args.steps graph-launched steps
+    w.sync()
+    t0 = time.perf_counter()
+    w.step_graph(dt, args.steps)
+    w.sync()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    value = n_dyn * args.steps / sec
+    ms_per_step = 1000.0 * sec / args.steps
+    # per-stage breakdown from HIP events (eager steps after the timed region)
+    w.enable_timing(True)
+    reps = 5
+    acc = np.zeros(6)
+    for _ in range(reps):
+        w.step(dt)
+        acc += w.stage_times()
+    st = acc / reps
+    times = {"aabb": st[0], "broadphase": st[1], "narrowphase": st[2], "pgs+setup": st[3] + st[4], "integrate": st[5]}
+    n_pairs = len(w.pairs())
+    n_contacts = len(w.contacts())
+    sb = stage_bytes(sc.num_bodies, n_pairs, n_contacts, args.iters)
+    dom = max(times, key=lambda s: times[s])
+    achieved = sb[dom] / (times[dom] * 1e-3) / 1e9
+    total_bytes = sum(sb.values())
+    out = {
+        "metric": "body-steps/s at 128k convexes",
+        "value": round(value, 1),
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "convex pile %d dynamic bodies (boxes, 16/32-point hulls), grid broadphase, "
+                               "%d PGS iterations, dt 1/60" % (n_dyn, args.iters),
+                   "bodies": n_dyn, "pairs": n_pairs, "contacts": n_contacts, "pgs_iterations": args.iters,
+                   "setup_s": round(setup_s, 2)},
+        "stages_ms": {k: round(float(v), 4) for k, v in times.items()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "step_algorithmic_bytes": int(total_bytes),
+                     "step_frac": round(total_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, args)
+        if out["cpu_baseline"]["value"] > 0:
+            out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    w.close()
+    return out
+
+
+def cpu_baseline(w, args):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import OracleWorld
+    ow = OracleWorld(w)
+    dt = 1.0 / 60.0
+    n_dyn = int((ow.bodies["invMass"] > 0).sum())
+    t0 = time.perf_counter()
+    ow.step(dt)
+    one = time.perf_counter() - t0
+    steps = max(1, int(args.cpu_seconds / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ow.step(dt)
+    sec = time.perf_counter() - t0
+    return {"value": round(n_dyn * steps / sec, 1), "unit": "body-steps/s", "cores": 1, "kind": "port",
+            "sample": "%d oracle steps of the same pile from the GPU's post-warmup state (%.1f s)" % (steps, sec)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--iters", type=int, default=10)
+    ap.add_argument("--friction", type=float, default=None)
+    ap.add_argument("--config", default="pile", choices=["pile", "cubes"])
+    ap.add_argument("--nx", type=int, default=64)
+    ap.add_argument("--ny", type=int, default=32)
+    ap.add_argument("--nz", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (the step shards by scene or "
+              "spatial slab, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    print(json.dumps(run(args)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1373 @@
+/*
+ * b3h_api.cpp -- host side of the C ABI: the narrowphase shape/body registry,
+ * the two broadphases and the pipeline that chains the device stages.
+ *
+ * Mirrors the reference classes' host logic:
+ *   b3GpuNarrowPhase       src/Bullet3OpenCL/RigidBody/b3GpuNarrowPhase.cpp
+ *   b3GpuGridBroadphase    src/Bullet3OpenCL/BroadphaseCollision/b3GpuGridBroadphase.cpp
+ *   b3GpuSapBroadphase     src/Bullet3OpenCL/BroadphaseCollision/b3GpuSapBroadphase.cpp
+ *   b3GpuRigidBodyPipeline src/Bullet3OpenCL/RigidBody/b3GpuRigidBodyPipeline.cpp
+ * Device buffers are allocated once at the configured capacities (as the
+ * reference's b3OpenCLArrays are), so a step allocates nothing and can be
+ * captured into a HIP graph.  A step never waits on the host: counts produced
+ * by one stage (pairs, contacts) stay on the device and size the next stage.
+ */
+#include "b3h_internal.h"
+#include "hull.h"
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+
+static thread_local char g_err[512] = "";
+
+void b3h_set_error(const char* fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_err, sizeof(g_err), fmt, ap);
+	va_end(ap);
+}
+
+extern "C" const char* b3h_last_error(void) { return g_err; }
+extern "C" const char* b3h_version(void) { return "b3hip 0.1 (gfx950)"; }
+
+#define CHECK_ARG(c)                                   \
+	do                                                 \
+	{                                                  \
+		if (!(c))                                      \
+		{                                              \
+			b3h_set_error("invalid argument: %s", #c); \
+			return B3H_ERR_INVALID;                    \
+		}                                              \
+	} while (0)
+
+template <typename T>
+static int dalloc(T** p, size_t n)
+{
+	*p = nullptr;
+	if (n == 0) n = 1;
+	hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+	if (e != hipSuccess)
+	{
+		b3h_set_error("hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+		*p = nullptr;
+		return B3H_ERR_HIP;
+	}
+	return B3H_OK;
+}
+template <typename T>
+static void dfree(T*& p)
+{
+	if (p) hipFree((void*)p);
+	p = nullptr;
+}
+
+static int ensure_device()
+{
+	int n = 0;
+	hipError_t e = hipGetDeviceCount(&n);
+	if (e != hipSuccess || n <= 0)
+	{
+		b3h_set_error("no HIP device available (hipGetDeviceCount: %s, count %d)", hipGetErrorString(e), n);
+		return B3H_ERR_NO_DEVICE;
+	}
+	return B3H_OK;
+}
+
+static int make_stream(void* user, hipStream_t* s, bool* own)
+{
+	if (user)
+	{
+		*s = (hipStream_t)user;
+		*own = false;
+		return B3H_OK;
+	}
+	B3H_CHECK(hipStreamCreateWithFlags(s, hipStreamNonBlocking));
+	*own = true;
+	return B3H_OK;
+}
+
+extern "C" int b3h_config_default(b3hConfig* c)
+{
+	CHECK_ARG(c);
+	c->m_maxConvexBodies = 128 * 1024;
+	c->m_maxVerticesPerFace = 64;
+	c->m_maxFacesPerShape = 12;
+	c->m_maxConvexVertices = 8192;
+	c->m_maxConvexIndices = 81920;
+	c->m_maxConvexUniqueEdges = 8192;
+	c->m_maxCompoundChildShapes = 8192;
+	c->m_maxTriConvexPairCapacity = 256 * 1024;
+	c->m_maxConvexShapes = c->m_maxConvexBodies;
+	c->m_maxBroadphasePairs = 16 * c->m_maxConvexBodies;
+	c->m_maxContactCapacity = c->m_maxBroadphasePairs;
+	c->m_compoundPairCapacity = 1024 * 1024;
+	return B3H_OK;
+}
+
+extern "C" int b3h_solver_params_default(b3hSolverParams* p)
+{
+	CHECK_ARG(p);
+	p->m_solverDt = 1.f / 60.f;
+	p->m_numIterations = 4;
+	p->m_frictionCoeff = 0.7f;
+	p->m_positionDrift = 0.005f;
+	p->m_positionConstraintCoeff = 0.2f;
+	p->m_batchCellSize = 6.f;
+	p->m_angularDamping = 0.99f;
+	p->m_warmStartMode = 0;
+	return B3H_OK;
+}
+
+/* unit-sphere directions live in constant memory of the narrowphase module */
+int b3h_np_module_init();
+
+/* ======================================================================= */
+struct b3hNarrowPhase
+{
+	b3hConfig cfg;
+	hipStream_t stream;
+	bool ownStream;
+	std::vector<b3hCollidable> cols;
+	std::vector<b3hConvexPolyhedronData> convexes;
+	std::vector<b3hFloat4> verts, edges;
+	std::vector<b3hGpuFace> faces;
+	std::vector<int> indices;
+	std::vector<b3hAabb> localAabbs;
+	std::vector<b3hRigidBodyData> bodies;
+	std::vector<b3hInertiaData> inertias;
+	int static0Index;
+	/* device */
+	b3hRigidBodyData* dBodies;
+	b3hInertiaData* dInertias;
+	b3hCollidable* dCols;
+	b3hConvexPolyhedronData* dConvex;
+	b3hFloat4 *dVerts, *dEdges;
+	b3hGpuFace* dFaces;
+	int* dIndices;
+	b3hAabb* dLocalAabbs;
+	size_t capCols, capConvex, capVerts, capEdges, capFaces, capIndices, capLocal;
+	b3hContact4Data* dStaging;
+	int *dFlags, *dFlagScan, *dContactPair, *dNumContacts, *dNumPairs;
+	unsigned* dScanTmp;
+	b3hContact4Data* dContacts;
+	int* dSepState;
+	b3hFloat4* dSepNormal;
+	bool contactsValid;
+	int lanesPerPair;
+};
+
+static int np_alloc_work(b3hNarrowPhase* np)
+{
+	int mp = np->cfg.m_maxBroadphasePairs, mc = np->cfg.m_maxContactCapacity;
+	int r;
+	if ((r = dalloc(&np->dBodies, np->cfg.m_maxConvexBodies))) return r;
+	if ((r = dalloc(&np->dInertias, np->cfg.m_maxConvexBodies))) return r;
+	if ((r = dalloc(&np->dStaging, mp))) return r;
+	if ((r = dalloc(&np->dFlags, mp + 1))) return r;
+	if ((r = dalloc(&np->dFlagScan, mp + 2))) return r;
+	if ((r = dalloc(&np->dScanTmp, b3h_scan_tmp_size(mp)))) return r;
+	if ((r = dalloc(&np->dContactPair, mc))) return r;
+	if ((r = dalloc(&np->dNumContacts, 2))) return r;
+	if ((r = dalloc(&np->dNumPairs, 1))) return r;
+	if ((r = dalloc(&np->dContacts, mc))) return r;
+	if ((r = dalloc(&np->dSepState, mp))) return r;
+	if ((r = dalloc(&np->dSepNormal, mp))) return r;
+	B3H_CHECK(hipMemset(np->dNumContacts, 0, 2 * sizeof(int)));
+	return B3H_OK;
+}
+
+extern "C" int b3h_np_create(const b3hConfig* cfg, void* stream, b3hNarrowPhase** out)
+{
+	CHECK_ARG(cfg && out);
+	*out = nullptr;
+	int r = ensure_device();
+	if (r) return r;
+	CHECK_ARG(cfg->m_maxConvexBodies > 0 && cfg->m_maxBroadphasePairs > 0 && cfg->m_maxContactCapacity > 0);
+	b3hNarrowPhase* np = new b3hNarrowPhase();
+	memset((void*)&np->cfg, 0, sizeof(np->cfg));
+	np->cfg = *cfg;
+	np->static0Index = -1;
+	np->dBodies = nullptr;
+	np->dInertias = nullptr;
+	np->dCols = nullptr;
+	np->dConvex = nullptr;
+	np->dVerts = np->dEdges = nullptr;
+	np->dFaces = nullptr;
+	np->dIndices = nullptr;
+	np->dLocalAabbs = nullptr;
+	np->capCols = np->capConvex = np->capVerts = np->capEdges = np->capFaces = np->capIndices = np->capLocal = 0;
+	np->dStaging = nullptr;
+	np->dFlags = np->dFlagScan = np->dContactPair = np->dNumContacts = np->dNumPairs = nullptr;
+	np->dScanTmp = nullptr;
+	np->dContacts = nullptr;
+	np->dSepState = nullptr;
+	np->dSepNormal = nullptr;
+	np->contactsValid = true;
+	np->lanesPerPair = 1;
+	if ((r = make_stream(stream, &np->stream, &np->ownStream)) || (r = np_alloc_work(np)) || (r = b3h_np_module_init()))
+	{
+		b3h_np_destroy(np);
+		return r;
+	}
+	*out = np;
+	return B3H_OK;
+}
+
+extern "C" int b3h_np_destroy(b3hNarrowPhase* np)
+{
+	if (!np) return B3H_OK;
+	hipStreamSynchronize(np->stream);
+	dfree(np->dBodies);
+	dfree(np->dInertias);
+	dfree(np->dCols);
+	dfree(np->dConvex);
+	dfree(np->dVerts);
+	dfree(np->dEdges);
+	dfree(np->dFaces);
+	dfree(np->dIndices);
+	dfree(np->dLocalAabbs);
+	dfree(np->dStaging);
+	dfree(np->dFlags);
+	dfree(np->dFlagScan);
+	dfree(np->dScanTmp);
+	dfree(np->dContactPair);
+	dfree(np->dNumContacts);
+	dfree(np->dNumPairs);
+	dfree(np->dContacts);
+	dfree(np->dSepState);
+	dfree(np->dSepNormal);
+	if (np->ownStream) hipStreamDestroy(np->stream);
+	delete np;
+	return B3H_OK;
+}
+
+extern "C" int b3h_np_register_convex_hull_shape(b3hNarrowPhase* np, const float* vertices, int strideInBytes,
+												  int numVertices, const float* scaling3, int* outCollidable)
+{
+	CHECK_ARG(np && vertices && numVertices > 0 && strideInBytes >= 12 && (strideInBytes % 4) == 0 && outCollidable);
+	*outCollidable = -1;
+	if ((int)np->cols.size() >= np->cfg.m_maxConvexShapes)
+	{
+		b3h_set_error("registerConvexHullShape: exceeding the number of shapes (%d)", np->cfg.m_maxConvexShapes);
+		return B3H_ERR_CAPACITY;
+	}
+	float sc[3] = {1.f, 1.f, 1.f};
+	if (scaling3)
+		for (int k = 0; k < 3; k++) sc[k] = scaling3[k];
+	std::vector<float> pts(3 * (size_t)numVertices);
+	int stride = strideInBytes / 4;
+	for (int i = 0; i < numVertices; i++)
+		for (int k = 0; k < 3; k++) pts[3 * i + k] = vertices[(size_t)i * stride + k] * sc[k];
+	b3hPolyhedron poly;
+	if (b3h_build_polyhedron(pts.data(), numVertices, 3, &poly) != 0)
+	{
+		b3h_set_error("registerConvexHullShape: degenerate point set (need >= 4 non-coplanar points)");
+		return B3H_ERR_INVALID;
+	}
+	int nv = (int)poly.vertices.size() / 3;
+	if ((int)np->verts.size() + nv > np->cfg.m_maxConvexVertices)
+	{
+		b3h_set_error("registerConvexHullShape: exceeding the number of hull vertices (%d)", np->cfg.m_maxConvexVertices);
+		return B3H_ERR_CAPACITY;
+	}
+	b3hConvexPolyhedronData cv;
+	memset(&cv, 0, sizeof(cv));
+	/* local centre = vertex mean (b3GpuNarrowPhase.cpp:330-336) */
+	float c[3] = {0.f, 0.f, 0.f};
+	for (int i = 0; i < nv; i++)
+		for (int k = 0; k < 3; k++) c[k] += poly.vertices[3 * i + k];
+	float inv = 1.f / (float)nv;
+	cv.m_localCenter = f4(c[0] * inv, c[1] * inv, c[2] * inv, 0.f);
+	cv.m_radius = 0.f;
+	cv.m_uniqueEdgesOffset = (int)np->edges.size();
+	cv.m_numUniqueEdges = (int)poly.uniqueEdges.size() / 3;
+	for (int i = 0; i < cv.m_numUniqueEdges; i++)
+		np->edges.push_back(f4(poly.uniqueEdges[3 * i], poly.uniqueEdges[3 * i + 1], poly.uniqueEdges[3 * i + 2], 0.f));
+	cv.m_faceOffset = (int)np->faces.size();
+	cv.m_numFaces = (int)poly.faces.size();
+	for (auto& f : poly.faces)
+	{
+		b3hGpuFace g;
+		memset(&g, 0, sizeof(g));
+		g.m_plane = f4(f.plane[0], f.plane[1], f.plane[2], f.plane[3]);
+		g.m_indexOffset = (int)np->indices.size();
+		g.m_numIndices = (int)f.indices.size();
+		for (int id : f.indices) np->indices.push_back(id);
+		np->faces.push_back(g);
+	}
+	cv.m_vertexOffset = (int)np->verts.size();
+	cv.m_numVertices = nv;
+	for (int i = 0; i < nv; i++)
+		np->verts.push_back(f4(poly.vertices[3 * i], poly.vertices[3 * i + 1], poly.vertices[3 * i + 2], 0.f));
+	b3hCollidable col;
+	col.m_numChildShapes = 0;
+	col.m_radius = 0.f;
+	col.m_shapeType = B3H_SHAPE_CONVEX_HULL;
+	col.m_shapeIndex = (int)np->convexes.size();
+	np->convexes.push_back(cv);
+	/* local AABB of the hull vertices (b3GpuNarrowPhase.cpp:341-365) */
+	float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+	for (int i = 0; i < nv; i++)
+		for (int k = 0; k < 3; k++)
+		{
+			float v = poly.vertices[3 * i + k];
+			mn[k] = v < mn[k] ? v : mn[k];
+			mx[k] = v > mx[k] ? v : mx[k];
+		}
+	b3hAabb la;
+	la.m_min = f4(mn[0], mn[1], mn[2], 0.f);
+	la.m_max = f4(mx[0], mx[1], mx[2], 0.f);
+	np->localAabbs.push_back(la);
+	*outCollidable = (int)np->cols.size();
+	np->cols.push_back(col);
+	return B3H_OK;
+}
+
+/* b3Matrix3x3::setRotation (scalar path, b3Matrix3x3.h:201-268) */
+static void quat_to_basis(const float q[4], float m[3][3])
+{
+	float d = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+	float s = 2.0f / d;
+	float xs = q[0] * s, ys = q[1] * s, zs = q[2] * s;
+	float wx = q[3] * xs, wy = q[3] * ys, wz = q[3] * zs;
+	float xx = q[0] * xs, xy = q[0] * ys, xz = q[0] * zs;
+	float yy = q[1] * ys, yz = q[1] * zs, zz = q[2] * zs;
+	m[0][0] = 1.0f - (yy + zz);
+	m[0][1] = xy - wz;
+	m[0][2] = xz + wy;
+	m[1][0] = xy + wz;
+	m[1][1] = 1.0f - (xx + zz);
+	m[1][2] = yz - wx;
+	m[2][0] = xz - wy;
+	m[2][1] = yz + wx;
+	m[2][2] = 1.0f - (xx + yy);
+}
+
+extern "C" int b3h_np_register_rigid_body(b3hNarrowPhase* np, int collidableIndex, float mass, const float* pos,
+										  const float* orn, const float* aabbMin, const float* aabbMax, int writeToGpu,
+										  int* outBody)
+{
+	CHECK_ARG(np && pos && orn && aabbMin && aabbMax && outBody);
+	*outBody = -1;
+	int idx = (int)np->bodies.size();
+	if (idx >= np->cfg.m_maxConvexBodies)
+	{
+		b3h_set_error("registerRigidBody: exceeding the number of rigid bodies, %d > %d", idx, np->cfg.m_maxConvexBodies);
+		return B3H_ERR_CAPACITY;
+	}
+	b3hRigidBodyData b;
+	memset(&b, 0, sizeof(b));
+	b.m_frictionCoeff = 1.f;
+	b.m_restituitionCoeff = 0.f;
+	b.m_angVel = f4(0, 0, 0, 0);
+	b.m_linVel = f4(0, 0, 0, 0);
+	b.m_pos = f4(pos[0], pos[1], pos[2], 0.f);
+	b.m_quat = f4(orn[0], orn[1], orn[2], orn[3]);
+	b.m_collidableIdx = collidableIndex;
+	b.m_invMass = mass ? 1.f / mass : 0.f;
+	b3hInertiaData in;
+	memset(&in, 0, sizeof(in));
+	if (mass == 0.f)
+	{
+		if (idx == 0) np->static0Index = 0;
+	}
+	else
+	{
+		/* inertia of a box with edges 2*(aabbMax-aabbMin) (b3GpuNarrowPhase.cpp:876-901) */
+		float he[3] = {aabbMax[0] - aabbMin[0], aabbMax[1] - aabbMin[1], aabbMax[2] - aabbMin[2]};
+		float lx = 2.f * he[0], ly = 2.f * he[1], lz = 2.f * he[2];
+		float li[3] = {(mass / 12.0f) * (ly * ly + lz * lz), (mass / 12.0f) * (lx * lx + lz * lz),
+					   (mass / 12.0f) * (lx * lx + ly * ly)};
+		float il[3] = {1.f / li[0], 1.f / li[1], 1.f / li[2]};
+		in.m_initInvInertia.m_row[0] = f4(il[0], 0, 0, 0);
+		in.m_initInvInertia.m_row[1] = f4(0, il[1], 0, 0);
+		in.m_initInvInertia.m_row[2] = f4(0, 0, il[2], 0);
+		float m[3][3];
+		quat_to_basis(orn, m);
+		float ms[3][3];
+		for (int r = 0; r < 3; r++)
+			for (int k = 0; k < 3; k++) ms[r][k] = m[r][k] * il[k];
+		/* (m.scaled(il)) * m^T, via tdotx/tdoty/tdotz of m^T (b3Matrix3x3.h:1190-1230) */
+		for (int r = 0; r < 3; r++)
+		{
+			float w[3];
+			for (int k = 0; k < 3; k++)
+			{
+				/* (m^T).tdot_k(row) = m^T[0][k]*row.x + m^T[1][k]*row.y + m^T[2][k]*row.z = m[k][0]*x + m[k][1]*y + m[k][2]*z */
+				w[k] = m[k][0] * ms[r][0] + m[k][1] * ms[r][1] + m[k][2] * ms[r][2];
+			}
+			in.m_invInertiaWorld.m_row[r] = f4(w[0], w[1], w[2], 0.f);
+		}
+	}
+	np->bodies.push_back(b);
+	np->inertias.push_back(in);
+	if (writeToGpu)
+	{
+		B3H_CHECK(hipMemcpyAsync(np->dBodies + idx, &np->bodies[idx], sizeof(b), hipMemcpyHostToDevice, np->stream));
+		B3H_CHECK(hipMemcpyAsync(np->dInertias + idx, &np->inertias[idx], sizeof(in), hipMemcpyHostToDevice, np->stream));
+		B3H_CHECK(hipStreamSynchronize(np->stream));
+	}
+	*outBody = idx;
+	return B3H_OK;
+}
+
+template <typename T>
+static int upload_table(T** d, size_t* cap, const std::vector<T>& h, hipStream_t s)
+{
+	if (h.size() > *cap || !*d)
+	{
+		dfree(*d);
+		int r = dalloc(d, h.size() + 16);
+		if (r) return r;
+		*cap = h.size() + 16;
+	}
+	if (!h.empty()) B3H_CHECK(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+	return B3H_OK;
+}
+
+extern "C" int b3h_np_write_all_bodies_to_gpu(b3hNarrowPhase* np)
+{
+	CHECK_ARG(np);
+	int r;
+	if ((r = upload_table(&np->dCols, &np->capCols, np->cols, np->stream))) return r;
+	if ((r = upload_table(&np->dLocalAabbs, &np->capLocal, np->localAabbs, np->stream))) return r;
+	if ((r = upload_table(&np->dConvex, &np->capConvex, np->convexes, np->stream))) return r;
+	if ((r = upload_table(&np->dVerts, &np->capVerts, np->verts, np->stream))) return r;
+	if ((r = upload_table(&np->dEdges, &np->capEdges, np->edges, np->stream))) return r;
+	if ((r = upload_table(&np->dFaces, &np->capFaces, np->faces, np->stream))) return r;
+	if ((r = upload_table(&np->dIndices, &np->capIndices, np->indices, np->stream))) return r;
+	if (!np->bodies.empty())
+	{
+		B3H_CHECK(hipMemcpyAsync(np->dBodies, np->bodies.data(), np->bodies.size() * sizeof(b3hRigidBodyData),
+								 hipMemcpyHostToDevice, np->stream));
+		B3H_CHECK(hipMemcpyAsync(np->dInertias, np->inertias.data(), np->inertias.size() * sizeof(b3hInertiaData),
+								 hipMemcpyHostToDevice, np->stream));
+	}
+	B3H_CHECK(hipStreamSynchronize(np->stream));
+	return B3H_OK;
+}
+
+extern "C" int b3h_np_reset(b3hNarrowPhase* np)
+{
+	CHECK_ARG(np);
+	np->cols.clear();
+	np->convexes.clear();
+	np->verts.clear();
+	np->edges.clear();
+	np->faces.clear();
+	np->indices.clear();
+	np->localAabbs.clear();
+	np->bodies.clear();
+	np->inertias.clear();
+	np->static0Index = -1;
+	return B3H_OK;
+}
+
+extern "C" int b3h_np_readback_all_bodies_to_cpu(b3hNarrowPhase* np, b3hRigidBodyData* dst)
+{
+	CHECK_ARG(np);
+	if (np->bodies.empty()) return B3H_OK;
+	B3H_CHECK(hipMemcpyAsync(np->bodies.data(), np->dBodies, np->bodies.size() * sizeof(b3hRigidBodyData),
+							 hipMemcpyDeviceToHost, np->stream));
+	B3H_CHECK(hipStreamSynchronize(np->stream));
+	if (dst) memcpy(dst, np->bodies.data(), np->bodies.size() * sizeof(b3hRigidBodyData));
+	return B3H_OK;
+}
+
+extern "C" int b3h_np_set_object_transform_cpu(b3hNarrowPhase* np, const float* p, const float* o, int i)
+{
+	CHECK_ARG(np && p && o && i >= 0 && i < (int)np->bodies.size());
+	np->bodies[i].m_pos = f4(p[0], p[1], p[2], 0.f);
+	np->bodies[i].m_quat = f4(o[0], o[1], o[2], o[3]);
+	return B3H_OK;
+}
+extern "C" int b3h_np_set_object_velocity_cpu(b3hNarrowPhase* np, const float* lv, const float* av, int i)
+{
+	CHECK_ARG(np && lv && av && i >= 0 && i < (int)np->bodies.size());
+	np->bodies[i].m_linVel = f4(lv[0], lv[1], lv[2], 0.f);
+	np->bodies[i].m_angVel = f4(av[0], av[1], av[2], 0.f);
+	return B3H_OK;
+}
+extern "C" int b3h_np_get_object_transform_from_cpu(b3hNarrowPhase* np, float* p, float* o, int i)
+{
+	CHECK_ARG(np && p && o && i >= 0 && i < (int)np->bodies.size());
+	const b3hRigidBodyData& b = np->bodies[i];
+	p[0] = b.m_pos.x;
+	p[1] = b.m_pos.y;
+	p[2] = b.m_pos.z;
+	p[3] = 1.f;
+	o[0] = b.m_quat.x;
+	o[1] = b.m_quat.y;
+	o[2] = b.m_quat.z;
+	o[3] = b.m_quat.w;
+	return B3H_OK;
+}
+
+static b3hShapeTables np_tables(b3hNarrowPhase* np)
+{
+	b3hShapeTables t;
+	t.collidables = np->dCols;
+	t.convexes = np->dConvex;
+	t.vertices = np->dVerts;
+	t.uniqueEdges = np->dEdges;
+	t.faces = np->dFaces;
+	t.indices = np->dIndices;
+	return t;
+}
+
+/* device-count form used inside a step */
+static void np_compute_contacts_dev(b3hNarrowPhase* np, b3hInt4* pairs, const int* dNumPairs, int maxPairs)
+{
+	b3h_narrowphase(pairs, dNumPairs, maxPairs, np->dBodies, np_tables(np), np->dStaging, np->dFlags, np->dFlagScan,
+					np->dScanTmp, np->dContactPair, np->cfg.m_maxContactCapacity, np->dNumContacts, np->dSepState,
+					np->dSepNormal, np->lanesPerPair, np->stream);
+	np->contactsValid = false;
+}
+
+extern "C" int b3h_np_compute_contacts(b3hNarrowPhase* np, b3hInt4* pairs, int numPairs, const b3hAabb* aabbs, int numObjects)
+{
+	CHECK_ARG(np && numPairs >= 0 && numPairs <= np->cfg.m_maxBroadphasePairs);
+	(void)aabbs;
+	(void)numObjects;
+	if (numPairs > 0) CHECK_ARG(pairs);
+	B3H_CHECK(hipMemcpyAsync(np->dNumPairs, &numPairs, sizeof(int), hipMemcpyHostToDevice, np->stream));
+	np_compute_contacts_dev(np, pairs, np->dNumPairs, numPairs > 0 ? numPairs : 1);
+	B3H_CHECK(hipGetLastError());
+	return B3H_OK;
+}
+
+extern "C" int b3h_np_get_bodies_gpu(b3hNarrowPhase* np, b3hRigidBodyData** dev, int* n)
+{
+	CHECK_ARG(np && dev);
+	*dev = np->dBodies;
+	if (n) *n = (int)np->bodies.size();
+	return B3H_OK;
+}
+extern "C" int b3h_np_get_body_inertias_gpu(b3hNarrowPhase* np, b3hInertiaData** dev, int* n)
+{
+	CHECK_ARG(np && dev);
+	*dev = np->dInertias;
+	if (n) *n = (int)np->inertias.size();
+	return B3H_OK;
+}
+extern "C" int b3h_np_get_collidables_gpu(b3hNarrowPhase* np, b3hCollidable** dev, int* n)
+{
+	CHECK_ARG(np && dev);
+	*dev = np->dCols;
+	if (n) *n = (int)np->cols.size();
+	return B3H_OK;
+}
+extern "C" int b3h_np_get_contacts_gpu(b3hNarrowPhase* np, b3hContact4Data** dev, int* n)
+{
+	CHECK_ARG(np && dev);
+	if (!np->contactsValid)
+	{
+		b3h_gather_contacts(np->dContactPair, np->dNumContacts, np->cfg.m_maxContactCapacity, np->dStaging,
+							np->dContacts, np->stream);
+		np->contactsValid = true;
+	}
+	*dev = np->dContacts;
+	if (n)
+	{
+		B3H_CHECK(hipMemcpyAsync(n, np->dNumContacts, sizeof(int), hipMemcpyDeviceToHost, np->stream));
+	}
+	B3H_CHECK(hipStreamSynchronize(np->stream));
+	return B3H_OK;
+}
+extern "C" int b3h_np_get_static0_index(b3hNarrowPhase* np, int* index)
+{
+	CHECK_ARG(np && index);
+	*index = np->static0Index;
+	return B3H_OK;
+}
+extern "C" int b3h_np_get_num_rigid_bodies(b3hNarrowPhase* np, int* n)
+{
+	CHECK_ARG(np && n);
+	*n = (int)np->bodies.size();
+	return B3H_OK;
+}
+extern "C" int b3h_np_get_local_space_aabb(b3hNarrowPhase* np, int ci, b3hAabb* out)
+{
+	CHECK_ARG(np && out && ci >= 0 && ci < (int)np->localAabbs.size());
+	*out = np->localAabbs[ci];
+	return B3H_OK;
+}
+extern "C" int b3h_np_get_shape_tables(b3hNarrowPhase* np, int* counts, b3hCollidable* cols, b3hConvexPolyhedronData* cv,
+									   b3hFloat4* verts, b3hFloat4* edges, b3hGpuFace* faces, int* indices, b3hAabb* la)
+{
+	CHECK_ARG(np && counts);
+	counts[0] = (int)np->cols.size();
+	counts[1] = (int)np->convexes.size();
+	counts[2] = (int)np->verts.size();
+	counts[3] = (int)np->edges.size();
+	counts[4] = (int)np->faces.size();
+	counts[5] = (int)np->indices.size();
+	counts[6] = (int)np->localAabbs.size();
+	if (cols && !np->cols.empty()) memcpy(cols, np->cols.data(), np->cols.size() * sizeof(*cols));
+	if (cv && !np->convexes.empty()) memcpy(cv, np->convexes.data(), np->convexes.size() * sizeof(*cv));
+	if (verts && !np->verts.empty()) memcpy(verts, np->verts.data(), np->verts.size() * sizeof(*verts));
+	if (edges && !np->edges.empty()) memcpy(edges, np->edges.data(), np->edges.size() * sizeof(*edges));
+	if (faces && !np->faces.empty()) memcpy(faces, np->faces.data(), np->faces.size() * sizeof(*faces));
+	if (indices && !np->indices.empty()) memcpy(indices, np->indices.data(), np->indices.size() * sizeof(*indices));
+	if (la && !np->localAabbs.empty()) memcpy(la, np->localAabbs.data(), np->localAabbs.size() * sizeof(*la));
+	return B3H_OK;
+}
+extern "C" int b3h_np_get_inertias_cpu(b3hNarrowPhase* np, b3hInertiaData* dst, int n)
+{
+	CHECK_ARG(np && dst && n <= (int)np->inertias.size());
+	if (n > 0) memcpy(dst, np->inertias.data(), (size_t)n * sizeof(*dst));
+	return B3H_OK;
+}
+extern "C" int b3h_np_set_pairs_per_group(b3hNarrowPhase* np, int lanes)
+{
+	CHECK_ARG(np && lanes >= 1 && lanes <= 64 && (lanes & (lanes - 1)) == 0);
+	np->lanesPerPair = lanes;
+	return B3H_OK;
+}
+extern "C" int b3h_np_get_sat_debug(b3hNarrowPhase* np, int** st, b3hFloat4** sn)
+{
+	CHECK_ARG(np && st && sn);
+	*st = np->dSepState;
+	*sn = np->dSepNormal;
+	return B3H_OK;
+}
+
+/* ======================================================================= */
+struct b3hBroadphase
+{
+	int kind;
+	b3hConfig cfg;
+	hipStream_t stream;
+	bool ownStream;
+	std::vector<b3hAabb> all;
+	std::vector<int> smallMap, largeMap;
+	b3hGridParams gp;
+	float cellSize[3];
+	int numCells;
+	b3hAabb* dAll;
+	int *dSmall, *dLarge;
+	b3hInt4* dPairs;
+	int* dPairCount; /* [count, total] */
+	b3hBroadphaseScratch sc;
+	int scCapSmall;
+	int scCells;
+};
+
+static void bp_free_scratch(b3hBroadphase* bp)
+{
+	dfree(bp->sc.keys);
+	dfree(bp->sc.vals);
+	dfree(bp->sc.tmpKeys);
+	dfree(bp->sc.tmpVals);
+	dfree(bp->sc.hist);
+	dfree(bp->sc.cellStart);
+	dfree(bp->sc.counts);
+	dfree(bp->sc.offsets);
+	dfree(bp->sc.scanTmp);
+	dfree(bp->sc.sortedAabbs);
+	dfree(bp->sc.sapPartials);
+	dfree(bp->sc.axis);
+}
+
+extern "C" int b3h_bp_create(int kind, const b3hConfig* cfg, void* stream, b3hBroadphase** out)
+{
+	CHECK_ARG(cfg && out && (kind == B3H_BROADPHASE_GRID || kind == B3H_BROADPHASE_SAP));
+	*out = nullptr;
+	int r = ensure_device();
+	if (r) return r;
+	b3hBroadphase* bp = new b3hBroadphase();
+	bp->kind = kind;
+	bp->cfg = *cfg;
+	memset(&bp->sc, 0, sizeof(bp->sc));
+	bp->scCapSmall = 0;
+	bp->scCells = 0;
+	bp->dAll = nullptr;
+	bp->dSmall = bp->dLarge = nullptr;
+	bp->dPairs = nullptr;
+	bp->dPairCount = nullptr;
+	float cs[3] = {3.f, 3.f, 3.f};
+	int dims[3] = {128, 128, 128};
+	for (int k = 0; k < 3; k++)
+	{
+		bp->cellSize[k] = cs[k];
+		bp->gp.invCell[k] = 1.f / cs[k];
+		bp->gp.dims[k] = dims[k];
+	}
+	bp->gp.maxBodiesPerCell = 256;
+	bp->numCells = dims[0] * dims[1] * dims[2];
+	if ((r = make_stream(stream, &bp->stream, &bp->ownStream)) || (r = dalloc(&bp->dAll, cfg->m_maxConvexBodies)) ||
+		(r = dalloc(&bp->dPairs, cfg->m_maxBroadphasePairs)) || (r = dalloc(&bp->dPairCount, 2)) ||
+		(r = dalloc(&bp->dSmall, cfg->m_maxConvexBodies)) || (r = dalloc(&bp->dLarge, cfg->m_maxConvexBodies)))
+	{
+		b3h_bp_destroy(bp);
+		return r;
+	}
+	hipMemset(bp->dPairCount, 0, 2 * sizeof(int));
+	*out = bp;
+	return B3H_OK;
+}
+
+extern "C" int b3h_bp_destroy(b3hBroadphase* bp)
+{
+	if (!bp) return B3H_OK;
+	hipStreamSynchronize(bp->stream);
+	bp_free_scratch(bp);
+	dfree(bp->dAll);
+	dfree(bp->dSmall);
+	dfree(bp->dLarge);
+	dfree(bp->dPairs);
+	dfree(bp->dPairCount);
+	if (bp->ownStream) hipStreamDestroy(bp->stream);
+	delete bp;
+	return B3H_OK;
+}
+
+extern "C" int b3h_bp_set_grid_params(b3hBroadphase* bp, const float* cs, const int* dims, int maxPerCell)
+{
+	CHECK_ARG(bp && cs && dims && maxPerCell > 0);
+	for (int k = 0; k < 3; k++)
+	{
+		CHECK_ARG(cs[k] > 0.f && dims[k] > 0 && (dims[k] & (dims[k] - 1)) == 0);
+	}
+	CHECK_ARG((long long)dims[0] * dims[1] * dims[2] <= (1ll << 26));
+	for (int k = 0; k < 3; k++)
+	{
+		bp->cellSize[k] = cs[k];
+		bp->gp.invCell[k] = 1.f / cs[k];
+		bp->gp.dims[k] = dims[k];
+	}
+	bp->gp.maxBodiesPerCell = maxPerCell;
+	bp->numCells = dims[0] * dims[1] * dims[2];
+	return B3H_OK;
+}
+
+static int bp_proxy(b3hBroadphase* bp, const float* mn, const float* mx, int user, bool large)
+{
+	CHECK_ARG(bp && mn && mx);
+	if ((int)bp->all.size() >= bp->cfg.m_maxConvexBodies)
+	{
+		b3h_set_error("createProxy: exceeding capacity %d", bp->cfg.m_maxConvexBodies);
+		return B3H_ERR_CAPACITY;
+	}
+	b3hAabb a;
+	a.m_min = f4(mn[0], mn[1], mn[2], i2f(user));
+	a.m_max = f4(mx[0], mx[1], mx[2], i2f((int)bp->all.size()));
+	(large ? bp->largeMap : bp->smallMap).push_back((int)bp->all.size());
+	bp->all.push_back(a);
+	return B3H_OK;
+}
+extern "C" int b3h_bp_create_proxy(b3hBroadphase* bp, const float* mn, const float* mx, int user, int g, int m)
+{
+	(void)g;
+	(void)m;
+	return bp_proxy(bp, mn, mx, user, false);
+}
+extern "C" int b3h_bp_create_large_proxy(b3hBroadphase* bp, const float* mn, const float* mx, int user, int g, int m)
+{
+	(void)g;
+	(void)m;
+	return bp_proxy(bp, mn, mx, user, true);
+}
+
+static int bp_ensure_scratch(b3hBroadphase* bp)
+{
+	int ns = (int)bp->smallMap.size();
+	if (ns <= bp->scCapSmall && bp->scCells == bp->numCells && bp->sc.keys) return B3H_OK;
+	bp_free_scratch(bp);
+	int cap = ns + 64;
+	int r;
+	if ((r = dalloc(&bp->sc.keys, cap)) || (r = dalloc(&bp->sc.vals, cap)) || (r = dalloc(&bp->sc.tmpKeys, cap)) ||
+		(r = dalloc(&bp->sc.tmpVals, cap)) || (r = dalloc(&bp->sc.hist, b3h_radix_hist_size(cap))) ||
+		(r = dalloc(&bp->sc.cellStart, bp->numCells)) || (r = dalloc(&bp->sc.counts, 2 * cap + 1)) ||
+		(r = dalloc(&bp->sc.offsets, 2 * cap + 2)) || (r = dalloc(&bp->sc.scanTmp, b3h_scan_tmp_size(2 * cap))) ||
+		(r = dalloc(&bp->sc.sortedAabbs, cap)) || (r = dalloc(&bp->sc.sapPartials, 6 * (size_t)(b3h_sap_num_partials(cap) + 1))) ||
+		(r = dalloc(&bp->sc.axis, 1)))
+		return r;
+	bp->scCapSmall = ns;
+	bp->scCells = bp->numCells;
+	B3H_CHECK(hipMemset(bp->sc.axis, 0, sizeof(int)));
+	return B3H_OK;
+}
+
+extern "C" int b3h_bp_write_aabbs_to_gpu(b3hBroadphase* bp)
+{
+	CHECK_ARG(bp);
+	int r = bp_ensure_scratch(bp);
+	if (r) return r;
+	if (!bp->all.empty())
+		B3H_CHECK(hipMemcpyAsync(bp->dAll, bp->all.data(), bp->all.size() * sizeof(b3hAabb), hipMemcpyHostToDevice, bp->stream));
+	if (!bp->smallMap.empty())
+		B3H_CHECK(hipMemcpyAsync(bp->dSmall, bp->smallMap.data(), bp->smallMap.size() * sizeof(int), hipMemcpyHostToDevice,
+								 bp->stream));
+	if (!bp->largeMap.empty())
+		B3H_CHECK(hipMemcpyAsync(bp->dLarge, bp->largeMap.data(), bp->largeMap.size() * sizeof(int), hipMemcpyHostToDevice,
+								 bp->stream));
+	B3H_CHECK(hipStreamSynchronize(bp->stream));
+	return B3H_OK;
+}
+
+static int bp_calc_pairs(b3hBroadphase* bp, int maxPairs, hipStream_t s)
+{
+	if (maxPairs > bp->cfg.m_maxBroadphasePairs) maxPairs = bp->cfg.m_maxBroadphasePairs;
+	int r = bp_ensure_scratch(bp);
+	if (r) return r;
+	int ns = (int)bp->smallMap.size(), nl = (int)bp->largeMap.size();
+	if (bp->kind == B3H_BROADPHASE_GRID)
+		b3h_grid_pairs(bp->dAll, bp->dSmall, ns, bp->dLarge, nl, bp->gp, bp->numCells, bp->dPairs, maxPairs, bp->dPairCount,
+					   bp->dPairCount + 1, bp->sc, s);
+	else
+		b3h_sap_pairs(bp->dAll, bp->dSmall, ns, bp->dLarge, nl, bp->dPairs, maxPairs, bp->dPairCount, bp->dPairCount + 1,
+					  bp->sc, s);
+	return B3H_OK;
+}
+
+extern "C" int b3h_bp_calculate_overlapping_pairs(b3hBroadphase* bp, int maxPairs)
+{
+	CHECK_ARG(bp && maxPairs > 0);
+	int r = bp_calc_pairs(bp, maxPairs, bp->stream);
+	if (r) return r;
+	B3H_CHECK(hipGetLastError());
+	return B3H_OK;
+}
+extern "C" int b3h_bp_get_aabb_buffer_ws(b3hBroadphase* bp, b3hAabb** dev, int* n)
+{
+	CHECK_ARG(bp && dev);
+	*dev = bp->dAll;
+	if (n) *n = (int)bp->all.size();
+	return B3H_OK;
+}
+static int read_int(const int* d, int* h, hipStream_t s)
+{
+	B3H_CHECK(hipMemcpyAsync(h, d, sizeof(int), hipMemcpyDeviceToHost, s));
+	B3H_CHECK(hipStreamSynchronize(s));
+	return B3H_OK;
+}
+extern "C" int b3h_bp_get_num_overlap(b3hBroadphase* bp, int* n)
+{
+	CHECK_ARG(bp && n);
+	return read_int(bp->dPairCount, n, bp->stream);
+}
+extern "C" int b3h_bp_get_total_found(b3hBroadphase* bp, int* n)
+{
+	CHECK_ARG(bp && n);
+	return read_int(bp->dPairCount + 1, n, bp->stream);
+}
+extern "C" int b3h_bp_get_overlapping_pair_buffer(b3hBroadphase* bp, b3hInt4** dev)
+{
+	CHECK_ARG(bp && dev);
+	*dev = bp->dPairs;
+	return B3H_OK;
+}
+extern "C" int b3h_bp_get_sap_axis(b3hBroadphase* bp, int* axis)
+{
+	CHECK_ARG(bp && axis);
+	if (!bp->sc.axis)
+	{
+		*axis = 0;
+		return B3H_OK;
+	}
+	return read_int(bp->sc.axis, axis, bp->stream);
+}
+extern "C" int b3h_bp_get_proxy_maps(b3hBroadphase* bp, int* counts2, int* sm, int* lm)
+{
+	CHECK_ARG(bp && counts2);
+	counts2[0] = (int)bp->smallMap.size();
+	counts2[1] = (int)bp->largeMap.size();
+	if (sm && !bp->smallMap.empty()) memcpy(sm, bp->smallMap.data(), bp->smallMap.size() * sizeof(int));
+	if (lm && !bp->largeMap.empty()) memcpy(lm, bp->largeMap.data(), bp->largeMap.size() * sizeof(int));
+	return B3H_OK;
+}
+
+/* ======================================================================= */
+struct b3hPipeline
+{
+	b3hNarrowPhase* np;
+	b3hBroadphase* bp;
+	b3hConfig cfg;
+	hipStream_t stream;
+	bool ownStream;
+	F4 gravity;
+	b3hSolverParams prm;
+	b3hSolverBuffers sb;
+	bool trackResidual;
+	bool timing;
+	hipEvent_t ev[7];
+	bool evValid;
+	hipGraph_t graph;
+	hipGraphExec_t graphExec;
+	float graphDt;
+	int graphBodies;
+	float* dWarm;
+	int warmCount;
+};
+
+static void pl_free(b3hPipeline* p)
+{
+	dfree(p->sb.cellOf);
+	dfree(p->sb.cellCounts);
+	dfree(p->sb.cellOffsets);
+	dfree(p->sb.cellCursor);
+	dfree(p->sb.cellSlots);
+	dfree(p->sb.perm);
+	dfree(p->sb.batchIdx);
+	dfree(p->sb.batchSizes);
+	dfree(p->sb.sortKeys);
+	dfree(p->sb.constraints);
+	dfree(p->sb.sortedContacts);
+	dfree(p->sb.resid);
+}
+
+extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const b3hConfig* cfg, void* stream,
+								   b3hPipeline** out)
+{
+	CHECK_ARG(np && bp && cfg && out);
+	*out = nullptr;
+	b3hPipeline* p = new b3hPipeline();
+	memset((void*)&p->sb, 0, sizeof(p->sb));
+	p->np = np;
+	p->bp = bp;
+	p->cfg = *cfg;
+	p->gravity = f4(0.f, -9.8f, 0.f, 0.f);
+	b3h_solver_params_default(&p->prm);
+	p->trackResidual = false;
+	p->timing = false;
+	p->evValid = false;
+	p->graph = nullptr;
+	p->graphExec = nullptr;
+	p->graphDt = 0.f;
+	p->graphBodies = -1;
+	p->dWarm = nullptr;
+	p->warmCount = 0;
+	int r;
+	/* one stream for all three objects keeps the step in order; the
+	 * narrowphase's and broadphase's own streams are used for their
+	 * stand-alone calls */
+	p->stream = np->stream;
+	p->ownStream = false;
+	(void)stream;
+	if ((r = b3h_solver_module_init()))
+	{
+		delete p;
+		return r;
+	}
+	int mc = cfg->m_maxContactCapacity;
+	p->sb.maxContacts = mc;
+	if ((r = dalloc(&p->sb.cellOf, mc)) || (r = dalloc(&p->sb.cellCounts, B3H_N_CELLS)) ||
+		(r = dalloc(&p->sb.cellOffsets, B3H_N_CELLS + 1)) || (r = dalloc(&p->sb.cellCursor, B3H_N_CELLS)) ||
+		(r = dalloc(&p->sb.cellSlots, mc)) || (r = dalloc(&p->sb.perm, mc)) || (r = dalloc(&p->sb.batchIdx, mc)) ||
+		(r = dalloc(&p->sb.batchSizes, B3H_N_CELLS)) || (r = dalloc(&p->sb.sortKeys, 4 * (size_t)mc + 64)) ||
+		(r = dalloc(&p->sb.constraints, mc)) || (r = dalloc(&p->sb.sortedContacts, mc)) || (r = dalloc(&p->sb.resid, 4 * 64)))
+	{
+		pl_free(p);
+		delete p;
+		return r;
+	}
+	B3H_CHECK(hipMemset(p->sb.cellCounts, 0, B3H_N_CELLS * sizeof(int)));
+	B3H_CHECK(hipMemset(p->sb.cellOffsets, 0, (B3H_N_CELLS + 1) * sizeof(int)));
+	B3H_CHECK(hipMemset(p->sb.batchSizes, 0, B3H_N_CELLS * sizeof(int)));
+	*out = p;
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_destroy(b3hPipeline* p)
+{
+	if (!p) return B3H_OK;
+	hipStreamSynchronize(p->stream);
+	if (p->graphExec) hipGraphExecDestroy(p->graphExec);
+	if (p->graph) hipGraphDestroy(p->graph);
+	if (p->evValid)
+		for (int i = 0; i < 7; i++) hipEventDestroy(p->ev[i]);
+	pl_free(p);
+	delete p;
+	return B3H_OK;
+}
+
+static int num_bodies(b3hPipeline* p) { return (int)p->np->bodies.size(); }
+
+static void pl_aabbs(b3hPipeline* p)
+{
+	b3h_launch_world_aabbs(num_bodies(p), p->np->dBodies, p->np->dCols, p->np->dLocalAabbs, p->bp->dAll, p->stream);
+}
+
+static void pl_solve(b3hPipeline* p)
+{
+	p->sb.warmImpulses = p->dWarm;
+	b3h_solve_contacts(num_bodies(p), p->np->dBodies, p->np->dInertias, p->np->dNumContacts, p->cfg.m_maxContactCapacity,
+					   p->np->dStaging, p->np->dContactPair, p->prm, p->np->static0Index, p->sb, true, p->trackResidual,
+					   p->stream);
+}
+
+static int pl_step(b3hPipeline* p, float dt, bool events)
+{
+	hipStream_t s = p->stream;
+	if (events) hipEventRecord(p->ev[0], s);
+	pl_aabbs(p);
+	if (events) hipEventRecord(p->ev[1], s);
+	int r = bp_calc_pairs(p->bp, p->cfg.m_maxBroadphasePairs, s);
+	if (r) return r;
+	if (events) hipEventRecord(p->ev[2], s);
+	np_compute_contacts_dev(p->np, p->bp->dPairs, p->bp->dPairCount, p->cfg.m_maxBroadphasePairs);
+	if (events) hipEventRecord(p->ev[3], s);
+	pl_solve(p);
+	if (events) hipEventRecord(p->ev[4], s);
+	if (events) hipEventRecord(p->ev[5], s);
+	b3h_launch_integrate(num_bodies(p), p->np->dBodies, dt, p->prm.m_angularDamping, p->gravity, s);
+	if (events) hipEventRecord(p->ev[6], s);
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_step_simulation(b3hPipeline* p, float dt)
+{
+	CHECK_ARG(p);
+	if (num_bodies(p) == 0) return B3H_OK;
+	if (p->bp->all.size() != p->np->bodies.size())
+	{
+		b3h_set_error("stepSimulation: %zu proxies for %zu bodies (register bodies through registerPhysicsInstance)",
+					  p->bp->all.size(), p->np->bodies.size());
+		return B3H_ERR_STATE;
+	}
+	if (p->timing && !p->evValid)
+	{
+		for (int i = 0; i < 7; i++) B3H_CHECK(hipEventCreate(&p->ev[i]));
+		p->evValid = true;
+	}
+	int r = pl_step(p, dt, p->timing);
+	if (r) return r;
+	B3H_CHECK(hipGetLastError());
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_step_graph(b3hPipeline* p, float dt, int count)
+{
+	CHECK_ARG(p && count >= 0);
+	if (num_bodies(p) == 0) return B3H_OK;
+	if (!p->graphExec || p->graphDt != dt || p->graphBodies != num_bodies(p))
+	{
+		if (p->graphExec) hipGraphExecDestroy(p->graphExec);
+		if (p->graph) hipGraphDestroy(p->graph);
+		p->graphExec = nullptr;
+		p->graph = nullptr;
+		/* scratch sizes and one-time module setup must exist before capture */
+		int r = bp_ensure_scratch(p->bp);
+		if (r) return r;
+		B3H_CHECK(hipStreamSynchronize(p->stream));
+		B3H_CHECK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+		r = pl_step(p, dt, false);
+		hipGraph_t g = nullptr;
+		hipError_t e = hipStreamEndCapture(p->stream, &g);
+		if (r) return r;
+		if (e != hipSuccess)
+		{
+			b3h_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+			return B3H_ERR_HIP;
+		}
+		p->graph = g;
+		B3H_CHECK(hipGraphInstantiate(&p->graphExec, g, nullptr, nullptr, 0));
+		p->graphDt = dt;
+		p->graphBodies = num_bodies(p);
+	}
+	for (int i = 0; i < count; i++) B3H_CHECK(hipGraphLaunch(p->graphExec, p->stream));
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_integrate(b3hPipeline* p, float dt)
+{
+	CHECK_ARG(p);
+	b3h_launch_integrate(num_bodies(p), p->np->dBodies, dt, p->prm.m_angularDamping, p->gravity, p->stream);
+	B3H_CHECK(hipGetLastError());
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_setup_gpu_aabbs_full(b3hPipeline* p)
+{
+	CHECK_ARG(p);
+	pl_aabbs(p);
+	B3H_CHECK(hipGetLastError());
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_solve_contacts(b3hPipeline* p)
+{
+	CHECK_ARG(p);
+	pl_solve(p);
+	B3H_CHECK(hipGetLastError());
+	return B3H_OK;
+}
+
+/* b3TransformAabb with the b3Transform of (position, orientation)
+ * (b3AabbUtil.h:182-196, b3Transform.h:90-93) */
+static void transform_aabb(const b3hAabb& la, float margin, const float* pos, const float* orn, float* mn, float* mx)
+{
+	float m[3][3];
+	quat_to_basis(orn, m);
+	float he[3], lc[3];
+	const float lmin[3] = {la.m_min.x, la.m_min.y, la.m_min.z};
+	const float lmax[3] = {la.m_max.x, la.m_max.y, la.m_max.z};
+	for (int k = 0; k < 3; k++)
+	{
+		he[k] = 0.5f * (lmax[k] - lmin[k]);
+		he[k] += margin;
+		lc[k] = 0.5f * (lmax[k] + lmin[k]);
+	}
+	for (int r = 0; r < 3; r++)
+	{
+		float c = lc[0] * m[r][0] + lc[1] * m[r][1] + lc[2] * m[r][2];
+		c = c + pos[r];
+		float e = he[0] * std::fabs(m[r][0]) + he[1] * std::fabs(m[r][1]) + he[2] * std::fabs(m[r][2]);
+		mn[r] = c - e;
+		mx[r] = c + e;
+	}
+}
+
+extern "C" int b3h_pipeline_register_physics_instance(b3hPipeline* p, float mass, const float* pos, const float* orn,
+													  int collidableIndex, int userIndex, int writeToGpu, int* outBody)
+{
+	CHECK_ARG(p && pos && orn && outBody);
+	(void)userIndex;
+	*outBody = -1;
+	if (collidableIndex < 0 || collidableIndex >= (int)p->np->localAabbs.size())
+	{
+		b3h_set_error("registerPhysicsInstance using invalid collidableIndex %d", collidableIndex);
+		return B3H_ERR_INVALID;
+	}
+	float mn[3], mx[3];
+	transform_aabb(p->np->localAabbs[collidableIndex], 0.01f, pos, orn, mn, mx);
+	int body = -1;
+	int r = b3h_np_register_rigid_body(p->np, collidableIndex, mass, pos, orn, mn, mx, 0, &body);
+	if (r) return r;
+	if (mass)
+		r = b3h_bp_create_proxy(p->bp, mn, mx, body, 1, 1);
+	else
+		r = b3h_bp_create_large_proxy(p->bp, mn, mx, body, 1, 1);
+	if (r) return r;
+	if (writeToGpu)
+	{
+		if ((r = b3h_np_write_all_bodies_to_gpu(p->np)) || (r = b3h_bp_write_aabbs_to_gpu(p->bp))) return r;
+	}
+	*outBody = body;
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_write_all_instances_to_gpu(b3hPipeline* p)
+{
+	CHECK_ARG(p);
+	int r;
+	if ((r = b3h_np_write_all_bodies_to_gpu(p->np)) || (r = b3h_bp_write_aabbs_to_gpu(p->bp))) return r;
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_set_gravity(b3hPipeline* p, const float* g)
+{
+	CHECK_ARG(p && g);
+	p->gravity = f4(g[0], g[1], g[2], 0.f);
+	if (p->graphExec)
+	{
+		hipGraphExecDestroy(p->graphExec);
+		p->graphExec = nullptr;
+	}
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_reset(b3hPipeline* p)
+{
+	CHECK_ARG(p);
+	p->bp->all.clear();
+	p->bp->smallMap.clear();
+	p->bp->largeMap.clear();
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_get_body_buffer(b3hPipeline* p, b3hRigidBodyData** dev)
+{
+	CHECK_ARG(p && dev);
+	*dev = p->np->dBodies;
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_get_num_bodies(b3hPipeline* p, int* n)
+{
+	CHECK_ARG(p && n);
+	*n = num_bodies(p);
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_set_solver_params(b3hPipeline* p, const b3hSolverParams* prm)
+{
+	CHECK_ARG(p && prm && prm->m_numIterations >= 0 && prm->m_numIterations <= 64 && prm->m_solverDt > 0.f &&
+			  prm->m_batchCellSize > 0.f);
+	p->prm = *prm;
+	if (p->graphExec)
+	{
+		hipGraphExecDestroy(p->graphExec);
+		p->graphExec = nullptr;
+	}
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_get_solver_params(b3hPipeline* p, b3hSolverParams* prm)
+{
+	CHECK_ARG(p && prm);
+	*prm = p->prm;
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_get_solver_debug(b3hPipeline* p, b3hContact4Data** sc, b3hContactConstraint4** cs, int** cc,
+											 int** co, int** bs, int* n)
+{
+	CHECK_ARG(p);
+	if (sc) *sc = p->sb.sortedContacts;
+	if (cs) *cs = p->sb.constraints;
+	if (cc) *cc = p->sb.cellCounts;
+	if (co) *co = p->sb.cellOffsets;
+	if (bs) *bs = p->sb.batchSizes;
+	if (n) return read_int(p->np->dNumContacts, n, p->stream);
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_set_track_residual(b3hPipeline* p, int enable)
+{
+	CHECK_ARG(p);
+	p->trackResidual = enable != 0;
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_get_residuals(b3hPipeline* p, float* out, int n)
+{
+	CHECK_ARG(p && out && n >= 0 && n <= 2 * p->prm.m_numIterations);
+	float tmp[4 * 64];
+	B3H_CHECK(hipMemcpyAsync(tmp, p->sb.resid, sizeof(float) * 4 * p->prm.m_numIterations, hipMemcpyDeviceToHost, p->stream));
+	B3H_CHECK(hipStreamSynchronize(p->stream));
+	for (int i = 0; i < n; i++)
+	{
+		float d = tmp[2 * i], l = tmp[2 * i + 1];
+		out[i] = d / (l > 1e-9f ? l : 1e-9f);
+	}
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_set_warm_start_impulses(b3hPipeline* p, const float* dev, int n)
+{
+	CHECK_ARG(p && n >= 0);
+	p->dWarm = (float*)dev;
+	p->warmCount = n;
+	if (p->graphExec)
+	{
+		hipGraphExecDestroy(p->graphExec);
+		p->graphExec = nullptr;
+	}
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_enable_timing(b3hPipeline* p, int enable)
+{
+	CHECK_ARG(p);
+	p->timing = enable != 0;
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_get_stage_times(b3hPipeline* p, float* ms6)
+{
+	CHECK_ARG(p && ms6);
+	if (!p->evValid)
+	{
+		b3h_set_error("timing not enabled");
+		return B3H_ERR_STATE;
+	}
+	B3H_CHECK(hipEventSynchronize(p->ev[6]));
+	for (int i = 0; i < 6; i++)
+	{
+		float ms = 0.f;
+		B3H_CHECK(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
+		ms6[i] = ms;
+	}
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_get_overflow(b3hPipeline* p, int* po, int* co)
+{
+	CHECK_ARG(p);
+	int pc[2], cc[2];
+	B3H_CHECK(hipMemcpyAsync(pc, p->bp->dPairCount, sizeof(pc), hipMemcpyDeviceToHost, p->stream));
+	B3H_CHECK(hipMemcpyAsync(cc, p->np->dNumContacts, sizeof(cc), hipMemcpyDeviceToHost, p->stream));
+	B3H_CHECK(hipStreamSynchronize(p->stream));
+	if (po) *po = pc[1] > pc[0];
+	if (co) *co = cc[1] > cc[0];
+	return B3H_OK;
+}
+
+extern "C" int b3h_compute_polyhedral_features(const float* vertices, int strideInBytes, int numVertices, int cap,
+											   int* counts, float* v4, float* p4, int* fc, int* idx, float* e4)
+{
+	CHECK_ARG(vertices && counts && numVertices > 0 && strideInBytes >= 12 && (strideInBytes % 4) == 0);
+	b3hPolyhedron poly;
+	if (b3h_build_polyhedron(vertices, numVertices, strideInBytes / 4, &poly) != 0)
+	{
+		b3h_set_error("degenerate point set");
+		return B3H_ERR_INVALID;
+	}
+	int nv = (int)poly.vertices.size() / 3, nf = (int)poly.faces.size(), ne = (int)poly.uniqueEdges.size() / 3;
+	int ni = 0;
+	for (auto& f : poly.faces) ni += (int)f.indices.size();
+	counts[0] = nv;
+	counts[1] = nf;
+	counts[2] = ni;
+	counts[3] = ne;
+	if (nv > cap || nf > cap || ni > cap || ne > cap)
+	{
+		if (v4 || p4 || fc || idx || e4)
+		{
+			b3h_set_error("capacity %d too small", cap);
+			return B3H_ERR_CAPACITY;
+		}
+		return B3H_OK;
+	}
+	if (v4)
+		for (int i = 0; i < nv; i++)
+			for (int k = 0; k < 4; k++) v4[4 * i + k] = k < 3 ? poly.vertices[3 * i + k] : 0.f;
+	int w = 0;
+	for (int f = 0; f < nf; f++)
+	{
+		if (p4)
+			for (int k = 0; k < 4; k++) p4[4 * f + k] = poly.faces[f].plane[k];
+		if (fc) fc[f] = (int)poly.faces[f].indices.size();
+		for (int id : poly.faces[f].indices)
+		{
+			if (idx) idx[w] = id;
+			w++;
+		}
+	}
+	if (e4)
+		for (int i = 0; i < ne; i++)
+			for (int k = 0; k < 4; k++) e4[4 * i + k] = k < 3 ? poly.uniqueEdges[3 * i + k] : 0.f;
+	return B3H_OK;
+}
+
+/* ---------------- device memory helpers ---------------- */
+extern "C" int b3h_device_count(int* n)
+{
+	CHECK_ARG(n);
+	*n = 0;
+	hipError_t e = hipGetDeviceCount(n);
+	if (e != hipSuccess) *n = 0;
+	return B3H_OK;
+}
+extern "C" int b3h_malloc(void** d, size_t bytes)
+{
+	CHECK_ARG(d);
+	int r = ensure_device();
+	if (r) return r;
+	B3H_CHECK(hipMalloc(d, bytes ? bytes : 1));
+	return B3H_OK;
+}
+extern "C" int b3h_free(void* d)
+{
+	if (d) B3H_CHECK(hipFree(d));
+	return B3H_OK;
+}
+extern "C" int b3h_memcpy_htod(void* d, const void* h, size_t b)
+{
+	B3H_CHECK(hipMemcpy(d, h, b, hipMemcpyHostToDevice));
+	return B3H_OK;
+}
+extern "C" int b3h_memcpy_dtoh(void* h, const void* d, size_t b)
+{
+	B3H_CHECK(hipMemcpy(h, d, b, hipMemcpyDeviceToHost));
+	return B3H_OK;
+}
+extern "C" int b3h_memcpy_dtod(void* d, const void* s, size_t b)
+{
+	B3H_CHECK(hipMemcpy(d, s, b, hipMemcpyDeviceToDevice));
+	return B3H_OK;
+}
+extern "C" int b3h_synchronize(void)
+{
+	B3H_CHECK(hipDeviceSynchronize());
+	return B3H_OK;
+}

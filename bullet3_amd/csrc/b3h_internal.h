@@ -1,0 +1,114 @@
+/*
+ * b3h_internal.h -- library-internal declarations: kernel launchers and the
+ * state behind the opaque C-ABI handles.
+ */
+#ifndef B3H_INTERNAL_H
+#define B3H_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <vector>
+#include <string>
+#include "b3h_math.h"
+#include "../../include/b3hip.h"
+
+#define B3H_CHECK(x)                                                      \
+	do                                                                    \
+	{                                                                     \
+		hipError_t e_ = (x);                                              \
+		if (e_ != hipSuccess)                                             \
+		{                                                                 \
+			b3h_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+			return B3H_ERR_HIP;                                           \
+		}                                                                 \
+	} while (0)
+
+void b3h_set_error(const char* fmt, ...);
+
+/* ---------------- primitives (k_prims.hip) ---------------- */
+void b3h_radix_sort(unsigned* keys, unsigned* vals, unsigned* tmpKeys, unsigned* tmpVals, unsigned* hist, int n, int bits,
+					hipStream_t s);
+size_t b3h_radix_hist_size(int n);
+void b3h_exclusive_scan(const int* in, int* out, const int* nDev, int maxN, unsigned* tileSums, hipStream_t s);
+size_t b3h_scan_tmp_size(int maxN);
+void b3h_fill_i32(int* p, int v, int n, hipStream_t s);
+
+/* ---------------- per-body kernels (k_body.hip) ---------------- */
+void b3h_launch_world_aabbs(int n, const b3hRigidBodyData* bodies, const b3hCollidable* cols, const b3hAabb* localAabbs,
+							b3hAabb* world, hipStream_t s);
+void b3h_launch_integrate(int n, b3hRigidBodyData* bodies, float dt, float damping, F4 gravity, hipStream_t s);
+
+/* ---------------- broadphase (k_broadphase.hip) ---------------- */
+struct b3hGridParams
+{
+	float invCell[3];
+	int dims[3];
+	int maxBodiesPerCell;
+};
+struct b3hBroadphaseScratch
+{
+	unsigned *keys, *vals, *tmpKeys, *tmpVals, *hist;
+	int* cellStart; /* dims product */
+	int* counts;    /* 2 * nSmall + 1 (large-small counts, then grid/sap counts) */
+	int* offsets;   /* 2 * nSmall + 2 */
+	unsigned* scanTmp;
+	b3hAabb* sortedAabbs;
+	double* sapPartials;
+	int* axis;
+};
+void b3h_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
+					const b3hGridParams& gp, int numCells, b3hInt4* pairs, int maxPairs, int* pairCount,
+					int* pairTotal, b3hBroadphaseScratch& sc, hipStream_t s);
+void b3h_sap_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
+				   b3hInt4* pairs, int maxPairs, int* pairCount, int* pairTotal, b3hBroadphaseScratch& sc,
+				   hipStream_t s);
+int b3h_sap_num_partials(int nSmall);
+
+/* ---------------- narrowphase (k_narrowphase.hip) ---------------- */
+struct b3hShapeTables
+{
+	const b3hCollidable* collidables;
+	const b3hConvexPolyhedronData* convexes;
+	const b3hFloat4* vertices;
+	const b3hFloat4* uniqueEdges;
+	const b3hGpuFace* faces;
+	const int* indices;
+};
+void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3hRigidBodyData* bodies,
+					 const b3hShapeTables& sh, b3hContact4Data* staging, int* flags, int* flagScan,
+					 unsigned* scanTmp, int* contactPair, int maxContacts, int* numContacts, int* sepState,
+					 b3hFloat4* sepNormal, int pairsPerGroup, hipStream_t s);
+void b3h_gather_contacts(const int* contactPair, const int* numContacts, int maxContacts,
+						 const b3hContact4Data* staging, b3hContact4Data* out, hipStream_t s);
+
+/* ---------------- solver (k_solver.hip) ---------------- */
+#define B3H_N_SPLIT_X 8
+#define B3H_N_SPLIT_Y 4
+#define B3H_N_SPLIT_Z 8
+#define B3H_N_CELLS (B3H_N_SPLIT_X * B3H_N_SPLIT_Y * B3H_N_SPLIT_Z)
+#define B3H_N_CELL_BATCHES 8
+
+struct b3hSolverBuffers
+{
+	int* cellOf;          /* per contact slot */
+	int* cellCounts;      /* 256 */
+	int* cellOffsets;     /* 257 */
+	int* cellCursor;      /* 256 */
+	int* cellSlots;       /* per position: contact slot (unsorted within cell) */
+	int* perm;            /* per final position: contact slot */
+	int* batchIdx;        /* per final position */
+	int* batchSizes;      /* 256 */
+	unsigned long long* sortKeys; /* per position (global fallback) */
+	b3hContactConstraint4* constraints;
+	b3hContact4Data* sortedContacts; /* optional */
+	float* resid;         /* 2 * maxIters * 2 (maxDelta, maxLambda) */
+	float* warmImpulses;  /* optional, 6 per constraint */
+	int maxContacts;
+};
+int b3h_solver_module_init();
+int b3h_solver_lds_bytes();
+void b3h_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaData* inertias,
+						const int* numContacts, int maxContacts, const b3hContact4Data* staging,
+						const int* contactPair, const b3hSolverParams& prm, int staticIdx, b3hSolverBuffers& sb,
+						bool writeSortedContacts, bool trackResidual, hipStream_t s);
+
+#endif

@@ -1,0 +1,324 @@
+/*
+ * k_broadphase.hip -- uniform-grid and 1-axis sweep-and-prune broadphase.
+ *
+ * Grid (b3GpuGridBroadphase::calculateOverlappingPairs, b3GpuGridBroadphase.cpp:117-259;
+ *       kernels gridBroadphase.cl:3-211):
+ *   hash of the AABB centre cell (wrapped to the grid dims) -> stable radix sort
+ *   of (hash, proxy) -> first index of every hash run -> per sorted proxy a scan
+ *   of the 27 neighbour cells (bucket capped at maxBodiesPerCell, inclusive AABB
+ *   test, pair emitted once as (lower, higher) body index).
+ * SAP (b3GpuSapBroadphase::calculateOverlappingPairs, b3GpuSapBroadphase.cpp:997-1231;
+ *      kernels sap.cl:231-389): axis of largest centre variance, stable sort by
+ *   the order-preserving bits of min[axis], sweep j = i+1.. while
+ *   min_j[axis] <= max_i[axis]; pairs are (sorted-earlier, later) as in the reference.
+ * Large x small pairs (the static ground): computePairsKernelTwoArrays, sap.cl:66-105.
+ *
+ * Pair compaction is atomic-free: a counting pass, a prefix sum and a writing
+ * pass.  The pair buffer therefore has a fixed order -- large x small pairs
+ * (small-major) first, then per sorted proxy in neighbour/bucket order -- that
+ * does not depend on scheduling.
+ */
+#include "b3h_internal.h"
+
+__device__ __forceinline__ bool bp_overlap(const b3hAabb& a, const b3hAabb& b)
+{
+	return (a.m_min.x <= b.m_max.x) && (b.m_min.x <= a.m_max.x) && (a.m_min.y <= b.m_max.y) &&
+		   (b.m_min.y <= a.m_max.y) && (a.m_min.z <= b.m_max.z) && (b.m_min.z <= a.m_max.z);
+}
+__device__ __forceinline__ int bp_handle(const b3hAabb& a) { return f2i(a.m_min.w); }
+
+/* ---------------- large x small ---------------- */
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_large_small(int nSmall, int nLarge, const b3hAabb* __restrict__ all,
+													 const int* __restrict__ smallMap, const int* __restrict__ largeMap,
+													 int* __restrict__ counts, const int* __restrict__ offsets,
+													 b3hInt4* __restrict__ pairs, int maxPairs)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nSmall) return;
+	b3hAabb s = all[smallMap[i]];
+	int hs = bp_handle(s);
+	int c = 0;
+	int o = WRITE ? offsets[i] : 0;
+	for (int j = 0; j < nLarge; j++)
+	{
+		b3hAabb l = all[largeMap[j]];
+		if (bp_overlap(l, s))
+		{
+			if (WRITE)
+			{
+				int x = bp_handle(l), y = hs;
+				if (x > y)
+				{
+					int t = x;
+					x = y;
+					y = t;
+				}
+				int at = o + c;
+				if (at < maxPairs) pairs[at] = b3hInt4{x, y, -1, -1};
+			}
+			c++;
+		}
+	}
+	if (!WRITE) counts[i] = c;
+}
+
+/* ---------------- grid ---------------- */
+__device__ __forceinline__ void grid_pos(const b3hAabb& a, const b3hGridParams& gp, int g[3])
+{
+	float px = (a.m_min.x + a.m_max.x) * 0.5f;
+	float py = (a.m_min.y + a.m_max.y) * 0.5f;
+	float pz = (a.m_min.z + a.m_max.z) * 0.5f;
+	g[0] = (int)floorf(px * gp.invCell[0]) & (gp.dims[0] - 1);
+	g[1] = (int)floorf(py * gp.invCell[1]) & (gp.dims[1] - 1);
+	g[2] = (int)floorf(pz * gp.invCell[2]) & (gp.dims[2] - 1);
+}
+__device__ __forceinline__ int grid_hash(int x, int y, int z, const b3hGridParams& gp)
+{
+	x &= gp.dims[0] - 1;
+	y &= gp.dims[1] - 1;
+	z &= gp.dims[2] - 1;
+	return z * gp.dims[1] * gp.dims[0] + y * gp.dims[0] + x;
+}
+
+__global__ __launch_bounds__(256) void k_grid_hash(int n, const b3hAabb* __restrict__ all, const int* __restrict__ smallMap,
+												   b3hGridParams gp, unsigned* __restrict__ keys, unsigned* __restrict__ vals)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int g[3];
+	grid_pos(all[smallMap[i]], gp, g);
+	keys[i] = (unsigned)grid_hash(g[0], g[1], g[2], gp);
+	vals[i] = (unsigned)i;
+}
+
+__global__ __launch_bounds__(256) void k_cell_start(int n, const unsigned* __restrict__ keys, int* __restrict__ cellStart)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	unsigned k = keys[i];
+	if (i == 0 || keys[i - 1] != k) cellStart[k] = i;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_grid_pairs(int n, const unsigned* __restrict__ keys, const unsigned* __restrict__ vals,
+													const b3hAabb* __restrict__ all, const int* __restrict__ smallMap,
+													const int* __restrict__ cellStart, b3hGridParams gp,
+													int* __restrict__ counts, const int* __restrict__ offsets,
+													b3hInt4* __restrict__ pairs, int maxPairs)
+{
+	int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= n) return;
+	int ui = (int)vals[s];
+	b3hAabb a = all[smallMap[ui]];
+	int ha = bp_handle(a);
+	int g[3];
+	grid_pos(a, gp, g);
+	int c = 0;
+	int o = WRITE ? offsets[s] : 0;
+	for (int z = -1; z <= 1; z++)
+		for (int y = -1; y <= 1; y++)
+			for (int x = -1; x <= 1; x++)
+			{
+				int h = grid_hash(g[0] + x, g[1] + y, g[2] + z, gp);
+				int st = cellStart[h];
+				if (st == -1) continue;
+				int end = st + gp.maxBodiesPerCell;
+				if (end > n) end = n;
+				for (int k = st; k < end; k++)
+				{
+					if ((int)keys[k] != h) break;
+					int uj = (int)vals[k];
+					if (uj == ui) continue;
+					b3hAabb b = all[smallMap[uj]];
+					if (!bp_overlap(a, b)) continue;
+					int hb = bp_handle(b);
+					if (ha < hb)
+					{
+						if (WRITE)
+						{
+							int at = o + c;
+							if (at < maxPairs) pairs[at] = b3hInt4{ha, hb, -1, -1};
+						}
+						c++;
+					}
+				}
+			}
+	if (!WRITE) counts[s] = c;
+}
+
+__global__ void k_pairs_finish(const int* __restrict__ total, int maxPairs, int* __restrict__ pairCount,
+							   int* __restrict__ pairTotal)
+{
+	int t = *total;
+	*pairTotal = t;
+	*pairCount = t < maxPairs ? t : maxPairs;
+}
+
+static inline int nblk(int n) { return (n + 255) / 256; }
+
+void b3h_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
+					const b3hGridParams& gp, int numCells, b3hInt4* pairs, int maxPairs, int* pairCount,
+					int* pairTotal, b3hBroadphaseScratch& sc, hipStream_t s)
+{
+	int* countsLS = sc.counts;
+	int* countsG = sc.counts + nSmall;
+	if (nSmall > 0)
+	{
+		hipLaunchKernelGGL(k_large_small<false>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, nLarge, all, smallMap,
+						   largeMap, countsLS, (const int*)nullptr, (b3hInt4*)nullptr, maxPairs);
+		hipLaunchKernelGGL(k_grid_hash, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, all, smallMap, gp, sc.keys, sc.vals);
+		int bits = 0;
+		while ((1 << bits) < numCells) bits++;
+		b3h_radix_sort(sc.keys, sc.vals, sc.tmpKeys, sc.tmpVals, sc.hist, nSmall, bits, s);
+		b3h_fill_i32(sc.cellStart, -1, numCells, s);
+		hipLaunchKernelGGL(k_cell_start, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys, sc.cellStart);
+		hipLaunchKernelGGL(k_grid_pairs<false>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys, sc.vals, all,
+						   smallMap, sc.cellStart, gp, countsG, (const int*)nullptr, (b3hInt4*)nullptr, maxPairs);
+	}
+	int nTot = 2 * nSmall;
+	b3h_exclusive_scan(sc.counts, sc.offsets, nullptr, nTot, sc.scanTmp, s);
+	if (nSmall > 0)
+	{
+		hipLaunchKernelGGL(k_large_small<true>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, nLarge, all, smallMap,
+						   largeMap, (int*)nullptr, sc.offsets, pairs, maxPairs);
+		hipLaunchKernelGGL(k_grid_pairs<true>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys, sc.vals, all,
+						   smallMap, sc.cellStart, gp, (int*)nullptr, sc.offsets + nSmall, pairs, maxPairs);
+	}
+	hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, sc.offsets + nTot, maxPairs, pairCount, pairTotal);
+}
+
+/* ---------------- SAP ---------------- */
+#define SAP_BLOCK 256
+int b3h_sap_num_partials(int nSmall) { return (nSmall + SAP_BLOCK - 1) / SAP_BLOCK; }
+
+__global__ __launch_bounds__(SAP_BLOCK) void k_sap_partials(int n, const b3hAabb* __restrict__ all,
+															const int* __restrict__ smallMap, double* __restrict__ partials)
+{
+	__shared__ double red[6][SAP_BLOCK];
+	int t = threadIdx.x;
+	int i = blockIdx.x * SAP_BLOCK + t;
+	double v[6] = {0, 0, 0, 0, 0, 0};
+	if (i < n)
+	{
+		b3hAabb a = all[smallMap[i]];
+		float c[3] = {(a.m_max.x + a.m_min.x) * 0.5f, (a.m_max.y + a.m_min.y) * 0.5f, (a.m_max.z + a.m_min.z) * 0.5f};
+		for (int k = 0; k < 3; k++)
+		{
+			v[k] = (double)c[k];
+			v[3 + k] = (double)(c[k] * c[k]);
+		}
+	}
+	for (int k = 0; k < 6; k++) red[k][t] = v[k];
+	__syncthreads();
+	for (int off = SAP_BLOCK / 2; off > 0; off >>= 1)
+	{
+		if (t < off)
+			for (int k = 0; k < 6; k++) red[k][t] += red[k][t + off];
+		__syncthreads();
+	}
+	if (t < 6) partials[blockIdx.x * 6 + t] = red[t][0];
+}
+
+__global__ void k_sap_axis(int numPartials, const double* __restrict__ partials, int n, int* __restrict__ axisOut)
+{
+	double s[6] = {0, 0, 0, 0, 0, 0};
+	for (int b = 0; b < numPartials; b++)
+		for (int k = 0; k < 6; k++) s[k] += partials[b * 6 + k];
+	int axis = 0;
+	if (n > 0)
+	{
+		double v[3];
+		for (int k = 0; k < 3; k++) v[k] = s[3 + k] - (s[k] * s[k]) / (double)n;
+		if (v[1] > v[0]) axis = 1;
+		if (v[2] > v[axis]) axis = 2;
+	}
+	*axisOut = axis;
+}
+
+__device__ __forceinline__ unsigned float_flip(float f)
+{
+	unsigned u = f2u(f);
+	unsigned mask = (unsigned)(-(int)(u >> 31)) | 0x80000000u;
+	return u ^ mask;
+}
+__device__ __forceinline__ float axis_val(F4 v, int axis) { return axis == 0 ? v.x : (axis == 1 ? v.y : v.z); }
+
+__global__ __launch_bounds__(256) void k_sap_keys(int n, const b3hAabb* __restrict__ all, const int* __restrict__ smallMap,
+												  const int* __restrict__ axisPtr, unsigned* __restrict__ keys,
+												  unsigned* __restrict__ vals)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int axis = *axisPtr;
+	keys[i] = float_flip(axis_val(all[smallMap[i]].m_min, axis));
+	vals[i] = (unsigned)i;
+}
+
+__global__ __launch_bounds__(256) void k_sap_scatter(int n, const b3hAabb* __restrict__ all, const int* __restrict__ smallMap,
+													 const unsigned* __restrict__ vals, b3hAabb* __restrict__ sorted)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	sorted[i] = all[smallMap[vals[i]]];
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_sap_sweep(int n, const b3hAabb* __restrict__ sorted, const int* __restrict__ axisPtr,
+												   int* __restrict__ counts, const int* __restrict__ offsets,
+												   b3hInt4* __restrict__ pairs, int maxPairs)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int axis = *axisPtr;
+	b3hAabb a = sorted[i];
+	float amax = axis_val(a.m_max, axis);
+	int ha = bp_handle(a);
+	int c = 0;
+	int o = WRITE ? offsets[i] : 0;
+	for (int j = i + 1; j < n; j++)
+	{
+		b3hAabb b = sorted[j];
+		if (amax < axis_val(b.m_min, axis)) break;
+		if (bp_overlap(a, b))
+		{
+			if (WRITE)
+			{
+				int at = o + c;
+				if (at < maxPairs) pairs[at] = b3hInt4{ha, bp_handle(b), -1, -1};
+			}
+			c++;
+		}
+	}
+	if (!WRITE) counts[i] = c;
+}
+
+void b3h_sap_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
+				   b3hInt4* pairs, int maxPairs, int* pairCount, int* pairTotal, b3hBroadphaseScratch& sc, hipStream_t s)
+{
+	int* countsLS = sc.counts;
+	int* countsS = sc.counts + nSmall;
+	if (nSmall > 0)
+	{
+		int np = b3h_sap_num_partials(nSmall);
+		hipLaunchKernelGGL(k_sap_partials, dim3(np), dim3(SAP_BLOCK), 0, s, nSmall, all, smallMap, sc.sapPartials);
+		hipLaunchKernelGGL(k_sap_axis, dim3(1), dim3(1), 0, s, np, sc.sapPartials, nSmall, sc.axis);
+		hipLaunchKernelGGL(k_large_small<false>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, nLarge, all, smallMap,
+						   largeMap, countsLS, (const int*)nullptr, (b3hInt4*)nullptr, maxPairs);
+		hipLaunchKernelGGL(k_sap_keys, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, all, smallMap, sc.axis, sc.keys, sc.vals);
+		b3h_radix_sort(sc.keys, sc.vals, sc.tmpKeys, sc.tmpVals, sc.hist, nSmall, 32, s);
+		hipLaunchKernelGGL(k_sap_scatter, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, all, smallMap, sc.vals, sc.sortedAabbs);
+		hipLaunchKernelGGL(k_sap_sweep<false>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.sortedAabbs, sc.axis, countsS,
+						   (const int*)nullptr, (b3hInt4*)nullptr, maxPairs);
+	}
+	int nTot = 2 * nSmall;
+	b3h_exclusive_scan(sc.counts, sc.offsets, nullptr, nTot, sc.scanTmp, s);
+	if (nSmall > 0)
+	{
+		hipLaunchKernelGGL(k_large_small<true>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, nLarge, all, smallMap,
+						   largeMap, (int*)nullptr, sc.offsets, pairs, maxPairs);
+		hipLaunchKernelGGL(k_sap_sweep<true>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.sortedAabbs, sc.axis,
+						   (int*)nullptr, sc.offsets + nSmall, pairs, maxPairs);
+	}
+	hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, sc.offsets + nTot, maxPairs, pairCount, pairTotal);
+}

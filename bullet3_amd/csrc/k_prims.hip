@@ -1,0 +1,226 @@
+/*
+ * k_prims.hip -- parallel primitives on gfx950: stable LSD radix sort of
+ * (uint32 key, uint32 value) pairs with 8-bit digits, and a device-wide
+ * exclusive prefix sum.
+ *
+ * Radix pass (replaces b3RadixSort32CL, src/Bullet3OpenCL/ParallelPrimitives/
+ * b3RadixSort32CL.cpp:174-518, 4-bit digits there):
+ *   hist    one block per tile of TILE keys counts its digits (LDS atomics)
+ *   scan    digit-major exclusive scan of the [256][tiles] counts
+ *   scatter each block walks its tile in order, 256 keys at a time; a key's
+ *           rank among equal digits of its wavefront comes from 8 ballots
+ *           (lanes with the same digit) and a popcount below the lane, waves
+ *           are prefixed through LDS -> stable, deterministic placement.
+ * Only as many passes as the key width needs are run.
+ */
+#include "b3h_internal.h"
+
+#define RS_THREADS 256
+#define RS_ITEMS 8
+#define RS_TILE (RS_THREADS * RS_ITEMS)
+#define RS_BUCKETS 256
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const unsigned* __restrict__ keys, int n, int shift,
+														   unsigned* __restrict__ hist, int numTiles)
+{
+	__shared__ unsigned cnt[RS_BUCKETS];
+	int t = threadIdx.x;
+	cnt[t] = 0;
+	__syncthreads();
+	int base = blockIdx.x * RS_TILE;
+	for (int k = 0; k < RS_ITEMS; k++)
+	{
+		int i = base + k * RS_THREADS + t;
+		if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 0xff], 1u);
+	}
+	__syncthreads();
+	hist[t * numTiles + blockIdx.x] = cnt[t];
+}
+
+/* exclusive scan of a small array (<= a few hundred thousand) by one block */
+__global__ __launch_bounds__(1024) void k_scan_single(unsigned* __restrict__ data, int n)
+{
+	__shared__ unsigned part[1024];
+	int t = threadIdx.x;
+	int per = (n + 1023) / 1024;
+	int s = t * per, e = min(n, s + per);
+	unsigned sum = 0;
+	for (int i = s; i < e; i++) sum += data[i];
+	part[t] = sum;
+	__syncthreads();
+	for (int off = 1; off < 1024; off <<= 1)
+	{
+		unsigned v = t >= off ? part[t - off] : 0;
+		__syncthreads();
+		part[t] += v;
+		__syncthreads();
+	}
+	unsigned run = part[t] - sum;
+	for (int i = s; i < e; i++)
+	{
+		unsigned v = data[i];
+		data[i] = run;
+		run += v;
+	}
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const unsigned* __restrict__ keysIn,
+															  const unsigned* __restrict__ valsIn, int n, int shift,
+															  const unsigned* __restrict__ histScanned, int numTiles,
+															  unsigned* __restrict__ keysOut,
+															  unsigned* __restrict__ valsOut)
+{
+	__shared__ unsigned base[RS_BUCKETS];
+	__shared__ unsigned waveCnt[RS_THREADS / 64][RS_BUCKETS];
+	int t = threadIdx.x;
+	int lane = t & 63;
+	int wave = t >> 6;
+	base[t] = histScanned[t * numTiles + blockIdx.x];
+	unsigned long long lanesBelow = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+	int tileStart = blockIdx.x * RS_TILE;
+	for (int k = 0; k < RS_ITEMS; k++)
+	{
+		for (int w = 0; w < RS_THREADS / 64; w++) waveCnt[w][t] = 0;
+		__syncthreads();
+		int i = tileStart + k * RS_THREADS + t;
+		bool valid = i < n;
+		unsigned key = valid ? keysIn[i] : 0u;
+		unsigned val = valid ? valsIn[i] : 0u;
+		unsigned d = (key >> shift) & 0xff;
+		unsigned long long m = __ballot(valid);
+#pragma unroll
+		for (int b = 0; b < 8; b++)
+		{
+			unsigned long long bb = __ballot(valid && ((d >> b) & 1));
+			m &= ((d >> b) & 1) ? bb : ~bb;
+		}
+		int rank = __popcll(m & lanesBelow);
+		int total = __popcll(m);
+		/* the highest lane of each digit group publishes the group size */
+		if (valid && rank == total - 1) waveCnt[wave][d] = (unsigned)total;
+		__syncthreads();
+		if (valid)
+		{
+			unsigned off = base[d];
+			for (int w = 0; w < wave; w++) off += waveCnt[w][d];
+			unsigned pos = off + (unsigned)rank;
+			keysOut[pos] = key;
+			valsOut[pos] = val;
+		}
+		__syncthreads();
+		unsigned add = 0;
+		for (int w = 0; w < RS_THREADS / 64; w++) add += waveCnt[w][t];
+		base[t] += add;
+		__syncthreads();
+	}
+}
+
+/* Sorts (keys,vals) of length n by the low `bits` bits of the key, stably.
+ * keys/vals receive the result; tmpKeys/tmpVals are scratch of length n,
+ * hist is scratch of 256 * ceil(n / RS_TILE) entries. */
+void b3h_radix_sort(unsigned* keys, unsigned* vals, unsigned* tmpKeys, unsigned* tmpVals, unsigned* hist, int n, int bits,
+					hipStream_t s)
+{
+	if (n <= 0) return;
+	int tiles = (n + RS_TILE - 1) / RS_TILE;
+	int passes = (bits + 7) / 8;
+	unsigned *ki = keys, *vi = vals, *ko = tmpKeys, *vo = tmpVals;
+	for (int p = 0; p < passes; p++)
+	{
+		int shift = 8 * p;
+		hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(RS_THREADS), 0, s, ki, n, shift, hist, tiles);
+		hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, hist, RS_BUCKETS * tiles);
+		hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(RS_THREADS), 0, s, ki, vi, n, shift, hist, tiles, ko, vo);
+		unsigned* t;
+		t = ki; ki = ko; ko = t;
+		t = vi; vi = vo; vo = t;
+	}
+	if (ki != keys)
+	{
+		hipMemcpyAsync(keys, ki, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, s);
+		hipMemcpyAsync(vals, vi, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, s);
+	}
+}
+
+size_t b3h_radix_hist_size(int n) { return (size_t)RS_BUCKETS * ((n + RS_TILE - 1) / RS_TILE + 1); }
+
+/* ---- device-wide exclusive scan of int counts (n read from the host) ----
+ * out[i] = sum_{j<i} in[j] for i <= n (out has n+1 entries). */
+#define SC_THREADS 256
+#define SC_ITEMS 16
+#define SC_TILE (SC_THREADS * SC_ITEMS)
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_tiles(const int* __restrict__ in, const int* __restrict__ nPtr,
+														   int nFixed, int* __restrict__ out, unsigned* __restrict__ tileSums)
+{
+	__shared__ int part[SC_THREADS];
+	int n = nPtr ? *nPtr : nFixed;
+	int t = threadIdx.x;
+	int base = blockIdx.x * SC_TILE + t * SC_ITEMS;
+	int v[SC_ITEMS];
+	int sum = 0;
+#pragma unroll
+	for (int k = 0; k < SC_ITEMS; k++)
+	{
+		int i = base + k;
+		v[k] = i < n ? in[i] : 0;
+		sum += v[k];
+	}
+	part[t] = sum;
+	__syncthreads();
+	for (int off = 1; off < SC_THREADS; off <<= 1)
+	{
+		int x = t >= off ? part[t - off] : 0;
+		__syncthreads();
+		part[t] += x;
+		__syncthreads();
+	}
+	int run = part[t] - sum;
+#pragma unroll
+	for (int k = 0; k < SC_ITEMS; k++)
+	{
+		int i = base + k;
+		if (i < n) out[i] = run;
+		run += v[k];
+	}
+	if (t == SC_THREADS - 1) tileSums[blockIdx.x] = (unsigned)part[t];
+}
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_add(int* __restrict__ out, const int* __restrict__ nPtr, int nFixed,
+														 const unsigned* __restrict__ tileSumsScanned, int numTiles)
+{
+	int n = nPtr ? *nPtr : nFixed;
+	int t = threadIdx.x;
+	int add = (int)tileSumsScanned[blockIdx.x];
+	int base = blockIdx.x * SC_TILE + t * SC_ITEMS;
+	for (int k = 0; k < SC_ITEMS; k++)
+	{
+		int i = base + k;
+		if (i < n) out[i] += add;
+	}
+	if (blockIdx.x == 0 && t == 0) out[n] = (int)tileSumsScanned[numTiles];
+}
+
+/* tileSums needs maxN/SC_TILE + 2 entries.  The grid is sized from maxN so the
+ * call can be captured in a graph while n itself lives on the device. */
+void b3h_exclusive_scan(const int* in, int* out, const int* nDev, int maxN, unsigned* tileSums, hipStream_t s)
+{
+	int tiles = (maxN + SC_TILE - 1) / SC_TILE;
+	if (tiles < 1) tiles = 1;
+	hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(SC_THREADS), 0, s, in, nDev, maxN, out, tileSums);
+	hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, tileSums, tiles + 1);
+	hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(SC_THREADS), 0, s, out, nDev, maxN, tileSums, tiles);
+}
+size_t b3h_scan_tmp_size(int maxN) { return (size_t)(maxN + SC_TILE - 1) / SC_TILE + 2; }
+
+/* fill helpers (used instead of memset nodes so steps can be graph-captured) */
+__global__ void k_fill_i32(int* p, int v, int n)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+void b3h_fill_i32(int* p, int v, int n, hipStream_t s)
+{
+	if (n <= 0) return;
+	hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+}

@@ -1,0 +1,191 @@
+"""Python mirror of the reference's demo set-up (GpuRigidBodyDemo::initPhysics,
+examples/OpenCL/rigidbody/GpuRigidBodyDemo.cpp:110-152, and the scene
+registration of GpuConvexScene.cpp:171-304) over the C ABI:
+
+    np = b3GpuNarrowPhase(config); bp = b3Gpu{Grid,Sap}Broadphase();
+    pipeline = b3GpuRigidBodyPipeline(np, bp, config)
+    registerConvexHullShape per shape, registerPhysicsInstance per body,
+    writeAllInstancesToGpu, then stepSimulation(dt) per frame.
+
+Every call goes through bullet3_amd/libb3hip.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import capi
+from .capi import call, ptr
+from . import layout
+
+
+class World:
+    def __init__(self, scene, broadphase="grid", max_bodies=None, solver=None, grid_dims=(128, 128, 128),
+                 max_bodies_per_cell=256, cell_size=None, pairs_per_body=16):
+        n = scene.num_bodies
+        if max_bodies is None:
+            max_bodies = max(128 * 1024, n + 10)
+        self.cfg = capi.default_config(max_bodies, pairs_per_body)
+        self.scene = scene
+        self.np_ = ctypes.c_void_p()
+        self.bp = ctypes.c_void_p()
+        self.pl = ctypes.c_void_p()
+        call("b3h_np_create", ctypes.byref(self.cfg), None, ctypes.byref(self.np_))
+        kind = capi.BROADPHASE_GRID if broadphase == "grid" else capi.BROADPHASE_SAP
+        call("b3h_bp_create", kind, ctypes.byref(self.cfg), None, ctypes.byref(self.bp))
+        cs = float(cell_size if cell_size is not None else scene.cell_size)
+        self.cell_size = cs
+        call("b3h_bp_set_grid_params", self.bp, ptr(np.array([cs, cs, cs], np.float32)),
+             ptr(np.array(grid_dims, np.int32)), int(max_bodies_per_cell))
+        call("b3h_pipeline_create", self.np_, self.bp, ctypes.byref(self.cfg), None, ctypes.byref(self.pl))
+        self.collidables = []
+        for pts in scene.shapes:
+            pts = np.ascontiguousarray(pts, np.float32)
+            out = ctypes.c_int(-1)
+            call("b3h_np_register_convex_hull_shape", self.np_, ptr(pts), 12, len(pts),
+                 ptr(np.ones(3, np.float32)), ctypes.byref(out))
+            self.collidables.append(out.value)
+        pos = np.ascontiguousarray(scene.pos, np.float32)
+        orn = np.ascontiguousarray(scene.orn, np.float32)
+        out = ctypes.c_int(-1)
+        reg = capi.lib().b3h_pipeline_register_physics_instance
+        for i in range(n):
+            rc = reg(self.pl, ctypes.c_float(float(scene.mass[i])), ptr(pos[i]), ptr(orn[i]),
+                     self.collidables[int(scene.shape[i])], i, 0, ctypes.byref(out))
+            capi.check(rc, "registerPhysicsInstance")
+        call("b3h_pipeline_write_all_instances_to_gpu", self.pl)
+        if solver is not None:
+            self.set_solver(solver)
+
+    # -- parameters
+    def solver_params(self):
+        p = capi.SolverParams()
+        call("b3h_pipeline_get_solver_params", self.pl, ctypes.byref(p))
+        return p
+
+    def set_solver(self, p):
+        call("b3h_pipeline_set_solver_params", self.pl, ctypes.byref(p))
+
+    # -- stepping
+    def step(self, dt=1.0 / 60.0):
+        call("b3h_pipeline_step_simulation", self.pl, ctypes.c_float(dt))
+
+    def step_graph(self, dt=1.0 / 60.0, count=1):
+        call("b3h_pipeline_step_graph", self.pl, ctypes.c_float(dt), int(count))
+
+    def sync(self):
+        call("b3h_synchronize")
+
+    # -- state access
+    @property
+    def num_bodies(self):
+        n = ctypes.c_int()
+        call("b3h_pipeline_get_num_bodies", self.pl, ctypes.byref(n))
+        return n.value
+
+    def bodies(self):
+        out = np.zeros(self.num_bodies, layout.BODY)
+        call("b3h_np_readback_all_bodies_to_cpu", self.np_, ptr(out))
+        return out
+
+    def set_bodies(self, bodies):
+        d = ctypes.c_void_p()
+        call("b3h_np_get_bodies_gpu", self.np_, ctypes.byref(d), None)
+        capi.htod(d.value, np.ascontiguousarray(bodies))
+
+    def inertias(self):
+        out = np.zeros(self.num_bodies, layout.INERTIA)
+        call("b3h_np_get_inertias_cpu", self.np_, ptr(out), len(out))
+        return out
+
+    def shape_tables(self):
+        counts = np.zeros(7, np.int32)
+        call("b3h_np_get_shape_tables", self.np_, ptr(counts), None, None, None, None, None, None, None)
+        t = dict(collidables=np.zeros(counts[0], layout.COLLIDABLE), convexes=np.zeros(counts[1], layout.CONVEX),
+                 vertices=np.zeros((counts[2], 4), np.float32), uniqueEdges=np.zeros((max(counts[3], 1), 4), np.float32),
+                 faces=np.zeros(max(counts[4], 1), layout.FACE), indices=np.zeros(max(counts[5], 1), np.int32),
+                 localAabbs=np.zeros(counts[6], layout.AABB))
+        call("b3h_np_get_shape_tables", self.np_, ptr(counts), ptr(t["collidables"]), ptr(t["convexes"]),
+             ptr(t["vertices"]), ptr(t["uniqueEdges"]), ptr(t["faces"]), ptr(t["indices"]), ptr(t["localAabbs"]))
+        return t
+
+    def proxy_maps(self):
+        c = np.zeros(2, np.int32)
+        call("b3h_bp_get_proxy_maps", self.bp, ptr(c), None, None)
+        sm = np.zeros(max(c[0], 1), np.int32)
+        lm = np.zeros(max(c[1], 1), np.int32)
+        call("b3h_bp_get_proxy_maps", self.bp, ptr(c), ptr(sm), ptr(lm))
+        return sm[:c[0]].copy(), lm[:c[1]].copy()
+
+    def world_aabbs(self):
+        d = ctypes.c_void_p()
+        n = ctypes.c_int()
+        call("b3h_bp_get_aabb_buffer_ws", self.bp, ctypes.byref(d), ctypes.byref(n))
+        return capi.dtoh(d.value, n.value * 32, layout.AABB)
+
+    def pairs(self):
+        n = ctypes.c_int()
+        call("b3h_bp_get_num_overlap", self.bp, ctypes.byref(n))
+        d = ctypes.c_void_p()
+        call("b3h_bp_get_overlapping_pair_buffer", self.bp, ctypes.byref(d))
+        return capi.dtoh(d.value, n.value * 16, np.int32).reshape(-1, 4)
+
+    def contacts(self):
+        d = ctypes.c_void_p()
+        n = ctypes.c_int()
+        call("b3h_np_get_contacts_gpu", self.np_, ctypes.byref(d), ctypes.byref(n))
+        return capi.dtoh(d.value, n.value * 112, layout.CONTACT)
+
+    def solver_debug(self):
+        sc, cs, cc, co, bs = (ctypes.c_void_p() for _ in range(5))
+        n = ctypes.c_int()
+        call("b3h_pipeline_get_solver_debug", self.pl, ctypes.byref(sc), ctypes.byref(cs), ctypes.byref(cc),
+             ctypes.byref(co), ctypes.byref(bs), ctypes.byref(n))
+        k = n.value
+        return dict(sorted_contacts=capi.dtoh(sc.value, k * 112, layout.CONTACT),
+                    constraints=capi.dtoh(cs.value, k * 176, layout.CONSTRAINT),
+                    counts=capi.dtoh(cc.value, 256 * 4, np.int32), offsets=capi.dtoh(co.value, 257 * 4, np.int32),
+                    batch_sizes=capi.dtoh(bs.value, 256 * 4, np.int32))
+
+    def static0_index(self):
+        v = ctypes.c_int()
+        call("b3h_np_get_static0_index", self.np_, ctypes.byref(v))
+        return v.value
+
+    def enable_timing(self, on=True):
+        call("b3h_pipeline_enable_timing", self.pl, int(on))
+
+    def stage_times(self):
+        out = np.zeros(6, np.float32)
+        call("b3h_pipeline_get_stage_times", self.pl, ptr(out))
+        return out
+
+    def track_residual(self, on=True):
+        call("b3h_pipeline_set_track_residual", self.pl, int(on))
+
+    def residuals(self):
+        k = self.solver_params().m_numIterations
+        out = np.zeros(2 * k, np.float32)
+        call("b3h_pipeline_get_residuals", self.pl, ptr(out), len(out))
+        return out
+
+    def overflow(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        call("b3h_pipeline_get_overflow", self.pl, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def close(self):
+        if self.pl:
+            call("b3h_pipeline_destroy", self.pl)
+            self.pl = ctypes.c_void_p()
+        if self.bp:
+            call("b3h_bp_destroy", self.bp)
+            self.bp = ctypes.c_void_p()
+        if self.np_:
+            call("b3h_np_destroy", self.np_)
+            self.np_ = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,217 @@
+/*
+ * b3hip.h -- C ABI of the MI355X-native Bullet3 GPU rigid-body step.
+ *
+ * The three handle types stand in for the reference's three classes and every
+ * entry point names the method it replaces:
+ *   b3hNarrowPhase  <- b3GpuNarrowPhase          src/Bullet3OpenCL/RigidBody/b3GpuNarrowPhase.h:64-154
+ *   b3hBroadphase   <- b3GpuBroadphaseInterface  src/Bullet3OpenCL/BroadphaseCollision/b3GpuBroadphaseInterface.h:12-40
+ *                      (b3GpuGridBroadphase / b3GpuSapBroadphase)
+ *   b3hPipeline     <- b3GpuRigidBodyPipeline    src/Bullet3OpenCL/RigidBody/b3GpuRigidBodyPipeline.h:25-68
+ *
+ * Conventions: plain pointers and sizes only.  Pointers documented as "device"
+ * are HIP device pointers; all others are host pointers.  Every call returns an
+ * int status (B3H_OK = 0, negative on error; b3h_last_error() describes it).
+ * Where the reference returns an index (or -1 on failure) we return the status
+ * and write the index through an out-parameter.  Work is queued on the HIP
+ * stream given at creation (NULL = the library creates its own stream of
+ * default priority); calls that return data to the host synchronise that
+ * stream.  stepSimulation queues the whole step without host synchronisation.
+ * Not thread-safe per handle, like the reference.
+ */
+#ifndef B3HIP_H
+#define B3HIP_H
+
+#include "b3hip_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define B3H_OK 0
+#define B3H_ERR_INVALID -1
+#define B3H_ERR_CAPACITY -2
+#define B3H_ERR_HIP -3
+#define B3H_ERR_NO_DEVICE -4
+#define B3H_ERR_STATE -5
+
+typedef struct b3hNarrowPhase b3hNarrowPhase;
+typedef struct b3hBroadphase b3hBroadphase;
+typedef struct b3hPipeline b3hPipeline;
+
+#define B3H_BROADPHASE_GRID 0
+#define B3H_BROADPHASE_SAP 1
+
+/* Text of the last error on this thread ("" if none). */
+const char* b3h_last_error(void);
+/* Library version string. */
+const char* b3h_version(void);
+/* Default capacities: b3Config::b3Config, src/Bullet3Collision/NarrowPhaseCollision/b3Config.h:21-36 */
+int b3h_config_default(b3hConfig* cfg);
+/* Default solver parameters (the reference's hard-coded values, see b3hip_types.h) */
+int b3h_solver_params_default(b3hSolverParams* p);
+
+/* ---------------- b3GpuNarrowPhase ---------------- */
+/* b3GpuNarrowPhase::b3GpuNarrowPhase(ctx, device, queue, config), b3GpuNarrowPhase.cpp:18-93 */
+int b3h_np_create(const b3hConfig* cfg, void* hipStream, b3hNarrowPhase** out);
+/* b3GpuNarrowPhase::~b3GpuNarrowPhase, b3GpuNarrowPhase.cpp:95-140 */
+int b3h_np_destroy(b3hNarrowPhase* np);
+/* b3GpuNarrowPhase::registerConvexHullShape(const float* vertices, int strideInBytes,
+ * int numVertices, const float* scaling), b3GpuNarrowPhase.cpp:298-319; builds
+ * the hull, merges coplanar faces (b3ConvexUtility.cpp:26-248) and writes the
+ * new collidable index. */
+int b3h_np_register_convex_hull_shape(b3hNarrowPhase* np, const float* vertices, int strideInBytes, int numVertices,
+									  const float* scaling3, int* outCollidableIndex);
+/* b3GpuNarrowPhase::registerRigidBody(collidableIndex, mass, position, orientation,
+ * aabbMin, aabbMax, writeToGpu), b3GpuNarrowPhase.cpp:816-908 */
+int b3h_np_register_rigid_body(b3hNarrowPhase* np, int collidableIndex, float mass, const float* position3,
+							   const float* orientation4, const float* aabbMin3, const float* aabbMax3, int writeToGpu,
+							   int* outBodyIndex);
+/* b3GpuNarrowPhase::writeAllBodiesToGpu, b3GpuNarrowPhase.cpp:915-944 */
+int b3h_np_write_all_bodies_to_gpu(b3hNarrowPhase* np);
+/* b3GpuNarrowPhase::reset, b3GpuNarrowPhase.cpp:946-963 */
+int b3h_np_reset(b3hNarrowPhase* np);
+/* b3GpuNarrowPhase::readbackAllBodiesToCpu, b3GpuNarrowPhase.cpp:965-968; copies
+ * the device bodies into the host mirror and, if dst != NULL, into dst. */
+int b3h_np_readback_all_bodies_to_cpu(b3hNarrowPhase* np, b3hRigidBodyData* dst);
+/* b3GpuNarrowPhase::setObjectTransformCpu / setObjectVelocityCpu /
+ * getObjectTransformFromCpu, b3GpuNarrowPhase.cpp:969-1013 (host mirror) */
+int b3h_np_set_object_transform_cpu(b3hNarrowPhase* np, const float* position3, const float* orientation4, int bodyIndex);
+int b3h_np_set_object_velocity_cpu(b3hNarrowPhase* np, const float* linVel3, const float* angVel3, int bodyIndex);
+int b3h_np_get_object_transform_from_cpu(b3hNarrowPhase* np, float* position4, float* orientation4, int bodyIndex);
+/* b3GpuNarrowPhase::computeContacts(pairs, numPairs, aabbsWorldSpace, numObjects),
+ * b3GpuNarrowPhase.cpp:751-809.  pairs: device b3hInt4[numPairs]; .z receives the
+ * contact slot or -1. */
+int b3h_np_compute_contacts(b3hNarrowPhase* np, b3hInt4* pairs, int numPairs, const b3hAabb* aabbsWorldSpace,
+							int numObjects);
+/* getBodiesGpu/getNumRigidBodies, getBodyInertiasGpu, getCollidablesGpu,
+ * getContactsGpu/getNumContactsGpu, getStatic0Index: b3GpuNarrowPhase.h:110-136 */
+int b3h_np_get_bodies_gpu(b3hNarrowPhase* np, b3hRigidBodyData** dev, int* numBodies);
+int b3h_np_get_body_inertias_gpu(b3hNarrowPhase* np, b3hInertiaData** dev, int* numBodies);
+int b3h_np_get_collidables_gpu(b3hNarrowPhase* np, b3hCollidable** dev, int* numCollidables);
+int b3h_np_get_contacts_gpu(b3hNarrowPhase* np, b3hContact4Data** dev, int* numContacts);
+int b3h_np_get_static0_index(b3hNarrowPhase* np, int* index);
+int b3h_np_get_num_rigid_bodies(b3hNarrowPhase* np, int* n);
+/* getLocalSpaceAabb(collidableIndex), b3GpuNarrowPhase.cpp:811-814 (host) */
+int b3h_np_get_local_space_aabb(b3hNarrowPhase* np, int collidableIndex, b3hAabb* out);
+/* Host copies of the registered shape tables (collidables, hull records, hull
+ * vertices, unique edges, faces, face indices, local AABBs): counts[7] are
+ * written; arrays are filled when non-NULL (sized by a first call). */
+int b3h_np_get_shape_tables(b3hNarrowPhase* np, int* counts7, b3hCollidable* collidables,
+							b3hConvexPolyhedronData* convexes, b3hFloat4* vertices, b3hFloat4* uniqueEdges,
+							b3hGpuFace* faces, int* indices, b3hAabb* localAabbs);
+/* Host copy of the registered inertias. */
+int b3h_np_get_inertias_cpu(b3hNarrowPhase* np, b3hInertiaData* dst, int n);
+/* Narrowphase tuning: lanes cooperating on one pair (1 = one pair per lane). */
+int b3h_np_set_pairs_per_group(b3hNarrowPhase* np, int lanesPerPair);
+/* Per-pair separating-axis state and normal of the last computeContacts (device). */
+int b3h_np_get_sat_debug(b3hNarrowPhase* np, int** sepStateDev, b3hFloat4** sepNormalDev);
+
+/* ---------------- b3GpuBroadphaseInterface ---------------- */
+/* b3GpuGridBroadphase / b3GpuSapBroadphase constructors
+ * (b3GpuGridBroadphase.cpp:24-81, b3GpuSapBroadphase.cpp:20-170) */
+int b3h_bp_create(int kind, const b3hConfig* cfg, void* hipStream, b3hBroadphase** out);
+int b3h_bp_destroy(b3hBroadphase* bp);
+/* Grid cell size, grid dimensions (powers of two) and bucket cap; the reference
+ * hard-codes 3 m, 128^3 and 256 (b3GpuGridBroadphase.cpp:22,38-49). */
+int b3h_bp_set_grid_params(b3hBroadphase* bp, const float* cellSize3, const int* dims3, int maxBodiesPerCell);
+/* createProxy / createLargeProxy(aabbMin, aabbMax, userPtr, group, mask),
+ * b3GpuBroadphaseInterface.h:20-21 */
+int b3h_bp_create_proxy(b3hBroadphase* bp, const float* aabbMin3, const float* aabbMax3, int userPtr, int group,
+						int mask);
+int b3h_bp_create_large_proxy(b3hBroadphase* bp, const float* aabbMin3, const float* aabbMax3, int userPtr,
+							  int group, int mask);
+/* writeAabbsToGpu, b3GpuBroadphaseInterface.h:27 */
+int b3h_bp_write_aabbs_to_gpu(b3hBroadphase* bp);
+/* calculateOverlappingPairs(maxPairs), b3GpuBroadphaseInterface.h:24 (no host sync) */
+int b3h_bp_calculate_overlapping_pairs(b3hBroadphase* bp, int maxPairs);
+/* getAabbBufferWS (device b3hAabb[numAabbs]), getNumOverlap (syncs),
+ * getOverlappingPairBuffer (device b3hInt4[]): b3GpuBroadphaseInterface.h:30-34 */
+int b3h_bp_get_aabb_buffer_ws(b3hBroadphase* bp, b3hAabb** dev, int* numAabbs);
+int b3h_bp_get_num_overlap(b3hBroadphase* bp, int* n);
+int b3h_bp_get_overlapping_pair_buffer(b3hBroadphase* bp, b3hInt4** dev);
+/* Number of pairs found before clamping to maxPairs (syncs). */
+int b3h_bp_get_total_found(b3hBroadphase* bp, int* n);
+/* SAP axis chosen by the last calculateOverlappingPairs (syncs). */
+int b3h_bp_get_sap_axis(b3hBroadphase* bp, int* axis);
+/* Host access to the proxy lists (copies). */
+int b3h_bp_get_proxy_maps(b3hBroadphase* bp, int* counts2, int* smallMap, int* largeMap);
+
+/* ---------------- b3GpuRigidBodyPipeline ---------------- */
+/* b3GpuRigidBodyPipeline(ctx, device, queue, narrowphase, broadphaseSap,
+ * broadphaseDbvt, config), b3GpuRigidBodyPipeline.cpp:65-114.  np and bp stay
+ * owned by the caller. */
+int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const b3hConfig* cfg, void* hipStream,
+						b3hPipeline** out);
+int b3h_pipeline_destroy(b3hPipeline* p);
+/* stepSimulation(deltaTime), b3GpuRigidBodyPipeline.cpp:221-463 */
+int b3h_pipeline_step_simulation(b3hPipeline* p, float deltaTime);
+/* integrate(timeStep), b3GpuRigidBodyPipeline.cpp:465-498 */
+int b3h_pipeline_integrate(b3hPipeline* p, float timeStep);
+/* setupGpuAabbsFull, b3GpuRigidBodyPipeline.cpp:500-575 */
+int b3h_pipeline_setup_gpu_aabbs_full(b3hPipeline* p);
+/* b3GpuPgsContactSolver::solveContacts on the narrowphase's current contacts
+ * (b3GpuPgsContactSolver.cpp:568-1103) */
+int b3h_pipeline_solve_contacts(b3hPipeline* p);
+/* registerPhysicsInstance(mass, position, orientation, collidableIndex, userIndex,
+ * writeInstanceToGpu), b3GpuRigidBodyPipeline.cpp:603-669 */
+int b3h_pipeline_register_physics_instance(b3hPipeline* p, float mass, const float* position3,
+										   const float* orientation4, int collidableIndex, int userIndex,
+										   int writeInstanceToGpu, int* outBodyIndex);
+/* writeAllInstancesToGpu, b3GpuRigidBodyPipeline.cpp:597-601 */
+int b3h_pipeline_write_all_instances_to_gpu(b3hPipeline* p);
+/* setGravity, b3GpuRigidBodyPipeline.cpp:589-592 */
+int b3h_pipeline_set_gravity(b3hPipeline* p, const float* gravity3);
+/* reset, b3GpuRigidBodyPipeline.cpp:141-147 */
+int b3h_pipeline_reset(b3hPipeline* p);
+/* getBodyBuffer / getNumBodies, b3GpuRigidBodyPipeline.cpp:577-587 */
+int b3h_pipeline_get_body_buffer(b3hPipeline* p, b3hRigidBodyData** dev);
+int b3h_pipeline_get_num_bodies(b3hPipeline* p, int* n);
+/* Solver parameters (iterations, friction, dt, Baumgarte, drift, cell size,
+ * damping, warm start); the reference hard-codes them. */
+int b3h_pipeline_set_solver_params(b3hPipeline* p, const b3hSolverParams* prm);
+int b3h_pipeline_get_solver_params(b3hPipeline* p, b3hSolverParams* prm);
+/* Solver internals of the last solve (device pointers): contacts in solve
+ * order, constraints, per-cell counts / offsets / batch counts. */
+int b3h_pipeline_get_solver_debug(b3hPipeline* p, b3hContact4Data** sortedContacts,
+								  b3hContactConstraint4** constraints, int** cellCounts, int** cellOffsets,
+								  int** batchSizes, int* numConstraints);
+/* Per-sweep residuals of the last solve (host, 2*iterations floats: normal sweeps
+ * then friction sweeps); requires b3h_pipeline_set_track_residual(p, 1). */
+int b3h_pipeline_set_track_residual(b3hPipeline* p, int enable);
+int b3h_pipeline_get_residuals(b3hPipeline* p, float* out, int n);
+/* Initial impulses for the next solve (device float[6*numContacts] in solve
+ * order: 4 normal rows then 2 friction rows per constraint); NULL clears. */
+int b3h_pipeline_set_warm_start_impulses(b3hPipeline* p, const float* dev, int numConstraints);
+/* Stage timings of the last step in ms (enable first; adds events). Order:
+ * aabb, broadphase, narrowphase, solver setup, pgs, integrate. */
+int b3h_pipeline_enable_timing(b3hPipeline* p, int enable);
+int b3h_pipeline_get_stage_times(b3hPipeline* p, float* ms6);
+/* Overflow flags of the last step (pairs, contacts) -- syncs. */
+int b3h_pipeline_get_overflow(b3hPipeline* p, int* pairsOverflow, int* contactsOverflow);
+/* Capture stepSimulation(dt) into a HIP graph and replay it `count` times. */
+int b3h_pipeline_step_graph(b3hPipeline* p, float deltaTime, int count);
+
+/* ---------------- host-only utility ---------------- */
+/* Polyhedral features of a convex point cloud exactly as registerConvexHullShape
+ * stores them (b3ConvexUtility::initializePolyhedralFeatures,
+ * src/Bullet3Collision/NarrowPhaseCollision/b3ConvexUtility.cpp:26-248); no
+ * device needed.  counts4 = {numVertices, numFaces, numIndices, numUniqueEdges};
+ * arrays (capacity `cap` entries each) are filled when non-NULL: vertices and
+ * planes and edges as float4, faceIndexCounts and indices as int. */
+int b3h_compute_polyhedral_features(const float* vertices, int strideInBytes, int numVertices, int cap, int* counts4,
+									float* vertices4, float* planes4, int* faceIndexCounts, int* indices, float* edges4);
+
+/* ---------------- device memory helpers for bindings ---------------- */
+int b3h_device_count(int* n);
+int b3h_malloc(void** dev, size_t bytes);
+int b3h_free(void* dev);
+int b3h_memcpy_htod(void* dev, const void* host, size_t bytes);
+int b3h_memcpy_dtoh(void* host, const void* dev, size_t bytes);
+int b3h_memcpy_dtod(void* dst, const void* src, size_t bytes);
+int b3h_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* B3HIP_H */
