@@ -105,6 +105,8 @@ _SIGS = {
     "b3h_pipeline_get_stage_times": ([_vp, _vp], _i),
     "b3h_pipeline_get_overflow": ([_vp, _pi, _pi], _i),
     "b3h_pipeline_step_graph": ([_vp, _f, _i], _i),
+    "b3h_radix_sort32": ([_vp, _vp, _i, _i], _i),
+    "b3h_prefix_scan": ([_vp, _vp, _i, _pi], _i),
     "b3h_compute_polyhedral_features": ([_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "b3h_device_count": ([_pi], _i),
     "b3h_malloc": ([_pvp, _sz], _i),
@@ -231,3 +233,26 @@ def polyhedral_features(points, cap=4096):
         faces.append(idx[off:off + fc[f]].copy())
         off += fc[f]
     return dict(vertices=v[:nv, :3].copy(), planes=p[:nf].copy(), faces=faces, edges=e[:ne, :3].copy())
+
+
+def radix_sort32(keys, values, sort_bits=32):
+    """Stable (key, value) sort on the device by the low sort_bits key bits."""
+    k = np.ascontiguousarray(keys, np.uint32)
+    v = np.ascontiguousarray(values, np.uint32)
+    n = len(k)
+    dk, dv = DeviceBuffer(4 * max(n, 1)), DeviceBuffer(4 * max(n, 1))
+    htod(dk.ptr, k)
+    htod(dv.ptr, v)
+    call("b3h_radix_sort32", ctypes.c_void_p(dk.ptr), ctypes.c_void_p(dv.ptr), n, sort_bits)
+    return dtoh(dk.ptr, 4 * n, np.uint32).copy(), dtoh(dv.ptr, 4 * n, np.uint32).copy()
+
+
+def prefix_scan(values):
+    """Exclusive prefix sum on the device; returns (scan of n+1 ints, total)."""
+    a = np.ascontiguousarray(values, np.int32)
+    n = len(a)
+    ds, dd = DeviceBuffer(4 * max(n, 1)), DeviceBuffer(4 * (n + 1))
+    htod(ds.ptr, a)
+    total = ctypes.c_int()
+    call("b3h_prefix_scan", ctypes.c_void_p(ds.ptr), ctypes.c_void_p(dd.ptr), n, ctypes.byref(total))
+    return dtoh(dd.ptr, 4 * (n + 1), np.int32).copy(), total.value

@@ -639,7 +639,8 @@ struct b3hBroadphase
 {
 	int kind;
 	b3hConfig cfg;
-	hipStream_t stream;
+	hipStream_t stream;     /* stream every call is issued on */
+	hipStream_t homeStream; /* the stream given at create (or owned) */
 	bool ownStream;
 	std::vector<b3hAabb> all;
 	std::vector<int> smallMap, largeMap;
@@ -697,7 +698,14 @@ extern "C" int b3h_bp_create(int kind, const b3hConfig* cfg, void* stream, b3hBr
 	}
 	bp->gp.maxBodiesPerCell = 256;
 	bp->numCells = dims[0] * dims[1] * dims[2];
-	if ((r = make_stream(stream, &bp->stream, &bp->ownStream)) || (r = dalloc(&bp->dAll, cfg->m_maxConvexBodies)) ||
+	bp->ownStream = false;
+	if ((r = make_stream(stream, &bp->homeStream, &bp->ownStream)))
+	{
+		delete bp;
+		return r;
+	}
+	bp->stream = bp->homeStream;
+	if ((r = dalloc(&bp->dAll, cfg->m_maxConvexBodies)) ||
 		(r = dalloc(&bp->dPairs, cfg->m_maxBroadphasePairs)) || (r = dalloc(&bp->dPairCount, 2)) ||
 		(r = dalloc(&bp->dSmall, cfg->m_maxConvexBodies)) || (r = dalloc(&bp->dLarge, cfg->m_maxConvexBodies)))
 	{
@@ -719,7 +727,7 @@ extern "C" int b3h_bp_destroy(b3hBroadphase* bp)
 	dfree(bp->dLarge);
 	dfree(bp->dPairs);
 	dfree(bp->dPairCount);
-	if (bp->ownStream) hipStreamDestroy(bp->stream);
+	if (bp->ownStream) hipStreamDestroy(bp->homeStream);
 	delete bp;
 	return B3H_OK;
 }
@@ -941,12 +949,15 @@ extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const 
 	p->dWarm = nullptr;
 	p->warmCount = 0;
 	int r;
-	/* one stream for all three objects keeps the step in order; the
-	 * narrowphase's and broadphase's own streams are used for their
-	 * stand-alone calls */
+	/* one stream for all three objects keeps the step in order: while the
+	 * pipeline exists the broadphase issues its stand-alone calls on the
+	 * narrowphase's stream too, so they are ordered after the pipeline's
+	 * AABB update */
 	p->stream = np->stream;
 	p->ownStream = false;
 	(void)stream;
+	hipStreamSynchronize(bp->stream);
+	bp->stream = np->stream;
 	if ((r = b3h_solver_module_init()))
 	{
 		delete p;
@@ -975,6 +986,7 @@ extern "C" int b3h_pipeline_destroy(b3hPipeline* p)
 {
 	if (!p) return B3H_OK;
 	hipStreamSynchronize(p->stream);
+	p->bp->stream = p->bp->homeStream;
 	if (p->graphExec) hipGraphExecDestroy(p->graphExec);
 	if (p->graph) hipGraphDestroy(p->graph);
 	if (p->evValid)
@@ -1279,6 +1291,44 @@ extern "C" int b3h_pipeline_get_overflow(b3hPipeline* p, int* po, int* co)
 	B3H_CHECK(hipStreamSynchronize(p->stream));
 	if (po) *po = pc[1] > pc[0];
 	if (co) *co = cc[1] > cc[0];
+	return B3H_OK;
+}
+
+extern "C" int b3h_radix_sort32(unsigned* keys, unsigned* vals, int n, int bits)
+{
+	CHECK_ARG(n >= 0 && bits >= 0 && bits <= 32);
+	if (n == 0) return B3H_OK;
+	CHECK_ARG(keys && vals);
+	int r = ensure_device();
+	if (r) return r;
+	unsigned *tk = nullptr, *tv = nullptr, *h = nullptr;
+	if ((r = dalloc(&tk, n)) || (r = dalloc(&tv, n)) || (r = dalloc(&h, b3h_radix_hist_size(n))))
+	{
+		dfree(tk);
+		dfree(tv);
+		return r;
+	}
+	b3h_radix_sort(keys, vals, tk, tv, h, n, bits, 0);
+	hipError_t e = hipDeviceSynchronize();
+	dfree(tk);
+	dfree(tv);
+	dfree(h);
+	B3H_CHECK(e);
+	return B3H_OK;
+}
+
+extern "C" int b3h_prefix_scan(const int* src, int* dst, int n, int* sumOut)
+{
+	CHECK_ARG(n >= 0 && src && dst);
+	int r = ensure_device();
+	if (r) return r;
+	unsigned* tmp = nullptr;
+	if ((r = dalloc(&tmp, b3h_scan_tmp_size(n)))) return r;
+	b3h_exclusive_scan(src, dst, nullptr, n, tmp, 0);
+	hipError_t e = hipDeviceSynchronize();
+	if (e == hipSuccess && sumOut) e = hipMemcpy(sumOut, dst + n, sizeof(int), hipMemcpyDeviceToHost);
+	dfree(tmp);
+	B3H_CHECK(e);
 	return B3H_OK;
 }
 

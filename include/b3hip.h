@@ -191,6 +191,16 @@ int b3h_pipeline_get_overflow(b3hPipeline* p, int* pairsOverflow, int* contactsO
 /* Capture stepSimulation(dt) into a HIP graph and replay it `count` times. */
 int b3h_pipeline_step_graph(b3hPipeline* p, float deltaTime, int count);
 
+/* ---------------- parallel primitives ---------------- */
+/* Stable sort of n (key, value) pairs (b3hSortData, device) by the low
+ * sortBits bits of the key; b3RadixSort32CL::execute(keyValues, sortBits),
+ * src/Bullet3OpenCL/ParallelPrimitives/b3RadixSort32CL.h:84.  Keys and values
+ * are taken as separate device arrays of uint32. */
+int b3h_radix_sort32(unsigned* keysDev, unsigned* valuesDev, int n, int sortBits);
+/* Exclusive prefix sum of n ints (device) into dst (n+1 ints, dst[n] = total);
+ * b3PrefixScanCL::execute(src, dst, n, sum), b3PrefixScanCL.h:24. */
+int b3h_prefix_scan(const int* srcDev, int* dstDev, int n, int* sumOut);
+
 /* ---------------- host-only utility ---------------- */
 /* Polyhedral features of a convex point cloud exactly as registerConvexHullShape
  * stores them (b3ConvexUtility::initializePolyhedralFeatures,

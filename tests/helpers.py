@@ -96,7 +96,8 @@ def body_mismatch(a, b, fields=("pos", "quat", "linVel", "angVel")):
 
 
 def bits_equal(a, b):
-    return np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8))
+    """identical bytes (shapes/dtypes of the two record views may differ)"""
+    return np.ascontiguousarray(a).tobytes() == np.ascontiguousarray(b).tobytes()
 
 
 def sort_pairs(p):
@@ -106,3 +107,22 @@ def sort_pairs(p):
     q = np.stack([lo, hi], 1)
     order = np.lexsort((q[:, 1], q[:, 0]))
     return q[order]
+
+
+def first_diff(a, b, limit=4):
+    """describe where two record arrays differ (first records, differing fields)"""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b).view(a.dtype).reshape(a.shape)
+    if a.shape != b.shape:
+        return "shape %s vs %s" % (a.shape, b.shape)
+    ab = a.view(np.uint8).reshape(len(a), -1)
+    bb = b.view(np.uint8).reshape(len(b), -1)
+    rows = np.nonzero(np.any(ab != bb, axis=1))[0]
+    out = ["%d of %d records differ" % (len(rows), len(a))]
+    for r in rows[:limit]:
+        if a.dtype.names:
+            fs = [f for f in a.dtype.names if a[f][r].tobytes() != b[f][r].tobytes()]
+            out.append("rec %d: " % r + "; ".join("%s %s vs %s" % (f, a[f][r], b[f][r]) for f in fs))
+        else:
+            out.append("rec %d: %s vs %s" % (r, a[r], b[r]))
+    return "\n".join(out)

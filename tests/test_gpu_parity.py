@@ -14,7 +14,7 @@ import pytest
 import pyoracle as po
 from bullet3_amd import capi, layout, scenes
 from bullet3_amd.world import World
-from helpers import OracleWorld, bits_equal, sort_pairs
+from helpers import OracleWorld, bits_equal, first_diff, sort_pairs
 
 pytestmark = pytest.mark.gpu
 
@@ -61,7 +61,7 @@ def test_integrate_bit_exact(scene):
     for dt in (1.0 / 60.0, 1.0 / 240.0):
         capi.call("b3h_pipeline_integrate", w.pl, ctypes.c_float(dt))
         po.integrate(b, dt, 0.99, (0.0, -9.8, 0.0))
-        assert bits_equal(w.bodies(), b)
+        assert bits_equal(w.bodies(), b), first_diff(w.bodies(), b)
     w.close()
 
 
@@ -137,8 +137,9 @@ def test_solver_bit_exact(scene):
     assert np.array_equal(dbg["offsets"][:256], cpu["offsets"].astype(np.int32))
     assert np.array_equal(dbg["batch_sizes"], cpu["batch_sizes"])
     assert bits_equal(dbg["sorted_contacts"], cpu["sorted_contacts"].view(layout.CONTACT))
-    assert bits_equal(dbg["constraints"], cpu["constraints"].view(layout.CONSTRAINT))
-    assert bits_equal(w.bodies(), b)
+    assert bits_equal(dbg["constraints"], cpu["constraints"].view(layout.CONSTRAINT)), \
+        first_diff(dbg["constraints"], cpu["constraints"].view(layout.CONSTRAINT))
+    assert bits_equal(w.bodies(), b), first_diff(w.bodies(), b)
     w.close()
 
 
