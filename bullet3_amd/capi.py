@@ -83,6 +83,7 @@ _SIGS = {
     "b3h_bp_get_total_found": ([_vp, _pi], _i),
     "b3h_bp_get_sap_axis": ([_vp, _pi], _i),
     "b3h_bp_get_proxy_maps": ([_vp, _vp, _vp, _vp], _i),
+    "b3h_bp_get_grid_sort": ([_vp, _pvp, _pvp, _pi], _i),
     "b3h_pipeline_create": ([_vp, _vp, ctypes.POINTER(Config), _vp, _pvp], _i),
     "b3h_pipeline_destroy": ([_vp], _i),
     "b3h_pipeline_step_simulation": ([_vp, _f], _i),
@@ -155,7 +156,9 @@ def ptr(a):
     if isinstance(a, int):
         return ctypes.c_void_p(a)
     assert a.flags["C_CONTIGUOUS"], "array must be contiguous"
-    return ctypes.c_void_p(a.ctypes.data)
+    # data_as keeps a reference to the array, so a temporary argument stays
+    # alive for the duration of the foreign call
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 def default_config(max_bodies=None, pairs_per_body=16):

@@ -878,6 +878,15 @@ extern "C" int b3h_bp_get_sap_axis(b3hBroadphase* bp, int* axis)
 	}
 	return read_int(bp->sc.axis, axis, bp->stream);
 }
+extern "C" int b3h_bp_get_grid_sort(b3hBroadphase* bp, const unsigned** k, const unsigned** v, int* n)
+{
+	CHECK_ARG(bp && k && v && n);
+	*k = bp->sc.keys;
+	*v = bp->sc.vals;
+	*n = (int)bp->smallMap.size();
+	return B3H_OK;
+}
+
 extern "C" int b3h_bp_get_proxy_maps(b3hBroadphase* bp, int* counts2, int* sm, int* lm)
 {
 	CHECK_ARG(bp && counts2);

@@ -33,6 +33,10 @@ B3H_HD F4 sub4(F4 a, F4 b) { return f4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.
 B3H_HD F4 mul4(F4 a, float s) { return f4(a.x * s, a.y * s, a.z * s, a.w * s); }
 B3H_HD F4 smul4(float s, F4 a) { return f4(s * a.x, s * a.y, s * a.z, s * a.w); }
 B3H_HD F4 neg4(F4 a) { return f4(-a.x, -a.y, -a.z, -a.w); }
+/* OpenCL C's common max/min: max(x,y) = x < y ? y : x, min(x,y) = y < x ? y : x
+ * (this fixes which zero a clamp at +-0 returns) */
+B3H_HD float cl_max(float x, float y) { return x < y ? y : x; }
+B3H_HD float cl_min(float x, float y) { return y < x ? y : x; }
 B3H_HD float dot3(F4 a, F4 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 B3H_HD float dot4(F4 a, F4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
 B3H_HD F4 cross3(F4 a, F4 b)

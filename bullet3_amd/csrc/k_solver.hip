@@ -405,8 +405,8 @@ __device__ void solve_one(b3hContactConstraint4* __restrict__ csp, b3hRigidBodyD
 				float prev = cs.m_appliedRambdaDt[ic];
 				float upd = prev;
 				upd += rl;
-				upd = fmaxf(upd, 0.f);
-				upd = fminf(upd, FLT_MAX);
+				upd = cl_max(upd, 0.f);
+				upd = cl_min(upd, FLT_MAX);
 				rl = upd - prev;
 				cs.m_appliedRambdaDt[ic] = upd;
 				maxD = fmaxf(maxD, fabsf(rl));
@@ -414,7 +414,7 @@ __device__ void solve_one(b3hContactConstraint4* __restrict__ csp, b3hRigidBodyD
 			}
 			else
 			{
-				rl = fmaxf(warm[ic], 0.f);
+				rl = cl_max(warm[ic], 0.f);
 				cs.m_appliedRambdaDt[ic] = rl;
 			}
 			apply_impulse(linear, ang0, ang1, rl, imA, iiA, imB, iiB, va, vb);
@@ -433,7 +433,7 @@ __device__ void solve_one(b3hContactConstraint4* __restrict__ csp, b3hRigidBodyD
 				F4 linear = f4(-t[i].x, -t[i].y, -t[i].z, 0.f);
 				F4 ang0 = neg4(cross3(r0, t[i]));
 				F4 ang1 = cross3(r1, t[i]);
-				float l = fminf(fmaxf(warm[4 + i], -maxR), maxR);
+				float l = cl_min(cl_max(warm[4 + i], -maxR), maxR);
 				cs.m_fAppliedRambdaDt[i] = l;
 				apply_impulse(linear, ang0, ang1, l, imA, iiA, imB, iiB, va, vb);
 			}
@@ -461,8 +461,8 @@ __device__ void solve_one(b3hContactConstraint4* __restrict__ csp, b3hRigidBodyD
 			float prev = cs.m_fAppliedRambdaDt[i];
 			float upd = prev;
 			upd += rl;
-			upd = fmaxf(upd, minR);
-			upd = fminf(upd, maxR);
+			upd = cl_max(upd, minR);
+			upd = cl_min(upd, maxR);
 			rl = upd - prev;
 			cs.m_fAppliedRambdaDt[i] = upd;
 			maxD = fmaxf(maxD, fabsf(rl));

@@ -34,8 +34,9 @@ class World:
         call("b3h_bp_create", kind, ctypes.byref(self.cfg), None, ctypes.byref(self.bp))
         cs = float(cell_size if cell_size is not None else scene.cell_size)
         self.cell_size = cs
-        call("b3h_bp_set_grid_params", self.bp, ptr(np.array([cs, cs, cs], np.float32)),
-             ptr(np.array(grid_dims, np.int32)), int(max_bodies_per_cell))
+        cell3 = np.array([cs, cs, cs], np.float32)
+        dims3 = np.array(grid_dims, np.int32)
+        call("b3h_bp_set_grid_params", self.bp, ptr(cell3), ptr(dims3), int(max_bodies_per_cell))
         call("b3h_pipeline_create", self.np_, self.bp, ctypes.byref(self.cfg), None, ctypes.byref(self.pl))
         self.collidables = []
         for pts in scene.shapes:

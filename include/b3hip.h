@@ -135,6 +135,10 @@ int b3h_bp_get_total_found(b3hBroadphase* bp, int* n);
 int b3h_bp_get_sap_axis(b3hBroadphase* bp, int* axis);
 /* Host access to the proxy lists (copies). */
 int b3h_bp_get_proxy_maps(b3hBroadphase* bp, int* counts2, int* smallMap, int* largeMap);
+/* Debug view of the grid's last (cell hash, small-proxy index) arrays after the
+ * sort, in device memory (n entries each) -- b3GpuGridBroadphase::m_hashGpu,
+ * b3GpuGridBroadphase.h:21; no public accessor there. */
+int b3h_bp_get_grid_sort(b3hBroadphase* bp, const unsigned** keysDev, const unsigned** valsDev, int* n);
 
 /* ---------------- b3GpuRigidBodyPipeline ---------------- */
 /* b3GpuRigidBodyPipeline(ctx, device, queue, narrowphase, broadphaseSap,

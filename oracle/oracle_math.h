@@ -25,6 +25,10 @@ static inline F4 sub4(F4 a, F4 b) { return f4(a.x - b.x, a.y - b.y, a.z - b.z, a
 static inline F4 mul4(F4 a, float s) { return f4(a.x * s, a.y * s, a.z * s, a.w * s); }
 static inline F4 smul4(float s, F4 a) { return f4(s * a.x, s * a.y, s * a.z, s * a.w); }
 static inline F4 neg4(F4 a) { return f4(-a.x, -a.y, -a.z, -a.w); }
+/* OpenCL C's common max/min: max(x,y) = x < y ? y : x, min(x,y) = y < x ? y : x
+ * (this fixes which zero a clamp at +-0 returns) */
+static inline float cl_max(float x, float y) { return x < y ? y : x; }
+static inline float cl_min(float x, float y) { return y < x ? y : x; }
 static inline float dot3(F4 a, F4 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static inline float dot4(F4 a, F4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
 static inline F4 cross3(F4 a, F4 b)

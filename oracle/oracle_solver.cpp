@@ -222,8 +222,8 @@ static void solve_contact(b3hContactConstraint4& cs, const b3hRigidBodyData& A, 
 		float prev = cs.m_appliedRambdaDt[ic];
 		float upd = prev;
 		upd += rl;
-		upd = fmaxf(upd, 0.f);
-		upd = fminf(upd, FLT_MAX);
+		upd = cl_max(upd, 0.f);
+		upd = cl_min(upd, FLT_MAX);
 		rl = upd - prev;
 		cs.m_appliedRambdaDt[ic] = upd;
 		float ad = fabsf(rl), al = fabsf(upd);
@@ -258,8 +258,8 @@ static bool solve_friction(b3hContactConstraint4& cs, float friction, const b3hR
 		float prev = cs.m_fAppliedRambdaDt[i];
 		float upd = prev;
 		upd += rl;
-		upd = fmaxf(upd, minR);
-		upd = fminf(upd, maxR);
+		upd = cl_max(upd, minR);
+		upd = cl_min(upd, maxR);
 		rl = upd - prev;
 		cs.m_fAppliedRambdaDt[i] = upd;
 		float ad = fabsf(rl), al = fabsf(upd);
@@ -325,7 +325,7 @@ static void warm_start(b3hContactConstraint4& cs, const float* l0, float frictio
 		F4 linear = f4(-n.x, -n.y, -n.z, 0.f);
 		F4 ang0 = neg4(cross3(r0, n));
 		F4 ang1 = cross3(r1, n);
-		float l = fmaxf(l0[ic], 0.f);
+		float l = cl_max(l0[ic], 0.f);
 		cs.m_appliedRambdaDt[ic] = l;
 		apply_impulse(linear, ang0, ang1, l, A.m_invMass, iiA, B.m_invMass, iiB, va, vb);
 	}
@@ -343,7 +343,7 @@ static void warm_start(b3hContactConstraint4& cs, const float* l0, float frictio
 			F4 linear = f4(-t[i].x, -t[i].y, -t[i].z, 0.f);
 			F4 ang0 = neg4(cross3(r0, t[i]));
 			F4 ang1 = cross3(r1, t[i]);
-			float l = fminf(fmaxf(l0[4 + i], -maxR), maxR);
+			float l = cl_min(cl_max(l0[4 + i], -maxR), maxR);
 			cs.m_fAppliedRambdaDt[i] = l;
 			apply_impulse(linear, ang0, ang1, l, A.m_invMass, iiA, B.m_invMass, iiB, va, vb);
 		}

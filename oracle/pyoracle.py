@@ -28,7 +28,8 @@ def ptr(a):
     if a is None:
         return None
     assert a.flags["C_CONTIGUOUS"]
-    return ctypes.c_void_p(a.ctypes.data)
+    # data_as keeps a reference to the array (temporaries stay alive)
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 class SolverParams(ctypes.Structure):

@@ -65,6 +65,18 @@ def test_integrate_bit_exact(scene):
     w.close()
 
 
+def grid_hash_order(aabbs, small, cell, dims=128):
+    """(cell hash, proxy) of every small proxy, stably sorted by hash
+    (gridBroadphase.cl:38-49 hash of the AABB centre; float32 arithmetic)"""
+    a = aabbs[small]
+    c = (a["min"][:, :3] + a["max"][:, :3]) * np.float32(0.5)
+    inv = np.float32(1.0) / np.float32(cell)
+    g = np.floor(c * inv).astype(np.int64) & (dims - 1)
+    h = (g[:, 2] * dims * dims + g[:, 1] * dims + g[:, 0]).astype(np.uint32)
+    order = np.argsort(h, kind="stable").astype(np.uint32)
+    return h[order], order
+
+
 @pytest.mark.parametrize("kind", ["grid", "sap"])
 def test_pairs_bit_exact(scene, kind):
     w = World(scene, broadphase=kind)
@@ -76,6 +88,13 @@ def test_pairs_bit_exact(scene, kind):
     aabbs = w.world_aabbs()
     if kind == "grid":
         cs = w.cell_size
+        kp, vp, nn = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_int()
+        capi.call("b3h_bp_get_grid_sort", w.bp, ctypes.byref(kp), ctypes.byref(vp), ctypes.byref(nn))
+        keys = capi.dtoh(kp.value, 4 * nn.value, np.uint32)
+        vals = capi.dtoh(vp.value, 4 * nn.value, np.uint32)
+        ek, ev = grid_hash_order(aabbs, ow.small, cs)
+        assert np.array_equal(keys, ek), "hash keys: %s" % first_diff(keys, ek)
+        assert np.array_equal(vals, ev), "proxy order: %s" % first_diff(vals, ev)
         cpu, _ = po.grid_pairs(aabbs, ow.small, ow.large, cell=(cs, cs, cs))
     else:
         axis = ctypes.c_int()
