@@ -7,6 +7,28 @@
  * bit for bit): no FMA contraction (built with -ffp-contract=off), left-to-right
  * evaluation, dot3 = (x*x' + y*y') + z*z', cross3 as in the OpenCL builtin,
  * correctly rounded / and sqrtf, and the short polynomial sine/cosine below.
+ *
+ * What pins each part (tests/):
+ *   world AABBs, integration      the reference's own C++ (oracle/_ref), bit for bit
+ *                                 or within 2e-6; closed forms of free fall
+ *   MPR                           the reference's known-answer kernel dumps
+ *                                 (counts 0, 1, 46, 98) and its own C++ MPR
+ *   SAT (faces, edge-edge, 162)   float64 brute force of the minimum overlap over
+ *                                 the same candidate axes; box closed forms
+ *   clipping + manifold reduction box/ground and tilted-box closed forms (points
+ *                                 are the penetrating corners, depth -delta).
+ *                                 Which 3 of 4 equal-depth corners the reduction
+ *                                 keeps is parity unpinned (only the GPU kernel
+ *                                 is compared with it, bit for bit)
+ *   grid / SAP pairs              brute force, as sets; the stable key sort
+ *                                 against numpy's stable argsort
+ *   batching                      invariants (every contact batched, no shared
+ *                                 dynamic body in a 64-row, cells partition)
+ *   constraint setup              the reference's own C++ setConstraint4
+ *   PGS sweeps                    closed forms: weight support, penetration bias,
+ *                                 no pull, friction bound, momentum conservation
+ *   OpenCL native_* rounding      parity unpinned: the reference kernels cannot
+ *                                 run here; correctly rounded operations are used
  */
 #ifndef B3ORACLE_MATH_H
 #define B3ORACLE_MATH_H

@@ -1,0 +1,145 @@
+"""Host-only construction of the oracle's world for CPU tests (no device needed).
+
+Shape tables come from the library's host hull routine (the one behind
+registerConvexHullShape, b3GpuNarrowPhase.cpp:298-368); bodies and inertias
+follow registerRigidBody (b3GpuNarrowPhase.cpp:816-908) and the registration
+AABB of registerPhysicsInstance (b3GpuRigidBodyPipeline.cpp:603-669, margin
+0.01).  The arithmetic here is numpy float32, so records can differ from the
+library's in the last bit; that is irrelevant for the closed-form and
+invariant checks that use it (the GPU tests take the library's own records).
+"""
+import ctypes
+
+import numpy as np
+
+import pyoracle as po
+from bullet3_amd import capi, layout
+
+SHAPE_CONVEX_HULL = 3
+
+
+def quat_basis(q):
+    x, y, z, w = (np.float64(v) for v in q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def shape_tables(shapes):
+    cols, convs, verts, edges, faces, idx, laabbs = [], [], [], [], [], [], []
+    for s, pts in enumerate(shapes):
+        f = capi.polyhedral_features(np.asarray(pts, np.float32))
+        v = f["vertices"].astype(np.float32)
+        cv = np.zeros(1, layout.CONVEX)[0]
+        c = np.zeros(3, np.float32)
+        for p in v:
+            c = c + p
+        c = c * np.float32(1.0 / len(v))
+        cv["localCenter"][:3] = c
+        cv["uniqueEdgesOffset"] = len(edges)
+        cv["numUniqueEdges"] = len(f["edges"])
+        edges.extend(np.concatenate([f["edges"], np.zeros((len(f["edges"]), 1), np.float32)], 1))
+        cv["faceOffset"] = len(faces)
+        cv["numFaces"] = len(f["faces"])
+        for plane, ind in zip(f["planes"], f["faces"]):
+            fr = np.zeros(1, layout.FACE)[0]
+            fr["plane"] = plane
+            fr["indexOffset"] = len(idx)
+            fr["numIndices"] = len(ind)
+            idx.extend(int(i) for i in ind)
+            faces.append(fr)
+        cv["vertexOffset"] = len(verts)
+        cv["numVertices"] = len(v)
+        verts.extend(np.concatenate([v, np.zeros((len(v), 1), np.float32)], 1))
+        col = np.zeros(1, layout.COLLIDABLE)[0]
+        col["shapeType"] = SHAPE_CONVEX_HULL
+        col["shapeIndex"] = s
+        cols.append(col)
+        convs.append(cv)
+        la = np.zeros(1, layout.AABB)[0]
+        la["min"][:3] = v.min(0)
+        la["max"][:3] = v.max(0)
+        laabbs.append(la)
+    return dict(collidables=np.array(cols, layout.COLLIDABLE), convexes=np.array(convs, layout.CONVEX),
+                vertices=np.array(verts, np.float32).reshape(-1, 4),
+                uniqueEdges=np.array(edges, np.float32).reshape(-1, 4) if edges else np.zeros((1, 4), np.float32),
+                faces=np.array(faces, layout.FACE), indices=np.array(idx, np.int32),
+                localAabbs=np.array(laabbs, layout.AABB))
+
+
+def bodies_and_inertias(scene, tables, margin=0.01):
+    n = scene.num_bodies
+    b = np.zeros(n, layout.BODY)
+    inr = np.zeros(n, layout.INERTIA)
+    for i in range(n):
+        m = np.float32(scene.mass[i])
+        b["pos"][i, :3] = scene.pos[i]
+        b["quat"][i] = scene.orn[i]
+        b["collidableIdx"][i] = int(scene.shape[i])
+        b["invMass"][i] = np.float32(1.0) / m if m else np.float32(0.0)
+        b["friction"][i] = 1.0
+        if not m:
+            continue
+        la = tables["localAabbs"][int(scene.shape[i])]
+        R = quat_basis(scene.orn[i])
+        he = 0.5 * (la["max"][:3] - la["min"][:3]) + margin
+        e = np.abs(R) @ he
+        lx, ly, lz = (2.0 * 2.0 * e).astype(np.float32)
+        li = np.array([(m / 12.0) * (ly * ly + lz * lz), (m / 12.0) * (lx * lx + lz * lz),
+                       (m / 12.0) * (lx * lx + ly * ly)], np.float32)
+        il = (np.float32(1.0) / li).astype(np.float32)
+        inr["initInvInertia"][i, :, :3] = np.diag(il)
+        inr["invInertiaWorld"][i, :, :3] = (R * il) @ R.T
+    return b, inr
+
+
+class HostWorld:
+    """The oracle's world description built on the host from a Scene."""
+
+    def __init__(self, scene, broadphase=0, solver=None, cell_size=None, max_pairs=None, max_contacts=None):
+        self.t = shape_tables(scene.shapes)
+        self.bodies, self.inertias = bodies_and_inertias(scene, self.t)
+        dyn = np.asarray(scene.mass) > 0
+        self.small = np.nonzero(dyn)[0].astype(np.int32)
+        self.large = np.nonzero(~dyn)[0].astype(np.int32)
+        n = scene.num_bodies
+        self.max_pairs = max_pairs or 32 * n + 64
+        self.max_contacts = max_contacts or 16 * n + 64
+        self.solver = solver or po.solver_params()
+        self.aabbs = np.zeros(n, layout.AABB)
+        self.pairs = np.zeros((self.max_pairs, 4), np.int32)
+        self.contacts = np.zeros(self.max_contacts, layout.CONTACT)
+        self.resid = np.zeros(2 * max(1, self.solver.m_numIterations), np.float32)
+        w = po.World()
+        w.numBodies = n
+        w.bodies = self.bodies.ctypes.data
+        w.inertias = self.inertias.ctypes.data
+        for k in ("collidables", "convexes", "vertices", "uniqueEdges", "faces", "indices", "localAabbs"):
+            setattr(w, k, self.t[k].ctypes.data)
+        w.smallMap = self.small.ctypes.data if len(self.small) else None
+        w.nSmall = len(self.small)
+        w.largeMap = self.large.ctypes.data if len(self.large) else None
+        w.nLarge = len(self.large)
+        w.broadphase = broadphase
+        cs = float(cell_size or scene.cell_size)
+        w.cellSize[0] = w.cellSize[1] = w.cellSize[2] = cs
+        for k in range(3):
+            w.gridDims[k] = 128
+        w.maxBodiesPerCell = 256
+        w.maxPairs = self.max_pairs
+        w.maxContacts = self.max_contacts
+        w.solver = self.solver
+        w.gravity[0], w.gravity[1], w.gravity[2] = 0.0, -9.8, 0.0
+        w.staticIdx = int(self.large[0]) if len(self.large) and self.large[0] == 0 else -1
+        w.worldAabbs = self.aabbs.ctypes.data
+        w.pairs = self.pairs.ctypes.data
+        w.contacts = self.contacts.ctypes.data
+        w.residuals = self.resid.ctypes.data
+        self.w = w
+
+    def step(self, dt=1.0 / 60.0):
+        po.lib().b3o_step(ctypes.byref(self.w), ctypes.c_float(dt))
+
+    @property
+    def num_contacts(self):
+        return self.w.numContacts
