@@ -21,6 +21,7 @@
 #ifndef B3HIP_H
 #define B3HIP_H
 
+#include <stddef.h>
 #include "b3hip_types.h"
 
 #ifdef __cplusplus

@@ -135,3 +135,16 @@ def test_hull_matches_reference_convex_utility(name, pts):
     assert matched >= 0.9 * len(a["faces"])
     if matched == len(a["faces"]):
         assert canon_dirs(a["edges"]) == canon_dirs(b["edges"])
+
+
+def test_public_headers_compile_standalone(tmp_path):
+    """include/b3hip.h is valid C and C++ on its own; include/b3hip.hpp is valid C++"""
+    import subprocess
+    inc = os.path.join(ROOT, "include")
+    c = tmp_path / "use.c"
+    c.write_text('#include "b3hip.h"\nint main(void){ b3hConfig c; return b3h_config_default(&c); }\n')
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I", inc, str(c)], check=True)
+    cc = tmp_path / "use.cpp"
+    cc.write_text('#include "b3hip.hpp"\nint main(){ b3hConfig c; b3h_config_default(&c); '
+                  'b3GpuNarrowPhase np(c); return np.isValid() ? 0 : 1; }\n')
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", inc, str(cc)], check=True)
