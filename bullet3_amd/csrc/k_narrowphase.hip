@@ -1,22 +1,35 @@
 /*
- * k_narrowphase.hip -- convex-convex contact generation, fused into one kernel
- * per pair followed by an order-preserving compaction.
+ * k_narrowphase.hip -- convex-convex contact generation, one pair per lane
+ * group (L lanes of a wavefront; L = 64 by default), fused into one kernel,
+ * followed by an order-preserving compaction.
  *
  * Per pair (reference kernels, all in src/Bullet3OpenCL/NarrowphaseCollision/kernels/):
  *   1. MPR penetration + 1-point contact      mpr.cl:14-90 -> b3MprPenetration.h:816-886
  *   2. SAT over face normals of A then B      sat.cl:597-651, 1417-1509
  *   3. exact edge-edge SAT if |EA|*|EB| <= 162 sat.cl:719-798, 1512-1607
  *      else the 162-direction approximation   mpr.cl:180-311
- *   4. clip B's reference face against A's     satClipHullContacts.cl:235-382, 488-541
- *      incident-face side planes, keep depth <= 0.02, reduce to <= 4 points
- *      (extractManifoldSequential :720-810); point 0 stays the MPR point
+ *   4. clip B's incident face against the side planes of A's reference face,
+ *      keep depth <= 0.02, reduce to <= 4 points (satClipHullContacts.cl:235-382,
+ *      488-541, extractManifoldSequential :720-810); point 0 stays the MPR point
  *      when MPR found one (clipHullHullKernel :890-1000).
- * The reference runs these as 7 launches with a host sync after each; here one
- * launch writes a pair-indexed contact record and a flag, and a prefix sum over
- * the flags assigns contact slots in pair order (deterministic; the reference
- * uses an atomic counter).  The hull tables are small and instanced, so they
- * stay resident in L2/L1; the kernel is bound by the divergent per-pair
- * arithmetic rather than by HBM.
+ *
+ * Work split inside a group:
+ *   - both hulls' vertices are staged in LDS (per-group slice);
+ *   - MPR runs its portal arithmetic redundantly on every lane; each support
+ *     query is a lane-parallel dot product over the vertices plus a group
+ *     arg-max (first index wins, as the sequential b3MaxDot);
+ *   - every SAT family spreads its axes over the lanes; a lane projects both
+ *     hulls on its axis in vertex order (so lo/hi are the reference's values),
+ *     the family stops at the first round that finds a separating axis, and the
+ *     minimum overlap is an arg-min with first-index tie-break, which equals the
+ *     sequential "if (d < dmin)" scan;
+ *   - each Sutherland-Hodgman pass gives one polygon edge per lane; lanes emit
+ *     0, 1 or 2 points and a ballot prefix places them in sequential order.
+ * The reduction orders therefore reproduce the sequential kernels exactly.
+ * The reference runs these stages as 7 launches with a host sync after each;
+ * here one launch writes a pair-indexed contact record and a flag, and a prefix
+ * sum over the flags assigns contact slots in pair order (deterministic; the
+ * reference uses an atomic counter).
  */
 #include "b3h_internal.h"
 #include "unit_sphere_162.h"
@@ -25,16 +38,78 @@
 #define MPR_MAX_ITER 1000
 #define LARGE_FLOAT 1e18f
 #define CLIP_CAP 64
+#define NP_VCAP 64   /* hull vertices staged in LDS per hull; larger hulls are read from global memory */
+#define NP_THREADS 256
 
 __constant__ float c_unitSphere[162][3];
 
-struct NpCtx
+/* LDS ordering between lanes of one wavefront (groups never span waves) */
+__device__ __forceinline__ void wave_sync()
 {
-	const b3hRigidBodyData* bodies;
-	b3hShapeTables sh;
-	__device__ const b3hConvexPolyhedronData& hullOf(int body) const
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int L>
+struct Group
+{
+	int lane;                 /* 0..L-1 */
+	int base;                 /* first wavefront lane of the group */
+	unsigned long long mask;  /* the group's lanes */
+	unsigned long long below; /* the group's lanes below this one */
+	__device__ Group()
 	{
-		return sh.convexes[sh.collidables[bodies[body].m_collidableIdx].m_shapeIndex];
+		int wl = threadIdx.x & 63;
+		lane = wl & (L - 1);
+		base = wl & ~(L - 1);
+		mask = (L == 64) ? ~0ull : (((1ull << L) - 1ull) << base);
+		below = ((1ull << wl) - 1ull) & mask;
+	}
+	__device__ __forceinline__ bool any(bool p) const { return (__ballot(p) & mask) != 0ull; }
+	__device__ __forceinline__ int count(bool p) const { return __popcll(__ballot(p) & mask); }
+	__device__ __forceinline__ int count_below(bool p) const { return __popcll(__ballot(p) & below); }
+	__device__ __forceinline__ int src(int item) const { return base + (item & (L - 1)); }
+	/* (v, i) with the largest v, ties -> smallest i; losers carry (-inf, INT_MAX) */
+	__device__ __forceinline__ void argmax(float& v, int& i) const
+	{
+		for (int m = 1; m < L; m <<= 1)
+		{
+			float ov = __shfl_xor(v, m);
+			int oi = __shfl_xor(i, m);
+			if (ov > v || (ov == v && oi < i))
+			{
+				v = ov;
+				i = oi;
+			}
+		}
+	}
+	/* (v, i) with the smallest v, ties -> smallest i; losers carry (+inf, INT_MAX) */
+	__device__ __forceinline__ void argmin(float& v, int& i) const
+	{
+		for (int m = 1; m < L; m <<= 1)
+		{
+			float ov = __shfl_xor(v, m);
+			int oi = __shfl_xor(i, m);
+			if (ov < v || (ov == v && oi < i))
+			{
+				v = ov;
+				i = oi;
+			}
+		}
+	}
+	__device__ __forceinline__ int imax(int v) const
+	{
+		for (int m = 1; m < L; m <<= 1)
+		{
+			int o = __shfl_xor(v, m);
+			v = o > v ? o : v;
+		}
+		return v;
+	}
+	__device__ __forceinline__ F4 bcast(F4 v, int srcLane) const
+	{
+		return f4(__shfl(v.x, srcLane), __shfl(v.y, srcLane), __shfl(v.z, srcLane), __shfl(v.w, srcLane));
 	}
 };
 
@@ -71,38 +146,6 @@ __device__ __forceinline__ void vadd3(F4* v, F4 w)
 	v->z += w.z;
 }
 
-__device__ int max_dot(F4 dir, const F4* v, int n)
-{
-	float best = -LARGE_FLOAT;
-	int idx = -1;
-	for (int i = 0; i < n; i++)
-	{
-		float d = dot3(v[i], dir);
-		if (d > best)
-		{
-			best = d;
-			idx = i;
-		}
-	}
-	return idx < 0 ? 0 : idx;
-}
-__device__ F4 convex_support(const NpCtx& C, int body, F4 dirIn)
-{
-	const b3hRigidBodyData& b = C.bodies[body];
-	F4 q = b.m_quat;
-	F4 dir = f4(dirIn.x, dirIn.y, dirIn.z, 0.f);
-	F4 localDir = qrot(qinv(q), dir);
-	const b3hConvexPolyhedronData& h = C.hullOf(body);
-	const F4* vs = C.sh.vertices + h.m_vertexOffset;
-	F4 p = vs[max_dot(localDir, vs, h.m_numVertices)];
-	return xform(p, b.m_pos, q);
-}
-__device__ void mpr_support(const NpCtx& C, int a, int b, F4 dir, Sup* s)
-{
-	s->v1 = convex_support(C, a, dir);
-	s->v2 = convex_support(C, b, mul4(dir, -1.f));
-	s->v = sub4(s->v1, s->v2);
-}
 __device__ void portal_dir(const Simplex* p, F4* dir)
 {
 	F4 v2v1 = sub4(p->ps[2].v, p->ps[1].v);
@@ -142,12 +185,13 @@ __device__ void expand_portal(Simplex* p, const Sup* v4)
 			p->ps[1] = *v4;
 	}
 }
-__device__ int discover_portal(const NpCtx& C, int a, int b, Simplex* p)
+template <class SP>
+__device__ int discover_portal(SP& S, Simplex* p)
 {
 	F4 dir, va, vb;
 	float d;
-	p->ps[0].v1 = C.bodies[a].m_pos;
-	p->ps[0].v2 = C.bodies[b].m_pos;
+	p->ps[0].v1 = S.posA;
+	p->ps[0].v2 = S.posB;
 	p->ps[0].v = sub4(p->ps[0].v1, p->ps[0].v2);
 	p->last = 0;
 	F4 zero = f4(0, 0, 0, 0);
@@ -158,7 +202,7 @@ __device__ int discover_portal(const NpCtx& C, int a, int b, Simplex* p)
 	}
 	dir = mul4(p->ps[0].v, -1.f);
 	mpr_normalize(&dir);
-	mpr_support(C, a, b, dir, &p->ps[1]);
+	S.support(dir, &p->ps[1]);
 	p->last = 1;
 	d = dot3(p->ps[1].v, dir);
 	if (mpr_is_zero(d) || d < 0.f) return -1;
@@ -169,7 +213,7 @@ __device__ int discover_portal(const NpCtx& C, int a, int b, Simplex* p)
 		return 2;
 	}
 	mpr_normalize(&dir);
-	mpr_support(C, a, b, dir, &p->ps[2]);
+	S.support(dir, &p->ps[2]);
 	d = dot3(p->ps[2].v, dir);
 	if (mpr_is_zero(d) || d < 0.f) return -1;
 	p->last = 2;
@@ -187,7 +231,7 @@ __device__ int discover_portal(const NpCtx& C, int a, int b, Simplex* p)
 	}
 	while (p->last + 1 < 4)
 	{
-		mpr_support(C, a, b, dir, &p->ps[3]);
+		S.support(dir, &p->ps[3]);
 		d = dot3(p->ps[3].v, dir);
 		if (mpr_is_zero(d) || d < 0.f) return -1;
 		int cont = 0;
@@ -220,7 +264,8 @@ __device__ int discover_portal(const NpCtx& C, int a, int b, Simplex* p)
 	}
 	return 0;
 }
-__device__ int refine_portal(const NpCtx& C, int a, int b, Simplex* p)
+template <class SP>
+__device__ int refine_portal(SP& S, Simplex* p)
 {
 	F4 dir;
 	Sup v4;
@@ -229,7 +274,7 @@ __device__ int refine_portal(const NpCtx& C, int a, int b, Simplex* p)
 		portal_dir(p, &dir);
 		float d = dot3(dir, p->ps[1].v);
 		if (mpr_is_zero(d) || d > 0.f) return 0;
-		mpr_support(C, a, b, dir, &v4);
+		S.support(dir, &v4);
 		float dv4 = dot3(v4.v, dir);
 		int canEnc = mpr_is_zero(dv4) || dv4 > 0.f;
 		if (!canEnc || portal_reach_tolerance(p, &v4, dir)) return -1;
@@ -344,7 +389,8 @@ __device__ float point_tri_dist2(F4 P, F4 x0, F4 B, F4 Cc, F4* witness)
 	}
 	return dist;
 }
-__device__ void find_penetr(const NpCtx& C, int a, int b, Simplex* p, float* depth, F4* pdir, F4* pos)
+template <class SP>
+__device__ void find_penetr(SP& S, Simplex* p, float* depth, F4* pdir, F4* pos)
 {
 	F4 dir;
 	Sup v4;
@@ -353,7 +399,7 @@ __device__ void find_penetr(const NpCtx& C, int a, int b, Simplex* p, float* dep
 	for (int i = 0; i < MPR_MAX_ITER; i++)
 	{
 		portal_dir(p, &dir);
-		mpr_support(C, a, b, dir, &v4);
+		S.support(dir, &v4);
 		if (portal_reach_tolerance(p, &v4, dir) || iterations == MPR_MAX_ITER)
 		{
 			*depth = point_tri_dist2(zero, p->ps[1].v, p->ps[2].v, p->ps[3].v, pdir);
@@ -367,14 +413,15 @@ __device__ void find_penetr(const NpCtx& C, int a, int b, Simplex* p, float* dep
 		iterations++;
 	}
 }
-__device__ int mpr_penetration(const NpCtx& C, int a, int b, float* depth, F4* dir, F4* pos)
+template <class SP>
+__device__ int mpr_penetration(SP& S, float* depth, F4* dir, F4* pos)
 {
 	Simplex portal;
-	int res = discover_portal(C, a, b, &portal);
+	int res = discover_portal(S, &portal);
 	if (res == 0)
 	{
-		if (refine_portal(C, a, b, &portal) < 0) return -1;
-		find_penetr(C, a, b, &portal, depth, dir, pos);
+		if (refine_portal(S, &portal) < 0) return -1;
+		find_penetr(S, &portal, depth, dir, pos);
 		return 0;
 	}
 	if (res == 1)
@@ -399,14 +446,51 @@ __device__ int mpr_penetration(const NpCtx& C, int a, int b, float* depth, F4* d
 	return -1;
 }
 
+
+/* MPR support of one pair for a lane group: b3MprConvexSupport (b3MprPenetration.h:139-179)
+ * with b3MaxDot's first-maximum rule (shared/b3Float4.h:59-88) */
+template <int L>
+struct GroupSupport
+{
+	const Group<L>& g;
+	const F4* vA;
+	const F4* vB;
+	int nA, nB;
+	F4 posA, posB, qA, qB;
+	__device__ F4 one(const F4* vs, int nv, F4 pos, F4 q, F4 dirIn) const
+	{
+		F4 dir = f4(dirIn.x, dirIn.y, dirIn.z, 0.f);
+		F4 localDir = qrot(qinv(q), dir);
+		float bv = -INFINITY;
+		int bi = INT_MAX;
+		for (int i = g.lane; i < nv; i += L)
+		{
+			float d = dot3(vs[i], localDir);
+			if (d > -LARGE_FLOAT && (bi == INT_MAX || d > bv))
+			{
+				bv = d;
+				bi = i;
+			}
+		}
+		g.argmax(bv, bi);
+		if (bi == INT_MAX) bi = 0;
+		return xform(vs[bi], pos, q);
+	}
+	__device__ void support(F4 dir, Sup* s) const
+	{
+		s->v1 = one(vA, nA, posA, qA, dir);
+		s->v2 = one(vB, nB, posB, qB, mul4(dir, -1.f));
+		s->v = sub4(s->v1, s->v2);
+	}
+};
+
 /* ---------------- SAT ---------------- */
-__device__ void project(const NpCtx& C, const b3hConvexPolyhedronData& h, F4 pos, F4 orn, F4 dir, float* mn, float* mx)
+__device__ __forceinline__ void project(const F4* vs, int nv, F4 pos, F4 orn, F4 dir, float* mn, float* mx)
 {
 	float lo = FLT_MAX, hi = -FLT_MAX;
 	F4 localDir = qinvrot(orn, dir);
 	float offset = dot4(pos, dir);
-	const F4* vs = C.sh.vertices + h.m_vertexOffset;
-	for (int i = 0; i < h.m_numVertices; i++)
+	for (int i = 0; i < nv; i++)
 	{
 		float dp = dot4(vs[i], localDir);
 		if (dp < lo) lo = dp;
@@ -421,76 +505,109 @@ __device__ void project(const NpCtx& C, const b3hConvexPolyhedronData& h, F4 pos
 	*mn = lo + offset;
 	*mx = hi + offset;
 }
-__device__ bool test_sep_axis(const NpCtx& C, const b3hConvexPolyhedronData& ha, const b3hConvexPolyhedronData& hb, F4 posA,
-							  F4 ornA, F4 posB, F4 ornB, F4 axis, float* depth)
+struct PairGeom
+{
+	const F4* vA;
+	const F4* vB;
+	int nA, nB;
+	F4 posA, ornA, posB, ornB;
+};
+/* TestSepAxis: false if the axis separates, else the overlap */
+__device__ __forceinline__ bool test_sep_axis(const PairGeom& P, F4 axis, float* depth)
 {
 	float mn0, mx0, mn1, mx1;
-	project(C, ha, posA, ornA, axis, &mn0, &mx0);
-	project(C, hb, posB, ornB, axis, &mn1, &mx1);
+	project(P.vA, P.nA, P.posA, P.ornA, axis, &mn0, &mx0);
+	project(P.vB, P.nB, P.posB, P.ornB, axis, &mn1, &mx1);
 	if (mx0 < mn1 || mx1 < mn0) return false;
 	float d0 = mx0 - mn1, d1 = mx1 - mn0;
 	*depth = d0 < d1 ? d0 : d1;
 	return true;
 }
-__device__ bool sat_faces(const NpCtx& C, const b3hConvexPolyhedronData& ha, const b3hConvexPolyhedronData& hb, F4 posA,
-						  F4 ornA, F4 posB, F4 ornB, F4 dc, F4* sep, float* dmin)
+__device__ __forceinline__ bool almost_zero(F4 v) { return !(fabsf(v.x) > 1e-6f || fabsf(v.y) > 1e-6f || fabsf(v.z) > 1e-6f); }
+
+/* Candidate axis `t` of a family; returns false when the family has no axis t
+ * (skipped edge pairs). kind 0: faces of A (tested with A as "A"), kind 1:
+ * faces of B (tested with B as "A", sat.cl:1463-1489), kind 2: edge crosses,
+ * kind 3: the 162 fixed directions. */
+struct Family
 {
-	for (int i = 0; i < ha.m_numFaces; i++)
+	int kind;
+	int count;
+	const b3hGpuFace* faces; /* kind 0/1 */
+	F4 orn;                  /* kind 0/1 */
+	const F4* edgesA;        /* kind 2 */
+	const F4* edgesB;
+	int nEB;
+};
+__device__ __forceinline__ bool family_axis(const Family& F, const PairGeom& P, F4 dc, int t, F4* axis)
+{
+	if (F.kind <= 1)
 	{
-		F4 n = C.sh.faces[ha.m_faceOffset + i].m_plane;
-		F4 nw = qrot(ornA, n);
+		F4 nw = qrot(F.orn, F.faces[t].m_plane);
 		if (dot3(dc, nw) < 0) nw = mul4(nw, -1.f);
-		float d;
-		if (!test_sep_axis(C, ha, hb, posA, ornA, posB, ornB, nw, &d)) return false;
-		if (d < *dmin)
-		{
-			*dmin = d;
-			*sep = nw;
-		}
+		*axis = nw;
+		return true;
 	}
-	if (dot3(neg4(dc), *sep) > 0.0f) *sep = neg4(*sep);
+	if (F.kind == 2)
+	{
+		int e0 = t / F.nEB, e1 = t - e0 * F.nEB;
+		F4 ew0 = qrot(P.ornA, F.edgesA[e0]);
+		F4 ew1 = qrot(P.ornB, F.edgesB[e1]);
+		F4 c = cross3(ew0, ew1);
+		if (almost_zero(c)) return false;
+		c = nrm3(c);
+		if (dot3(dc, c) < 0) c = mul4(c, -1.f);
+		*axis = c;
+		return true;
+	}
+	F4 c = f4(c_unitSphere[t][0], c_unitSphere[t][1], c_unitSphere[t][2], 0.f);
+	if (dot3(dc, c) > 0) c = mul4(c, -1.f);
+	*axis = c;
 	return true;
 }
-__device__ __forceinline__ bool almost_zero(F4 v) { return !(fabsf(v.x) > 1e-6f || fabsf(v.y) > 1e-6f || fabsf(v.z) > 1e-6f); }
-__device__ bool sat_edges(const NpCtx& C, const b3hConvexPolyhedronData& ha, const b3hConvexPolyhedronData& hb, F4 posA,
-						  F4 ornA, F4 posB, F4 ornB, F4 dc, F4* sep, float* dmin)
+/* One SAT family over the group: false if some axis separates; otherwise the
+ * sequential min-overlap update of (sep, dmin) and the final orientation flip. */
+template <int L>
+__device__ bool sat_family(const Group<L>& g, const Family& F, const PairGeom& P, F4 dc, F4* sep, float* dmin)
 {
-	for (int e0 = 0; e0 < ha.m_numUniqueEdges; e0++)
+	float bv = INFINITY;
+	int bi = INT_MAX;
+	F4 bn = f4(0, 0, 0, 0);
+	for (int t0 = 0; t0 < F.count; t0 += L)
 	{
-		F4 ew0 = qrot(ornA, C.sh.uniqueEdges[ha.m_uniqueEdgesOffset + e0]);
-		for (int e1 = 0; e1 < hb.m_numUniqueEdges; e1++)
+		int t = t0 + g.lane;
+		bool separated = false;
+		F4 axis;
+		if (t < F.count && family_axis(F, P, dc, t, &axis))
 		{
-			F4 ew1 = qrot(ornB, C.sh.uniqueEdges[hb.m_uniqueEdgesOffset + e1]);
-			F4 c = cross3(ew0, ew1);
-			if (almost_zero(c)) continue;
-			c = nrm3(c);
-			if (dot3(dc, c) < 0) c = mul4(c, -1.f);
 			float d;
-			if (!test_sep_axis(C, ha, hb, posA, ornA, posB, ornB, c, &d)) return false;
-			if (d < *dmin)
+			bool overlap;
+			if (F.kind == 1)
 			{
-				*dmin = d;
-				*sep = c;
+				PairGeom Q = {P.vB, P.vA, P.nB, P.nA, P.posB, P.ornB, P.posA, P.ornA};
+				overlap = test_sep_axis(Q, axis, &d);
+			}
+			else
+				overlap = test_sep_axis(P, axis, &d);
+			if (!overlap)
+				separated = true;
+			else if (d < bv)
+			{
+				bv = d;
+				bi = t;
+				bn = axis;
 			}
 		}
+		if (g.any(separated)) return false;
 	}
-	if (dot3(neg4(dc), *sep) > 0.0f) *sep = neg4(*sep);
-	return true;
-}
-__device__ bool sat_sphere(const NpCtx& C, const b3hConvexPolyhedronData& ha, const b3hConvexPolyhedronData& hb, F4 posA,
-						   F4 ornA, F4 posB, F4 ornB, F4 dc, F4* sep, float* dmin)
-{
-	for (int i = 0; i < B3_UNIT_SPHERE_162_COUNT; i++)
+	int own = bi;
+	g.argmin(bv, bi);
+	F4 n = g.bcast(bn, g.src(bi == INT_MAX ? 0 : bi));
+	(void)own;
+	if (bi != INT_MAX && bv < *dmin)
 	{
-		F4 c = f4(c_unitSphere[i][0], c_unitSphere[i][1], c_unitSphere[i][2], 0.f);
-		if (dot3(dc, c) > 0) c = mul4(c, -1.f);
-		float d;
-		if (!test_sep_axis(C, ha, hb, posA, ornA, posB, ornB, c, &d)) return false;
-		if (d < *dmin)
-		{
-			*dmin = d;
-			*sep = c;
-		}
+		*dmin = bv;
+		*sep = n;
 	}
 	if (dot3(neg4(dc), *sep) > 0.0f) *sep = neg4(*sep);
 	return true;
@@ -501,116 +618,134 @@ __device__ __forceinline__ F4 lerp3(F4 a, F4 b, float t)
 {
 	return f4(a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t, a.z + (b.z - a.z) * t, 0.f);
 }
-__device__ int clip_face(const F4* in, int nIn, F4 n, float eq, F4* out)
+/* one Sutherland-Hodgman pass (clipFace, satClipHullContacts.cl:235-290), one
+ * polygon edge per lane, outputs placed in the sequential order */
+template <int L>
+__device__ int clip_pass(const Group<L>& g, const F4* in, int nIn, F4 n, float eq, F4* out)
 {
-	int nOut = 0;
 	if (nIn < 2) return 0;
-	F4 first = in[nIn - 1];
-	float ds = dot3(n, first) + eq;
-	for (int ve = 0; ve < nIn; ve++)
+	int total = 0;
+	for (int c0 = 0; c0 < nIn; c0 += L)
 	{
-		F4 end = in[ve];
-		float de = dot3(n, end) + eq;
-		if (ds < 0)
+		int j = c0 + g.lane;
+		int cnt = 0;
+		F4 o0 = f4(0, 0, 0, 0), o1 = f4(0, 0, 0, 0);
+		if (j < nIn)
 		{
-			if (de < 0)
+			F4 first = in[j == 0 ? nIn - 1 : j - 1];
+			F4 end = in[j];
+			float ds = dot3(n, first) + eq;
+			float de = dot3(n, end) + eq;
+			if (ds < 0)
 			{
-				if (nOut < CLIP_CAP) out[nOut++] = end;
+				cnt = 1;
+				o0 = (de < 0) ? end : lerp3(first, end, ds * 1.f / (ds - de));
 			}
-			else
+			else if (de < 0)
 			{
-				if (nOut < CLIP_CAP) out[nOut++] = lerp3(first, end, ds * 1.f / (ds - de));
+				cnt = 2;
+				o0 = lerp3(first, end, ds * 1.f / (ds - de));
+				o1 = end;
 			}
 		}
-		else
-		{
-			if (de < 0)
-			{
-				if (nOut < CLIP_CAP) out[nOut++] = lerp3(first, end, ds * 1.f / (ds - de));
-				if (nOut < CLIP_CAP) out[nOut++] = end;
-			}
-		}
-		first = end;
-		ds = de;
+		int pre = total + g.count_below(cnt >= 1) + g.count_below(cnt == 2);
+		if (cnt >= 1 && pre < CLIP_CAP) out[pre] = o0;
+		if (cnt == 2 && pre + 1 < CLIP_CAP) out[pre + 1] = o1;
+		total += g.count(cnt >= 1) + g.count(cnt == 2);
 	}
-	return nOut;
+	wave_sync();
+	return total < CLIP_CAP ? total : CLIP_CAP;
 }
-__device__ int clip_hull_hull(const NpCtx& C, F4 sep, const b3hConvexPolyhedronData& ha, const b3hConvexPolyhedronData& hb,
-							  F4 posA, F4 ornA, F4 posB, F4 ornB, F4* bufA, F4* bufB, F4* out)
+/* clipHullAgainstHull (satClipHullContacts.cl:488-541): returns the kept points
+ * (xyz + depth) in `out` */
+template <int L>
+__device__ int clip_hull_hull(const Group<L>& g, F4 sep, const PairGeom& P, const b3hConvexPolyhedronData& ha,
+							  const b3hConvexPolyhedronData& hb, const b3hShapeTables& sh, F4* bufA, F4* bufB, F4* out)
 {
 	const float minDist = -1e30f, maxDist = 0.02f;
-	int closestB = -1;
-	float dmax = -FLT_MAX;
-	for (int f = 0; f < hb.m_numFaces; f++)
+	/* incident face of B: largest dot with the separating normal */
+	float bv = -INFINITY;
+	int bi = INT_MAX;
+	for (int f = g.lane; f < hb.m_numFaces; f += L)
 	{
-		F4 wn = qrot(ornB, xyz0(C.sh.faces[hb.m_faceOffset + f].m_plane));
+		F4 wn = qrot(P.ornB, xyz0(sh.faces[hb.m_faceOffset + f].m_plane));
 		float d = dot3(wn, sep);
-		if (d > dmax)
+		if (d > -FLT_MAX && (bi == INT_MAX || d > bv))
 		{
-			dmax = d;
-			closestB = f;
+			bv = d;
+			bi = f;
 		}
 	}
-	if (closestB < 0) return 0;
-	int nB = 0;
+	g.argmax(bv, bi);
+	if (bi == INT_MAX) return 0;
+	const b3hGpuFace polyB = sh.faces[hb.m_faceOffset + bi];
+	int nB = polyB.m_numIndices < CLIP_CAP ? polyB.m_numIndices : CLIP_CAP;
+	for (int e = g.lane; e < nB; e += L)
+		bufA[e] = xform(P.vB[sh.indices[polyB.m_indexOffset + e]], P.posB, P.ornB);
+	wave_sync();
+	/* reference face of A: smallest dot with the separating normal */
+	float av = INFINITY;
+	int ai = INT_MAX;
+	for (int f = g.lane; f < ha.m_numFaces; f += L)
 	{
-		const b3hGpuFace& poly = C.sh.faces[hb.m_faceOffset + closestB];
-		for (int e = 0; e < poly.m_numIndices && nB < CLIP_CAP; e++)
+		F4 wn = qrot(P.ornA, xyz0(sh.faces[ha.m_faceOffset + f].m_plane));
+		float d = dot3(wn, sep);
+		if (d < FLT_MAX && (ai == INT_MAX || d < av))
 		{
-			F4 v = C.sh.vertices[hb.m_vertexOffset + C.sh.indices[poly.m_indexOffset + e]];
-			bufA[nB++] = xform(v, posB, ornB);
+			av = d;
+			ai = f;
 		}
 	}
+	g.argmin(av, ai);
+	if (ai == INT_MAX) return 0;
+	const b3hGpuFace polyA = sh.faces[ha.m_faceOffset + ai];
+	int nvA = polyA.m_numIndices;
+	const int* ia = sh.indices + polyA.m_indexOffset;
+	F4 wpn = qrot(P.ornA, xyz0(polyA.m_plane));
 	F4* pin = bufA;
 	F4* pout = bufB;
 	int nIn = nB;
-	int closestA = -1;
-	float dmin = FLT_MAX;
-	for (int f = 0; f < ha.m_numFaces; f++)
-	{
-		F4 wn = qrot(ornA, xyz0(C.sh.faces[ha.m_faceOffset + f].m_plane));
-		float d = dot3(wn, sep);
-		if (d < dmin)
-		{
-			dmin = d;
-			closestA = f;
-		}
-	}
-	if (closestA < 0) return 0;
-	const b3hGpuFace polyA = C.sh.faces[ha.m_faceOffset + closestA];
-	int nvA = polyA.m_numIndices;
-	const F4* va = C.sh.vertices + ha.m_vertexOffset;
-	const int* ia = C.sh.indices + polyA.m_indexOffset;
-	F4 wpn = qrot(ornA, xyz0(polyA.m_plane));
 	for (int e0 = 0; e0 < nvA; e0++)
 	{
-		F4 a = va[ia[e0]];
-		F4 b = va[ia[(e0 + 1) % nvA]];
-		F4 we0 = qrot(ornA, sub4(a, b));
+		F4 a = P.vA[ia[e0]];
+		F4 b = P.vA[ia[(e0 + 1) % nvA]];
+		F4 we0 = qrot(P.ornA, sub4(a, b));
 		F4 pn = neg4(cross3(we0, wpn));
-		F4 wa = xform(a, posA, ornA);
+		F4 wa = xform(a, P.posA, P.ornA);
 		float peq = -dot3(wa, pn);
-		int nOut = clip_face(pin, nIn, pn, peq, pout);
+		int nOut = clip_pass(g, pin, nIn, pn, peq, pout);
 		F4* t = pout;
 		pout = pin;
 		pin = t;
 		nIn = nOut;
 	}
+	float peq = polyA.m_plane.w - dot3(wpn, P.posA);
 	int nc = 0;
-	float peq = polyA.m_plane.w - dot3(wpn, posA);
-	for (int i = 0; i < nIn; i++)
+	for (int c0 = 0; c0 < nIn; c0 += L)
 	{
-		float depth = dot3(wpn, pin[i]) + peq;
-		if (depth <= minDist) depth = minDist;
-		if (depth <= maxDist)
+		int i = c0 + g.lane;
+		bool keep = false;
+		F4 p = f4(0, 0, 0, 0);
+		if (i < nIn)
 		{
-			F4 p = pin[i];
-			out[nc++] = f4(p.x, p.y, p.z, depth);
+			p = pin[i];
+			float depth = dot3(wpn, p) + peq;
+			if (depth <= minDist) depth = minDist;
+			keep = depth <= maxDist;
+			p.w = depth;
 		}
+		int pre = nc + g.count_below(keep);
+		if (keep) out[pre] = p;
+		nc += g.count(keep);
 	}
+	wave_sync();
 	return nc;
 }
-__device__ int extract_manifold(const F4* p, int n, F4 nearNormal, int* idx)
+/* extractManifoldSequential (satClipHullContacts.cl:720-810); the centre is the
+ * sequential sum (every lane runs it), the four extreme points come from an
+ * integer max whose low byte is the point index (order-free) */
+template <int L>
+__device__ int extract_manifold(const Group<L>& g, const F4* p, int n, F4 nearNormal, int* idx)
 {
 	if (n == 0) return 0;
 	n = n < 64 ? n : 64;
@@ -628,8 +763,8 @@ __device__ int extract_manifold(const F4* p, int n, F4 nearNormal, int* idx)
 	F4 v = cross3(nearNormal, u);
 	u = nrm3(u);
 	v = nrm3(v);
-	int best0 = 0, best1 = 0, best2 = 0, best3 = 0;
-	for (int ie = 0; ie < n; ie++)
+	int b0 = INT_MIN, b1 = INT_MIN, b2 = INT_MIN, b3 = INT_MIN;
+	for (int ie = g.lane; ie < n; ie += L)
 	{
 		F4 r = sub4(p[ie], center);
 		unsigned lo = 0xffu & (unsigned)ie;
@@ -637,34 +772,39 @@ __device__ int extract_manifold(const F4* p, int n, F4 nearNormal, int* idx)
 		int a1 = (int)((f2u(dot3(neg4(u), r)) & 0xffffff00u) | lo);
 		int a2 = (int)((f2u(dot3(v, r)) & 0xffffff00u) | lo);
 		int a3 = (int)((f2u(dot3(neg4(v), r)) & 0xffffff00u) | lo);
-		if (ie == 0 || a0 > best0) best0 = a0;
-		if (ie == 0 || a1 > best1) best1 = a1;
-		if (ie == 0 || a2 > best2) best2 = a2;
-		if (ie == 0 || a3 > best3) best3 = a3;
+		b0 = a0 > b0 ? a0 : b0;
+		b1 = a1 > b1 ? a1 : b1;
+		b2 = a2 > b2 ? a2 : b2;
+		b3 = a3 > b3 ? a3 : b3;
 	}
-	idx[0] = best0 & 0xff;
-	idx[1] = best1 & 0xff;
-	idx[2] = best2 & 0xff;
-	idx[3] = best3 & 0xff;
+	idx[0] = g.imax(b0) & 0xff;
+	idx[1] = g.imax(b1) & 0xff;
+	idx[2] = g.imax(b2) & 0xff;
+	idx[3] = g.imax(b3) & 0xff;
 	return 4;
 }
 
-/* ---------------- the fused per-pair kernel ---------------- */
-__global__ __launch_bounds__(64) void k_narrowphase(const b3hInt4* __restrict__ pairs, const int* __restrict__ numPairsPtr,
-													const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh,
-													b3hContact4Data* __restrict__ staging, int* __restrict__ flags,
-													int* __restrict__ sepStateOut, b3hFloat4* __restrict__ sepOut)
+/* ---------------- the fused kernel ---------------- */
+template <int L>
+__global__ __launch_bounds__(NP_THREADS) void k_narrowphase(const b3hInt4* __restrict__ pairs,
+															 const int* __restrict__ numPairsPtr,
+															 const b3hRigidBodyData* __restrict__ bodies,
+															 b3hShapeTables sh, b3hContact4Data* __restrict__ staging,
+															 int* __restrict__ flags, int* __restrict__ sepStateOut,
+															 b3hFloat4* __restrict__ sepOut)
 {
+	constexpr int G = NP_THREADS / L;
+	__shared__ F4 sVerts[G][2][NP_VCAP];
+	__shared__ F4 sClip[G][3][CLIP_CAP];
+	const Group<L> g;
+	const int gi = threadIdx.x / L;
 	int numPairs = *numPairsPtr;
-	NpCtx C;
-	C.bodies = bodies;
-	C.sh = sh;
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < numPairs; i += gridDim.x * blockDim.x)
+	for (int i = blockIdx.x * G + gi; i < numPairs; i += gridDim.x * G)
 	{
 		b3hInt4 pr = pairs[i];
 		int a = pr.x, b = pr.y;
-		const b3hRigidBodyData& A = bodies[a];
-		const b3hRigidBodyData& B = bodies[b];
+		const b3hRigidBodyData A = bodies[a];
+		const b3hRigidBodyData B = bodies[b];
 		bool hull = !(A.m_invMass == 0 && B.m_invMass == 0) &&
 					sh.collidables[A.m_collidableIdx].m_shapeType == B3H_SHAPE_CONVEX_HULL &&
 					sh.collidables[B.m_collidableIdx].m_shapeType == B3H_SHAPE_CONVEX_HULL;
@@ -673,27 +813,51 @@ __global__ __launch_bounds__(64) void k_narrowphase(const b3hInt4* __restrict__ 
 		bool mprHit = false;
 		float mprDepth = 0.f;
 		F4 mprDir = f4(0, 0, 0, 0), mprPos = f4(0, 0, 0, 0);
-		if (hull) mprHit = mpr_penetration(C, a, b, &mprDepth, &mprDir, &mprPos) == 0;
-		if (mprHit) sep = neg4(mprDir);
-		F4 posA = A.m_pos, posB = B.m_pos;
-		F4 ornA = A.m_quat, ornB = B.m_quat;
+		int nReduced = 0;
+		int idx[4] = {0, 0, 0, 0};
+		F4* pts = sClip[gi][2];
 		if (hull)
 		{
-			const b3hConvexPolyhedronData& ha = C.hullOf(a);
-			const b3hConvexPolyhedronData& hb = C.hullOf(b);
-			F4 pA = posA, pB = posB;
-			pA.w = 0.f;
-			pB.w = 0.f;
-			F4 c0 = xform(ha.m_localCenter, pA, ornA);
-			F4 c1 = xform(hb.m_localCenter, pB, ornB);
+			const b3hConvexPolyhedronData ha = sh.convexes[sh.collidables[A.m_collidableIdx].m_shapeIndex];
+			const b3hConvexPolyhedronData hb = sh.convexes[sh.collidables[B.m_collidableIdx].m_shapeIndex];
+			const F4* gA = sh.vertices + ha.m_vertexOffset;
+			const F4* gB = sh.vertices + hb.m_vertexOffset;
+			bool ldsA = ha.m_numVertices <= NP_VCAP, ldsB = hb.m_numVertices <= NP_VCAP;
+			wave_sync(); /* the previous pair's LDS reads are done */
+			if (ldsA)
+				for (int k = g.lane; k < ha.m_numVertices; k += L) sVerts[gi][0][k] = gA[k];
+			if (ldsB)
+				for (int k = g.lane; k < hb.m_numVertices; k += L) sVerts[gi][1][k] = gB[k];
+			wave_sync();
+			PairGeom P;
+			P.vA = ldsA ? sVerts[gi][0] : gA;
+			P.vB = ldsB ? sVerts[gi][1] : gB;
+			P.nA = ha.m_numVertices;
+			P.nB = hb.m_numVertices;
+			P.posA = A.m_pos;
+			P.posB = B.m_pos;
+			P.ornA = A.m_quat;
+			P.ornB = B.m_quat;
+			/* MPR */
+			GroupSupport<L> S = {g, P.vA, P.vB, P.nA, P.nB, A.m_pos, B.m_pos, A.m_quat, B.m_quat};
+			mprHit = mpr_penetration(S, &mprDepth, &mprDir, &mprPos) == 0;
+			if (mprHit) sep = neg4(mprDir);
+			/* SAT (findSeparatingAxis* kernels) */
+			PairGeom Q = P;
+			Q.posA.w = 0.f;
+			Q.posB.w = 0.f;
+			F4 c0 = xform(ha.m_localCenter, Q.posA, Q.ornA);
+			F4 c1 = xform(hb.m_localCenter, Q.posB, Q.ornB);
 			F4 dc = sub4(c0, c1);
 			float dmin = FLT_MAX;
 			F4 sepN = sep;
-			bool sepA = sat_faces(C, ha, hb, pA, ornA, pB, ornB, dc, &sepN, &dmin);
+			Family fa = {0, ha.m_numFaces, sh.faces + ha.m_faceOffset, Q.ornA, nullptr, nullptr, 1};
+			bool sepA = sat_family(g, fa, Q, dc, &sepN, &dmin);
 			float dminSaved = FLT_MAX;
 			if (sepA)
 			{
-				bool sepB = sat_faces(C, hb, ha, pB, ornB, pA, ornA, dc, &sepN, &dmin);
+				Family fb = {1, hb.m_numFaces, sh.faces + hb.m_faceOffset, Q.ornB, nullptr, nullptr, 1};
+				bool sepB = sat_family(g, fb, Q, dc, &sepN, &dmin);
 				if (sepB)
 				{
 					state = 1;
@@ -707,12 +871,14 @@ __global__ __launch_bounds__(64) void k_narrowphase(const b3hInt4* __restrict__ 
 			{
 				float dm = dminSaved;
 				F4 s2 = sep;
-				bool ok;
-				if (ha.m_numUniqueEdges * hb.m_numUniqueEdges <= B3_UNIT_SPHERE_162_COUNT)
-					ok = sat_edges(C, ha, hb, pA, ornA, pB, ornB, dc, &s2, &dm);
+				int nEE = ha.m_numUniqueEdges * hb.m_numUniqueEdges;
+				Family fe;
+				if (nEE <= B3_UNIT_SPHERE_162_COUNT)
+					fe = {2, nEE, nullptr, Q.ornA, sh.uniqueEdges + ha.m_uniqueEdgesOffset,
+						  sh.uniqueEdges + hb.m_uniqueEdgesOffset, hb.m_numUniqueEdges};
 				else
-					ok = sat_sphere(C, ha, hb, pA, ornA, pB, ornB, dc, &s2, &dm);
-				if (!ok)
+					fe = {3, B3_UNIT_SPHERE_162_COUNT, nullptr, Q.ornA, nullptr, nullptr, 1};
+				if (!sat_family(g, fe, Q, dc, &s2, &dm))
 					state = 0;
 				else
 				{
@@ -720,49 +886,50 @@ __global__ __launch_bounds__(64) void k_narrowphase(const b3hInt4* __restrict__ 
 					sep = s2;
 				}
 			}
-		}
-		if (sepStateOut) sepStateOut[i] = state;
-		if (sepOut) sepOut[i] = sep;
-		int nReduced = 0;
-		int idx[4] = {-1, -1, -1, -1};
-		F4 pts[CLIP_CAP];
-		if (hull && state == 1)
-		{
-			F4 bufA[CLIP_CAP], bufB[CLIP_CAP];
-			int np = clip_hull_hull(C, sep, C.hullOf(a), C.hullOf(b), posA, ornA, posB, ornB, bufA, bufB, pts);
-			if (np > 0) nReduced = extract_manifold(pts, np, neg4(sep), idx);
+			if (state == 1)
+			{
+				int np = clip_hull_hull(g, sep, P, ha, hb, sh, sClip[gi][0], sClip[gi][1], pts);
+				if (np > 0) nReduced = extract_manifold(g, pts, np, neg4(sep), idx);
+			}
 		}
 		int flag = (mprHit || nReduced > 0) ? 1 : 0;
-		flags[i] = flag;
-		if (!flag) continue;
-		b3hContact4Data c;
-		for (int k = 0; k < 4; k++) c.m_worldPosB[k] = f4(0, 0, 0, 0);
-		c.m_restituitionCoeffCmp = (unsigned short)(0.f * 0xffff);
-		c.m_frictionCoeffCmp = (unsigned short)(0.7f * 0xffff);
-		c.m_batchIdx = i;
-		c.m_bodyAPtrAndSignBit = A.m_invMass == 0 ? -a : a;
-		c.m_bodyBPtrAndSignBit = B.m_invMass == 0 ? -b : b;
-		c.m_childIndexA = -1;
-		c.m_childIndexB = -1;
-		c.m_unused1 = 0;
-		c.m_unused2 = 0;
-		c.m_worldNormalOnB = f4(0, 0, 0, 0);
-		if (mprHit)
+		if (g.lane == 0)
 		{
-			c.m_worldNormalOnB = neg4(mprDir);
-			F4 p = mprPos;
-			p.w = -mprDepth;
-			c.m_worldPosB[0] = p;
-			c.m_worldNormalOnB.w = 1.f;
+			if (sepStateOut) sepStateOut[i] = state;
+			if (sepOut) sepOut[i] = sep;
+			flags[i] = flag;
+			if (flag)
+			{
+				b3hContact4Data c;
+				for (int k = 0; k < 4; k++) c.m_worldPosB[k] = f4(0, 0, 0, 0);
+				c.m_restituitionCoeffCmp = (unsigned short)(0.f * 0xffff);
+				c.m_frictionCoeffCmp = (unsigned short)(0.7f * 0xffff);
+				c.m_batchIdx = i;
+				c.m_bodyAPtrAndSignBit = A.m_invMass == 0 ? -a : a;
+				c.m_bodyBPtrAndSignBit = B.m_invMass == 0 ? -b : b;
+				c.m_childIndexA = -1;
+				c.m_childIndexB = -1;
+				c.m_unused1 = 0;
+				c.m_unused2 = 0;
+				c.m_worldNormalOnB = f4(0, 0, 0, 0);
+				if (mprHit)
+				{
+					c.m_worldNormalOnB = neg4(mprDir);
+					F4 p = mprPos;
+					p.w = -mprDepth;
+					c.m_worldPosB[0] = p;
+					c.m_worldNormalOnB.w = 1.f;
+				}
+				if (nReduced > 0)
+				{
+					c.m_worldNormalOnB = sep;
+					for (int k = 0; k < nReduced; k++)
+						if (k > 0 || !mprHit) c.m_worldPosB[k] = pts[idx[k]];
+					c.m_worldNormalOnB.w = (float)nReduced;
+				}
+				staging[i] = c;
+			}
 		}
-		if (nReduced > 0)
-		{
-			c.m_worldNormalOnB = sep;
-			for (int k = 0; k < nReduced; k++)
-				if (k > 0 || !mprHit) c.m_worldPosB[k] = pts[idx[k]];
-			c.m_worldNormalOnB.w = (float)nReduced;
-		}
-		staging[i] = c;
 	}
 }
 
@@ -808,11 +975,12 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 					 int pairsPerGroup, hipStream_t s)
 {
 	(void)pairsPerGroup;
-	int blocks = (maxPairs + 63) / 64;
-	if (blocks > 65536) blocks = 65536;
+	const int groups = NP_THREADS / 64;
+	int blocks = (maxPairs + groups - 1) / groups;
+	if (blocks > 16384) blocks = 16384;
 	if (blocks < 1) blocks = 1;
-	hipLaunchKernelGGL(k_narrowphase, dim3(blocks), dim3(64), 0, s, pairs, numPairs, bodies, sh, staging, flags, sepState,
-					   sepNormal);
+	hipLaunchKernelGGL(k_narrowphase<64>, dim3(blocks), dim3(NP_THREADS), 0, s, pairs, numPairs, bodies, sh, staging,
+					   flags, sepState, sepNormal);
 	b3h_exclusive_scan(flags, flagScan, numPairs, maxPairs, scanTmp, s);
 	int cblocks = (maxPairs + 255) / 256;
 	if (cblocks > 16384) cblocks = 16384;
