@@ -7,11 +7,15 @@ the lattice collapse into a pile (and capture the step into a HIP graph); then
 exactly --steps steps are timed.  A "step" is one full stepSimulation: world
 AABBs, broadphase, narrowphase, solver set-up, 2x10 PGS sweeps, integration.
 
-Prints one JSON line (metric body-steps/s).  roofline: the stage with the
-largest share of the step, its algorithmic HBM bytes (records read once and
-written once, per the byte counts below) over its duration measured with HIP
-events on the pipeline's stream.  cpu_baseline: the scalar oracle (same
-algorithm, one core) stepped from the GPU's post-warmup state for ~10 s.
+Prints one JSON line (metric body-steps/s).  roofline: the kernel with the
+largest share of the step (the fused narrowphase kernel, or the normal or the
+friction PGS sweep launches), its algorithmic HBM bytes per launch (records
+read once and written once, kernel_bytes below) over its average launch
+duration, measured with HIP events on the pipeline's stream during eager steps
+taken right after the timed region.  traffic: measured HBM bytes per launch of
+that kernel from profiles/traffic.json (rocprof FETCH_SIZE and WRITE_SIZE runs),
+when it was measured for the same kernel.  cpu_baseline: the scalar oracle
+(same algorithm, one core) stepped from the GPU's post-warmup state for ~10 s.
 """
 import argparse
 import json
@@ -26,19 +30,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-STAGES = ["aabb", "broadphase", "narrowphase", "solver", "pgs+setup", "integrate"]
 
 
-def stage_bytes(n, p, c, k):
-    """algorithmic bytes per stage (record sizes: body 80, AABB 32, pair 16,
-    contact 112, constraint 176, inertia 48 per matrix)"""
+def kernel_bytes(n, p, c):
+    """algorithmic HBM bytes per launch of the kernels that can dominate a step
+    (records read once, written once: body 80, pair 16, contact 112,
+    constraint 176 with 16 of impulses written, velocities 32 read + 32 written,
+    world inverse inertia 48, position 16)"""
+    sweep = 192 * c + 128 * n
     return {
-        "aabb": 112 * n,                       # body 80 + local AABB 32 read (write 32 counted in broadphase input)
-        "broadphase": 32 * n + 16 * p,         # AABBs read, pairs written
-        "narrowphase": 80 * p + 112 * c,       # pair + 2 body poses read, contacts written
-        "pgs+setup": 556 * c + 2 * k * (192 * c + 128 * n),
-        "integrate": 160 * n,
+        "narrowphase_kernel": 80 * p + 112 * c,   # pair + 2 poses read, contact written
+        "pgs_normal": sweep / 8.0,                # one of the 8 cell-parity launches of a sweep
+        "pgs_friction": sweep / 8.0,
     }
+
+
+def step_bytes(n, p, c, k):
+    return 304 * n + 96 * p + 668 * c + 2 * k * (192 * c + 128 * n)
 
 
 def run(args):
@@ -73,21 +81,31 @@ def run(args):
     sec = t1 - t0
     value = n_dyn * args.steps / sec
     ms_per_step = 1000.0 * sec / args.steps
-    # per-stage breakdown from HIP events (eager steps after the timed region)
+    # per-stage breakdown from HIP events recorded on the pipeline's stream
+    # (eager steps right after the timed region, same state)
+    from bullet3_amd.capi import STAGE_NAMES
     w.enable_timing(True)
     reps = 5
-    acc = np.zeros(6)
+    acc = np.zeros(len(STAGE_NAMES))
     for _ in range(reps):
         w.step(dt)
         acc += w.stage_times()
-    st = acc / reps
-    times = {"aabb": st[0], "broadphase": st[1], "narrowphase": st[2], "pgs+setup": st[3] + st[4], "integrate": st[5]}
+    st = dict(zip(STAGE_NAMES, (acc / reps).tolist()))
     n_pairs = len(w.pairs())
     n_contacts = len(w.contacts())
-    sb = stage_bytes(sc.num_bodies, n_pairs, n_contacts, args.iters)
-    dom = max(times, key=lambda s: times[s])
-    achieved = sb[dom] / (times[dom] * 1e-3) / 1e9
-    total_bytes = sum(sb.values())
+    launches = {"narrowphase_kernel": 1, "pgs_normal": 8 * args.iters, "pgs_friction": 8 * args.iters}
+    kb = kernel_bytes(sc.num_bodies, n_pairs, n_contacts)
+    dom = max(launches, key=lambda k: st[k])
+    avg_ms = st[dom] / launches[dom]
+    achieved = kb[dom] / (avg_ms * 1e-3) / 1e9
+    total_bytes = step_bytes(sc.num_bodies, n_pairs, n_contacts, args.iters)
+    traffic = args.traffic
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if traffic is None and os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        if t.get("kernel") == dom:
+            traffic = t.get("bytes_per_launch")
     out = {
         "metric": "body-steps/s at 128k convexes",
         "value": round(value, 1),
@@ -105,9 +123,10 @@ def run(args):
                                "%d PGS iterations, dt 1/60" % (n_dyn, args.iters),
                    "bodies": n_dyn, "pairs": n_pairs, "contacts": n_contacts, "pgs_iterations": args.iters,
                    "setup_s": round(setup_s, 2)},
-        "stages_ms": {k: round(float(v), 4) for k, v in times.items()},
+        "stages_ms": {k: round(float(v), 4) for k, v in st.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "launch_ms": round(avg_ms, 5), "bytes_per_launch": int(kb[dom]),
                      "step_algorithmic_bytes": int(total_bytes),
                      "step_frac": round(total_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
     }
@@ -138,6 +157,18 @@ def cpu_baseline(w, args):
             "sample": "%d oracle steps of the same pile from the GPU's post-warmup state (%.1f s)" % (steps, sec)}
 
 
+def calibrate_copy(mb):
+    import ctypes
+    from bullet3_amd import capi
+    nbytes = mb << 20
+    a, b = capi.DeviceBuffer(nbytes), capi.DeviceBuffer(nbytes)
+    capi.call("b3h_memcpy_dtod", ctypes.c_void_p(b.ptr), ctypes.c_void_p(a.ptr), nbytes)
+    capi.call("b3h_synchronize")
+    print(json.dumps({"calibration_copy_bytes": nbytes}), flush=True)
+    a.free()
+    b.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,12 +182,19 @@ def main():
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="after the run, one device-to-device copy of this many MiB (a plain copy kernel of known "
+                         "size, to scale rocprof FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per launch of the dominant kernel from rocprof FETCH_SIZE+WRITE_SIZE runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (the step shards by scene or "
-              "spatial slab, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path here (the step would shard by scene "
+              "or by spatial slab); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     print(json.dumps(run(args)), flush=True)
+    if args.calibrate_copy_mb > 0:
+        calibrate_copy(args.calibrate_copy_mb)
 
 
 if __name__ == "__main__":

@@ -14,6 +14,9 @@ LIB_PATH = os.path.join(_HERE, "libb3hip.so")
 _LIB = None
 
 B3H_OK = 0
+NUM_STAGES = 8
+STAGE_NAMES = ("aabb", "broadphase", "narrowphase_kernel", "narrowphase_compact", "solver_setup",
+               "pgs_normal", "pgs_friction", "integrate")
 BROADPHASE_GRID = 0
 BROADPHASE_SAP = 1
 
@@ -99,6 +102,8 @@ _SIGS = {
     "b3h_pipeline_set_solver_params": ([_vp, ctypes.POINTER(SolverParams)], _i),
     "b3h_pipeline_get_solver_params": ([_vp, ctypes.POINTER(SolverParams)], _i),
     "b3h_pipeline_get_solver_debug": ([_vp, _pvp, _pvp, _pvp, _pvp, _pvp, _pi], _i),
+    "b3h_pipeline_set_pgs_body_cache": ([_vp, _i], _i),
+    "b3h_pipeline_get_pgs_levels": ([_vp, _vp], _i),
     "b3h_pipeline_set_track_residual": ([_vp, _i], _i),
     "b3h_pipeline_get_residuals": ([_vp, _vp, _i], _i),
     "b3h_pipeline_set_warm_start_impulses": ([_vp, _vp, _i], _i),

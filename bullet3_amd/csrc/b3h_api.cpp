@@ -518,11 +518,12 @@ static b3hShapeTables np_tables(b3hNarrowPhase* np)
 }
 
 /* device-count form used inside a step */
-static void np_compute_contacts_dev(b3hNarrowPhase* np, b3hInt4* pairs, const int* dNumPairs, int maxPairs)
+static void np_compute_contacts_dev(b3hNarrowPhase* np, b3hInt4* pairs, const int* dNumPairs, int maxPairs,
+									hipEvent_t evAfterKernel = nullptr)
 {
 	b3h_narrowphase(pairs, dNumPairs, maxPairs, np->dBodies, np_tables(np), np->dStaging, np->dFlags, np->dFlagScan,
 					np->dScanTmp, np->dContactPair, np->cfg.m_maxContactCapacity, np->dNumContacts, np->dSepState,
-					np->dSepNormal, np->lanesPerPair, np->stream);
+					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel);
 	np->contactsValid = false;
 }
 
@@ -910,7 +911,7 @@ struct b3hPipeline
 	b3hSolverBuffers sb;
 	bool trackResidual;
 	bool timing;
-	hipEvent_t ev[7];
+	hipEvent_t ev[B3H_NUM_STAGES + 1];
 	bool evValid;
 	hipGraph_t graph;
 	hipGraphExec_t graphExec;
@@ -934,6 +935,14 @@ static void pl_free(b3hPipeline* p)
 	dfree(p->sb.constraints);
 	dfree(p->sb.sortedContacts);
 	dfree(p->sb.resid);
+	dfree(p->sb.cellBodies);
+	dfree(p->sb.cellBodyCount);
+	dfree(p->sb.cslot);
+	dfree(p->sb.recN);
+	dfree(p->sb.recF);
+	dfree(p->sb.levelOrder);
+	dfree(p->sb.levelStart);
+	dfree(p->sb.cellLevels);
 }
 
 extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const b3hConfig* cfg, void* stream,
@@ -974,11 +983,16 @@ extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const 
 	}
 	int mc = cfg->m_maxContactCapacity;
 	p->sb.maxContacts = mc;
+	p->sb.pgsBodyCap = -1;
 	if ((r = dalloc(&p->sb.cellOf, mc)) || (r = dalloc(&p->sb.cellCounts, B3H_N_CELLS)) ||
 		(r = dalloc(&p->sb.cellOffsets, B3H_N_CELLS + 1)) || (r = dalloc(&p->sb.cellCursor, B3H_N_CELLS)) ||
 		(r = dalloc(&p->sb.cellSlots, mc)) || (r = dalloc(&p->sb.perm, mc)) || (r = dalloc(&p->sb.batchIdx, mc)) ||
 		(r = dalloc(&p->sb.batchSizes, B3H_N_CELLS)) || (r = dalloc(&p->sb.sortKeys, 4 * (size_t)mc + 64)) ||
-		(r = dalloc(&p->sb.constraints, mc)) || (r = dalloc(&p->sb.sortedContacts, mc)) || (r = dalloc(&p->sb.resid, 4 * 64)))
+		(r = dalloc(&p->sb.constraints, mc)) || (r = dalloc(&p->sb.sortedContacts, mc)) || (r = dalloc(&p->sb.resid, 4 * 64)) ||
+		(r = dalloc(&p->sb.cellBodies, 2 * (size_t)mc + 64)) || (r = dalloc(&p->sb.cellBodyCount, B3H_N_CELLS)) ||
+		(r = dalloc(&p->sb.cslot, 2 * (size_t)mc + 2)) || (r = dalloc(&p->sb.recN, 21 * (size_t)mc)) ||
+		(r = dalloc(&p->sb.recF, 16 * (size_t)mc)) || (r = dalloc(&p->sb.levelOrder, mc)) ||
+		(r = dalloc(&p->sb.levelStart, (size_t)mc + B3H_N_CELLS + 8)) || (r = dalloc(&p->sb.cellLevels, B3H_N_CELLS)))
 	{
 		pl_free(p);
 		delete p;
@@ -999,7 +1013,7 @@ extern "C" int b3h_pipeline_destroy(b3hPipeline* p)
 	if (p->graphExec) hipGraphExecDestroy(p->graphExec);
 	if (p->graph) hipGraphDestroy(p->graph);
 	if (p->evValid)
-		for (int i = 0; i < 7; i++) hipEventDestroy(p->ev[i]);
+		for (int i = 0; i <= B3H_NUM_STAGES; i++) hipEventDestroy(p->ev[i]);
 	pl_free(p);
 	delete p;
 	return B3H_OK;
@@ -1012,30 +1026,33 @@ static void pl_aabbs(b3hPipeline* p)
 	b3h_launch_world_aabbs(num_bodies(p), p->np->dBodies, p->np->dCols, p->np->dLocalAabbs, p->bp->dAll, p->stream);
 }
 
-static void pl_solve(b3hPipeline* p)
+static void pl_solve(b3hPipeline* p, hipEvent_t evSetupDone = nullptr, hipEvent_t evNormalDone = nullptr)
 {
 	p->sb.warmImpulses = p->dWarm;
 	b3h_solve_contacts(num_bodies(p), p->np->dBodies, p->np->dInertias, p->np->dNumContacts, p->cfg.m_maxContactCapacity,
 					   p->np->dStaging, p->np->dContactPair, p->prm, p->np->static0Index, p->sb, true, p->trackResidual,
-					   p->stream);
+					   p->stream, evSetupDone, evNormalDone);
 }
 
 static int pl_step(b3hPipeline* p, float dt, bool events)
 {
 	hipStream_t s = p->stream;
+	/* stage events: 0 start | aabb | 1 | broadphase | 2 | narrowphase kernel | 3 |
+	 * narrowphase compaction | 4 | solver set-up | 5 | normal sweeps | 6 |
+	 * friction sweeps | 7 | integrate | 8 */
 	if (events) hipEventRecord(p->ev[0], s);
 	pl_aabbs(p);
 	if (events) hipEventRecord(p->ev[1], s);
 	int r = bp_calc_pairs(p->bp, p->cfg.m_maxBroadphasePairs, s);
 	if (r) return r;
 	if (events) hipEventRecord(p->ev[2], s);
-	np_compute_contacts_dev(p->np, p->bp->dPairs, p->bp->dPairCount, p->cfg.m_maxBroadphasePairs);
-	if (events) hipEventRecord(p->ev[3], s);
-	pl_solve(p);
+	np_compute_contacts_dev(p->np, p->bp->dPairs, p->bp->dPairCount, p->cfg.m_maxBroadphasePairs,
+							events ? p->ev[3] : nullptr);
 	if (events) hipEventRecord(p->ev[4], s);
-	if (events) hipEventRecord(p->ev[5], s);
+	pl_solve(p, events ? p->ev[5] : nullptr, events ? p->ev[6] : nullptr);
+	if (events) hipEventRecord(p->ev[7], s);
 	b3h_launch_integrate(num_bodies(p), p->np->dBodies, dt, p->prm.m_angularDamping, p->gravity, s);
-	if (events) hipEventRecord(p->ev[6], s);
+	if (events) hipEventRecord(p->ev[8], s);
 	return B3H_OK;
 }
 
@@ -1051,7 +1068,7 @@ extern "C" int b3h_pipeline_step_simulation(b3hPipeline* p, float dt)
 	}
 	if (p->timing && !p->evValid)
 	{
-		for (int i = 0; i < 7; i++) B3H_CHECK(hipEventCreate(&p->ev[i]));
+		for (int i = 0; i <= B3H_NUM_STAGES; i++) B3H_CHECK(hipEventCreate(&p->ev[i]));
 		p->evValid = true;
 	}
 	int r = pl_step(p, dt, p->timing);
@@ -1237,6 +1254,31 @@ extern "C" int b3h_pipeline_get_solver_debug(b3hPipeline* p, b3hContact4Data** s
 	if (n) return read_int(p->np->dNumContacts, n, p->stream);
 	return B3H_OK;
 }
+extern "C" int b3h_pipeline_set_pgs_body_cache(b3hPipeline* p, int maxBodiesPerCell)
+{
+	CHECK_ARG(p);
+	p->sb.pgsBodyCap = maxBodiesPerCell;
+	if (p->graphExec)
+	{
+		hipGraphExecDestroy(p->graphExec);
+		p->graphExec = nullptr;
+	}
+	if (p->graph)
+	{
+		hipGraphDestroy(p->graph);
+		p->graph = nullptr;
+	}
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_get_pgs_levels(b3hPipeline* p, int* levels)
+{
+	CHECK_ARG(p && levels);
+	B3H_CHECK(hipMemcpyAsync(levels, p->sb.cellLevels, sizeof(int) * B3H_N_CELLS, hipMemcpyDeviceToHost, p->stream));
+	B3H_CHECK(hipStreamSynchronize(p->stream));
+	return B3H_OK;
+}
+
 extern "C" int b3h_pipeline_set_track_residual(b3hPipeline* p, int enable)
 {
 	CHECK_ARG(p);
@@ -1274,20 +1316,20 @@ extern "C" int b3h_pipeline_enable_timing(b3hPipeline* p, int enable)
 	p->timing = enable != 0;
 	return B3H_OK;
 }
-extern "C" int b3h_pipeline_get_stage_times(b3hPipeline* p, float* ms6)
+extern "C" int b3h_pipeline_get_stage_times(b3hPipeline* p, float* ms)
 {
-	CHECK_ARG(p && ms6);
+	CHECK_ARG(p && ms);
 	if (!p->evValid)
 	{
 		b3h_set_error("timing not enabled");
 		return B3H_ERR_STATE;
 	}
-	B3H_CHECK(hipEventSynchronize(p->ev[6]));
-	for (int i = 0; i < 6; i++)
+	B3H_CHECK(hipEventSynchronize(p->ev[B3H_NUM_STAGES]));
+	for (int i = 0; i < B3H_NUM_STAGES; i++)
 	{
-		float ms = 0.f;
-		B3H_CHECK(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
-		ms6[i] = ms;
+		float t = 0.f;
+		B3H_CHECK(hipEventElapsedTime(&t, p->ev[i], p->ev[i + 1]));
+		ms[i] = t;
 	}
 	return B3H_OK;
 }

@@ -76,7 +76,7 @@ struct b3hShapeTables
 void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3hRigidBodyData* bodies,
 					 const b3hShapeTables& sh, b3hContact4Data* staging, int* flags, int* flagScan,
 					 unsigned* scanTmp, int* contactPair, int maxContacts, int* numContacts, int* sepState,
-					 b3hFloat4* sepNormal, int pairsPerGroup, hipStream_t s);
+					 b3hFloat4* sepNormal, int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel = nullptr);
 void b3h_gather_contacts(const int* contactPair, const int* numContacts, int maxContacts,
 						 const b3hContact4Data* staging, b3hContact4Data* out, hipStream_t s);
 
@@ -102,6 +102,15 @@ struct b3hSolverBuffers
 	b3hContact4Data* sortedContacts; /* optional */
 	float* resid;         /* 2 * maxIters * 2 (maxDelta, maxLambda) */
 	float* warmImpulses;  /* optional, 6 per constraint */
+	int* cellBodies;      /* per cell at 2*cellOffsets[cell]: its distinct dynamic bodies, ascending */
+	int* cellBodyCount;   /* 256 */
+	int* cslot;           /* per constraint: local slots of bodyA, bodyB in its cell's list (-1: static) */
+	int pgsBodyCap;       /* < 0: the LDS budget; otherwise a lower cap on cached bodies per cell */
+	b3hFloat4* recN;      /* per constraint: 21 float4 of normal-row invariants */
+	b3hFloat4* recF;      /* per constraint: 16 float4 of friction-row invariants */
+	int* levelOrder;      /* per cell region [off, off+n): constraints grouped by dependency level */
+	int* levelStart;      /* per cell region [off+cell, off+cell+L+1) */
+	int* cellLevels;      /* 256: levels per cell, -1 = global walk */
 	int maxContacts;
 };
 int b3h_solver_module_init();
@@ -109,6 +118,7 @@ int b3h_solver_lds_bytes();
 void b3h_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaData* inertias,
 						const int* numContacts, int maxContacts, const b3hContact4Data* staging,
 						const int* contactPair, const b3hSolverParams& prm, int staticIdx, b3hSolverBuffers& sb,
-						bool writeSortedContacts, bool trackResidual, hipStream_t s);
+						bool writeSortedContacts, bool trackResidual, hipStream_t s, hipEvent_t evSetupDone = nullptr,
+						hipEvent_t evNormalDone = nullptr);
 
 #endif

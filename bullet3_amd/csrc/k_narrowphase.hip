@@ -146,14 +146,14 @@ __device__ __forceinline__ void vadd3(F4* v, F4 w)
 	v->z += w.z;
 }
 
-__device__ void portal_dir(const Simplex* p, F4* dir)
+__device__ __forceinline__ void portal_dir(const Simplex* p, F4* dir)
 {
 	F4 v2v1 = sub4(p->ps[2].v, p->ps[1].v);
 	F4 v3v1 = sub4(p->ps[3].v, p->ps[1].v);
 	*dir = cross3(v2v1, v3v1);
 	mpr_normalize(dir);
 }
-__device__ int portal_reach_tolerance(const Simplex* p, const Sup* v4, F4 dir)
+__device__ __forceinline__ int portal_reach_tolerance(const Simplex* p, const Sup* v4, F4 dir)
 {
 	float dv1 = dot3(p->ps[1].v, dir);
 	float dv2 = dot3(p->ps[2].v, dir);
@@ -164,29 +164,35 @@ __device__ int portal_reach_tolerance(const Simplex* p, const Sup* v4, F4 dir)
 	d1 = d1 < d3 ? d1 : d3;
 	return mpr_eq(d1, MPR_TOL) || d1 < MPR_TOL;
 }
-__device__ void expand_portal(Simplex* p, const Sup* v4)
+/* conditional copies as value selects (keeps the simplex in registers) */
+__device__ __forceinline__ F4 sel4(bool c, F4 a, F4 b) { return f4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w); }
+__device__ __forceinline__ void sup_set_if(Sup& dst, const Sup& src, bool c)
+{
+	dst.v = sel4(c, src.v, dst.v);
+	dst.v1 = sel4(c, src.v1, dst.v1);
+	dst.v2 = sel4(c, src.v2, dst.v2);
+}
+__device__ __forceinline__ void expand_portal(Simplex* p, const Sup* v4)
 {
 	F4 v4v0 = cross3(v4->v, p->ps[0].v);
 	float d = dot3(p->ps[1].v, v4v0);
+	int slot;
 	if (d > 0.f)
 	{
 		d = dot3(p->ps[2].v, v4v0);
-		if (d > 0.f)
-			p->ps[1] = *v4;
-		else
-			p->ps[3] = *v4;
+		slot = d > 0.f ? 1 : 3;
 	}
 	else
 	{
 		d = dot3(p->ps[3].v, v4v0);
-		if (d > 0.f)
-			p->ps[2] = *v4;
-		else
-			p->ps[1] = *v4;
+		slot = d > 0.f ? 2 : 1;
 	}
+	sup_set_if(p->ps[1], *v4, slot == 1);
+	sup_set_if(p->ps[2], *v4, slot == 2);
+	sup_set_if(p->ps[3], *v4, slot == 3);
 }
 template <class SP>
-__device__ int discover_portal(SP& S, Simplex* p)
+__device__ __forceinline__ int discover_portal(SP& S, Simplex* p)
 {
 	F4 dir, va, vb;
 	float d;
@@ -222,12 +228,12 @@ __device__ int discover_portal(SP& S, Simplex* p)
 	dir = cross3(va, vb);
 	mpr_normalize(&dir);
 	d = dot3(dir, p->ps[0].v);
-	if (d > 0.f)
 	{
+		bool swap = d > 0.f;
 		Sup t = p->ps[1];
-		p->ps[1] = p->ps[2];
-		p->ps[2] = t;
-		dir = mul4(dir, -1.f);
+		sup_set_if(p->ps[1], p->ps[2], swap);
+		sup_set_if(p->ps[2], t, swap);
+		dir = sel4(swap, mul4(dir, -1.f), dir);
 	}
 	while (p->last + 1 < 4)
 	{
@@ -237,20 +243,16 @@ __device__ int discover_portal(SP& S, Simplex* p)
 		int cont = 0;
 		va = cross3(p->ps[1].v, p->ps[3].v);
 		d = dot3(va, p->ps[0].v);
-		if (d < 0.f && !mpr_is_zero(d))
-		{
-			p->ps[2] = p->ps[3];
-			cont = 1;
-		}
+		bool c1 = d < 0.f && !mpr_is_zero(d);
+		sup_set_if(p->ps[2], p->ps[3], c1);
+		cont = c1 ? 1 : 0;
 		if (!cont)
 		{
 			va = cross3(p->ps[3].v, p->ps[2].v);
 			d = dot3(va, p->ps[0].v);
-			if (d < 0.f && !mpr_is_zero(d))
-			{
-				p->ps[1] = p->ps[3];
-				cont = 1;
-			}
+			bool c2 = d < 0.f && !mpr_is_zero(d);
+			sup_set_if(p->ps[1], p->ps[3], c2);
+			cont = c2 ? 1 : 0;
 		}
 		if (cont)
 		{
@@ -265,7 +267,7 @@ __device__ int discover_portal(SP& S, Simplex* p)
 	return 0;
 }
 template <class SP>
-__device__ int refine_portal(SP& S, Simplex* p)
+__device__ __forceinline__ int refine_portal(SP& S, Simplex* p)
 {
 	F4 dir;
 	Sup v4;
@@ -282,7 +284,7 @@ __device__ int refine_portal(SP& S, Simplex* p)
 	}
 	return -1;
 }
-__device__ void find_pos(const Simplex* p, F4* pos)
+__device__ __forceinline__ void find_pos(const Simplex* p, F4* pos)
 {
 	F4 dir, vec, p1, p2;
 	float b[4], sum, inv;
@@ -310,20 +312,30 @@ __device__ void find_pos(const Simplex* p, F4* pos)
 	inv = 1.f / sum;
 	p1 = f4(0, 0, 0, 0);
 	p2 = f4(0, 0, 0, 0);
-	for (int i = 0; i < 4; i++)
-	{
-		vec = mul4(p->ps[i].v1, b[i]);
-		vadd3(&p1, vec);
-		vec = mul4(p->ps[i].v2, b[i]);
-		vadd3(&p2, vec);
-	}
+	/* i = 0..3, written out so the simplex stays in registers */
+	vec = mul4(p->ps[0].v1, b[0]);
+	vadd3(&p1, vec);
+	vec = mul4(p->ps[0].v2, b[0]);
+	vadd3(&p2, vec);
+	vec = mul4(p->ps[1].v1, b[1]);
+	vadd3(&p1, vec);
+	vec = mul4(p->ps[1].v2, b[1]);
+	vadd3(&p2, vec);
+	vec = mul4(p->ps[2].v1, b[2]);
+	vadd3(&p1, vec);
+	vec = mul4(p->ps[2].v2, b[2]);
+	vadd3(&p2, vec);
+	vec = mul4(p->ps[3].v1, b[3]);
+	vadd3(&p1, vec);
+	vec = mul4(p->ps[3].v2, b[3]);
+	vadd3(&p2, vec);
 	p1 = mul4(p1, inv);
 	p2 = mul4(p2, inv);
 	*pos = p1;
 	vadd3(pos, p2);
 	*pos = mul4(*pos, 0.5f);
 }
-__device__ float point_segment_dist2(F4 P, F4 x0, F4 b, F4* witness)
+__device__ __forceinline__ float point_segment_dist2(F4 P, F4 x0, F4 b, F4* witness)
 {
 	float dist, t;
 	F4 d = sub4(b, x0);
@@ -351,7 +363,7 @@ __device__ float point_segment_dist2(F4 P, F4 x0, F4 b, F4* witness)
 	}
 	return dist;
 }
-__device__ float point_tri_dist2(F4 P, F4 x0, F4 B, F4 Cc, F4* witness)
+__device__ __forceinline__ float point_tri_dist2(F4 P, F4 x0, F4 B, F4 Cc, F4* witness)
 {
 	F4 d1 = sub4(B, x0), d2 = sub4(Cc, x0), a = sub4(x0, P);
 	float v = dot3(d1, d1), w = dot3(d2, d2);
@@ -390,7 +402,7 @@ __device__ float point_tri_dist2(F4 P, F4 x0, F4 B, F4 Cc, F4* witness)
 	return dist;
 }
 template <class SP>
-__device__ void find_penetr(SP& S, Simplex* p, float* depth, F4* pdir, F4* pos)
+__device__ __forceinline__ void find_penetr(SP& S, Simplex* p, float* depth, F4* pdir, F4* pos)
 {
 	F4 dir;
 	Sup v4;
@@ -414,7 +426,7 @@ __device__ void find_penetr(SP& S, Simplex* p, float* depth, F4* pdir, F4* pos)
 	}
 }
 template <class SP>
-__device__ int mpr_penetration(SP& S, float* depth, F4* dir, F4* pos)
+__device__ __forceinline__ int mpr_penetration(SP& S, float* depth, F4* dir, F4* pos)
 {
 	Simplex portal;
 	int res = discover_portal(S, &portal);
@@ -457,7 +469,7 @@ struct GroupSupport
 	const F4* vB;
 	int nA, nB;
 	F4 posA, posB, qA, qB;
-	__device__ F4 one(const F4* vs, int nv, F4 pos, F4 q, F4 dirIn) const
+	__device__ __forceinline__ F4 one(const F4* vs, int nv, F4 pos, F4 q, F4 dirIn) const
 	{
 		F4 dir = f4(dirIn.x, dirIn.y, dirIn.z, 0.f);
 		F4 localDir = qrot(qinv(q), dir);
@@ -476,8 +488,49 @@ struct GroupSupport
 		if (bi == INT_MAX) bi = 0;
 		return xform(vs[bi], pos, q);
 	}
-	__device__ void support(F4 dir, Sup* s) const
+	__device__ __forceinline__ void support(F4 dir, Sup* s) const
 	{
+		if (L == 64 && nA <= 32 && nB <= 32)
+		{
+			/* both hulls at once: lanes 0-31 take A along dir, lanes 32-63 take B
+			 * along -dir; each half runs the same arithmetic as one() */
+			bool upper = g.lane >= 32;
+			F4 q = upper ? qB : qA;
+			F4 pos = upper ? posB : posA;
+			const F4* vs = upper ? vB : vA;
+			int nv = upper ? nB : nA;
+			F4 dirIn = upper ? mul4(dir, -1.f) : dir;
+			F4 d3 = f4(dirIn.x, dirIn.y, dirIn.z, 0.f);
+			F4 localDir = qrot(qinv(q), d3);
+			int i = g.lane & 31;
+			float bv = -INFINITY;
+			int bi = INT_MAX;
+			if (i < nv)
+			{
+				float d = dot3(vs[i], localDir);
+				if (d > -LARGE_FLOAT)
+				{
+					bv = d;
+					bi = i;
+				}
+			}
+			for (int m = 1; m < 32; m <<= 1)
+			{
+				float ov = __shfl_xor(bv, m);
+				int oi = __shfl_xor(bi, m);
+				if (ov > bv || (ov == bv && oi < bi))
+				{
+					bv = ov;
+					bi = oi;
+				}
+			}
+			if (bi == INT_MAX) bi = 0;
+			F4 w = xform(vs[bi], pos, q);
+			s->v1 = g.bcast(w, g.base);
+			s->v2 = g.bcast(w, g.base + 32);
+			s->v = sub4(s->v1, s->v2);
+			return;
+		}
 		s->v1 = one(vA, nA, posA, qA, dir);
 		s->v2 = one(vB, nB, posB, qB, mul4(dir, -1.f));
 		s->v = sub4(s->v1, s->v2);
@@ -923,8 +976,9 @@ __global__ __launch_bounds__(NP_THREADS) void k_narrowphase(const b3hInt4* __res
 				if (nReduced > 0)
 				{
 					c.m_worldNormalOnB = sep;
-					for (int k = 0; k < nReduced; k++)
-						if (k > 0 || !mprHit) c.m_worldPosB[k] = pts[idx[k]];
+#pragma unroll
+					for (int k = 0; k < 4; k++)
+						if (k < nReduced && (k > 0 || !mprHit)) c.m_worldPosB[k] = pts[idx[k]];
 					c.m_worldNormalOnB.w = (float)nReduced;
 				}
 				staging[i] = c;
@@ -972,7 +1026,7 @@ int b3h_np_module_init()
 void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3hRigidBodyData* bodies,
 					 const b3hShapeTables& sh, b3hContact4Data* staging, int* flags, int* flagScan, unsigned* scanTmp,
 					 int* contactPair, int maxContacts, int* numContacts, int* sepState, b3hFloat4* sepNormal,
-					 int pairsPerGroup, hipStream_t s)
+					 int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel)
 {
 	(void)pairsPerGroup;
 	const int groups = NP_THREADS / 64;
@@ -981,6 +1035,7 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 	if (blocks < 1) blocks = 1;
 	hipLaunchKernelGGL(k_narrowphase<64>, dim3(blocks), dim3(NP_THREADS), 0, s, pairs, numPairs, bodies, sh, staging,
 					   flags, sepState, sepNormal);
+	if (evAfterKernel) hipEventRecord(evAfterKernel, s);
 	b3h_exclusive_scan(flags, flagScan, numPairs, maxPairs, scanTmp, s);
 	int cblocks = (maxPairs + 255) / 256;
 	if (cblocks > 16384) cblocks = 16384;

@@ -25,6 +25,14 @@
 #include "b3h_internal.h"
 
 #define CHECK_SIZE 64
+
+/* LDS ordering between lanes of one wavefront */
+__device__ __forceinline__ void wave_sync_solver()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 #define SIMD_WIDTH 64
 
 __device__ __forceinline__ int cell_index(F4 p, float scale)
@@ -99,7 +107,6 @@ __global__ __launch_bounds__(256) void k_batch_cells(const int* __restrict__ cel
 													 int* __restrict__ batchIdxOut, int* __restrict__ batchSizes)
 {
 	extern __shared__ unsigned long long smem[];
-	__shared__ unsigned bitmap[CHECK_SIZE];
 	int cell = blockIdx.x;
 	int n = cellCounts[cell];
 	int off = cellOffsets[cell];
@@ -162,49 +169,94 @@ __global__ __launch_bounds__(256) void k_batch_cells(const int* __restrict__ cel
 			}
 			__syncthreads();
 		}
-	/* greedy batching (serial, as the reference): batch ids go into the high
-	 * half of pay[] */
-	if (t == 0)
+	/* greedy batching, wave-parallel with the sequential semantics of
+	 * CreateBatchesNew: the 2048-bit bitmap is held one 32-bit word per lane;
+	 * each chunk of 64 candidates is pre-checked against the bitmap at chunk
+	 * start, then acceptances are resolved in candidate order (lowest pending
+	 * lane first; later candidates that hit its bits drop out; after every 64th
+	 * accepted contact the bitmap is cleared and the later lanes are
+	 * re-examined against the empty map).  Accepted contacts are then swapped
+	 * to the front one by one, exactly as the reference moves them (that
+	 * permutes the rejected contacts, and so the next pass's scan order).
+	 * Positions not yet examined are never written by the swaps, so each chunk
+	 * can be read up front.  Batch ids go into the high half of pay[]. */
+	if (t < 64)
 	{
+		const int lane = t;
 		int numValid = 0, batch = 0;
 		while (numValid < n)
 		{
 			int nCur = 0;
-			for (int w = 0; w < CHECK_SIZE; w++) bitmap[w] = 0;
-			for (int i = numValid; i < n; i++)
+			unsigned word = 0u; /* bitmap word `lane` */
+			for (int c0 = numValid; c0 < n; c0 += 64)
 			{
-				unsigned long long kk = key[i];
+				int i = c0 + lane;
+				bool valid = i < n;
+				unsigned long long kk = valid ? key[i] : 0ull;
 				int aS = (int)(unsigned)(kk >> 32), bS = (int)(unsigned)(kk & 0xffffffffu);
 				int a = iabs(aS), b = iabs(bS);
 				bool aStatic = (aS < 0) || aS == staticIdx;
 				bool bStatic = (bS < 0) || bS == staticIdx;
 				int ia = a % (32 * CHECK_SIZE), ib = b % (32 * CHECK_SIZE);
-				unsigned aU = aStatic ? 0u : (bitmap[ia >> 5] & (1u << (ia & 31)));
-				unsigned bU = bStatic ? 0u : (bitmap[ib >> 5] & (1u << (ib & 31)));
-				if (aU == 0 && bU == 0)
+				int wa = ia >> 5, wb = ib >> 5;
+				unsigned ma = aStatic ? 0u : (1u << (ia & 31));
+				unsigned mb = bStatic ? 0u : (1u << (ib & 31));
+				unsigned wA = __shfl(word, wa), wB = __shfl(word, wb);
+				bool pre = valid && (wA & ma) == 0u && (wB & mb) == 0u;
+				unsigned long long cand = __ballot(pre);
+				unsigned long long acc = 0ull;
+				const unsigned long long validMask = __ballot(valid);
+				while (cand)
 				{
-					if (!aStatic) bitmap[ia >> 5] |= (1u << (ia & 31));
-					if (!bStatic) bitmap[ib >> 5] |= (1u << (ib & 31));
-					unsigned long long pi = (pay[i] & 0xffffffffull) | ((unsigned long long)(unsigned)batch << 32);
-					if (i != numValid)
-					{
-						key[i] = key[numValid];
-						pay[i] = pay[numValid];
-						key[numValid] = kk;
-					}
-					pay[numValid] = pi;
-					numValid++;
+					int j = __ffsll((long long)cand) - 1;
+					acc |= 1ull << j;
+					int jwa = __shfl(wa, j), jwb = __shfl(wb, j);
+					unsigned jma = __shfl(ma, j), jmb = __shfl(mb, j);
+					if (lane == jwa) word |= jma;
+					if (lane == jwb) word |= jmb;
 					nCur++;
+					unsigned long long later = (j == 63) ? 0ull : (~0ull << (j + 1));
 					if (nCur == SIMD_WIDTH)
 					{
 						nCur = 0;
-						for (int w = 0; w < CHECK_SIZE; w++) bitmap[w] = 0;
+						word = 0u;
+						cand = validMask & later;
+					}
+					else
+					{
+						bool hit = (wa == jwa && (ma & jma)) || (wa == jwb && (ma & jmb)) || (wb == jwa && (mb & jma)) ||
+								   (wb == jwb && (mb & jmb));
+						cand &= later & ~__ballot(hit);
 					}
 				}
+				/* move the accepted contacts to the front, in order */
+				if (lane == 0)
+				{
+					unsigned long long m = acc;
+					while (m)
+					{
+						int j = __ffsll((long long)m) - 1;
+						m &= m - 1;
+						int ii = c0 + j;
+						unsigned long long kj = key[ii];
+						unsigned long long pj = pay[ii];
+						unsigned long long pi = (pj & 0xffffffffull) | ((unsigned long long)(unsigned)batch << 32);
+						if (ii != numValid)
+						{
+							key[ii] = key[numValid];
+							pay[ii] = pay[numValid];
+							key[numValid] = kj;
+						}
+						pay[numValid] = pi;
+						numValid++;
+					}
+				}
+				numValid = __shfl(numValid, 0);
+				wave_sync_solver();
 			}
 			batch++;
 		}
-		batchSizes[cell] = batch;
+		if (lane == 0) batchSizes[cell] = batch;
 	}
 	__syncthreads();
 	for (int i = t; i < n; i += blockDim.x)
@@ -506,33 +558,656 @@ __device__ __forceinline__ int cell_of(int wgIdx, int ib)
 	return xIdx + yIdx * B3H_N_SPLIT_X + zIdx * (B3H_N_SPLIT_X * B3H_N_SPLIT_Y);
 }
 
-/* one wave per cell; BatchSolveKernelContact's batch walk (solveContact.cl:457-479) */
-__global__ __launch_bounds__(64) void k_pgs(b3hRigidBodyData* __restrict__ bodies, const b3hInertiaData* __restrict__ inertias,
-											b3hContactConstraint4* __restrict__ cs, const int* __restrict__ cellCounts,
-											const int* __restrict__ cellOffsets, const int* __restrict__ batchSizes,
-											const int* __restrict__ batchIdx, int ib, int phase, float friction,
-											const float* __restrict__ warm, float* __restrict__ resid)
+/* ---------------- per-cell body lists (for the LDS velocity cache) ----------------
+ * For every cell: its distinct dynamic bodies in ascending order (at
+ * cellBodies + 2*cellOffsets[cell]) and, per constraint, the positions of
+ * bodyA / bodyB in that list (-1 for a static body).  One work-group per cell:
+ * bitonic sort of the 2n body references in LDS (global scratch when the cell
+ * is larger than the LDS budget), a block scan of the run heads, then a binary
+ * search per reference. */
+__global__ __launch_bounds__(256) void k_cell_bodies(const int* __restrict__ cellCounts, const int* __restrict__ cellOffsets,
+													 const b3hContactConstraint4* __restrict__ cs,
+													 const b3hRigidBodyData* __restrict__ bodies, int ldsCapKeys,
+													 unsigned* __restrict__ gScratch, int* __restrict__ cellBodies,
+													 int* __restrict__ cellBodyCount, int* __restrict__ cslot)
 {
+	extern __shared__ unsigned long long smem[];
+	__shared__ int waveSums[4];
+	__shared__ int runTotal;
+	int cell = blockIdx.x;
+	int n = cellCounts[cell];
+	int off = cellOffsets[cell];
+	int t = threadIdx.x;
+	if (n == 0)
+	{
+		if (t == 0) cellBodyCount[cell] = 0;
+		return;
+	}
+	int m = 2 * n;
+	int np2 = 1;
+	while (np2 < m) np2 <<= 1;
+	/* keys[np2] then rank[np2]; global fallback region of 8n >= 2*np2 entries */
+	unsigned* key = (2 * np2 <= ldsCapKeys) ? (unsigned*)smem : gScratch + 8 * (size_t)off;
+	unsigned* rank = key + np2;
+	for (int i = t; i < np2; i += blockDim.x)
+	{
+		unsigned k = ~0u;
+		if (i < m)
+		{
+			const b3hContactConstraint4& c = cs[off + (i >> 1)];
+			unsigned body = (i & 1) ? c.m_bodyB : c.m_bodyA;
+			if (bodies[body].m_invMass != 0.f) k = body;
+		}
+		key[i] = k;
+	}
+	__syncthreads();
+	for (int k = 2; k <= np2; k <<= 1)
+		for (int j = k >> 1; j > 0; j >>= 1)
+		{
+			for (int i = t; i < np2; i += blockDim.x)
+			{
+				int ixj = i ^ j;
+				if (ixj > i)
+				{
+					unsigned a = key[i], b = key[ixj];
+					if ((a > b) == ((i & k) == 0))
+					{
+						key[i] = b;
+						key[ixj] = a;
+					}
+				}
+			}
+			__syncthreads();
+		}
+	/* rank of each sorted entry among the distinct values (run heads scanned) */
+	if (t == 0) runTotal = 0;
+	__syncthreads();
+	int lane = t & 63, wave = t >> 6;
+	for (int c0 = 0; c0 < np2; c0 += blockDim.x)
+	{
+		int i = c0 + t;
+		unsigned k = i < np2 ? key[i] : ~0u;
+		bool head = k != ~0u && (i == 0 || key[i - 1] != k);
+		unsigned long long bal = __ballot(head);
+		int inWave = __popcll(bal & ((1ull << lane) - 1ull));
+		if (lane == 0) waveSums[wave] = __popcll(bal);
+		__syncthreads();
+		int before = runTotal;
+		for (int w = 0; w < wave; w++) before += waveSums[w];
+		int r = before + inWave + (head ? 0 : -1);
+		if (i < np2) rank[i] = (unsigned)r;
+		if (head) cellBodies[2 * (size_t)off + r] = (int)k;
+		__syncthreads();
+		if (t == 0)
+			for (int w = 0; w < 4; w++) runTotal += waveSums[w];
+		__syncthreads();
+	}
+	int total = runTotal;
+	if (t == 0) cellBodyCount[cell] = total;
+	/* slots: first occurrence of the body in the sorted keys -> its rank */
+	for (int i = t; i < m; i += blockDim.x)
+	{
+		const b3hContactConstraint4& c = cs[off + (i >> 1)];
+		unsigned body = (i & 1) ? c.m_bodyB : c.m_bodyA;
+		int slot = -1;
+		if (bodies[body].m_invMass != 0.f)
+		{
+			int lo = 0, hi = np2;
+			while (lo < hi)
+			{
+				int mid = (lo + hi) >> 1;
+				if (key[mid] < body)
+					lo = mid + 1;
+				else
+					hi = mid;
+			}
+			slot = (int)rank[lo];
+		}
+		cslot[2 * (size_t)(off + (i >> 1)) + (i & 1)] = slot;
+	}
+}
+
+/* ---------------- PGS: per-step preparation ----------------
+ * During the solve only velocities and impulses change, so every vector of a
+ * row that depends on positions, masses and inertias alone is computed once per
+ * step with exactly the operations the sweep would use (solveContact.cl:259-303,
+ * solveFriction.cl:288-408):
+ *   normal record  lin, P0 = imA*lin, P1 = imB*(-lin), per point (ang0, ang1,
+ *                  Q0 = iiA*ang0, Q1 = iiB*ang1), jacCoeffInv, b      21 float4
+ *   friction rec.  per direction (lin, ang0, ang1, Q0, Q1, P0, P1), n, (fJac0,
+ *                  fJac1, twist-damping flag)                         16 float4
+ * The sweep then evaluates relVel = dot3(lin,vA)+dot3(ang0,wA)+dot3(-lin,vB)+dot3(ang1,wB)
+ * and v += P*rl, w += Q*rl: the same roundings as recomputing them. */
+#define PGS_NREC 20
+#define PGS_FREC 13
+#define PGS_GATHER 8
+#define PGS_THREADS 256
+#define PGS_NSLOT (PGS_NREC + 1) /* record + applied normal impulses */
+#define PGS_FSLOT (PGS_FREC + 2) /* record + normal impulses + friction impulses */
+__device__ __forceinline__ float ibits(int v) { return __builtin_bit_cast(float, v); }
+__device__ __forceinline__ int fbits(float v) { return __builtin_bit_cast(int, v); }
+__global__ __launch_bounds__(256) void k_pgs_prep(const int* __restrict__ nPtr, const b3hContactConstraint4* __restrict__ cs,
+												  const int* __restrict__ cslot, const b3hRigidBodyData* __restrict__ bodies,
+												  const b3hInertiaData* __restrict__ inertias, F4* __restrict__ recN,
+												  F4* __restrict__ recF)
+{
+	int n = *nPtr;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		const b3hContactConstraint4 c = cs[i];
+		int a = (int)c.m_bodyA, b = (int)c.m_bodyB;
+		F4 posA = bodies[a].m_pos, posB = bodies[b].m_pos;
+		float imA = bodies[a].m_invMass, imB = bodies[b].m_invMass;
+		const b3hMat3x3 iiA = inertias[a].m_invInertiaWorld;
+		const b3hMat3x3 iiB = inertias[b].m_invInertiaWorld;
+		int sA = cslot[2 * (size_t)i], sB = cslot[2 * (size_t)i + 1];
+		F4 meta = f4(imB, ibits(sA), ibits(sB), ibits(i));
+		F4* N = recN + (size_t)i * PGS_NREC;
+		F4 n = neg4(c.m_linear);
+		N[0] = f4(c.m_linear.x, c.m_linear.y, c.m_linear.z, imA);
+		for (int ic = 0; ic < 4; ic++)
+		{
+			F4 r0 = sub4(c.m_worldPos[ic], posA);
+			F4 r1 = sub4(c.m_worldPos[ic], posB);
+			F4 ang0 = neg4(cross3(r0, n));
+			F4 ang1 = cross3(r1, n);
+			N[1 + 4 * ic] = ang0;
+			N[2 + 4 * ic] = ang1;
+			N[3 + 4 * ic] = mtMul1(iiA, ang0);
+			N[4 + 4 * ic] = mtMul1(iiB, ang1);
+		}
+		N[17] = f4(c.m_jacCoeffInv[0], c.m_jacCoeffInv[1], c.m_jacCoeffInv[2], c.m_jacCoeffInv[3]);
+		N[18] = f4(c.m_b[0], c.m_b[1], c.m_b[2], c.m_b[3]);
+		N[19] = meta;
+		F4* Fr = recF + (size_t)i * PGS_FREC;
+		F4 t[2];
+		plane_space(n, &t[0], &t[1]);
+		F4 r0 = sub4(c.m_center, posA), r1 = sub4(c.m_center, posB);
+		for (int k = 0; k < 2; k++)
+		{
+			F4 lin = f4(-t[k].x, -t[k].y, -t[k].z, 0.f);
+			F4 ang0 = neg4(cross3(r0, t[k]));
+			F4 ang1 = cross3(r1, t[k]);
+			Fr[5 * k + 0] = lin;
+			Fr[5 * k + 1] = ang0;
+			Fr[5 * k + 2] = ang1;
+			Fr[5 * k + 3] = mtMul1(iiA, ang0);
+			Fr[5 * k + 4] = mtMul1(iiB, ang1);
+		}
+		Fr[10] = n;
+		F4 ab = nrm3(sub4(posB, posA));
+		F4 ac = nrm3(sub4(c.m_center, posA));
+		bool damp = dot3(ab, ac) > 0.95f || (imA == 0.f || imB == 0.f);
+		Fr[11] = f4(c.m_fJacCoeffInv[0], c.m_fJacCoeffInv[1], damp ? 1.f : 0.f, imA);
+		Fr[12] = meta;
+	}
+}
+
+/* LDS bytes the level sweep needs for a cell of n constraints and nb bodies */
+__host__ __device__ __forceinline__ size_t pgs_level_lds(int nb, int n)
+{
+	return (size_t)(nb + 1) * 36 + (size_t)n * 4 + (size_t)(n + 1) * 4;
+}
+
+/* ---------------- per-cell dependency levels ----------------
+ * Gauss-Seidel results depend only on the order in which each body's updates
+ * happen.  Walking the cell in the reference order (rows of up to 64 aligned to
+ * batch starts; a row shares no dynamic body), a constraint's level is one more
+ * than the last level that touched either of its bodies.  Constraints of one
+ * level share no dynamic body, and every body sees its updates in the reference
+ * order, so solving level by level reproduces the sequential sweep bit for
+ * bit.  Output: per cell the constraints grouped by level (levelOrder, region
+ * [off, off+n)), level starts (levelStart, region [off+cell, off+cell+L+1)) and
+ * L (cellLevels; -1 when the cell is too large for LDS: it then takes the
+ * global walk). */
+__global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cellCounts,
+													 const int* __restrict__ cellOffsets,
+													 const int* __restrict__ batchIdx,
+													 const int* __restrict__ cellBodyCount, const int* __restrict__ cslot,
+													 int bodyCap, int ldsInts, int pgsLdsBytes, int* __restrict__ levelOrder,
+													 int* __restrict__ levelStart, int* __restrict__ cellLevels)
+{
+	extern __shared__ int sm[];
+	__shared__ int sL;
+	int cell = blockIdx.x;
+	int n = cellCounts[cell];
+	int off = cellOffsets[cell];
+	int nb = cellBodyCount[cell];
+	int t = threadIdx.x;
+	if (n == 0)
+	{
+		if (t == 0) cellLevels[cell] = 0;
+		return;
+	}
+	/* last[nb] | lev[n] | cnt[n + 1] */
+	if (nb > bodyCap || nb + 2 * n + 1 > ldsInts || pgs_level_lds(nb, n) > (size_t)pgsLdsBytes)
+	{
+		if (t == 0) cellLevels[cell] = -1;
+		return;
+	}
+	int* last = sm;
+	int* lev = sm + nb;
+	int* cnt = lev + n;
+	for (int k = t; k < nb; k += blockDim.x) last[k] = 0;
+	for (int k = t; k <= n; k += blockDim.x) cnt[k] = 0;
+	__syncthreads();
+	if (t < 64)
+	{
+		int end = off + n;
+		int p = off;
+		int L = 0;
+		while (p < end)
+		{
+			int idx = p + t;
+			int b0 = batchIdx[p];
+			bool active = idx < end && batchIdx[idx] == b0;
+			int c = __popcll(__ballot(active));
+			int lv = 0;
+			int sA = -1, sB = -1;
+			if (active)
+			{
+				sA = cslot[2 * (size_t)idx];
+				sB = cslot[2 * (size_t)idx + 1];
+				int la = sA >= 0 ? last[sA] : 0;
+				int lb = sB >= 0 ? last[sB] : 0;
+				lv = la > lb ? la : lb;
+				lev[idx - off] = lv;
+			}
+			wave_sync_solver();
+			if (active)
+			{
+				if (sA >= 0) last[sA] = lv + 1;
+				if (sB >= 0) last[sB] = lv + 1;
+				L = L > lv + 1 ? L : lv + 1;
+			}
+			wave_sync_solver();
+			p += c;
+		}
+		for (int m = 1; m < 64; m <<= 1)
+		{
+			int o = __shfl_xor(L, m);
+			L = o > L ? o : L;
+		}
+		if (t == 0) sL = L;
+	}
+	__syncthreads();
+	int L = sL;
+	for (int k = t; k < n; k += blockDim.x) atomicAdd(&cnt[lev[k]], 1);
+	__syncthreads();
+	if (t == 0)
+	{
+		int run = 0;
+		for (int l = 0; l < L; l++)
+		{
+			int c = cnt[l];
+			cnt[l] = run;
+			levelStart[off + cell + l] = run;
+			run += c;
+		}
+		levelStart[off + cell + L] = run;
+		cellLevels[cell] = L;
+	}
+	__syncthreads();
+	for (int k = t; k < n; k += blockDim.x)
+	{
+		int pos = atomicAdd(&cnt[lev[k]], 1);
+		levelOrder[off + pos] = off + k;
+	}
+}
+
+/* ---------------- PGS sweep, level by level, velocities in LDS ----------------
+ * Each thread owns positions t, t+T, ... of the cell's level-ordered list.  The
+ * record of a thread's next item is fetched from global memory while its
+ * current item is solved and is parked in the thread's LDS slot, so the only
+ * wait on global memory comes after the arithmetic.  Level boundaries use an
+ * LDS-only barrier (velocities are the only shared state), which does not wait
+ * for outstanding global loads. */
+__device__ __forceinline__ void lds_barrier()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+	__builtin_amdgcn_s_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+__device__ __forceinline__ float rel_vel(F4 l0, F4 a0, F4 a1, const Vel& va, const Vel& vb)
+{
+	F4 l = f4(l0.x, l0.y, l0.z, 0.f);
+	return dot3(l, va.lv) + dot3(a0, va.av) + dot3(neg4(l), vb.lv) + dot3(a1, vb.av);
+}
+/* apply_impulse with P0 = imA*lin, P1 = imB*(-lin) formed as the reference forms them */
+__device__ __forceinline__ void apply_pre(F4 lin, float imA, float imB, F4 Q0, F4 Q1, float rl, Vel& a, Vel& b)
+{
+	F4 linear = f4(lin.x, lin.y, lin.z, 0.f);
+	F4 P0 = smul4(imA, linear);
+	F4 P1 = smul4(imB, neg4(linear));
+	a.lv = add4(a.lv, mul4(P0, rl));
+	a.av = add4(a.av, mul4(Q0, rl));
+	b.lv = add4(b.lv, mul4(P1, rl));
+	b.av = add4(b.av, mul4(Q1, rl));
+}
+
+template <int PHASE>
+struct Slot
+{
+	static constexpr int W = PHASE == 1 ? PGS_FSLOT : PGS_NSLOT;
+	F4 r[W];
+};
+template <int PHASE>
+__device__ __forceinline__ void slot_fetch(Slot<PHASE>& s, int idx, const F4* __restrict__ recN,
+										   const F4* __restrict__ recF, const b3hContactConstraint4* __restrict__ cs)
+{
+	const b3hContactConstraint4& c = cs[idx];
+	if (PHASE == 1)
+	{
+		const F4* F = recF + (size_t)idx * PGS_FREC;
+#pragma unroll
+		for (int k = 0; k < PGS_FREC; k++) s.r[k] = F[k];
+		s.r[PGS_FREC] = f4(c.m_appliedRambdaDt[0], c.m_appliedRambdaDt[1], c.m_appliedRambdaDt[2], c.m_appliedRambdaDt[3]);
+		s.r[PGS_FREC + 1] = f4(c.m_fAppliedRambdaDt[0], c.m_fAppliedRambdaDt[1], 0.f, 0.f);
+	}
+	else
+	{
+		const F4* N = recN + (size_t)idx * PGS_NREC;
+#pragma unroll
+		for (int k = 0; k < PGS_NREC; k++) s.r[k] = N[k];
+		s.r[PGS_NREC] = f4(c.m_appliedRambdaDt[0], c.m_appliedRambdaDt[1], c.m_appliedRambdaDt[2], c.m_appliedRambdaDt[3]);
+	}
+}
+
+template <int PHASE>
+__device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4* sAv,
+										  b3hContactConstraint4* __restrict__ cs, float friction, float& maxD,
+										  float& maxL)
+{
+	struct
+	{
+		const F4* r;
+	} S = {Sr};
+	const F4 zero = f4(0.f, 0.f, 0.f, 0.f);
+	const F4 meta = S.r[PHASE == 1 ? PGS_FREC - 1 : PGS_NREC - 1];
+	float imB = meta.x;
+	int sA = fbits(meta.y), sB = fbits(meta.z), idx = fbits(meta.w);
+	Vel va, vb;
+	va.lv = sA >= 0 ? sLv[sA] : zero;
+	va.av = sA >= 0 ? sAv[sA] : zero;
+	vb.lv = sB >= 0 ? sLv[sB] : zero;
+	vb.av = sB >= 0 ? sAv[sB] : zero;
+	if (PHASE == 1)
+	{
+		F4 fj = S.r[11];
+		if (fj.x == 0) return; /* solveFriction.cl: rows without friction are skipped */
+		float imA = fj.w;
+		F4 lamN = S.r[PGS_FREC];
+		float sum = 0;
+		sum += lamN.x;
+		sum += lamN.y;
+		sum += lamN.z;
+		sum += lamN.w;
+		float maxR = friction * sum;
+		float minR = -maxR;
+		float fl[2] = {S.r[PGS_FREC + 1].x, S.r[PGS_FREC + 1].y};
+		float fjac[2] = {fj.x, fj.y};
+#pragma unroll
+		for (int k = 0; k < 2; k++)
+		{
+			float rl = rel_vel(S.r[5 * k], S.r[5 * k + 1], S.r[5 * k + 2], va, vb);
+			rl *= fjac[k];
+			float prev = fl[k];
+			float upd = prev;
+			upd += rl;
+			upd = cl_max(upd, minR);
+			upd = cl_min(upd, maxR);
+			rl = upd - prev;
+			fl[k] = upd;
+			maxD = fmaxf(maxD, fabsf(rl));
+			maxL = fmaxf(maxL, fabsf(upd));
+			apply_pre(S.r[5 * k], imA, imB, S.r[5 * k + 3], S.r[5 * k + 4], rl, va, vb);
+		}
+		if (fj.z != 0.f)
+		{
+			F4 n = S.r[10];
+			float angNA = dot3(n, va.av);
+			float angNB = dot3(n, vb.av);
+			va.av = sub4(va.av, smul4(angNA * 0.1f, n));
+			vb.av = sub4(vb.av, smul4(angNB * 0.1f, n));
+		}
+		cs[idx].m_fAppliedRambdaDt[0] = fl[0];
+		cs[idx].m_fAppliedRambdaDt[1] = fl[1];
+	}
+	else
+	{
+		F4 lin = S.r[0];
+		float imA = lin.w;
+		F4 lamv = S.r[PGS_NREC];
+		float lam[4] = {lamv.x, lamv.y, lamv.z, lamv.w};
+		F4 jv = S.r[17], bv = S.r[18];
+		float jac[4] = {jv.x, jv.y, jv.z, jv.w};
+		float bb[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+		for (int ic = 0; ic < 4; ic++)
+		{
+			if (jac[ic] != 0.f)
+			{
+				float rl = rel_vel(lin, S.r[1 + 4 * ic], S.r[2 + 4 * ic], va, vb) + bb[ic];
+				rl *= jac[ic];
+				float prev = lam[ic];
+				float upd = prev;
+				upd += rl;
+				upd = cl_max(upd, 0.f);
+				upd = cl_min(upd, FLT_MAX);
+				rl = upd - prev;
+				lam[ic] = upd;
+				maxD = fmaxf(maxD, fabsf(rl));
+				maxL = fmaxf(maxL, fabsf(upd));
+				apply_pre(lin, imA, imB, S.r[3 + 4 * ic], S.r[4 + 4 * ic], rl, va, vb);
+			}
+		}
+#pragma unroll
+		for (int ic = 0; ic < 4; ic++) cs[idx].m_appliedRambdaDt[ic] = lam[ic];
+	}
+	if (sA >= 0)
+	{
+		sLv[sA] = va.lv;
+		sAv[sA] = va.av;
+	}
+	if (sB >= 0)
+	{
+		sLv[sB] = vb.lv;
+		sAv[sB] = vb.av;
+	}
+}
+
+/* warm start (phase 2, once per step): applies the given impulses in level
+ * order, reading the records directly */
+__device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restrict__ recF, F4* sLv, F4* sAv,
+						 b3hContactConstraint4* __restrict__ cs, float friction, const float* __restrict__ warm)
+{
+	const F4 zero = f4(0.f, 0.f, 0.f, 0.f);
+	const F4* N = recN + (size_t)idx * PGS_NREC;
+	const F4* Fr = recF + (size_t)idx * PGS_FREC;
+	F4 meta = N[19];
+	float imB = meta.x;
+	int sA = fbits(meta.y), sB = fbits(meta.z);
+	Vel va, vb;
+	va.lv = sA >= 0 ? sLv[sA] : zero;
+	va.av = sA >= 0 ? sAv[sA] : zero;
+	vb.lv = sB >= 0 ? sLv[sB] : zero;
+	vb.av = sB >= 0 ? sAv[sB] : zero;
+	F4 lin = N[0];
+	float imA = lin.w;
+	float lam[4];
+	for (int ic = 0; ic < 4; ic++) lam[ic] = cs[idx].m_appliedRambdaDt[ic];
+	F4 jv = N[17];
+	float jac[4] = {jv.x, jv.y, jv.z, jv.w};
+	for (int ic = 0; ic < 4; ic++)
+	{
+		if (jac[ic] == 0.f) continue;
+		float rl = cl_max(warm[6 * (size_t)idx + ic], 0.f);
+		lam[ic] = rl;
+		apply_pre(lin, imA, imB, N[3 + 4 * ic], N[4 + 4 * ic], rl, va, vb);
+	}
+	for (int ic = 0; ic < 4; ic++) cs[idx].m_appliedRambdaDt[ic] = lam[ic];
+	F4 fj = Fr[11];
+	if (!(fj.x == 0))
+	{
+		float sum = 0;
+		for (int j = 0; j < 4; j++) sum += lam[j];
+		float maxR = friction * sum;
+		for (int k = 0; k < 2; k++)
+		{
+			float l = cl_min(cl_max(warm[6 * (size_t)idx + 4 + k], -maxR), maxR);
+			cs[idx].m_fAppliedRambdaDt[k] = l;
+			apply_pre(Fr[5 * k], imA, imB, Fr[5 * k + 3], Fr[5 * k + 4], l, va, vb);
+		}
+	}
+	if (sA >= 0)
+	{
+		sLv[sA] = va.lv;
+		sAv[sA] = va.av;
+	}
+	if (sB >= 0)
+	{
+		sLv[sB] = vb.lv;
+		sAv[sB] = vb.av;
+	}
+}
+
+/* the next position of this thread: T on, in the level that holds it */
+__device__ __forceinline__ void next_item(const int* ls, int L, int T, int& l, int& k)
+{
+	k += T;
+	while (l < L && k >= ls[l + 1]) l++;
+}
+
+template <int PHASE>
+__global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __restrict__ bodies,
+														   const b3hInertiaData* __restrict__ inertias,
+														   b3hContactConstraint4* __restrict__ cs,
+														   const int* __restrict__ cellCounts,
+														   const int* __restrict__ cellOffsets,
+														   const int* __restrict__ batchSizes,
+														   const int* __restrict__ batchIdx,
+														   const int* __restrict__ cellBodies,
+														   const int* __restrict__ cellBodyCount,
+														   const int* __restrict__ cellLevels,
+														   const int* __restrict__ levelStart,
+														   const int* __restrict__ levelOrder, const F4* __restrict__ recN,
+														   const F4* __restrict__ recF, int ib, float friction,
+														   const float* __restrict__ warm, float* __restrict__ resid)
+{
+	extern __shared__ F4 smem4[];
 	int cell = cell_of(blockIdx.x, ib);
 	int n = cellCounts[cell];
 	if (n == 0) return;
-	int maxBatch = batchSizes[cell];
 	int start = cellOffsets[cell];
 	int end = start + n;
-	int lane = threadIdx.x;
-	int idx = start + lane;
-	for (int cur = 0; cur < maxBatch; cur++)
+	int t = threadIdx.x;
+	const int T = PGS_THREADS;
+	int L = cellLevels[cell];
+	if (L < 0)
 	{
-		while (idx < end && batchIdx[idx] == cur)
+		/* cell too large for the LDS cache: the reference's global walk on one wave */
+		int maxBatch = batchSizes[cell];
+		int idx = start + t;
+		for (int cur = 0; cur < maxBatch; cur++)
 		{
-			solve_one(cs + idx, bodies, inertias, phase, friction, warm ? warm + 6 * (size_t)idx : nullptr, resid);
-			idx += 64;
+			if (t < 64)
+				while (idx < end && batchIdx[idx] == cur)
+				{
+					solve_one(cs + idx, bodies, inertias, PHASE, friction, warm ? warm + 6 * (size_t)idx : nullptr,
+							  resid);
+					idx += 64;
+				}
+			__syncthreads();
 		}
-		__syncthreads();
+		return;
+	}
+	int nb = cellBodyCount[cell];
+	F4* sLv = smem4;          /* nb + 1 (the last is padding) */
+	F4* sAv = sLv + nb + 1;   /* nb + 1 */
+	int* sList = (int*)(sAv + nb + 1);
+	int* sOrder = sList + nb + 1;
+	const int* list = cellBodies + 2 * (size_t)start;
+	for (int k = t; k < nb; k += T) sList[k] = list[k];
+	__syncthreads();
+	/* gather the velocities, four bodies per thread per round with every load
+	 * issued before any is stored (lanes past the end read body 0, which
+	 * exists, and store nothing) */
+	for (int k0 = t; k0 < nb; k0 += 4 * T)
+	{
+		int k1 = k0 + T, k2 = k0 + 2 * T, k3 = k0 + 3 * T;
+		int b0 = sList[k0];
+		int b1 = k1 < nb ? sList[k1] : 0;
+		int b2 = k2 < nb ? sList[k2] : 0;
+		int b3 = k3 < nb ? sList[k3] : 0;
+		F4 l0 = bodies[b0].m_linVel, a0 = bodies[b0].m_angVel;
+		F4 l1 = bodies[b1].m_linVel, a1 = bodies[b1].m_angVel;
+		F4 l2 = bodies[b2].m_linVel, a2 = bodies[b2].m_angVel;
+		F4 l3 = bodies[b3].m_linVel, a3 = bodies[b3].m_angVel;
+		/* out-of-range results go to the padding slot nb */
+		int s1 = k1 < nb ? k1 : nb, s2 = k2 < nb ? k2 : nb, s3 = k3 < nb ? k3 : nb;
+		sLv[k0] = l0;
+		sAv[k0] = a0;
+		sLv[s1] = l1;
+		sAv[s1] = a1;
+		sLv[s2] = l2;
+		sAv[s2] = a2;
+		sLv[s3] = l3;
+		sAv[s3] = a3;
+	}
+	const int* order = levelOrder + start;
+	for (int k = t; k < n; k += T) sOrder[k] = order[k];
+	int* sLs = sOrder + n; /* level starts, relative to start */
+	for (int k = t; k <= L; k += T) sLs[k] = levelStart[start + cell + k];
+	const int* ls = sLs;
+	float maxD = 0.f, maxL = 0.f;
+	__syncthreads();
+	if (PHASE == 2)
+	{
+		for (int cl = 0; cl < L; cl++)
+		{
+			for (int k = ls[cl] + t; k < ls[cl + 1]; k += T)
+				pgs_warm(sOrder[k], recN, recF, sLv, sAv, cs, friction, warm);
+			lds_barrier();
+		}
+	}
+	else
+	{
+		int l = 0, k = t;
+		while (l < L && k >= ls[l + 1]) l++;
+		Slot<PHASE> cur, nxt;
+		if (l < L) slot_fetch<PHASE>(cur, sOrder[k], recN, recF, cs);
+		for (int cl = 0; cl < L; cl++)
+		{
+			while (l == cl)
+			{
+				int l2 = l, k2 = k;
+				next_item(ls, L, T, l2, k2);
+				if (l2 < L) slot_fetch<PHASE>(nxt, sOrder[k2], recN, recF, cs);
+				pgs_solve<PHASE>(cur.r, sLv, sAv, cs, friction, maxD, maxL);
+				cur = nxt;
+				l = l2;
+				k = k2;
+			}
+			lds_barrier();
+		}
+	}
+	__syncthreads();
+	for (int q = t; q < nb; q += T)
+	{
+		int b = sList[q];
+		bodies[b].m_linVel = sLv[q];
+		bodies[b].m_angVel = sAv[q];
+	}
+	if (resid)
+	{
+		if (maxD > 0.f) atomic_max_pos(resid, maxD);
+		if (maxL > 0.f) atomic_max_pos(resid + 1, maxL);
 	}
 }
 
 static int s_ldsCapBytes = -1;
+static int s_pgsBodyCap = 1 << 30;
+static int s_pgsLdsBytes = 32768;
 
 /* one-time set-up (outside any graph capture): LDS budget of the per-cell
  * batching kernel */
@@ -545,8 +1220,21 @@ int b3h_solver_module_init()
 	if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) v = 65536;
 	if (v > 160 * 1024) v = 160 * 1024;
 	int want = v - 1024;
-	if (hipFuncSetAttribute((const void*)k_batch_cells, hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess)
+	if (hipFuncSetAttribute((const void*)k_batch_cells, hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess ||
+		hipFuncSetAttribute((const void*)k_cell_bodies, hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess)
 		want = 65536 - 1024;
+	if (hipFuncSetAttribute((const void*)k_cell_levels, hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess)
+		want = 65536 - 1024;
+	int pgsBytes = want;
+	if (hipFuncSetAttribute((const void*)k_pgs_level<0>, hipFuncAttributeMaxDynamicSharedMemorySize, pgsBytes) !=
+			hipSuccess ||
+		hipFuncSetAttribute((const void*)k_pgs_level<1>, hipFuncAttributeMaxDynamicSharedMemorySize, pgsBytes) !=
+			hipSuccess ||
+		hipFuncSetAttribute((const void*)k_pgs_level<2>, hipFuncAttributeMaxDynamicSharedMemorySize, pgsBytes) !=
+			hipSuccess)
+		pgsBytes = 32768;
+	s_pgsLdsBytes = pgsBytes;
+	s_pgsBodyCap = 1 << 30;
 	s_ldsCapBytes = want;
 	return B3H_OK;
 }
@@ -562,7 +1250,7 @@ static inline int gblocks(int maxN)
 void b3h_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaData* inertias, const int* numContacts,
 						int maxContacts, const b3hContact4Data* staging, const int* contactPair,
 						const b3hSolverParams& prm, int staticIdx, b3hSolverBuffers& sb, bool writeSortedContacts,
-						bool trackResidual, hipStream_t s)
+						bool trackResidual, hipStream_t s, hipEvent_t evSetupDone, hipEvent_t evNormalDone)
 {
 	(void)numBodies;
 	int ldsCapBytes = s_ldsCapBytes > 0 ? s_ldsCapBytes : 32768;
@@ -580,18 +1268,38 @@ void b3h_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const b3hInerti
 	hipLaunchKernelGGL(k_to_constraints, dim3(gb), dim3(256), 0, s, numContacts, sb.perm, sb.batchIdx, staging,
 					   contactPair, bodies, inertias, prm, sb.constraints,
 					   writeSortedContacts ? sb.sortedContacts : (b3hContact4Data*)nullptr);
+	hipLaunchKernelGGL(k_cell_bodies, dim3(B3H_N_CELLS), dim3(256), (size_t)ldsCapBytes, s, sb.cellCounts,
+					   sb.cellOffsets, sb.constraints, bodies, ldsCapBytes / 4, (unsigned*)sb.sortKeys, sb.cellBodies,
+					   sb.cellBodyCount, sb.cslot);
 	int K = prm.m_numIterations;
+	int cap = (sb.pgsBodyCap >= 0 && sb.pgsBodyCap < s_pgsBodyCap) ? sb.pgsBodyCap : s_pgsBodyCap;
+	size_t pgsLds = (size_t)s_pgsLdsBytes;
+	hipLaunchKernelGGL(k_pgs_prep, dim3(gb), dim3(256), 0, s, numContacts, sb.constraints, sb.cslot, bodies, inertias,
+					   sb.recN, sb.recF);
+	hipLaunchKernelGGL(k_cell_levels, dim3(B3H_N_CELLS), dim3(256), (size_t)ldsCapBytes, s, sb.cellCounts,
+					   sb.cellOffsets, sb.batchIdx, sb.cellBodyCount, sb.cslot, cap, ldsCapBytes / 4, (int)pgsLds,
+					   sb.levelOrder, sb.levelStart, sb.cellLevels);
 	if (trackResidual) b3h_fill_i32((int*)sb.resid, 0, 4 * K, s);
+	const dim3 grid(B3H_N_CELLS / B3H_N_CELL_BATCHES), block(PGS_THREADS);
+	if (evSetupDone) hipEventRecord(evSetupDone, s);
+#define B3H_PGS_ARGS(ib, resid_, warm_)                                                                         \
+	bodies, inertias, sb.constraints, sb.cellCounts, sb.cellOffsets, sb.batchSizes, sb.batchIdx, sb.cellBodies, \
+		sb.cellBodyCount, sb.cellLevels, sb.levelStart, sb.levelOrder, (const F4*)sb.recN, (const F4*)sb.recF, ib, \
+		prm.m_frictionCoeff, warm_, resid_
 	if (sb.warmImpulses)
 		for (int ib = 0; ib < B3H_N_CELL_BATCHES; ib++)
-			hipLaunchKernelGGL(k_pgs, dim3(B3H_N_CELLS / B3H_N_CELL_BATCHES), dim3(64), 0, s, bodies, inertias,
-							   sb.constraints, sb.cellCounts, sb.cellOffsets, sb.batchSizes, sb.batchIdx, ib, 2,
-							   prm.m_frictionCoeff, (const float*)sb.warmImpulses, (float*)nullptr);
+			hipLaunchKernelGGL(k_pgs_level<2>, grid, block, pgsLds, s,
+							   B3H_PGS_ARGS(ib, (float*)nullptr, (const float*)sb.warmImpulses));
 	for (int phase = 0; phase < 2; phase++)
 		for (int it = 0; it < K; it++)
 			for (int ib = 0; ib < B3H_N_CELL_BATCHES; ib++)
-				hipLaunchKernelGGL(k_pgs, dim3(B3H_N_CELLS / B3H_N_CELL_BATCHES), dim3(64), 0, s, bodies, inertias,
-								   sb.constraints, sb.cellCounts, sb.cellOffsets, sb.batchSizes, sb.batchIdx, ib, phase,
-								   prm.m_frictionCoeff, (const float*)nullptr,
-								   trackResidual ? sb.resid + 2 * (phase * K + it) : (float*)nullptr);
+			{
+				if (phase == 1 && it == 0 && ib == 0 && evNormalDone) hipEventRecord(evNormalDone, s);
+				float* r = trackResidual ? sb.resid + 2 * (phase * K + it) : (float*)nullptr;
+				if (phase == 0)
+					hipLaunchKernelGGL(k_pgs_level<0>, grid, block, pgsLds, s, B3H_PGS_ARGS(ib, r, (const float*)nullptr));
+				else
+					hipLaunchKernelGGL(k_pgs_level<1>, grid, block, pgsLds, s, B3H_PGS_ARGS(ib, r, (const float*)nullptr));
+			}
+#undef B3H_PGS_ARGS
 }

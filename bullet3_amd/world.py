@@ -156,7 +156,7 @@ class World:
         call("b3h_pipeline_enable_timing", self.pl, int(on))
 
     def stage_times(self):
-        out = np.zeros(6, np.float32)
+        out = np.zeros(capi.NUM_STAGES, np.float32)
         call("b3h_pipeline_get_stage_times", self.pl, ptr(out))
         return out
 

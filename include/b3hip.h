@@ -180,6 +180,14 @@ int b3h_pipeline_get_solver_params(b3hPipeline* p, b3hSolverParams* prm);
 int b3h_pipeline_get_solver_debug(b3hPipeline* p, b3hContact4Data** sortedContacts,
 								  b3hContactConstraint4** constraints, int** cellCounts, int** cellOffsets,
 								  int** batchSizes, int* numConstraints);
+/* Upper bound on the bodies per solver cell whose velocities the PGS sweep
+ * keeps in LDS (negative: the device's LDS budget; 0: every cell takes the
+ * global read-modify-write walk).  No counterpart in the reference (the
+ * reference always walks global memory); for tests and tuning. */
+int b3h_pipeline_set_pgs_body_cache(b3hPipeline* p, int maxBodiesPerCell);
+/* Dependency levels per solver cell of the last solve (host int[256]; -1: the
+ * cell took the global walk) -- syncs.  No counterpart in the reference. */
+int b3h_pipeline_get_pgs_levels(b3hPipeline* p, int* levels256);
 /* Per-sweep residuals of the last solve (host, 2*iterations floats: normal sweeps
  * then friction sweeps); requires b3h_pipeline_set_track_residual(p, 1). */
 int b3h_pipeline_set_track_residual(b3hPipeline* p, int enable);
@@ -187,10 +195,14 @@ int b3h_pipeline_get_residuals(b3hPipeline* p, float* out, int n);
 /* Initial impulses for the next solve (device float[6*numContacts] in solve
  * order: 4 normal rows then 2 friction rows per constraint); NULL clears. */
 int b3h_pipeline_set_warm_start_impulses(b3hPipeline* p, const float* dev, int numConstraints);
-/* Stage timings of the last step in ms (enable first; adds events). Order:
- * aabb, broadphase, narrowphase, solver setup, pgs, integrate. */
+/* Stage timings of the last eager step in ms (enable first; adds HIP events on
+ * the pipeline stream).  B3H_NUM_STAGES values, in order: world AABBs,
+ * broadphase, narrowphase kernel, narrowphase compaction, solver set-up
+ * (ordering, batching, constraints), normal sweeps, friction sweeps, integrate.
+ * No counterpart in the reference. */
+#define B3H_NUM_STAGES 8
 int b3h_pipeline_enable_timing(b3hPipeline* p, int enable);
-int b3h_pipeline_get_stage_times(b3hPipeline* p, float* ms6);
+int b3h_pipeline_get_stage_times(b3hPipeline* p, float* msOut);
 /* Overflow flags of the last step (pairs, contacts) -- syncs. */
 int b3h_pipeline_get_overflow(b3hPipeline* p, int* pairsOverflow, int* contactsOverflow);
 /* Capture stepSimulation(dt) into a HIP graph and replay it `count` times. */

@@ -134,11 +134,15 @@ def test_narrowphase_bit_exact(scene):
     w.close()
 
 
-def test_solver_bit_exact(scene):
+@pytest.mark.parametrize("cache", [-1, 0, 3])
+def test_solver_bit_exact(scene, cache):
+    """cache: bodies per cell kept in LDS (-1: the LDS budget; 0: every cell on
+    the global walk; 3: mixed, small cells cached, larger ones global)"""
     w = World(scene)
     sp = w.solver_params()
     sp.m_numIterations = 10
     w.set_solver(sp)
+    capi.call("b3h_pipeline_set_pgs_body_cache", w.pl, cache)
     ow = OracleWorld(w)
     run_steps(w, ow, 2)
     capi.call("b3h_pipeline_setup_gpu_aabbs_full", w.pl)
