@@ -154,6 +154,7 @@ struct b3hNarrowPhase
 	b3hContact4Data* dContacts;
 	int* dSepState;
 	b3hFloat4* dSepNormal;
+	b3hFloat4* dMpr; /* per pair: MPR direction, point, (depth, hit) */
 	bool contactsValid;
 	int lanesPerPair;
 };
@@ -174,6 +175,7 @@ static int np_alloc_work(b3hNarrowPhase* np)
 	if ((r = dalloc(&np->dContacts, mc))) return r;
 	if ((r = dalloc(&np->dSepState, mp))) return r;
 	if ((r = dalloc(&np->dSepNormal, mp))) return r;
+	if ((r = dalloc(&np->dMpr, 3 * (size_t)mp))) return r;
 	B3H_CHECK(hipMemset(np->dNumContacts, 0, 2 * sizeof(int)));
 	return B3H_OK;
 }
@@ -204,6 +206,7 @@ extern "C" int b3h_np_create(const b3hConfig* cfg, void* stream, b3hNarrowPhase*
 	np->dContacts = nullptr;
 	np->dSepState = nullptr;
 	np->dSepNormal = nullptr;
+	np->dMpr = nullptr;
 	np->contactsValid = true;
 	np->lanesPerPair = 1;
 	if ((r = make_stream(stream, &np->stream, &np->ownStream)) || (r = np_alloc_work(np)) || (r = b3h_np_module_init()))
@@ -238,6 +241,7 @@ extern "C" int b3h_np_destroy(b3hNarrowPhase* np)
 	dfree(np->dContacts);
 	dfree(np->dSepState);
 	dfree(np->dSepNormal);
+	dfree(np->dMpr);
 	if (np->ownStream) hipStreamDestroy(np->stream);
 	delete np;
 	return B3H_OK;
@@ -523,7 +527,7 @@ static void np_compute_contacts_dev(b3hNarrowPhase* np, b3hInt4* pairs, const in
 {
 	b3h_narrowphase(pairs, dNumPairs, maxPairs, np->dBodies, np_tables(np), np->dStaging, np->dFlags, np->dFlagScan,
 					np->dScanTmp, np->dContactPair, np->cfg.m_maxContactCapacity, np->dNumContacts, np->dSepState,
-					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel);
+					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr);
 	np->contactsValid = false;
 }
 

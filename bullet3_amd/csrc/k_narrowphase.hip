@@ -15,9 +15,8 @@
  *
  * Work split inside a group:
  *   - both hulls' vertices are staged in LDS (per-group slice);
- *   - MPR runs its portal arithmetic redundantly on every lane; each support
- *     query is a lane-parallel dot product over the vertices plus a group
- *     arg-max (first index wins, as the sequential b3MaxDot);
+ *   - MPR runs in its own kernel, one pair per thread (the portal iteration
+ *     is serial; support queries use the sequential b3MaxDot);
  *   - every SAT family spreads its axes over the lanes; a lane projects both
  *     hulls on its axis in vertex order (so lo/hi are the reference's values),
  *     the family stops at the first round that finds a separating axis, and the
@@ -459,84 +458,6 @@ __device__ __forceinline__ int mpr_penetration(SP& S, float* depth, F4* dir, F4*
 }
 
 
-/* MPR support of one pair for a lane group: b3MprConvexSupport (b3MprPenetration.h:139-179)
- * with b3MaxDot's first-maximum rule (shared/b3Float4.h:59-88) */
-template <int L>
-struct GroupSupport
-{
-	const Group<L>& g;
-	const F4* vA;
-	const F4* vB;
-	int nA, nB;
-	F4 posA, posB, qA, qB;
-	__device__ __forceinline__ F4 one(const F4* vs, int nv, F4 pos, F4 q, F4 dirIn) const
-	{
-		F4 dir = f4(dirIn.x, dirIn.y, dirIn.z, 0.f);
-		F4 localDir = qrot(qinv(q), dir);
-		float bv = -INFINITY;
-		int bi = INT_MAX;
-		for (int i = g.lane; i < nv; i += L)
-		{
-			float d = dot3(vs[i], localDir);
-			if (d > -LARGE_FLOAT && (bi == INT_MAX || d > bv))
-			{
-				bv = d;
-				bi = i;
-			}
-		}
-		g.argmax(bv, bi);
-		if (bi == INT_MAX) bi = 0;
-		return xform(vs[bi], pos, q);
-	}
-	__device__ __forceinline__ void support(F4 dir, Sup* s) const
-	{
-		if (L == 64 && nA <= 32 && nB <= 32)
-		{
-			/* both hulls at once: lanes 0-31 take A along dir, lanes 32-63 take B
-			 * along -dir; each half runs the same arithmetic as one() */
-			bool upper = g.lane >= 32;
-			F4 q = upper ? qB : qA;
-			F4 pos = upper ? posB : posA;
-			const F4* vs = upper ? vB : vA;
-			int nv = upper ? nB : nA;
-			F4 dirIn = upper ? mul4(dir, -1.f) : dir;
-			F4 d3 = f4(dirIn.x, dirIn.y, dirIn.z, 0.f);
-			F4 localDir = qrot(qinv(q), d3);
-			int i = g.lane & 31;
-			float bv = -INFINITY;
-			int bi = INT_MAX;
-			if (i < nv)
-			{
-				float d = dot3(vs[i], localDir);
-				if (d > -LARGE_FLOAT)
-				{
-					bv = d;
-					bi = i;
-				}
-			}
-			for (int m = 1; m < 32; m <<= 1)
-			{
-				float ov = __shfl_xor(bv, m);
-				int oi = __shfl_xor(bi, m);
-				if (ov > bv || (ov == bv && oi < bi))
-				{
-					bv = ov;
-					bi = oi;
-				}
-			}
-			if (bi == INT_MAX) bi = 0;
-			F4 w = xform(vs[bi], pos, q);
-			s->v1 = g.bcast(w, g.base);
-			s->v2 = g.bcast(w, g.base + 32);
-			s->v = sub4(s->v1, s->v2);
-			return;
-		}
-		s->v1 = one(vA, nA, posA, qA, dir);
-		s->v2 = one(vB, nB, posB, qB, mul4(dir, -1.f));
-		s->v = sub4(s->v1, s->v2);
-	}
-};
-
 /* ---------------- SAT ---------------- */
 __device__ __forceinline__ void project(const F4* vs, int nv, F4 pos, F4 orn, F4 dir, float* mn, float* mx)
 {
@@ -837,6 +758,80 @@ __device__ int extract_manifold(const Group<L>& g, const F4* p, int n, F4 nearNo
 	return 4;
 }
 
+/* MPR support of one pair for one thread: b3MprConvexSupport with the
+ * sequential b3MaxDot (first maximum wins), vertices read from the (L1/L2
+ * resident) hull table */
+struct ThreadSupport
+{
+	const F4* vA;
+	const F4* vB;
+	int nA, nB;
+	F4 posA, posB, qA, qB;
+	__device__ __forceinline__ F4 one(const F4* vs, int nv, F4 pos, F4 q, F4 dirIn) const
+	{
+		F4 dir = f4(dirIn.x, dirIn.y, dirIn.z, 0.f);
+		F4 localDir = qrot(qinv(q), dir);
+		float best = -LARGE_FLOAT;
+		int idx = -1;
+		for (int i = 0; i < nv; i++)
+		{
+			float d = dot3(vs[i], localDir);
+			if (d > best)
+			{
+				best = d;
+				idx = i;
+			}
+		}
+		if (idx < 0) idx = 0;
+		return xform(vs[idx], pos, q);
+	}
+	__device__ __forceinline__ void support(F4 dir, Sup* s) const
+	{
+		s->v1 = one(vA, nA, posA, qA, dir);
+		s->v2 = one(vB, nB, posB, qB, mul4(dir, -1.f));
+		s->v = sub4(s->v1, s->v2);
+	}
+};
+
+__device__ __forceinline__ bool pair_is_hull_hull(const b3hRigidBodyData& A, const b3hRigidBodyData& B,
+												  const b3hShapeTables& sh)
+{
+	return !(A.m_invMass == 0 && B.m_invMass == 0) &&
+		   sh.collidables[A.m_collidableIdx].m_shapeType == B3H_SHAPE_CONVEX_HULL &&
+		   sh.collidables[B.m_collidableIdx].m_shapeType == B3H_SHAPE_CONVEX_HULL;
+}
+
+/* MPR penetration of every pair, one pair per thread (mprPenetrationKernel,
+ * mpr.cl:14-90).  The portal iteration is inherently serial, so a thread per
+ * pair keeps all lanes busy; the result (direction, point, depth, hit) is
+ * picked up by the SAT/clipping kernel. */
+__global__ __launch_bounds__(256) void k_np_mpr(const b3hInt4* __restrict__ pairs, const int* __restrict__ numPairsPtr,
+												const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh,
+												F4* __restrict__ mprOut)
+{
+	int numPairs = *numPairsPtr;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < numPairs; i += gridDim.x * blockDim.x)
+	{
+		b3hInt4 pr = pairs[i];
+		const b3hRigidBodyData A = bodies[pr.x];
+		const b3hRigidBodyData B = bodies[pr.y];
+		F4 dir = f4(0, 0, 0, 0), pos = f4(0, 0, 0, 0);
+		float depth = 0.f;
+		bool hit = false;
+		if (pair_is_hull_hull(A, B, sh))
+		{
+			const b3hConvexPolyhedronData& ha = sh.convexes[sh.collidables[A.m_collidableIdx].m_shapeIndex];
+			const b3hConvexPolyhedronData& hb = sh.convexes[sh.collidables[B.m_collidableIdx].m_shapeIndex];
+			ThreadSupport S = {sh.vertices + ha.m_vertexOffset, sh.vertices + hb.m_vertexOffset, ha.m_numVertices,
+							   hb.m_numVertices, A.m_pos, B.m_pos, A.m_quat, B.m_quat};
+			hit = mpr_penetration(S, &depth, &dir, &pos) == 0;
+		}
+		mprOut[3 * (size_t)i] = dir;
+		mprOut[3 * (size_t)i + 1] = pos;
+		mprOut[3 * (size_t)i + 2] = f4(depth, hit ? 1.f : 0.f, 0.f, 0.f);
+	}
+}
+
 /* ---------------- the fused kernel ---------------- */
 template <int L>
 __global__ __launch_bounds__(NP_THREADS) void k_narrowphase(const b3hInt4* __restrict__ pairs,
@@ -844,7 +839,7 @@ __global__ __launch_bounds__(NP_THREADS) void k_narrowphase(const b3hInt4* __res
 															 const b3hRigidBodyData* __restrict__ bodies,
 															 b3hShapeTables sh, b3hContact4Data* __restrict__ staging,
 															 int* __restrict__ flags, int* __restrict__ sepStateOut,
-															 b3hFloat4* __restrict__ sepOut)
+															 b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn)
 {
 	constexpr int G = NP_THREADS / L;
 	__shared__ F4 sVerts[G][2][NP_VCAP];
@@ -858,14 +853,13 @@ __global__ __launch_bounds__(NP_THREADS) void k_narrowphase(const b3hInt4* __res
 		int a = pr.x, b = pr.y;
 		const b3hRigidBodyData A = bodies[a];
 		const b3hRigidBodyData B = bodies[b];
-		bool hull = !(A.m_invMass == 0 && B.m_invMass == 0) &&
-					sh.collidables[A.m_collidableIdx].m_shapeType == B3H_SHAPE_CONVEX_HULL &&
-					sh.collidables[B.m_collidableIdx].m_shapeType == B3H_SHAPE_CONVEX_HULL;
+		bool hull = pair_is_hull_hull(A, B, sh);
 		int state = 0;
 		F4 sep = f4(0, 0, 0, 0);
-		bool mprHit = false;
-		float mprDepth = 0.f;
-		F4 mprDir = f4(0, 0, 0, 0), mprPos = f4(0, 0, 0, 0);
+		F4 mprDir = mprIn[3 * (size_t)i], mprPos = mprIn[3 * (size_t)i + 1];
+		F4 mprDH = mprIn[3 * (size_t)i + 2];
+		float mprDepth = mprDH.x;
+		bool mprHit = mprDH.y != 0.f;
 		int nReduced = 0;
 		int idx[4] = {0, 0, 0, 0};
 		F4* pts = sClip[gi][2];
@@ -891,9 +885,6 @@ __global__ __launch_bounds__(NP_THREADS) void k_narrowphase(const b3hInt4* __res
 			P.posB = B.m_pos;
 			P.ornA = A.m_quat;
 			P.ornB = B.m_quat;
-			/* MPR */
-			GroupSupport<L> S = {g, P.vA, P.vB, P.nA, P.nB, A.m_pos, B.m_pos, A.m_quat, B.m_quat};
-			mprHit = mpr_penetration(S, &mprDepth, &mprDir, &mprPos) == 0;
 			if (mprHit) sep = neg4(mprDir);
 			/* SAT (findSeparatingAxis* kernels) */
 			PairGeom Q = P;
@@ -1026,15 +1017,21 @@ int b3h_np_module_init()
 void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3hRigidBodyData* bodies,
 					 const b3hShapeTables& sh, b3hContact4Data* staging, int* flags, int* flagScan, unsigned* scanTmp,
 					 int* contactPair, int maxContacts, int* numContacts, int* sepState, b3hFloat4* sepNormal,
-					 int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel)
+					 int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel, b3hFloat4* mprScratch)
 {
+	{
+		int mb = (maxPairs + 255) / 256;
+		if (mb > 16384) mb = 16384;
+		if (mb < 1) mb = 1;
+		hipLaunchKernelGGL(k_np_mpr, dim3(mb), dim3(256), 0, s, pairs, numPairs, bodies, sh, (F4*)mprScratch);
+	}
 	(void)pairsPerGroup;
 	const int groups = NP_THREADS / 64;
 	int blocks = (maxPairs + groups - 1) / groups;
 	if (blocks > 16384) blocks = 16384;
 	if (blocks < 1) blocks = 1;
 	hipLaunchKernelGGL(k_narrowphase<64>, dim3(blocks), dim3(NP_THREADS), 0, s, pairs, numPairs, bodies, sh, staging,
-					   flags, sepState, sepNormal);
+					   flags, sepState, sepNormal, (const F4*)mprScratch);
 	if (evAfterKernel) hipEventRecord(evAfterKernel, s);
 	b3h_exclusive_scan(flags, flagScan, numPairs, maxPairs, scanTmp, s);
 	int cblocks = (maxPairs + 255) / 256;
