@@ -86,6 +86,7 @@ _SIGS = {
     "b3h_bp_get_total_found": ([_vp, _pi], _i),
     "b3h_bp_get_sap_axis": ([_vp, _pi], _i),
     "b3h_bp_get_proxy_maps": ([_vp, _vp, _vp, _vp], _i),
+    "b3h_bp_create_proxies": ([_vp, _i, _vp, _vp, _vp], _i),
     "b3h_bp_get_grid_sort": ([_vp, _pvp, _pvp, _pi], _i),
     "b3h_pipeline_create": ([_vp, _vp, ctypes.POINTER(Config), _vp, _pvp], _i),
     "b3h_pipeline_destroy": ([_vp], _i),

@@ -658,6 +658,7 @@ struct b3hBroadphase
 	int* dPairCount; /* [count, total] */
 	b3hBroadphaseScratch sc;
 	int scCapSmall;
+	int scCapLarge;
 	int scCells;
 };
 
@@ -675,6 +676,13 @@ static void bp_free_scratch(b3hBroadphase* bp)
 	dfree(bp->sc.sortedAabbs);
 	dfree(bp->sc.sapPartials);
 	dfree(bp->sc.axis);
+	dfree(bp->sc.lkeys);
+	dfree(bp->sc.lvals);
+	dfree(bp->sc.ltmpKeys);
+	dfree(bp->sc.ltmpVals);
+	dfree(bp->sc.lhist);
+	dfree(bp->sc.lmaxExt);
+	dfree(bp->sc.lminx);
 }
 
 extern "C" int b3h_bp_create(int kind, const b3hConfig* cfg, void* stream, b3hBroadphase** out)
@@ -688,6 +696,7 @@ extern "C" int b3h_bp_create(int kind, const b3hConfig* cfg, void* stream, b3hBr
 	bp->cfg = *cfg;
 	memset(&bp->sc, 0, sizeof(bp->sc));
 	bp->scCapSmall = 0;
+	bp->scCapLarge = 0;
 	bp->scCells = 0;
 	bp->dAll = nullptr;
 	bp->dSmall = bp->dLarge = nullptr;
@@ -777,6 +786,23 @@ extern "C" int b3h_bp_create_proxy(b3hBroadphase* bp, const float* mn, const flo
 	(void)m;
 	return bp_proxy(bp, mn, mx, user, false);
 }
+extern "C" int b3h_bp_create_proxies(b3hBroadphase* bp, int n, const float* mn, const float* mx, const int* user)
+{
+	CHECK_ARG(bp && n >= 0 && (n == 0 || (mn && mx)));
+	if ((long long)bp->all.size() + n > bp->cfg.m_maxConvexBodies)
+	{
+		b3h_set_error("createProxies: %d more proxies exceed capacity %d", n, bp->cfg.m_maxConvexBodies);
+		return B3H_ERR_CAPACITY;
+	}
+	bp->all.reserve(bp->all.size() + n);
+	bp->smallMap.reserve(bp->smallMap.size() + n);
+	for (int i = 0; i < n; i++)
+	{
+		int r = bp_proxy(bp, mn + 3 * (size_t)i, mx + 3 * (size_t)i, user ? user[i] : i, false);
+		if (r) return r;
+	}
+	return B3H_OK;
+}
 extern "C" int b3h_bp_create_large_proxy(b3hBroadphase* bp, const float* mn, const float* mx, int user, int g, int m)
 {
 	(void)g;
@@ -787,10 +813,17 @@ extern "C" int b3h_bp_create_large_proxy(b3hBroadphase* bp, const float* mn, con
 static int bp_ensure_scratch(b3hBroadphase* bp)
 {
 	int ns = (int)bp->smallMap.size();
-	if (ns <= bp->scCapSmall && bp->scCells == bp->numCells && bp->sc.keys) return B3H_OK;
+	int nl = (int)bp->largeMap.size();
+	if (ns <= bp->scCapSmall && nl <= bp->scCapLarge && bp->scCells == bp->numCells && bp->sc.keys) return B3H_OK;
 	bp_free_scratch(bp);
 	int cap = ns + 64;
+	int capL = nl + 64;
 	int r;
+	if ((r = dalloc(&bp->sc.lkeys, capL)) || (r = dalloc(&bp->sc.lvals, capL)) || (r = dalloc(&bp->sc.ltmpKeys, capL)) ||
+		(r = dalloc(&bp->sc.ltmpVals, capL)) || (r = dalloc(&bp->sc.lhist, b3h_radix_hist_size(capL))) ||
+		(r = dalloc(&bp->sc.lmaxExt, 1)) || (r = dalloc(&bp->sc.lminx, capL)))
+		return r;
+	bp->scCapLarge = nl;
 	if ((r = dalloc(&bp->sc.keys, cap)) || (r = dalloc(&bp->sc.vals, cap)) || (r = dalloc(&bp->sc.tmpKeys, cap)) ||
 		(r = dalloc(&bp->sc.tmpVals, cap)) || (r = dalloc(&bp->sc.hist, b3h_radix_hist_size(cap))) ||
 		(r = dalloc(&bp->sc.cellStart, bp->numCells)) || (r = dalloc(&bp->sc.counts, 2 * cap + 1)) ||

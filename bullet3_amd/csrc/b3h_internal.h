@@ -54,7 +54,12 @@ struct b3hBroadphaseScratch
 	b3hAabb* sortedAabbs;
 	double* sapPartials;
 	int* axis;
+	/* large proxies sorted by min.x (used when there are many of them) */
+	unsigned *lkeys, *lvals, *ltmpKeys, *ltmpVals, *lhist;
+	unsigned* lmaxExt; /* float bits of the largest x-extent */
+	float* lminx;      /* min.x in sorted order */
 };
+#define B3H_LARGE_BRUTE_MAX 32
 void b3h_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
 					const b3hGridParams& gp, int numCells, b3hInt4* pairs, int maxPairs, int* pairCount,
 					int* pairTotal, b3hBroadphaseScratch& sc, hipStream_t s);

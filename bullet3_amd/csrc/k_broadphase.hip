@@ -63,6 +63,147 @@ __global__ __launch_bounds__(256) void k_large_small(int nSmall, int nLarge, con
 	if (!WRITE) counts[i] = c;
 }
 
+/* Many large proxies (batched scenes: one ground per scene): sort them by
+ * min.x once per step; a small proxy then tests only the large proxies whose
+ * min.x lies in [small.min.x - 2*maxExtent - 1, small.max.x] (every overlapping
+ * large proxy is in that window; the exact overlap test decides), and emits
+ * its matches in ascending large-proxy order, as the all-pairs loop does. */
+__global__ __launch_bounds__(256) void k_large_keys(int nLarge, const b3hAabb* __restrict__ all,
+													const int* __restrict__ largeMap, unsigned* __restrict__ keys,
+													unsigned* __restrict__ vals, unsigned* __restrict__ maxExt)
+{
+	int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= nLarge) return;
+	b3hAabb l = all[largeMap[j]];
+	unsigned u = f2u(l.m_min.x);
+	unsigned mask = (unsigned)(-(int)(u >> 31)) | 0x80000000u;
+	keys[j] = u ^ mask;
+	vals[j] = (unsigned)j;
+	float e = l.m_max.x - l.m_min.x;
+	if (e > 0.f) atomicMax(maxExt, f2u(e));
+}
+__global__ __launch_bounds__(256) void k_large_minx(int nLarge, const b3hAabb* __restrict__ all,
+													const int* __restrict__ largeMap, const unsigned* __restrict__ vals,
+													float* __restrict__ minx)
+{
+	int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= nLarge) return;
+	minx[k] = all[largeMap[vals[k]]].m_min.x;
+}
+#define LS_LOCAL 16
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_large_small_sorted(int nSmall, int nLarge, const b3hAabb* __restrict__ all,
+															const int* __restrict__ smallMap,
+															const int* __restrict__ largeMap,
+															const unsigned* __restrict__ lvals,
+															const float* __restrict__ lminx,
+															const unsigned* __restrict__ maxExt,
+															int* __restrict__ counts, const int* __restrict__ offsets,
+															b3hInt4* __restrict__ pairs, int maxPairs)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nSmall) return;
+	b3hAabb s = all[smallMap[i]];
+	int hs = bp_handle(s);
+	float ext = __builtin_bit_cast(float, *maxExt);
+	float t = s.m_min.x - 2.f * ext - 1.f;
+	/* first sorted position with min.x >= t */
+	int lo = 0, hi = nLarge;
+	while (lo < hi)
+	{
+		int mid = (lo + hi) >> 1;
+		if (lminx[mid] < t)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	int found[LS_LOCAL];
+	int nf = 0;
+	bool overflow = false;
+	for (int k = lo; k < nLarge && lminx[k] <= s.m_max.x; k++)
+	{
+		int j = (int)lvals[k];
+		b3hAabb l = all[largeMap[j]];
+		if (bp_overlap(l, s))
+		{
+			if (nf < LS_LOCAL)
+				found[nf++] = j;
+			else
+				overflow = true;
+		}
+	}
+	int c = 0;
+	int o = WRITE ? offsets[i] : 0;
+	if (!overflow)
+	{
+		/* ascending large order (insertion sort of a handful) */
+		for (int a = 1; a < nf; a++)
+		{
+			int v = found[a], b = a - 1;
+			while (b >= 0 && found[b] > v)
+			{
+				found[b + 1] = found[b];
+				b--;
+			}
+			found[b + 1] = v;
+		}
+		for (int a = 0; a < nf; a++)
+		{
+			if (WRITE)
+			{
+				int x = bp_handle(all[largeMap[found[a]]]), y = hs;
+				if (x > y)
+				{
+					int tmp = x;
+					x = y;
+					y = tmp;
+				}
+				int at = o + c;
+				if (at < maxPairs) pairs[at] = b3hInt4{x, y, -1, -1};
+			}
+			c++;
+		}
+	}
+	else
+	{
+		/* more matches than the local list holds: the all-pairs loop */
+		for (int j = 0; j < nLarge; j++)
+		{
+			b3hAabb l = all[largeMap[j]];
+			if (bp_overlap(l, s))
+			{
+				if (WRITE)
+				{
+					int x = bp_handle(l), y = hs;
+					if (x > y)
+					{
+						int tmp = x;
+						x = y;
+						y = tmp;
+					}
+					int at = o + c;
+					if (at < maxPairs) pairs[at] = b3hInt4{x, y, -1, -1};
+				}
+				c++;
+			}
+		}
+	}
+	if (!WRITE) counts[i] = c;
+}
+
+static inline int nblk(int n) { return (n + 255) / 256; }
+/* large x small pairs: all-pairs for a few large proxies, the sorted window
+ * for many; prepare() once per step before the counting pass */
+static void large_small_prepare(const b3hAabb* all, const int* largeMap, int nLarge, b3hBroadphaseScratch& sc,
+								hipStream_t s)
+{
+	if (nLarge <= B3H_LARGE_BRUTE_MAX) return;
+	b3h_fill_i32((int*)sc.lmaxExt, 0, 1, s);
+	hipLaunchKernelGGL(k_large_keys, dim3(nblk(nLarge)), dim3(256), 0, s, nLarge, all, largeMap, sc.lkeys, sc.lvals,
+					   sc.lmaxExt);
+	b3h_radix_sort(sc.lkeys, sc.lvals, sc.ltmpKeys, sc.ltmpVals, sc.lhist, nLarge, 32, s);
+	hipLaunchKernelGGL(k_large_minx, dim3(nblk(nLarge)), dim3(256), 0, s, nLarge, all, largeMap, sc.lvals, sc.lminx);
+}
 /* ---------------- grid ---------------- */
 __device__ __forceinline__ void grid_pos(const b3hAabb& a, const b3hGridParams& gp, int g[3])
 {
@@ -100,17 +241,25 @@ __global__ __launch_bounds__(256) void k_cell_start(int n, const unsigned* __res
 	if (i == 0 || keys[i - 1] != k) cellStart[k] = i;
 }
 
+/* AABBs in hash-sorted order, so the neighbour scans read contiguous records */
+__global__ __launch_bounds__(256) void k_grid_gather(int n, const unsigned* __restrict__ vals, const b3hAabb* __restrict__ all,
+													 const int* __restrict__ smallMap, b3hAabb* __restrict__ sorted)
+{
+	int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= n) return;
+	sorted[s] = all[smallMap[vals[s]]];
+}
+
 template <bool WRITE>
-__global__ __launch_bounds__(256) void k_grid_pairs(int n, const unsigned* __restrict__ keys, const unsigned* __restrict__ vals,
-													const b3hAabb* __restrict__ all, const int* __restrict__ smallMap,
+__global__ __launch_bounds__(256) void k_grid_pairs(int n, const unsigned* __restrict__ keys,
+													const b3hAabb* __restrict__ sorted,
 													const int* __restrict__ cellStart, b3hGridParams gp,
 													int* __restrict__ counts, const int* __restrict__ offsets,
 													b3hInt4* __restrict__ pairs, int maxPairs)
 {
 	int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= n) return;
-	int ui = (int)vals[s];
-	b3hAabb a = all[smallMap[ui]];
+	b3hAabb a = sorted[s];
 	int ha = bp_handle(a);
 	int g[3];
 	grid_pos(a, gp, g);
@@ -128,9 +277,8 @@ __global__ __launch_bounds__(256) void k_grid_pairs(int n, const unsigned* __res
 				for (int k = st; k < end; k++)
 				{
 					if ((int)keys[k] != h) break;
-					int uj = (int)vals[k];
-					if (uj == ui) continue;
-					b3hAabb b = all[smallMap[uj]];
+					if (k == s) continue;
+					b3hAabb b = sorted[k];
 					if (!bp_overlap(a, b)) continue;
 					int hb = bp_handle(b);
 					if (ha < hb)
@@ -155,7 +303,19 @@ __global__ void k_pairs_finish(const int* __restrict__ total, int maxPairs, int*
 	*pairCount = t < maxPairs ? t : maxPairs;
 }
 
-static inline int nblk(int n) { return (n + 255) / 256; }
+template <bool WRITE>
+static void large_small_pass(int nSmall, int nLarge, const b3hAabb* all, const int* smallMap, const int* largeMap,
+							 b3hBroadphaseScratch& sc, int* counts, const int* offsets, b3hInt4* pairs, int maxPairs,
+							 hipStream_t s)
+{
+	if (nLarge <= B3H_LARGE_BRUTE_MAX)
+		hipLaunchKernelGGL(k_large_small<WRITE>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, nLarge, all, smallMap,
+						   largeMap, counts, offsets, pairs, maxPairs);
+	else
+		hipLaunchKernelGGL(k_large_small_sorted<WRITE>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, nLarge, all,
+						   smallMap, largeMap, (const unsigned*)sc.lvals, (const float*)sc.lminx,
+						   (const unsigned*)sc.lmaxExt, counts, offsets, pairs, maxPairs);
+}
 
 void b3h_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
 					const b3hGridParams& gp, int numCells, b3hInt4* pairs, int maxPairs, int* pairCount,
@@ -165,25 +325,30 @@ void b3h_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const i
 	int* countsG = sc.counts + nSmall;
 	if (nSmall > 0)
 	{
-		hipLaunchKernelGGL(k_large_small<false>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, nLarge, all, smallMap,
-						   largeMap, countsLS, (const int*)nullptr, (b3hInt4*)nullptr, maxPairs);
+		large_small_prepare(all, largeMap, nLarge, sc, s);
+		large_small_pass<false>(nSmall, nLarge, all, smallMap, largeMap, sc, countsLS, (const int*)nullptr,
+								(b3hInt4*)nullptr, maxPairs, s);
 		hipLaunchKernelGGL(k_grid_hash, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, all, smallMap, gp, sc.keys, sc.vals);
 		int bits = 0;
 		while ((1 << bits) < numCells) bits++;
 		b3h_radix_sort(sc.keys, sc.vals, sc.tmpKeys, sc.tmpVals, sc.hist, nSmall, bits, s);
 		b3h_fill_i32(sc.cellStart, -1, numCells, s);
 		hipLaunchKernelGGL(k_cell_start, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys, sc.cellStart);
-		hipLaunchKernelGGL(k_grid_pairs<false>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys, sc.vals, all,
-						   smallMap, sc.cellStart, gp, countsG, (const int*)nullptr, (b3hInt4*)nullptr, maxPairs);
+		hipLaunchKernelGGL(k_grid_gather, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.vals, all, smallMap,
+						   sc.sortedAabbs);
+		hipLaunchKernelGGL(k_grid_pairs<false>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys,
+						   (const b3hAabb*)sc.sortedAabbs, sc.cellStart, gp, countsG, (const int*)nullptr,
+						   (b3hInt4*)nullptr, maxPairs);
 	}
 	int nTot = 2 * nSmall;
 	b3h_exclusive_scan(sc.counts, sc.offsets, nullptr, nTot, sc.scanTmp, s);
 	if (nSmall > 0)
 	{
-		hipLaunchKernelGGL(k_large_small<true>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, nLarge, all, smallMap,
-						   largeMap, (int*)nullptr, sc.offsets, pairs, maxPairs);
-		hipLaunchKernelGGL(k_grid_pairs<true>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys, sc.vals, all,
-						   smallMap, sc.cellStart, gp, (int*)nullptr, sc.offsets + nSmall, pairs, maxPairs);
+		large_small_pass<true>(nSmall, nLarge, all, smallMap, largeMap, sc, (int*)nullptr, sc.offsets, pairs,
+							   maxPairs, s);
+		hipLaunchKernelGGL(k_grid_pairs<true>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys,
+						   (const b3hAabb*)sc.sortedAabbs, sc.cellStart, gp, (int*)nullptr, sc.offsets + nSmall, pairs,
+						   maxPairs);
 	}
 	hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, sc.offsets + nTot, maxPairs, pairCount, pairTotal);
 }
@@ -303,8 +468,9 @@ void b3h_sap_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const in
 		int np = b3h_sap_num_partials(nSmall);
 		hipLaunchKernelGGL(k_sap_partials, dim3(np), dim3(SAP_BLOCK), 0, s, nSmall, all, smallMap, sc.sapPartials);
 		hipLaunchKernelGGL(k_sap_axis, dim3(1), dim3(1), 0, s, np, sc.sapPartials, nSmall, sc.axis);
-		hipLaunchKernelGGL(k_large_small<false>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, nLarge, all, smallMap,
-						   largeMap, countsLS, (const int*)nullptr, (b3hInt4*)nullptr, maxPairs);
+		large_small_prepare(all, largeMap, nLarge, sc, s);
+		large_small_pass<false>(nSmall, nLarge, all, smallMap, largeMap, sc, countsLS, (const int*)nullptr,
+								(b3hInt4*)nullptr, maxPairs, s);
 		hipLaunchKernelGGL(k_sap_keys, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, all, smallMap, sc.axis, sc.keys, sc.vals);
 		b3h_radix_sort(sc.keys, sc.vals, sc.tmpKeys, sc.tmpVals, sc.hist, nSmall, 32, s);
 		hipLaunchKernelGGL(k_sap_scatter, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, all, smallMap, sc.vals, sc.sortedAabbs);
@@ -315,8 +481,8 @@ void b3h_sap_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const in
 	b3h_exclusive_scan(sc.counts, sc.offsets, nullptr, nTot, sc.scanTmp, s);
 	if (nSmall > 0)
 	{
-		hipLaunchKernelGGL(k_large_small<true>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, nLarge, all, smallMap,
-						   largeMap, (int*)nullptr, sc.offsets, pairs, maxPairs);
+		large_small_pass<true>(nSmall, nLarge, all, smallMap, largeMap, sc, (int*)nullptr, sc.offsets, pairs,
+							   maxPairs, s);
 		hipLaunchKernelGGL(k_sap_sweep<true>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.sortedAabbs, sc.axis,
 						   (int*)nullptr, sc.offsets + nSmall, pairs, maxPairs);
 	}

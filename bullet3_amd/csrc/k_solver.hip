@@ -50,8 +50,13 @@ __global__ __launch_bounds__(256) void k_solver_keys(const int* __restrict__ nPt
 													 const b3hRigidBodyData* __restrict__ bodies, int staticIdx,
 													 float scale, int* __restrict__ cellOf, int* __restrict__ cellCounts)
 {
+	/* per-block histogram in LDS, one global add per (block, cell) */
+	__shared__ int hist[B3H_N_CELLS];
+	int t = threadIdx.x;
+	hist[t] = 0;
+	__syncthreads();
 	int n = *nPtr;
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	for (int i = blockIdx.x * blockDim.x + t; i < n; i += gridDim.x * blockDim.x)
 	{
 		const b3hContact4Data& c = staging[contactPair[i]];
 		int aS = c.m_bodyAPtrAndSignBit, bS = c.m_bodyBPtrAndSignBit;
@@ -59,8 +64,10 @@ __global__ __launch_bounds__(256) void k_solver_keys(const int* __restrict__ nPt
 		int idx = aStatic ? iabs(bS) : iabs(aS);
 		int cell = cell_index(bodies[idx].m_pos, scale);
 		cellOf[i] = cell;
-		atomicAdd(&cellCounts[cell], 1);
+		atomicAdd(&hist[cell], 1);
 	}
+	__syncthreads();
+	if (hist[t]) atomicAdd(&cellCounts[t], hist[t]);
 }
 
 __global__ __launch_bounds__(B3H_N_CELLS) void k_cell_scan(const int* __restrict__ counts, int* __restrict__ offsets,
@@ -87,11 +94,30 @@ __global__ __launch_bounds__(B3H_N_CELLS) void k_cell_scan(const int* __restrict
 __global__ __launch_bounds__(256) void k_cell_scatter(const int* __restrict__ nPtr, const int* __restrict__ cellOf,
 													  int* __restrict__ cursor, int* __restrict__ cellSlots)
 {
+	/* each block reserves one range per cell (one global add per (block, cell))
+	 * and places its contacts inside it by LDS ranks; the order inside a cell is
+	 * arbitrary here and fixed by the per-cell sort that follows */
+	__shared__ int cnt[B3H_N_CELLS];
+	__shared__ int base[B3H_N_CELLS];
+	int t = threadIdx.x;
 	int n = *nPtr;
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	const int stride = gridDim.x * blockDim.x;
+	for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride)
 	{
-		int pos = atomicAdd(&cursor[cellOf[i]], 1);
-		cellSlots[pos] = i;
+		cnt[t] = 0;
+		__syncthreads();
+		int i = i0 + t;
+		int cell = -1, rank = 0;
+		if (i < n)
+		{
+			cell = cellOf[i];
+			rank = atomicAdd(&cnt[cell], 1);
+		}
+		__syncthreads();
+		if (cnt[t]) base[t] = atomicAdd(&cursor[t], cnt[t]);
+		__syncthreads();
+		if (cell >= 0) cellSlots[base[cell] + rank] = i;
+		__syncthreads();
 	}
 }
 
@@ -229,8 +255,28 @@ __global__ __launch_bounds__(256) void k_batch_cells(const int* __restrict__ cel
 						cand &= later & ~__ballot(hit);
 					}
 				}
-				/* move the accepted contacts to the front, in order */
-				if (lane == 0)
+				/* move the accepted contacts to the front, in order.  When at least
+				 * as many rejected contacts lie before the chunk as the chunk
+				 * accepts, the swaps (accepted m <-> numValid + m) are disjoint and
+				 * run in parallel; otherwise one lane performs them in sequence. */
+				int kAcc = __popcll(acc);
+				if (c0 - numValid >= kAcc)
+				{
+					if ((acc >> lane) & 1ull)
+					{
+						int m = __popcll(acc & ((1ull << lane) - 1ull));
+						int dst = numValid + m;
+						int ii = c0 + lane;
+						unsigned long long pj = pay[ii];
+						unsigned long long kd = key[dst], pd = pay[dst];
+						key[ii] = kd;
+						pay[ii] = pd;
+						key[dst] = kk;
+						pay[dst] = (pj & 0xffffffffull) | ((unsigned long long)(unsigned)batch << 32);
+					}
+					numValid += kAcc;
+				}
+				else if (lane == 0)
 				{
 					unsigned long long m = acc;
 					while (m)
@@ -1258,10 +1304,11 @@ void b3h_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const b3hInerti
 	while ((ldsCap * 2) * 16 <= ldsCapBytes) ldsCap *= 2;
 	b3h_fill_i32(sb.cellCounts, 0, B3H_N_CELLS, s);
 	int gb = gblocks(maxContacts);
-	hipLaunchKernelGGL(k_solver_keys, dim3(gb), dim3(256), 0, s, numContacts, staging, contactPair, bodies, staticIdx,
+	int kb = gb < B3H_N_CELLS ? gb : B3H_N_CELLS;
+	hipLaunchKernelGGL(k_solver_keys, dim3(kb), dim3(256), 0, s, numContacts, staging, contactPair, bodies, staticIdx,
 					   1.f / prm.m_batchCellSize, sb.cellOf, sb.cellCounts);
 	hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(B3H_N_CELLS), 0, s, sb.cellCounts, sb.cellOffsets, sb.cellCursor);
-	hipLaunchKernelGGL(k_cell_scatter, dim3(gb), dim3(256), 0, s, numContacts, sb.cellOf, sb.cellCursor, sb.cellSlots);
+	hipLaunchKernelGGL(k_cell_scatter, dim3(kb), dim3(256), 0, s, numContacts, sb.cellOf, sb.cellCursor, sb.cellSlots);
 	hipLaunchKernelGGL(k_batch_cells, dim3(B3H_N_CELLS), dim3(256), (size_t)ldsCap * 16, s, sb.cellCounts, sb.cellOffsets,
 					   sb.cellSlots, staging, contactPair, staticIdx, ldsCap, sb.sortKeys, sb.perm, sb.batchIdx,
 					   sb.batchSizes);

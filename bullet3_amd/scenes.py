@@ -147,3 +147,52 @@ def aabb_cloud(n=1 << 20, mean_overlap=8.0, hmin=0.05, hmax=1.0, seed=3, side=No
     mn = (c - h[:, None]).astype(np.float32)
     mx = (c + h[:, None]).astype(np.float32)
     return mn, mx, float(side)
+
+
+def batched_scenes(n_scenes=4096, min_bodies=16, max_bodies=512, spacing=24.0, n_hull_shapes=32, seed=4):
+    """Many independent scenes packed into one world: scene k sits on its own
+    static ground (16 x 1 x 16 m) at a lattice position `spacing` m from its
+    neighbours, so no two scenes interact.  Body counts are log-uniform in
+    [min_bodies, max_bodies]; even scenes are piles (boxes and 8-24-point hulls,
+    random orientations, lattice spacing 1.2 m), odd scenes are box stacks
+    (columns, spacing 1.02 m).  Returns one Scene; meta["scene_of_body"] maps
+    every body to its scene."""
+    rng = np.random.default_rng(seed)
+    shapes = [box_points(8.0, 0.5, 8.0), box_points(0.5, 0.5, 0.5)]
+    for k in range(n_hull_shapes):
+        shapes.append(sphere_points(rng, int(rng.integers(8, 25))))
+    side = int(math.ceil(math.sqrt(n_scenes)))
+    counts = np.exp(rng.uniform(math.log(min_bodies), math.log(max_bodies), n_scenes)).round().astype(int)
+    mass, pos, orn, shape, owner = [], [], [], [], []
+    for k in range(n_scenes):
+        ox = (k % side - (side - 1) / 2.0) * spacing
+        oz = (k // side - (side - 1) / 2.0) * spacing
+        mass.append(0.0)
+        pos.append((ox, -0.5, oz))
+        orn.append((0.0, 0.0, 0.0, 1.0))
+        shape.append(0)
+        owner.append(k)
+        n = int(counts[k])
+        if k % 2 == 0:
+            w = int(min(8, math.ceil(math.sqrt(n))))
+            q = random_quats(rng, n)
+            for i in range(n):
+                x, z, y = i % w, (i // w) % w, i // (w * w)
+                pos.append((ox + (x - (w - 1) / 2.0) * 1.2, 1.0 + y * 1.2, oz + (z - (w - 1) / 2.0) * 1.2))
+                orn.append(tuple(q[i]))
+                shape.append(1 if rng.random() < 0.4 else 2 + int(rng.integers(0, n_hull_shapes)))
+                mass.append(float(np.exp(rng.uniform(math.log(0.5), math.log(4.0)))))
+                owner.append(k)
+        else:
+            w = int(min(6, math.ceil(math.sqrt(n / 8.0))))
+            for i in range(n):
+                x, z, y = i % w, (i // w) % w, i // (w * w)
+                pos.append((ox + (x - (w - 1) / 2.0) * 1.2, 0.5 + y * 1.02, oz + (z - (w - 1) / 2.0) * 1.2))
+                orn.append((0.0, 0.0, 0.0, 1.0))
+                shape.append(1)
+                mass.append(1.0)
+                owner.append(k)
+    return Scene(name="batched_%d" % n_scenes, shapes=shapes, mass=np.array(mass, np.float32),
+                 pos=np.array(pos, np.float32), orn=np.array(orn, np.float32), shape=np.array(shape, np.int32),
+                 cell_size=2.0 * math.sqrt(3.0), meta={"scene_of_body": np.array(owner, np.int32),
+                                                       "bodies_per_scene": counts})

@@ -122,6 +122,10 @@ int b3h_bp_create_proxy(b3hBroadphase* bp, const float* aabbMin3, const float* a
 int b3h_bp_create_large_proxy(b3hBroadphase* bp, const float* aabbMin3, const float* aabbMax3, int userPtr,
 							  int group, int mask);
 /* writeAabbsToGpu, b3GpuBroadphaseInterface.h:27 */
+/* createProxy for n AABBs at once (host arrays of n*3 floats; userPtr[i] or i
+ * when userPtrs is NULL).  No counterpart in the reference, which registers
+ * one proxy per call (b3GpuBroadphaseInterface.h:23). */
+int b3h_bp_create_proxies(b3hBroadphase* bp, int n, const float* aabbMins3, const float* aabbMaxs3, const int* userPtrs);
 int b3h_bp_write_aabbs_to_gpu(b3hBroadphase* bp);
 /* calculateOverlappingPairs(maxPairs), b3GpuBroadphaseInterface.h:24 (no host sync) */
 int b3h_bp_calculate_overlapping_pairs(b3hBroadphase* bp, int maxPairs);

@@ -1,0 +1,92 @@
+"""Broadphase stress: 1,048,576 cubic AABBs, half-extents log-uniform in
+[0.05, 1.0] m, centres uniform in a cube sized for about 8 overlapping partners
+per box (about 4.2 M pairs).  Uniform grid (cell 2 m, every extent <= cell, so
+the grid is complete) and SAP run on the device and are compared with the
+oracle's grid (the same deterministic pair order, bit for bit), and with each
+other as normalised sorted sets (PairBench.cpp:593-656 protocol).  For SAP the
+axis must be the oracle's and the list must have SAP's order: every pair is
+(earlier, later) in the stable sort of the order-preserving bits of
+min[axis] (sap.cl:231-389), lexicographic in those ranks.  (The oracle's own
+SAP sweep takes about a minute at this size; the pair-for-pair comparison with
+it runs on the smaller scenes of the parity tests.)"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from bullet3_amd import capi, layout, scenes
+
+pytestmark = pytest.mark.gpu
+
+N = 1 << 20
+
+
+def device_pairs(kind, mn, mx, cell=2.0):
+    n = len(mn)
+    cfg = capi.default_config(n + 64, 16)
+    bp = ctypes.c_void_p()
+    capi.call("b3h_bp_create", kind, ctypes.byref(cfg), None, ctypes.byref(bp))
+    try:
+        cell3 = np.array([cell] * 3, np.float32)
+        dims = np.array([128, 128, 128], np.int32)
+        capi.call("b3h_bp_set_grid_params", bp, capi.ptr(cell3), capi.ptr(dims), 256)
+        capi.call("b3h_bp_create_proxies", bp, n, capi.ptr(mn), capi.ptr(mx), None)
+        capi.call("b3h_bp_write_aabbs_to_gpu", bp)
+        capi.call("b3h_bp_calculate_overlapping_pairs", bp, cfg.m_maxBroadphasePairs)
+        cnt, tot = ctypes.c_int(), ctypes.c_int()
+        capi.call("b3h_bp_get_num_overlap", bp, ctypes.byref(cnt))
+        capi.call("b3h_bp_get_total_found", bp, ctypes.byref(tot))
+        d = ctypes.c_void_p()
+        capi.call("b3h_bp_get_overlapping_pair_buffer", bp, ctypes.byref(d))
+        pairs = capi.dtoh(d.value, cnt.value * 16, np.int32).reshape(-1, 4).copy()
+        axis = ctypes.c_int(-1)
+        if kind == capi.BROADPHASE_SAP:
+            capi.call("b3h_bp_get_sap_axis", bp, ctypes.byref(axis))
+        return pairs, tot.value, axis.value
+    finally:
+        capi.call("b3h_bp_destroy", bp)
+
+
+def aabb_records(mn, mx):
+    a = np.zeros(len(mn), layout.AABB)
+    a["min"][:, :3] = mn
+    a["max"][:, :3] = mx
+    a["min"][:, 3] = np.arange(len(mn), dtype=np.int32).view(np.float32)
+    a["max"][:, 3] = np.arange(len(mn), dtype=np.int32).view(np.float32)
+    return a
+
+
+def float_flip(f):
+    u = f.view(np.uint32)
+    mask = np.where(u >> 31 == 1, np.uint32(0xFFFFFFFF), np.uint32(0x80000000))
+    return u ^ mask
+
+
+def normalized(p):
+    q = np.sort(p[:, :2], axis=1).astype(np.int64)
+    return np.unique(q[:, 0] << 32 | q[:, 1])
+
+
+def test_million_aabbs_grid_sap_oracle():
+    mn, mx, side = scenes.aabb_cloud(N)
+    mn = np.ascontiguousarray(mn, np.float32)
+    mx = np.ascontiguousarray(mx, np.float32)
+    grid, gtot, _ = device_pairs(capi.BROADPHASE_GRID, mn, mx)
+    sap, stot, axis = device_pairs(capi.BROADPHASE_SAP, mn, mx)
+    assert len(grid) == gtot and len(sap) == stot  # nothing dropped
+    assert 3.5 * N < len(grid) < 5.0 * N, len(grid)
+    a = aabb_records(mn, mx)
+    small = np.arange(N, dtype=np.int32)
+    large = np.zeros(0, np.int32)
+    og, _ = po.grid_pairs(a, small, large, cell=(2.0, 2.0, 2.0), max_pairs=16 * N)
+    assert np.array_equal(grid, og)
+    assert axis == po.lib().b3o_sap_axis(po.ptr(a), po.ptr(small), N)
+    rank = np.empty(N, np.int64)
+    rank[np.argsort(float_flip(np.ascontiguousarray(mn[:, axis])), kind="stable")] = np.arange(N)
+    rx, ry = rank[sap[:, 0]], rank[sap[:, 1]]
+    assert np.all(rx < ry)
+    key = rx * N + ry
+    assert np.all(np.diff(key) > 0)
+    ng, ns = normalized(grid), normalized(sap)
+    assert len(ng) == len(grid) and np.array_equal(ng, ns)
