@@ -107,11 +107,26 @@ _SIGS = {
     "b3h_pipeline_get_pgs_levels": ([_vp, _vp], _i),
     "b3h_pipeline_set_track_residual": ([_vp, _i], _i),
     "b3h_pipeline_get_residuals": ([_vp, _vp, _i], _i),
+    "b3h_pipeline_get_residuals_l2": ([_vp, _vp, _i], _i),
     "b3h_pipeline_set_warm_start_impulses": ([_vp, _vp, _i], _i),
+    "b3h_pipeline_set_warm_start_model": ([_vp, _vp], _i),
+    "b3h_pipeline_clear_warm_start_history": ([_vp], _i),
+    "b3h_pipeline_get_warm_start_debug": ([_vp, _pvp, _pvp, _pvp], _i),
+    "b3h_pipeline_set_contact_log": ([_vp, _vp, _vp, _i], _i),
+    "b3h_pipeline_get_contact_log_count": ([_vp, _pi], _i),
     "b3h_pipeline_enable_timing": ([_vp, _i], _i),
     "b3h_pipeline_get_stage_times": ([_vp, _vp], _i),
     "b3h_pipeline_get_overflow": ([_vp, _pi, _pi], _i),
     "b3h_pipeline_step_graph": ([_vp, _f, _i], _i),
+    "b3h_mlp_create": ([_i, _i, _i, ctypes.c_uint, _vp, _pvp], _i),
+    "b3h_mlp_destroy": ([_vp], _i),
+    "b3h_mlp_get_dims": ([_vp, _pi, _pi, _pi], _i),
+    "b3h_mlp_num_params": ([_vp, _pi], _i),
+    "b3h_mlp_set_params": ([_vp, _vp], _i),
+    "b3h_mlp_get_params": ([_vp, _vp], _i),
+    "b3h_mlp_forward": ([_vp, _vp, _i, _i, _vp, _i], _i),
+    "b3h_mlp_loss_gradients": ([_vp, _vp, _i, _vp, _i, _i, _vp, _vp], _i),
+    "b3h_mlp_train": ([_vp, _vp, _i, _vp, _i, _i, _i, _i, _f, ctypes.c_uint, _vp], _i),
     "b3h_radix_sort32": ([_vp, _vp, _i, _i], _i),
     "b3h_prefix_scan": ([_vp, _vp, _i, _pi], _i),
     "b3h_compute_polyhedral_features": ([_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),

@@ -956,6 +956,12 @@ struct b3hPipeline
 	int graphBodies;
 	float* dWarm;
 	int warmCount;
+	/* warm start and contact logging */
+	b3hWarmBuffers wb;
+	bool warmAlloc, warmFeatAlloc;
+	b3hMlp* warmNet;
+	bool logRows;
+	float matchTol;
 };
 
 static void pl_free(b3hPipeline* p)
@@ -980,6 +986,64 @@ static void pl_free(b3hPipeline* p)
 	dfree(p->sb.levelOrder);
 	dfree(p->sb.levelStart);
 	dfree(p->sb.cellLevels);
+	dfree(p->sb.lastClass);
+	dfree(p->sb.residSnap);
+	dfree(p->sb.residL2);
+	b3hWarmBuffers& w = p->wb;
+	dfree(w.htKeys);
+	dfree(w.htVals);
+	dfree(w.hist);
+	dfree(w.histValid);
+	dfree(w.prev);
+	dfree(w.warm);
+	dfree(w.X);
+	dfree(w.Y);
+	dfree(w.T);
+	dfree(w.rowCount);
+	dfree(w.rowScan);
+	dfree(w.scanTmp);
+	dfree(w.logCount);
+}
+
+/* warm-start buffers, allocated on first use (features: the per-point rows of
+ * the learned mode and of contact logging) */
+static int warm_alloc(b3hPipeline* p, bool features)
+{
+	b3hWarmBuffers& w = p->wb;
+	size_t mc = (size_t)p->cfg.m_maxContactCapacity;
+	int r = 0;
+	if (!p->warmAlloc)
+	{
+		int cap = 1;
+		while ((size_t)cap < 2 * mc) cap *= 2;
+		w.capC = (int)mc;
+		w.htCap = cap;
+		if ((r = dalloc(&w.htKeys, (size_t)cap)) || (r = dalloc(&w.htVals, (size_t)cap)) || (r = dalloc(&w.hist, mc)) ||
+			(r = dalloc(&w.histValid, 1)) || (r = dalloc(&w.prev, 8 * mc)) || (r = dalloc(&w.warm, 6 * mc)) ||
+			(r = dalloc(&w.logCount, 1)))
+			return r;
+		B3H_CHECK(hipMemset(w.histValid, 0, sizeof(int)));
+		B3H_CHECK(hipMemset(w.logCount, 0, sizeof(int)));
+		B3H_CHECK(hipMemset(w.htKeys, 0xff, sizeof(unsigned long long) * cap));
+		p->warmAlloc = true;
+	}
+	if (features && !p->warmFeatAlloc)
+	{
+		if ((r = dalloc(&w.X, 4 * mc * B3H_WARM_FEATURES)) || (r = dalloc(&w.Y, 4 * mc * B3H_WARM_OUTPUTS)) ||
+			(r = dalloc(&w.T, 4 * mc * B3H_WARM_OUTPUTS)) || (r = dalloc(&w.rowCount, mc + 1)) ||
+			(r = dalloc(&w.rowScan, mc + 2)) || (r = dalloc(&w.scanTmp, b3h_scan_tmp_size((int)mc))))
+			return r;
+		p->warmFeatAlloc = true;
+	}
+	return B3H_OK;
+}
+static void drop_graph(b3hPipeline* p)
+{
+	if (p->graphExec)
+	{
+		hipGraphExecDestroy(p->graphExec);
+		p->graphExec = nullptr;
+	}
 }
 
 extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const b3hConfig* cfg, void* stream,
@@ -1003,6 +1067,11 @@ extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const 
 	p->graphBodies = -1;
 	p->dWarm = nullptr;
 	p->warmCount = 0;
+	memset((void*)&p->wb, 0, sizeof(p->wb));
+	p->warmAlloc = p->warmFeatAlloc = false;
+	p->warmNet = nullptr;
+	p->logRows = false;
+	p->matchTol = 0.1f;
 	int r;
 	/* one stream for all three objects keeps the step in order: while the
 	 * pipeline exists the broadphase issues its stand-alone calls on the
@@ -1021,6 +1090,7 @@ extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const 
 	int mc = cfg->m_maxContactCapacity;
 	p->sb.maxContacts = mc;
 	p->sb.pgsBodyCap = -1;
+	p->sb.numContacts = np->dNumContacts;
 	if ((r = dalloc(&p->sb.cellOf, mc)) || (r = dalloc(&p->sb.cellCounts, B3H_N_CELLS)) ||
 		(r = dalloc(&p->sb.cellOffsets, B3H_N_CELLS + 1)) || (r = dalloc(&p->sb.cellCursor, B3H_N_CELLS)) ||
 		(r = dalloc(&p->sb.cellSlots, mc)) || (r = dalloc(&p->sb.perm, mc)) || (r = dalloc(&p->sb.batchIdx, mc)) ||
@@ -1029,7 +1099,9 @@ extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const 
 		(r = dalloc(&p->sb.cellBodies, 2 * (size_t)mc + 64)) || (r = dalloc(&p->sb.cellBodyCount, B3H_N_CELLS)) ||
 		(r = dalloc(&p->sb.cslot, 2 * (size_t)mc + 2)) || (r = dalloc(&p->sb.recN, 21 * (size_t)mc)) ||
 		(r = dalloc(&p->sb.recF, 16 * (size_t)mc)) || (r = dalloc(&p->sb.levelOrder, mc)) ||
-		(r = dalloc(&p->sb.levelStart, (size_t)mc + B3H_N_CELLS + 8)) || (r = dalloc(&p->sb.cellLevels, B3H_N_CELLS)))
+		(r = dalloc(&p->sb.levelStart, (size_t)mc + B3H_N_CELLS + 8)) || (r = dalloc(&p->sb.cellLevels, B3H_N_CELLS)) ||
+		(r = dalloc(&p->sb.lastClass, (size_t)cfg->m_maxConvexBodies + 16)) ||
+		(r = dalloc(&p->sb.residSnap, 6 * (size_t)mc)) || (r = dalloc(&p->sb.residL2, 4 * B3H_MAX_ITERATIONS)))
 	{
 		pl_free(p);
 		delete p;
@@ -1063,12 +1135,27 @@ static void pl_aabbs(b3hPipeline* p)
 	b3h_launch_world_aabbs(num_bodies(p), p->np->dBodies, p->np->dCols, p->np->dLocalAabbs, p->bp->dAll, p->stream);
 }
 
-static void pl_solve(b3hPipeline* p, hipEvent_t evSetupDone = nullptr, hipEvent_t evNormalDone = nullptr)
+static void pl_solve(b3hPipeline* p, hipEvent_t evSetupDone = nullptr, hipEvent_t evNormalDone = nullptr,
+					 const b3hIntegrateParams* integrate = nullptr)
 {
-	p->sb.warmImpulses = p->dWarm;
-	b3h_solve_contacts(num_bodies(p), p->np->dBodies, p->np->dInertias, p->np->dNumContacts, p->cfg.m_maxContactCapacity,
-					   p->np->dStaging, p->np->dContactPair, p->prm, p->np->static0Index, p->sb, true, p->trackResidual,
-					   p->stream, evSetupDone, evNormalDone);
+	hipStream_t s = p->stream;
+	int mc = p->cfg.m_maxContactCapacity;
+	b3h_solve_setup(num_bodies(p), p->np->dBodies, p->np->dInertias, p->np->dNumContacts, mc, p->np->dStaging,
+					p->np->dContactPair, p->prm, p->np->static0Index, p->sb, true, s);
+	int mode = p->prm.m_warmStartMode;
+	bool warmOn = (mode >= 1 || p->logRows) && p->warmAlloc;
+	b3hWarmParams wp;
+	float g = sqrtf(dot3(p->gravity, p->gravity)) * p->prm.m_solverDt;
+	wp.velUnit = g > 0.f ? g : 1.f;
+	wp.matchTol = p->matchTol;
+	wp.mode = mode;
+	if (warmOn)
+		b3h_warm_prepare(p->np->dNumContacts, mc, p->sb.constraints, p->np->dBodies, p->wb, wp,
+						 mode == 2 ? p->warmNet : nullptr, p->logRows, s);
+	p->sb.warmImpulses = p->dWarm ? p->dWarm : ((warmOn && mode >= 1) ? p->wb.warm : nullptr);
+	if (evSetupDone) hipEventRecord(evSetupDone, s);
+	b3h_solve_sweeps(p->np->dBodies, p->np->dInertias, p->prm, p->sb, p->trackResidual, s, evNormalDone, integrate);
+	if (warmOn) b3h_warm_finish(p->np->dNumContacts, mc, p->sb.constraints, p->wb, wp, p->logRows, s);
 }
 
 static int pl_step(b3hPipeline* p, float dt, bool events)
@@ -1086,9 +1173,14 @@ static int pl_step(b3hPipeline* p, float dt, bool events)
 	np_compute_contacts_dev(p->np, p->bp->dPairs, p->bp->dPairCount, p->cfg.m_maxBroadphasePairs,
 							events ? p->ev[3] : nullptr);
 	if (events) hipEventRecord(p->ev[4], s);
-	pl_solve(p, events ? p->ev[5] : nullptr, events ? p->ev[6] : nullptr);
+	/* integration is fused into the last friction sweep for every body a
+	 * contact touches; the rest are integrated by the pass below */
+	b3hIntegrateParams ip = {dt, p->prm.m_angularDamping, p->gravity};
+	bool fused = p->prm.m_numIterations > 0;
+	pl_solve(p, events ? p->ev[5] : nullptr, events ? p->ev[6] : nullptr, fused ? &ip : nullptr);
 	if (events) hipEventRecord(p->ev[7], s);
-	b3h_launch_integrate(num_bodies(p), p->np->dBodies, dt, p->prm.m_angularDamping, p->gravity, s);
+	b3h_launch_integrate(num_bodies(p), p->np->dBodies, dt, p->prm.m_angularDamping, p->gravity, s,
+						 fused ? p->sb.lastClass : nullptr);
 	if (events) hipEventRecord(p->ev[8], s);
 	return B3H_OK;
 }
@@ -1247,6 +1339,7 @@ extern "C" int b3h_pipeline_reset(b3hPipeline* p)
 	p->bp->all.clear();
 	p->bp->smallMap.clear();
 	p->bp->largeMap.clear();
+	if (p->warmAlloc) B3H_CHECK(hipMemsetAsync(p->wb.histValid, 0, sizeof(int), p->stream));
 	return B3H_OK;
 }
 extern "C" int b3h_pipeline_get_body_buffer(b3hPipeline* p, b3hRigidBodyData** dev)
@@ -1265,12 +1358,81 @@ extern "C" int b3h_pipeline_set_solver_params(b3hPipeline* p, const b3hSolverPar
 {
 	CHECK_ARG(p && prm && prm->m_numIterations >= 0 && prm->m_numIterations <= 64 && prm->m_solverDt > 0.f &&
 			  prm->m_batchCellSize > 0.f);
-	p->prm = *prm;
-	if (p->graphExec)
+	CHECK_ARG(prm->m_warmStartMode >= 0 && prm->m_warmStartMode <= 2);
+	if (prm->m_warmStartMode == 2 && !p->warmNet)
 	{
-		hipGraphExecDestroy(p->graphExec);
-		p->graphExec = nullptr;
+		b3h_set_error("setSolverParams: the learned warm start (mode 2) needs a network "
+					  "(b3h_pipeline_set_warm_start_model first)");
+		return B3H_ERR_STATE;
 	}
+	if (prm->m_warmStartMode >= 1)
+	{
+		int r = warm_alloc(p, prm->m_warmStartMode == 2 || p->logRows);
+		if (r) return r;
+	}
+	p->prm = *prm;
+	drop_graph(p);
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_set_warm_start_model(b3hPipeline* p, b3hMlp* net)
+{
+	CHECK_ARG(p);
+	if (net)
+	{
+		int in = 0, out = 0;
+		b3h_mlp_get_dims(net, &in, nullptr, &out);
+		if (in != B3H_WARM_FEATURES || out != B3H_WARM_OUTPUTS)
+		{
+			b3h_set_error("setWarmStartModel: the network maps %d features to %d outputs, the warm start needs %d -> %d",
+						  in, out, B3H_WARM_FEATURES, B3H_WARM_OUTPUTS);
+			return B3H_ERR_INVALID;
+		}
+	}
+	else if (p->prm.m_warmStartMode == 2)
+	{
+		b3h_set_error("setWarmStartModel: cannot remove the network while the learned warm start is selected");
+		return B3H_ERR_STATE;
+	}
+	p->warmNet = net;
+	drop_graph(p);
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_set_contact_log(b3hPipeline* p, float* featuresDev, float* targetsDev, int capacityRows)
+{
+	CHECK_ARG(p && capacityRows >= 0 && ((featuresDev && targetsDev) || capacityRows == 0));
+	if (capacityRows > 0)
+	{
+		int r = warm_alloc(p, true);
+		if (r) return r;
+	}
+	p->logRows = capacityRows > 0;
+	p->wb.logX = capacityRows > 0 ? featuresDev : nullptr;
+	p->wb.logT = capacityRows > 0 ? targetsDev : nullptr;
+	p->wb.logCap = capacityRows;
+	if (p->warmAlloc) B3H_CHECK(hipMemsetAsync(p->wb.logCount, 0, sizeof(int), p->stream));
+	drop_graph(p);
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_get_contact_log_count(b3hPipeline* p, int* rows)
+{
+	CHECK_ARG(p && rows);
+	*rows = 0;
+	if (!p->warmAlloc) return B3H_OK;
+	return read_int(p->wb.logCount, rows, p->stream);
+}
+extern "C" int b3h_pipeline_get_warm_start_debug(b3hPipeline* p, float** warmDev, float** prevDev,
+												 float** featuresDev)
+{
+	CHECK_ARG(p);
+	if (warmDev) *warmDev = p->wb.warm;
+	if (prevDev) *prevDev = p->wb.prev;
+	if (featuresDev) *featuresDev = p->wb.X;
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_clear_warm_start_history(b3hPipeline* p)
+{
+	CHECK_ARG(p);
+	if (p->warmAlloc) B3H_CHECK(hipMemsetAsync(p->wb.histValid, 0, sizeof(int), p->stream));
 	return B3H_OK;
 }
 extern "C" int b3h_pipeline_get_solver_params(b3hPipeline* p, b3hSolverParams* prm)
@@ -1332,6 +1494,20 @@ extern "C" int b3h_pipeline_get_residuals(b3hPipeline* p, float* out, int n)
 	{
 		float d = tmp[2 * i], l = tmp[2 * i + 1];
 		out[i] = d / (l > 1e-9f ? l : 1e-9f);
+	}
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_get_residuals_l2(b3hPipeline* p, float* out, int n)
+{
+	CHECK_ARG(p && out && n >= 0 && n <= 2 * p->prm.m_numIterations);
+	double tmp[4 * B3H_MAX_ITERATIONS];
+	B3H_CHECK(hipMemcpyAsync(tmp, p->sb.residL2, sizeof(double) * 4 * p->prm.m_numIterations, hipMemcpyDeviceToHost,
+							 p->stream));
+	B3H_CHECK(hipStreamSynchronize(p->stream));
+	for (int i = 0; i < n; i++)
+	{
+		double d = tmp[2 * i], l = tmp[2 * i + 1];
+		out[i] = (float)(l > 0.0 ? sqrt(d / l) : 0.0);
 	}
 	return B3H_OK;
 }

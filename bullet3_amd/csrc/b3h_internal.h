@@ -35,7 +35,9 @@ void b3h_fill_i32(int* p, int v, int n, hipStream_t s);
 /* ---------------- per-body kernels (k_body.hip) ---------------- */
 void b3h_launch_world_aabbs(int n, const b3hRigidBodyData* bodies, const b3hCollidable* cols, const b3hAabb* localAabbs,
 							b3hAabb* world, hipStream_t s);
-void b3h_launch_integrate(int n, b3hRigidBodyData* bodies, float dt, float damping, F4 gravity, hipStream_t s);
+/* skip (optional, per body): bodies with skip[i] >= 0 were integrated by the solver */
+void b3h_launch_integrate(int n, b3hRigidBodyData* bodies, float dt, float damping, F4 gravity, hipStream_t s,
+						  const int* skip = nullptr);
 
 /* ---------------- broadphase (k_broadphase.hip) ---------------- */
 struct b3hGridParams
@@ -93,6 +95,13 @@ void b3h_gather_contacts(const int* contactPair, const int* numContacts, int max
 #define B3H_N_CELLS (B3H_N_SPLIT_X * B3H_N_SPLIT_Y * B3H_N_SPLIT_Z)
 #define B3H_N_CELL_BATCHES 8
 
+/* integration fused into the last friction sweep */
+struct b3hIntegrateParams
+{
+	float dt, damping;
+	F4 gravity;
+};
+
 struct b3hSolverBuffers
 {
 	int* cellOf;          /* per contact slot */
@@ -117,14 +126,69 @@ struct b3hSolverBuffers
 	int* levelOrder;      /* per cell region [off, off+n): constraints grouped by dependency level */
 	int* levelStart;      /* per cell region [off+cell, off+cell+L+1) */
 	int* cellLevels;      /* 256: levels per cell, -1 = global walk */
+	int* lastClass;       /* per body: the last cell-parity class whose cells touch it (-1: none) */
 	int maxContacts;
+	float* residSnap;     /* per constraint: 6 impulses before the sweep (residual tracking) */
+	double* residL2;      /* per sweep: sum of squared impulse changes, sum of squared impulses */
+	const int* numContacts;
 };
+#define B3H_MAX_ITERATIONS 64
 int b3h_solver_module_init();
 int b3h_solver_lds_bytes();
+/* set-up (ordering, batching, constraints, PGS records and levels), then the sweeps */
+void b3h_solve_setup(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaData* inertias, const int* numContacts,
+					 int maxContacts, const b3hContact4Data* staging, const int* contactPair,
+					 const b3hSolverParams& prm, int staticIdx, b3hSolverBuffers& sb, bool writeSortedContacts,
+					 hipStream_t s);
+void b3h_solve_sweeps(b3hRigidBodyData* bodies, const b3hInertiaData* inertias, const b3hSolverParams& prm,
+					  b3hSolverBuffers& sb, bool trackResidual, hipStream_t s, hipEvent_t evNormalDone = nullptr,
+					  const b3hIntegrateParams* integrate = nullptr);
 void b3h_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaData* inertias,
 						const int* numContacts, int maxContacts, const b3hContact4Data* staging,
 						const int* contactPair, const b3hSolverParams& prm, int staticIdx, b3hSolverBuffers& sb,
 						bool writeSortedContacts, bool trackResidual, hipStream_t s, hipEvent_t evSetupDone = nullptr,
-						hipEvent_t evNormalDone = nullptr);
+						hipEvent_t evNormalDone = nullptr, const b3hIntegrateParams* integrate = nullptr);
+
+/* ---------------- warm start (k_warm.hip) ---------------- */
+#define B3H_WARM_FEATURES 24
+#define B3H_WARM_OUTPUTS 3
+struct b3hWarmHist
+{
+	F4 pos[4]; /* xyz: contact point, w: final normal impulse */
+	F4 fric;   /* xyz: final friction impulse as a world vector, w: number of points */
+};
+struct b3hWarmParams
+{
+	float velUnit;  /* |g| dt */
+	float matchTol; /* contact point matching distance */
+	int mode;       /* 1 previous frame, 2 learned */
+};
+struct b3hWarmBuffers
+{
+	int capC;                  /* constraints */
+	int htCap;                 /* hash slots, a power of two >= 2 capC */
+	unsigned long long* htKeys;
+	int* htVals;
+	b3hWarmHist* hist;         /* [capC] */
+	int* histValid;            /* 1 int */
+	float* prev;               /* [capC][8]: matched normal impulses 4, friction 2, pair matched, points */
+	float* warm;               /* [capC][6]: initial impulses in solve order */
+	float* X;                  /* [4 capC][24] features (learned mode, logging) */
+	float* Y;                  /* [4 capC][3] network outputs */
+	float* T;                  /* [4 capC][3] targets (logging) */
+	int* rowCount;             /* [capC] */
+	int* rowScan;              /* [capC + 1] */
+	unsigned* scanTmp;
+	float* logX;               /* caller's log: [logCap][24] */
+	float* logT;               /* [logCap][3] */
+	int logCap;
+	int* logCount;             /* device int */
+};
+struct b3hMlp;
+void b3h_warm_prepare(const int* numContacts, int maxContacts, const b3hContactConstraint4* cs,
+					  const b3hRigidBodyData* bodies, b3hWarmBuffers& wb, const b3hWarmParams& wp, const b3hMlp* net,
+					  bool features, hipStream_t s);
+void b3h_warm_finish(const int* numContacts, int maxContacts, const b3hContactConstraint4* cs, b3hWarmBuffers& wb,
+					 const b3hWarmParams& wp, bool logRows, hipStream_t s);
 
 #endif

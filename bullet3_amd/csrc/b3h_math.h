@@ -108,4 +108,63 @@ B3H_HD F4 mtMul3(F4 a, const b3hMat3x3& b)
 /* 16-byte vector loads of the AoS records */
 __device__ __forceinline__ F4 ld4(const F4* p) { return *p; }
 
+/* integrateSingleTransform (b3IntegrateTransforms.h:5-55): angular damping,
+ * quaternion step (pi/4 clamp, Taylor branch below 0.001 rad/s), explicit
+ * Euler position then gravity.  Static bodies (invMass 0) are left alone. */
+B3H_HD void b3h_integrate_body(b3hRigidBodyData& b, float dt, float damp, F4 g)
+{
+	if (b.m_invMass == 0.f) return;
+	const float thr = 0.25f * 3.14159254f;
+	b.m_angVel.x *= damp;
+	b.m_angVel.y *= damp;
+	b.m_angVel.z *= damp;
+	F4 av = b.m_angVel;
+	float ang = sqrtf(dot3(av, av));
+	if (ang * dt > thr) ang = thr / dt;
+	F4 axis;
+	if (ang < 0.001f)
+		axis = mul4(av, 0.5f * dt - (dt * dt * dt) * 0.020833333333f * ang * ang);
+	else
+		axis = mul4(av, b3h_sin_small(0.5f * ang * dt) / ang);
+	F4 dorn = f4(axis.x, axis.y, axis.z, b3h_cos_small(ang * dt * 0.5f));
+	F4 q = qmul(dorn, b.m_quat);
+	float len = sqrtf(dot4(q, q));
+	if (len > 0.f)
+		q = mul4(q, 1.f / len);
+	else
+		q = f4(0, 0, 0, 1);
+	b.m_quat = q;
+	b.m_pos = add4(b.m_pos, mul4(b.m_linVel, dt));
+	b.m_linVel = add4(b.m_linVel, mul4(g, dt));
+}
+
+/* b3PlaneSpace1: two unit tangents orthogonal to the unit normal n */
+B3H_HD void plane_space(F4 n, F4* p, F4* q)
+{
+	if (fabsf(n.z) > 0.70710678f)
+	{
+		float a = n.y * n.y + n.z * n.z;
+		float k = 1.f / sqrtf(a);
+		p->x = 0;
+		p->y = -n.z * k;
+		p->z = n.y * k;
+		q->x = a * k;
+		q->y = -n.x * p->z;
+		q->z = n.x * p->y;
+	}
+	else
+	{
+		float a = n.x * n.x + n.y * n.y;
+		float k = 1.f / sqrtf(a);
+		p->x = -n.y * k;
+		p->y = n.x * k;
+		p->z = 0;
+		q->x = -n.z * p->y;
+		q->y = n.z * p->x;
+		q->z = a * k;
+	}
+	p->w = 0.f;
+	q->w = 0.f;
+}
+
 #endif

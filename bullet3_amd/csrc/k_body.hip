@@ -61,42 +61,24 @@ void b3h_launch_world_aabbs(int n, const b3hRigidBodyData* bodies, const b3hColl
 	hipLaunchKernelGGL(k_world_aabbs, dim3((n + 255) / 256), dim3(256), 0, s, n, bodies, cols, localAabbs, world);
 }
 
-__global__ __launch_bounds__(256) void k_integrate(int n, b3hRigidBodyData* __restrict__ bodies, float dt, float damp, F4 g)
+__global__ __launch_bounds__(256) void k_integrate(int n, b3hRigidBodyData* __restrict__ bodies, float dt, float damp, F4 g,
+													const int* __restrict__ skip)
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
+	if (skip && skip[i] >= 0) return; /* integrated by the last PGS sweep */
 	b3hRigidBodyData b = bodies[i];
 	if (b.m_invMass == 0.f) return;
-	const float thr = 0.25f * 3.14159254f;
-	b.m_angVel.x *= damp;
-	b.m_angVel.y *= damp;
-	b.m_angVel.z *= damp;
-	F4 av = b.m_angVel;
-	float ang = sqrtf(dot3(av, av));
-	if (ang * dt > thr) ang = thr / dt;
-	F4 axis;
-	if (ang < 0.001f)
-		axis = mul4(av, 0.5f * dt - (dt * dt * dt) * 0.020833333333f * ang * ang);
-	else
-		axis = mul4(av, b3h_sin_small(0.5f * ang * dt) / ang);
-	F4 dorn = f4(axis.x, axis.y, axis.z, b3h_cos_small(ang * dt * 0.5f));
-	F4 q = qmul(dorn, b.m_quat);
-	float len = sqrtf(dot4(q, q));
-	if (len > 0.f)
-		q = mul4(q, 1.f / len);
-	else
-		q = f4(0, 0, 0, 1);
-	b.m_quat = q;
-	b.m_pos = add4(b.m_pos, mul4(b.m_linVel, dt));
-	b.m_linVel = add4(b.m_linVel, mul4(g, dt));
+	b3h_integrate_body(b, dt, damp, g);
 	bodies[i].m_pos = b.m_pos;
 	bodies[i].m_quat = b.m_quat;
 	bodies[i].m_linVel = b.m_linVel;
 	bodies[i].m_angVel = b.m_angVel;
 }
 
-void b3h_launch_integrate(int n, b3hRigidBodyData* bodies, float dt, float damping, F4 gravity, hipStream_t s)
+void b3h_launch_integrate(int n, b3hRigidBodyData* bodies, float dt, float damping, F4 gravity, hipStream_t s,
+						  const int* skip)
 {
 	if (n <= 0) return;
-	hipLaunchKernelGGL(k_integrate, dim3((n + 255) / 256), dim3(256), 0, s, n, bodies, dt, damping, gravity);
+	hipLaunchKernelGGL(k_integrate, dim3((n + 255) / 256), dim3(256), 0, s, n, bodies, dt, damping, gravity, skip);
 }

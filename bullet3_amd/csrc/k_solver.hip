@@ -314,33 +314,6 @@ __global__ __launch_bounds__(256) void k_batch_cells(const int* __restrict__ cel
 }
 
 /* ---------------- contact -> constraint (setConstraint4, b3ConvertConstraint4.h:62-148) ---------------- */
-__device__ void plane_space(F4 n, F4* p, F4* q)
-{
-	if (fabsf(n.z) > 0.70710678f)
-	{
-		float a = n.y * n.y + n.z * n.z;
-		float k = 1.f / sqrtf(a);
-		p->x = 0;
-		p->y = -n.z * k;
-		p->z = n.y * k;
-		q->x = a * k;
-		q->y = -n.x * p->z;
-		q->z = n.x * p->y;
-	}
-	else
-	{
-		float a = n.x * n.x + n.y * n.y;
-		float k = 1.f / sqrtf(a);
-		p->x = -n.y * k;
-		p->y = n.x * k;
-		p->z = 0;
-		q->x = -n.z * p->y;
-		q->y = n.z * p->x;
-		q->z = a * k;
-	}
-	p->w = 0.f;
-	q->w = 0.f;
-}
 __device__ __forceinline__ float calc_jac_coeff(F4 ang0, F4 ang1, float im0, const b3hMat3x3& ii0, float im1,
 												 const b3hMat3x3& ii1)
 {
@@ -615,7 +588,8 @@ __global__ __launch_bounds__(256) void k_cell_bodies(const int* __restrict__ cel
 													 const b3hContactConstraint4* __restrict__ cs,
 													 const b3hRigidBodyData* __restrict__ bodies, int ldsCapKeys,
 													 unsigned* __restrict__ gScratch, int* __restrict__ cellBodies,
-													 int* __restrict__ cellBodyCount, int* __restrict__ cslot)
+													 int* __restrict__ cellBodyCount, int* __restrict__ cslot,
+													 int* __restrict__ lastClass)
 {
 	extern __shared__ unsigned long long smem[];
 	__shared__ int waveSums[4];
@@ -624,6 +598,9 @@ __global__ __launch_bounds__(256) void k_cell_bodies(const int* __restrict__ cel
 	int n = cellCounts[cell];
 	int off = cellOffsets[cell];
 	int t = threadIdx.x;
+	/* launch order of the cell within a sweep: its 2x2x2 parity class */
+	const int cellClass = (cell & 1) | (((cell / B3H_N_SPLIT_X) & 1) << 1) |
+						  (((cell / (B3H_N_SPLIT_X * B3H_N_SPLIT_Y)) & 1) << 2);
 	if (n == 0)
 	{
 		if (t == 0) cellBodyCount[cell] = 0;
@@ -682,7 +659,11 @@ __global__ __launch_bounds__(256) void k_cell_bodies(const int* __restrict__ cel
 		for (int w = 0; w < wave; w++) before += waveSums[w];
 		int r = before + inWave + (head ? 0 : -1);
 		if (i < np2) rank[i] = (unsigned)r;
-		if (head) cellBodies[2 * (size_t)off + r] = (int)k;
+		if (head)
+		{
+			cellBodies[2 * (size_t)off + r] = (int)k;
+			if (lastClass) atomicMax(&lastClass[k], cellClass);
+		}
 		__syncthreads();
 		if (t == 0)
 			for (int w = 0; w < 4; w++) runTotal += waveSums[w];
@@ -1138,7 +1119,9 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 														   const int* __restrict__ levelStart,
 														   const int* __restrict__ levelOrder, const F4* __restrict__ recN,
 														   const F4* __restrict__ recF, int ib, float friction,
-														   const float* __restrict__ warm, float* __restrict__ resid)
+														   const float* __restrict__ warm, float* __restrict__ resid,
+														   const int* __restrict__ lastClass, int fuseIntegrate,
+														   b3hIntegrateParams ip)
 {
 	extern __shared__ F4 smem4[];
 	int cell = cell_of(blockIdx.x, ib);
@@ -1164,6 +1147,22 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 					idx += 64;
 				}
 			__syncthreads();
+		}
+		if (fuseIntegrate)
+		{
+			const int* list = cellBodies + 2 * (size_t)start;
+			int nbg = cellBodyCount[cell];
+			for (int q = t; q < nbg; q += T)
+			{
+				int b = list[q];
+				if (lastClass[b] != ib) continue;
+				b3hRigidBodyData body = bodies[b];
+				b3h_integrate_body(body, ip.dt, ip.damping, ip.gravity);
+				bodies[b].m_pos = body.m_pos;
+				bodies[b].m_quat = body.m_quat;
+				bodies[b].m_linVel = body.m_linVel;
+				bodies[b].m_angVel = body.m_angVel;
+			}
 		}
 		return;
 	}
@@ -1241,8 +1240,23 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	for (int q = t; q < nb; q += T)
 	{
 		int b = sList[q];
-		bodies[b].m_linVel = sLv[q];
-		bodies[b].m_angVel = sAv[q];
+		if (fuseIntegrate && lastClass[b] == ib)
+		{
+			/* the body's velocity is final: integrate it here (integrateTransformsKernel) */
+			b3hRigidBodyData body = bodies[b];
+			body.m_linVel = sLv[q];
+			body.m_angVel = sAv[q];
+			b3h_integrate_body(body, ip.dt, ip.damping, ip.gravity);
+			bodies[b].m_pos = body.m_pos;
+			bodies[b].m_quat = body.m_quat;
+			bodies[b].m_linVel = body.m_linVel;
+			bodies[b].m_angVel = body.m_angVel;
+		}
+		else
+		{
+			bodies[b].m_linVel = sLv[q];
+			bodies[b].m_angVel = sAv[q];
+		}
 	}
 	if (resid)
 	{
@@ -1293,12 +1307,11 @@ static inline int gblocks(int maxN)
 	return b < 1 ? 1 : b;
 }
 
-void b3h_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaData* inertias, const int* numContacts,
-						int maxContacts, const b3hContact4Data* staging, const int* contactPair,
-						const b3hSolverParams& prm, int staticIdx, b3hSolverBuffers& sb, bool writeSortedContacts,
-						bool trackResidual, hipStream_t s, hipEvent_t evSetupDone, hipEvent_t evNormalDone)
+void b3h_solve_setup(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaData* inertias, const int* numContacts,
+					 int maxContacts, const b3hContact4Data* staging, const int* contactPair,
+					 const b3hSolverParams& prm, int staticIdx, b3hSolverBuffers& sb, bool writeSortedContacts,
+					 hipStream_t s)
 {
-	(void)numBodies;
 	int ldsCapBytes = s_ldsCapBytes > 0 ? s_ldsCapBytes : 32768;
 	int ldsCap = 1;
 	while ((ldsCap * 2) * 16 <= ldsCapBytes) ldsCap *= 2;
@@ -1315,10 +1328,10 @@ void b3h_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const b3hInerti
 	hipLaunchKernelGGL(k_to_constraints, dim3(gb), dim3(256), 0, s, numContacts, sb.perm, sb.batchIdx, staging,
 					   contactPair, bodies, inertias, prm, sb.constraints,
 					   writeSortedContacts ? sb.sortedContacts : (b3hContact4Data*)nullptr);
+	b3h_fill_i32(sb.lastClass, -1, numBodies, s);
 	hipLaunchKernelGGL(k_cell_bodies, dim3(B3H_N_CELLS), dim3(256), (size_t)ldsCapBytes, s, sb.cellCounts,
 					   sb.cellOffsets, sb.constraints, bodies, ldsCapBytes / 4, (unsigned*)sb.sortKeys, sb.cellBodies,
-					   sb.cellBodyCount, sb.cslot);
-	int K = prm.m_numIterations;
+					   sb.cellBodyCount, sb.cslot, sb.lastClass);
 	int cap = (sb.pgsBodyCap >= 0 && sb.pgsBodyCap < s_pgsBodyCap) ? sb.pgsBodyCap : s_pgsBodyCap;
 	size_t pgsLds = (size_t)s_pgsLdsBytes;
 	hipLaunchKernelGGL(k_pgs_prep, dim3(gb), dim3(256), 0, s, numContacts, sb.constraints, sb.cslot, bodies, inertias,
@@ -1326,27 +1339,123 @@ void b3h_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const b3hInerti
 	hipLaunchKernelGGL(k_cell_levels, dim3(B3H_N_CELLS), dim3(256), (size_t)ldsCapBytes, s, sb.cellCounts,
 					   sb.cellOffsets, sb.batchIdx, sb.cellBodyCount, sb.cslot, cap, ldsCapBytes / 4, (int)pgsLds,
 					   sb.levelOrder, sb.levelStart, sb.cellLevels);
+}
+
+/* residual tracking (off the hot path): over the rows one sweep of `phase`
+ * updates (0 normal, 1 friction), the sums of (impulse - impulse before the
+ * sweep)^2 and impulse^2; refreshes the snapshot (phase < 0: snapshot only) */
+__global__ __launch_bounds__(256) void k_resid_l2(const int* __restrict__ nPtr,
+												  const b3hContactConstraint4* __restrict__ cs, float* __restrict__ snap,
+												  int phase, double* __restrict__ sums)
+{
+	__shared__ double red[2][256];
+	int n = *nPtr, t = threadIdx.x;
+	double d2 = 0.0, l2 = 0.0;
+	for (int k = blockIdx.x * blockDim.x + t; k < n; k += gridDim.x * blockDim.x)
+	{
+		const b3hContactConstraint4& c = cs[k];
+		float* sn = snap + 6 * (size_t)k;
+		if (phase != 1)
+			for (int i = 0; i < 4; i++)
+			{
+				float l = c.m_appliedRambdaDt[i];
+				if (phase == 0 && c.m_jacCoeffInv[i] != 0.f)
+				{
+					double d = (double)l - sn[i];
+					d2 += d * d;
+					l2 += (double)l * l;
+				}
+				sn[i] = l;
+			}
+		if (phase != 0)
+			for (int m = 0; m < 2; m++)
+			{
+				float l = c.m_fAppliedRambdaDt[m];
+				if (phase == 1)
+				{
+					double d = (double)l - sn[4 + m];
+					d2 += d * d;
+					l2 += (double)l * l;
+				}
+				sn[4 + m] = l;
+			}
+	}
+	red[0][t] = d2;
+	red[1][t] = l2;
+	__syncthreads();
+	for (int o = 128; o > 0; o >>= 1)
+	{
+		if (t < o)
+		{
+			red[0][t] += red[0][t + o];
+			red[1][t] += red[1][t + o];
+		}
+		__syncthreads();
+	}
+	if (t == 0 && phase >= 0)
+	{
+		atomicAdd(sums, red[0][0]);
+		atomicAdd(sums + 1, red[1][0]);
+	}
+}
+
+void b3h_solve_sweeps(b3hRigidBodyData* bodies, const b3hInertiaData* inertias, const b3hSolverParams& prm,
+					  b3hSolverBuffers& sb, bool trackResidual, hipStream_t s, hipEvent_t evNormalDone,
+					  const b3hIntegrateParams* integrate)
+{
+	int K = prm.m_numIterations;
+	size_t pgsLds = (size_t)s_pgsLdsBytes;
 	if (trackResidual) b3h_fill_i32((int*)sb.resid, 0, 4 * K, s);
 	const dim3 grid(B3H_N_CELLS / B3H_N_CELL_BATCHES), block(PGS_THREADS);
-	if (evSetupDone) hipEventRecord(evSetupDone, s);
-#define B3H_PGS_ARGS(ib, resid_, warm_)                                                                         \
+	b3hIntegrateParams ip = integrate ? *integrate : b3hIntegrateParams{0.f, 0.f, f4(0, 0, 0, 0)};
+#define B3H_PGS_ARGS(ib, resid_, warm_, fuse_)                                                                         \
 	bodies, inertias, sb.constraints, sb.cellCounts, sb.cellOffsets, sb.batchSizes, sb.batchIdx, sb.cellBodies, \
 		sb.cellBodyCount, sb.cellLevels, sb.levelStart, sb.levelOrder, (const F4*)sb.recN, (const F4*)sb.recF, ib, \
-		prm.m_frictionCoeff, warm_, resid_
+		prm.m_frictionCoeff, warm_, resid_, (const int*)sb.lastClass, fuse_, ip
 	if (sb.warmImpulses)
 		for (int ib = 0; ib < B3H_N_CELL_BATCHES; ib++)
 			hipLaunchKernelGGL(k_pgs_level<2>, grid, block, pgsLds, s,
-							   B3H_PGS_ARGS(ib, (float*)nullptr, (const float*)sb.warmImpulses));
+							   B3H_PGS_ARGS(ib, (float*)nullptr, (const float*)sb.warmImpulses, 0));
+	bool l2 = trackResidual && sb.residSnap && sb.residL2 && sb.numContacts;
+	int rb = gblocks(sb.maxContacts);
+	if (rb > 1024) rb = 1024;
+	if (l2)
+	{
+		b3h_fill_i32((int*)sb.residL2, 0, 8 * K, s);
+		hipLaunchKernelGGL(k_resid_l2, dim3(rb), dim3(256), 0, s, sb.numContacts, (const b3hContactConstraint4*)sb.constraints,
+						   sb.residSnap, -1, (double*)nullptr);
+	}
 	for (int phase = 0; phase < 2; phase++)
 		for (int it = 0; it < K; it++)
+		{
 			for (int ib = 0; ib < B3H_N_CELL_BATCHES; ib++)
 			{
 				if (phase == 1 && it == 0 && ib == 0 && evNormalDone) hipEventRecord(evNormalDone, s);
 				float* r = trackResidual ? sb.resid + 2 * (phase * K + it) : (float*)nullptr;
+				int fuse = (integrate && phase == 1 && it == K - 1) ? 1 : 0;
 				if (phase == 0)
-					hipLaunchKernelGGL(k_pgs_level<0>, grid, block, pgsLds, s, B3H_PGS_ARGS(ib, r, (const float*)nullptr));
+					hipLaunchKernelGGL(k_pgs_level<0>, grid, block, pgsLds, s,
+									   B3H_PGS_ARGS(ib, r, (const float*)nullptr, 0));
 				else
-					hipLaunchKernelGGL(k_pgs_level<1>, grid, block, pgsLds, s, B3H_PGS_ARGS(ib, r, (const float*)nullptr));
+					hipLaunchKernelGGL(k_pgs_level<1>, grid, block, pgsLds, s,
+									   B3H_PGS_ARGS(ib, r, (const float*)nullptr, fuse));
 			}
+			if (l2)
+				hipLaunchKernelGGL(k_resid_l2, dim3(rb), dim3(256), 0, s, sb.numContacts,
+								   (const b3hContactConstraint4*)sb.constraints, sb.residSnap, phase,
+								   sb.residL2 + 2 * (phase * K + it));
+		}
 #undef B3H_PGS_ARGS
+}
+
+void b3h_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaData* inertias, const int* numContacts,
+						int maxContacts, const b3hContact4Data* staging, const int* contactPair,
+						const b3hSolverParams& prm, int staticIdx, b3hSolverBuffers& sb, bool writeSortedContacts,
+						bool trackResidual, hipStream_t s, hipEvent_t evSetupDone, hipEvent_t evNormalDone,
+						const b3hIntegrateParams* integrate)
+{
+	b3h_solve_setup(numBodies, bodies, inertias, numContacts, maxContacts, staging, contactPair, prm, staticIdx, sb,
+					writeSortedContacts, s);
+	if (evSetupDone) hipEventRecord(evSetupDone, s);
+	b3h_solve_sweeps(bodies, inertias, prm, sb, trackResidual, s, evNormalDone, integrate);
 }

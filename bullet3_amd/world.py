@@ -169,6 +169,41 @@ class World:
         call("b3h_pipeline_get_residuals", self.pl, ptr(out), len(out))
         return out
 
+    def residuals_l2(self):
+        k = self.solver_params().m_numIterations
+        out = np.zeros(2 * k, np.float32)
+        call("b3h_pipeline_get_residuals_l2", self.pl, ptr(out), len(out))
+        return out
+
+    # -- warm start (mode 0 cold as the reference, 1 previous frame, 2 learned)
+    def set_warm_start(self, mode, net=None):
+        if net is not None:
+            call("b3h_pipeline_set_warm_start_model", self.pl, net.h)
+        sp = self.solver_params()
+        sp.m_warmStartMode = int(mode)
+        self.set_solver(sp)
+
+    def warm_debug(self, n):
+        """(initial impulses [n][6], matched previous impulses [n][8], feature rows [4n][24] or None)"""
+        w, pr, x = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        call("b3h_pipeline_get_warm_start_debug", self.pl, ctypes.byref(w), ctypes.byref(pr), ctypes.byref(x))
+        warm = capi.dtoh(w.value, n * 24, np.float32).reshape(n, 6) if w.value else None
+        prev = capi.dtoh(pr.value, n * 32, np.float32).reshape(n, 8) if pr.value else None
+        feats = capi.dtoh(x.value, n * 4 * 96, np.float32).reshape(4 * n, 24) if x.value else None
+        return warm, prev, feats
+
+    def clear_warm_history(self):
+        call("b3h_pipeline_clear_warm_start_history", self.pl)
+
+    def set_contact_log(self, x_dev, t_dev, capacity_rows):
+        call("b3h_pipeline_set_contact_log", self.pl, ctypes.c_void_p(x_dev) if x_dev else None,
+             ctypes.c_void_p(t_dev) if t_dev else None, int(capacity_rows))
+
+    def contact_log_count(self):
+        n = ctypes.c_int()
+        call("b3h_pipeline_get_contact_log_count", self.pl, ctypes.byref(n))
+        return n.value
+
     def overflow(self):
         a, b = ctypes.c_int(), ctypes.c_int()
         call("b3h_pipeline_get_overflow", self.pl, ctypes.byref(a), ctypes.byref(b))

@@ -38,6 +38,7 @@ extern "C" {
 typedef struct b3hNarrowPhase b3hNarrowPhase;
 typedef struct b3hBroadphase b3hBroadphase;
 typedef struct b3hPipeline b3hPipeline;
+typedef struct b3hMlp b3hMlp;
 
 #define B3H_BROADPHASE_GRID 0
 #define B3H_BROADPHASE_SAP 1
@@ -196,9 +197,40 @@ int b3h_pipeline_get_pgs_levels(b3hPipeline* p, int* levels256);
  * then friction sweeps); requires b3h_pipeline_set_track_residual(p, 1). */
 int b3h_pipeline_set_track_residual(b3hPipeline* p, int enable);
 int b3h_pipeline_get_residuals(b3hPipeline* p, float* out, int n);
+/* Per-sweep relative L2 residuals of the last solve: sqrt(sum (impulse change)^2 /
+ * sum impulse^2) over the rows the sweep updates (host, 2*iterations floats,
+ * normal sweeps then friction sweeps); requires residual tracking.  No
+ * counterpart in the reference, which computes no residual. */
+int b3h_pipeline_get_residuals_l2(b3hPipeline* p, float* out, int n);
 /* Initial impulses for the next solve (device float[6*numContacts] in solve
- * order: 4 normal rows then 2 friction rows per constraint); NULL clears. */
+ * order: 4 normal rows then 2 friction rows per constraint); NULL clears.
+ * Takes precedence over the warm-start mode. */
 int b3h_pipeline_set_warm_start_impulses(b3hPipeline* p, const float* dev, int numConstraints);
+/* Warm start (b3hSolverParams::m_warmStartMode): 0 starts every solve from
+ * zero impulses as the reference does (b3ConvertConstraint4.h:73,107,132);
+ * 1 starts from the previous step's final impulses of the same body pair
+ * (normal rows by nearest contact point within 0.1, friction as a world
+ * vector projected on the new tangents); 2 corrects those with the network set
+ * here (24 features -> 3 outputs, see b3h_mlp_create).  Only the initial
+ * iterate changes; the solve's fixed point does not.  The network must outlive
+ * its use by the pipeline. */
+int b3h_pipeline_set_warm_start_model(b3hPipeline* p, b3hMlp* net);
+/* Device buffers of the last warm-started solve, in solve order (NULL until a
+ * warm-start mode or the contact log is used): initial impulses float[6] per
+ * constraint; matched previous impulses float[8] per constraint (4 normal, 2
+ * friction, pair matched, points); feature rows float[4][24] per constraint
+ * (learned mode or logging).  For tests. */
+int b3h_pipeline_get_warm_start_debug(b3hPipeline* p, float** warmDev, float** prevDev, float** featuresDev);
+/* Forget the impulse history (the next solve finds no previous frame). */
+int b3h_pipeline_clear_warm_start_history(b3hPipeline* p);
+/* Contact logging for training the network: after every solve, each contact
+ * point appends its 24 features (as computed before the solve) to
+ * featuresDev[rows][24] and its 3 targets (the solve's final impulses relative
+ * to the previous-frame guess, in the network's output units) to
+ * targetsDev[rows][3], until capacityRows rows are held; capacityRows 0 turns
+ * logging off.  Setting the log resets its row count. */
+int b3h_pipeline_set_contact_log(b3hPipeline* p, float* featuresDev, float* targetsDev, int capacityRows);
+int b3h_pipeline_get_contact_log_count(b3hPipeline* p, int* rows);
 /* Stage timings of the last eager step in ms (enable first; adds HIP events on
  * the pipeline stream).  B3H_NUM_STAGES values, in order: world AABBs,
  * broadphase, narrowphase kernel, narrowphase compaction, solver set-up
@@ -221,6 +253,36 @@ int b3h_radix_sort32(unsigned* keysDev, unsigned* valuesDev, int n, int sortBits
 /* Exclusive prefix sum of n ints (device) into dst (n+1 ints, dst[n] = total);
  * b3PrefixScanCL::execute(src, dst, n, sum), b3PrefixScanCL.h:24. */
 int b3h_prefix_scan(const int* srcDev, int* dstDev, int n, int* sumOut);
+
+/* ---------------- learned warm start: the network ----------------
+ * No counterpart in the reference: its solver starts every step from zero
+ * impulses (src/Bullet3OpenCL/RigidBody/kernels/solverSetup.cl via
+ * b3ConvertConstraint4.h:73,107,132).  A 3-layer MLP inDim -> 128 -> 128 ->
+ * outDim (inDim, outDim <= 32), GELU (tanh form), bf16 weights and activations
+ * with fp32 accumulation on the matrix cores; fp32 master weights and Adam
+ * (beta 0.9/0.999, eps 1e-8) on the device.  Parameters cross the boundary in
+ * one packed fp32 array: w1[in][128] b1[128] w2[128][128] b2[128]
+ * w3[128][out] b3[out] (row = input unit, column = output unit).  Device
+ * pointers are row-major with leading dimensions ldx/ldy/ldt (in floats). */
+#define B3H_MLP_HIDDEN 128
+int b3h_mlp_create(int inDim, int outDim, int maxTrainRows, unsigned seed, void* hipStream, b3hMlp** out);
+int b3h_mlp_destroy(b3hMlp* m);
+int b3h_mlp_get_dims(const b3hMlp* m, int* inDim, int* hidden, int* outDim);
+int b3h_mlp_num_params(const b3hMlp* m, int* n);
+/* host float[num_params]; set_params also resets the Adam state */
+int b3h_mlp_set_params(b3hMlp* m, const float* host);
+int b3h_mlp_get_params(b3hMlp* m, float* host);
+/* y[rows][outDim] = net(x[rows][inDim]), asynchronous on the network's stream */
+int b3h_mlp_forward(b3hMlp* m, const float* xDev, int ldx, int rows, float* yDev, int ldy);
+/* mean squared error over rows x outDim and its gradient (host float[num_params],
+ * may be NULL) -- syncs; rows <= maxTrainRows */
+int b3h_mlp_loss_gradients(b3hMlp* m, const float* xDev, int ldx, const float* tDev, int ldt, int rows,
+						   float* gradsHost, float* lossOut);
+/* `steps` Adam steps on minibatches of batchRows rows drawn with replacement
+ * from the rows x / t (a hash of seed, step and row picks each one); the loss
+ * of every step goes to lossHistory (host float[steps], may be NULL) -- syncs */
+int b3h_mlp_train(b3hMlp* m, const float* xDev, int ldx, const float* tDev, int ldt, int rows, int batchRows, int steps,
+				  float learningRate, unsigned seed, float* lossHistory);
 
 /* ---------------- host-only utility ---------------- */
 /* Polyhedral features of a convex point cloud exactly as registerConvexHullShape
