@@ -32,6 +32,17 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def workload_name(args, n_dyn):
+    if args.config == "pile":
+        return ("convex pile %d dynamic bodies (boxes, 16/32-point hulls), grid broadphase, %d PGS iterations, "
+                "dt 1/60" % (n_dyn, args.iters))
+    if args.config == "batched":
+        return ("%d batched scenes, %d dynamic bodies (piles of boxes and 8-24-point hulls, box stacks, 16-512 "
+                "bodies each, own grounds), grid broadphase, %d PGS iterations, dt 1/60" % (args.scenes, n_dyn,
+                                                                                            args.iters))
+    return "10x10x10 unit cubes, grid broadphase, %d PGS iterations, dt 1/60" % args.iters
+
+
 def kernel_bytes(n, p, c):
     """algorithmic HBM bytes per launch of the kernels that can dominate a step
     (records read once, written once: body 80, pair 16, contact 112,
@@ -57,6 +68,8 @@ def run(args):
         raise SystemExit("bench.py needs a HIP device")
     if args.config == "pile":
         sc = scenes.convex_pile(args.nx, args.ny, args.nz)
+    elif args.config == "batched":
+        sc = scenes.batched_scenes(args.scenes)
     else:
         sc = scenes.cube_grid(10, 10, 10)
     n_dyn = int((sc.mass > 0).sum())
@@ -107,7 +120,7 @@ def run(args):
         if t.get("kernel") == dom:
             traffic = t.get("bytes_per_launch")
     out = {
-        "metric": "body-steps/s at 128k convexes",
+        "metric": "body-steps/s at 128k convexes; PGS iters to residual 1e-4, learned vs plain",
         "value": round(value, 1),
         "unit": "body-steps/s",
         "n_gpus": 1,
@@ -119,8 +132,7 @@ def run(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "convex pile %d dynamic bodies (boxes, 16/32-point hulls), grid broadphase, "
-                               "%d PGS iterations, dt 1/60" % (n_dyn, args.iters),
+        "config": {"workload": workload_name(args, n_dyn),
                    "bodies": n_dyn, "pairs": n_pairs, "contacts": n_contacts, "pgs_iterations": args.iters,
                    "setup_s": round(setup_s, 2)},
         "stages_ms": {k: round(float(v), 4) for k, v in st.items()},
@@ -130,12 +142,85 @@ def run(args):
                      "step_algorithmic_bytes": int(total_bytes),
                      "step_frac": round(total_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
     }
+    if not args.no_learned:
+        out["warm_start"] = learned_eval(w, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args)
         if out["cpu_baseline"]["value"] > 0:
             out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     w.close()
     return out
+
+
+def learned_eval(w, args):
+    """PGS iterations to a residual, learned warm start against plain PGS.
+
+    From the pile's post-benchmark state: log every contact point of
+    --learn-collect-steps steps of plain cold-start PGS at 20 iterations, train
+    the network on the device (--learn-train-steps Adam steps of 8192 rows),
+    then for cold start (the reference), previous-frame and learned warm start
+    and each iteration count run --learn-eval-steps steps from that state (no
+    impulse history at the first) and average the last normal sweep's L2
+    residual over the steps after the first --learn-skip.  Reported: the
+    iterations each mode needs to reach plain 20-iteration PGS's residual and
+    1e-4, and graph-timed ms/step at those counts."""
+    from bullet3_amd import warmstart as ws
+    from bullet3_amd.learned import WarmStartNet
+    dt = 1.0 / 60.0
+    state = w.bodies()
+    sp0 = w.solver_params()
+    t0 = time.time()
+    X, T, rows = ws.collect(w, args.learn_collect_steps, args.learn_capacity, dt=dt, mode=ws.COLD, iterations=20)
+    collect_s = time.time() - t0
+    net = WarmStartNet(24, 3, max_train_rows=8192, seed=1)
+    t0 = time.time()
+    hist = ws.train(net, X, T, rows, args.learn_train_steps, batch=8192, lr=1e-3, lr_final=1e-4)
+    train_s = time.time() - t0
+    X.free()
+    T.free()
+    iters = [2, 3, 4, 5, 6, 8, 10, 12, 14, 16, 18, 20]
+    curves = {}
+    for mode in (ws.COLD, ws.PREVIOUS, ws.LEARNED):
+        curves[mode] = {}
+        for k in iters:
+            curves[mode][k], _ = ws.run_residuals(w, state, mode, k, args.learn_eval_steps, args.learn_skip, dt=dt,
+                                                  net=net)
+    name = ws.MODE_NAMES
+    targets = {"plain_cold_20": curves[ws.COLD][20], "plain_warm_20": curves[ws.PREVIOUS][20], "1e-4": 1e-4}
+    k_to = {t: {name[m]: ws.iterations_to(curves[m], r) for m in curves} for t, r in targets.items()}
+    ms = {}
+
+    def timed(mode, k):
+        key = "%s_%d" % (name[mode], k)
+        if key not in ms:
+            ms[key] = round(ws.time_steps(w, state, mode, k, args.learn_time_steps, dt=dt), 4)
+        return ms[key]
+
+    speed = {}
+    for t in ("plain_cold_20", "plain_warm_20"):
+        base = ws.COLD if t == "plain_cold_20" else ws.PREVIOUS
+        k_l = k_to[t]["learned"]
+        if k_l:
+            speed[t] = {"iteration_ratio": round(20.0 / k_l, 3),
+                        "steps_per_s_ratio": round(timed(base, 20) / timed(ws.LEARNED, k_l), 3)}
+        else:
+            speed[t] = None
+    w.set_bodies(state)
+    w.clear_warm_history()
+    w.set_solver(sp0)
+    net.close()
+    return {
+        "residual": "L2 of the last normal sweep, mean over steps",
+        "targets": {t: float("%.4e" % r) for t, r in targets.items()},
+        "iterations_to": k_to,
+        "learned_vs_plain_20": speed,
+        "ms_per_step": ms,
+        "curves": {name[m]: {str(k): float("%.4e" % v) for k, v in curves[m].items()} for m in curves},
+        "training": {"rows": int(rows), "collect_steps": args.learn_collect_steps, "collect_s": round(collect_s, 2),
+                     "adam_steps": args.learn_train_steps, "batch": 8192, "train_s": round(train_s, 2),
+                     "loss_first_100": float(np.mean(hist[:100])) if len(hist) else None,
+                     "loss_last_100": float(np.mean(hist[-100:])) if len(hist) else None},
+    }
 
 
 def cpu_baseline(w, args):
@@ -176,12 +261,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--friction", type=float, default=None)
-    ap.add_argument("--config", default="pile", choices=["pile", "cubes"])
+    ap.add_argument("--config", default="pile", choices=["pile", "cubes", "batched"])
+    ap.add_argument("--scenes", type=int, default=4096, help="scenes of the batched workload")
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--ny", type=int, default=32)
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start evaluation")
+    ap.add_argument("--learn-collect-steps", type=int, default=40)
+    ap.add_argument("--learn-capacity", type=int, default=60_000_000)
+    ap.add_argument("--learn-train-steps", type=int, default=3000)
+    ap.add_argument("--learn-eval-steps", type=int, default=8)
+    ap.add_argument("--learn-skip", type=int, default=3)
+    ap.add_argument("--learn-time-steps", type=int, default=30)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a plain copy kernel of known "
                          "size, to scale rocprof FETCH_SIZE/WRITE_SIZE)")

@@ -29,19 +29,23 @@ __device__ __forceinline__ unsigned short f2bf(float f)
 	return (unsigned short)(u >> 16);
 }
 
-/* GELU, tanh form: 0.5 x (1 + tanh(c (x + 0.044715 x^3))), c = sqrt(2/pi) */
+/* GELU, tanh form: 0.5 x (1 + tanh(u)), u = c (x + 0.044715 x^3), c = sqrt(2/pi),
+ * evaluated through 0.5 (1 + tanh(u)) = 1 / (1 + exp(-2u)) with the hardware
+ * exponential and a fast reciprocal (a few VALU operations instead of a full
+ * tanh).  Derivative: s + 2 x s (1 - s) c (1 + 3 * 0.044715 x^2), s = that
+ * sigmoid. */
 #define GELU_C 0.7978845608028654f
 #define GELU_A 0.044715f
-__device__ __forceinline__ float gelu(float x)
+__device__ __forceinline__ float gelu_sig(float x)
 {
 	float u = GELU_C * (x + GELU_A * x * x * x);
-	return 0.5f * x * (1.f + tanhf(u));
+	return __frcp_rn(1.f + __expf(-2.f * u));
 }
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
 __device__ __forceinline__ float gelu_grad(float x)
 {
-	float u = GELU_C * (x + GELU_A * x * x * x);
-	float th = tanhf(u);
-	return 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * GELU_C * (1.f + 3.f * GELU_A * x * x);
+	float s = gelu_sig(x);
+	return s + 2.f * x * s * (1.f - s) * GELU_C * (1.f + 3.f * GELU_A * x * x);
 }
 
 __device__ __forceinline__ void wave_lds_sync()
@@ -110,7 +114,9 @@ struct MlpOps
 	const float* b2;
 	const float* b3;
 };
-__global__ __launch_bounds__(256) void k_mlp_forward(int M, int Mp, const int* __restrict__ mDev, int mMul, int inDim,
+#define FWD_WAVES 8
+#define FWD_THREADS (64 * FWD_WAVES)
+__global__ __launch_bounds__(FWD_THREADS) void k_mlp_forward(int M, int Mp, const int* __restrict__ mDev, int mMul, int inDim,
 													 int outDim, const float* __restrict__ X, int ldx, MlpOps net,
 													 float* __restrict__ Y, int ldy, int save,
 													 unsigned short* __restrict__ Xb, float* __restrict__ Z1,
@@ -126,19 +132,19 @@ __global__ __launch_bounds__(256) void k_mlp_forward(int M, int Mp, const int* _
 	__shared__ __attribute__((aligned(16))) unsigned short sW1[MLP_H * LDS_K1];
 	__shared__ __attribute__((aligned(16))) unsigned short sW2[MLP_H * LDS_H];
 	__shared__ __attribute__((aligned(16))) unsigned short sW3[MLP_K3 * LDS_H];
-	__shared__ __attribute__((aligned(16))) unsigned short sAct[4 * 32 * LDS_H];
+	__shared__ __attribute__((aligned(16))) unsigned short sAct[FWD_WAVES * 32 * LDS_H];
 	int t = threadIdx.x;
-	for (int q = t; q < MLP_H * (MLP_K1 / 8); q += 256)
+	for (int q = t; q < MLP_H * (MLP_K1 / 8); q += FWD_THREADS)
 	{
 		int row = q / (MLP_K1 / 8), c8 = q - row * (MLP_K1 / 8);
 		*(uint4*)(sW1 + row * LDS_K1 + 8 * c8) = ((const uint4*)net.w1t)[q];
 	}
-	for (int q = t; q < MLP_H * (MLP_H / 8); q += 256)
+	for (int q = t; q < MLP_H * (MLP_H / 8); q += FWD_THREADS)
 	{
 		int row = q / (MLP_H / 8), c8 = q - row * (MLP_H / 8);
 		*(uint4*)(sW2 + row * LDS_H + 8 * c8) = ((const uint4*)net.w2t)[q];
 	}
-	for (int q = t; q < MLP_K3 * (MLP_H / 8); q += 256)
+	for (int q = t; q < MLP_K3 * (MLP_H / 8); q += FWD_THREADS)
 	{
 		int row = q / (MLP_H / 8), c8 = q - row * (MLP_H / 8);
 		*(uint4*)(sW3 + row * LDS_H + 8 * c8) = ((const uint4*)net.w3t)[q];
@@ -146,7 +152,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(int M, int Mp, const int* _
 	__syncthreads();
 	int wave = t >> 6, lane = t & 63, r = lane & 31, h = lane >> 5;
 	unsigned short* act = sAct + wave * 32 * LDS_H;
-	for (int row0 = (blockIdx.x * 4 + wave) * 32; row0 < Mp; row0 += gridDim.x * 4 * 32)
+	for (int row0 = (blockIdx.x * FWD_WAVES + wave) * 32; row0 < Mp; row0 += gridDim.x * FWD_WAVES * 32)
 	{
 		/* input tile -> bf16 [32][K1] */
 		for (int q = lane; q < 32 * MLP_K1; q += 64)
@@ -431,10 +437,10 @@ static void forward(const b3hMlp& m, int rows, const float* X, int ldx, float* Y
 	if (Mp == 0) return;
 	MlpOps net = {m.w1t, m.w2t, m.w3t, m.params + MLP_OFF_B1, m.params + MLP_OFF_B2, m.params + MLP_OFF_B3};
 	int tiles = Mp / 32;
-	int blocks = (tiles + 3) / 4;
-	if (blocks > 1024) blocks = 1024;
+	int blocks = (tiles + FWD_WAVES - 1) / FWD_WAVES;
+	if (blocks > 512) blocks = 512;
 	const b3hMlpWork& w = m.wk;
-	hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(256), 0, s, rows, Mp, (const int*)nullptr, 0, m.inDim,
+	hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(FWD_THREADS), 0, s, rows, Mp, (const int*)nullptr, 0, m.inDim,
 					   m.outDim, X, ldx, net, Y, ldy, save ? 1 : 0, save ? w.xb : nullptr, save ? w.z1 : nullptr, save ? w.h1 : nullptr,
 					   save ? w.z2 : nullptr, save ? w.h2 : nullptr);
 }
@@ -449,10 +455,10 @@ void b3h_mlp_run_forward_dev(const b3hMlp& m, const int* countDev, int rowsPerIt
 {
 	MlpOps net = {m.w1t, m.w2t, m.w3t, m.params + MLP_OFF_B1, m.params + MLP_OFF_B2, m.params + MLP_OFF_B3};
 	int tiles = (maxRows + 31) / 32;
-	int blocks = (tiles + 3) / 4;
-	if (blocks > 1024) blocks = 1024;
+	int blocks = (tiles + FWD_WAVES - 1) / FWD_WAVES;
+	if (blocks > 512) blocks = 512;
 	if (blocks < 1) blocks = 1;
-	hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(256), 0, s, 0, 0, countDev, rowsPerItem, m.inDim, m.outDim, X,
+	hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(FWD_THREADS), 0, s, 0, 0, countDev, rowsPerItem, m.inDim, m.outDim, X,
 					   ldx, net, Y, ldy, 0, (unsigned short*)nullptr, (float*)nullptr, (unsigned short*)nullptr,
 					   (float*)nullptr, (unsigned short*)nullptr);
 }

@@ -282,6 +282,61 @@ public:
 	{
 		b3hip::ok(b3h_pipeline_step_graph(m_p, deltaTime, count));
 	}
+	/* warm start (no counterpart in the reference, which starts every solve
+	 * from zero impulses): b3hSolverParams::m_warmStartMode 0 cold, 1 previous
+	 * frame, 2 learned with the network given here */
+	bool setWarmStartModel(b3hMlp* net) { return b3hip::ok(b3h_pipeline_set_warm_start_model(m_p, net)); }
+	void clearWarmStartHistory() { b3hip::ok(b3h_pipeline_clear_warm_start_history(m_p)); }
+	/* contact logging for training the network (device rows [cap][24] and [cap][3]) */
+	bool setContactLog(float* featuresDev, float* targetsDev, int capacityRows)
+	{
+		return b3hip::ok(b3h_pipeline_set_contact_log(m_p, featuresDev, targetsDev, capacityRows));
+	}
+	int getContactLogCount()
+	{
+		int n = 0;
+		b3hip::ok(b3h_pipeline_get_contact_log_count(m_p, &n));
+		return n;
+	}
+	/* PGS residuals per sweep (after setTrackResidual(true)) */
+	void setTrackResidual(bool on) { b3hip::ok(b3h_pipeline_set_track_residual(m_p, on ? 1 : 0)); }
+	bool getResidualsL2(float* out, int n) { return b3hip::ok(b3h_pipeline_get_residuals_l2(m_p, out, n)); }
+};
+
+/* The learned warm start's network (3-layer MLP, GELU, bf16 weights, fp32
+ * accumulation on the matrix cores); no counterpart in the reference. */
+class b3WarmStartNetwork
+{
+	b3hMlp* m_m = nullptr;
+
+public:
+	explicit b3WarmStartNetwork(int inDim = 24, int outDim = 3, int maxTrainRows = 8192, unsigned seed = 1,
+								void* hipStream = nullptr)
+	{
+		b3hip::ok(b3h_mlp_create(inDim, outDim, maxTrainRows, seed, hipStream, &m_m));
+	}
+	~b3WarmStartNetwork() { b3h_mlp_destroy(m_m); }
+	b3WarmStartNetwork(const b3WarmStartNetwork&) = delete;
+	b3WarmStartNetwork& operator=(const b3WarmStartNetwork&) = delete;
+	b3hMlp* handle() const { return m_m; }
+	int numParams() const
+	{
+		int n = 0;
+		b3hip::ok(b3h_mlp_num_params(m_m, &n));
+		return n;
+	}
+	bool setParams(const float* host) { return b3hip::ok(b3h_mlp_set_params(m_m, host)); }
+	bool getParams(float* host) { return b3hip::ok(b3h_mlp_get_params(m_m, host)); }
+	bool forward(const float* xDev, int ldx, int rows, float* yDev, int ldy)
+	{
+		return b3hip::ok(b3h_mlp_forward(m_m, xDev, ldx, rows, yDev, ldy));
+	}
+	bool train(const float* xDev, int ldx, const float* tDev, int ldt, int rows, int batchRows, int steps,
+			   float learningRate, unsigned seed, float* lossHistory)
+	{
+		return b3hip::ok(b3h_mlp_train(m_m, xDev, ldx, tDev, ldt, rows, batchRows, steps, learningRate, seed,
+									   lossHistory));
+	}
 };
 
 #endif
