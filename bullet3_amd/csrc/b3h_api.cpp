@@ -1097,8 +1097,8 @@ extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const 
 		(r = dalloc(&p->sb.batchSizes, B3H_N_CELLS)) || (r = dalloc(&p->sb.sortKeys, 4 * (size_t)mc + 64)) ||
 		(r = dalloc(&p->sb.constraints, mc)) || (r = dalloc(&p->sb.sortedContacts, mc)) || (r = dalloc(&p->sb.resid, 4 * 64)) ||
 		(r = dalloc(&p->sb.cellBodies, 2 * (size_t)mc + 64)) || (r = dalloc(&p->sb.cellBodyCount, B3H_N_CELLS)) ||
-		(r = dalloc(&p->sb.cslot, 2 * (size_t)mc + 2)) || (r = dalloc(&p->sb.recN, 21 * (size_t)mc)) ||
-		(r = dalloc(&p->sb.recF, 16 * (size_t)mc)) || (r = dalloc(&p->sb.levelOrder, mc)) ||
+		(r = dalloc(&p->sb.cslot, 2 * (size_t)mc + 2)) || (r = dalloc(&p->sb.recN, 10 * (size_t)mc)) ||
+		(r = dalloc(&p->sb.recF, 8 * (size_t)mc)) || (r = dalloc(&p->sb.levelOrder, mc)) ||
 		(r = dalloc(&p->sb.levelStart, (size_t)mc + B3H_N_CELLS + 8)) || (r = dalloc(&p->sb.cellLevels, B3H_N_CELLS)) ||
 		(r = dalloc(&p->sb.lastClass, (size_t)cfg->m_maxConvexBodies + 16)) ||
 		(r = dalloc(&p->sb.residSnap, 6 * (size_t)mc)) || (r = dalloc(&p->sb.residL2, 4 * B3H_MAX_ITERATIONS)))

@@ -121,8 +121,8 @@ struct b3hSolverBuffers
 	int* cellBodyCount;   /* 256 */
 	int* cslot;           /* per constraint: local slots of bodyA, bodyB in its cell's list (-1: static) */
 	int pgsBodyCap;       /* < 0: the LDS budget; otherwise a lower cap on cached bodies per cell */
-	b3hFloat4* recN;      /* per constraint: 21 float4 of normal-row invariants */
-	b3hFloat4* recF;      /* per constraint: 16 float4 of friction-row invariants */
+	b3hFloat4* recN;      /* per constraint: 10 float4 of normal-row invariants */
+	b3hFloat4* recF;      /* per constraint: 8 float4 of friction-row invariants */
 	int* levelOrder;      /* per cell region [off, off+n): constraints grouped by dependency level */
 	int* levelStart;      /* per cell region [off+cell, off+cell+L+1) */
 	int* cellLevels;      /* 256: levels per cell, -1 = global walk */

@@ -443,7 +443,8 @@ __device__ __forceinline__ void write_back(b3hRigidBodyData* body, float invMass
 }
 
 /* phase 0: normal rows (solveContact.cl:259-377); 1: friction rows
- * (solveFriction.cl:288-408); 2: warm start (apply initial impulses once) */
+ * (solveFriction.cl:288-408); 2 / 3: warm start of the normal / friction rows
+ * (apply the initial impulses once) */
 __device__ void solve_one(b3hContactConstraint4* __restrict__ csp, b3hRigidBodyData* __restrict__ bodies,
 						  const b3hInertiaData* __restrict__ inertias, int phase, float friction, const float* warm,
 						  float* resid)
@@ -490,7 +491,10 @@ __device__ void solve_one(b3hContactConstraint4* __restrict__ csp, b3hRigidBodyD
 			}
 			apply_impulse(linear, ang0, ang1, rl, imA, iiA, imB, iiB, va, vb);
 		}
-		if (phase == 2 && !(cs.m_fJacCoeffInv[0] == 0))
+	}
+	else if (phase == 3)
+	{
+		if (!(cs.m_fJacCoeffInv[0] == 0))
 		{
 			float sum = 0;
 			for (int j = 0; j < 4; j++) sum += cs.m_appliedRambdaDt[j];
@@ -550,15 +554,10 @@ __device__ void solve_one(b3hContactConstraint4* __restrict__ csp, b3hRigidBodyD
 			vb.av = sub4(vb.av, smul4(angNB * 0.1f, n));
 		}
 	}
-	if (phase == 0)
+	if (phase == 0 || phase == 2)
 		for (int ic = 0; ic < 4; ic++) csp->m_appliedRambdaDt[ic] = cs.m_appliedRambdaDt[ic];
-	else if (phase == 1)
-		for (int i = 0; i < 2; i++) csp->m_fAppliedRambdaDt[i] = cs.m_fAppliedRambdaDt[i];
 	else
-	{
-		for (int ic = 0; ic < 4; ic++) csp->m_appliedRambdaDt[ic] = cs.m_appliedRambdaDt[ic];
 		for (int i = 0; i < 2; i++) csp->m_fAppliedRambdaDt[i] = cs.m_fAppliedRambdaDt[i];
-	}
 	write_back(bodies + aIdx, imA, va);
 	write_back(bodies + bIdx, imB, vb);
 	if (resid)
@@ -698,15 +697,19 @@ __global__ __launch_bounds__(256) void k_cell_bodies(const int* __restrict__ cel
  * During the solve only velocities and impulses change, so every vector of a
  * row that depends on positions, masses and inertias alone is computed once per
  * step with exactly the operations the sweep would use (solveContact.cl:259-303,
- * solveFriction.cl:288-408):
- *   normal record  lin, P0 = imA*lin, P1 = imB*(-lin), per point (ang0, ang1,
- *                  Q0 = iiA*ang0, Q1 = iiB*ang1), jacCoeffInv, b      21 float4
- *   friction rec.  per direction (lin, ang0, ang1, Q0, Q1, P0, P1), n, (fJac0,
- *                  fJac1, twist-damping flag)                         16 float4
- * The sweep then evaluates relVel = dot3(lin,vA)+dot3(ang0,wA)+dot3(-lin,vB)+dot3(ang1,wB)
- * and v += P*rl, w += Q*rl: the same roundings as recomputing them. */
-#define PGS_NREC 20
-#define PGS_FREC 13
+ * solveFriction.cl:288-408), packed without padding:
+ *   normal record   lin = (n, imA) | 4 points x (ang0.xyz, ang1.xyz) in 6
+ *                   float4 | jacCoeffInv | b | meta (imB, slotA, slotB, index)
+ *                                                                     10 float4
+ *   friction record 2 directions x (lin.xyz, ang0.xyz, ang1.xyz), n.xyz in 6
+ *                   float4 | (fJac0, fJac1, twist-damping flag, imA) | meta
+ *                                                                      8 float4
+ * The sweep evaluates relVel = dot3(lin,vA)+dot3(ang0,wA)+dot3(-lin,vB)+dot3(ang1,wB)
+ * and v += imA*lin*rl, w += (iiA*ang0)*rl with the bodies' inverse inertias
+ * cached in LDS: the same operations, so the same roundings, as recomputing
+ * everything from the constraint. */
+#define PGS_NREC 10
+#define PGS_FREC 8
 #define PGS_GATHER 8
 #define PGS_THREADS 256
 #define PGS_NSLOT (PGS_NREC + 1) /* record + applied normal impulses */
@@ -725,55 +728,66 @@ __global__ __launch_bounds__(256) void k_pgs_prep(const int* __restrict__ nPtr, 
 		int a = (int)c.m_bodyA, b = (int)c.m_bodyB;
 		F4 posA = bodies[a].m_pos, posB = bodies[b].m_pos;
 		float imA = bodies[a].m_invMass, imB = bodies[b].m_invMass;
-		const b3hMat3x3 iiA = inertias[a].m_invInertiaWorld;
-		const b3hMat3x3 iiB = inertias[b].m_invInertiaWorld;
 		int sA = cslot[2 * (size_t)i], sB = cslot[2 * (size_t)i + 1];
 		F4 meta = f4(imB, ibits(sA), ibits(sB), ibits(i));
 		F4* N = recN + (size_t)i * PGS_NREC;
 		F4 n = neg4(c.m_linear);
 		N[0] = f4(c.m_linear.x, c.m_linear.y, c.m_linear.z, imA);
+		float pk[24];
 		for (int ic = 0; ic < 4; ic++)
 		{
 			F4 r0 = sub4(c.m_worldPos[ic], posA);
 			F4 r1 = sub4(c.m_worldPos[ic], posB);
 			F4 ang0 = neg4(cross3(r0, n));
 			F4 ang1 = cross3(r1, n);
-			N[1 + 4 * ic] = ang0;
-			N[2 + 4 * ic] = ang1;
-			N[3 + 4 * ic] = mtMul1(iiA, ang0);
-			N[4 + 4 * ic] = mtMul1(iiB, ang1);
+			pk[6 * ic + 0] = ang0.x;
+			pk[6 * ic + 1] = ang0.y;
+			pk[6 * ic + 2] = ang0.z;
+			pk[6 * ic + 3] = ang1.x;
+			pk[6 * ic + 4] = ang1.y;
+			pk[6 * ic + 5] = ang1.z;
 		}
-		N[17] = f4(c.m_jacCoeffInv[0], c.m_jacCoeffInv[1], c.m_jacCoeffInv[2], c.m_jacCoeffInv[3]);
-		N[18] = f4(c.m_b[0], c.m_b[1], c.m_b[2], c.m_b[3]);
-		N[19] = meta;
+		for (int q = 0; q < 6; q++) N[1 + q] = f4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
+		N[7] = f4(c.m_jacCoeffInv[0], c.m_jacCoeffInv[1], c.m_jacCoeffInv[2], c.m_jacCoeffInv[3]);
+		N[8] = f4(c.m_b[0], c.m_b[1], c.m_b[2], c.m_b[3]);
+		N[9] = meta;
 		F4* Fr = recF + (size_t)i * PGS_FREC;
 		F4 t[2];
 		plane_space(n, &t[0], &t[1]);
 		F4 r0 = sub4(c.m_center, posA), r1 = sub4(c.m_center, posB);
+		float fk[24];
 		for (int k = 0; k < 2; k++)
 		{
-			F4 lin = f4(-t[k].x, -t[k].y, -t[k].z, 0.f);
 			F4 ang0 = neg4(cross3(r0, t[k]));
 			F4 ang1 = cross3(r1, t[k]);
-			Fr[5 * k + 0] = lin;
-			Fr[5 * k + 1] = ang0;
-			Fr[5 * k + 2] = ang1;
-			Fr[5 * k + 3] = mtMul1(iiA, ang0);
-			Fr[5 * k + 4] = mtMul1(iiB, ang1);
+			fk[9 * k + 0] = -t[k].x;
+			fk[9 * k + 1] = -t[k].y;
+			fk[9 * k + 2] = -t[k].z;
+			fk[9 * k + 3] = ang0.x;
+			fk[9 * k + 4] = ang0.y;
+			fk[9 * k + 5] = ang0.z;
+			fk[9 * k + 6] = ang1.x;
+			fk[9 * k + 7] = ang1.y;
+			fk[9 * k + 8] = ang1.z;
 		}
-		Fr[10] = n;
+		fk[18] = n.x;
+		fk[19] = n.y;
+		fk[20] = n.z;
+		fk[21] = n.w; /* the twist damping subtracts the whole float4 (w included) */
+		fk[22] = fk[23] = 0.f;
+		for (int q = 0; q < 6; q++) Fr[q] = f4(fk[4 * q], fk[4 * q + 1], fk[4 * q + 2], fk[4 * q + 3]);
 		F4 ab = nrm3(sub4(posB, posA));
 		F4 ac = nrm3(sub4(c.m_center, posA));
 		bool damp = dot3(ab, ac) > 0.95f || (imA == 0.f || imB == 0.f);
-		Fr[11] = f4(c.m_fJacCoeffInv[0], c.m_fJacCoeffInv[1], damp ? 1.f : 0.f, imA);
-		Fr[12] = meta;
+		Fr[6] = f4(c.m_fJacCoeffInv[0], c.m_fJacCoeffInv[1], damp ? 1.f : 0.f, imA);
+		Fr[7] = meta;
 	}
 }
 
 /* LDS bytes the level sweep needs for a cell of n constraints and nb bodies */
 __host__ __device__ __forceinline__ size_t pgs_level_lds(int nb, int n)
 {
-	return (size_t)(nb + 1) * 36 + (size_t)n * 4 + (size_t)(n + 1) * 4;
+	return (size_t)(nb + 1) * 36 + (size_t)(nb + 2) * 48 + (size_t)n * 4 + (size_t)(n + 1) * 4;
 }
 
 /* ---------------- per-cell dependency levels ----------------
@@ -940,8 +954,32 @@ __device__ __forceinline__ void slot_fetch(Slot<PHASE>& s, int idx, const F4* __
 	}
 }
 
+/* float j of a packed record (j is a constant after unrolling) */
+__device__ __forceinline__ float rc(const F4* r, int j)
+{
+	const F4 v = r[j >> 2];
+	switch (j & 3)
+	{
+	case 0: return v.x;
+	case 1: return v.y;
+	case 2: return v.z;
+	default: return v.w;
+	}
+}
+__device__ __forceinline__ F4 rc3(const F4* r, int j) { return f4(rc(r, j), rc(r, j + 1), rc(r, j + 2), 0.f); }
+/* the inverse world inertia of a cached body (static bodies: the zero slot nb) */
+__device__ __forceinline__ b3hMat3x3 lds_inertia(const F4* sI, int slot, int nb)
+{
+	const F4* m = sI + 3 * (slot >= 0 ? slot : nb);
+	b3hMat3x3 r;
+	r.m_row[0] = m[0];
+	r.m_row[1] = m[1];
+	r.m_row[2] = m[2];
+	return r;
+}
+
 template <int PHASE>
-__device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4* sAv,
+__device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4* sAv, const F4* sI, int nb,
 										  b3hContactConstraint4* __restrict__ cs, float friction, float& maxD,
 										  float& maxL)
 {
@@ -958,9 +996,11 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 	va.av = sA >= 0 ? sAv[sA] : zero;
 	vb.lv = sB >= 0 ? sLv[sB] : zero;
 	vb.av = sB >= 0 ? sAv[sB] : zero;
+	const b3hMat3x3 iiA = lds_inertia(sI, sA, nb);
+	const b3hMat3x3 iiB = lds_inertia(sI, sB, nb);
 	if (PHASE == 1)
 	{
-		F4 fj = S.r[11];
+		F4 fj = S.r[6];
 		if (fj.x == 0) return; /* solveFriction.cl: rows without friction are skipped */
 		float imA = fj.w;
 		F4 lamN = S.r[PGS_FREC];
@@ -976,7 +1016,8 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 #pragma unroll
 		for (int k = 0; k < 2; k++)
 		{
-			float rl = rel_vel(S.r[5 * k], S.r[5 * k + 1], S.r[5 * k + 2], va, vb);
+			F4 lin = rc3(S.r, 9 * k), a0 = rc3(S.r, 9 * k + 3), a1 = rc3(S.r, 9 * k + 6);
+			float rl = rel_vel(lin, a0, a1, va, vb);
 			rl *= fjac[k];
 			float prev = fl[k];
 			float upd = prev;
@@ -987,11 +1028,11 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 			fl[k] = upd;
 			maxD = fmaxf(maxD, fabsf(rl));
 			maxL = fmaxf(maxL, fabsf(upd));
-			apply_pre(S.r[5 * k], imA, imB, S.r[5 * k + 3], S.r[5 * k + 4], rl, va, vb);
+			apply_pre(lin, imA, imB, mtMul1(iiA, a0), mtMul1(iiB, a1), rl, va, vb);
 		}
 		if (fj.z != 0.f)
 		{
-			F4 n = S.r[10];
+			F4 n = f4(rc(S.r, 18), rc(S.r, 19), rc(S.r, 20), rc(S.r, 21));
 			float angNA = dot3(n, va.av);
 			float angNB = dot3(n, vb.av);
 			va.av = sub4(va.av, smul4(angNA * 0.1f, n));
@@ -1006,7 +1047,7 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 		float imA = lin.w;
 		F4 lamv = S.r[PGS_NREC];
 		float lam[4] = {lamv.x, lamv.y, lamv.z, lamv.w};
-		F4 jv = S.r[17], bv = S.r[18];
+		F4 jv = S.r[7], bv = S.r[8];
 		float jac[4] = {jv.x, jv.y, jv.z, jv.w};
 		float bb[4] = {bv.x, bv.y, bv.z, bv.w};
 #pragma unroll
@@ -1014,7 +1055,8 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 		{
 			if (jac[ic] != 0.f)
 			{
-				float rl = rel_vel(lin, S.r[1 + 4 * ic], S.r[2 + 4 * ic], va, vb) + bb[ic];
+				F4 a0 = rc3(S.r, 4 + 6 * ic), a1 = rc3(S.r, 7 + 6 * ic);
+				float rl = rel_vel(lin, a0, a1, va, vb) + bb[ic];
 				rl *= jac[ic];
 				float prev = lam[ic];
 				float upd = prev;
@@ -1025,7 +1067,7 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 				lam[ic] = upd;
 				maxD = fmaxf(maxD, fabsf(rl));
 				maxL = fmaxf(maxL, fabsf(upd));
-				apply_pre(lin, imA, imB, S.r[3 + 4 * ic], S.r[4 + 4 * ic], rl, va, vb);
+				apply_pre(lin, imA, imB, mtMul1(iiA, a0), mtMul1(iiB, a1), rl, va, vb);
 			}
 		}
 #pragma unroll
@@ -1043,15 +1085,21 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 	}
 }
 
-/* warm start (phase 2, once per step): applies the given impulses in level
- * order, reading the records directly */
+/* warm start, in level order, reading the records directly: PART 0 sets and
+ * applies the normal impulses before the first normal sweep, PART 1 the
+ * friction impulses (clamped by the normal impulses the normal sweeps left)
+ * before the first friction sweep.  Each phase thus starts from its own guess
+ * and keeps the cold start's fixed point (all normal sweeps run before the
+ * friction sweeps). */
+template <int PART>
 __device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restrict__ recF, F4* sLv, F4* sAv,
-						 b3hContactConstraint4* __restrict__ cs, float friction, const float* __restrict__ warm)
+						 const F4* sI, int nb, b3hContactConstraint4* __restrict__ cs, float friction,
+						 const float* __restrict__ warm)
 {
 	const F4 zero = f4(0.f, 0.f, 0.f, 0.f);
 	const F4* N = recN + (size_t)idx * PGS_NREC;
 	const F4* Fr = recF + (size_t)idx * PGS_FREC;
-	F4 meta = N[19];
+	F4 meta = N[9];
 	float imB = meta.x;
 	int sA = fbits(meta.y), sB = fbits(meta.z);
 	Vel va, vb;
@@ -1059,22 +1107,28 @@ __device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restr
 	va.av = sA >= 0 ? sAv[sA] : zero;
 	vb.lv = sB >= 0 ? sLv[sB] : zero;
 	vb.av = sB >= 0 ? sAv[sB] : zero;
+	const b3hMat3x3 iiA = lds_inertia(sI, sA, nb);
+	const b3hMat3x3 iiB = lds_inertia(sI, sB, nb);
 	F4 lin = N[0];
 	float imA = lin.w;
 	float lam[4];
 	for (int ic = 0; ic < 4; ic++) lam[ic] = cs[idx].m_appliedRambdaDt[ic];
-	F4 jv = N[17];
-	float jac[4] = {jv.x, jv.y, jv.z, jv.w};
-	for (int ic = 0; ic < 4; ic++)
+	if (PART == 0)
 	{
-		if (jac[ic] == 0.f) continue;
-		float rl = cl_max(warm[6 * (size_t)idx + ic], 0.f);
-		lam[ic] = rl;
-		apply_pre(lin, imA, imB, N[3 + 4 * ic], N[4 + 4 * ic], rl, va, vb);
+		F4 jv = N[7];
+		float jac[4] = {jv.x, jv.y, jv.z, jv.w};
+		for (int ic = 0; ic < 4; ic++)
+		{
+			if (jac[ic] == 0.f) continue;
+			float rl = cl_max(warm[6 * (size_t)idx + ic], 0.f);
+			lam[ic] = rl;
+			F4 a0 = rc3(N, 4 + 6 * ic), a1 = rc3(N, 7 + 6 * ic);
+			apply_pre(lin, imA, imB, mtMul1(iiA, a0), mtMul1(iiB, a1), rl, va, vb);
+		}
+		for (int ic = 0; ic < 4; ic++) cs[idx].m_appliedRambdaDt[ic] = lam[ic];
 	}
-	for (int ic = 0; ic < 4; ic++) cs[idx].m_appliedRambdaDt[ic] = lam[ic];
-	F4 fj = Fr[11];
-	if (!(fj.x == 0))
+	F4 fj = Fr[6];
+	if (PART == 1 && !(fj.x == 0))
 	{
 		float sum = 0;
 		for (int j = 0; j < 4; j++) sum += lam[j];
@@ -1083,7 +1137,8 @@ __device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restr
 		{
 			float l = cl_min(cl_max(warm[6 * (size_t)idx + 4 + k], -maxR), maxR);
 			cs[idx].m_fAppliedRambdaDt[k] = l;
-			apply_pre(Fr[5 * k], imA, imB, Fr[5 * k + 3], Fr[5 * k + 4], l, va, vb);
+			F4 fl = rc3(Fr, 9 * k), a0 = rc3(Fr, 9 * k + 3), a1 = rc3(Fr, 9 * k + 6);
+			apply_pre(fl, imA, imB, mtMul1(iiA, a0), mtMul1(iiB, a1), l, va, vb);
 		}
 	}
 	if (sA >= 0)
@@ -1167,37 +1222,38 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		return;
 	}
 	int nb = cellBodyCount[cell];
-	F4* sLv = smem4;          /* nb + 1 (the last is padding) */
+	F4* sI = smem4;           /* 3 (nb + 2) inverse world inertia rows: slot nb is zero (static bodies), nb + 1 padding */
+	F4* sLv = sI + 3 * (nb + 2); /* nb + 1 (the last is padding) */
 	F4* sAv = sLv + nb + 1;   /* nb + 1 */
 	int* sList = (int*)(sAv + nb + 1);
 	int* sOrder = sList + nb + 1;
 	const int* list = cellBodies + 2 * (size_t)start;
 	for (int k = t; k < nb; k += T) sList[k] = list[k];
+	if (t < 3) sI[3 * nb + t] = f4(0.f, 0.f, 0.f, 0.f);
 	__syncthreads();
-	/* gather the velocities, four bodies per thread per round with every load
-	 * issued before any is stored (lanes past the end read body 0, which
-	 * exists, and store nothing) */
-	for (int k0 = t; k0 < nb; k0 += 4 * T)
+	/* gather the velocities and inverse inertias, two bodies per thread per
+	 * round with every load issued before any is stored (lanes past the end
+	 * read body 0, which exists, and store into the padding slot) */
+	for (int k0 = t; k0 < nb; k0 += 2 * T)
 	{
-		int k1 = k0 + T, k2 = k0 + 2 * T, k3 = k0 + 3 * T;
+		int k1 = k0 + T;
 		int b0 = sList[k0];
 		int b1 = k1 < nb ? sList[k1] : 0;
-		int b2 = k2 < nb ? sList[k2] : 0;
-		int b3 = k3 < nb ? sList[k3] : 0;
 		F4 l0 = bodies[b0].m_linVel, a0 = bodies[b0].m_angVel;
 		F4 l1 = bodies[b1].m_linVel, a1 = bodies[b1].m_angVel;
-		F4 l2 = bodies[b2].m_linVel, a2 = bodies[b2].m_angVel;
-		F4 l3 = bodies[b3].m_linVel, a3 = bodies[b3].m_angVel;
-		/* out-of-range results go to the padding slot nb */
-		int s1 = k1 < nb ? k1 : nb, s2 = k2 < nb ? k2 : nb, s3 = k3 < nb ? k3 : nb;
+		b3hMat3x3 i0 = inertias[b0].m_invInertiaWorld;
+		b3hMat3x3 i1 = inertias[b1].m_invInertiaWorld;
+		int s1 = k1 < nb ? k1 : nb, i1s = k1 < nb ? k1 : nb + 1;
 		sLv[k0] = l0;
 		sAv[k0] = a0;
+		sI[3 * k0 + 0] = i0.m_row[0];
+		sI[3 * k0 + 1] = i0.m_row[1];
+		sI[3 * k0 + 2] = i0.m_row[2];
 		sLv[s1] = l1;
 		sAv[s1] = a1;
-		sLv[s2] = l2;
-		sAv[s2] = a2;
-		sLv[s3] = l3;
-		sAv[s3] = a3;
+		sI[3 * i1s + 0] = i1.m_row[0];
+		sI[3 * i1s + 1] = i1.m_row[1];
+		sI[3 * i1s + 2] = i1.m_row[2];
 	}
 	const int* order = levelOrder + start;
 	for (int k = t; k < n; k += T) sOrder[k] = order[k];
@@ -1206,32 +1262,40 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	const int* ls = sLs;
 	float maxD = 0.f, maxL = 0.f;
 	__syncthreads();
-	if (PHASE == 2)
+	if (PHASE >= 2)
 	{
 		for (int cl = 0; cl < L; cl++)
 		{
 			for (int k = ls[cl] + t; k < ls[cl + 1]; k += T)
-				pgs_warm(sOrder[k], recN, recF, sLv, sAv, cs, friction, warm);
+				pgs_warm<PHASE - 2>(sOrder[k], recN, recF, sLv, sAv, sI, nb, cs, friction, warm);
 			lds_barrier();
 		}
 	}
 	else
 	{
+		/* this thread's items: the current one and the next two are in flight
+		 * (records are fetched two items ahead of their solve) */
 		int l = 0, k = t;
 		while (l < L && k >= ls[l + 1]) l++;
-		Slot<PHASE> cur, nxt;
+		int l1 = l, k1 = k;
+		next_item(ls, L, T, l1, k1);
+		Slot<PHASE> cur, nxt, nx2;
 		if (l < L) slot_fetch<PHASE>(cur, sOrder[k], recN, recF, cs);
+		if (l1 < L) slot_fetch<PHASE>(nxt, sOrder[k1], recN, recF, cs);
 		for (int cl = 0; cl < L; cl++)
 		{
 			while (l == cl)
 			{
-				int l2 = l, k2 = k;
+				int l2 = l1, k2 = k1;
 				next_item(ls, L, T, l2, k2);
-				if (l2 < L) slot_fetch<PHASE>(nxt, sOrder[k2], recN, recF, cs);
-				pgs_solve<PHASE>(cur.r, sLv, sAv, cs, friction, maxD, maxL);
+				if (l2 < L) slot_fetch<PHASE>(nx2, sOrder[k2], recN, recF, cs);
+				pgs_solve<PHASE>(cur.r, sLv, sAv, sI, nb, cs, friction, maxD, maxL);
 				cur = nxt;
-				l = l2;
-				k = k2;
+				nxt = nx2;
+				l = l1;
+				k = k1;
+				l1 = l2;
+				k1 = k2;
 			}
 			lds_barrier();
 		}
@@ -1291,6 +1355,8 @@ int b3h_solver_module_init()
 		hipFuncSetAttribute((const void*)k_pgs_level<1>, hipFuncAttributeMaxDynamicSharedMemorySize, pgsBytes) !=
 			hipSuccess ||
 		hipFuncSetAttribute((const void*)k_pgs_level<2>, hipFuncAttributeMaxDynamicSharedMemorySize, pgsBytes) !=
+			hipSuccess ||
+		hipFuncSetAttribute((const void*)k_pgs_level<3>, hipFuncAttributeMaxDynamicSharedMemorySize, pgsBytes) !=
 			hipSuccess)
 		pgsBytes = 32768;
 	s_pgsLdsBytes = pgsBytes;
@@ -1426,11 +1492,26 @@ void b3h_solve_sweeps(b3hRigidBodyData* bodies, const b3hInertiaData* inertias, 
 						   sb.residSnap, -1, (double*)nullptr);
 	}
 	for (int phase = 0; phase < 2; phase++)
+	{
+		if (phase == 1)
+		{
+			if (evNormalDone) hipEventRecord(evNormalDone, s);
+			if (sb.warmImpulses)
+			{
+				/* the friction part of the warm start, after the normal sweeps */
+				for (int ib = 0; ib < B3H_N_CELL_BATCHES; ib++)
+					hipLaunchKernelGGL(k_pgs_level<3>, grid, block, pgsLds, s,
+									   B3H_PGS_ARGS(ib, (float*)nullptr, (const float*)sb.warmImpulses, 0));
+				if (l2)
+					hipLaunchKernelGGL(k_resid_l2, dim3(rb), dim3(256), 0, s, sb.numContacts,
+									   (const b3hContactConstraint4*)sb.constraints, sb.residSnap, -1,
+									   (double*)nullptr);
+			}
+		}
 		for (int it = 0; it < K; it++)
 		{
 			for (int ib = 0; ib < B3H_N_CELL_BATCHES; ib++)
 			{
-				if (phase == 1 && it == 0 && ib == 0 && evNormalDone) hipEventRecord(evNormalDone, s);
 				float* r = trackResidual ? sb.resid + 2 * (phase * K + it) : (float*)nullptr;
 				int fuse = (integrate && phase == 1 && it == K - 1) ? 1 : 0;
 				if (phase == 0)
@@ -1445,6 +1526,7 @@ void b3h_solve_sweeps(b3hRigidBodyData* bodies, const b3hInertiaData* inertias, 
 								   (const b3hContactConstraint4*)sb.constraints, sb.residSnap, phase,
 								   sb.residL2 + 2 * (phase * K + it));
 		}
+	}
 #undef B3H_PGS_ARGS
 }
 

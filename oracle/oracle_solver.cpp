@@ -306,17 +306,21 @@ static int cell_of(int wgIdx, int ib)
 }
 
 /* Warm start: constraint impulses start from initialImpulses (6 floats per
- * constraint: 4 normal rows, 2 friction rows) and the matching impulses are
- * applied to the bodies once, in solve order, before the first sweep. */
+ * constraint: 4 normal rows, 2 friction rows), applied to the bodies once in
+ * solve order: the normal rows before the first normal sweep (part 0), the
+ * friction rows -- clamped by the normal impulses the normal sweeps left --
+ * before the first friction sweep (part 1).  As the normal sweeps all run
+ * before the friction sweeps, each phase then starts from its own guess and
+ * keeps the fixed point of the cold start. */
 static void warm_start(b3hContactConstraint4& cs, const float* l0, float friction, b3hRigidBodyData* bodies,
-					   const b3hInertiaData* inertias)
+					   const b3hInertiaData* inertias, int part)
 {
 	b3hRigidBodyData& A = bodies[cs.m_bodyA];
 	b3hRigidBodyData& B = bodies[cs.m_bodyB];
 	const b3hMat3x3& iiA = inertias[cs.m_bodyA].m_invInertiaWorld;
 	const b3hMat3x3& iiB = inertias[cs.m_bodyB].m_invInertiaWorld;
 	Vel va = {A.m_linVel, A.m_angVel}, vb = {B.m_linVel, B.m_angVel};
-	for (int ic = 0; ic < 4; ic++)
+	for (int ic = 0; ic < 4 && part == 0; ic++)
 	{
 		if (cs.m_jacCoeffInv[ic] == 0.f) continue;
 		F4 r0 = sub4(cs.m_worldPos[ic], A.m_pos);
@@ -329,7 +333,7 @@ static void warm_start(b3hContactConstraint4& cs, const float* l0, float frictio
 		cs.m_appliedRambdaDt[ic] = l;
 		apply_impulse(linear, ang0, ang1, l, A.m_invMass, iiA, B.m_invMass, iiB, va, vb);
 	}
-	if (!(cs.m_fJacCoeffInv[0] == 0))
+	if (part == 1 && !(cs.m_fJacCoeffInv[0] == 0))
 	{
 		float sum = 0;
 		for (int j = 0; j < 4; j++) sum += cs.m_appliedRambdaDt[j];
@@ -407,22 +411,22 @@ extern "C" int b3o_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const
 		con[i].m_batchIdx = cs[i].m_batchIdx;
 		con[i].m_paddings = 0;
 	}
-	/* 5. PGS: warm start, all normal sweeps, then all friction sweeps */
+	/* 5. PGS: all normal sweeps, then all friction sweeps, each phase after
+	 * its part of the warm start */
 	int K = prm->m_numIterations;
-	if (initialImpulses)
-	{
-		for (int ib = 0; ib < N_CELL_BATCHES; ib++)
-			for (int wg = 0; wg < N_CELLS / N_CELL_BATCHES; wg++)
-			{
-				int cell = cell_of(wg, ib);
-				for (unsigned k = 0; k < counts[cell]; k++)
-				{
-					int i = (int)(offsets[cell] + k);
-					warm_start(con[i], initialImpulses + 6 * i, prm->m_frictionCoeff, bodies, inertias);
-				}
-			}
-	}
 	for (int phase = 0; phase < 2; phase++)
+	{
+		if (initialImpulses)
+			for (int ib = 0; ib < N_CELL_BATCHES; ib++)
+				for (int wg = 0; wg < N_CELLS / N_CELL_BATCHES; wg++)
+				{
+					int cell = cell_of(wg, ib);
+					for (unsigned k = 0; k < counts[cell]; k++)
+					{
+						int i = (int)(offsets[cell] + k);
+						warm_start(con[i], initialImpulses + 6 * i, prm->m_frictionCoeff, bodies, inertias, phase);
+					}
+				}
 		for (int it = 0; it < K; it++)
 		{
 			Resid res = {0.f, 0.f};
@@ -452,6 +456,7 @@ extern "C" int b3o_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const
 				residualsOut[phase * K + it] = res.maxDelta / denom;
 			}
 		}
+	}
 	if (sortedContactsOut) memcpy(sortedContactsOut, cs.data(), sizeof(b3hContact4Data) * n);
 	if (constraintsOut) memcpy(constraintsOut, con.data(), sizeof(b3hContactConstraint4) * n);
 	if (cellCountsOut) memcpy(cellCountsOut, counts, sizeof(counts));

@@ -153,18 +153,19 @@ def test_graph_replay_with_learned_warm_start():
 
 
 def test_warm_and_cold_share_the_fixed_point():
-    """only the initial iterate changes: with many sweeps the warm and cold
-    solves reach the same velocities, far closer than after a few sweeps.
-    Each warm solve starts from the impulse history the solve before it left
-    (the last real step, then the 3-sweep warm solve of the same state)."""
-    sc = cubes()
+    """only the initial iterate of each phase changes (normal impulses before
+    the normal sweeps, friction impulses before the friction sweeps): with many
+    sweeps the warm and cold solves reach the same velocities, far closer than
+    after a few sweeps.  Each warm solve starts from the impulse history the
+    solve before it left."""
+    sc = scenes.cube_grid(4, 2, 4, spacing=1.0, drop=0.0, seed=1)
     w = World(sc)
     w.set_warm_start(1)
     for _ in range(40):
         w.step(DT)
     state = w.bodies()
     out = {}
-    for iters in (3, 64):
+    for iters in (2, 64):
         for mode in (1, 0):
             w.set_bodies(state)
             sp = w.solver_params()
@@ -174,9 +175,9 @@ def test_warm_and_cold_share_the_fixed_point():
             contacts_then_solve(w)
             out[(mode, iters)] = w.bodies()
     v = {k: np.concatenate([o["linVel"][:, :3], o["angVel"][:, :3]], 1) for k, o in out.items()}
-    far = np.max(np.abs(v[(1, 3)] - v[(0, 3)]))
+    far = np.max(np.abs(v[(1, 2)] - v[(0, 2)]))
     near = np.max(np.abs(v[(1, 64)] - v[(0, 64)]))
-    print("warm-cold max velocity difference: 3 sweeps %.3e, 64 sweeps %.3e" % (far, near), flush=True)
+    print("warm-cold max velocity difference: 2 sweeps %.3e, 64 sweeps %.3e" % (far, near), flush=True)
     assert far > 0
     assert near < 0.1 * far
     w.close()
