@@ -156,7 +156,8 @@ def learned_eval(w, args):
     """PGS iterations to a residual, learned warm start against plain PGS.
 
     From the pile's post-benchmark state: log every contact point of
-    --learn-collect-steps steps of plain cold-start PGS at 20 iterations, train
+    --learn-collect-steps steps of plain previous-frame warm-start PGS at
+    --learn-collect-iters iterations (near-converged impulses as targets), train
     the network on the device (--learn-train-steps Adam steps of 8192 rows),
     then for cold start (the reference), previous-frame and learned warm start
     and each iteration count run --learn-eval-steps steps from that state (no
@@ -170,7 +171,8 @@ def learned_eval(w, args):
     state = w.bodies()
     sp0 = w.solver_params()
     t0 = time.time()
-    X, T, rows = ws.collect(w, args.learn_collect_steps, args.learn_capacity, dt=dt, mode=ws.COLD, iterations=20)
+    X, T, rows = ws.collect(w, args.learn_collect_steps, args.learn_capacity, dt=dt, mode=ws.PREVIOUS,
+                            iterations=args.learn_collect_iters)
     collect_s = time.time() - t0
     net = WarmStartNet(24, 3, max_train_rows=8192, seed=1)
     t0 = time.time()
@@ -216,7 +218,8 @@ def learned_eval(w, args):
         "learned_vs_plain_20": speed,
         "ms_per_step": ms,
         "curves": {name[m]: {str(k): float("%.4e" % v) for k, v in curves[m].items()} for m in curves},
-        "training": {"rows": int(rows), "collect_steps": args.learn_collect_steps, "collect_s": round(collect_s, 2),
+        "training": {"rows": int(rows), "collect_steps": args.learn_collect_steps,
+                     "collect_iterations": args.learn_collect_iters, "collect_s": round(collect_s, 2),
                      "adam_steps": args.learn_train_steps, "batch": 8192, "train_s": round(train_s, 2),
                      "loss_first_100": float(np.mean(hist[:100])) if len(hist) else None,
                      "loss_last_100": float(np.mean(hist[-100:])) if len(hist) else None},
@@ -272,8 +275,9 @@ def main():
     ap.add_argument("--learn-collect-steps", type=int, default=40)
     ap.add_argument("--learn-capacity", type=int, default=60_000_000)
     ap.add_argument("--learn-train-steps", type=int, default=3000)
-    ap.add_argument("--learn-eval-steps", type=int, default=8)
-    ap.add_argument("--learn-skip", type=int, default=3)
+    ap.add_argument("--learn-collect-iters", type=int, default=60)
+    ap.add_argument("--learn-eval-steps", type=int, default=16)
+    ap.add_argument("--learn-skip", type=int, default=6)
     ap.add_argument("--learn-time-steps", type=int, default=30)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a plain copy kernel of known "

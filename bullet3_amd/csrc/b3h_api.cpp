@@ -1470,6 +1470,13 @@ extern "C" int b3h_pipeline_set_pgs_body_cache(b3hPipeline* p, int maxBodiesPerC
 	return B3H_OK;
 }
 
+extern "C" int b3h_pipeline_set_pgs_inertia_cache(b3hPipeline* p, int enable)
+{
+	CHECK_ARG(p);
+	p->sb.pgsNoInertiaCache = enable ? 0 : 1;
+	drop_graph(p);
+	return B3H_OK;
+}
 extern "C" int b3h_pipeline_get_pgs_levels(b3hPipeline* p, int* levels)
 {
 	CHECK_ARG(p && levels);

@@ -121,6 +121,7 @@ struct b3hSolverBuffers
 	int* cellBodyCount;   /* 256 */
 	int* cslot;           /* per constraint: local slots of bodyA, bodyB in its cell's list (-1: static) */
 	int pgsBodyCap;       /* < 0: the LDS budget; otherwise a lower cap on cached bodies per cell */
+	int pgsNoInertiaCache; /* 1: never cache inverse inertias in LDS (tests) */
 	b3hFloat4* recN;      /* per constraint: 10 float4 of normal-row invariants */
 	b3hFloat4* recF;      /* per constraint: 8 float4 of friction-row invariants */
 	int* levelOrder;      /* per cell region [off, off+n): constraints grouped by dependency level */

@@ -785,9 +785,9 @@ __global__ __launch_bounds__(256) void k_pgs_prep(const int* __restrict__ nPtr, 
 }
 
 /* LDS bytes the level sweep needs for a cell of n constraints and nb bodies */
-__host__ __device__ __forceinline__ size_t pgs_level_lds(int nb, int n)
+__host__ __device__ __forceinline__ size_t pgs_level_lds(int nb, int n, bool cacheInertia)
 {
-	return (size_t)(nb + 1) * 36 + (size_t)(nb + 2) * 48 + (size_t)n * 4 + (size_t)(n + 1) * 4;
+	return (size_t)(nb + 1) * 36 + (cacheInertia ? (size_t)(nb + 2) * 48 : 0) + (size_t)n * 4 + (size_t)(n + 1) * 4;
 }
 
 /* ---------------- per-cell dependency levels ----------------
@@ -821,7 +821,7 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 		return;
 	}
 	/* last[nb] | lev[n] | cnt[n + 1] */
-	if (nb > bodyCap || nb + 2 * n + 1 > ldsInts || pgs_level_lds(nb, n) > (size_t)pgsLdsBytes)
+	if (nb > bodyCap || nb + 2 * n + 1 > ldsInts || pgs_level_lds(nb, n, false) > (size_t)pgsLdsBytes)
 	{
 		if (t == 0) cellLevels[cell] = -1;
 		return;
@@ -967,19 +967,35 @@ __device__ __forceinline__ float rc(const F4* r, int j)
 	}
 }
 __device__ __forceinline__ F4 rc3(const F4* r, int j) { return f4(rc(r, j), rc(r, j + 1), rc(r, j + 2), 0.f); }
-/* the inverse world inertia of a cached body (static bodies: the zero slot nb) */
-__device__ __forceinline__ b3hMat3x3 lds_inertia(const F4* sI, int slot, int nb)
+/* where a sweep finds the inverse world inertia of a cell body: the LDS cache
+ * (static bodies: its zero slot nb) or, for cells too large for the cache, the
+ * global record through the cell's body list (static bodies: zero) */
+struct InertiaSrc
 {
-	const F4* m = sI + 3 * (slot >= 0 ? slot : nb);
+	const F4* sI;
+	const b3hInertiaData* g;
+	const int* sList;
+	int nb;
+};
+__device__ __forceinline__ b3hMat3x3 body_inertia(const InertiaSrc& is, int slot)
+{
 	b3hMat3x3 r;
-	r.m_row[0] = m[0];
-	r.m_row[1] = m[1];
-	r.m_row[2] = m[2];
+	if (is.sI)
+	{
+		const F4* m = is.sI + 3 * (slot >= 0 ? slot : is.nb);
+		r.m_row[0] = m[0];
+		r.m_row[1] = m[1];
+		r.m_row[2] = m[2];
+	}
+	else if (slot >= 0)
+		r = is.g[is.sList[slot]].m_invInertiaWorld;
+	else
+		r.m_row[0] = r.m_row[1] = r.m_row[2] = f4(0.f, 0.f, 0.f, 0.f);
 	return r;
 }
 
 template <int PHASE>
-__device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4* sAv, const F4* sI, int nb,
+__device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4* sAv, const InertiaSrc& is,
 										  b3hContactConstraint4* __restrict__ cs, float friction, float& maxD,
 										  float& maxL)
 {
@@ -996,8 +1012,8 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 	va.av = sA >= 0 ? sAv[sA] : zero;
 	vb.lv = sB >= 0 ? sLv[sB] : zero;
 	vb.av = sB >= 0 ? sAv[sB] : zero;
-	const b3hMat3x3 iiA = lds_inertia(sI, sA, nb);
-	const b3hMat3x3 iiB = lds_inertia(sI, sB, nb);
+	const b3hMat3x3 iiA = body_inertia(is, sA);
+	const b3hMat3x3 iiB = body_inertia(is, sB);
 	if (PHASE == 1)
 	{
 		F4 fj = S.r[6];
@@ -1093,7 +1109,7 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
  * friction sweeps). */
 template <int PART>
 __device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restrict__ recF, F4* sLv, F4* sAv,
-						 const F4* sI, int nb, b3hContactConstraint4* __restrict__ cs, float friction,
+						 const InertiaSrc& is, b3hContactConstraint4* __restrict__ cs, float friction,
 						 const float* __restrict__ warm)
 {
 	const F4 zero = f4(0.f, 0.f, 0.f, 0.f);
@@ -1107,8 +1123,8 @@ __device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restr
 	va.av = sA >= 0 ? sAv[sA] : zero;
 	vb.lv = sB >= 0 ? sLv[sB] : zero;
 	vb.av = sB >= 0 ? sAv[sB] : zero;
-	const b3hMat3x3 iiA = lds_inertia(sI, sA, nb);
-	const b3hMat3x3 iiB = lds_inertia(sI, sB, nb);
+	const b3hMat3x3 iiA = body_inertia(is, sA);
+	const b3hMat3x3 iiB = body_inertia(is, sB);
 	F4 lin = N[0];
 	float imA = lin.w;
 	float lam[4];
@@ -1176,7 +1192,7 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 														   const F4* __restrict__ recF, int ib, float friction,
 														   const float* __restrict__ warm, float* __restrict__ resid,
 														   const int* __restrict__ lastClass, int fuseIntegrate,
-														   b3hIntegrateParams ip)
+														   b3hIntegrateParams ip, int ldsBytes)
 {
 	extern __shared__ F4 smem4[];
 	int cell = cell_of(blockIdx.x, ib);
@@ -1222,14 +1238,17 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		return;
 	}
 	int nb = cellBodyCount[cell];
-	F4* sI = smem4;           /* 3 (nb + 2) inverse world inertia rows: slot nb is zero (static bodies), nb + 1 padding */
-	F4* sLv = sI + 3 * (nb + 2); /* nb + 1 (the last is padding) */
+	/* with room for it, 3 (nb + 2) inverse world inertia rows (slot nb is zero
+	 * for static bodies, nb + 1 padding); then the velocities */
+	bool cacheI = pgs_level_lds(nb, n, true) <= (size_t)ldsBytes;
+	F4* sI = smem4;
+	F4* sLv = cacheI ? sI + 3 * (nb + 2) : smem4; /* nb + 1 (the last is padding) */
 	F4* sAv = sLv + nb + 1;   /* nb + 1 */
 	int* sList = (int*)(sAv + nb + 1);
 	int* sOrder = sList + nb + 1;
 	const int* list = cellBodies + 2 * (size_t)start;
 	for (int k = t; k < nb; k += T) sList[k] = list[k];
-	if (t < 3) sI[3 * nb + t] = f4(0.f, 0.f, 0.f, 0.f);
+	if (cacheI && t < 3) sI[3 * nb + t] = f4(0.f, 0.f, 0.f, 0.f);
 	__syncthreads();
 	/* gather the velocities and inverse inertias, two bodies per thread per
 	 * round with every load issued before any is stored (lanes past the end
@@ -1246,15 +1265,19 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		int s1 = k1 < nb ? k1 : nb, i1s = k1 < nb ? k1 : nb + 1;
 		sLv[k0] = l0;
 		sAv[k0] = a0;
-		sI[3 * k0 + 0] = i0.m_row[0];
-		sI[3 * k0 + 1] = i0.m_row[1];
-		sI[3 * k0 + 2] = i0.m_row[2];
 		sLv[s1] = l1;
 		sAv[s1] = a1;
-		sI[3 * i1s + 0] = i1.m_row[0];
-		sI[3 * i1s + 1] = i1.m_row[1];
-		sI[3 * i1s + 2] = i1.m_row[2];
+		if (cacheI)
+		{
+			sI[3 * k0 + 0] = i0.m_row[0];
+			sI[3 * k0 + 1] = i0.m_row[1];
+			sI[3 * k0 + 2] = i0.m_row[2];
+			sI[3 * i1s + 0] = i1.m_row[0];
+			sI[3 * i1s + 1] = i1.m_row[1];
+			sI[3 * i1s + 2] = i1.m_row[2];
+		}
 	}
+	const InertiaSrc is = {cacheI ? (const F4*)sI : (const F4*)nullptr, inertias, sList, nb};
 	const int* order = levelOrder + start;
 	for (int k = t; k < n; k += T) sOrder[k] = order[k];
 	int* sLs = sOrder + n; /* level starts, relative to start */
@@ -1267,7 +1290,7 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		for (int cl = 0; cl < L; cl++)
 		{
 			for (int k = ls[cl] + t; k < ls[cl + 1]; k += T)
-				pgs_warm<PHASE - 2>(sOrder[k], recN, recF, sLv, sAv, sI, nb, cs, friction, warm);
+				pgs_warm<PHASE - 2>(sOrder[k], recN, recF, sLv, sAv, is, cs, friction, warm);
 			lds_barrier();
 		}
 	}
@@ -1289,7 +1312,7 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 				int l2 = l1, k2 = k1;
 				next_item(ls, L, T, l2, k2);
 				if (l2 < L) slot_fetch<PHASE>(nx2, sOrder[k2], recN, recF, cs);
-				pgs_solve<PHASE>(cur.r, sLv, sAv, sI, nb, cs, friction, maxD, maxL);
+				pgs_solve<PHASE>(cur.r, sLv, sAv, is, cs, friction, maxD, maxL);
 				cur = nxt;
 				nxt = nx2;
 				l = l1;
@@ -1477,7 +1500,7 @@ void b3h_solve_sweeps(b3hRigidBodyData* bodies, const b3hInertiaData* inertias, 
 #define B3H_PGS_ARGS(ib, resid_, warm_, fuse_)                                                                         \
 	bodies, inertias, sb.constraints, sb.cellCounts, sb.cellOffsets, sb.batchSizes, sb.batchIdx, sb.cellBodies, \
 		sb.cellBodyCount, sb.cellLevels, sb.levelStart, sb.levelOrder, (const F4*)sb.recN, (const F4*)sb.recF, ib, \
-		prm.m_frictionCoeff, warm_, resid_, (const int*)sb.lastClass, fuse_, ip
+		prm.m_frictionCoeff, warm_, resid_, (const int*)sb.lastClass, fuse_, ip, sb.pgsNoInertiaCache ? 0 : (int)pgsLds
 	if (sb.warmImpulses)
 		for (int ib = 0; ib < B3H_N_CELL_BATCHES; ib++)
 			hipLaunchKernelGGL(k_pgs_level<2>, grid, block, pgsLds, s,

@@ -27,7 +27,9 @@
  *   13 lp_i/s_i | 14-15 fp_k/(np sf_k) | 16 pair matched | 17 point matched |
  *   18 np/4 | 19 sum(lp)/(np s_i) | 20-21 (p_i - center).t_k/0.5 |
  *   22 (wA - wB).n 0.5/u | 23 friction coefficient
- * Targets (logged after the solve, same units): (lambda_i - lp_i)/s_i and
+ * Targets (logged after the solve, same units): (sum_j lambda_j - sum_j lp_j)
+ * / (np s_i) -- the manifold's total load change, shared by its points, since
+ * the split of the load over the points is not determined -- and
  * (f_k - fp_k)/(np sf_k); the initial impulses are lp_i + s_i y0 and
  * fp_k + sf_k sum_i y_k.
  */
@@ -235,13 +237,22 @@ __global__ __launch_bounds__(256) void k_warm_targets(const int* __restrict__ nP
 		float sf0 = fabsf(c.m_fJacCoeffInv[0]) * velUnit, sf1 = fabsf(c.m_fJacCoeffInv[1]) * velUnit;
 		float t1 = (c.m_fAppliedRambdaDt[0] - pr[4]) / (fnp * sf0);
 		float t2 = (c.m_fAppliedRambdaDt[1] - pr[5]) / (fnp * sf1);
+		/* the normal target is the manifold's total: how the load spreads over
+		 * its points is not determined by the contact problem (PGS leaves it to
+		 * the sweep order and the initial iterate) */
+		float tot = 0.f, totPrev = 0.f;
+		for (int i = 0; i < np; i++)
+		{
+			tot += c.m_appliedRambdaDt[i];
+			totPrev += pr[i];
+		}
 		for (int i = 0; i < 4; i++)
 		{
 			float* t = T + (size_t)(4 * k + i) * B3H_WARM_OUTPUTS;
 			if (i < np)
 			{
 				float si = fabsf(c.m_jacCoeffInv[i]) * velUnit;
-				t[0] = (c.m_appliedRambdaDt[i] - pr[i]) / si;
+				t[0] = (tot - totPrev) / (fnp * si);
 				t[1] = t1;
 				t[2] = t2;
 			}

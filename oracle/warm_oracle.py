@@ -160,7 +160,8 @@ def features(cons, bodies, prev, pmatch, vel_unit):
 
 
 def targets(cons_after, prev, vel_unit):
-    """logged targets [4n][3] from the solve's final impulses"""
+    """logged targets [4n][3] from the solve's final impulses: the manifold's
+    total normal impulse change shared by its points, the friction changes"""
     u = f32(vel_unit)
     T = np.zeros((4 * len(cons_after), 3), f32)
     for k, c in enumerate(cons_after):
@@ -169,7 +170,11 @@ def targets(cons_after, prev, vel_unit):
         sf = np.abs(c["fJacCoeffInv"]).astype(f32) * u
         t1 = (c["fAppliedRambdaDt"][0] - prev[k, 4]) / (fnp * sf[0])
         t2 = (c["fAppliedRambdaDt"][1] - prev[k, 5]) / (fnp * sf[1])
+        tot, tot_prev = f32(0), f32(0)
+        for i in range(npt):
+            tot = f32(tot + c["appliedRambdaDt"][i])
+            tot_prev = f32(tot_prev + prev[k, i])
         for i in range(npt):
             si = f32(abs(c["jacCoeffInv"][i]) * u)
-            T[4 * k + i] = ((c["appliedRambdaDt"][i] - prev[k, i]) / si, t1, t2)
+            T[4 * k + i] = ((tot - tot_prev) / (fnp * si), t1, t2)
     return T
