@@ -977,8 +977,28 @@ struct InertiaSrc
 	const int* sList;
 	int nb;
 };
+/* CI: the cache is in use (1), not in use (0), or decided at run time (-1) */
+template <int CI = -1>
 __device__ __forceinline__ b3hMat3x3 body_inertia(const InertiaSrc& is, int slot)
 {
+	if (CI == 1)
+	{
+		const F4* m = is.sI + 3 * (slot >= 0 ? slot : is.nb);
+		b3hMat3x3 r;
+		r.m_row[0] = m[0];
+		r.m_row[1] = m[1];
+		r.m_row[2] = m[2];
+		return r;
+	}
+	if (CI == 0)
+	{
+		b3hMat3x3 r;
+		if (slot >= 0)
+			r = is.g[is.sList[slot]].m_invInertiaWorld;
+		else
+			r.m_row[0] = r.m_row[1] = r.m_row[2] = f4(0.f, 0.f, 0.f, 0.f);
+		return r;
+	}
 	b3hMat3x3 r;
 	if (is.sI)
 	{
@@ -994,7 +1014,7 @@ __device__ __forceinline__ b3hMat3x3 body_inertia(const InertiaSrc& is, int slot
 	return r;
 }
 
-template <int PHASE>
+template <int PHASE, int CI>
 __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4* sAv, const InertiaSrc& is,
 										  b3hContactConstraint4* __restrict__ cs, float friction, float& maxD,
 										  float& maxL)
@@ -1012,8 +1032,8 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 	va.av = sA >= 0 ? sAv[sA] : zero;
 	vb.lv = sB >= 0 ? sLv[sB] : zero;
 	vb.av = sB >= 0 ? sAv[sB] : zero;
-	const b3hMat3x3 iiA = body_inertia(is, sA);
-	const b3hMat3x3 iiB = body_inertia(is, sB);
+	const b3hMat3x3 iiA = body_inertia<CI>(is, sA);
+	const b3hMat3x3 iiB = body_inertia<CI>(is, sB);
 	if (PHASE == 1)
 	{
 		F4 fj = S.r[6];
@@ -1169,11 +1189,59 @@ __device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restr
 	}
 }
 
-/* the next position of this thread: T on, in the level that holds it */
-__device__ __forceinline__ void next_item(const int* ls, int L, int T, int& l, int& k)
+/* one sweep of a cell by dependency levels (this thread: positions ls[c] + t,
+ * + T, ... of each level c) */
+template <int PHASE, int CI>
+__device__ __forceinline__ void pgs_walk(int t, int n, int L, const int* ls, const int* sOrder,
+										 const F4* __restrict__ recN, const F4* __restrict__ recF,
+										 b3hContactConstraint4* __restrict__ cs, F4* sLv, F4* sAv,
+										 const InertiaSrc& is, float friction, float& maxD, float& maxL)
 {
-	k += T;
-	while (l < L && k >= ls[l + 1]) l++;
+	const int T = PGS_THREADS;
+	/* Three record slots in fixed registers, one per level residue: level
+	 * c's first item of this thread (position ls[c] + t) is fetched into
+	 * slot c % 3 while level c - 3 is solved, so every record is in flight
+	 * for two levels.  The fetch is unconditional (a dummy index when the
+	 * thread has no item there), which lets the compiler wait for exactly
+	 * the slot it solves.  Further items of a level wider than the block
+	 * are fetched on the spot. */
+	Slot<PHASE> A, B, C;
+	bool hA = false, hB = false, hC = false;
+	const int dummy = sOrder[0];
+	{
+		int p0 = ls[0] + t, p1 = 1 < L ? ls[1] + t : n, p2 = 2 < L ? ls[2] + t : n;
+		hA = 0 < L && p0 < ls[1];
+		hB = 1 < L && p1 < ls[2];
+		hC = 2 < L && p2 < ls[3];
+		slot_fetch<PHASE>(A, hA ? sOrder[p0] : dummy, recN, recF, cs);
+		slot_fetch<PHASE>(B, hB ? sOrder[p1] : dummy, recN, recF, cs);
+		slot_fetch<PHASE>(C, hC ? sOrder[p2] : dummy, recN, recF, cs);
+	}
+#define PGS_LEVEL(X, hX, c)                                                                   \
+	{                                                                                         \
+		bool live = (c) < L;                                                                  \
+		if (hX) pgs_solve<PHASE, CI>(X.r, sLv, sAv, is, cs, friction, maxD, maxL);           \
+		for (int q = live ? ls[c] + t + T : n; q < (live ? ls[(c) + 1] : 0); q += T)          \
+		{                                                                                     \
+			Slot<PHASE> W;                                                                    \
+			slot_fetch<PHASE>(W, sOrder[q], recN, recF, cs);                                  \
+			pgs_solve<PHASE, CI>(W.r, sLv, sAv, is, cs, friction, maxD, maxL);               \
+		}                                                                                     \
+		int pn = (c) + 3 < L ? ls[(c) + 3] + t : n;                                          \
+		hX = (c) + 3 < L && pn < ls[(c) + 4];                                                 \
+		slot_fetch<PHASE>(X, hX ? sOrder[pn] : dummy, recN, recF, cs);                        \
+		lds_barrier();                                                                        \
+	}
+	/* every pass runs all three slots (refills and barriers unconditional,
+	 * 3 ceil(L / 3) barriers in all) so the loads in flight are the same on
+	 * every path */
+	for (int c = 0; c < L; c += 3)
+	{
+		PGS_LEVEL(A, hA, c)
+		PGS_LEVEL(B, hB, c + 1)
+		PGS_LEVEL(C, hC, c + 2)
+	}
+#undef PGS_LEVEL
 }
 
 template <int PHASE>
@@ -1296,32 +1364,10 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	}
 	else
 	{
-		/* this thread's items: the current one and the next two are in flight
-		 * (records are fetched two items ahead of their solve) */
-		int l = 0, k = t;
-		while (l < L && k >= ls[l + 1]) l++;
-		int l1 = l, k1 = k;
-		next_item(ls, L, T, l1, k1);
-		Slot<PHASE> cur, nxt, nx2;
-		if (l < L) slot_fetch<PHASE>(cur, sOrder[k], recN, recF, cs);
-		if (l1 < L) slot_fetch<PHASE>(nxt, sOrder[k1], recN, recF, cs);
-		for (int cl = 0; cl < L; cl++)
-		{
-			while (l == cl)
-			{
-				int l2 = l1, k2 = k1;
-				next_item(ls, L, T, l2, k2);
-				if (l2 < L) slot_fetch<PHASE>(nx2, sOrder[k2], recN, recF, cs);
-				pgs_solve<PHASE>(cur.r, sLv, sAv, is, cs, friction, maxD, maxL);
-				cur = nxt;
-				nxt = nx2;
-				l = l1;
-				k = k1;
-				l1 = l2;
-				k1 = k2;
-			}
-			lds_barrier();
-		}
+		if (is.sI)
+			pgs_walk<PHASE, 1>(t, n, L, ls, sOrder, recN, recF, cs, sLv, sAv, is, friction, maxD, maxL);
+		else
+			pgs_walk<PHASE, 0>(t, n, L, ls, sOrder, recN, recF, cs, sLv, sAv, is, friction, maxD, maxL);
 	}
 	__syncthreads();
 	for (int q = t; q < nb; q += T)
