@@ -8,13 +8,13 @@ exactly --steps steps are timed.  A "step" is one full stepSimulation: world
 AABBs, broadphase, narrowphase, solver set-up, 2x10 PGS sweeps, integration.
 
 Prints one JSON line (metric body-steps/s).  roofline: the kernel with the
-largest share of the step (the fused narrowphase kernel, or the normal or the
-friction PGS sweep launches), its algorithmic HBM bytes per launch (records
-read once and written once, kernel_bytes below) over its average launch
-duration, measured with HIP events on the pipeline's stream during eager steps
-taken right after the timed region.  traffic: measured HBM bytes per launch of
-that kernel from profiles/traffic.json (rocprof FETCH_SIZE and WRITE_SIZE runs),
-when it was measured for the same kernel.  cpu_baseline: the scalar oracle
+largest share of the step (the narrowphase SAT/clipping kernel or its MPR
+kernel, or the normal or the friction PGS sweep launches), its algorithmic HBM
+bytes per launch (records read once and written once, kernel_bytes below) over
+its average launch duration, measured with HIP events on the pipeline's stream
+during eager steps taken right after the timed region.  traffic: measured HBM
+bytes per launch of that kernel from profiles/traffic.json (rocprof FETCH_SIZE
+and WRITE_SIZE runs scaled by a copy of known size).  cpu_baseline: the scalar oracle
 (same algorithm, one core) stepped from the GPU's post-warmup state for ~10 s.
 """
 import argparse
@@ -50,7 +50,8 @@ def kernel_bytes(n, p, c):
     world inverse inertia 48, position 16)"""
     sweep = 192 * c + 128 * n
     return {
-        "narrowphase_kernel": 80 * p + 112 * c,   # pair + 2 poses read, contact written
+        "narrowphase_mpr": 128 * p,               # pair + 2 poses read, direction/point/depth written
+        "narrowphase_sat": 128 * p + 112 * c,     # pair + 2 poses + MPR result read, contact written
         "pgs_normal": sweep / 8.0,                # one of the 8 cell-parity launches of a sweep
         "pgs_friction": sweep / 8.0,
     }
@@ -83,12 +84,19 @@ def run(args):
     setup_s = time.time() - t0
     dt = 1.0 / 60.0
     for _ in range(args.warmup):
-        w.step_graph(dt, 1)
+        if args.eager:
+            w.step(dt)
+        else:
+            w.step_graph(dt, 1)
     w.sync()
-    # timed region: exactly args.steps graph-launched steps
+    # timed region: exactly args.steps steps, graph-launched (or eager with --eager)
     w.sync()
     t0 = time.perf_counter()
-    w.step_graph(dt, args.steps)
+    if args.eager:
+        for _ in range(args.steps):
+            w.step(dt)
+    else:
+        w.step_graph(dt, args.steps)
     w.sync()
     t1 = time.perf_counter()
     sec = t1 - t0
@@ -106,7 +114,8 @@ def run(args):
     st = dict(zip(STAGE_NAMES, (acc / reps).tolist()))
     n_pairs = len(w.pairs())
     n_contacts = len(w.contacts())
-    launches = {"narrowphase_kernel": 1, "pgs_normal": 8 * args.iters, "pgs_friction": 8 * args.iters}
+    launches = {"narrowphase_mpr": 1, "narrowphase_sat": 1, "pgs_normal": 8 * args.iters,
+                "pgs_friction": 8 * args.iters}
     kb = kernel_bytes(sc.num_bodies, n_pairs, n_contacts)
     dom = max(launches, key=lambda k: st[k])
     avg_ms = st[dom] / launches[dom]
@@ -117,8 +126,7 @@ def run(args):
     if traffic is None and os.path.exists(tpath):
         with open(tpath) as f:
             t = json.load(f)
-        if t.get("kernel") == dom:
-            traffic = t.get("bytes_per_launch")
+        traffic = t.get("bytes_per_launch", {}).get(dom)
     out = {
         "metric": "body-steps/s at 128k convexes; PGS iters to residual 1e-4, learned vs plain",
         "value": round(value, 1),
@@ -272,6 +280,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start evaluation")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every step's kernels one by one instead of replaying a captured HIP graph")
     ap.add_argument("--learn-collect-steps", type=int, default=40)
     ap.add_argument("--learn-capacity", type=int, default=60_000_000)
     ap.add_argument("--learn-train-steps", type=int, default=3000)

@@ -14,8 +14,8 @@ LIB_PATH = os.path.join(_HERE, "libb3hip.so")
 _LIB = None
 
 B3H_OK = 0
-NUM_STAGES = 8
-STAGE_NAMES = ("aabb", "broadphase", "narrowphase_kernel", "narrowphase_compact", "solver_setup",
+NUM_STAGES = 9
+STAGE_NAMES = ("aabb", "broadphase", "narrowphase_mpr", "narrowphase_sat", "narrowphase_compact", "solver_setup",
                "pgs_normal", "pgs_friction", "integrate")
 BROADPHASE_GRID = 0
 BROADPHASE_SAP = 1

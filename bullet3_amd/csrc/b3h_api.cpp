@@ -523,11 +523,11 @@ static b3hShapeTables np_tables(b3hNarrowPhase* np)
 
 /* device-count form used inside a step */
 static void np_compute_contacts_dev(b3hNarrowPhase* np, b3hInt4* pairs, const int* dNumPairs, int maxPairs,
-									hipEvent_t evAfterKernel = nullptr)
+									hipEvent_t evAfterKernel = nullptr, hipEvent_t evAfterMpr = nullptr)
 {
 	b3h_narrowphase(pairs, dNumPairs, maxPairs, np->dBodies, np_tables(np), np->dStaging, np->dFlags, np->dFlagScan,
 					np->dScanTmp, np->dContactPair, np->cfg.m_maxContactCapacity, np->dNumContacts, np->dSepState,
-					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr);
+					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr, evAfterMpr);
 	np->contactsValid = false;
 }
 
@@ -1161,9 +1161,9 @@ static void pl_solve(b3hPipeline* p, hipEvent_t evSetupDone = nullptr, hipEvent_
 static int pl_step(b3hPipeline* p, float dt, bool events)
 {
 	hipStream_t s = p->stream;
-	/* stage events: 0 start | aabb | 1 | broadphase | 2 | narrowphase kernel | 3 |
-	 * narrowphase compaction | 4 | solver set-up | 5 | normal sweeps | 6 |
-	 * friction sweeps | 7 | integrate | 8 */
+	/* stage events: 0 start | aabb | 1 | broadphase | 2 | narrowphase MPR | 3 |
+	 * narrowphase SAT and clipping | 4 | narrowphase compaction | 5 | solver
+	 * set-up | 6 | normal sweeps | 7 | friction sweeps | 8 | integrate | 9 */
 	if (events) hipEventRecord(p->ev[0], s);
 	pl_aabbs(p);
 	if (events) hipEventRecord(p->ev[1], s);
@@ -1171,17 +1171,17 @@ static int pl_step(b3hPipeline* p, float dt, bool events)
 	if (r) return r;
 	if (events) hipEventRecord(p->ev[2], s);
 	np_compute_contacts_dev(p->np, p->bp->dPairs, p->bp->dPairCount, p->cfg.m_maxBroadphasePairs,
-							events ? p->ev[3] : nullptr);
-	if (events) hipEventRecord(p->ev[4], s);
+							events ? p->ev[4] : nullptr, events ? p->ev[3] : nullptr);
+	if (events) hipEventRecord(p->ev[5], s);
 	/* integration is fused into the last friction sweep for every body a
 	 * contact touches; the rest are integrated by the pass below */
 	b3hIntegrateParams ip = {dt, p->prm.m_angularDamping, p->gravity};
 	bool fused = p->prm.m_numIterations > 0;
-	pl_solve(p, events ? p->ev[5] : nullptr, events ? p->ev[6] : nullptr, fused ? &ip : nullptr);
-	if (events) hipEventRecord(p->ev[7], s);
+	pl_solve(p, events ? p->ev[6] : nullptr, events ? p->ev[7] : nullptr, fused ? &ip : nullptr);
+	if (events) hipEventRecord(p->ev[8], s);
 	b3h_launch_integrate(num_bodies(p), p->np->dBodies, dt, p->prm.m_angularDamping, p->gravity, s,
 						 fused ? p->sb.lastClass : nullptr);
-	if (events) hipEventRecord(p->ev[8], s);
+	if (events) hipEventRecord(p->ev[9], s);
 	return B3H_OK;
 }
 

@@ -1017,7 +1017,8 @@ int b3h_np_module_init()
 void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3hRigidBodyData* bodies,
 					 const b3hShapeTables& sh, b3hContact4Data* staging, int* flags, int* flagScan, unsigned* scanTmp,
 					 int* contactPair, int maxContacts, int* numContacts, int* sepState, b3hFloat4* sepNormal,
-					 int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel, b3hFloat4* mprScratch)
+					 int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel, b3hFloat4* mprScratch,
+					 hipEvent_t evAfterMpr)
 {
 	{
 		int mb = (maxPairs + 255) / 256;
@@ -1025,6 +1026,7 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 		if (mb < 1) mb = 1;
 		hipLaunchKernelGGL(k_np_mpr, dim3(mb), dim3(256), 0, s, pairs, numPairs, bodies, sh, (F4*)mprScratch);
 	}
+	if (evAfterMpr) hipEventRecord(evAfterMpr, s);
 	(void)pairsPerGroup;
 	const int groups = NP_THREADS / 64;
 	int blocks = (maxPairs + groups - 1) / groups;

@@ -237,10 +237,10 @@ int b3h_pipeline_set_contact_log(b3hPipeline* p, float* featuresDev, float* targ
 int b3h_pipeline_get_contact_log_count(b3hPipeline* p, int* rows);
 /* Stage timings of the last eager step in ms (enable first; adds HIP events on
  * the pipeline stream).  B3H_NUM_STAGES values, in order: world AABBs,
- * broadphase, narrowphase kernel, narrowphase compaction, solver set-up
- * (ordering, batching, constraints), normal sweeps, friction sweeps, integrate.
- * No counterpart in the reference. */
-#define B3H_NUM_STAGES 8
+ * broadphase, narrowphase MPR, narrowphase SAT and clipping, narrowphase
+ * compaction, solver set-up (ordering, batching, constraints, warm start),
+ * normal sweeps, friction sweeps, integrate.  No counterpart in the reference. */
+#define B3H_NUM_STAGES 9
 int b3h_pipeline_enable_timing(b3hPipeline* p, int enable);
 int b3h_pipeline_get_stage_times(b3hPipeline* p, float* msOut);
 /* Overflow flags of the last step (pairs, contacts) -- syncs. */
