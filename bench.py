@@ -184,7 +184,7 @@ def learned_eval(w, args):
     collect_s = time.time() - t0
     net = WarmStartNet(24, 3, max_train_rows=8192, seed=1)
     t0 = time.time()
-    hist = ws.train(net, X, T, rows, args.learn_train_steps, batch=8192, lr=1e-3, lr_final=1e-4)
+    hist = ws.train(net, X, T, rows, args.learn_train_steps, batch=8192, lr=1e-3, lr_final=1e-4, normalize=True)
     train_s = time.time() - t0
     X.free()
     T.free()
@@ -284,7 +284,7 @@ def main():
                     help="launch every step's kernels one by one instead of replaying a captured HIP graph")
     ap.add_argument("--learn-collect-steps", type=int, default=40)
     ap.add_argument("--learn-capacity", type=int, default=60_000_000)
-    ap.add_argument("--learn-train-steps", type=int, default=3000)
+    ap.add_argument("--learn-train-steps", type=int, default=12000)
     ap.add_argument("--learn-collect-iters", type=int, default=60)
     ap.add_argument("--learn-eval-steps", type=int, default=16)
     ap.add_argument("--learn-skip", type=int, default=6)

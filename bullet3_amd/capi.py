@@ -125,6 +125,8 @@ _SIGS = {
     "b3h_mlp_num_params": ([_vp, _pi], _i),
     "b3h_mlp_set_params": ([_vp, _vp], _i),
     "b3h_mlp_get_params": ([_vp, _vp], _i),
+    "b3h_mlp_set_input_normalization": ([_vp, _vp, _vp], _i),
+    "b3h_mlp_get_input_normalization": ([_vp, _vp, _vp], _i),
     "b3h_mlp_forward": ([_vp, _vp, _i, _i, _vp, _i], _i),
     "b3h_mlp_loss_gradients": ([_vp, _vp, _i, _vp, _i, _i, _vp, _vp], _i),
     "b3h_mlp_train": ([_vp, _vp, _i, _vp, _i, _i, _i, _i, _f, ctypes.c_uint, _vp], _i),

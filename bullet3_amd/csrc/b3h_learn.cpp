@@ -140,6 +140,7 @@ extern "C" int b3h_mlp_create(int inDim, int outDim, int maxTrainRows, unsigned 
 	m->maxTrainRows = (maxTrainRows + 31) & ~31;
 	int r = 0;
 	(r = mlp_alloc(&m->params, MLP_NPARAM)) || (r = mlp_alloc(&m->grads, MLP_NPARAM)) ||
+		(r = mlp_alloc(&m->inShift, MLP_K1)) || (r = mlp_alloc(&m->inScale, MLP_K1)) ||
 		(r = mlp_alloc(&m->adamM, MLP_NPARAM)) || (r = mlp_alloc(&m->adamV, MLP_NPARAM)) ||
 		(r = mlp_alloc(&m->w1t, (size_t)MLP_H * MLP_K1)) || (r = mlp_alloc(&m->w2t, (size_t)MLP_H * MLP_H)) ||
 		(r = mlp_alloc(&m->w3t, (size_t)MLP_K3 * MLP_H)) || (r = mlp_alloc(&m->w2n, (size_t)MLP_H * MLP_H)) ||
@@ -149,6 +150,17 @@ extern "C" int b3h_mlp_create(int inDim, int outDim, int maxTrainRows, unsigned 
 	{
 		b3h_mlp_destroy(m);
 		return r;
+	}
+	{
+		std::vector<float> zero(MLP_K1, 0.f), one(MLP_K1, 1.f);
+		hipError_t e1 = hipMemcpy(m->inShift, zero.data(), sizeof(float) * MLP_K1, hipMemcpyHostToDevice);
+		hipError_t e2 = hipMemcpy(m->inScale, one.data(), sizeof(float) * MLP_K1, hipMemcpyHostToDevice);
+		if (e1 != hipSuccess || e2 != hipSuccess)
+		{
+			b3h_set_error("hipMemcpy: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+			b3h_mlp_destroy(m);
+			return B3H_ERR_HIP;
+		}
 	}
 	std::vector<float> packed;
 	init_params(inDim, outDim, seed, packed);
@@ -162,6 +174,7 @@ extern "C" int b3h_mlp_destroy(b3hMlp* m)
 	else hipDeviceSynchronize();
 	free_work(m->wk);
 	mlp_free(m->params), mlp_free(m->grads), mlp_free(m->adamM), mlp_free(m->adamV);
+	mlp_free(m->inShift), mlp_free(m->inScale);
 	mlp_free(m->w1t), mlp_free(m->w2t), mlp_free(m->w3t), mlp_free(m->w2n), mlp_free(m->w3n);
 	delete m;
 	return B3H_OK;
@@ -241,6 +254,34 @@ extern "C" int b3h_mlp_train(b3hMlp* m, const float* xDev, int ldx, const float*
 	return B3H_OK;
 }
 
+extern "C" int b3h_mlp_set_input_normalization(b3hMlp* m, const float* shift, const float* scale)
+{
+	CHECK_ARG(m && shift && scale);
+	std::vector<float> sh(MLP_K1, 0.f), sc(MLP_K1, 1.f);
+	for (int i = 0; i < m->inDim; i++)
+	{
+		sh[i] = shift[i];
+		sc[i] = scale[i];
+	}
+	B3H_CHECK(hipMemcpyAsync(m->inShift, sh.data(), sizeof(float) * MLP_K1, hipMemcpyHostToDevice, m->stream));
+	B3H_CHECK(hipMemcpyAsync(m->inScale, sc.data(), sizeof(float) * MLP_K1, hipMemcpyHostToDevice, m->stream));
+	B3H_CHECK(hipStreamSynchronize(m->stream));
+	return B3H_OK;
+}
+extern "C" int b3h_mlp_get_input_normalization(b3hMlp* m, float* shift, float* scale)
+{
+	CHECK_ARG(m && shift && scale);
+	float sh[MLP_K1], sc[MLP_K1];
+	B3H_CHECK(hipMemcpyAsync(sh, m->inShift, sizeof(float) * MLP_K1, hipMemcpyDeviceToHost, m->stream));
+	B3H_CHECK(hipMemcpyAsync(sc, m->inScale, sizeof(float) * MLP_K1, hipMemcpyDeviceToHost, m->stream));
+	B3H_CHECK(hipStreamSynchronize(m->stream));
+	for (int i = 0; i < m->inDim; i++)
+	{
+		shift[i] = sh[i];
+		scale[i] = sc[i];
+	}
+	return B3H_OK;
+}
 extern "C" int b3h_mlp_get_dims(const b3hMlp* m, int* inDim, int* hidden, int* outDim)
 {
 	CHECK_ARG(m);

@@ -51,6 +51,8 @@ struct b3hMlp
 	int inDim, outDim;
 	hipStream_t stream;
 	float* params;             /* MLP_NPARAM fp32 masters */
+	float* inShift;            /* [K1] input normalisation x' = (x - shift) * scale (part of the model) */
+	float* inScale;            /* [K1] */
 	float* grads;              /* MLP_NPARAM */
 	float* adamM;
 	float* adamV;

@@ -113,6 +113,8 @@ struct MlpOps
 	const float* b1;
 	const float* b2;
 	const float* b3;
+	const float* inShift; /* [K1] input normalisation: x' = (x - shift) * scale */
+	const float* inScale;
 };
 #define FWD_WAVES 8
 #define FWD_THREADS (64 * FWD_WAVES)
@@ -159,7 +161,7 @@ __global__ __launch_bounds__(FWD_THREADS) void k_mlp_forward(int M, int Mp, cons
 		{
 			int rr = q / MLP_K1, c = q - rr * MLP_K1;
 			int row = row0 + rr;
-			float v = (row < M && c < inDim) ? X[(size_t)row * ldx + c] : 0.f;
+			float v = (row < M && c < inDim) ? (X[(size_t)row * ldx + c] - net.inShift[c]) * net.inScale[c] : 0.f;
 			unsigned short b = f2bf(v);
 			act[rr * LDS_H + c] = b;
 			if (save) Xb[(size_t)row * MLP_K1 + c] = b;
@@ -435,7 +437,8 @@ static void forward(const b3hMlp& m, int rows, const float* X, int ldx, float* Y
 {
 	int Mp = pad32(rows);
 	if (Mp == 0) return;
-	MlpOps net = {m.w1t, m.w2t, m.w3t, m.params + MLP_OFF_B1, m.params + MLP_OFF_B2, m.params + MLP_OFF_B3};
+	MlpOps net = {m.w1t,	m.w2t,	  m.w3t,	  m.params + MLP_OFF_B1, m.params + MLP_OFF_B2, m.params + MLP_OFF_B3,
+				  m.inShift, m.inScale};
 	int tiles = Mp / 32;
 	int blocks = (tiles + FWD_WAVES - 1) / FWD_WAVES;
 	if (blocks > 512) blocks = 512;
@@ -453,7 +456,8 @@ void b3h_mlp_run_forward(const b3hMlp& m, int rows, const float* X, int ldx, flo
 void b3h_mlp_run_forward_dev(const b3hMlp& m, const int* countDev, int rowsPerItem, int maxRows, const float* X,
 							 int ldx, float* Y, int ldy, hipStream_t s)
 {
-	MlpOps net = {m.w1t, m.w2t, m.w3t, m.params + MLP_OFF_B1, m.params + MLP_OFF_B2, m.params + MLP_OFF_B3};
+	MlpOps net = {m.w1t,	m.w2t,	  m.w3t,	  m.params + MLP_OFF_B1, m.params + MLP_OFF_B2, m.params + MLP_OFF_B3,
+				  m.inShift, m.inScale};
 	int tiles = (maxRows + 31) / 32;
 	int blocks = (tiles + FWD_WAVES - 1) / FWD_WAVES;
 	if (blocks > 512) blocks = 512;

@@ -585,7 +585,7 @@ __device__ __forceinline__ int cell_of(int wgIdx, int ib)
  * search per reference. */
 __global__ __launch_bounds__(256) void k_cell_bodies(const int* __restrict__ cellCounts, const int* __restrict__ cellOffsets,
 													 const b3hContactConstraint4* __restrict__ cs,
-													 const b3hRigidBodyData* __restrict__ bodies, int ldsCapKeys,
+													 const b3hRigidBodyData* __restrict__ bodies, int numBodies, int ldsCapKeys,
 													 unsigned* __restrict__ gScratch, int* __restrict__ cellBodies,
 													 int* __restrict__ cellBodyCount, int* __restrict__ cslot,
 													 int* __restrict__ lastClass)
@@ -606,6 +606,67 @@ __global__ __launch_bounds__(256) void k_cell_bodies(const int* __restrict__ cel
 		return;
 	}
 	int m = 2 * n;
+	int words = (numBodies + 31) >> 5;
+	if (2 * words <= ldsCapKeys)
+	{
+		/* a bitmap over all body indices: mark the cell's dynamic bodies, then
+		 * the rank of a body among them is the popcount of the bits before it
+		 * (ascending order, no sort) */
+		unsigned* bm = (unsigned*)smem;
+		int* wp = (int*)(bm + words); /* bits set before each word */
+		for (int i = t; i < words; i += blockDim.x) bm[i] = 0u;
+		__syncthreads();
+		for (int i = t; i < m; i += blockDim.x)
+		{
+			const b3hContactConstraint4& c = cs[off + (i >> 1)];
+			unsigned body = (i & 1) ? c.m_bodyB : c.m_bodyA;
+			if (bodies[body].m_invMass != 0.f) atomicOr(&bm[body >> 5], 1u << (body & 31));
+		}
+		__syncthreads();
+		int per = (words + (int)blockDim.x - 1) / (int)blockDim.x;
+		int w0 = t * per, w1 = w0 + per < words ? w0 + per : words;
+		int cnt = 0;
+		for (int w = w0; w < w1; w++) cnt += __popc(bm[w]);
+		/* exclusive scan of the per-thread counts over the block */
+		int lane = t & 63, wave = t >> 6;
+		int incl = cnt;
+		for (int o = 1; o < 64; o <<= 1)
+		{
+			int v = __shfl_up(incl, o);
+			if (lane >= o) incl += v;
+		}
+		if (lane == 63) waveSums[wave] = incl;
+		__syncthreads();
+		int base = incl - cnt;
+		for (int w = 0; w < wave; w++) base += waveSums[w];
+		int total = waveSums[0] + waveSums[1] + waveSums[2] + waveSums[3];
+		for (int w = w0; w < w1; w++)
+		{
+			wp[w] = base;
+			unsigned bits = bm[w];
+			while (bits)
+			{
+				int b = __ffs(bits) - 1;
+				bits &= bits - 1u;
+				int body = w * 32 + b;
+				cellBodies[2 * (size_t)off + base] = body;
+				if (lastClass) atomicMax(&lastClass[body], cellClass);
+				base++;
+			}
+		}
+		if (t == 0) cellBodyCount[cell] = total;
+		__syncthreads();
+		for (int i = t; i < m; i += blockDim.x)
+		{
+			const b3hContactConstraint4& c = cs[off + (i >> 1)];
+			unsigned body = (i & 1) ? c.m_bodyB : c.m_bodyA;
+			int slot = -1;
+			if (bodies[body].m_invMass != 0.f)
+				slot = wp[body >> 5] + __popc(bm[body >> 5] & ((1u << (body & 31)) - 1u));
+			cslot[2 * (size_t)(off + (i >> 1)) + (i & 1)] = slot;
+		}
+		return;
+	}
 	int np2 = 1;
 	while (np2 < m) np2 <<= 1;
 	/* keys[np2] then rank[np2]; global fallback region of 8n >= 2*np2 entries */
@@ -1465,7 +1526,8 @@ void b3h_solve_setup(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaDa
 					   writeSortedContacts ? sb.sortedContacts : (b3hContact4Data*)nullptr);
 	b3h_fill_i32(sb.lastClass, -1, numBodies, s);
 	hipLaunchKernelGGL(k_cell_bodies, dim3(B3H_N_CELLS), dim3(256), (size_t)ldsCapBytes, s, sb.cellCounts,
-					   sb.cellOffsets, sb.constraints, bodies, ldsCapBytes / 4, (unsigned*)sb.sortKeys, sb.cellBodies,
+					   sb.cellOffsets, sb.constraints, bodies, numBodies, ldsCapBytes / 4, (unsigned*)sb.sortKeys,
+					   sb.cellBodies,
 					   sb.cellBodyCount, sb.cslot, sb.lastClass);
 	int cap = (sb.pgsBodyCap >= 0 && sb.pgsBodyCap < s_pgsBodyCap) ? sb.pgsBodyCap : s_pgsBodyCap;
 	size_t pgsLds = (size_t)s_pgsLdsBytes;

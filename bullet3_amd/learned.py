@@ -84,13 +84,27 @@ class WarmStartNet:
         assert flat.size == self.num_params
         call("b3h_mlp_set_params", self.h, ptr(flat))
 
+    def set_input_normalization(self, shift, scale):
+        shift = np.ascontiguousarray(shift, np.float32)
+        scale = np.ascontiguousarray(scale, np.float32)
+        assert shift.size == self.in_dim and scale.size == self.in_dim
+        call("b3h_mlp_set_input_normalization", self.h, ptr(shift), ptr(scale))
+
+    def get_input_normalization(self):
+        shift = np.zeros(self.in_dim, np.float32)
+        scale = np.zeros(self.in_dim, np.float32)
+        call("b3h_mlp_get_input_normalization", self.h, ptr(shift), ptr(scale))
+        return shift, scale
+
     def save(self, path):
-        np.savez(path, params=self.get_params(), in_dim=self.in_dim, out_dim=self.out_dim)
+        shift, scale = self.get_input_normalization()
+        np.savez(path, params=self.get_params(), in_dim=self.in_dim, out_dim=self.out_dim, shift=shift, scale=scale)
 
     def load(self, path):
         z = np.load(path)
         assert int(z["in_dim"]) == self.in_dim and int(z["out_dim"]) == self.out_dim
         self.set_params(z["params"])
+        self.set_input_normalization(z["shift"], z["scale"])
 
     # -- compute
     def forward_dev(self, x_dev, rows, y_dev, ldx=None, ldy=None):

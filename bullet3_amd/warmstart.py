@@ -38,9 +38,23 @@ def collect(world, steps, capacity_rows, dt=1.0 / 60.0, mode=COLD, iterations=20
     return X, T, rows
 
 
-def train(net, X, T, rows, steps, batch=8192, lr=1e-3, lr_final=1e-4, seed=1, chunk=250):
+def fit_input_normalization(net, X, rows, sample=400_000):
+    """standardise every feature: shift = mean, scale = 1 / std over the first
+    `sample` logged rows (constant features keep scale 1)"""
+    x = X.numpy(min(rows, sample)).astype(np.float64)
+    mean = x.mean(0)
+    std = x.std(0)
+    scale = np.where(std > 1e-6, 1.0 / np.maximum(std, 1e-6), 1.0)
+    net.set_input_normalization(mean.astype(np.float32), scale.astype(np.float32))
+    return mean, scale
+
+
+def train(net, X, T, rows, steps, batch=8192, lr=1e-3, lr_final=1e-4, seed=1, chunk=250, normalize=True):
     """Adam on minibatches drawn from the log, learning rate decayed
-    geometrically from lr to lr_final; returns the loss of every step"""
+    geometrically from lr to lr_final; returns the loss of every step.  With
+    `normalize`, the network's input normalisation is fitted to the log first."""
+    if normalize and rows > 0:
+        fit_input_normalization(net, X, rows)
     hist = []
     done = 0
     while done < steps:

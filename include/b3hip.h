@@ -276,6 +276,11 @@ int b3h_mlp_num_params(const b3hMlp* m, int* n);
 /* host float[num_params]; set_params also resets the Adam state */
 int b3h_mlp_set_params(b3hMlp* m, const float* host);
 int b3h_mlp_get_params(b3hMlp* m, float* host);
+/* Input normalisation, part of the model: every input feature is mapped to
+ * (x - shift) * scale before the first layer (host float[inDim] each; the
+ * default is shift 0, scale 1).  Used in inference and training alike. */
+int b3h_mlp_set_input_normalization(b3hMlp* m, const float* shift, const float* scale);
+int b3h_mlp_get_input_normalization(b3hMlp* m, float* shift, float* scale);
 /* y[rows][outDim] = net(x[rows][inDim]), asynchronous on the network's stream */
 int b3h_mlp_forward(b3hMlp* m, const float* xDev, int ldx, int rows, float* yDev, int ldy);
 /* mean squared error over rows x outDim and its gradient (host float[num_params],

@@ -66,9 +66,18 @@ def _f32(x, exact):
     return np.asarray(x, np.float64) if exact else np.asarray(x, np.float32).astype(np.float64)
 
 
-def forward(flat, x, in_dim, out_dim, exact=False, keep=False):
+def normalize(x, shift=None, scale=None, exact=False):
+    """the network's input normalisation (x - shift) * scale (float32 on the device)"""
+    if shift is None:
+        return x
+    if exact:
+        return (np.asarray(x, np.float64) - shift) * scale
+    return ((np.asarray(x, np.float32) - np.float32(shift)) * np.float32(scale)).astype(np.float32)
+
+
+def forward(flat, x, in_dim, out_dim, exact=False, keep=False, shift=None, scale=None):
     p = unpack(flat, in_dim, out_dim)
-    xb = _r(x, exact)
+    xb = _r(normalize(x, shift, scale, exact), exact)
     w1, w2, w3 = _r(p["w1"], exact), _r(p["w2"], exact), _r(p["w3"], exact)
     z1 = _f32(xb @ w1 + p["b1"], exact)
     h1 = _r(gelu(z1), exact)
@@ -80,9 +89,9 @@ def forward(flat, x, in_dim, out_dim, exact=False, keep=False):
     return y
 
 
-def loss_gradients(flat, x, t, in_dim, out_dim, exact=False):
+def loss_gradients(flat, x, t, in_dim, out_dim, exact=False, shift=None, scale=None):
     """mean squared error over rows x out and its gradient (packed layout)"""
-    y, c = forward(flat, x, in_dim, out_dim, exact, keep=True)
+    y, c = forward(flat, x, in_dim, out_dim, exact, keep=True, shift=shift, scale=scale)
     m = x.shape[0]
     d = y - np.asarray(t, np.float64)
     loss = float(np.mean(d * d))

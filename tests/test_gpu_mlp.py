@@ -145,3 +145,25 @@ def test_one_adam_step_matches_oracle(learned):
     print("agree %.4f of %d" % (agree[big].mean(), big.sum()), flush=True)
     assert agree[big].mean() > 0.99
     net.close()
+
+
+def test_input_normalization_matches_oracle(learned):
+    rng = np.random.default_rng(31)
+    net = learned.WarmStartNet(IN, OUT, max_train_rows=512, seed=1)
+    flat = rand_params(rng, IN, OUT)
+    net.set_params(flat)
+    shift = rng.normal(size=IN).astype(np.float32) * 3
+    scale = np.exp(rng.normal(size=IN)).astype(np.float32)
+    net.set_input_normalization(shift, scale)
+    s2, c2 = net.get_input_normalization()
+    assert np.array_equal(s2, shift) and np.array_equal(c2, scale)
+    x = (rng.normal(size=(300, IN)) * 4 + 2).astype(np.float32)
+    t = rng.normal(size=(300, OUT)).astype(np.float32)
+    y = net.forward(x)
+    yo = mo.forward(flat, x, IN, OUT, exact=False, shift=shift, scale=scale)
+    assert rel(y, yo) < 5e-3
+    loss, g = net.loss_gradients(x, t)
+    lo, go = mo.loss_gradients(flat, x, t, IN, OUT, exact=False, shift=shift, scale=scale)
+    assert abs(loss - lo) <= 1e-3 * lo
+    assert rel(g, go) < 1e-2
+    net.close()
