@@ -848,7 +848,8 @@ __global__ __launch_bounds__(256) void k_pgs_prep(const int* __restrict__ nPtr, 
 /* LDS bytes the level sweep needs for a cell of n constraints and nb bodies */
 __host__ __device__ __forceinline__ size_t pgs_level_lds(int nb, int n, bool cacheInertia)
 {
-	return (size_t)(nb + 1) * 36 + (cacheInertia ? (size_t)(nb + 2) * 48 : 0) + (size_t)n * 4 + (size_t)(n + 1) * 4;
+	return (size_t)(nb + 1) * 36 + (cacheInertia ? (size_t)(nb + 2) * 48 : 0) + (size_t)(n + 1) * 4 +
+		   (size_t)(n + 2) * 4;
 }
 
 /* ---------------- per-cell dependency levels ----------------
@@ -1305,6 +1306,29 @@ __device__ __forceinline__ void pgs_walk(int t, int n, int L, const int* ls, con
 #undef PGS_LEVEL
 }
 
+/* global -> LDS copy of n ints, eight per thread per round with every load
+ * issued before any store (out-of-range lanes load element 0 and store into
+ * the padding entry dst[n]) */
+__device__ __forceinline__ void stage_ints(int* dst, const int* __restrict__ src, int n, int t)
+{
+	for (int k0 = t; k0 < n; k0 += 8 * PGS_THREADS)
+	{
+		int v[8];
+#pragma unroll
+		for (int u = 0; u < 8; u++)
+		{
+			int k = k0 + u * PGS_THREADS;
+			v[u] = src[k < n ? k : 0];
+		}
+#pragma unroll
+		for (int u = 0; u < 8; u++)
+		{
+			int k = k0 + u * PGS_THREADS;
+			dst[k < n ? k : n] = v[u];
+		}
+	}
+}
+
 template <int PHASE>
 __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __restrict__ bodies,
 														   const b3hInertiaData* __restrict__ inertias,
@@ -1373,44 +1397,52 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	F4* sI = smem4;
 	F4* sLv = cacheI ? sI + 3 * (nb + 2) : smem4; /* nb + 1 (the last is padding) */
 	F4* sAv = sLv + nb + 1;   /* nb + 1 */
-	int* sList = (int*)(sAv + nb + 1);
-	int* sOrder = sList + nb + 1;
-	const int* list = cellBodies + 2 * (size_t)start;
-	for (int k = t; k < nb; k += T) sList[k] = list[k];
+	int* sList = (int*)(sAv + nb + 1); /* nb + 1 */
+	int* sOrder = sList + nb + 1;      /* n + 1 */
+	int* sLs = sOrder + n + 1;         /* L + 2: level starts relative to start */
+	/* stage the body list, the level order and the level starts together */
+	stage_ints(sList, cellBodies + 2 * (size_t)start, nb, t);
+	stage_ints(sOrder, levelOrder + start, n, t);
+	stage_ints(sLs, levelStart + start + cell, L + 1, t);
 	if (cacheI && t < 3) sI[3 * nb + t] = f4(0.f, 0.f, 0.f, 0.f);
 	__syncthreads();
-	/* gather the velocities and inverse inertias, two bodies per thread per
+	/* gather the velocities and inverse inertias, four bodies per thread per
 	 * round with every load issued before any is stored (lanes past the end
-	 * read body 0, which exists, and store into the padding slot) */
-	for (int k0 = t; k0 < nb; k0 += 2 * T)
+	 * read body 0, which exists, and store into the padding slots) */
+	for (int k0 = t; k0 < nb; k0 += 4 * T)
 	{
-		int k1 = k0 + T;
-		int b0 = sList[k0];
-		int b1 = k1 < nb ? sList[k1] : 0;
-		F4 l0 = bodies[b0].m_linVel, a0 = bodies[b0].m_angVel;
-		F4 l1 = bodies[b1].m_linVel, a1 = bodies[b1].m_angVel;
-		b3hMat3x3 i0 = inertias[b0].m_invInertiaWorld;
-		b3hMat3x3 i1 = inertias[b1].m_invInertiaWorld;
-		int s1 = k1 < nb ? k1 : nb, i1s = k1 < nb ? k1 : nb + 1;
-		sLv[k0] = l0;
-		sAv[k0] = a0;
-		sLv[s1] = l1;
-		sAv[s1] = a1;
-		if (cacheI)
+		int kk[4], bb[4];
+		F4 lv[4], av[4], r0[4], r1[4], r2[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++)
 		{
-			sI[3 * k0 + 0] = i0.m_row[0];
-			sI[3 * k0 + 1] = i0.m_row[1];
-			sI[3 * k0 + 2] = i0.m_row[2];
-			sI[3 * i1s + 0] = i1.m_row[0];
-			sI[3 * i1s + 1] = i1.m_row[1];
-			sI[3 * i1s + 2] = i1.m_row[2];
+			kk[u] = k0 + u * T;
+			bb[u] = kk[u] < nb ? sList[kk[u]] : 0;
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++)
+		{
+			lv[u] = bodies[bb[u]].m_linVel;
+			av[u] = bodies[bb[u]].m_angVel;
+			r0[u] = inertias[bb[u]].m_invInertiaWorld.m_row[0];
+			r1[u] = inertias[bb[u]].m_invInertiaWorld.m_row[1];
+			r2[u] = inertias[bb[u]].m_invInertiaWorld.m_row[2];
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++)
+		{
+			int sv = kk[u] < nb ? kk[u] : nb, si = kk[u] < nb ? kk[u] : nb + 1;
+			sLv[sv] = lv[u];
+			sAv[sv] = av[u];
+			if (cacheI)
+			{
+				sI[3 * si + 0] = r0[u];
+				sI[3 * si + 1] = r1[u];
+				sI[3 * si + 2] = r2[u];
+			}
 		}
 	}
 	const InertiaSrc is = {cacheI ? (const F4*)sI : (const F4*)nullptr, inertias, sList, nb};
-	const int* order = levelOrder + start;
-	for (int k = t; k < n; k += T) sOrder[k] = order[k];
-	int* sLs = sOrder + n; /* level starts, relative to start */
-	for (int k = t; k <= L; k += T) sLs[k] = levelStart[start + cell + k];
 	const int* ls = sLs;
 	float maxD = 0.f, maxL = 0.f;
 	__syncthreads();
@@ -1431,27 +1463,48 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 			pgs_walk<PHASE, 0>(t, n, L, ls, sOrder, recN, recF, cs, sLv, sAv, is, friction, maxD, maxL);
 	}
 	__syncthreads();
-	for (int q = t; q < nb; q += T)
-	{
-		int b = sList[q];
-		if (fuseIntegrate && lastClass[b] == ib)
+	if (!fuseIntegrate)
+		for (int q = t; q < nb; q += T)
 		{
-			/* the body's velocity is final: integrate it here (integrateTransformsKernel) */
-			b3hRigidBodyData body = bodies[b];
-			body.m_linVel = sLv[q];
-			body.m_angVel = sAv[q];
-			b3h_integrate_body(body, ip.dt, ip.damping, ip.gravity);
-			bodies[b].m_pos = body.m_pos;
-			bodies[b].m_quat = body.m_quat;
-			bodies[b].m_linVel = body.m_linVel;
-			bodies[b].m_angVel = body.m_angVel;
-		}
-		else
-		{
+			int b = sList[q];
 			bodies[b].m_linVel = sLv[q];
 			bodies[b].m_angVel = sAv[q];
 		}
-	}
+	else
+		/* bodies whose last touching class this is have their final velocity:
+		 * integrate them here (integrateTransformsKernel).  Two bodies per
+		 * thread per round, loads first; the integration is evaluated for
+		 * every body and kept only where it applies */
+		for (int q0 = t; q0 < nb; q0 += 2 * T)
+		{
+			int qq0 = q0, qq1 = q0 + T;
+			int bq0 = sList[qq0], bq1 = qq1 < nb ? sList[qq1] : 0;
+			int lc0 = lastClass[bq0], lc1 = lastClass[bq1];
+			b3hRigidBodyData bd0 = bodies[bq0], bd1 = bodies[bq1];
+#pragma unroll
+			for (int u = 0; u < 2; u++)
+			{
+				int qq = u ? qq1 : qq0, b = u ? bq1 : bq0, lc = u ? lc1 : lc0;
+				int sv = qq < nb ? qq : nb;
+				b3hRigidBodyData body = u ? bd1 : bd0;
+				body.m_linVel = sLv[sv];
+				body.m_angVel = sAv[sv];
+				b3h_integrate_body(body, ip.dt, ip.damping, ip.gravity);
+				if (qq >= nb) continue;
+				if (lc == ib)
+				{
+					bodies[b].m_pos = body.m_pos;
+					bodies[b].m_quat = body.m_quat;
+					bodies[b].m_linVel = body.m_linVel;
+					bodies[b].m_angVel = body.m_angVel;
+				}
+				else
+				{
+					bodies[b].m_linVel = sLv[sv];
+					bodies[b].m_angVel = sAv[sv];
+				}
+			}
+		}
 	if (resid)
 	{
 		if (maxD > 0.f) atomic_max_pos(resid, maxD);
