@@ -993,6 +993,7 @@ static void pl_free(b3hPipeline* p)
 	dfree(w.htKeys);
 	dfree(w.htVals);
 	dfree(w.hist);
+	dfree(w.histKeys);
 	dfree(w.histValid);
 	dfree(w.prev);
 	dfree(w.warm);
@@ -1019,7 +1020,7 @@ static int warm_alloc(b3hPipeline* p, bool features)
 		w.capC = (int)mc;
 		w.htCap = cap;
 		if ((r = dalloc(&w.htKeys, (size_t)cap)) || (r = dalloc(&w.htVals, (size_t)cap)) || (r = dalloc(&w.hist, mc)) ||
-			(r = dalloc(&w.histValid, 1)) || (r = dalloc(&w.prev, 8 * mc)) || (r = dalloc(&w.warm, 6 * mc)) ||
+			(r = dalloc(&w.histKeys, mc)) || (r = dalloc(&w.histValid, 1)) || (r = dalloc(&w.prev, 8 * mc)) || (r = dalloc(&w.warm, 6 * mc)) ||
 			(r = dalloc(&w.logCount, 1)))
 			return r;
 		B3H_CHECK(hipMemset(w.histValid, 0, sizeof(int)));
@@ -1433,6 +1434,140 @@ extern "C" int b3h_pipeline_clear_warm_start_history(b3hPipeline* p)
 {
 	CHECK_ARG(p);
 	if (p->warmAlloc) B3H_CHECK(hipMemsetAsync(p->wb.histValid, 0, sizeof(int), p->stream));
+	return B3H_OK;
+}
+
+/* Simulation state as one byte image: header, bodies, inertias, then the
+ * warm-start history (key and entry per constraint of the last solve).  The
+ * shape tables and registrations are set-up, not state: a state loads into a
+ * pipeline with the same bodies registered. */
+namespace
+{
+const char kStateMagic[8] = {'B', '3', 'H', 'S', 'T', 'A', 'T', 'E'};
+const int kStateVersion = 1;
+struct StateHeader
+{
+	char magic[8];
+	int version;
+	int headerBytes;
+	int numBodies;
+	int numHist;
+	F4 gravity;
+	b3hSolverParams prm;
+};
+size_t state_bytes(int nb, int nh)
+{
+	return sizeof(StateHeader) + (size_t)nb * (sizeof(b3hRigidBodyData) + sizeof(b3hInertiaData)) +
+		   (size_t)nh * (sizeof(unsigned long long) + sizeof(b3hWarmHist));
+}
+}  // namespace
+
+extern "C" int b3h_pipeline_save_state(b3hPipeline* p, void* host, size_t capacityBytes, size_t* bytesOut)
+{
+	CHECK_ARG(p && bytesOut);
+	int nb = num_bodies(p), nh = 0;
+	B3H_CHECK(hipStreamSynchronize(p->stream));
+	if (p->warmAlloc) B3H_CHECK(hipMemcpy(&nh, p->wb.histValid, sizeof(int), hipMemcpyDeviceToHost));
+	size_t need = state_bytes(nb, nh);
+	*bytesOut = need;
+	if (!host) return B3H_OK;
+	if (capacityBytes < need)
+	{
+		b3h_set_error("saveState: %zu bytes needed, %zu given", need, capacityBytes);
+		return B3H_ERR_CAPACITY;
+	}
+	StateHeader h;
+	memset(&h, 0, sizeof(h));
+	memcpy(h.magic, kStateMagic, 8);
+	h.version = kStateVersion;
+	h.headerBytes = (int)sizeof(StateHeader);
+	h.numBodies = nb;
+	h.numHist = nh;
+	h.gravity = p->gravity;
+	h.prm = p->prm;
+	char* o = (char*)host;
+	memcpy(o, &h, sizeof(h));
+	o += sizeof(h);
+	if (nb > 0)
+	{
+		B3H_CHECK(hipMemcpy(o, p->np->dBodies, nb * sizeof(b3hRigidBodyData), hipMemcpyDeviceToHost));
+		o += nb * sizeof(b3hRigidBodyData);
+		B3H_CHECK(hipMemcpy(o, p->np->dInertias, nb * sizeof(b3hInertiaData), hipMemcpyDeviceToHost));
+		o += nb * sizeof(b3hInertiaData);
+	}
+	if (nh > 0)
+	{
+		B3H_CHECK(hipMemcpy(o, p->wb.histKeys, nh * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+		o += nh * sizeof(unsigned long long);
+		B3H_CHECK(hipMemcpy(o, p->wb.hist, nh * sizeof(b3hWarmHist), hipMemcpyDeviceToHost));
+	}
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_load_state(b3hPipeline* p, const void* host, size_t bytes)
+{
+	CHECK_ARG(p && host);
+	StateHeader h;
+	if (bytes < sizeof(h))
+	{
+		b3h_set_error("loadState: %zu bytes is shorter than the header", bytes);
+		return B3H_ERR_INVALID;
+	}
+	memcpy(&h, host, sizeof(h));
+	if (memcmp(h.magic, kStateMagic, 8) != 0 || h.version != kStateVersion || h.headerBytes != (int)sizeof(h))
+	{
+		b3h_set_error("loadState: not a state image of this version");
+		return B3H_ERR_INVALID;
+	}
+	int nb = num_bodies(p);
+	if (h.numBodies != nb || h.numHist < 0 || bytes != state_bytes(h.numBodies, h.numHist))
+	{
+		b3h_set_error("loadState: image holds %d bodies and %d history entries in %zu bytes; the pipeline has %d bodies",
+					  h.numBodies, h.numHist, bytes, nb);
+		return B3H_ERR_STATE;
+	}
+	if (h.numHist > p->cfg.m_maxContactCapacity)
+	{
+		b3h_set_error("loadState: %d history entries exceed the contact capacity %d", h.numHist,
+					  p->cfg.m_maxContactCapacity);
+		return B3H_ERR_CAPACITY;
+	}
+	const char* src = (const char*)host + sizeof(h);
+	const b3hRigidBodyData* bodies = (const b3hRigidBodyData*)src;
+	for (int i = 0; i < nb; i++)
+		if (bodies[i].m_collidableIdx != p->np->bodies[i].m_collidableIdx)
+		{
+			b3h_set_error("loadState: body %d has collidable %d in the image, %d in the pipeline", i,
+						  bodies[i].m_collidableIdx, p->np->bodies[i].m_collidableIdx);
+			return B3H_ERR_STATE;
+		}
+	int r = b3h_pipeline_set_solver_params(p, &h.prm);
+	if (r) return r;
+	if (h.numHist > 0 && (r = warm_alloc(p, false))) return r;
+	B3H_CHECK(hipStreamSynchronize(p->stream));
+	p->gravity = h.gravity;
+	drop_graph(p);
+	if (nb > 0)
+	{
+		memcpy(p->np->bodies.data(), src, nb * sizeof(b3hRigidBodyData));
+		B3H_CHECK(hipMemcpy(p->np->dBodies, src, nb * sizeof(b3hRigidBodyData), hipMemcpyHostToDevice));
+		src += nb * sizeof(b3hRigidBodyData);
+		memcpy(p->np->inertias.data(), src, nb * sizeof(b3hInertiaData));
+		B3H_CHECK(hipMemcpy(p->np->dInertias, src, nb * sizeof(b3hInertiaData), hipMemcpyHostToDevice));
+		src += nb * sizeof(b3hInertiaData);
+	}
+	if (p->warmAlloc)
+	{
+		if (h.numHist > 0)
+		{
+			B3H_CHECK(hipMemcpy(p->wb.histKeys, src, h.numHist * sizeof(unsigned long long), hipMemcpyHostToDevice));
+			src += h.numHist * sizeof(unsigned long long);
+			B3H_CHECK(hipMemcpy(p->wb.hist, src, h.numHist * sizeof(b3hWarmHist), hipMemcpyHostToDevice));
+		}
+		b3h_warm_rebuild(h.numHist, p->wb, p->stream);
+		B3H_CHECK(hipGetLastError());
+		B3H_CHECK(hipStreamSynchronize(p->stream));
+	}
 	return B3H_OK;
 }
 extern "C" int b3h_pipeline_get_solver_params(b3hPipeline* p, b3hSolverParams* prm)

@@ -171,7 +171,8 @@ struct b3hWarmBuffers
 	unsigned long long* htKeys;
 	int* htVals;
 	b3hWarmHist* hist;         /* [capC] */
-	int* histValid;            /* 1 int */
+	unsigned long long* histKeys; /* [capC]: (bodyA, bodyB) of each history entry */
+	int* histValid;            /* 1 int: number of history entries (0: none) */
 	float* prev;               /* [capC][8]: matched normal impulses 4, friction 2, pair matched, points */
 	float* warm;               /* [capC][6]: initial impulses in solve order */
 	float* X;                  /* [4 capC][24] features (learned mode, logging) */
@@ -191,5 +192,8 @@ void b3h_warm_prepare(const int* numContacts, int maxContacts, const b3hContactC
 					  bool features, hipStream_t s);
 void b3h_warm_finish(const int* numContacts, int maxContacts, const b3hContactConstraint4* cs, b3hWarmBuffers& wb,
 					 const b3hWarmParams& wp, bool logRows, hipStream_t s);
+/* rebuilds the hash table over entries [0, n) of wb.histKeys / wb.hist (after
+ * a state load) and sets the entry count */
+void b3h_warm_rebuild(int n, b3hWarmBuffers& wb, hipStream_t s);
 
 #endif

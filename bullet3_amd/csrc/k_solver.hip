@@ -411,17 +411,21 @@ struct Vel
 {
 	F4 lv, av;
 };
+/* v + d*s on x, y, z (multiply, then add: the order of mul4 then add4).  The w
+ * lane is carried unchanged: the reference adds d.w*s = (+-0)*s to it, which
+ * leaves every velocity w lane at the +0 that registration and integration
+ * give it, so skipping that add changes no bit. */
+__device__ __forceinline__ F4 add_scaled3(F4 v, F4 d, float s)
+{
+	return f4(v.x + d.x * s, v.y + d.y * s, v.z + d.z * s, v.w);
+}
 __device__ __forceinline__ void apply_impulse(F4 linear, F4 ang0, F4 ang1, float rl, float imA, const b3hMat3x3& iiA,
 											  float imB, const b3hMat3x3& iiB, Vel& a, Vel& b)
 {
-	F4 linImp0 = mul4(smul4(imA, linear), rl);
-	F4 linImp1 = mul4(smul4(imB, neg4(linear)), rl);
-	F4 angImp0 = mul4(mtMul1(iiA, ang0), rl);
-	F4 angImp1 = mul4(mtMul1(iiB, ang1), rl);
-	a.lv = add4(a.lv, linImp0);
-	a.av = add4(a.av, angImp0);
-	b.lv = add4(b.lv, linImp1);
-	b.av = add4(b.av, angImp1);
+	a.lv = add_scaled3(a.lv, smul4(imA, linear), rl);
+	a.av = add_scaled3(a.av, mtMul1(iiA, ang0), rl);
+	b.lv = add_scaled3(b.lv, smul4(imB, neg4(linear)), rl);
+	b.av = add_scaled3(b.av, mtMul1(iiB, ang1), rl);
 }
 __device__ __forceinline__ void atomic_max_pos(float* addr, float v)
 {
@@ -982,10 +986,10 @@ __device__ __forceinline__ void apply_pre(F4 lin, float imA, float imB, F4 Q0, F
 	F4 linear = f4(lin.x, lin.y, lin.z, 0.f);
 	F4 P0 = smul4(imA, linear);
 	F4 P1 = smul4(imB, neg4(linear));
-	a.lv = add4(a.lv, mul4(P0, rl));
-	a.av = add4(a.av, mul4(Q0, rl));
-	b.lv = add4(b.lv, mul4(P1, rl));
-	b.av = add4(b.av, mul4(Q1, rl));
+	a.lv = add_scaled3(a.lv, P0, rl);
+	a.av = add_scaled3(a.av, Q0, rl);
+	b.lv = add_scaled3(b.lv, P1, rl);
+	b.av = add_scaled3(b.av, Q1, rl);
 }
 
 template <int PHASE>

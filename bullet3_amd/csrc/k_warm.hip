@@ -294,10 +294,25 @@ __global__ void k_warm_log_advance(const int* __restrict__ nPtr, const int* __re
 	*logCount = c < logCap ? (int)c : logCap;
 }
 
+__device__ __forceinline__ void ht_insert(unsigned long long key, int k, unsigned long long* __restrict__ htKeys,
+										  int* __restrict__ htVals, int htMask)
+{
+	for (unsigned slot = mix64(key) & (unsigned)htMask;; slot = (slot + 1) & (unsigned)htMask)
+	{
+		unsigned long long old = atomicCAS(&htKeys[slot], WARM_EMPTY, key);
+		if (old == WARM_EMPTY || old == key)
+		{
+			htVals[slot] = k;
+			break;
+		}
+	}
+}
+
 /* history for the next step */
 __global__ __launch_bounds__(256) void k_warm_store(const int* __restrict__ nPtr,
 													const b3hContactConstraint4* __restrict__ cs,
 													b3hWarmHist* __restrict__ hist,
+													unsigned long long* __restrict__ histKeys,
 													unsigned long long* __restrict__ htKeys, int* __restrict__ htVals,
 													int htMask)
 {
@@ -319,18 +334,20 @@ __global__ __launch_bounds__(256) void k_warm_store(const int* __restrict__ nPtr
 		H.fric = f4(fr.x, fr.y, fr.z, (float)np);
 		hist[k] = H;
 		unsigned long long key = pair_key(c.m_bodyA, c.m_bodyB);
-		for (unsigned slot = mix64(key) & (unsigned)htMask;; slot = (slot + 1) & (unsigned)htMask)
-		{
-			unsigned long long old = atomicCAS(&htKeys[slot], WARM_EMPTY, key);
-			if (old == WARM_EMPTY || old == key)
-			{
-				htVals[slot] = k;
-				break;
-			}
-		}
+		histKeys[k] = key;
+		ht_insert(key, k, htKeys, htVals, htMask);
 	}
 }
-__global__ void k_warm_valid(const int* __restrict__ nPtr, int* __restrict__ histValid) { *histValid = *nPtr > 0; }
+/* the entry count doubles as the validity flag of the history */
+__global__ void k_warm_valid(const int* __restrict__ nPtr, int* __restrict__ histValid) { *histValid = *nPtr; }
+__global__ void k_warm_set_count(int n, int* __restrict__ histValid) { *histValid = n; }
+__global__ __launch_bounds__(256) void k_warm_reinsert(int n, const unsigned long long* __restrict__ histKeys,
+													   unsigned long long* __restrict__ htKeys,
+													   int* __restrict__ htVals, int htMask)
+{
+	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+		ht_insert(histKeys[k], k, htKeys, htVals, htMask);
+}
 
 static inline int wblocks(int maxN)
 {
@@ -374,7 +391,16 @@ void b3h_warm_finish(const int* numContacts, int maxContacts, const b3hContactCo
 						   wb.logCount);
 	}
 	b3h_fill_i32((int*)wb.htKeys, -1, 2 * wb.htCap, s);
-	hipLaunchKernelGGL(k_warm_store, dim3(gb), dim3(256), 0, s, numContacts, cs, wb.hist, wb.htKeys, wb.htVals,
-					   wb.htCap - 1);
+	hipLaunchKernelGGL(k_warm_store, dim3(gb), dim3(256), 0, s, numContacts, cs, wb.hist, wb.histKeys, wb.htKeys,
+					   wb.htVals, wb.htCap - 1);
 	hipLaunchKernelGGL(k_warm_valid, dim3(1), dim3(1), 0, s, numContacts, wb.histValid);
+}
+
+void b3h_warm_rebuild(int n, b3hWarmBuffers& wb, hipStream_t s)
+{
+	b3h_fill_i32((int*)wb.htKeys, -1, 2 * wb.htCap, s);
+	if (n > 0)
+		hipLaunchKernelGGL(k_warm_reinsert, dim3(wblocks(n)), dim3(256), 0, s, n,
+						   (const unsigned long long*)wb.histKeys, wb.htKeys, wb.htVals, wb.htCap - 1);
+	hipLaunchKernelGGL(k_warm_set_count, dim3(1), dim3(1), 0, s, n, wb.histValid);
 }

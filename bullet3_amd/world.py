@@ -195,6 +195,18 @@ class World:
     def clear_warm_history(self):
         call("b3h_pipeline_clear_warm_start_history", self.pl)
 
+    # -- state image (bodies, inertias, gravity, solver parameters, warm-start history)
+    def save_state(self):
+        n = ctypes.c_size_t()
+        call("b3h_pipeline_save_state", self.pl, None, 0, ctypes.byref(n))
+        buf = np.zeros(n.value, np.uint8)
+        call("b3h_pipeline_save_state", self.pl, ptr(buf), len(buf), ctypes.byref(n))
+        return buf[:n.value].tobytes()
+
+    def load_state(self, image):
+        buf = np.frombuffer(image, np.uint8).copy()
+        call("b3h_pipeline_load_state", self.pl, ptr(buf), len(buf))
+
     def set_contact_log(self, x_dev, t_dev, capacity_rows):
         call("b3h_pipeline_set_contact_log", self.pl, ctypes.c_void_p(x_dev) if x_dev else None,
              ctypes.c_void_p(t_dev) if t_dev else None, int(capacity_rows))

@@ -3,7 +3,7 @@
  *
  * The three handle types stand in for the reference's three classes and every
  * entry point names the method it replaces:
- *   b3hNarrowPhase  <- b3GpuNarrowPhase          src/Bullet3OpenCL/RigidBody/b3GpuNarrowPhase.h:64-154
+ *   b3hNarrowPhase  <- b3GpuNarrowPhase          src/Bullet3OpenCL/RigidBody/b3GpuNarrowPhase.h:9-99
  *   b3hBroadphase   <- b3GpuBroadphaseInterface  src/Bullet3OpenCL/BroadphaseCollision/b3GpuBroadphaseInterface.h:12-40
  *                      (b3GpuGridBroadphase / b3GpuSapBroadphase)
  *   b3hPipeline     <- b3GpuRigidBodyPipeline    src/Bullet3OpenCL/RigidBody/b3GpuRigidBodyPipeline.h:25-68
@@ -86,7 +86,7 @@ int b3h_np_get_object_transform_from_cpu(b3hNarrowPhase* np, float* position4, f
 int b3h_np_compute_contacts(b3hNarrowPhase* np, b3hInt4* pairs, int numPairs, const b3hAabb* aabbsWorldSpace,
 							int numObjects);
 /* getBodiesGpu/getNumRigidBodies, getBodyInertiasGpu, getCollidablesGpu,
- * getContactsGpu/getNumContactsGpu, getStatic0Index: b3GpuNarrowPhase.h:110-136 */
+ * getContactsGpu/getNumContactsGpu, getStatic0Index: b3GpuNarrowPhase.h:55-83 */
 int b3h_np_get_bodies_gpu(b3hNarrowPhase* np, b3hRigidBodyData** dev, int* numBodies);
 int b3h_np_get_body_inertias_gpu(b3hNarrowPhase* np, b3hInertiaData** dev, int* numBodies);
 int b3h_np_get_collidables_gpu(b3hNarrowPhase* np, b3hCollidable** dev, int* numCollidables);
@@ -227,6 +227,18 @@ int b3h_pipeline_set_warm_start_model(b3hPipeline* p, b3hMlp* net);
 int b3h_pipeline_get_warm_start_debug(b3hPipeline* p, float** warmDev, float** prevDev, float** featuresDev);
 /* Forget the impulse history (the next solve finds no previous frame). */
 int b3h_pipeline_clear_warm_start_history(b3hPipeline* p);
+/* State dump / load.  The image holds the body states and inertias, gravity,
+ * the solver parameters and the warm-start history of the last solve, so that
+ * stepping after a load repeats bit for bit what stepping after the save did.
+ * Shapes and registrations are not part of it: load into a pipeline with the
+ * same bodies registered (same count and collidable per body).  save_state
+ * with host NULL only reports the size in *bytesOut; otherwise the image is
+ * written if capacityBytes suffices (else B3H_ERR_CAPACITY).  Both synchronise
+ * the pipeline stream.  The reference keeps no state image; its nearest calls
+ * are b3GpuNarrowPhase::readbackAllBodiesToCpu / writeAllBodiesToGpu
+ * (b3GpuNarrowPhase.h:45-47), which move the bodies only. */
+int b3h_pipeline_save_state(b3hPipeline* p, void* host, size_t capacityBytes, size_t* bytesOut);
+int b3h_pipeline_load_state(b3hPipeline* p, const void* host, size_t bytes);
 /* Contact logging for training the network: after every solve, each contact
  * point appends its 24 features (as computed before the solve) to
  * featuresDev[rows][24] and its 3 targets (the solve's final impulses relative

@@ -1,7 +1,7 @@
 /*
  * b3hip.hpp -- C++ host classes with the reference's names and methods, over
  * the C ABI of b3hip.h.  A program written against
- *   b3GpuNarrowPhase          src/Bullet3OpenCL/RigidBody/b3GpuNarrowPhase.h:64-154
+ *   b3GpuNarrowPhase          src/Bullet3OpenCL/RigidBody/b3GpuNarrowPhase.h:9-99
  *   b3GpuBroadphaseInterface  src/Bullet3OpenCL/BroadphaseCollision/b3GpuBroadphaseInterface.h:12-40
  *   b3GpuGridBroadphase / b3GpuSapBroadphase
  *   b3GpuRigidBodyPipeline    src/Bullet3OpenCL/RigidBody/b3GpuRigidBodyPipeline.h:25-68
@@ -17,6 +17,8 @@
  */
 #ifndef B3HIP_HPP
 #define B3HIP_HPP
+
+#include <vector>
 
 #include "b3hip.h"
 
@@ -40,7 +42,7 @@ class b3GpuNarrowPhase
 	b3hNarrowPhase* m_np = nullptr;
 
 public:
-	/* b3GpuNarrowPhase(cl_context, cl_device_id, cl_command_queue, const b3Config&), b3GpuNarrowPhase.h:80 */
+	/* b3GpuNarrowPhase(cl_context, cl_device_id, cl_command_queue, const b3Config&), b3GpuNarrowPhase.h:25 */
 	explicit b3GpuNarrowPhase(const b3hConfig& config, void* hipStream = nullptr)
 	{
 		b3hip::ok(b3h_np_create(&config, hipStream, &m_np));
@@ -51,7 +53,7 @@ public:
 	bool isValid() const { return m_np != nullptr; }
 	b3hNarrowPhase* handle() const { return m_np; }
 
-	/* registerConvexHullShape(const float*, int, int, const float*), b3GpuNarrowPhase.h:94 */
+	/* registerConvexHullShape(const float*, int, int, const float*), b3GpuNarrowPhase.h:40 */
 	int registerConvexHullShape(const float* vertices, int strideInBytes, int numVertices, const float* scaling)
 	{
 		int idx = -1;
@@ -59,7 +61,7 @@ public:
 				   ? idx
 				   : -1;
 	}
-	/* registerRigidBody(...), b3GpuNarrowPhase.h:97 */
+	/* registerRigidBody(...), b3GpuNarrowPhase.h:42 */
 	int registerRigidBody(int collidableIndex, float mass, const float* position, const float* orientation,
 						  const float* aabbMin, const float* aabbMax, bool writeToGpu)
 	{
@@ -85,7 +87,7 @@ public:
 	{
 		return b3hip::ok(b3h_np_get_object_transform_from_cpu(m_np, position, orientation, bodyIndex));
 	}
-	/* computeContacts(cl_mem pairs, int numPairs, cl_mem aabbsWorldSpace, int numObjects), b3GpuNarrowPhase.h:108 */
+	/* computeContacts(cl_mem pairs, int numPairs, cl_mem aabbsWorldSpace, int numObjects), b3GpuNarrowPhase.h:53 */
 	void computeContacts(b3hInt4* pairs, int numPairs, const b3hAabb* aabbsWorldSpace, int numObjects)
 	{
 		b3hip::ok(b3h_np_compute_contacts(m_np, pairs, numPairs, aabbsWorldSpace, numObjects));
@@ -287,6 +289,19 @@ public:
 	 * frame, 2 learned with the network given here */
 	bool setWarmStartModel(b3hMlp* net) { return b3hip::ok(b3h_pipeline_set_warm_start_model(m_p, net)); }
 	void clearWarmStartHistory() { b3hip::ok(b3h_pipeline_clear_warm_start_history(m_p)); }
+	/* state image: bodies, inertias, gravity, solver parameters and the
+	 * warm-start history (no counterpart in the reference) */
+	bool saveState(std::vector<unsigned char>& out)
+	{
+		size_t n = 0;
+		if (!b3hip::ok(b3h_pipeline_save_state(m_p, nullptr, 0, &n))) return false;
+		out.resize(n);
+		return b3hip::ok(b3h_pipeline_save_state(m_p, out.data(), out.size(), &n));
+	}
+	bool loadState(const std::vector<unsigned char>& in)
+	{
+		return b3hip::ok(b3h_pipeline_load_state(m_p, in.data(), in.size()));
+	}
 	/* contact logging for training the network (device rows [cap][24] and [cap][3]) */
 	bool setContactLog(float* featuresDev, float* targetsDev, int capacityRows)
 	{
