@@ -105,6 +105,8 @@ _SIGS = {
     "b3h_pipeline_get_solver_debug": ([_vp, _pvp, _pvp, _pvp, _pvp, _pvp, _pi], _i),
     "b3h_pipeline_set_pgs_body_cache": ([_vp, _i], _i),
     "b3h_pipeline_get_pgs_levels": ([_vp, _vp], _i),
+    "b3h_pipeline_get_pgs_parts": ([_vp, _vp, _vp], _i),
+    "b3h_pipeline_set_pgs_max_parts": ([_vp, _i], _i),
     "b3h_pipeline_set_pgs_inertia_cache": ([_vp, _i], _i),
     "b3h_pipeline_set_track_residual": ([_vp, _i], _i),
     "b3h_pipeline_get_residuals": ([_vp, _vp, _i], _i),

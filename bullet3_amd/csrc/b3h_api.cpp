@@ -985,6 +985,8 @@ static void pl_free(b3hPipeline* p)
 	dfree(p->sb.recF);
 	dfree(p->sb.levelOrder);
 	dfree(p->sb.levelStart);
+	dfree(p->sb.partMeta);
+	dfree(p->sb.bodyPart);
 	dfree(p->sb.cellLevels);
 	dfree(p->sb.lastClass);
 	dfree(p->sb.residSnap);
@@ -1100,7 +1102,8 @@ extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const 
 		(r = dalloc(&p->sb.cellBodies, 2 * (size_t)mc + 64)) || (r = dalloc(&p->sb.cellBodyCount, B3H_N_CELLS)) ||
 		(r = dalloc(&p->sb.cslot, 2 * (size_t)mc + 2)) || (r = dalloc(&p->sb.recN, 10 * (size_t)mc)) ||
 		(r = dalloc(&p->sb.recF, 8 * (size_t)mc)) || (r = dalloc(&p->sb.levelOrder, mc)) ||
-		(r = dalloc(&p->sb.levelStart, (size_t)mc + B3H_N_CELLS + 8)) || (r = dalloc(&p->sb.cellLevels, B3H_N_CELLS)) ||
+		(r = dalloc(&p->sb.levelStart, (size_t)mc + B3H_N_CELLS * B3H_PGS_PARTS + 8)) ||
+		(r = dalloc(&p->sb.partMeta, B3H_N_CELLS * B3H_PGS_PARTS)) || (r = dalloc(&p->sb.bodyPart, 2 * (size_t)mc + 64)) || (r = dalloc(&p->sb.cellLevels, B3H_N_CELLS)) ||
 		(r = dalloc(&p->sb.lastClass, (size_t)cfg->m_maxConvexBodies + 16)) ||
 		(r = dalloc(&p->sb.residSnap, 6 * (size_t)mc)) || (r = dalloc(&p->sb.residL2, 4 * B3H_MAX_ITERATIONS)))
 	{
@@ -1612,11 +1615,39 @@ extern "C" int b3h_pipeline_set_pgs_inertia_cache(b3hPipeline* p, int enable)
 	drop_graph(p);
 	return B3H_OK;
 }
+extern "C" int b3h_pipeline_set_pgs_max_parts(b3hPipeline* p, int maxParts)
+{
+	CHECK_ARG(p && maxParts >= 1 && maxParts <= B3H_PGS_PARTS);
+	p->sb.pgsMaxParts = maxParts;
+	drop_graph(p);
+	return B3H_OK;
+}
 extern "C" int b3h_pipeline_get_pgs_levels(b3hPipeline* p, int* levels)
 {
 	CHECK_ARG(p && levels);
 	B3H_CHECK(hipMemcpyAsync(levels, p->sb.cellLevels, sizeof(int) * B3H_N_CELLS, hipMemcpyDeviceToHost, p->stream));
 	B3H_CHECK(hipStreamSynchronize(p->stream));
+	return B3H_OK;
+}
+extern "C" int b3h_pipeline_get_pgs_parts(b3hPipeline* p, int* parts, int* largest)
+{
+	CHECK_ARG(p && parts);
+	std::vector<b3hInt4> meta((size_t)B3H_N_CELLS * B3H_PGS_PARTS);
+	B3H_CHECK(hipMemcpyAsync(meta.data(), p->sb.partMeta, meta.size() * sizeof(b3hInt4), hipMemcpyDeviceToHost,
+							 p->stream));
+	B3H_CHECK(hipStreamSynchronize(p->stream));
+	for (int c = 0; c < B3H_N_CELLS; c++)
+	{
+		int k = 0, big = 0;
+		for (int q = 0; q < B3H_PGS_PARTS; q++)
+		{
+			const b3hInt4& m = meta[(size_t)c * B3H_PGS_PARTS + q];
+			if (m.y > 0) k++;
+			if (m.y > big) big = m.y;
+		}
+		parts[c] = k;
+		if (largest) largest[c] = big;
+	}
 	return B3H_OK;
 }
 

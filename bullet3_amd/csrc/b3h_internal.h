@@ -94,6 +94,9 @@ void b3h_gather_contacts(const int* contactPair, const int* numContacts, int max
 #define B3H_N_SPLIT_Z 8
 #define B3H_N_CELLS (B3H_N_SPLIT_X * B3H_N_SPLIT_Y * B3H_N_SPLIT_Z)
 #define B3H_N_CELL_BATCHES 8
+/* work-groups a cell's sweep may be split into: the cell's copies in space
+ * (cells wrap with the split counts) share no body and are solved apart */
+#define B3H_PGS_PARTS 8
 
 /* integration fused into the last friction sweep */
 struct b3hIntegrateParams
@@ -122,11 +125,14 @@ struct b3hSolverBuffers
 	int* cslot;           /* per constraint: local slots of bodyA, bodyB in its cell's list (-1: static) */
 	int pgsBodyCap;       /* < 0: the LDS budget; otherwise a lower cap on cached bodies per cell */
 	int pgsNoInertiaCache; /* 1: never cache inverse inertias in LDS (tests) */
+	int pgsMaxParts;       /* work-groups a cell may be split into (0: B3H_PGS_PARTS) */
 	b3hFloat4* recN;      /* per constraint: 10 float4 of normal-row invariants */
 	b3hFloat4* recF;      /* per constraint: 8 float4 of friction-row invariants */
-	int* levelOrder;      /* per cell region [off, off+n): constraints grouped by dependency level */
-	int* levelStart;      /* per cell region [off+cell, off+cell+L+1) */
+	int* levelOrder;      /* per cell region [off, off+n): per part, constraints grouped by dependency level */
+	int* levelStart;      /* per cell region [off+PARTS*cell, off+PARTS*cell+n+PARTS): per part its level starts */
 	int* cellLevels;      /* 256: levels per cell, -1 = global walk */
+	b3hInt4* partMeta;    /* 256 x PARTS: (first position, count, level-start offset, levels) */
+	int* bodyPart;        /* per cell at 2*cellOffsets[cell]: the part of each listed body */
 	int* lastClass;       /* per body: the last cell-parity class whose cells touch it (-1: none) */
 	int maxContacts;
 	float* residSnap;     /* per constraint: 6 impulses before the sweep (residual tracking) */

@@ -776,7 +776,9 @@ __global__ __launch_bounds__(256) void k_cell_bodies(const int* __restrict__ cel
 #define PGS_NREC 10
 #define PGS_FREC 8
 #define PGS_GATHER 8
+#ifndef PGS_THREADS
 #define PGS_THREADS 256
+#endif
 #define PGS_NSLOT (PGS_NREC + 1) /* record + applied normal impulses */
 #define PGS_FSLOT (PGS_FREC + 2) /* record + normal impulses + friction impulses */
 __device__ __forceinline__ float ibits(int v) { return __builtin_bit_cast(float, v); }
@@ -856,40 +858,78 @@ __host__ __device__ __forceinline__ size_t pgs_level_lds(int nb, int n, bool cac
 		   (size_t)(n + 2) * 4;
 }
 
-/* ---------------- per-cell dependency levels ----------------
+/* ---------------- per-cell dependency levels and parts ----------------
  * Gauss-Seidel results depend only on the order in which each body's updates
  * happen.  Walking the cell in the reference order (rows of up to 64 aligned to
  * batch starts; a row shares no dynamic body), a constraint's level is one more
  * than the last level that touched either of its bodies.  Constraints of one
  * level share no dynamic body, and every body sees its updates in the reference
  * order, so solving level by level reproduces the sequential sweep bit for
- * bit.  Output: per cell the constraints grouped by level (levelOrder, region
- * [off, off+n)), level starts (levelStart, region [off+cell, off+cell+L+1)) and
- * L (cellLevels; -1 when the cell is too large for LDS: it then takes the
- * global walk). */
+ * bit.
+ *
+ * Parts: cell indices wrap with the split counts, so one cell is several
+ * copies in space (tiles B3H_N_SPLIT_X x Y x Z cells apart).  When no dynamic
+ * body has constraints in two tiles of the cell (checked here), the tiles
+ * share no body and their constraints can be solved by different work-groups
+ * at once without changing any body's update order.  The tiles are packed into
+ * at most B3H_PGS_PARTS parts, largest first onto the least loaded part.
+ *
+ * Output per cell: the constraints part by part, each part grouped by level
+ * (levelOrder, region [off, off+n)); per part its level starts relative to the
+ * part's first position, empty levels dropped (levelStart, region starting at
+ * off + PARTS*cell); partMeta[cell][p] = (first position, count, offset of the
+ * level starts, levels); the part of each listed body (bodyPart); L
+ * (cellLevels; -1 when the cell is too large for LDS: it then takes the global
+ * walk, as one part). */
+__device__ __forceinline__ int floor_div_pow2(int x, int d) { return x >= 0 ? x / d : -((-x + d - 1) / d); }
+__device__ __forceinline__ unsigned tile_key(F4 p, float scale)
+{
+	int x = (int)((p.x - ((p.x < 0.f) ? 1.f : 0.f)) * scale);
+	int y = (int)((p.y - ((p.y < 0.f) ? 1.f : 0.f)) * scale);
+	int z = (int)((p.z - ((p.z < 0.f) ? 1.f : 0.f)) * scale);
+	unsigned tx = (unsigned)floor_div_pow2(x, B3H_N_SPLIT_X) & 1023u;
+	unsigned ty = (unsigned)floor_div_pow2(y, B3H_N_SPLIT_Y) & 1023u;
+	unsigned tz = (unsigned)floor_div_pow2(z, B3H_N_SPLIT_Z) & 1023u;
+	return tx | (ty << 10) | (tz << 20);
+}
+#define PGS_MAX_TILES 64
 __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cellCounts,
 													 const int* __restrict__ cellOffsets,
 													 const int* __restrict__ batchIdx,
 													 const int* __restrict__ cellBodyCount, const int* __restrict__ cslot,
+													 const b3hContactConstraint4* __restrict__ cs,
+													 const b3hRigidBodyData* __restrict__ bodies, float scale,
 													 int bodyCap, int ldsInts, int pgsLdsBytes, int* __restrict__ levelOrder,
-													 int* __restrict__ levelStart, int* __restrict__ cellLevels)
+													 int* __restrict__ levelStart, int* __restrict__ cellLevels,
+													 b3hInt4* __restrict__ partMeta, int* __restrict__ bodyPart, int maxParts)
 {
 	extern __shared__ int sm[];
-	__shared__ int sL;
+	__shared__ int sL, sP, sNumTiles;
+	__shared__ unsigned tKey[PGS_MAX_TILES];
+	__shared__ int tCount[PGS_MAX_TILES], tPart[PGS_MAX_TILES];
+	__shared__ int pCount[B3H_PGS_PARTS], pLevels[B3H_PGS_PARTS];
 	int cell = blockIdx.x;
 	int n = cellCounts[cell];
 	int off = cellOffsets[cell];
 	int nb = cellBodyCount[cell];
 	int t = threadIdx.x;
+	b3hInt4* meta = partMeta + cell * B3H_PGS_PARTS;
+	if (t < B3H_PGS_PARTS) meta[t] = b3hInt4{0, 0, 0, 0};
 	if (n == 0)
 	{
 		if (t == 0) cellLevels[cell] = 0;
 		return;
 	}
-	/* last[nb] | lev[n] | cnt[n + 1] */
+	int* lsOut = levelStart + off + B3H_PGS_PARTS * cell;
+	/* last[nb] | lev[n] | cnt[max(n + 1, PARTS (L + 1))] */
 	if (nb > bodyCap || nb + 2 * n + 1 > ldsInts || pgs_level_lds(nb, n, false) > (size_t)pgsLdsBytes)
 	{
-		if (t == 0) cellLevels[cell] = -1;
+		if (t == 0)
+		{
+			cellLevels[cell] = -1;
+			meta[0] = b3hInt4{0, n, 0, -1};
+		}
+		for (int k = t; k < nb; k += blockDim.x) bodyPart[2 * (size_t)off + k] = 0;
 		return;
 	}
 	int* last = sm;
@@ -897,6 +937,13 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 	int* cnt = lev + n;
 	for (int k = t; k < nb; k += blockDim.x) last[k] = 0;
 	for (int k = t; k <= n; k += blockDim.x) cnt[k] = 0;
+	if (t < PGS_MAX_TILES)
+	{
+		tKey[t] = 0xffffffffu;
+		tCount[t] = 0;
+		tPart[t] = 0;
+	}
+	if (t == 0) sNumTiles = 0;
 	__syncthreads();
 	if (t < 64)
 	{
@@ -939,25 +986,131 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 	}
 	__syncthreads();
 	int L = sL;
-	for (int k = t; k < n; k += blockDim.x) atomicAdd(&cnt[lev[k]], 1);
+	/* tiles: key of each constraint's first dynamic body, counted in a small
+	 * open-addressing table; an overflowing table leaves the cell in one part */
+	bool split = nb + n + B3H_PGS_PARTS * (L + 1) <= ldsInts;
+	for (int k = t; k < nb; k += blockDim.x) last[k] = -1; /* now: the part of each body */
+	for (int k = t; k < n && split; k += blockDim.x)
+	{
+		int sA = cslot[2 * (size_t)(off + k)];
+		const b3hContactConstraint4& c = cs[off + k];
+		unsigned body = sA >= 0 ? c.m_bodyA : c.m_bodyB;
+		unsigned key = tile_key(bodies[body].m_pos, scale);
+		int slot = (int)((key * 2654435761u) >> 26); /* 6 bits */
+		int probes = 0;
+		for (;; slot = (slot + 1) & (PGS_MAX_TILES - 1))
+		{
+			unsigned old = atomicCAS(&tKey[slot], 0xffffffffu, key);
+			if (old == 0xffffffffu || old == key) break;
+			if (++probes == PGS_MAX_TILES)
+			{
+				slot = -1;
+				break;
+			}
+		}
+		if (slot < 0)
+			sNumTiles = PGS_MAX_TILES + 1;
+		else
+		{
+			atomicAdd(&tCount[slot], 1);
+			lev[k] |= slot << 24;
+		}
+	}
 	__syncthreads();
 	if (t == 0)
 	{
-		int run = 0;
-		for (int l = 0; l < L; l++)
+		int P = 1;
+		if (split && sNumTiles <= PGS_MAX_TILES)
 		{
-			int c = cnt[l];
-			cnt[l] = run;
-			levelStart[off + cell + l] = run;
-			run += c;
+			for (int q = 0; q < B3H_PGS_PARTS; q++) pCount[q] = 0;
+			/* largest tile first (ties: smaller key) onto the least loaded part */
+			for (;;)
+			{
+				int best = -1;
+				for (int q = 0; q < PGS_MAX_TILES; q++)
+					if (tCount[q] > 0 && tPart[q] == 0 &&
+						(best < 0 || tCount[q] > tCount[best] || (tCount[q] == tCount[best] && tKey[q] < tKey[best])))
+						best = q;
+				if (best < 0) break;
+				int pm = 0;
+				for (int q = 1; q < maxParts; q++)
+					if (pCount[q] < pCount[pm]) pm = q;
+				pCount[pm] += tCount[best];
+				tPart[best] = pm + 1; /* 1-based while assigning */
+				P = P > pm + 1 ? P : pm + 1;
+			}
+			for (int q = 0; q < PGS_MAX_TILES; q++) tPart[q] = tPart[q] > 0 ? tPart[q] - 1 : 0;
 		}
-		levelStart[off + cell + L] = run;
+		else
+			for (int q = 0; q < PGS_MAX_TILES; q++) tPart[q] = 0;
+		sP = P;
+	}
+	__syncthreads();
+	int P = sP;
+	/* a body with constraints in two parts: solve the cell as one part */
+	__shared__ int conflict;
+	if (t == 0) conflict = 0;
+	__syncthreads();
+	for (int k = t; k < n && P > 1; k += blockDim.x)
+	{
+		int part = tPart[(unsigned)lev[k] >> 24];
+		int sA = cslot[2 * (size_t)(off + k)], sB = cslot[2 * (size_t)(off + k) + 1];
+		if (sA >= 0)
+		{
+			int old = atomicCAS(&last[sA], -1, part);
+			if (old != -1 && old != part) conflict = 1;
+		}
+		if (sB >= 0)
+		{
+			int old = atomicCAS(&last[sB], -1, part);
+			if (old != -1 && old != part) conflict = 1;
+		}
+	}
+	__syncthreads();
+	if (conflict) P = 1;
+	int stride = L + 1; /* counters of part p: cnt[p * stride + level] */
+	for (int k = t; k < P * stride; k += blockDim.x) cnt[k] = 0;
+	__syncthreads();
+	for (int k = t; k < n; k += blockDim.x)
+	{
+		int part = P > 1 ? tPart[(unsigned)lev[k] >> 24] : 0;
+		int lv = lev[k] & 0xffffff;
+		lev[k] = lv | (part << 24);
+		atomicAdd(&cnt[part * stride + lv], 1);
+	}
+	for (int k = t; k < nb; k += blockDim.x) bodyPart[2 * (size_t)off + k] = P > 1 ? last[k] : 0;
+	__syncthreads();
+	if (t == 0)
+	{
+		int run = 0, lsOfs = 0;
+		for (int q = 0; q < B3H_PGS_PARTS; q++)
+		{
+			if (q >= P)
+			{
+				meta[q] = b3hInt4{run, 0, lsOfs, 0};
+				continue;
+			}
+			int first = run, Lp = 0;
+			for (int l = 0; l < L; l++)
+			{
+				int c = cnt[q * stride + l];
+				cnt[q * stride + l] = run;
+				if (c == 0) continue;
+				lsOut[lsOfs + Lp] = run - first;
+				Lp++;
+				run += c;
+			}
+			lsOut[lsOfs + Lp] = run - first;
+			meta[q] = b3hInt4{first, run - first, lsOfs, Lp};
+			lsOfs += Lp + 1;
+		}
 		cellLevels[cell] = L;
 	}
 	__syncthreads();
 	for (int k = t; k < n; k += blockDim.x)
 	{
-		int pos = atomicAdd(&cnt[lev[k]], 1);
+		int part = (unsigned)lev[k] >> 24, lv = lev[k] & 0xffffff;
+		int pos = atomicAdd(&cnt[part * stride + lv], 1);
 		levelOrder[off + pos] = off + k;
 	}
 }
@@ -1345,23 +1498,30 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 														   const int* __restrict__ cellBodyCount,
 														   const int* __restrict__ cellLevels,
 														   const int* __restrict__ levelStart,
-														   const int* __restrict__ levelOrder, const F4* __restrict__ recN,
+														   const int* __restrict__ levelOrder,
+														   const b3hInt4* __restrict__ partMeta,
+														   const int* __restrict__ bodyPart, const F4* __restrict__ recN,
 														   const F4* __restrict__ recF, int ib, float friction,
 														   const float* __restrict__ warm, float* __restrict__ resid,
 														   const int* __restrict__ lastClass, int fuseIntegrate,
 														   b3hIntegrateParams ip, int ldsBytes)
 {
 	extern __shared__ F4 smem4[];
-	int cell = cell_of(blockIdx.x, ib);
-	int n = cellCounts[cell];
-	if (n == 0) return;
+	/* blocks [part * cells of the class, ...): part-major, so part 0 of every
+	 * cell of the class comes first */
+	constexpr int CELLS_PER_CLASS = B3H_N_CELLS / B3H_N_CELL_BATCHES;
+	const int part = blockIdx.x / CELLS_PER_CLASS;
+	int cell = cell_of(blockIdx.x % CELLS_PER_CLASS, ib);
+	int cellN = cellCounts[cell];
+	if (cellN == 0) return;
 	int start = cellOffsets[cell];
-	int end = start + n;
+	int end = start + cellN;
 	int t = threadIdx.x;
 	const int T = PGS_THREADS;
 	int L = cellLevels[cell];
 	if (L < 0)
 	{
+		if (part != 0) return;
 		/* cell too large for the LDS cache: the reference's global walk on one wave */
 		int maxBatch = batchSizes[cell];
 		int idx = start + t;
@@ -1394,10 +1554,16 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		}
 		return;
 	}
+	const b3hInt4 pm = partMeta[cell * B3H_PGS_PARTS + part];
+	const int n = pm.y; /* this part's constraints */
+	if (n == 0) return;
+	L = pm.w;
 	int nb = cellBodyCount[cell];
+	const int* ownerOf = bodyPart + 2 * (size_t)start; /* part of each listed body */
 	/* with room for it, 3 (nb + 2) inverse world inertia rows (slot nb is zero
-	 * for static bodies, nb + 1 padding); then the velocities */
-	bool cacheI = pgs_level_lds(nb, n, true) <= (size_t)ldsBytes;
+	 * for static bodies, nb + 1 padding); then the velocities.  Sized for the
+	 * whole cell (slots are the cell's), filled for this part's bodies only */
+	bool cacheI = pgs_level_lds(nb, cellN, true) <= (size_t)ldsBytes;
 	F4* sI = smem4;
 	F4* sLv = cacheI ? sI + 3 * (nb + 2) : smem4; /* nb + 1 (the last is padding) */
 	F4* sAv = sLv + nb + 1;   /* nb + 1 */
@@ -1406,8 +1572,8 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	int* sLs = sOrder + n + 1;         /* L + 2: level starts relative to start */
 	/* stage the body list, the level order and the level starts together */
 	stage_ints(sList, cellBodies + 2 * (size_t)start, nb, t);
-	stage_ints(sOrder, levelOrder + start, n, t);
-	stage_ints(sLs, levelStart + start + cell, L + 1, t);
+	stage_ints(sOrder, levelOrder + start + pm.x, n, t);
+	stage_ints(sLs, levelStart + start + B3H_PGS_PARTS * cell + pm.z, L + 1, t);
 	if (cacheI && t < 3) sI[3 * nb + t] = f4(0.f, 0.f, 0.f, 0.f);
 	__syncthreads();
 	/* gather the velocities and inverse inertias, four bodies per thread per
@@ -1421,6 +1587,7 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		for (int u = 0; u < 4; u++)
 		{
 			kk[u] = k0 + u * T;
+			if (kk[u] < nb && ownerOf[kk[u]] != part) kk[u] = nb; /* another part's body */
 			bb[u] = kk[u] < nb ? sList[kk[u]] : 0;
 		}
 #pragma unroll
@@ -1470,6 +1637,7 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	if (!fuseIntegrate)
 		for (int q = t; q < nb; q += T)
 		{
+			if (ownerOf[q] != part) continue;
 			int b = sList[q];
 			bodies[b].m_linVel = sLv[q];
 			bodies[b].m_angVel = sAv[q];
@@ -1482,7 +1650,9 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		for (int q0 = t; q0 < nb; q0 += 2 * T)
 		{
 			int qq0 = q0, qq1 = q0 + T;
-			int bq0 = sList[qq0], bq1 = qq1 < nb ? sList[qq1] : 0;
+			if (ownerOf[qq0] != part) qq0 = nb; /* another part's body: evaluated on body 0, not stored */
+			if (qq1 < nb && ownerOf[qq1] != part) qq1 = nb;
+			int bq0 = qq0 < nb ? sList[qq0] : 0, bq1 = qq1 < nb ? sList[qq1] : 0;
 			int lc0 = lastClass[bq0], lc1 = lastClass[bq1];
 			b3hRigidBodyData bd0 = bodies[bq0], bd1 = bodies[bq1];
 #pragma unroll
@@ -1591,8 +1761,10 @@ void b3h_solve_setup(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaDa
 	hipLaunchKernelGGL(k_pgs_prep, dim3(gb), dim3(256), 0, s, numContacts, sb.constraints, sb.cslot, bodies, inertias,
 					   sb.recN, sb.recF);
 	hipLaunchKernelGGL(k_cell_levels, dim3(B3H_N_CELLS), dim3(256), (size_t)ldsCapBytes, s, sb.cellCounts,
-					   sb.cellOffsets, sb.batchIdx, sb.cellBodyCount, sb.cslot, cap, ldsCapBytes / 4, (int)pgsLds,
-					   sb.levelOrder, sb.levelStart, sb.cellLevels);
+					   sb.cellOffsets, sb.batchIdx, sb.cellBodyCount, sb.cslot, (const b3hContactConstraint4*)sb.constraints,
+					   bodies, 1.f / prm.m_batchCellSize, cap, ldsCapBytes / 4, (int)pgsLds, sb.levelOrder,
+					   sb.levelStart, sb.cellLevels, sb.partMeta, sb.bodyPart,
+					   sb.pgsMaxParts > 0 && sb.pgsMaxParts < B3H_PGS_PARTS ? sb.pgsMaxParts : B3H_PGS_PARTS);
 }
 
 /* residual tracking (off the hot path): over the rows one sweep of `phase`
@@ -1660,11 +1832,12 @@ void b3h_solve_sweeps(b3hRigidBodyData* bodies, const b3hInertiaData* inertias, 
 	int K = prm.m_numIterations;
 	size_t pgsLds = (size_t)s_pgsLdsBytes;
 	if (trackResidual) b3h_fill_i32((int*)sb.resid, 0, 4 * K, s);
-	const dim3 grid(B3H_N_CELLS / B3H_N_CELL_BATCHES), block(PGS_THREADS);
+	const dim3 grid(B3H_N_CELLS / B3H_N_CELL_BATCHES * B3H_PGS_PARTS), block(PGS_THREADS);
 	b3hIntegrateParams ip = integrate ? *integrate : b3hIntegrateParams{0.f, 0.f, f4(0, 0, 0, 0)};
 #define B3H_PGS_ARGS(ib, resid_, warm_, fuse_)                                                                         \
 	bodies, inertias, sb.constraints, sb.cellCounts, sb.cellOffsets, sb.batchSizes, sb.batchIdx, sb.cellBodies, \
-		sb.cellBodyCount, sb.cellLevels, sb.levelStart, sb.levelOrder, (const F4*)sb.recN, (const F4*)sb.recF, ib, \
+		sb.cellBodyCount, sb.cellLevels, sb.levelStart, sb.levelOrder, (const b3hInt4*)sb.partMeta,               \
+		(const int*)sb.bodyPart, (const F4*)sb.recN, (const F4*)sb.recF, ib,                                   \
 		prm.m_frictionCoeff, warm_, resid_, (const int*)sb.lastClass, fuse_, ip, sb.pgsNoInertiaCache ? 0 : (int)pgsLds
 	if (sb.warmImpulses)
 		for (int ib = 0; ib < B3H_N_CELL_BATCHES; ib++)

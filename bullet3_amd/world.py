@@ -147,6 +147,12 @@ class World:
                     counts=capi.dtoh(cc.value, 256 * 4, np.int32), offsets=capi.dtoh(co.value, 257 * 4, np.int32),
                     batch_sizes=capi.dtoh(bs.value, 256 * 4, np.int32))
 
+    def pgs_parts(self):
+        """(work-groups per solver cell, constraints of its largest part) of the last solve"""
+        parts, big = np.zeros(256, np.int32), np.zeros(256, np.int32)
+        call("b3h_pipeline_get_pgs_parts", self.pl, ptr(parts), ptr(big))
+        return parts, big
+
     def static0_index(self):
         v = ctypes.c_int()
         call("b3h_np_get_static0_index", self.np_, ctypes.byref(v))

@@ -197,6 +197,15 @@ int b3h_pipeline_set_pgs_inertia_cache(b3hPipeline* p, int enable);
 /* Dependency levels per solver cell of the last solve (host int[256]; -1: the
  * cell took the global walk) -- syncs.  No counterpart in the reference. */
 int b3h_pipeline_get_pgs_levels(b3hPipeline* p, int* levels256);
+/* Work-groups each solver cell's sweep was split into in the last solve (host
+ * int[256]: the cell's copies in space that share no body are solved apart, up
+ * to 8 parts) and, if largest256 != NULL, the constraint count of its largest
+ * part -- syncs.  No counterpart in the reference (its sweep solves a cell on
+ * one work-group). */
+int b3h_pipeline_get_pgs_parts(b3hPipeline* p, int* parts256, int* largest256);
+/* Upper bound on those parts (1..8, default 8; 1 solves every cell on one
+ * work-group, as the reference does).  Results do not depend on it. */
+int b3h_pipeline_set_pgs_max_parts(b3hPipeline* p, int maxParts);
 /* Per-sweep residuals of the last solve (host, 2*iterations floats: normal sweeps
  * then friction sweeps); requires b3h_pipeline_set_track_residual(p, 1). */
 int b3h_pipeline_set_track_residual(b3hPipeline* p, int enable);

@@ -170,6 +170,44 @@ def test_solver_bit_exact(scene, cache):
     w.close()
 
 
+@pytest.mark.parametrize("parts", [1, 8])
+def test_solver_parts_bit_exact(parts):
+    """3 m solver cells wrap every 24 m in x and z and 12 m in y, so the cells
+    of a 26 m wide pile hold two copies in space, which share no body: with
+    up to 8 parts each such cell is solved by several work-groups at once,
+    with 1 part by one.  Either way the sweep must equal the oracle's
+    sequential one.  (Cells must stay wider than a body: cells of one parity
+    class are one cell apart and are solved at once, as in the reference.)"""
+    sc = small_pile(n=(26, 2, 26), spacing=1.0)
+    w = World(sc)
+    sp = w.solver_params()
+    sp.m_numIterations = 10
+    sp.m_batchCellSize = 3.0
+    w.set_solver(sp)
+    capi.call("b3h_pipeline_set_pgs_max_parts", w.pl, parts)
+    ow = OracleWorld(w)
+    run_steps(w, ow, 2)
+    assert bits_equal(w.bodies(), ow.bodies), first_diff(w.bodies(), ow.bodies)
+    capi.call("b3h_pipeline_setup_gpu_aabbs_full", w.pl)
+    capi.call("b3h_bp_calculate_overlapping_pairs", w.bp, w.cfg.m_maxBroadphasePairs)
+    pairs = w.pairs()
+    d = ctypes.c_void_p()
+    capi.call("b3h_bp_get_overlapping_pair_buffer", w.bp, ctypes.byref(d))
+    capi.call("b3h_np_compute_contacts", w.np_, d, len(pairs), None, 0)
+    contacts = w.contacts()
+    b = w.bodies()
+    capi.call("b3h_pipeline_solve_contacts", w.pl)
+    split, largest = w.pgs_parts()
+    print("parts cap %d: cells split %d, parts max %d" % (parts, int(np.sum(split > 1)), int(split.max())), flush=True)
+    assert (split.max() > 1) == (parts > 1)
+    dbg = w.solver_debug()
+    cpu = po.solve_contacts(b, ow.inertias, contacts, ow.solver, static_idx=w.static0_index())
+    assert bits_equal(dbg["constraints"], cpu["constraints"].view(layout.CONSTRAINT)), \
+        first_diff(dbg["constraints"], cpu["constraints"].view(layout.CONSTRAINT))
+    assert bits_equal(w.bodies(), b), first_diff(w.bodies(), b)
+    w.close()
+
+
 @pytest.mark.parametrize("kind", ["grid", "sap"])
 def test_steps_bit_exact(scene, kind):
     w = World(scene, broadphase=kind)
@@ -185,6 +223,27 @@ def test_steps_bit_exact(scene, kind):
             gb = w.bodies()
             assert bits_equal(gb, ow.bodies), "step %d" % s
     assert ow.w.numContacts > 0
+    w.close()
+
+
+def test_steps_split_cells_bit_exact():
+    """stepping (fused integration in the last friction sweep) with cells
+    solved as several parts, against the oracle"""
+    sc = small_pile(n=(26, 2, 26), spacing=1.0, drop=0.5)
+    w = World(sc)
+    sp = w.solver_params()
+    sp.m_numIterations = 8
+    sp.m_batchCellSize = 3.0
+    w.set_solver(sp)
+    ow = OracleWorld(w)
+    split = 0
+    for s in range(30):
+        w.step()
+        ow.step()
+        split = max(split, int(w.pgs_parts()[0].max()))
+        if s % 10 == 9:
+            assert bits_equal(w.bodies(), ow.bodies), "step %d" % s
+    assert split > 1 and ow.w.numContacts > 0
     w.close()
 
 
