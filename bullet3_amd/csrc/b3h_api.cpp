@@ -208,7 +208,7 @@ extern "C" int b3h_np_create(const b3hConfig* cfg, void* stream, b3hNarrowPhase*
 	np->dSepNormal = nullptr;
 	np->dMpr = nullptr;
 	np->contactsValid = true;
-	np->lanesPerPair = 1;
+	np->lanesPerPair = 64;
 	if ((r = make_stream(stream, &np->stream, &np->ownStream)) || (r = np_alloc_work(np)) || (r = b3h_np_module_init()))
 	{
 		b3h_np_destroy(np);

@@ -459,17 +459,57 @@ __device__ __forceinline__ int mpr_penetration(SP& S, float* depth, F4* dir, F4*
 
 
 /* ---------------- SAT ---------------- */
-__device__ __forceinline__ void project(const F4* vs, int nv, F4 pos, F4 orn, F4 dir, float* mn, float* mx)
+/* project (sat.cl): the interval of dot(vertex, localDir) over the hull's
+ * vertices, shifted by dot(pos, dir).  Hull vertices are stored with w = 0, so
+ * the w term of the reference's float4 dot adds a zero; it is left out, and
+ * the interval ends are taken with min/max instead of compare-and-select.
+ * Both can only change the sign of a zero end, and the ends are used in
+ * comparisons and differences whose outcome a signed zero does not change
+ * (the overlap test, the depth comparisons, the axis chosen). */
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) const F4 LdsF4;
+#define LDS_PTR(p) ((LdsF4*)(p))
+#else
+#define LDS_PTR(p) (p)
+#endif
+/* the interval loop, four vertices per round with the loads issued first;
+ * VP is an LDS or a global pointer so the loads are ds_read / global_load */
+template <typename VP>
+__device__ __forceinline__ float vdot3(VP vs, int i, F4 d)
+{
+	return vs[i].x * d.x + vs[i].y * d.y + vs[i].z * d.z;
+}
+template <typename VP>
+__device__ __forceinline__ void project_loop(VP vs, int nv, F4 d, float& lo, float& hi)
+{
+	int i = 0;
+	for (; i + 4 <= nv; i += 4)
+	{
+		float pa = vdot3(vs, i, d), pb = vdot3(vs, i + 1, d), pc = vdot3(vs, i + 2, d), pe = vdot3(vs, i + 3, d);
+		lo = fminf(lo, fminf(fminf(pa, pb), fminf(pc, pe)));
+		hi = fmaxf(hi, fmaxf(fmaxf(pa, pb), fmaxf(pc, pe)));
+	}
+	for (; i < nv; i++)
+	{
+		float pa = vdot3(vs, i, d);
+		lo = fminf(lo, pa);
+		hi = fmaxf(hi, pa);
+	}
+}
+/* UNIFORM: every lane of the wavefront projects the same hull (one pair per
+ * wavefront), so the vertex count is read as a scalar */
+template <bool UNIFORM>
+__device__ __forceinline__ void project(const F4* vs, bool inLds, int nv, F4 pos, F4 orn, F4 dir, float* mn,
+										float* mx)
 {
 	float lo = FLT_MAX, hi = -FLT_MAX;
 	F4 localDir = qinvrot(orn, dir);
 	float offset = dot4(pos, dir);
-	for (int i = 0; i < nv; i++)
-	{
-		float dp = dot4(vs[i], localDir);
-		if (dp < lo) lo = dp;
-		if (dp > hi) hi = dp;
-	}
+	if (UNIFORM) nv = __builtin_amdgcn_readfirstlane(nv);
+	if (inLds)
+		project_loop(LDS_PTR(vs), nv, localDir, lo, hi);
+	else
+		project_loop(vs, nv, localDir, lo, hi);
 	if (lo > hi)
 	{
 		float t = lo;
@@ -485,13 +525,15 @@ struct PairGeom
 	const F4* vB;
 	int nA, nB;
 	F4 posA, ornA, posB, ornB;
+	bool ldsA, ldsB; /* vA / vB point into LDS */
 };
 /* TestSepAxis: false if the axis separates, else the overlap */
+template <bool UNIFORM>
 __device__ __forceinline__ bool test_sep_axis(const PairGeom& P, F4 axis, float* depth)
 {
 	float mn0, mx0, mn1, mx1;
-	project(P.vA, P.nA, P.posA, P.ornA, axis, &mn0, &mx0);
-	project(P.vB, P.nB, P.posB, P.ornB, axis, &mn1, &mx1);
+	project<UNIFORM>(P.vA, P.ldsA, P.nA, P.posA, P.ornA, axis, &mn0, &mx0);
+	project<UNIFORM>(P.vB, P.ldsB, P.nB, P.posB, P.ornB, axis, &mn1, &mx1);
 	if (mx0 < mn1 || mx1 < mn0) return false;
 	float d0 = mx0 - mn1, d1 = mx1 - mn0;
 	*depth = d0 < d1 ? d0 : d1;
@@ -558,11 +600,11 @@ __device__ bool sat_family(const Group<L>& g, const Family& F, const PairGeom& P
 			bool overlap;
 			if (F.kind == 1)
 			{
-				PairGeom Q = {P.vB, P.vA, P.nB, P.nA, P.posB, P.ornB, P.posA, P.ornA};
-				overlap = test_sep_axis(Q, axis, &d);
+				PairGeom Q = {P.vB, P.vA, P.nB, P.nA, P.posB, P.ornB, P.posA, P.ornA, P.ldsB, P.ldsA};
+				overlap = test_sep_axis<L == 64>(Q, axis, &d);
 			}
 			else
-				overlap = test_sep_axis(P, axis, &d);
+				overlap = test_sep_axis<L == 64>(P, axis, &d);
 			if (!overlap)
 				separated = true;
 			else if (d < bv)
@@ -833,15 +875,15 @@ __global__ __launch_bounds__(256) void k_np_mpr(const b3hInt4* __restrict__ pair
 }
 
 /* ---------------- the fused kernel ---------------- */
-template <int L>
-__global__ __launch_bounds__(NP_THREADS) void k_narrowphase(const b3hInt4* __restrict__ pairs,
+template <int L, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_narrowphase(const b3hInt4* __restrict__ pairs,
 															 const int* __restrict__ numPairsPtr,
 															 const b3hRigidBodyData* __restrict__ bodies,
 															 b3hShapeTables sh, b3hContact4Data* __restrict__ staging,
 															 int* __restrict__ flags, int* __restrict__ sepStateOut,
 															 b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn)
 {
-	constexpr int G = NP_THREADS / L;
+	constexpr int G = THREADS / L;
 	__shared__ F4 sVerts[G][2][NP_VCAP];
 	__shared__ F4 sClip[G][3][CLIP_CAP];
 	const Group<L> g;
@@ -879,6 +921,8 @@ __global__ __launch_bounds__(NP_THREADS) void k_narrowphase(const b3hInt4* __res
 			PairGeom P;
 			P.vA = ldsA ? sVerts[gi][0] : gA;
 			P.vB = ldsB ? sVerts[gi][1] : gB;
+			P.ldsA = ldsA;
+			P.ldsB = ldsB;
 			P.nA = ha.m_numVertices;
 			P.nB = hb.m_numVertices;
 			P.posA = A.m_pos;
@@ -1027,13 +1071,22 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 		hipLaunchKernelGGL(k_np_mpr, dim3(mb), dim3(256), 0, s, pairs, numPairs, bodies, sh, (F4*)mprScratch);
 	}
 	if (evAfterMpr) hipEventRecord(evAfterMpr, s);
-	(void)pairsPerGroup;
-	const int groups = NP_THREADS / 64;
-	int blocks = (maxPairs + groups - 1) / groups;
-	if (blocks > 16384) blocks = 16384;
-	if (blocks < 1) blocks = 1;
-	hipLaunchKernelGGL(k_narrowphase<64>, dim3(blocks), dim3(NP_THREADS), 0, s, pairs, numPairs, bodies, sh, staging,
-					   flags, sepState, sepNormal, (const F4*)mprScratch);
+	/* lanes per pair: 64 (one pair per wavefront), 32 or 16; the block holds
+	 * eight groups' LDS staging (40 KB) for the narrower groups */
+	auto launch = [&](auto kern, int lanes, int threads) {
+		int groups = threads / lanes;
+		int blocks = (maxPairs + groups - 1) / groups;
+		if (blocks > 16384) blocks = 16384;
+		if (blocks < 1) blocks = 1;
+		hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, s, pairs, numPairs, bodies, sh, staging, flags,
+						   sepState, sepNormal, (const F4*)mprScratch);
+	};
+	if (pairsPerGroup == 16)
+		launch(k_narrowphase<16, 128>, 16, 128);
+	else if (pairsPerGroup == 32)
+		launch(k_narrowphase<32, 256>, 32, 256);
+	else
+		launch(k_narrowphase<64, NP_THREADS>, 64, NP_THREADS);
 	if (evAfterKernel) hipEventRecord(evAfterKernel, s);
 	b3h_exclusive_scan(flags, flagScan, numPairs, maxPairs, scanTmp, s);
 	int cblocks = (maxPairs + 255) / 256;

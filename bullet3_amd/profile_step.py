@@ -32,8 +32,10 @@ def main():
     ap.add_argument("--state", default=None, help="bodies saved by --save-state")
     ap.add_argument("--steps", type=int, default=3, help="eager steps (after --state)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=64, help="narrowphase lanes per pair (64, 32, 16)")
     args = ap.parse_args()
     w = World(scenes.convex_pile(args.nx, args.ny, args.nz), broadphase="grid")
+    capi.call("b3h_np_set_pairs_per_group", w.np_, args.lanes)
     sp = w.solver_params()
     sp.m_numIterations = args.iters
     w.set_solver(sp)

@@ -103,7 +103,10 @@ int b3h_np_get_shape_tables(b3hNarrowPhase* np, int* counts7, b3hCollidable* col
 							b3hGpuFace* faces, int* indices, b3hAabb* localAabbs);
 /* Host copy of the registered inertias. */
 int b3h_np_get_inertias_cpu(b3hNarrowPhase* np, b3hInertiaData* dst, int n);
-/* Narrowphase tuning: lanes cooperating on one pair (1 = one pair per lane). */
+/* Narrowphase tuning: lanes cooperating on one pair in the SAT, clipping and
+ * reduction kernel: 64 (default, one pair per wavefront), 32 or 16 (two or four
+ * pairs per wavefront); other powers of two up to 64 select 64.  Results do
+ * not depend on it. */
 int b3h_np_set_pairs_per_group(b3hNarrowPhase* np, int lanesPerPair);
 /* Per-pair separating-axis state and normal of the last computeContacts (device). */
 int b3h_np_get_sat_debug(b3hNarrowPhase* np, int** sepStateDev, b3hFloat4** sepNormalDev);

@@ -107,8 +107,12 @@ def test_pairs_bit_exact(scene, kind):
     w.close()
 
 
-def test_narrowphase_bit_exact(scene):
+@pytest.mark.parametrize("lanes", [64, 32, 16])
+def test_narrowphase_bit_exact(scene, lanes):
+    """lanes: lanes cooperating on one pair (64 = one pair per wavefront; 32 and
+    16 pack two and four pairs into a wavefront)"""
     w = World(scene)
+    capi.call("b3h_np_set_pairs_per_group", w.np_, lanes)
     ow = OracleWorld(w)
     run_steps(w, ow, 2)
     capi.call("b3h_pipeline_setup_gpu_aabbs_full", w.pl)
