@@ -986,7 +986,7 @@ static void pl_free(b3hPipeline* p)
 	dfree(p->sb.levelOrder);
 	dfree(p->sb.levelStart);
 	dfree(p->sb.partMeta);
-	dfree(p->sb.bodyPart);
+	dfree(p->sb.partBodies);
 	dfree(p->sb.cellLevels);
 	dfree(p->sb.lastClass);
 	dfree(p->sb.residSnap);
@@ -1103,7 +1103,8 @@ extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const 
 		(r = dalloc(&p->sb.cslot, 2 * (size_t)mc + 2)) || (r = dalloc(&p->sb.recN, 10 * (size_t)mc)) ||
 		(r = dalloc(&p->sb.recF, 8 * (size_t)mc)) || (r = dalloc(&p->sb.levelOrder, mc)) ||
 		(r = dalloc(&p->sb.levelStart, (size_t)mc + B3H_N_CELLS * B3H_PGS_PARTS + 8)) ||
-		(r = dalloc(&p->sb.partMeta, B3H_N_CELLS * B3H_PGS_PARTS)) || (r = dalloc(&p->sb.bodyPart, 2 * (size_t)mc + 64)) || (r = dalloc(&p->sb.cellLevels, B3H_N_CELLS)) ||
+		(r = dalloc(&p->sb.partMeta, 2 * B3H_N_CELLS * B3H_PGS_PARTS)) ||
+		(r = dalloc(&p->sb.partBodies, 2 * (2 * (size_t)mc + 64))) || (r = dalloc(&p->sb.cellLevels, B3H_N_CELLS)) ||
 		(r = dalloc(&p->sb.lastClass, (size_t)cfg->m_maxConvexBodies + 16)) ||
 		(r = dalloc(&p->sb.residSnap, 6 * (size_t)mc)) || (r = dalloc(&p->sb.residL2, 4 * B3H_MAX_ITERATIONS)))
 	{
@@ -1632,7 +1633,7 @@ extern "C" int b3h_pipeline_get_pgs_levels(b3hPipeline* p, int* levels)
 extern "C" int b3h_pipeline_get_pgs_parts(b3hPipeline* p, int* parts, int* largest)
 {
 	CHECK_ARG(p && parts);
-	std::vector<b3hInt4> meta((size_t)B3H_N_CELLS * B3H_PGS_PARTS);
+	std::vector<b3hInt4> meta(2 * (size_t)B3H_N_CELLS * B3H_PGS_PARTS);
 	B3H_CHECK(hipMemcpyAsync(meta.data(), p->sb.partMeta, meta.size() * sizeof(b3hInt4), hipMemcpyDeviceToHost,
 							 p->stream));
 	B3H_CHECK(hipStreamSynchronize(p->stream));
@@ -1641,7 +1642,7 @@ extern "C" int b3h_pipeline_get_pgs_parts(b3hPipeline* p, int* parts, int* large
 		int k = 0, big = 0;
 		for (int q = 0; q < B3H_PGS_PARTS; q++)
 		{
-			const b3hInt4& m = meta[(size_t)c * B3H_PGS_PARTS + q];
+			const b3hInt4& m = meta[2 * ((size_t)c * B3H_PGS_PARTS + q)];
 			if (m.y > 0) k++;
 			if (m.y > big) big = m.y;
 		}

@@ -131,8 +131,8 @@ struct b3hSolverBuffers
 	int* levelOrder;      /* per cell region [off, off+n): per part, constraints grouped by dependency level */
 	int* levelStart;      /* per cell region [off+PARTS*cell, off+PARTS*cell+n+PARTS): per part its level starts */
 	int* cellLevels;      /* 256: levels per cell, -1 = global walk */
-	b3hInt4* partMeta;    /* 256 x PARTS: (first position, count, level-start offset, levels) */
-	int* bodyPart;        /* per cell at 2*cellOffsets[cell]: the part of each listed body */
+	b3hInt4* partMeta;    /* 2 x 256 x PARTS: (first position, count, level-start offset, levels), (first body entry, bodies) */
+	int* partBodies;      /* per cell at 2*cellOffsets[cell]: (slot, body) pairs grouped by part */
 	int* lastClass;       /* per body: the last cell-parity class whose cells touch it (-1: none) */
 	int maxContacts;
 	float* residSnap;     /* per constraint: 6 impulses before the sweep (residual tracking) */

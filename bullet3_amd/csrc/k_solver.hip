@@ -877,8 +877,10 @@ __host__ __device__ __forceinline__ size_t pgs_level_lds(int nb, int n, bool cac
  * Output per cell: the constraints part by part, each part grouped by level
  * (levelOrder, region [off, off+n)); per part its level starts relative to the
  * part's first position, empty levels dropped (levelStart, region starting at
- * off + PARTS*cell); partMeta[cell][p] = (first position, count, offset of the
- * level starts, levels); the part of each listed body (bodyPart); L
+ * off + PARTS*cell); partMeta[2 (cell PARTS + p)] = (first position, count,
+ * offset of the level starts, levels) and partMeta[... + 1] = (first entry,
+ * count) of the part's bodies in partBodies, which lists (slot in the cell's
+ * body list, body) part by part in the cell's region 2 cellOffsets[cell]; L
  * (cellLevels; -1 when the cell is too large for LDS: it then takes the global
  * walk, as one part). */
 __device__ __forceinline__ int floor_div_pow2(int x, int d) { return x >= 0 ? x / d : -((-x + d - 1) / d); }
@@ -901,20 +903,21 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 													 const b3hRigidBodyData* __restrict__ bodies, float scale,
 													 int bodyCap, int ldsInts, int pgsLdsBytes, int* __restrict__ levelOrder,
 													 int* __restrict__ levelStart, int* __restrict__ cellLevels,
-													 b3hInt4* __restrict__ partMeta, int* __restrict__ bodyPart, int maxParts)
+													 const int* __restrict__ cellBodies, b3hInt4* __restrict__ partMeta,
+													 int* __restrict__ partBodies, int maxParts)
 {
 	extern __shared__ int sm[];
 	__shared__ int sL, sP, sNumTiles;
 	__shared__ unsigned tKey[PGS_MAX_TILES];
 	__shared__ int tCount[PGS_MAX_TILES], tPart[PGS_MAX_TILES];
-	__shared__ int pCount[B3H_PGS_PARTS], pLevels[B3H_PGS_PARTS];
+	__shared__ int pCount[B3H_PGS_PARTS], pbFirst[B3H_PGS_PARTS], pbCursor[B3H_PGS_PARTS];
 	int cell = blockIdx.x;
 	int n = cellCounts[cell];
 	int off = cellOffsets[cell];
 	int nb = cellBodyCount[cell];
 	int t = threadIdx.x;
-	b3hInt4* meta = partMeta + cell * B3H_PGS_PARTS;
-	if (t < B3H_PGS_PARTS) meta[t] = b3hInt4{0, 0, 0, 0};
+	b3hInt4* meta = partMeta + 2 * cell * B3H_PGS_PARTS; /* meta[2q], meta[2q + 1] */
+	if (t < 2 * B3H_PGS_PARTS) meta[t] = b3hInt4{0, 0, 0, 0};
 	if (n == 0)
 	{
 		if (t == 0) cellLevels[cell] = 0;
@@ -928,8 +931,8 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 		{
 			cellLevels[cell] = -1;
 			meta[0] = b3hInt4{0, n, 0, -1};
+			meta[1] = b3hInt4{0, nb, 0, 0};
 		}
-		for (int k = t; k < nb; k += blockDim.x) bodyPart[2 * (size_t)off + k] = 0;
 		return;
 	}
 	int* last = sm;
@@ -1078,16 +1081,26 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 		lev[k] = lv | (part << 24);
 		atomicAdd(&cnt[part * stride + lv], 1);
 	}
-	for (int k = t; k < nb; k += blockDim.x) bodyPart[2 * (size_t)off + k] = P > 1 ? last[k] : 0;
+	if (t < B3H_PGS_PARTS) pCount[t] = 0;
+	__syncthreads();
+	for (int k = t; k < nb; k += blockDim.x) atomicAdd(&pCount[P > 1 ? last[k] : 0], 1);
 	__syncthreads();
 	if (t == 0)
 	{
+		int runB = 0;
+		for (int q = 0; q < B3H_PGS_PARTS; q++)
+		{
+			pbFirst[q] = runB;
+			pbCursor[q] = runB;
+			meta[2 * q + 1] = b3hInt4{runB, pCount[q], 0, 0};
+			runB += pCount[q];
+		}
 		int run = 0, lsOfs = 0;
 		for (int q = 0; q < B3H_PGS_PARTS; q++)
 		{
 			if (q >= P)
 			{
-				meta[q] = b3hInt4{run, 0, lsOfs, 0};
+				meta[2 * q] = b3hInt4{run, 0, lsOfs, 0};
 				continue;
 			}
 			int first = run, Lp = 0;
@@ -1101,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 				run += c;
 			}
 			lsOut[lsOfs + Lp] = run - first;
-			meta[q] = b3hInt4{first, run - first, lsOfs, Lp};
+			meta[2 * q] = b3hInt4{first, run - first, lsOfs, Lp};
 			lsOfs += Lp + 1;
 		}
 		cellLevels[cell] = L;
@@ -1112,6 +1125,13 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 		int part = (unsigned)lev[k] >> 24, lv = lev[k] & 0xffffff;
 		int pos = atomicAdd(&cnt[part * stride + lv], 1);
 		levelOrder[off + pos] = off + k;
+	}
+	for (int k = t; k < nb; k += blockDim.x)
+	{
+		int pos = atomicAdd(&pbCursor[P > 1 ? last[k] : 0], 1);
+		int* e = partBodies + 2 * (2 * (size_t)off + pos);
+		e[0] = k;
+		e[1] = cellBodies[2 * (size_t)off + k];
 	}
 }
 
@@ -1500,7 +1520,7 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 														   const int* __restrict__ levelStart,
 														   const int* __restrict__ levelOrder,
 														   const b3hInt4* __restrict__ partMeta,
-														   const int* __restrict__ bodyPart, const F4* __restrict__ recN,
+														   const int* __restrict__ partBodies, const F4* __restrict__ recN,
 														   const F4* __restrict__ recF, int ib, float friction,
 														   const float* __restrict__ warm, float* __restrict__ resid,
 														   const int* __restrict__ lastClass, int fuseIntegrate,
@@ -1554,12 +1574,14 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		}
 		return;
 	}
-	const b3hInt4 pm = partMeta[cell * B3H_PGS_PARTS + part];
+	const b3hInt4 pm = partMeta[2 * (cell * B3H_PGS_PARTS + part)];
 	const int n = pm.y; /* this part's constraints */
 	if (n == 0) return;
 	L = pm.w;
+	const b3hInt4 pb = partMeta[2 * (cell * B3H_PGS_PARTS + part) + 1];
+	const int nbp = pb.y; /* this part's bodies: (slot, body) pairs */
+	const int* own = partBodies + 2 * (2 * (size_t)start + pb.x);
 	int nb = cellBodyCount[cell];
-	const int* ownerOf = bodyPart + 2 * (size_t)start; /* part of each listed body */
 	/* with room for it, 3 (nb + 2) inverse world inertia rows (slot nb is zero
 	 * for static bodies, nb + 1 padding); then the velocities.  Sized for the
 	 * whole cell (slots are the cell's), filled for this part's bodies only */
@@ -1567,28 +1589,46 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	F4* sI = smem4;
 	F4* sLv = cacheI ? sI + 3 * (nb + 2) : smem4; /* nb + 1 (the last is padding) */
 	F4* sAv = sLv + nb + 1;   /* nb + 1 */
-	int* sList = (int*)(sAv + nb + 1); /* nb + 1 */
+	int* sList = (int*)(sAv + nb + 1); /* nb + 1: body of each slot (this part's slots) */
 	int* sOrder = sList + nb + 1;      /* n + 1 */
 	int* sLs = sOrder + n + 1;         /* L + 2: level starts relative to start */
-	/* stage the body list, the level order and the level starts together */
-	stage_ints(sList, cellBodies + 2 * (size_t)start, nb, t);
-	stage_ints(sOrder, levelOrder + start + pm.x, n, t);
-	stage_ints(sLs, levelStart + start + B3H_PGS_PARTS * cell + pm.z, L + 1, t);
-	if (cacheI && t < 3) sI[3 * nb + t] = f4(0.f, 0.f, 0.f, 0.f);
-	__syncthreads();
-	/* gather the velocities and inverse inertias, four bodies per thread per
-	 * round with every load issued before any is stored (lanes past the end
-	 * read body 0, which exists, and store into the padding slots) */
-	for (int k0 = t; k0 < nb; k0 += 4 * T)
+	/* stage the level order and the level starts in one pass (sLs follows
+	 * sOrder's padding entry) */
 	{
-		int kk[4], bb[4];
+		const int* srcO = levelOrder + start + pm.x;
+		const int* srcL = levelStart + start + B3H_PGS_PARTS * cell + pm.z;
+		const int m = n + L + 1;
+		for (int k0 = t; k0 < m; k0 += 8 * PGS_THREADS)
+		{
+			int v[8];
+#pragma unroll
+			for (int u = 0; u < 8; u++)
+			{
+				int k = k0 + u * PGS_THREADS;
+				v[u] = k < n ? srcO[k] : (k < m ? srcL[k - n] : srcO[0]);
+			}
+#pragma unroll
+			for (int u = 0; u < 8; u++)
+			{
+				int k = k0 + u * PGS_THREADS;
+				sOrder[k < n ? k : (k < m ? k + 1 : n)] = v[u];
+			}
+		}
+	}
+	if (cacheI && t < 3) sI[3 * nb + t] = f4(0.f, 0.f, 0.f, 0.f);
+	/* gather this part's velocities and inverse inertias, four bodies per
+	 * thread per round with every load issued before any is stored (lanes past
+	 * the end read body 0, which exists, and store into the padding slots) */
+	for (int k0 = t; k0 < nbp; k0 += 4 * T)
+	{
+		int sv[4], bb[4];
 		F4 lv[4], av[4], r0[4], r1[4], r2[4];
 #pragma unroll
 		for (int u = 0; u < 4; u++)
 		{
-			kk[u] = k0 + u * T;
-			if (kk[u] < nb && ownerOf[kk[u]] != part) kk[u] = nb; /* another part's body */
-			bb[u] = kk[u] < nb ? sList[kk[u]] : 0;
+			int k = k0 + u * T;
+			sv[u] = k < nbp ? own[2 * k] : nb;
+			bb[u] = k < nbp ? own[2 * k + 1] : 0;
 		}
 #pragma unroll
 		for (int u = 0; u < 4; u++)
@@ -1602,9 +1642,10 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 #pragma unroll
 		for (int u = 0; u < 4; u++)
 		{
-			int sv = kk[u] < nb ? kk[u] : nb, si = kk[u] < nb ? kk[u] : nb + 1;
-			sLv[sv] = lv[u];
-			sAv[sv] = av[u];
+			int si = sv[u] < nb ? sv[u] : nb + 1;
+			sLv[sv[u]] = lv[u];
+			sAv[sv[u]] = av[u];
+			sList[sv[u]] = bb[u];
 			if (cacheI)
 			{
 				sI[3 * si + 0] = r0[u];
@@ -1635,36 +1676,33 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	}
 	__syncthreads();
 	if (!fuseIntegrate)
-		for (int q = t; q < nb; q += T)
+		for (int q = t; q < nbp; q += T)
 		{
-			if (ownerOf[q] != part) continue;
-			int b = sList[q];
-			bodies[b].m_linVel = sLv[q];
-			bodies[b].m_angVel = sAv[q];
+			int slot = own[2 * q], b = own[2 * q + 1];
+			bodies[b].m_linVel = sLv[slot];
+			bodies[b].m_angVel = sAv[slot];
 		}
 	else
 		/* bodies whose last touching class this is have their final velocity:
 		 * integrate them here (integrateTransformsKernel).  Two bodies per
 		 * thread per round, loads first; the integration is evaluated for
 		 * every body and kept only where it applies */
-		for (int q0 = t; q0 < nb; q0 += 2 * T)
+		for (int q0 = t; q0 < nbp; q0 += 2 * T)
 		{
-			int qq0 = q0, qq1 = q0 + T;
-			if (ownerOf[qq0] != part) qq0 = nb; /* another part's body: evaluated on body 0, not stored */
-			if (qq1 < nb && ownerOf[qq1] != part) qq1 = nb;
-			int bq0 = qq0 < nb ? sList[qq0] : 0, bq1 = qq1 < nb ? sList[qq1] : 0;
+			int qq1 = q0 + T;
+			int sq0 = own[2 * q0], bq0 = own[2 * q0 + 1];
+			int sq1 = qq1 < nbp ? own[2 * qq1] : nb, bq1 = qq1 < nbp ? own[2 * qq1 + 1] : 0;
 			int lc0 = lastClass[bq0], lc1 = lastClass[bq1];
 			b3hRigidBodyData bd0 = bodies[bq0], bd1 = bodies[bq1];
 #pragma unroll
 			for (int u = 0; u < 2; u++)
 			{
-				int qq = u ? qq1 : qq0, b = u ? bq1 : bq0, lc = u ? lc1 : lc0;
-				int sv = qq < nb ? qq : nb;
+				int sv = u ? sq1 : sq0, b = u ? bq1 : bq0, lc = u ? lc1 : lc0;
 				b3hRigidBodyData body = u ? bd1 : bd0;
 				body.m_linVel = sLv[sv];
 				body.m_angVel = sAv[sv];
 				b3h_integrate_body(body, ip.dt, ip.damping, ip.gravity);
-				if (qq >= nb) continue;
+				if (sv >= nb) continue;
 				if (lc == ib)
 				{
 					bodies[b].m_pos = body.m_pos;
@@ -1763,7 +1801,7 @@ void b3h_solve_setup(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaDa
 	hipLaunchKernelGGL(k_cell_levels, dim3(B3H_N_CELLS), dim3(256), (size_t)ldsCapBytes, s, sb.cellCounts,
 					   sb.cellOffsets, sb.batchIdx, sb.cellBodyCount, sb.cslot, (const b3hContactConstraint4*)sb.constraints,
 					   bodies, 1.f / prm.m_batchCellSize, cap, ldsCapBytes / 4, (int)pgsLds, sb.levelOrder,
-					   sb.levelStart, sb.cellLevels, sb.partMeta, sb.bodyPart,
+					   sb.levelStart, sb.cellLevels, (const int*)sb.cellBodies, sb.partMeta, sb.partBodies,
 					   sb.pgsMaxParts > 0 && sb.pgsMaxParts < B3H_PGS_PARTS ? sb.pgsMaxParts : B3H_PGS_PARTS);
 }
 
@@ -1837,7 +1875,7 @@ void b3h_solve_sweeps(b3hRigidBodyData* bodies, const b3hInertiaData* inertias, 
 #define B3H_PGS_ARGS(ib, resid_, warm_, fuse_)                                                                         \
 	bodies, inertias, sb.constraints, sb.cellCounts, sb.cellOffsets, sb.batchSizes, sb.batchIdx, sb.cellBodies, \
 		sb.cellBodyCount, sb.cellLevels, sb.levelStart, sb.levelOrder, (const b3hInt4*)sb.partMeta,               \
-		(const int*)sb.bodyPart, (const F4*)sb.recN, (const F4*)sb.recF, ib,                                   \
+		(const int*)sb.partBodies, (const F4*)sb.recN, (const F4*)sb.recF, ib,                                 \
 		prm.m_frictionCoeff, warm_, resid_, (const int*)sb.lastClass, fuse_, ip, sb.pgsNoInertiaCache ? 0 : (int)pgsLds
 	if (sb.warmImpulses)
 		for (int ib = 0; ib < B3H_N_CELL_BATCHES; ib++)
