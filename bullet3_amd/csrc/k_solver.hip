@@ -854,7 +854,7 @@ __global__ __launch_bounds__(256) void k_pgs_prep(const int* __restrict__ nPtr, 
 /* LDS bytes the level sweep needs for a cell of n constraints and nb bodies */
 __host__ __device__ __forceinline__ size_t pgs_level_lds(int nb, int n, bool cacheInertia)
 {
-	return (size_t)(nb + 1) * 36 + (cacheInertia ? (size_t)(nb + 2) * 48 : 0) + (size_t)(n + 1) * 4 +
+	return (size_t)(nb + 2) * 36 + (cacheInertia ? (size_t)(nb + 2) * 48 : 0) + (size_t)(n + 1) * 4 +
 		   (size_t)(n + 2) * 4;
 }
 
@@ -1262,15 +1262,16 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 	{
 		const F4* r;
 	} S = {Sr};
-	const F4 zero = f4(0.f, 0.f, 0.f, 0.f);
 	const F4 meta = S.r[PHASE == 1 ? PGS_FREC - 1 : PGS_NREC - 1];
 	float imB = meta.x;
 	int sA = fbits(meta.y), sB = fbits(meta.z), idx = fbits(meta.w);
 	Vel va, vb;
-	va.lv = sA >= 0 ? sLv[sA] : zero;
-	va.av = sA >= 0 ? sAv[sA] : zero;
-	vb.lv = sB >= 0 ? sLv[sB] : zero;
-	vb.av = sB >= 0 ? sAv[sB] : zero;
+	/* static bodies (slot -1) read the zero slot nb */
+	const int rA = sA >= 0 ? sA : is.nb, rB = sB >= 0 ? sB : is.nb;
+	va.lv = sLv[rA];
+	va.av = sAv[rA];
+	vb.lv = sLv[rB];
+	vb.av = sAv[rB];
 	const b3hMat3x3 iiA = body_inertia<CI>(is, sA);
 	const b3hMat3x3 iiB = body_inertia<CI>(is, sB);
 	if (PHASE == 1)
@@ -1371,17 +1372,18 @@ __device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restr
 						 const InertiaSrc& is, b3hContactConstraint4* __restrict__ cs, float friction,
 						 const float* __restrict__ warm)
 {
-	const F4 zero = f4(0.f, 0.f, 0.f, 0.f);
 	const F4* N = recN + (size_t)idx * PGS_NREC;
 	const F4* Fr = recF + (size_t)idx * PGS_FREC;
 	F4 meta = N[9];
 	float imB = meta.x;
 	int sA = fbits(meta.y), sB = fbits(meta.z);
 	Vel va, vb;
-	va.lv = sA >= 0 ? sLv[sA] : zero;
-	va.av = sA >= 0 ? sAv[sA] : zero;
-	vb.lv = sB >= 0 ? sLv[sB] : zero;
-	vb.av = sB >= 0 ? sAv[sB] : zero;
+	/* static bodies (slot -1) read the zero slot nb */
+	const int rA = sA >= 0 ? sA : is.nb, rB = sB >= 0 ? sB : is.nb;
+	va.lv = sLv[rA];
+	va.av = sAv[rA];
+	vb.lv = sLv[rB];
+	vb.av = sAv[rB];
 	const b3hMat3x3 iiA = body_inertia(is, sA);
 	const b3hMat3x3 iiB = body_inertia(is, sB);
 	F4 lin = N[0];
@@ -1587,10 +1589,10 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	 * whole cell (slots are the cell's), filled for this part's bodies only */
 	bool cacheI = pgs_level_lds(nb, cellN, true) <= (size_t)ldsBytes;
 	F4* sI = smem4;
-	F4* sLv = cacheI ? sI + 3 * (nb + 2) : smem4; /* nb + 1 (the last is padding) */
-	F4* sAv = sLv + nb + 1;   /* nb + 1 */
-	int* sList = (int*)(sAv + nb + 1); /* nb + 1: body of each slot (this part's slots) */
-	int* sOrder = sList + nb + 1;      /* n + 1 */
+	F4* sLv = cacheI ? sI + 3 * (nb + 2) : smem4; /* nb + 2: slot nb zero (static bodies), nb + 1 padding */
+	F4* sAv = sLv + nb + 2;   /* nb + 2 */
+	int* sList = (int*)(sAv + nb + 2); /* nb + 2: body of each slot (this part's slots) */
+	int* sOrder = sList + nb + 2;      /* n + 1 */
 	int* sLs = sOrder + n + 1;         /* L + 2: level starts relative to start */
 	/* stage the level order and the level starts in one pass (sLs follows
 	 * sOrder's padding entry) */
@@ -1616,6 +1618,8 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		}
 	}
 	if (cacheI && t < 3) sI[3 * nb + t] = f4(0.f, 0.f, 0.f, 0.f);
+	if (t == 3) sLv[nb] = f4(0.f, 0.f, 0.f, 0.f);
+	if (t == 4) sAv[nb] = f4(0.f, 0.f, 0.f, 0.f);
 	/* gather this part's velocities and inverse inertias, four bodies per
 	 * thread per round with every load issued before any is stored (lanes past
 	 * the end read body 0, which exists, and store into the padding slots) */
@@ -1627,7 +1631,7 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		for (int u = 0; u < 4; u++)
 		{
 			int k = k0 + u * T;
-			sv[u] = k < nbp ? own[2 * k] : nb;
+			sv[u] = k < nbp ? own[2 * k] : nb + 1;
 			bb[u] = k < nbp ? own[2 * k + 1] : 0;
 		}
 #pragma unroll
@@ -1691,7 +1695,7 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		{
 			int qq1 = q0 + T;
 			int sq0 = own[2 * q0], bq0 = own[2 * q0 + 1];
-			int sq1 = qq1 < nbp ? own[2 * qq1] : nb, bq1 = qq1 < nbp ? own[2 * qq1 + 1] : 0;
+			int sq1 = qq1 < nbp ? own[2 * qq1] : nb + 1, bq1 = qq1 < nbp ? own[2 * qq1 + 1] : 0;
 			int lc0 = lastClass[bq0], lc1 = lastClass[bq1];
 			b3hRigidBodyData bd0 = bodies[bq0], bd1 = bodies[bq1];
 #pragma unroll
