@@ -126,8 +126,11 @@ struct b3hSolverBuffers
 	int pgsBodyCap;       /* < 0: the LDS budget; otherwise a lower cap on cached bodies per cell */
 	int pgsNoInertiaCache; /* 1: never cache inverse inertias in LDS (tests) */
 	int pgsMaxParts;       /* work-groups a cell may be split into (0: B3H_PGS_PARTS) */
-	b3hFloat4* recN;      /* per constraint: 10 float4 of normal-row invariants */
-	b3hFloat4* recF;      /* per constraint: 8 float4 of friction-row invariants */
+	b3hFloat4* recN;      /* 10 fields x maxContacts float4 of normal-row invariants, by level position */
+	b3hFloat4* recF;      /* 8 fields x maxContacts float4 of friction-row invariants, by level position */
+	b3hFloat4* lamN;      /* per level position: normal impulses during the sweeps */
+	b3hFloat4* lamF;      /* per level position: friction impulses (x, y) during the sweeps */
+	int* posOf;           /* per constraint: its level position (-1: cell on the global walk) */
 	int* levelOrder;      /* per cell region [off, off+n): per part, constraints grouped by dependency level */
 	int* levelStart;      /* per cell region [off+PARTS*cell, off+PARTS*cell+n+PARTS): per part its level starts */
 	int* cellLevels;      /* 256: levels per cell, -1 = global walk */

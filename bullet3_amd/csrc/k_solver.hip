@@ -785,21 +785,23 @@ __device__ __forceinline__ float ibits(int v) { return __builtin_bit_cast(float,
 __device__ __forceinline__ int fbits(float v) { return __builtin_bit_cast(int, v); }
 __global__ __launch_bounds__(256) void k_pgs_prep(const int* __restrict__ nPtr, const b3hContactConstraint4* __restrict__ cs,
 												  const int* __restrict__ cslot, const b3hRigidBodyData* __restrict__ bodies,
-												  const b3hInertiaData* __restrict__ inertias, F4* __restrict__ recN,
-												  F4* __restrict__ recF)
+												  const b3hInertiaData* __restrict__ inertias, const int* __restrict__ posOf,
+												  int stride, F4* __restrict__ recN, F4* __restrict__ recF,
+												  F4* __restrict__ lamN, F4* __restrict__ lamF)
 {
 	int n = *nPtr;
 	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
 	{
+		const int p = posOf[i];
+		if (p < 0) continue; /* a cell on the global walk: no records */
 		const b3hContactConstraint4 c = cs[i];
 		int a = (int)c.m_bodyA, b = (int)c.m_bodyB;
 		F4 posA = bodies[a].m_pos, posB = bodies[b].m_pos;
 		float imA = bodies[a].m_invMass, imB = bodies[b].m_invMass;
 		int sA = cslot[2 * (size_t)i], sB = cslot[2 * (size_t)i + 1];
-		F4 meta = f4(imB, ibits(sA), ibits(sB), ibits(i));
-		F4* N = recN + (size_t)i * PGS_NREC;
+		F4 meta = f4(imB, ibits(sA), ibits(sB), ibits(p));
 		F4 n = neg4(c.m_linear);
-		N[0] = f4(c.m_linear.x, c.m_linear.y, c.m_linear.z, imA);
+		recN[p] = f4(c.m_linear.x, c.m_linear.y, c.m_linear.z, imA);
 		float pk[24];
 		for (int ic = 0; ic < 4; ic++)
 		{
@@ -814,11 +816,11 @@ __global__ __launch_bounds__(256) void k_pgs_prep(const int* __restrict__ nPtr, 
 			pk[6 * ic + 4] = ang1.y;
 			pk[6 * ic + 5] = ang1.z;
 		}
-		for (int q = 0; q < 6; q++) N[1 + q] = f4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
-		N[7] = f4(c.m_jacCoeffInv[0], c.m_jacCoeffInv[1], c.m_jacCoeffInv[2], c.m_jacCoeffInv[3]);
-		N[8] = f4(c.m_b[0], c.m_b[1], c.m_b[2], c.m_b[3]);
-		N[9] = meta;
-		F4* Fr = recF + (size_t)i * PGS_FREC;
+		for (int q = 0; q < 6; q++)
+			recN[(size_t)(1 + q) * stride + p] = f4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
+		recN[(size_t)7 * stride + p] = f4(c.m_jacCoeffInv[0], c.m_jacCoeffInv[1], c.m_jacCoeffInv[2], c.m_jacCoeffInv[3]);
+		recN[(size_t)8 * stride + p] = f4(c.m_b[0], c.m_b[1], c.m_b[2], c.m_b[3]);
+		recN[(size_t)9 * stride + p] = meta;
 		F4 t[2];
 		plane_space(n, &t[0], &t[1]);
 		F4 r0 = sub4(c.m_center, posA), r1 = sub4(c.m_center, posB);
@@ -842,12 +844,35 @@ __global__ __launch_bounds__(256) void k_pgs_prep(const int* __restrict__ nPtr, 
 		fk[20] = n.z;
 		fk[21] = n.w; /* the twist damping subtracts the whole float4 (w included) */
 		fk[22] = fk[23] = 0.f;
-		for (int q = 0; q < 6; q++) Fr[q] = f4(fk[4 * q], fk[4 * q + 1], fk[4 * q + 2], fk[4 * q + 3]);
+		for (int q = 0; q < 6; q++)
+			recF[(size_t)q * stride + p] = f4(fk[4 * q], fk[4 * q + 1], fk[4 * q + 2], fk[4 * q + 3]);
 		F4 ab = nrm3(sub4(posB, posA));
 		F4 ac = nrm3(sub4(c.m_center, posA));
 		bool damp = dot3(ab, ac) > 0.95f || (imA == 0.f || imB == 0.f);
-		Fr[6] = f4(c.m_fJacCoeffInv[0], c.m_fJacCoeffInv[1], damp ? 1.f : 0.f, imA);
-		Fr[7] = meta;
+		recF[(size_t)6 * stride + p] = f4(c.m_fJacCoeffInv[0], c.m_fJacCoeffInv[1], damp ? 1.f : 0.f, imA);
+		recF[(size_t)7 * stride + p] = meta;
+		lamN[p] = f4(c.m_appliedRambdaDt[0], c.m_appliedRambdaDt[1], c.m_appliedRambdaDt[2], c.m_appliedRambdaDt[3]);
+		lamF[p] = f4(c.m_fAppliedRambdaDt[0], c.m_fAppliedRambdaDt[1], 0.f, 0.f);
+	}
+}
+
+/* the impulses of the level-walked constraints back into the constraints */
+__global__ __launch_bounds__(256) void k_lambda_writeback(const int* __restrict__ nPtr, const int* __restrict__ posOf,
+														  const F4* __restrict__ lamN, const F4* __restrict__ lamF,
+														  b3hContactConstraint4* __restrict__ cs)
+{
+	int n = *nPtr;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		const int p = posOf[i];
+		if (p < 0) continue;
+		const F4 ln = lamN[p], lf = lamF[p];
+		cs[i].m_appliedRambdaDt[0] = ln.x;
+		cs[i].m_appliedRambdaDt[1] = ln.y;
+		cs[i].m_appliedRambdaDt[2] = ln.z;
+		cs[i].m_appliedRambdaDt[3] = ln.w;
+		cs[i].m_fAppliedRambdaDt[0] = lf.x;
+		cs[i].m_fAppliedRambdaDt[1] = lf.y;
 	}
 }
 
@@ -880,7 +905,8 @@ __host__ __device__ __forceinline__ size_t pgs_level_lds(int nb, int n, bool cac
  * off + PARTS*cell); partMeta[2 (cell PARTS + p)] = (first position, count,
  * offset of the level starts, levels) and partMeta[... + 1] = (first entry,
  * count) of the part's bodies in partBodies, which lists (slot in the cell's
- * body list, body) part by part in the cell's region 2 cellOffsets[cell]; L
+ * body list, body) part by part in the cell's region 2 cellOffsets[cell]; per
+ * constraint its position in levelOrder (posOf; -1 on the global walk); L
  * (cellLevels; -1 when the cell is too large for LDS: it then takes the global
  * walk, as one part). */
 __device__ __forceinline__ int floor_div_pow2(int x, int d) { return x >= 0 ? x / d : -((-x + d - 1) / d); }
@@ -904,7 +930,7 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 													 int bodyCap, int ldsInts, int pgsLdsBytes, int* __restrict__ levelOrder,
 													 int* __restrict__ levelStart, int* __restrict__ cellLevels,
 													 const int* __restrict__ cellBodies, b3hInt4* __restrict__ partMeta,
-													 int* __restrict__ partBodies, int maxParts)
+													 int* __restrict__ partBodies, int* __restrict__ posOf, int maxParts)
 {
 	extern __shared__ int sm[];
 	__shared__ int sL, sP, sNumTiles;
@@ -933,6 +959,7 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 			meta[0] = b3hInt4{0, n, 0, -1};
 			meta[1] = b3hInt4{0, nb, 0, 0};
 		}
+		for (int k = t; k < n; k += blockDim.x) posOf[off + k] = -1;
 		return;
 	}
 	int* last = sm;
@@ -1125,6 +1152,7 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 		int part = (unsigned)lev[k] >> 24, lv = lev[k] & 0xffffff;
 		int pos = atomicAdd(&cnt[part * stride + lv], 1);
 		levelOrder[off + pos] = off + k;
+		posOf[off + k] = off + pos;
 	}
 	for (int k = t; k < nb; k += blockDim.x)
 	{
@@ -1171,25 +1199,32 @@ struct Slot
 	static constexpr int W = PHASE == 1 ? PGS_FSLOT : PGS_NSLOT;
 	F4 r[W];
 };
-template <int PHASE>
-__device__ __forceinline__ void slot_fetch(Slot<PHASE>& s, int idx, const F4* __restrict__ recN,
-										   const F4* __restrict__ recF, const b3hContactConstraint4* __restrict__ cs)
+/* records and impulses of the constraint at level position p: field k of
+ * every record is contiguous over positions (stride apart), so the lanes of a
+ * wavefront, which take consecutive positions, read contiguous memory */
+struct RecSrc
 {
-	const b3hContactConstraint4& c = cs[idx];
+	const F4* recN;
+	const F4* recF;
+	F4* lamN; /* normal impulses per position */
+	F4* lamF; /* friction impulses per position (x, y) */
+	int stride;
+};
+template <int PHASE>
+__device__ __forceinline__ void slot_fetch(Slot<PHASE>& s, int p, const RecSrc& R)
+{
 	if (PHASE == 1)
 	{
-		const F4* F = recF + (size_t)idx * PGS_FREC;
 #pragma unroll
-		for (int k = 0; k < PGS_FREC; k++) s.r[k] = F[k];
-		s.r[PGS_FREC] = f4(c.m_appliedRambdaDt[0], c.m_appliedRambdaDt[1], c.m_appliedRambdaDt[2], c.m_appliedRambdaDt[3]);
-		s.r[PGS_FREC + 1] = f4(c.m_fAppliedRambdaDt[0], c.m_fAppliedRambdaDt[1], 0.f, 0.f);
+		for (int k = 0; k < PGS_FREC; k++) s.r[k] = R.recF[(size_t)k * R.stride + p];
+		s.r[PGS_FREC] = R.lamN[p];
+		s.r[PGS_FREC + 1] = R.lamF[p];
 	}
 	else
 	{
-		const F4* N = recN + (size_t)idx * PGS_NREC;
 #pragma unroll
-		for (int k = 0; k < PGS_NREC; k++) s.r[k] = N[k];
-		s.r[PGS_NREC] = f4(c.m_appliedRambdaDt[0], c.m_appliedRambdaDt[1], c.m_appliedRambdaDt[2], c.m_appliedRambdaDt[3]);
+		for (int k = 0; k < PGS_NREC; k++) s.r[k] = R.recN[(size_t)k * R.stride + p];
+		s.r[PGS_NREC] = R.lamN[p];
 	}
 }
 
@@ -1255,8 +1290,7 @@ __device__ __forceinline__ b3hMat3x3 body_inertia(const InertiaSrc& is, int slot
 
 template <int PHASE, int CI>
 __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4* sAv, const InertiaSrc& is,
-										  b3hContactConstraint4* __restrict__ cs, float friction, float& maxD,
-										  float& maxL)
+										  const RecSrc& R, float friction, float& maxD, float& maxL)
 {
 	struct
 	{
@@ -1264,7 +1298,7 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 	} S = {Sr};
 	const F4 meta = S.r[PHASE == 1 ? PGS_FREC - 1 : PGS_NREC - 1];
 	float imB = meta.x;
-	int sA = fbits(meta.y), sB = fbits(meta.z), idx = fbits(meta.w);
+	int sA = fbits(meta.y), sB = fbits(meta.z), pos = fbits(meta.w);
 	Vel va, vb;
 	/* static bodies (slot -1) read the zero slot nb */
 	const int rA = sA >= 0 ? sA : is.nb, rB = sB >= 0 ? sB : is.nb;
@@ -1314,8 +1348,7 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 			va.av = sub4(va.av, smul4(angNA * 0.1f, n));
 			vb.av = sub4(vb.av, smul4(angNB * 0.1f, n));
 		}
-		cs[idx].m_fAppliedRambdaDt[0] = fl[0];
-		cs[idx].m_fAppliedRambdaDt[1] = fl[1];
+		R.lamF[pos] = f4(fl[0], fl[1], 0.f, 0.f);
 	}
 	else
 	{
@@ -1346,8 +1379,7 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 				apply_pre(lin, imA, imB, mtMul1(iiA, a0), mtMul1(iiB, a1), rl, va, vb);
 			}
 		}
-#pragma unroll
-		for (int ic = 0; ic < 4; ic++) cs[idx].m_appliedRambdaDt[ic] = lam[ic];
+		R.lamN[pos] = f4(lam[0], lam[1], lam[2], lam[3]);
 	}
 	if (sA >= 0)
 	{
@@ -1368,12 +1400,14 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
  * and keeps the cold start's fixed point (all normal sweeps run before the
  * friction sweeps). */
 template <int PART>
-__device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restrict__ recF, F4* sLv, F4* sAv,
-						 const InertiaSrc& is, b3hContactConstraint4* __restrict__ cs, float friction,
-						 const float* __restrict__ warm)
+__device__ void pgs_warm(int p, const RecSrc& R, const int* __restrict__ levelOrder, F4* sLv, F4* sAv,
+						 const InertiaSrc& is, float friction, const float* __restrict__ warm)
 {
-	const F4* N = recN + (size_t)idx * PGS_NREC;
-	const F4* Fr = recF + (size_t)idx * PGS_FREC;
+	const size_t st = (size_t)R.stride;
+	const int idx = levelOrder[p]; /* the constraint, for its initial impulses */
+	F4 N[PGS_NREC], Fr[PGS_FREC];
+	for (int k = 0; k < PGS_NREC; k++) N[k] = R.recN[k * st + p];
+	for (int k = 0; k < PGS_FREC; k++) Fr[k] = R.recF[k * st + p];
 	F4 meta = N[9];
 	float imB = meta.x;
 	int sA = fbits(meta.y), sB = fbits(meta.z);
@@ -1388,8 +1422,8 @@ __device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restr
 	const b3hMat3x3 iiB = body_inertia(is, sB);
 	F4 lin = N[0];
 	float imA = lin.w;
-	float lam[4];
-	for (int ic = 0; ic < 4; ic++) lam[ic] = cs[idx].m_appliedRambdaDt[ic];
+	const F4 ln = R.lamN[p];
+	float lam[4] = {ln.x, ln.y, ln.z, ln.w};
 	if (PART == 0)
 	{
 		F4 jv = N[7];
@@ -1402,7 +1436,7 @@ __device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restr
 			F4 a0 = rc3(N, 4 + 6 * ic), a1 = rc3(N, 7 + 6 * ic);
 			apply_pre(lin, imA, imB, mtMul1(iiA, a0), mtMul1(iiB, a1), rl, va, vb);
 		}
-		for (int ic = 0; ic < 4; ic++) cs[idx].m_appliedRambdaDt[ic] = lam[ic];
+		R.lamN[p] = f4(lam[0], lam[1], lam[2], lam[3]);
 	}
 	F4 fj = Fr[6];
 	if (PART == 1 && !(fj.x == 0))
@@ -1410,13 +1444,15 @@ __device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restr
 		float sum = 0;
 		for (int j = 0; j < 4; j++) sum += lam[j];
 		float maxR = friction * sum;
+		float fl[2];
 		for (int k = 0; k < 2; k++)
 		{
 			float l = cl_min(cl_max(warm[6 * (size_t)idx + 4 + k], -maxR), maxR);
-			cs[idx].m_fAppliedRambdaDt[k] = l;
-			F4 fl = rc3(Fr, 9 * k), a0 = rc3(Fr, 9 * k + 3), a1 = rc3(Fr, 9 * k + 6);
-			apply_pre(fl, imA, imB, mtMul1(iiA, a0), mtMul1(iiB, a1), l, va, vb);
+			fl[k] = l;
+			F4 fd = rc3(Fr, 9 * k), a0 = rc3(Fr, 9 * k + 3), a1 = rc3(Fr, 9 * k + 6);
+			apply_pre(fd, imA, imB, mtMul1(iiA, a0), mtMul1(iiB, a1), l, va, vb);
 		}
+		R.lamF[p] = f4(fl[0], fl[1], 0.f, 0.f);
 	}
 	if (sA >= 0)
 	{
@@ -1430,47 +1466,44 @@ __device__ void pgs_warm(int idx, const F4* __restrict__ recN, const F4* __restr
 	}
 }
 
-/* one sweep of a cell by dependency levels (this thread: positions ls[c] + t,
- * + T, ... of each level c) */
+/* one sweep of a part by dependency levels (this thread: positions ls[c] + t,
+ * + T, ... of each level c, relative to the part's first position base) */
 template <int PHASE, int CI>
-__device__ __forceinline__ void pgs_walk(int t, int n, int L, const int* ls, const int* sOrder,
-										 const F4* __restrict__ recN, const F4* __restrict__ recF,
-										 b3hContactConstraint4* __restrict__ cs, F4* sLv, F4* sAv,
-										 const InertiaSrc& is, float friction, float& maxD, float& maxL)
+__device__ __forceinline__ void pgs_walk(int t, int n, int L, const int* ls, int base, const RecSrc& R, F4* sLv,
+										 F4* sAv, const InertiaSrc& is, float friction, float& maxD, float& maxL)
 {
 	const int T = PGS_THREADS;
 	/* Three record slots in fixed registers, one per level residue: level
 	 * c's first item of this thread (position ls[c] + t) is fetched into
 	 * slot c % 3 while level c - 3 is solved, so every record is in flight
-	 * for two levels.  The fetch is unconditional (a dummy index when the
-	 * thread has no item there), which lets the compiler wait for exactly
-	 * the slot it solves.  Further items of a level wider than the block
-	 * are fetched on the spot. */
+	 * for two levels.  The fetch is unconditional (the part's first position
+	 * when the thread has no item there), which lets the compiler wait for
+	 * exactly the slot it solves.  Further items of a level wider than the
+	 * block are fetched on the spot. */
 	Slot<PHASE> A, B, C;
 	bool hA = false, hB = false, hC = false;
-	const int dummy = sOrder[0];
 	{
 		int p0 = ls[0] + t, p1 = 1 < L ? ls[1] + t : n, p2 = 2 < L ? ls[2] + t : n;
 		hA = 0 < L && p0 < ls[1];
 		hB = 1 < L && p1 < ls[2];
 		hC = 2 < L && p2 < ls[3];
-		slot_fetch<PHASE>(A, hA ? sOrder[p0] : dummy, recN, recF, cs);
-		slot_fetch<PHASE>(B, hB ? sOrder[p1] : dummy, recN, recF, cs);
-		slot_fetch<PHASE>(C, hC ? sOrder[p2] : dummy, recN, recF, cs);
+		slot_fetch<PHASE>(A, base + (hA ? p0 : 0), R);
+		slot_fetch<PHASE>(B, base + (hB ? p1 : 0), R);
+		slot_fetch<PHASE>(C, base + (hC ? p2 : 0), R);
 	}
 #define PGS_LEVEL(X, hX, c)                                                                   \
 	{                                                                                         \
 		bool live = (c) < L;                                                                  \
-		if (hX) pgs_solve<PHASE, CI>(X.r, sLv, sAv, is, cs, friction, maxD, maxL);           \
+		if (hX) pgs_solve<PHASE, CI>(X.r, sLv, sAv, is, R, friction, maxD, maxL);            \
 		for (int q = live ? ls[c] + t + T : n; q < (live ? ls[(c) + 1] : 0); q += T)          \
 		{                                                                                     \
 			Slot<PHASE> W;                                                                    \
-			slot_fetch<PHASE>(W, sOrder[q], recN, recF, cs);                                  \
-			pgs_solve<PHASE, CI>(W.r, sLv, sAv, is, cs, friction, maxD, maxL);               \
+			slot_fetch<PHASE>(W, base + q, R);                                                \
+			pgs_solve<PHASE, CI>(W.r, sLv, sAv, is, R, friction, maxD, maxL);                \
 		}                                                                                     \
 		int pn = (c) + 3 < L ? ls[(c) + 3] + t : n;                                          \
 		hX = (c) + 3 < L && pn < ls[(c) + 4];                                                 \
-		slot_fetch<PHASE>(X, hX ? sOrder[pn] : dummy, recN, recF, cs);                        \
+		slot_fetch<PHASE>(X, base + (hX ? pn : 0), R);                                        \
 		lds_barrier();                                                                        \
 	}
 	/* every pass runs all three slots (refills and barriers unconditional,
@@ -1522,8 +1555,8 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 														   const int* __restrict__ levelStart,
 														   const int* __restrict__ levelOrder,
 														   const b3hInt4* __restrict__ partMeta,
-														   const int* __restrict__ partBodies, const F4* __restrict__ recN,
-														   const F4* __restrict__ recF, int ib, float friction,
+														   const int* __restrict__ partBodies, RecSrc R, int ib,
+														   float friction,
 														   const float* __restrict__ warm, float* __restrict__ resid,
 														   const int* __restrict__ lastClass, int fuseIntegrate,
 														   b3hIntegrateParams ip, int ldsBytes)
@@ -1583,6 +1616,7 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	const b3hInt4 pb = partMeta[2 * (cell * B3H_PGS_PARTS + part) + 1];
 	const int nbp = pb.y; /* this part's bodies: (slot, body) pairs */
 	const int* own = partBodies + 2 * (2 * (size_t)start + pb.x);
+	const int base = start + pm.x; /* the part's first position */
 	int nb = cellBodyCount[cell];
 	/* with room for it, 3 (nb + 2) inverse world inertia rows (slot nb is zero
 	 * for static bodies, nb + 1 padding); then the velocities.  Sized for the
@@ -1594,28 +1628,11 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	int* sList = (int*)(sAv + nb + 2); /* nb + 2: body of each slot (this part's slots) */
 	int* sOrder = sList + nb + 2;      /* n + 1 */
 	int* sLs = sOrder + n + 1;         /* L + 2: level starts relative to start */
-	/* stage the level order and the level starts in one pass (sLs follows
-	 * sOrder's padding entry) */
+	/* stage the level starts (positions are the part's first position + q;
+	 * records are read by position, so the level order itself is not needed) */
 	{
-		const int* srcO = levelOrder + start + pm.x;
 		const int* srcL = levelStart + start + B3H_PGS_PARTS * cell + pm.z;
-		const int m = n + L + 1;
-		for (int k0 = t; k0 < m; k0 += 8 * PGS_THREADS)
-		{
-			int v[8];
-#pragma unroll
-			for (int u = 0; u < 8; u++)
-			{
-				int k = k0 + u * PGS_THREADS;
-				v[u] = k < n ? srcO[k] : (k < m ? srcL[k - n] : srcO[0]);
-			}
-#pragma unroll
-			for (int u = 0; u < 8; u++)
-			{
-				int k = k0 + u * PGS_THREADS;
-				sOrder[k < n ? k : (k < m ? k + 1 : n)] = v[u];
-			}
-		}
+		for (int k = t; k <= L; k += PGS_THREADS) sLs[k] = srcL[k];
 	}
 	if (cacheI && t < 3) sI[3 * nb + t] = f4(0.f, 0.f, 0.f, 0.f);
 	if (t == 3) sLv[nb] = f4(0.f, 0.f, 0.f, 0.f);
@@ -1667,16 +1684,16 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		for (int cl = 0; cl < L; cl++)
 		{
 			for (int k = ls[cl] + t; k < ls[cl + 1]; k += T)
-				pgs_warm<PHASE - 2>(sOrder[k], recN, recF, sLv, sAv, is, cs, friction, warm);
+				pgs_warm<PHASE - 2>(base + k, R, levelOrder, sLv, sAv, is, friction, warm);
 			lds_barrier();
 		}
 	}
 	else
 	{
 		if (is.sI)
-			pgs_walk<PHASE, 1>(t, n, L, ls, sOrder, recN, recF, cs, sLv, sAv, is, friction, maxD, maxL);
+			pgs_walk<PHASE, 1>(t, n, L, ls, base, R, sLv, sAv, is, friction, maxD, maxL);
 		else
-			pgs_walk<PHASE, 0>(t, n, L, ls, sOrder, recN, recF, cs, sLv, sAv, is, friction, maxD, maxL);
+			pgs_walk<PHASE, 0>(t, n, L, ls, base, R, sLv, sAv, is, friction, maxD, maxL);
 	}
 	__syncthreads();
 	if (!fuseIntegrate)
@@ -1800,21 +1817,22 @@ void b3h_solve_setup(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaDa
 					   sb.cellBodyCount, sb.cslot, sb.lastClass);
 	int cap = (sb.pgsBodyCap >= 0 && sb.pgsBodyCap < s_pgsBodyCap) ? sb.pgsBodyCap : s_pgsBodyCap;
 	size_t pgsLds = (size_t)s_pgsLdsBytes;
-	hipLaunchKernelGGL(k_pgs_prep, dim3(gb), dim3(256), 0, s, numContacts, sb.constraints, sb.cslot, bodies, inertias,
-					   sb.recN, sb.recF);
 	hipLaunchKernelGGL(k_cell_levels, dim3(B3H_N_CELLS), dim3(256), (size_t)ldsCapBytes, s, sb.cellCounts,
 					   sb.cellOffsets, sb.batchIdx, sb.cellBodyCount, sb.cslot, (const b3hContactConstraint4*)sb.constraints,
 					   bodies, 1.f / prm.m_batchCellSize, cap, ldsCapBytes / 4, (int)pgsLds, sb.levelOrder,
-					   sb.levelStart, sb.cellLevels, (const int*)sb.cellBodies, sb.partMeta, sb.partBodies,
+					   sb.levelStart, sb.cellLevels, (const int*)sb.cellBodies, sb.partMeta, sb.partBodies, sb.posOf,
 					   sb.pgsMaxParts > 0 && sb.pgsMaxParts < B3H_PGS_PARTS ? sb.pgsMaxParts : B3H_PGS_PARTS);
+	hipLaunchKernelGGL(k_pgs_prep, dim3(gb), dim3(256), 0, s, numContacts, sb.constraints, sb.cslot, bodies, inertias,
+					   (const int*)sb.posOf, sb.maxContacts, sb.recN, sb.recF, sb.lamN, sb.lamF);
 }
 
 /* residual tracking (off the hot path): over the rows one sweep of `phase`
  * updates (0 normal, 1 friction), the sums of (impulse - impulse before the
  * sweep)^2 and impulse^2; refreshes the snapshot (phase < 0: snapshot only) */
 __global__ __launch_bounds__(256) void k_resid_l2(const int* __restrict__ nPtr,
-												  const b3hContactConstraint4* __restrict__ cs, float* __restrict__ snap,
-												  int phase, double* __restrict__ sums)
+												  const b3hContactConstraint4* __restrict__ cs, const int* __restrict__ posOf,
+												  const F4* __restrict__ lamN, const F4* __restrict__ lamF,
+												  float* __restrict__ snap, int phase, double* __restrict__ sums)
 {
 	__shared__ double red[2][256];
 	int n = *nPtr, t = threadIdx.x;
@@ -1822,11 +1840,26 @@ __global__ __launch_bounds__(256) void k_resid_l2(const int* __restrict__ nPtr,
 	for (int k = blockIdx.x * blockDim.x + t; k < n; k += gridDim.x * blockDim.x)
 	{
 		const b3hContactConstraint4& c = cs[k];
+		/* impulses of level-walked constraints live at their positions during
+		 * the sweeps; those of global-walk cells in the constraints */
+		const int p = posOf[k];
+		float ln[4], lf[2];
+		if (p >= 0)
+		{
+			const F4 a = lamN[p], b = lamF[p];
+			ln[0] = a.x, ln[1] = a.y, ln[2] = a.z, ln[3] = a.w;
+			lf[0] = b.x, lf[1] = b.y;
+		}
+		else
+		{
+			for (int i = 0; i < 4; i++) ln[i] = c.m_appliedRambdaDt[i];
+			for (int m = 0; m < 2; m++) lf[m] = c.m_fAppliedRambdaDt[m];
+		}
 		float* sn = snap + 6 * (size_t)k;
 		if (phase != 1)
 			for (int i = 0; i < 4; i++)
 			{
-				float l = c.m_appliedRambdaDt[i];
+				float l = ln[i];
 				if (phase == 0 && c.m_jacCoeffInv[i] != 0.f)
 				{
 					double d = (double)l - sn[i];
@@ -1838,7 +1871,7 @@ __global__ __launch_bounds__(256) void k_resid_l2(const int* __restrict__ nPtr,
 		if (phase != 0)
 			for (int m = 0; m < 2; m++)
 			{
-				float l = c.m_fAppliedRambdaDt[m];
+				float l = lf[m];
 				if (phase == 1)
 				{
 					double d = (double)l - sn[4 + m];
@@ -1879,7 +1912,9 @@ void b3h_solve_sweeps(b3hRigidBodyData* bodies, const b3hInertiaData* inertias, 
 #define B3H_PGS_ARGS(ib, resid_, warm_, fuse_)                                                                         \
 	bodies, inertias, sb.constraints, sb.cellCounts, sb.cellOffsets, sb.batchSizes, sb.batchIdx, sb.cellBodies, \
 		sb.cellBodyCount, sb.cellLevels, sb.levelStart, sb.levelOrder, (const b3hInt4*)sb.partMeta,               \
-		(const int*)sb.partBodies, (const F4*)sb.recN, (const F4*)sb.recF, ib,                                 \
+		(const int*)sb.partBodies, RecSrc{(const F4*)sb.recN, (const F4*)sb.recF, (F4*)sb.lamN, (F4*)sb.lamF,      \
+										 sb.maxContacts},                                                      \
+		ib,                                                                                                    \
 		prm.m_frictionCoeff, warm_, resid_, (const int*)sb.lastClass, fuse_, ip, sb.pgsNoInertiaCache ? 0 : (int)pgsLds
 	if (sb.warmImpulses)
 		for (int ib = 0; ib < B3H_N_CELL_BATCHES; ib++)
@@ -1892,7 +1927,8 @@ void b3h_solve_sweeps(b3hRigidBodyData* bodies, const b3hInertiaData* inertias, 
 	{
 		b3h_fill_i32((int*)sb.residL2, 0, 8 * K, s);
 		hipLaunchKernelGGL(k_resid_l2, dim3(rb), dim3(256), 0, s, sb.numContacts, (const b3hContactConstraint4*)sb.constraints,
-						   sb.residSnap, -1, (double*)nullptr);
+						   (const int*)sb.posOf, (const F4*)sb.lamN, (const F4*)sb.lamF, sb.residSnap, -1,
+						   (double*)nullptr);
 	}
 	for (int phase = 0; phase < 2; phase++)
 	{
@@ -1907,8 +1943,8 @@ void b3h_solve_sweeps(b3hRigidBodyData* bodies, const b3hInertiaData* inertias, 
 									   B3H_PGS_ARGS(ib, (float*)nullptr, (const float*)sb.warmImpulses, 0));
 				if (l2)
 					hipLaunchKernelGGL(k_resid_l2, dim3(rb), dim3(256), 0, s, sb.numContacts,
-									   (const b3hContactConstraint4*)sb.constraints, sb.residSnap, -1,
-									   (double*)nullptr);
+									   (const b3hContactConstraint4*)sb.constraints, (const int*)sb.posOf,
+									   (const F4*)sb.lamN, (const F4*)sb.lamF, sb.residSnap, -1, (double*)nullptr);
 			}
 		}
 		for (int it = 0; it < K; it++)
@@ -1926,10 +1962,14 @@ void b3h_solve_sweeps(b3hRigidBodyData* bodies, const b3hInertiaData* inertias, 
 			}
 			if (l2)
 				hipLaunchKernelGGL(k_resid_l2, dim3(rb), dim3(256), 0, s, sb.numContacts,
-								   (const b3hContactConstraint4*)sb.constraints, sb.residSnap, phase,
+								   (const b3hContactConstraint4*)sb.constraints, (const int*)sb.posOf,
+								   (const F4*)sb.lamN, (const F4*)sb.lamF, sb.residSnap, phase,
 								   sb.residL2 + 2 * (phase * K + it));
 		}
 	}
+	/* the final impulses of level-walked constraints back into the constraints */
+	hipLaunchKernelGGL(k_lambda_writeback, dim3(rb), dim3(256), 0, s, sb.numContacts, (const int*)sb.posOf,
+					   (const F4*)sb.lamN, (const F4*)sb.lamF, sb.constraints);
 #undef B3H_PGS_ARGS
 }
 
