@@ -1103,8 +1103,8 @@ extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const 
 		(r = dalloc(&p->sb.batchSizes, B3H_N_CELLS)) || (r = dalloc(&p->sb.sortKeys, 4 * (size_t)mc + 64)) ||
 		(r = dalloc(&p->sb.constraints, mc)) || (r = dalloc(&p->sb.sortedContacts, mc)) || (r = dalloc(&p->sb.resid, 4 * 64)) ||
 		(r = dalloc(&p->sb.cellBodies, 2 * (size_t)mc + 64)) || (r = dalloc(&p->sb.cellBodyCount, B3H_N_CELLS)) ||
-		(r = dalloc(&p->sb.cslot, 2 * (size_t)mc + 2)) || (r = dalloc(&p->sb.recN, 10 * (size_t)mc)) ||
-		(r = dalloc(&p->sb.recF, 8 * (size_t)mc)) || (r = dalloc(&p->sb.lamN, mc)) ||
+		(r = dalloc(&p->sb.cslot, 2 * (size_t)mc + 2)) || (r = dalloc(&p->sb.recN, 16 * (size_t)mc)) ||
+		(r = dalloc(&p->sb.recF, 11 * (size_t)mc)) || (r = dalloc(&p->sb.lamN, mc)) ||
 		(r = dalloc(&p->sb.lamF, mc)) || (r = dalloc(&p->sb.posOf, mc)) || (r = dalloc(&p->sb.levelOrder, mc)) ||
 		(r = dalloc(&p->sb.levelStart, (size_t)mc + B3H_N_CELLS * B3H_PGS_PARTS + 8)) ||
 		(r = dalloc(&p->sb.partMeta, 2 * B3H_N_CELLS * B3H_PGS_PARTS)) ||

@@ -773,8 +773,8 @@ __global__ __launch_bounds__(256) void k_cell_bodies(const int* __restrict__ cel
  * and v += imA*lin*rl, w += (iiA*ang0)*rl with the bodies' inverse inertias
  * cached in LDS: the same operations, so the same roundings, as recomputing
  * everything from the constraint. */
-#define PGS_NREC 10
-#define PGS_FREC 8
+#define PGS_NREC 16
+#define PGS_FREC 11
 #define PGS_GATHER 8
 #ifndef PGS_THREADS
 #define PGS_THREADS 256
@@ -800,57 +800,61 @@ __global__ __launch_bounds__(256) void k_pgs_prep(const int* __restrict__ nPtr, 
 		float imA = bodies[a].m_invMass, imB = bodies[b].m_invMass;
 		int sA = cslot[2 * (size_t)i], sB = cslot[2 * (size_t)i + 1];
 		F4 meta = f4(imB, ibits(sA), ibits(sB), ibits(p));
+		/* the sweep's inverse world inertias: the body's, zero for a static
+		 * body (slot -1) */
+		const F4 z4 = f4(0.f, 0.f, 0.f, 0.f);
+		b3hMat3x3 iiA, iiB;
+		if (sA >= 0)
+			iiA = inertias[a].m_invInertiaWorld;
+		else
+			iiA.m_row[0] = iiA.m_row[1] = iiA.m_row[2] = z4;
+		if (sB >= 0)
+			iiB = inertias[b].m_invInertiaWorld;
+		else
+			iiB.m_row[0] = iiB.m_row[1] = iiB.m_row[2] = z4;
 		F4 n = neg4(c.m_linear);
 		recN[p] = f4(c.m_linear.x, c.m_linear.y, c.m_linear.z, imA);
-		float pk[24];
+		float pk[48];
 		for (int ic = 0; ic < 4; ic++)
 		{
 			F4 r0 = sub4(c.m_worldPos[ic], posA);
 			F4 r1 = sub4(c.m_worldPos[ic], posB);
 			F4 ang0 = neg4(cross3(r0, n));
 			F4 ang1 = cross3(r1, n);
-			pk[6 * ic + 0] = ang0.x;
-			pk[6 * ic + 1] = ang0.y;
-			pk[6 * ic + 2] = ang0.z;
-			pk[6 * ic + 3] = ang1.x;
-			pk[6 * ic + 4] = ang1.y;
-			pk[6 * ic + 5] = ang1.z;
+			F4 q0 = mtMul1(iiA, ang0), q1 = mtMul1(iiB, ang1);
+			const float row[12] = {ang0.x, ang0.y, ang0.z, ang1.x, ang1.y, ang1.z, q0.x, q0.y, q0.z, q1.x, q1.y, q1.z};
+			for (int j = 0; j < 12; j++) pk[12 * ic + j] = row[j];
 		}
-		for (int q = 0; q < 6; q++)
+		for (int q = 0; q < 12; q++)
 			recN[(size_t)(1 + q) * stride + p] = f4(pk[4 * q], pk[4 * q + 1], pk[4 * q + 2], pk[4 * q + 3]);
-		recN[(size_t)7 * stride + p] = f4(c.m_jacCoeffInv[0], c.m_jacCoeffInv[1], c.m_jacCoeffInv[2], c.m_jacCoeffInv[3]);
-		recN[(size_t)8 * stride + p] = f4(c.m_b[0], c.m_b[1], c.m_b[2], c.m_b[3]);
-		recN[(size_t)9 * stride + p] = meta;
+		recN[(size_t)13 * stride + p] = f4(c.m_jacCoeffInv[0], c.m_jacCoeffInv[1], c.m_jacCoeffInv[2], c.m_jacCoeffInv[3]);
+		recN[(size_t)14 * stride + p] = f4(c.m_b[0], c.m_b[1], c.m_b[2], c.m_b[3]);
+		recN[(size_t)15 * stride + p] = meta;
 		F4 t[2];
 		plane_space(n, &t[0], &t[1]);
 		F4 r0 = sub4(c.m_center, posA), r1 = sub4(c.m_center, posB);
-		float fk[24];
+		float fk[36];
 		for (int k = 0; k < 2; k++)
 		{
 			F4 ang0 = neg4(cross3(r0, t[k]));
 			F4 ang1 = cross3(r1, t[k]);
-			fk[9 * k + 0] = -t[k].x;
-			fk[9 * k + 1] = -t[k].y;
-			fk[9 * k + 2] = -t[k].z;
-			fk[9 * k + 3] = ang0.x;
-			fk[9 * k + 4] = ang0.y;
-			fk[9 * k + 5] = ang0.z;
-			fk[9 * k + 6] = ang1.x;
-			fk[9 * k + 7] = ang1.y;
-			fk[9 * k + 8] = ang1.z;
+			F4 q0 = mtMul1(iiA, ang0), q1 = mtMul1(iiB, ang1);
+			const float dir[15] = {-t[k].x, -t[k].y, -t[k].z, ang0.x, ang0.y, ang0.z, ang1.x, ang1.y,
+								   ang1.z,  q0.x,    q0.y,    q0.z,   q1.x,   q1.y,   q1.z};
+			for (int j = 0; j < 15; j++) fk[15 * k + j] = dir[j];
 		}
-		fk[18] = n.x;
-		fk[19] = n.y;
-		fk[20] = n.z;
-		fk[21] = n.w; /* the twist damping subtracts the whole float4 (w included) */
-		fk[22] = fk[23] = 0.f;
-		for (int q = 0; q < 6; q++)
+		fk[30] = n.x;
+		fk[31] = n.y;
+		fk[32] = n.z;
+		fk[33] = n.w; /* the twist damping subtracts the whole float4 (w included) */
+		fk[34] = fk[35] = 0.f;
+		for (int q = 0; q < 9; q++)
 			recF[(size_t)q * stride + p] = f4(fk[4 * q], fk[4 * q + 1], fk[4 * q + 2], fk[4 * q + 3]);
 		F4 ab = nrm3(sub4(posB, posA));
 		F4 ac = nrm3(sub4(c.m_center, posA));
 		bool damp = dot3(ab, ac) > 0.95f || (imA == 0.f || imB == 0.f);
-		recF[(size_t)6 * stride + p] = f4(c.m_fJacCoeffInv[0], c.m_fJacCoeffInv[1], damp ? 1.f : 0.f, imA);
-		recF[(size_t)7 * stride + p] = meta;
+		recF[(size_t)9 * stride + p] = f4(c.m_fJacCoeffInv[0], c.m_fJacCoeffInv[1], damp ? 1.f : 0.f, imA);
+		recF[(size_t)10 * stride + p] = meta;
 		lamN[p] = f4(c.m_appliedRambdaDt[0], c.m_appliedRambdaDt[1], c.m_appliedRambdaDt[2], c.m_appliedRambdaDt[3]);
 		lamF[p] = f4(c.m_fAppliedRambdaDt[0], c.m_fAppliedRambdaDt[1], 0.f, 0.f);
 	}
@@ -1241,56 +1245,9 @@ __device__ __forceinline__ float rc(const F4* r, int j)
 	}
 }
 __device__ __forceinline__ F4 rc3(const F4* r, int j) { return f4(rc(r, j), rc(r, j + 1), rc(r, j + 2), 0.f); }
-/* where a sweep finds the inverse world inertia of a cell body: the LDS cache
- * (static bodies: its zero slot nb) or, for cells too large for the cache, the
- * global record through the cell's body list (static bodies: zero) */
-struct InertiaSrc
-{
-	const F4* sI;
-	const b3hInertiaData* g;
-	const int* sList;
-	int nb;
-};
-/* CI: the cache is in use (1), not in use (0), or decided at run time (-1) */
-template <int CI = -1>
-__device__ __forceinline__ b3hMat3x3 body_inertia(const InertiaSrc& is, int slot)
-{
-	if (CI == 1)
-	{
-		const F4* m = is.sI + 3 * (slot >= 0 ? slot : is.nb);
-		b3hMat3x3 r;
-		r.m_row[0] = m[0];
-		r.m_row[1] = m[1];
-		r.m_row[2] = m[2];
-		return r;
-	}
-	if (CI == 0)
-	{
-		b3hMat3x3 r;
-		if (slot >= 0)
-			r = is.g[is.sList[slot]].m_invInertiaWorld;
-		else
-			r.m_row[0] = r.m_row[1] = r.m_row[2] = f4(0.f, 0.f, 0.f, 0.f);
-		return r;
-	}
-	b3hMat3x3 r;
-	if (is.sI)
-	{
-		const F4* m = is.sI + 3 * (slot >= 0 ? slot : is.nb);
-		r.m_row[0] = m[0];
-		r.m_row[1] = m[1];
-		r.m_row[2] = m[2];
-	}
-	else if (slot >= 0)
-		r = is.g[is.sList[slot]].m_invInertiaWorld;
-	else
-		r.m_row[0] = r.m_row[1] = r.m_row[2] = f4(0.f, 0.f, 0.f, 0.f);
-	return r;
-}
-
-template <int PHASE, int CI>
-__device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4* sAv, const InertiaSrc& is,
-										  const RecSrc& R, float friction, float& maxD, float& maxL)
+template <int PHASE>
+__device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4* sAv, int nb, const RecSrc& R,
+										  float friction, float& maxD, float& maxL)
 {
 	struct
 	{
@@ -1301,16 +1258,14 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 	int sA = fbits(meta.y), sB = fbits(meta.z), pos = fbits(meta.w);
 	Vel va, vb;
 	/* static bodies (slot -1) read the zero slot nb */
-	const int rA = sA >= 0 ? sA : is.nb, rB = sB >= 0 ? sB : is.nb;
+	const int rA = sA >= 0 ? sA : nb, rB = sB >= 0 ? sB : nb;
 	va.lv = sLv[rA];
 	va.av = sAv[rA];
 	vb.lv = sLv[rB];
 	vb.av = sAv[rB];
-	const b3hMat3x3 iiA = body_inertia<CI>(is, sA);
-	const b3hMat3x3 iiB = body_inertia<CI>(is, sB);
 	if (PHASE == 1)
 	{
-		F4 fj = S.r[6];
+		F4 fj = S.r[9];
 		if (fj.x == 0) return; /* solveFriction.cl: rows without friction are skipped */
 		float imA = fj.w;
 		F4 lamN = S.r[PGS_FREC];
@@ -1326,7 +1281,7 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 #pragma unroll
 		for (int k = 0; k < 2; k++)
 		{
-			F4 lin = rc3(S.r, 9 * k), a0 = rc3(S.r, 9 * k + 3), a1 = rc3(S.r, 9 * k + 6);
+			F4 lin = rc3(S.r, 15 * k), a0 = rc3(S.r, 15 * k + 3), a1 = rc3(S.r, 15 * k + 6);
 			float rl = rel_vel(lin, a0, a1, va, vb);
 			rl *= fjac[k];
 			float prev = fl[k];
@@ -1338,11 +1293,11 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 			fl[k] = upd;
 			maxD = fmaxf(maxD, fabsf(rl));
 			maxL = fmaxf(maxL, fabsf(upd));
-			apply_pre(lin, imA, imB, mtMul1(iiA, a0), mtMul1(iiB, a1), rl, va, vb);
+			apply_pre(lin, imA, imB, rc3(S.r, 15 * k + 9), rc3(S.r, 15 * k + 12), rl, va, vb);
 		}
 		if (fj.z != 0.f)
 		{
-			F4 n = f4(rc(S.r, 18), rc(S.r, 19), rc(S.r, 20), rc(S.r, 21));
+			F4 n = f4(rc(S.r, 30), rc(S.r, 31), rc(S.r, 32), rc(S.r, 33));
 			float angNA = dot3(n, va.av);
 			float angNB = dot3(n, vb.av);
 			va.av = sub4(va.av, smul4(angNA * 0.1f, n));
@@ -1356,7 +1311,7 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 		float imA = lin.w;
 		F4 lamv = S.r[PGS_NREC];
 		float lam[4] = {lamv.x, lamv.y, lamv.z, lamv.w};
-		F4 jv = S.r[7], bv = S.r[8];
+		F4 jv = S.r[13], bv = S.r[14];
 		float jac[4] = {jv.x, jv.y, jv.z, jv.w};
 		float bb[4] = {bv.x, bv.y, bv.z, bv.w};
 #pragma unroll
@@ -1364,7 +1319,7 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 		{
 			if (jac[ic] != 0.f)
 			{
-				F4 a0 = rc3(S.r, 4 + 6 * ic), a1 = rc3(S.r, 7 + 6 * ic);
+				F4 a0 = rc3(S.r, 4 + 12 * ic), a1 = rc3(S.r, 7 + 12 * ic);
 				float rl = rel_vel(lin, a0, a1, va, vb) + bb[ic];
 				rl *= jac[ic];
 				float prev = lam[ic];
@@ -1376,7 +1331,7 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 				lam[ic] = upd;
 				maxD = fmaxf(maxD, fabsf(rl));
 				maxL = fmaxf(maxL, fabsf(upd));
-				apply_pre(lin, imA, imB, mtMul1(iiA, a0), mtMul1(iiB, a1), rl, va, vb);
+				apply_pre(lin, imA, imB, rc3(S.r, 10 + 12 * ic), rc3(S.r, 13 + 12 * ic), rl, va, vb);
 			}
 		}
 		R.lamN[pos] = f4(lam[0], lam[1], lam[2], lam[3]);
@@ -1400,45 +1355,42 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
  * and keeps the cold start's fixed point (all normal sweeps run before the
  * friction sweeps). */
 template <int PART>
-__device__ void pgs_warm(int p, const RecSrc& R, const int* __restrict__ levelOrder, F4* sLv, F4* sAv,
-						 const InertiaSrc& is, float friction, const float* __restrict__ warm)
+__device__ void pgs_warm(int p, const RecSrc& R, const int* __restrict__ levelOrder, F4* sLv, F4* sAv, int nb,
+						 float friction, const float* __restrict__ warm)
 {
 	const size_t st = (size_t)R.stride;
 	const int idx = levelOrder[p]; /* the constraint, for its initial impulses */
 	F4 N[PGS_NREC], Fr[PGS_FREC];
 	for (int k = 0; k < PGS_NREC; k++) N[k] = R.recN[k * st + p];
 	for (int k = 0; k < PGS_FREC; k++) Fr[k] = R.recF[k * st + p];
-	F4 meta = N[9];
+	F4 meta = N[PGS_NREC - 1];
 	float imB = meta.x;
 	int sA = fbits(meta.y), sB = fbits(meta.z);
 	Vel va, vb;
 	/* static bodies (slot -1) read the zero slot nb */
-	const int rA = sA >= 0 ? sA : is.nb, rB = sB >= 0 ? sB : is.nb;
+	const int rA = sA >= 0 ? sA : nb, rB = sB >= 0 ? sB : nb;
 	va.lv = sLv[rA];
 	va.av = sAv[rA];
 	vb.lv = sLv[rB];
 	vb.av = sAv[rB];
-	const b3hMat3x3 iiA = body_inertia(is, sA);
-	const b3hMat3x3 iiB = body_inertia(is, sB);
 	F4 lin = N[0];
 	float imA = lin.w;
 	const F4 ln = R.lamN[p];
 	float lam[4] = {ln.x, ln.y, ln.z, ln.w};
 	if (PART == 0)
 	{
-		F4 jv = N[7];
+		F4 jv = N[13];
 		float jac[4] = {jv.x, jv.y, jv.z, jv.w};
 		for (int ic = 0; ic < 4; ic++)
 		{
 			if (jac[ic] == 0.f) continue;
 			float rl = cl_max(warm[6 * (size_t)idx + ic], 0.f);
 			lam[ic] = rl;
-			F4 a0 = rc3(N, 4 + 6 * ic), a1 = rc3(N, 7 + 6 * ic);
-			apply_pre(lin, imA, imB, mtMul1(iiA, a0), mtMul1(iiB, a1), rl, va, vb);
+			apply_pre(lin, imA, imB, rc3(N, 10 + 12 * ic), rc3(N, 13 + 12 * ic), rl, va, vb);
 		}
 		R.lamN[p] = f4(lam[0], lam[1], lam[2], lam[3]);
 	}
-	F4 fj = Fr[6];
+	F4 fj = Fr[9];
 	if (PART == 1 && !(fj.x == 0))
 	{
 		float sum = 0;
@@ -1449,8 +1401,7 @@ __device__ void pgs_warm(int p, const RecSrc& R, const int* __restrict__ levelOr
 		{
 			float l = cl_min(cl_max(warm[6 * (size_t)idx + 4 + k], -maxR), maxR);
 			fl[k] = l;
-			F4 fd = rc3(Fr, 9 * k), a0 = rc3(Fr, 9 * k + 3), a1 = rc3(Fr, 9 * k + 6);
-			apply_pre(fd, imA, imB, mtMul1(iiA, a0), mtMul1(iiB, a1), l, va, vb);
+			apply_pre(rc3(Fr, 15 * k), imA, imB, rc3(Fr, 15 * k + 9), rc3(Fr, 15 * k + 12), l, va, vb);
 		}
 		R.lamF[p] = f4(fl[0], fl[1], 0.f, 0.f);
 	}
@@ -1468,9 +1419,9 @@ __device__ void pgs_warm(int p, const RecSrc& R, const int* __restrict__ levelOr
 
 /* one sweep of a part by dependency levels (this thread: positions ls[c] + t,
  * + T, ... of each level c, relative to the part's first position base) */
-template <int PHASE, int CI>
+template <int PHASE>
 __device__ __forceinline__ void pgs_walk(int t, int n, int L, const int* ls, int base, const RecSrc& R, F4* sLv,
-										 F4* sAv, const InertiaSrc& is, float friction, float& maxD, float& maxL)
+										 F4* sAv, int nb, float friction, float& maxD, float& maxL)
 {
 	const int T = PGS_THREADS;
 	/* Three record slots in fixed registers, one per level residue: level
@@ -1494,12 +1445,12 @@ __device__ __forceinline__ void pgs_walk(int t, int n, int L, const int* ls, int
 #define PGS_LEVEL(X, hX, c)                                                                   \
 	{                                                                                         \
 		bool live = (c) < L;                                                                  \
-		if (hX) pgs_solve<PHASE, CI>(X.r, sLv, sAv, is, R, friction, maxD, maxL);            \
+		if (hX) pgs_solve<PHASE>(X.r, sLv, sAv, nb, R, friction, maxD, maxL);                \
 		for (int q = live ? ls[c] + t + T : n; q < (live ? ls[(c) + 1] : 0); q += T)          \
 		{                                                                                     \
 			Slot<PHASE> W;                                                                    \
 			slot_fetch<PHASE>(W, base + q, R);                                                \
-			pgs_solve<PHASE, CI>(W.r, sLv, sAv, is, R, friction, maxD, maxL);                \
+			pgs_solve<PHASE>(W.r, sLv, sAv, nb, R, friction, maxD, maxL);                    \
 		}                                                                                     \
 		int pn = (c) + 3 < L ? ls[(c) + 3] + t : n;                                          \
 		hX = (c) + 3 < L && pn < ls[(c) + 4];                                                 \
@@ -1618,15 +1569,14 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	const int* own = partBodies + 2 * (2 * (size_t)start + pb.x);
 	const int base = start + pm.x; /* the part's first position */
 	int nb = cellBodyCount[cell];
-	/* with room for it, 3 (nb + 2) inverse world inertia rows (slot nb is zero
-	 * for static bodies, nb + 1 padding); then the velocities.  Sized for the
-	 * whole cell (slots are the cell's), filled for this part's bodies only */
-	bool cacheI = pgs_level_lds(nb, cellN, true) <= (size_t)ldsBytes;
-	F4* sI = smem4;
-	F4* sLv = cacheI ? sI + 3 * (nb + 2) : smem4; /* nb + 2: slot nb zero (static bodies), nb + 1 padding */
-	F4* sAv = sLv + nb + 2;   /* nb + 2 */
-	int* sList = (int*)(sAv + nb + 2); /* nb + 2: body of each slot (this part's slots) */
-	int* sOrder = sList + nb + 2;      /* n + 1 */
+	/* velocities, sized for the whole cell (slots are the cell's), filled for
+	 * this part's bodies only; the rows carry their inverse-inertia products,
+	 * so no inertia is read here */
+	(void)ldsBytes;
+	F4* sLv = smem4;                   /* nb + 2: slot nb zero (static bodies), nb + 1 padding */
+	F4* sAv = sLv + nb + 2;            /* nb + 2 */
+	int* sList = (int*)(sAv + nb + 2); /* nb + 2 (unused by the sweep) */
+	int* sOrder = sList + nb + 2;      /* n + 1 (unused by the sweep) */
 	int* sLs = sOrder + n + 1;         /* L + 2: level starts relative to start */
 	/* stage the level starts (positions are the part's first position + q;
 	 * records are read by position, so the level order itself is not needed) */
@@ -1634,16 +1584,15 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		const int* srcL = levelStart + start + B3H_PGS_PARTS * cell + pm.z;
 		for (int k = t; k <= L; k += PGS_THREADS) sLs[k] = srcL[k];
 	}
-	if (cacheI && t < 3) sI[3 * nb + t] = f4(0.f, 0.f, 0.f, 0.f);
 	if (t == 3) sLv[nb] = f4(0.f, 0.f, 0.f, 0.f);
 	if (t == 4) sAv[nb] = f4(0.f, 0.f, 0.f, 0.f);
-	/* gather this part's velocities and inverse inertias, four bodies per
-	 * thread per round with every load issued before any is stored (lanes past
-	 * the end read body 0, which exists, and store into the padding slots) */
+	/* gather this part's velocities, four bodies per thread per round with
+	 * every load issued before any is stored (lanes past the end read body 0,
+	 * which exists, and store into the padding slot) */
 	for (int k0 = t; k0 < nbp; k0 += 4 * T)
 	{
 		int sv[4], bb[4];
-		F4 lv[4], av[4], r0[4], r1[4], r2[4];
+		F4 lv[4], av[4];
 #pragma unroll
 		for (int u = 0; u < 4; u++)
 		{
@@ -1656,26 +1605,14 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		{
 			lv[u] = bodies[bb[u]].m_linVel;
 			av[u] = bodies[bb[u]].m_angVel;
-			r0[u] = inertias[bb[u]].m_invInertiaWorld.m_row[0];
-			r1[u] = inertias[bb[u]].m_invInertiaWorld.m_row[1];
-			r2[u] = inertias[bb[u]].m_invInertiaWorld.m_row[2];
 		}
 #pragma unroll
 		for (int u = 0; u < 4; u++)
 		{
-			int si = sv[u] < nb ? sv[u] : nb + 1;
 			sLv[sv[u]] = lv[u];
 			sAv[sv[u]] = av[u];
-			sList[sv[u]] = bb[u];
-			if (cacheI)
-			{
-				sI[3 * si + 0] = r0[u];
-				sI[3 * si + 1] = r1[u];
-				sI[3 * si + 2] = r2[u];
-			}
 		}
 	}
-	const InertiaSrc is = {cacheI ? (const F4*)sI : (const F4*)nullptr, inertias, sList, nb};
 	const int* ls = sLs;
 	float maxD = 0.f, maxL = 0.f;
 	__syncthreads();
@@ -1684,17 +1621,12 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		for (int cl = 0; cl < L; cl++)
 		{
 			for (int k = ls[cl] + t; k < ls[cl + 1]; k += T)
-				pgs_warm<PHASE - 2>(base + k, R, levelOrder, sLv, sAv, is, friction, warm);
+				pgs_warm<PHASE - 2>(base + k, R, levelOrder, sLv, sAv, nb, friction, warm);
 			lds_barrier();
 		}
 	}
 	else
-	{
-		if (is.sI)
-			pgs_walk<PHASE, 1>(t, n, L, ls, base, R, sLv, sAv, is, friction, maxD, maxL);
-		else
-			pgs_walk<PHASE, 0>(t, n, L, ls, base, R, sLv, sAv, is, friction, maxD, maxL);
-	}
+		pgs_walk<PHASE>(t, n, L, ls, base, R, sLv, sAv, nb, friction, maxD, maxL);
 	__syncthreads();
 	if (!fuseIntegrate)
 		for (int q = t; q < nbp; q += T)

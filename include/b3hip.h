@@ -193,9 +193,10 @@ int b3h_pipeline_get_solver_debug(b3hPipeline* p, b3hContact4Data** sortedContac
  * global read-modify-write walk).  No counterpart in the reference (the
  * reference always walks global memory); for tests and tuning. */
 int b3h_pipeline_set_pgs_body_cache(b3hPipeline* p, int maxBodiesPerCell);
-/* Whether the PGS sweep may keep the cell bodies' inverse inertias in LDS
- * (default 1; cells too large for it read them from global memory either way).
- * No counterpart in the reference; for tests and tuning. */
+/* Kept for callers of earlier versions; no effect: the sweep's rows carry
+ * their inverse-inertia products (computed once per step with the sweep's own
+ * operations), so no inertia is read during the sweeps.  No counterpart in
+ * the reference. */
 int b3h_pipeline_set_pgs_inertia_cache(b3hPipeline* p, int enable);
 /* Dependency levels per solver cell of the last solve (host int[256]; -1: the
  * cell took the global walk) -- syncs.  No counterpart in the reference. */
