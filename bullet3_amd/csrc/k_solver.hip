@@ -121,6 +121,144 @@ __global__ __launch_bounds__(256) void k_cell_scatter(const int* __restrict__ nP
 	}
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) unsigned long long LdsU64;
+#define LDS_U64(p) ((LdsU64*)(p))
+#else
+#define LDS_U64(p) (p)
+#endif
+/* bitonic sort ascending of np2 keys with their payloads (KP: LDS or global) */
+template <typename KP>
+__device__ __forceinline__ void bitonic_sort(KP key, KP pay, int np2, int t)
+{
+	for (int k = 2; k <= np2; k <<= 1)
+		for (int j = k >> 1; j > 0; j >>= 1)
+		{
+			for (int i = t; i < np2; i += blockDim.x)
+			{
+				int ixj = i ^ j;
+				if (ixj > i)
+				{
+					unsigned long long ki = key[i], kj = key[ixj];
+					bool up = (i & k) == 0;
+					if ((ki > kj) == up)
+					{
+						key[i] = kj;
+						key[ixj] = ki;
+						unsigned long long p = pay[i];
+						pay[i] = pay[ixj];
+						pay[ixj] = p;
+					}
+				}
+			}
+			__syncthreads();
+		}
+}
+/* the greedy pass of k_batch_cells on one wavefront (KP: LDS or global
+ * pointers to the keys and payloads); returns the number of batches */
+template <typename KP>
+__device__ __forceinline__ int batch_greedy(KP key, KP pay, int n, int staticIdx)
+{
+	const int lane = threadIdx.x & 63;
+	int numValid = 0, batch = 0;
+	while (numValid < n)
+	{
+		int nCur = 0;
+		unsigned word = 0u; /* bitmap word `lane` */
+		for (int c0 = numValid; c0 < n; c0 += 64)
+		{
+			int i = c0 + lane;
+			bool valid = i < n;
+			unsigned long long kk = valid ? key[i] : 0ull;
+			int aS = (int)(unsigned)(kk >> 32), bS = (int)(unsigned)(kk & 0xffffffffu);
+			int a = iabs(aS), b = iabs(bS);
+			bool aStatic = (aS < 0) || aS == staticIdx;
+			bool bStatic = (bS < 0) || bS == staticIdx;
+			int ia = a % (32 * CHECK_SIZE), ib = b % (32 * CHECK_SIZE);
+			int wa = ia >> 5, wb = ib >> 5;
+			unsigned ma = aStatic ? 0u : (1u << (ia & 31));
+			unsigned mb = bStatic ? 0u : (1u << (ib & 31));
+			unsigned wA = __shfl(word, wa), wB = __shfl(word, wb);
+			bool pre = valid && (wA & ma) == 0u && (wB & mb) == 0u;
+			unsigned long long cand = __ballot(pre);
+			unsigned long long acc = 0ull;
+			const unsigned long long validMask = __ballot(valid);
+			while (cand)
+			{
+				/* j is the same on every lane: read its values with readlane */
+				int j = __ffsll((long long)cand) - 1;
+				acc |= 1ull << j;
+				int jwa = __builtin_amdgcn_readlane(wa, j), jwb = __builtin_amdgcn_readlane(wb, j);
+				unsigned jma = (unsigned)__builtin_amdgcn_readlane((int)ma, j);
+				unsigned jmb = (unsigned)__builtin_amdgcn_readlane((int)mb, j);
+				if (lane == jwa) word |= jma;
+				if (lane == jwb) word |= jmb;
+				nCur++;
+				unsigned long long later = (j == 63) ? 0ull : (~0ull << (j + 1));
+				if (nCur == SIMD_WIDTH)
+				{
+					nCur = 0;
+					word = 0u;
+					cand = validMask & later;
+				}
+				else
+				{
+					/* bitwise, without short-circuit branches */
+					bool hit = ((wa == jwa) & ((ma & jma) != 0u)) | ((wa == jwb) & ((ma & jmb) != 0u)) |
+							   ((wb == jwa) & ((mb & jma) != 0u)) | ((wb == jwb) & ((mb & jmb) != 0u));
+					cand &= later & ~__ballot(hit);
+				}
+			}
+			/* move the accepted contacts to the front, in order.  When at least
+			 * as many rejected contacts lie before the chunk as the chunk
+			 * accepts, the swaps (accepted m <-> numValid + m) are disjoint and
+			 * run in parallel; otherwise one lane performs them in sequence. */
+			int kAcc = __popcll(acc);
+			if (c0 - numValid >= kAcc)
+			{
+				if ((acc >> lane) & 1ull)
+				{
+					int m = __popcll(acc & ((1ull << lane) - 1ull));
+					int dst = numValid + m;
+					int ii = c0 + lane;
+					unsigned long long pj = pay[ii];
+					unsigned long long kd = key[dst], pd = pay[dst];
+					key[ii] = kd;
+					pay[ii] = pd;
+					key[dst] = kk;
+					pay[dst] = (pj & 0xffffffffull) | ((unsigned long long)(unsigned)batch << 32);
+				}
+				numValid += kAcc;
+			}
+			else if (lane == 0)
+			{
+				unsigned long long m = acc;
+				while (m)
+				{
+					int j = __ffsll((long long)m) - 1;
+					m &= m - 1;
+					int ii = c0 + j;
+					unsigned long long kj = key[ii];
+					unsigned long long pj = pay[ii];
+					unsigned long long pi = (pj & 0xffffffffull) | ((unsigned long long)(unsigned)batch << 32);
+					if (ii != numValid)
+					{
+						key[ii] = key[numValid];
+						pay[ii] = pay[numValid];
+						key[numValid] = kj;
+					}
+					pay[numValid] = pi;
+					numValid++;
+				}
+			}
+			numValid = __shfl(numValid, 0);
+			wave_sync_solver();
+		}
+		batch++;
+	}
+	return batch;
+}
+
 /* One work-group per cell: sort the cell's (uint bodyA, uint bodyB) keys, then
  * CreateBatchesNew's greedy batching.  Arrays live in LDS when the cell fits
  * (ldsCap entries), otherwise in global scratch (sortKeys, 2 entries per
@@ -172,29 +310,10 @@ __global__ __launch_bounds__(256) void k_batch_cells(const int* __restrict__ cel
 		}
 	}
 	__syncthreads();
-	/* bitonic sort ascending */
-	for (int k = 2; k <= np2; k <<= 1)
-		for (int j = k >> 1; j > 0; j >>= 1)
-		{
-			for (int i = t; i < np2; i += blockDim.x)
-			{
-				int ixj = i ^ j;
-				if (ixj > i)
-				{
-					unsigned long long ki = key[i], kj = key[ixj];
-					bool up = (i & k) == 0;
-					if ((ki > kj) == up)
-					{
-						key[i] = kj;
-						key[ixj] = ki;
-						unsigned long long p = pay[i];
-						pay[i] = pay[ixj];
-						pay[ixj] = p;
-					}
-				}
-			}
-			__syncthreads();
-		}
+	if (inLds)
+		bitonic_sort(LDS_U64(key), LDS_U64(pay), np2, t);
+	else
+		bitonic_sort(key, pay, np2, t);
 	/* greedy batching, wave-parallel with the sequential semantics of
 	 * CreateBatchesNew: the 2048-bit bitmap is held one 32-bit word per lane;
 	 * each chunk of 64 candidates is pre-checked against the bitmap at chunk
@@ -208,101 +327,12 @@ __global__ __launch_bounds__(256) void k_batch_cells(const int* __restrict__ cel
 	 * can be read up front.  Batch ids go into the high half of pay[]. */
 	if (t < 64)
 	{
-		const int lane = t;
-		int numValid = 0, batch = 0;
-		while (numValid < n)
-		{
-			int nCur = 0;
-			unsigned word = 0u; /* bitmap word `lane` */
-			for (int c0 = numValid; c0 < n; c0 += 64)
-			{
-				int i = c0 + lane;
-				bool valid = i < n;
-				unsigned long long kk = valid ? key[i] : 0ull;
-				int aS = (int)(unsigned)(kk >> 32), bS = (int)(unsigned)(kk & 0xffffffffu);
-				int a = iabs(aS), b = iabs(bS);
-				bool aStatic = (aS < 0) || aS == staticIdx;
-				bool bStatic = (bS < 0) || bS == staticIdx;
-				int ia = a % (32 * CHECK_SIZE), ib = b % (32 * CHECK_SIZE);
-				int wa = ia >> 5, wb = ib >> 5;
-				unsigned ma = aStatic ? 0u : (1u << (ia & 31));
-				unsigned mb = bStatic ? 0u : (1u << (ib & 31));
-				unsigned wA = __shfl(word, wa), wB = __shfl(word, wb);
-				bool pre = valid && (wA & ma) == 0u && (wB & mb) == 0u;
-				unsigned long long cand = __ballot(pre);
-				unsigned long long acc = 0ull;
-				const unsigned long long validMask = __ballot(valid);
-				while (cand)
-				{
-					int j = __ffsll((long long)cand) - 1;
-					acc |= 1ull << j;
-					int jwa = __shfl(wa, j), jwb = __shfl(wb, j);
-					unsigned jma = __shfl(ma, j), jmb = __shfl(mb, j);
-					if (lane == jwa) word |= jma;
-					if (lane == jwb) word |= jmb;
-					nCur++;
-					unsigned long long later = (j == 63) ? 0ull : (~0ull << (j + 1));
-					if (nCur == SIMD_WIDTH)
-					{
-						nCur = 0;
-						word = 0u;
-						cand = validMask & later;
-					}
-					else
-					{
-						bool hit = (wa == jwa && (ma & jma)) || (wa == jwb && (ma & jmb)) || (wb == jwa && (mb & jma)) ||
-								   (wb == jwb && (mb & jmb));
-						cand &= later & ~__ballot(hit);
-					}
-				}
-				/* move the accepted contacts to the front, in order.  When at least
-				 * as many rejected contacts lie before the chunk as the chunk
-				 * accepts, the swaps (accepted m <-> numValid + m) are disjoint and
-				 * run in parallel; otherwise one lane performs them in sequence. */
-				int kAcc = __popcll(acc);
-				if (c0 - numValid >= kAcc)
-				{
-					if ((acc >> lane) & 1ull)
-					{
-						int m = __popcll(acc & ((1ull << lane) - 1ull));
-						int dst = numValid + m;
-						int ii = c0 + lane;
-						unsigned long long pj = pay[ii];
-						unsigned long long kd = key[dst], pd = pay[dst];
-						key[ii] = kd;
-						pay[ii] = pd;
-						key[dst] = kk;
-						pay[dst] = (pj & 0xffffffffull) | ((unsigned long long)(unsigned)batch << 32);
-					}
-					numValid += kAcc;
-				}
-				else if (lane == 0)
-				{
-					unsigned long long m = acc;
-					while (m)
-					{
-						int j = __ffsll((long long)m) - 1;
-						m &= m - 1;
-						int ii = c0 + j;
-						unsigned long long kj = key[ii];
-						unsigned long long pj = pay[ii];
-						unsigned long long pi = (pj & 0xffffffffull) | ((unsigned long long)(unsigned)batch << 32);
-						if (ii != numValid)
-						{
-							key[ii] = key[numValid];
-							pay[ii] = pay[numValid];
-							key[numValid] = kj;
-						}
-						pay[numValid] = pi;
-						numValid++;
-					}
-				}
-				numValid = __shfl(numValid, 0);
-				wave_sync_solver();
-			}
-			batch++;
-		}
-		if (lane == 0) batchSizes[cell] = batch;
+		int batch;
+		if (inLds)
+			batch = batch_greedy(LDS_U64(key), LDS_U64(pay), n, staticIdx);
+		else
+			batch = batch_greedy(key, pay, n, staticIdx);
+		if (t == 0) batchSizes[cell] = batch;
 	}
 	__syncthreads();
 	for (int i = t; i < n; i += blockDim.x)
