@@ -116,6 +116,8 @@ _SIGS = {
     "b3h_pipeline_clear_warm_start_history": ([_vp], _i),
     "b3h_pipeline_save_state": ([_vp, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)], _i),
     "b3h_pipeline_load_state": ([_vp, _vp, ctypes.c_size_t], _i),
+    "b3h_pipeline_set_scenes": ([_vp, _i, _vp], _i),
+    "b3h_pipeline_get_scene_contact_counts": ([_vp, _vp, _i], _i),
     "b3h_pipeline_get_warm_start_debug": ([_vp, _pvp, _pvp, _pvp], _i),
     "b3h_pipeline_set_contact_log": ([_vp, _vp, _vp, _i], _i),
     "b3h_pipeline_get_contact_log_count": ([_vp, _pi], _i),

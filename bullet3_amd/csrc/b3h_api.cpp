@@ -155,6 +155,7 @@ struct b3hNarrowPhase
 	int* dSepState;
 	b3hFloat4* dSepNormal;
 	b3hFloat4* dMpr; /* per pair: MPR direction, point, (depth, hit) */
+	float* dSphereProj; /* per body: (min, max) on each of the 162 SAT directions */
 	bool contactsValid;
 	int lanesPerPair;
 };
@@ -176,6 +177,7 @@ static int np_alloc_work(b3hNarrowPhase* np)
 	if ((r = dalloc(&np->dSepState, mp))) return r;
 	if ((r = dalloc(&np->dSepNormal, mp))) return r;
 	if ((r = dalloc(&np->dMpr, 3 * (size_t)mp))) return r;
+	if ((r = dalloc(&np->dSphereProj, 2 * 162 * (size_t)np->cfg.m_maxConvexBodies))) return r;
 	B3H_CHECK(hipMemset(np->dNumContacts, 0, 2 * sizeof(int)));
 	return B3H_OK;
 }
@@ -207,6 +209,7 @@ extern "C" int b3h_np_create(const b3hConfig* cfg, void* stream, b3hNarrowPhase*
 	np->dSepState = nullptr;
 	np->dSepNormal = nullptr;
 	np->dMpr = nullptr;
+	np->dSphereProj = nullptr;
 	np->contactsValid = true;
 	np->lanesPerPair = 64;
 	if ((r = make_stream(stream, &np->stream, &np->ownStream)) || (r = np_alloc_work(np)) || (r = b3h_np_module_init()))
@@ -242,6 +245,7 @@ extern "C" int b3h_np_destroy(b3hNarrowPhase* np)
 	dfree(np->dSepState);
 	dfree(np->dSepNormal);
 	dfree(np->dMpr);
+	dfree(np->dSphereProj);
 	if (np->ownStream) hipStreamDestroy(np->stream);
 	delete np;
 	return B3H_OK;
@@ -527,7 +531,8 @@ static void np_compute_contacts_dev(b3hNarrowPhase* np, b3hInt4* pairs, const in
 {
 	b3h_narrowphase(pairs, dNumPairs, maxPairs, np->dBodies, np_tables(np), np->dStaging, np->dFlags, np->dFlagScan,
 					np->dScanTmp, np->dContactPair, np->cfg.m_maxContactCapacity, np->dNumContacts, np->dSepState,
-					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr, evAfterMpr);
+					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr, evAfterMpr,
+					(int)np->bodies.size(), np->dSphereProj);
 	np->contactsValid = false;
 }
 
@@ -959,6 +964,11 @@ struct b3hPipeline
 	/* warm start and contact logging */
 	b3hWarmBuffers wb;
 	bool warmAlloc, warmFeatAlloc;
+	/* batched scenes: scene of every body, per-scene contact counts */
+	int numScenes;
+	int sceneBodies;
+	int* dSceneOf;
+	int* dSceneCounts;
 	b3hMlp* warmNet;
 	bool logRows;
 	float matchTol;
@@ -990,6 +1000,8 @@ static void pl_free(b3hPipeline* p)
 	dfree(p->sb.levelStart);
 	dfree(p->sb.partMeta);
 	dfree(p->sb.partBodies);
+	dfree(p->dSceneOf);
+	dfree(p->dSceneCounts);
 	dfree(p->sb.cellLevels);
 	dfree(p->sb.lastClass);
 	dfree(p->sb.residSnap);
@@ -1176,6 +1188,7 @@ static int pl_step(b3hPipeline* p, float dt, bool events)
 	if (events) hipEventRecord(p->ev[0], s);
 	pl_aabbs(p);
 	if (events) hipEventRecord(p->ev[1], s);
+	p->bp->sc.sceneOf = p->numScenes > 0 ? p->dSceneOf : nullptr;
 	int r = bp_calc_pairs(p->bp, p->cfg.m_maxBroadphasePairs, s);
 	if (r) return r;
 	if (events) hipEventRecord(p->ev[2], s);
@@ -1194,10 +1207,76 @@ static int pl_step(b3hPipeline* p, float dt, bool events)
 	return B3H_OK;
 }
 
+static int check_scenes(b3hPipeline* p)
+{
+	if (p->numScenes > 0 && p->sceneBodies != num_bodies(p))
+	{
+		b3h_set_error("stepSimulation: scenes cover %d bodies, the pipeline has %d (set the scenes again)",
+					  p->sceneBodies, num_bodies(p));
+		return B3H_ERR_STATE;
+	}
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_set_scenes(b3hPipeline* p, int numScenes, const int* bodyOffsets)
+{
+	CHECK_ARG(p && numScenes >= 0 && (numScenes == 0 || bodyOffsets));
+	int nb = num_bodies(p);
+	B3H_CHECK(hipStreamSynchronize(p->stream));
+	if (numScenes == 0)
+	{
+		p->numScenes = 0;
+		p->bp->sc.sceneOf = nullptr;
+		drop_graph(p);
+		return B3H_OK;
+	}
+	if (bodyOffsets[0] != 0 || bodyOffsets[numScenes] != nb)
+	{
+		b3h_set_error("setScenes: offsets must run from 0 to the body count %d (got %d .. %d)", nb, bodyOffsets[0],
+					  bodyOffsets[numScenes]);
+		return B3H_ERR_INVALID;
+	}
+	std::vector<int> sceneOf((size_t)nb);
+	for (int k = 0; k < numScenes; k++)
+	{
+		if (bodyOffsets[k + 1] < bodyOffsets[k])
+		{
+			b3h_set_error("setScenes: offsets decrease at scene %d", k);
+			return B3H_ERR_INVALID;
+		}
+		for (int b = bodyOffsets[k]; b < bodyOffsets[k + 1]; b++) sceneOf[(size_t)b] = k;
+	}
+	dfree(p->dSceneOf);
+	dfree(p->dSceneCounts);
+	p->dSceneOf = nullptr;
+	p->dSceneCounts = nullptr;
+	int r;
+	if ((r = dalloc(&p->dSceneOf, (size_t)(nb > 0 ? nb : 1))) || (r = dalloc(&p->dSceneCounts, (size_t)numScenes)))
+		return r;
+	if (nb > 0) B3H_CHECK(hipMemcpy(p->dSceneOf, sceneOf.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+	p->numScenes = numScenes;
+	p->sceneBodies = nb;
+	p->bp->sc.sceneOf = p->dSceneOf;
+	drop_graph(p);
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_get_scene_contact_counts(b3hPipeline* p, int* counts, int numScenes)
+{
+	CHECK_ARG(p && counts && numScenes == p->numScenes && numScenes > 0);
+	b3h_scene_contact_counts(p->bp->dPairCount, p->cfg.m_maxBroadphasePairs, p->bp->dPairs, p->dSceneOf,
+							 p->dSceneCounts, numScenes, p->stream);
+	B3H_CHECK(hipGetLastError());
+	B3H_CHECK(hipMemcpyAsync(counts, p->dSceneCounts, sizeof(int) * numScenes, hipMemcpyDeviceToHost, p->stream));
+	B3H_CHECK(hipStreamSynchronize(p->stream));
+	return B3H_OK;
+}
+
 extern "C" int b3h_pipeline_step_simulation(b3hPipeline* p, float dt)
 {
 	CHECK_ARG(p);
 	if (num_bodies(p) == 0) return B3H_OK;
+	if (int rs = check_scenes(p)) return rs;
 	if (p->bp->all.size() != p->np->bodies.size())
 	{
 		b3h_set_error("stepSimulation: %zu proxies for %zu bodies (register bodies through registerPhysicsInstance)",
@@ -1219,6 +1298,7 @@ extern "C" int b3h_pipeline_step_graph(b3hPipeline* p, float dt, int count)
 {
 	CHECK_ARG(p && count >= 0);
 	if (num_bodies(p) == 0) return B3H_OK;
+	if (int rs = check_scenes(p)) return rs;
 	if (!p->graphExec || p->graphDt != dt || p->graphBodies != num_bodies(p))
 	{
 		if (p->graphExec) hipGraphExecDestroy(p->graphExec);

@@ -60,6 +60,7 @@ struct b3hBroadphaseScratch
 	unsigned *lkeys, *lvals, *ltmpKeys, *ltmpVals, *lhist;
 	unsigned* lmaxExt; /* float bits of the largest x-extent */
 	float* lminx;      /* min.x in sorted order */
+	const int* sceneOf; /* per proxy handle: its scene (NULL: one scene); pairs across scenes are dropped */
 };
 #define B3H_LARGE_BRUTE_MAX 32
 void b3h_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
@@ -69,6 +70,8 @@ void b3h_sap_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const in
 				   b3hInt4* pairs, int maxPairs, int* pairCount, int* pairTotal, b3hBroadphaseScratch& sc,
 				   hipStream_t s);
 int b3h_sap_num_partials(int nSmall);
+void b3h_scene_contact_counts(const int* nPairs, int maxPairs, const b3hInt4* pairs, const int* sceneOf, int* counts,
+							  int numScenes, hipStream_t s);
 
 /* ---------------- narrowphase (k_narrowphase.hip) ---------------- */
 struct b3hShapeTables
@@ -84,7 +87,7 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 					 const b3hShapeTables& sh, b3hContact4Data* staging, int* flags, int* flagScan,
 					 unsigned* scanTmp, int* contactPair, int maxContacts, int* numContacts, int* sepState,
 					 b3hFloat4* sepNormal, int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel,
-					 b3hFloat4* mprScratch, hipEvent_t evAfterMpr = nullptr);
+					 b3hFloat4* mprScratch, hipEvent_t evAfterMpr, int numBodies, float* sphereProj);
 void b3h_gather_contacts(const int* contactPair, const int* numContacts, int maxContacts,
 						 const b3hContact4Data* staging, b3hContact4Data* out, hipStream_t s);
 

@@ -26,13 +26,19 @@ __device__ __forceinline__ bool bp_overlap(const b3hAabb& a, const b3hAabb& b)
 		   (b.m_min.y <= a.m_max.y) && (a.m_min.z <= b.m_max.z) && (b.m_min.z <= a.m_max.z);
 }
 __device__ __forceinline__ int bp_handle(const b3hAabb& a) { return f2i(a.m_min.w); }
+/* batched scenes (b3h_pipeline_set_scenes): proxies of different scenes never pair */
+__device__ __forceinline__ bool bp_same_scene(const int* sceneOf, int ha, int hb)
+{
+	return !sceneOf || sceneOf[ha] == sceneOf[hb];
+}
 
 /* ---------------- large x small ---------------- */
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_large_small(int nSmall, int nLarge, const b3hAabb* __restrict__ all,
 													 const int* __restrict__ smallMap, const int* __restrict__ largeMap,
 													 int* __restrict__ counts, const int* __restrict__ offsets,
-													 b3hInt4* __restrict__ pairs, int maxPairs)
+													 b3hInt4* __restrict__ pairs, int maxPairs,
+													 const int* __restrict__ sceneOf)
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= nSmall) return;
@@ -43,7 +49,7 @@ __global__ __launch_bounds__(256) void k_large_small(int nSmall, int nLarge, con
 	for (int j = 0; j < nLarge; j++)
 	{
 		b3hAabb l = all[largeMap[j]];
-		if (bp_overlap(l, s))
+		if (bp_overlap(l, s) && bp_same_scene(sceneOf, bp_handle(l), hs))
 		{
 			if (WRITE)
 			{
@@ -99,7 +105,8 @@ __global__ __launch_bounds__(256) void k_large_small_sorted(int nSmall, int nLar
 															const float* __restrict__ lminx,
 															const unsigned* __restrict__ maxExt,
 															int* __restrict__ counts, const int* __restrict__ offsets,
-															b3hInt4* __restrict__ pairs, int maxPairs)
+															b3hInt4* __restrict__ pairs, int maxPairs,
+															const int* __restrict__ sceneOf)
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= nSmall) return;
@@ -124,7 +131,7 @@ __global__ __launch_bounds__(256) void k_large_small_sorted(int nSmall, int nLar
 	{
 		int j = (int)lvals[k];
 		b3hAabb l = all[largeMap[j]];
-		if (bp_overlap(l, s))
+		if (bp_overlap(l, s) && bp_same_scene(sceneOf, bp_handle(l), hs))
 		{
 			if (nf < LS_LOCAL)
 				found[nf++] = j;
@@ -170,7 +177,7 @@ __global__ __launch_bounds__(256) void k_large_small_sorted(int nSmall, int nLar
 		for (int j = 0; j < nLarge; j++)
 		{
 			b3hAabb l = all[largeMap[j]];
-			if (bp_overlap(l, s))
+			if (bp_overlap(l, s) && bp_same_scene(sceneOf, bp_handle(l), hs))
 			{
 				if (WRITE)
 				{
@@ -255,7 +262,8 @@ __global__ __launch_bounds__(256) void k_grid_pairs(int n, const unsigned* __res
 													const b3hAabb* __restrict__ sorted,
 													const int* __restrict__ cellStart, b3hGridParams gp,
 													int* __restrict__ counts, const int* __restrict__ offsets,
-													b3hInt4* __restrict__ pairs, int maxPairs)
+													b3hInt4* __restrict__ pairs, int maxPairs,
+													const int* __restrict__ sceneOf)
 {
 	int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= n) return;
@@ -281,7 +289,7 @@ __global__ __launch_bounds__(256) void k_grid_pairs(int n, const unsigned* __res
 					b3hAabb b = sorted[k];
 					if (!bp_overlap(a, b)) continue;
 					int hb = bp_handle(b);
-					if (ha < hb)
+					if (ha < hb && bp_same_scene(sceneOf, ha, hb))
 					{
 						if (WRITE)
 						{
@@ -310,11 +318,11 @@ static void large_small_pass(int nSmall, int nLarge, const b3hAabb* all, const i
 {
 	if (nLarge <= B3H_LARGE_BRUTE_MAX)
 		hipLaunchKernelGGL(k_large_small<WRITE>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, nLarge, all, smallMap,
-						   largeMap, counts, offsets, pairs, maxPairs);
+						   largeMap, counts, offsets, pairs, maxPairs, sc.sceneOf);
 	else
 		hipLaunchKernelGGL(k_large_small_sorted<WRITE>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, nLarge, all,
 						   smallMap, largeMap, (const unsigned*)sc.lvals, (const float*)sc.lminx,
-						   (const unsigned*)sc.lmaxExt, counts, offsets, pairs, maxPairs);
+						   (const unsigned*)sc.lmaxExt, counts, offsets, pairs, maxPairs, sc.sceneOf);
 }
 
 void b3h_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
@@ -338,7 +346,7 @@ void b3h_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const i
 						   sc.sortedAabbs);
 		hipLaunchKernelGGL(k_grid_pairs<false>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys,
 						   (const b3hAabb*)sc.sortedAabbs, sc.cellStart, gp, countsG, (const int*)nullptr,
-						   (b3hInt4*)nullptr, maxPairs);
+						   (b3hInt4*)nullptr, maxPairs, sc.sceneOf);
 	}
 	int nTot = 2 * nSmall;
 	b3h_exclusive_scan(sc.counts, sc.offsets, nullptr, nTot, sc.scanTmp, s);
@@ -348,7 +356,7 @@ void b3h_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const i
 							   maxPairs, s);
 		hipLaunchKernelGGL(k_grid_pairs<true>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys,
 						   (const b3hAabb*)sc.sortedAabbs, sc.cellStart, gp, (int*)nullptr, sc.offsets + nSmall, pairs,
-						   maxPairs);
+						   maxPairs, sc.sceneOf);
 	}
 	hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, sc.offsets + nTot, maxPairs, pairCount, pairTotal);
 }
@@ -431,7 +439,8 @@ __global__ __launch_bounds__(256) void k_sap_scatter(int n, const b3hAabb* __res
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_sap_sweep(int n, const b3hAabb* __restrict__ sorted, const int* __restrict__ axisPtr,
 												   int* __restrict__ counts, const int* __restrict__ offsets,
-												   b3hInt4* __restrict__ pairs, int maxPairs)
+												   b3hInt4* __restrict__ pairs, int maxPairs,
+												   const int* __restrict__ sceneOf)
 {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
@@ -445,7 +454,7 @@ __global__ __launch_bounds__(256) void k_sap_sweep(int n, const b3hAabb* __restr
 	{
 		b3hAabb b = sorted[j];
 		if (amax < axis_val(b.m_min, axis)) break;
-		if (bp_overlap(a, b))
+		if (bp_overlap(a, b) && bp_same_scene(sceneOf, ha, bp_handle(b)))
 		{
 			if (WRITE)
 			{
@@ -475,7 +484,7 @@ void b3h_sap_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const in
 		b3h_radix_sort(sc.keys, sc.vals, sc.tmpKeys, sc.tmpVals, sc.hist, nSmall, 32, s);
 		hipLaunchKernelGGL(k_sap_scatter, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, all, smallMap, sc.vals, sc.sortedAabbs);
 		hipLaunchKernelGGL(k_sap_sweep<false>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.sortedAabbs, sc.axis, countsS,
-						   (const int*)nullptr, (b3hInt4*)nullptr, maxPairs);
+						   (const int*)nullptr, (b3hInt4*)nullptr, maxPairs, sc.sceneOf);
 	}
 	int nTot = 2 * nSmall;
 	b3h_exclusive_scan(sc.counts, sc.offsets, nullptr, nTot, sc.scanTmp, s);
@@ -484,7 +493,29 @@ void b3h_sap_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const in
 		large_small_pass<true>(nSmall, nLarge, all, smallMap, largeMap, sc, (int*)nullptr, sc.offsets, pairs,
 							   maxPairs, s);
 		hipLaunchKernelGGL(k_sap_sweep<true>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.sortedAabbs, sc.axis,
-						   (int*)nullptr, sc.offsets + nSmall, pairs, maxPairs);
+						   (int*)nullptr, sc.offsets + nSmall, pairs, maxPairs, sc.sceneOf);
 	}
 	hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, sc.offsets + nTot, maxPairs, pairCount, pairTotal);
+}
+
+/* contact manifolds per scene of the last step: pairs with a contact slot,
+ * counted under the scene of their first body */
+__global__ __launch_bounds__(256) void k_scene_counts(const int* __restrict__ nPairs, const b3hInt4* __restrict__ pairs,
+													  const int* __restrict__ sceneOf, int* __restrict__ counts)
+{
+	int n = *nPairs;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		b3hInt4 pr = pairs[i];
+		if (pr.z >= 0) atomicAdd(&counts[sceneOf[pr.x]], 1);
+	}
+}
+void b3h_scene_contact_counts(const int* nPairs, int maxPairs, const b3hInt4* pairs, const int* sceneOf, int* counts,
+							  int numScenes, hipStream_t s)
+{
+	b3h_fill_i32(counts, 0, numScenes, s);
+	int b = (maxPairs + 255) / 256;
+	if (b > 4096) b = 4096;
+	if (b < 1) b = 1;
+	hipLaunchKernelGGL(k_scene_counts, dim3(b), dim3(256), 0, s, nPairs, pairs, sceneOf, counts);
 }

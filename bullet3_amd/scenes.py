@@ -156,7 +156,8 @@ def batched_scenes(n_scenes=4096, min_bodies=16, max_bodies=512, spacing=24.0, n
     [min_bodies, max_bodies]; even scenes are piles (boxes and 8-24-point hulls,
     random orientations, lattice spacing 1.2 m), odd scenes are box stacks
     (columns, spacing 1.02 m).  Returns one Scene; meta["scene_of_body"] maps
-    every body to its scene."""
+    every body to its scene, meta["scene_offsets"] gives each scene's body range
+    (its ground first, then its bodies)."""
     rng = np.random.default_rng(seed)
     shapes = [box_points(8.0, 0.5, 8.0), box_points(0.5, 0.5, 0.5)]
     for k in range(n_hull_shapes):
@@ -195,4 +196,6 @@ def batched_scenes(n_scenes=4096, min_bodies=16, max_bodies=512, spacing=24.0, n
     return Scene(name="batched_%d" % n_scenes, shapes=shapes, mass=np.array(mass, np.float32),
                  pos=np.array(pos, np.float32), orn=np.array(orn, np.float32), shape=np.array(shape, np.int32),
                  cell_size=2.0 * math.sqrt(3.0), meta={"scene_of_body": np.array(owner, np.int32),
-                                                       "bodies_per_scene": counts})
+                                                       "bodies_per_scene": counts,
+                                                       "scene_offsets": np.concatenate(
+                                                           [[0], np.cumsum(counts + 1)]).astype(np.int32)})

@@ -201,6 +201,17 @@ class World:
     def clear_warm_history(self):
         call("b3h_pipeline_clear_warm_start_history", self.pl)
 
+    # -- batched scenes (body ranges; pairs across scenes are dropped)
+    def set_scenes(self, offsets):
+        off = np.ascontiguousarray(offsets, np.int32)
+        self.num_scenes = len(off) - 1 if len(off) else 0
+        call("b3h_pipeline_set_scenes", self.pl, self.num_scenes, ptr(off) if self.num_scenes else None)
+
+    def scene_contact_counts(self):
+        out = np.zeros(self.num_scenes, np.int32)
+        call("b3h_pipeline_get_scene_contact_counts", self.pl, ptr(out), self.num_scenes)
+        return out
+
     # -- state image (bodies, inertias, gravity, solver parameters, warm-start history)
     def save_state(self):
         n = ctypes.c_size_t()

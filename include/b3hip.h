@@ -252,6 +252,18 @@ int b3h_pipeline_clear_warm_start_history(b3hPipeline* p);
  * (b3GpuNarrowPhase.h:45-47), which move the bodies only. */
 int b3h_pipeline_save_state(b3hPipeline* p, void* host, size_t capacityBytes, size_t* bytesOut);
 int b3h_pipeline_load_state(b3hPipeline* p, const void* host, size_t bytes);
+/* Batched scenes: many independent scenes in one pipeline, stepped by the same
+ * launches.  Scene k owns the bodies [bodyOffsets[k], bodyOffsets[k+1]) (host
+ * array of numScenes + 1 ints from 0 to the body count); the broadphase then
+ * drops every pair whose bodies belong to different scenes, so scenes never
+ * interact even where they overlap in space (keep them apart anyway: the grid's
+ * cells fill with every scene's bodies).  numScenes 0 returns to one world.
+ * Registering bodies afterwards requires setting the scenes again (the step
+ * reports B3H_ERR_STATE otherwise).  No counterpart in the reference, whose
+ * batched demos place scenes apart in one world. */
+int b3h_pipeline_set_scenes(b3hPipeline* p, int numScenes, const int* bodyOffsets);
+/* Contact manifolds of the last step per scene (host int[numScenes]) -- syncs. */
+int b3h_pipeline_get_scene_contact_counts(b3hPipeline* p, int* counts, int numScenes);
 /* Contact logging for training the network: after every solve, each contact
  * point appends its 24 features (as computed before the solve) to
  * featuresDev[rows][24] and its 3 targets (the solve's final impulses relative

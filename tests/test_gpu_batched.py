@@ -37,3 +37,58 @@ def test_batched_scenes_bit_exact(kind):
     touched = set(np.concatenate([a, b]).tolist())
     assert sum(g in touched for g in grounds) > 0.8 * len(grounds)
     w.close()
+
+
+def _pairs_now(w):
+    import ctypes
+    from bullet3_amd import capi
+    capi.call("b3h_pipeline_setup_gpu_aabbs_full", w.pl)
+    capi.call("b3h_bp_calculate_overlapping_pairs", w.bp, w.cfg.m_maxBroadphasePairs)
+    return w.pairs()
+
+
+@pytest.mark.parametrize("kind", ["grid", "sap"])
+def test_scene_ranges_drop_cross_scene_pairs(kind):
+    """scenes 6 m apart on 16 m grounds overlap their neighbours: with the
+    scene ranges set, the pair list is the unfiltered list with every pair
+    across scenes taken out, in the same order"""
+    sc = scenes.batched_scenes(12, min_bodies=16, max_bodies=64, spacing=6.0, seed=3)
+    owner = sc.meta["scene_of_body"]
+    off = sc.meta["scene_offsets"]
+    assert off[-1] == sc.num_bodies and np.all(owner[off[:-1]] == np.arange(12))
+    w = World(sc, broadphase=kind)
+    for _ in range(3):
+        w.step()
+    full = _pairs_now(w)
+    cross = owner[full[:, 0]] != owner[full[:, 1]]
+    assert cross.sum() > 0
+    w.set_scenes(off)
+    kept = _pairs_now(w)
+    assert np.array_equal(kept[:, :2], full[~cross][:, :2])
+    # stepping with the ranges: no contact between scenes; per-scene counts
+    for _ in range(5):
+        w.step()
+    p = w.pairs()
+    assert np.all(owner[p[:, 0]] == owner[p[:, 1]])
+    counts = w.scene_contact_counts()
+    with_contact = p[p[:, 2] >= 0]
+    assert np.array_equal(counts, np.bincount(owner[with_contact[:, 0]], minlength=12))
+    assert counts.sum() == len(w.contacts()) > 0
+    # back to one world
+    w.set_scenes([])
+    assert np.any(owner[_pairs_now(w)[:, 0]] != owner[_pairs_now(w)[:, 1]])
+    w.close()
+
+
+def test_scene_ranges_are_checked():
+    from bullet3_amd import capi
+    sc = scenes.batched_scenes(4, min_bodies=16, max_bodies=32, seed=5)
+    w = World(sc)
+    lib = capi.lib()
+    bad = np.array([0, 5, 3, sc.num_bodies], np.int32)              # decreasing
+    assert lib.b3h_pipeline_set_scenes(w.pl, 3, capi.ptr(bad)) == -1
+    short = np.array([0, 10, sc.num_bodies - 1], np.int32)          # does not cover every body
+    assert lib.b3h_pipeline_set_scenes(w.pl, 2, capi.ptr(short)) == -1
+    w.set_scenes(sc.meta["scene_offsets"])
+    w.step()
+    w.close()
