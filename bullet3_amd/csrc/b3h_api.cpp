@@ -155,7 +155,6 @@ struct b3hNarrowPhase
 	int* dSepState;
 	b3hFloat4* dSepNormal;
 	b3hFloat4* dMpr; /* per pair: MPR direction, point, (depth, hit) */
-	float* dSphereProj; /* per body: (min, max) on each of the 162 SAT directions */
 	bool contactsValid;
 	int lanesPerPair;
 };
@@ -177,7 +176,6 @@ static int np_alloc_work(b3hNarrowPhase* np)
 	if ((r = dalloc(&np->dSepState, mp))) return r;
 	if ((r = dalloc(&np->dSepNormal, mp))) return r;
 	if ((r = dalloc(&np->dMpr, 3 * (size_t)mp))) return r;
-	if ((r = dalloc(&np->dSphereProj, 2 * 162 * (size_t)np->cfg.m_maxConvexBodies))) return r;
 	B3H_CHECK(hipMemset(np->dNumContacts, 0, 2 * sizeof(int)));
 	return B3H_OK;
 }
@@ -209,7 +207,6 @@ extern "C" int b3h_np_create(const b3hConfig* cfg, void* stream, b3hNarrowPhase*
 	np->dSepState = nullptr;
 	np->dSepNormal = nullptr;
 	np->dMpr = nullptr;
-	np->dSphereProj = nullptr;
 	np->contactsValid = true;
 	np->lanesPerPair = 64;
 	if ((r = make_stream(stream, &np->stream, &np->ownStream)) || (r = np_alloc_work(np)) || (r = b3h_np_module_init()))
@@ -245,7 +242,6 @@ extern "C" int b3h_np_destroy(b3hNarrowPhase* np)
 	dfree(np->dSepState);
 	dfree(np->dSepNormal);
 	dfree(np->dMpr);
-	dfree(np->dSphereProj);
 	if (np->ownStream) hipStreamDestroy(np->stream);
 	delete np;
 	return B3H_OK;
@@ -531,8 +527,7 @@ static void np_compute_contacts_dev(b3hNarrowPhase* np, b3hInt4* pairs, const in
 {
 	b3h_narrowphase(pairs, dNumPairs, maxPairs, np->dBodies, np_tables(np), np->dStaging, np->dFlags, np->dFlagScan,
 					np->dScanTmp, np->dContactPair, np->cfg.m_maxContactCapacity, np->dNumContacts, np->dSepState,
-					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr, evAfterMpr,
-					(int)np->bodies.size(), np->dSphereProj);
+					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr, evAfterMpr);
 	np->contactsValid = false;
 }
 

@@ -554,8 +554,6 @@ struct Family
 	const F4* edgesA;        /* kind 2 */
 	const F4* edgesB;
 	int nEB;
-	const float2* projA;     /* kind 3: the bodies' intervals on the 162 directions */
-	const float2* projB;
 };
 __device__ __forceinline__ bool family_axis(const Family& F, const PairGeom& P, F4 dc, int t, F4* axis)
 {
@@ -583,23 +581,6 @@ __device__ __forceinline__ bool family_axis(const Family& F, const PairGeom& P, 
 	*axis = c;
 	return true;
 }
-/* TestSepAxis on fixed direction t (kind 3) from the bodies' precomputed
- * intervals on +u_t (k_sphere_proj, the same projection as project()).  The
- * axis is u_t or -u_t (family_axis); on -u_t the interval is the negated,
- * swapped one: rotation, dot products and sums are odd in the direction, so
- * only the signs of zero values can differ, which no comparison sees. */
-__device__ __forceinline__ bool test_sep_axis_sphere(const Family& F, F4 dc, int t, float* depth)
-{
-	F4 u = f4(c_unitSphere[t][0], c_unitSphere[t][1], c_unitSphere[t][2], 0.f);
-	bool flip = dot3(dc, u) > 0;
-	float2 a = F.projA[t], b = F.projB[t];
-	float mn0 = flip ? -a.y : a.x, mx0 = flip ? -a.x : a.y;
-	float mn1 = flip ? -b.y : b.x, mx1 = flip ? -b.x : b.y;
-	if (mx0 < mn1 || mx1 < mn0) return false;
-	float d0 = mx0 - mn1, d1 = mx1 - mn0;
-	*depth = d0 < d1 ? d0 : d1;
-	return true;
-}
 /* One SAT family over the group: false if some axis separates; otherwise the
  * sequential min-overlap update of (sep, dmin) and the final orientation flip. */
 template <int L>
@@ -622,8 +603,6 @@ __device__ bool sat_family(const Group<L>& g, const Family& F, const PairGeom& P
 				PairGeom Q = {P.vB, P.vA, P.nB, P.nA, P.posB, P.ornB, P.posA, P.ornA, P.ldsB, P.ldsA};
 				overlap = test_sep_axis<L == 64>(Q, axis, &d);
 			}
-			else if (F.kind == 3)
-				overlap = test_sep_axis_sphere(F, dc, t, &d);
 			else
 				overlap = test_sep_axis<L == 64>(P, axis, &d);
 			if (!overlap)
@@ -895,32 +874,6 @@ __global__ __launch_bounds__(256) void k_np_mpr(const b3hInt4* __restrict__ pair
 	}
 }
 
-/* Intervals of every hull body on the 162 fixed directions of the SAT's last
- * family (findSeparatingAxisUnitSphere, b3FindConcaveSatAxis.h:127): a body's
- * interval on a direction depends on the body alone, so it is projected once
- * per step here (the same project() the pairs would run, position w = 0 as the
- * pair kernel sets it) and every pair testing that family reads two intervals
- * per direction instead of projecting both hulls 162 times. */
-__global__ __launch_bounds__(256) void k_sphere_proj(int numBodies, const b3hRigidBodyData* __restrict__ bodies,
-													 b3hShapeTables sh, float2* __restrict__ proj)
-{
-	const long total = (long)numBodies * B3_UNIT_SPHERE_162_COUNT;
-	for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (long)gridDim.x * blockDim.x)
-	{
-		int b = (int)(k / B3_UNIT_SPHERE_162_COUNT), t = (int)(k - (long)b * B3_UNIT_SPHERE_162_COUNT);
-		const b3hRigidBodyData& B = bodies[b];
-		const b3hCollidable& col = sh.collidables[B.m_collidableIdx];
-		if (col.m_shapeType != B3H_SHAPE_CONVEX_HULL) continue;
-		const b3hConvexPolyhedronData& h = sh.convexes[col.m_shapeIndex];
-		F4 pos = B.m_pos;
-		pos.w = 0.f;
-		F4 dir = f4(c_unitSphere[t][0], c_unitSphere[t][1], c_unitSphere[t][2], 0.f);
-		float mn, mx;
-		project<false>(sh.vertices + h.m_vertexOffset, false, h.m_numVertices, pos, B.m_quat, dir, &mn, &mx);
-		proj[k] = make_float2(mn, mx);
-	}
-}
-
 /* ---------------- the fused kernel ---------------- */
 template <int L, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_narrowphase(const b3hInt4* __restrict__ pairs,
@@ -928,8 +881,7 @@ __global__ __launch_bounds__(THREADS) void k_narrowphase(const b3hInt4* __restri
 															 const b3hRigidBodyData* __restrict__ bodies,
 															 b3hShapeTables sh, b3hContact4Data* __restrict__ staging,
 															 int* __restrict__ flags, int* __restrict__ sepStateOut,
-															 b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn,
-															 const float2* __restrict__ sphereProj)
+															 b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn)
 {
 	constexpr int G = THREADS / L;
 	__shared__ F4 sVerts[G][2][NP_VCAP];
@@ -987,12 +939,12 @@ __global__ __launch_bounds__(THREADS) void k_narrowphase(const b3hInt4* __restri
 			F4 dc = sub4(c0, c1);
 			float dmin = FLT_MAX;
 			F4 sepN = sep;
-			Family fa = {0, ha.m_numFaces, sh.faces + ha.m_faceOffset, Q.ornA, nullptr, nullptr, 1, nullptr, nullptr};
+			Family fa = {0, ha.m_numFaces, sh.faces + ha.m_faceOffset, Q.ornA, nullptr, nullptr, 1};
 			bool sepA = sat_family(g, fa, Q, dc, &sepN, &dmin);
 			float dminSaved = FLT_MAX;
 			if (sepA)
 			{
-				Family fb = {1, hb.m_numFaces, sh.faces + hb.m_faceOffset, Q.ornB, nullptr, nullptr, 1, nullptr, nullptr};
+				Family fb = {1, hb.m_numFaces, sh.faces + hb.m_faceOffset, Q.ornB, nullptr, nullptr, 1};
 				bool sepB = sat_family(g, fb, Q, dc, &sepN, &dmin);
 				if (sepB)
 				{
@@ -1011,11 +963,9 @@ __global__ __launch_bounds__(THREADS) void k_narrowphase(const b3hInt4* __restri
 				Family fe;
 				if (nEE <= B3_UNIT_SPHERE_162_COUNT)
 					fe = {2, nEE, nullptr, Q.ornA, sh.uniqueEdges + ha.m_uniqueEdgesOffset,
-						  sh.uniqueEdges + hb.m_uniqueEdgesOffset, hb.m_numUniqueEdges, nullptr, nullptr};
+						  sh.uniqueEdges + hb.m_uniqueEdgesOffset, hb.m_numUniqueEdges};
 				else
-					fe = {3, B3_UNIT_SPHERE_162_COUNT, nullptr, Q.ornA, nullptr, nullptr, 1,
-						  sphereProj + (size_t)a * B3_UNIT_SPHERE_162_COUNT,
-						  sphereProj + (size_t)b * B3_UNIT_SPHERE_162_COUNT};
+					fe = {3, B3_UNIT_SPHERE_162_COUNT, nullptr, Q.ornA, nullptr, nullptr, 1};
 				if (!sat_family(g, fe, Q, dc, &s2, &dm))
 					state = 0;
 				else
@@ -1112,15 +1062,8 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 					 const b3hShapeTables& sh, b3hContact4Data* staging, int* flags, int* flagScan, unsigned* scanTmp,
 					 int* contactPair, int maxContacts, int* numContacts, int* sepState, b3hFloat4* sepNormal,
 					 int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel, b3hFloat4* mprScratch,
-					 hipEvent_t evAfterMpr, int numBodies, float* sphereProj)
+					 hipEvent_t evAfterMpr)
 {
-	if (numBodies > 0)
-	{
-		long total = (long)numBodies * B3_UNIT_SPHERE_162_COUNT;
-		long pb = (total + 255) / 256;
-		int blocks = pb > 16384 ? 16384 : (int)pb;
-		hipLaunchKernelGGL(k_sphere_proj, dim3(blocks), dim3(256), 0, s, numBodies, bodies, sh, (float2*)sphereProj);
-	}
 	{
 		int mb = (maxPairs + 255) / 256;
 		if (mb > 16384) mb = 16384;
@@ -1136,7 +1079,7 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 		if (blocks > 16384) blocks = 16384;
 		if (blocks < 1) blocks = 1;
 		hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, s, pairs, numPairs, bodies, sh, staging, flags,
-						   sepState, sepNormal, (const F4*)mprScratch, (const float2*)sphereProj);
+						   sepState, sepNormal, (const F4*)mprScratch);
 	};
 	if (pairsPerGroup == 16)
 		launch(k_narrowphase<16, 128>, 16, 128);
