@@ -472,7 +472,7 @@ typedef __attribute__((address_space(3))) const F4 LdsF4;
 #else
 #define LDS_PTR(p) (p)
 #endif
-/* the interval loop, four vertices per round with the loads issued first;
+/* the interval loop, two vertices per round with the loads issued first;
  * VP is an LDS or a global pointer so the loads are ds_read / global_load */
 template <typename VP>
 __device__ __forceinline__ float vdot3(VP vs, int i, F4 d)
@@ -483,11 +483,11 @@ template <typename VP>
 __device__ __forceinline__ void project_loop(VP vs, int nv, F4 d, float& lo, float& hi)
 {
 	int i = 0;
-	for (; i + 4 <= nv; i += 4)
+	for (; i + 2 <= nv; i += 2)
 	{
-		float pa = vdot3(vs, i, d), pb = vdot3(vs, i + 1, d), pc = vdot3(vs, i + 2, d), pe = vdot3(vs, i + 3, d);
-		lo = fminf(lo, fminf(fminf(pa, pb), fminf(pc, pe)));
-		hi = fmaxf(hi, fmaxf(fmaxf(pa, pb), fmaxf(pc, pe)));
+		float pa = vdot3(vs, i, d), pb = vdot3(vs, i + 1, d);
+		lo = fminf(lo, fminf(pa, pb));
+		hi = fmaxf(hi, fmaxf(pa, pb));
 	}
 	for (; i < nv; i++)
 	{
@@ -876,7 +876,10 @@ __global__ __launch_bounds__(256) void k_np_mpr(const b3hInt4* __restrict__ pair
 
 /* ---------------- the fused kernel ---------------- */
 template <int L, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_narrowphase(const b3hInt4* __restrict__ pairs,
+/* launch bounds: 1024 / THREADS blocks per CU keep the register count at 128, so
+ * four waves share each SIMD (a few values spill to scratch; measured faster
+ * than three waves without spills) */
+__global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_narrowphase(const b3hInt4* __restrict__ pairs,
 															 const int* __restrict__ numPairsPtr,
 															 const b3hRigidBodyData* __restrict__ bodies,
 															 b3hShapeTables sh, b3hContact4Data* __restrict__ staging,
