@@ -847,7 +847,8 @@ __device__ __forceinline__ bool pair_is_hull_hull(const b3hRigidBodyData& A, con
  * mpr.cl:14-90).  The portal iteration is inherently serial, so a thread per
  * pair keeps all lanes busy; the result (direction, point, depth, hit) is
  * picked up by the SAT/clipping kernel. */
-__global__ __launch_bounds__(256) void k_np_mpr(const b3hInt4* __restrict__ pairs, const int* __restrict__ numPairsPtr,
+/* four blocks of 256 per CU: 128 VGPRs, four waves per SIMD */
+__global__ __launch_bounds__(256, 4) void k_np_mpr(const b3hInt4* __restrict__ pairs, const int* __restrict__ numPairsPtr,
 												const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh,
 												F4* __restrict__ mprOut)
 {
