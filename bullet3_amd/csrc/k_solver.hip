@@ -1275,7 +1275,8 @@ __device__ __forceinline__ float rc(const F4* r, int j)
 	}
 }
 __device__ __forceinline__ F4 rc3(const F4* r, int j) { return f4(rc(r, j), rc(r, j + 1), rc(r, j + 2), 0.f); }
-template <int PHASE>
+/* TRACK: keep the largest impulse change and impulse (residual tracking) */
+template <int PHASE, bool TRACK>
 __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4* sAv, int nb, const RecSrc& R,
 										  float friction, float& maxD, float& maxL)
 {
@@ -1321,8 +1322,14 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 			upd = cl_min(upd, maxR);
 			rl = upd - prev;
 			fl[k] = upd;
-			maxD = fmaxf(maxD, fabsf(rl));
-			maxL = fmaxf(maxL, fabsf(upd));
+			if (TRACK)
+			{
+				if (TRACK)
+				{
+					maxD = fmaxf(maxD, fabsf(rl));
+					maxL = fmaxf(maxL, fabsf(upd));
+				}
+			}
 			apply_pre(lin, imA, imB, rc3(S.r, 15 * k + 9), rc3(S.r, 15 * k + 12), rl, va, vb);
 		}
 		if (fj.z != 0.f)
@@ -1359,8 +1366,11 @@ __device__ __forceinline__ void pgs_solve(const F4* __restrict__ Sr, F4* sLv, F4
 				upd = cl_min(upd, FLT_MAX);
 				rl = upd - prev;
 				lam[ic] = upd;
-				maxD = fmaxf(maxD, fabsf(rl));
-				maxL = fmaxf(maxL, fabsf(upd));
+				if (TRACK)
+				{
+					maxD = fmaxf(maxD, fabsf(rl));
+					maxL = fmaxf(maxL, fabsf(upd));
+				}
 				apply_pre(lin, imA, imB, rc3(S.r, 10 + 12 * ic), rc3(S.r, 13 + 12 * ic), rl, va, vb);
 			}
 		}
@@ -1449,7 +1459,7 @@ __device__ void pgs_warm(int p, const RecSrc& R, const int* __restrict__ levelOr
 
 /* one sweep of a part by dependency levels (this thread: positions ls[c] + t,
  * + T, ... of each level c, relative to the part's first position base) */
-template <int PHASE>
+template <int PHASE, bool TRACK>
 __device__ __forceinline__ void pgs_walk(int t, int n, int L, const int* ls, int base, const RecSrc& R, F4* sLv,
 										 F4* sAv, int nb, float friction, float& maxD, float& maxL)
 {
@@ -1475,12 +1485,12 @@ __device__ __forceinline__ void pgs_walk(int t, int n, int L, const int* ls, int
 #define PGS_LEVEL(X, hX, c)                                                                   \
 	{                                                                                         \
 		bool live = (c) < L;                                                                  \
-		if (hX) pgs_solve<PHASE>(X.r, sLv, sAv, nb, R, friction, maxD, maxL);                \
+		if (hX) pgs_solve<PHASE, TRACK>(X.r, sLv, sAv, nb, R, friction, maxD, maxL);         \
 		for (int q = live ? ls[c] + t + T : n; q < (live ? ls[(c) + 1] : 0); q += T)          \
 		{                                                                                     \
 			Slot<PHASE> W;                                                                    \
 			slot_fetch<PHASE>(W, base + q, R);                                                \
-			pgs_solve<PHASE>(W.r, sLv, sAv, nb, R, friction, maxD, maxL);                    \
+			pgs_solve<PHASE, TRACK>(W.r, sLv, sAv, nb, R, friction, maxD, maxL);             \
 		}                                                                                     \
 		int pn = (c) + 3 < L ? ls[(c) + 3] + t : n;                                          \
 		hX = (c) + 3 < L && pn < ls[(c) + 4];                                                 \
@@ -1656,7 +1666,12 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 		}
 	}
 	else
-		pgs_walk<PHASE>(t, n, L, ls, base, R, sLv, sAv, nb, friction, maxD, maxL);
+	{
+		if (resid)
+			pgs_walk<PHASE, true>(t, n, L, ls, base, R, sLv, sAv, nb, friction, maxD, maxL);
+		else
+			pgs_walk<PHASE, false>(t, n, L, ls, base, R, sLv, sAv, nb, friction, maxD, maxL);
+	}
 	__syncthreads();
 	if (!fuseIntegrate)
 		for (int q = t; q < nbp; q += T)
