@@ -676,6 +676,7 @@ static void bp_free_scratch(b3hBroadphase* bp)
 	dfree(bp->sc.sortedAabbs);
 	dfree(bp->sc.sapPartials);
 	dfree(bp->sc.axis);
+	dfree(bp->sc.stash);
 	dfree(bp->sc.lkeys);
 	dfree(bp->sc.lvals);
 	dfree(bp->sc.ltmpKeys);
@@ -829,7 +830,7 @@ static int bp_ensure_scratch(b3hBroadphase* bp)
 		(r = dalloc(&bp->sc.cellStart, bp->numCells)) || (r = dalloc(&bp->sc.counts, 2 * cap + 1)) ||
 		(r = dalloc(&bp->sc.offsets, 2 * cap + 2)) || (r = dalloc(&bp->sc.scanTmp, b3h_scan_tmp_size(2 * cap))) ||
 		(r = dalloc(&bp->sc.sortedAabbs, cap)) || (r = dalloc(&bp->sc.sapPartials, 6 * (size_t)(b3h_sap_num_partials(cap) + 1))) ||
-		(r = dalloc(&bp->sc.axis, 1)))
+		(r = dalloc(&bp->sc.axis, 1)) || (r = dalloc(&bp->sc.stash, (size_t)cap * B3H_GRID_STASH)))
 		return r;
 	bp->scCapSmall = ns;
 	bp->scCells = bp->numCells;

@@ -257,22 +257,21 @@ __global__ __launch_bounds__(256) void k_grid_gather(int n, const unsigned* __re
 	sorted[s] = all[smallMap[vals[s]]];
 }
 
-template <bool WRITE>
-__global__ __launch_bounds__(256) void k_grid_pairs(int n, const unsigned* __restrict__ keys,
-													const b3hAabb* __restrict__ sorted,
-													const int* __restrict__ cellStart, b3hGridParams gp,
-													int* __restrict__ counts, const int* __restrict__ offsets,
-													b3hInt4* __restrict__ pairs, int maxPairs,
-													const int* __restrict__ sceneOf)
+/* the neighbour scan of one small proxy (gridBroadphase.cl:3-211): MODE 0
+ * counts its pairs and keeps the first B3H_GRID_STASH in its stash slots;
+ * MODE 1 writes all of them at its offset (used for proxies with more pairs
+ * than the stash holds).  Pairs (ha < hb) come in neighbour-cell, bucket order. */
+template <int MODE>
+__device__ __forceinline__ int grid_scan_one(int s, int n, const unsigned* __restrict__ keys,
+											 const b3hAabb* __restrict__ sorted, const int* __restrict__ cellStart,
+											 const b3hGridParams& gp, int2* __restrict__ stash, int o,
+											 b3hInt4* __restrict__ pairs, int maxPairs, const int* __restrict__ sceneOf)
 {
-	int s = blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= n) return;
 	b3hAabb a = sorted[s];
 	int ha = bp_handle(a);
 	int g[3];
 	grid_pos(a, gp, g);
 	int c = 0;
-	int o = WRITE ? offsets[s] : 0;
 	for (int z = -1; z <= 1; z++)
 		for (int y = -1; y <= 1; y++)
 			for (int x = -1; x <= 1; x++)
@@ -291,7 +290,11 @@ __global__ __launch_bounds__(256) void k_grid_pairs(int n, const unsigned* __res
 					int hb = bp_handle(b);
 					if (ha < hb && bp_same_scene(sceneOf, ha, hb))
 					{
-						if (WRITE)
+						if (MODE == 0)
+						{
+							if (c < B3H_GRID_STASH) stash[(size_t)s * B3H_GRID_STASH + c] = make_int2(ha, hb);
+						}
+						else
 						{
 							int at = o + c;
 							if (at < maxPairs) pairs[at] = b3hInt4{ha, hb, -1, -1};
@@ -300,7 +303,42 @@ __global__ __launch_bounds__(256) void k_grid_pairs(int n, const unsigned* __res
 					}
 				}
 			}
-	if (!WRITE) counts[s] = c;
+	return c;
+}
+/* one pass finds every pair and stashes the first few per proxy ... */
+__global__ __launch_bounds__(256) void k_grid_pairs(int n, const unsigned* __restrict__ keys,
+													const b3hAabb* __restrict__ sorted,
+													const int* __restrict__ cellStart, b3hGridParams gp,
+													int* __restrict__ counts, int2* __restrict__ stash,
+													const int* __restrict__ sceneOf)
+{
+	int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= n) return;
+	counts[s] = grid_scan_one<0>(s, n, keys, sorted, cellStart, gp, stash, 0, nullptr, 0, sceneOf);
+}
+/* ... and after the scan of the counts each proxy copies its stash to its
+ * offset; a proxy with more pairs than its stash holds scans again and writes
+ * them all.  The result is the two-pass (count, write) list, in its order. */
+__global__ __launch_bounds__(256) void k_grid_emit(int n, const unsigned* __restrict__ keys,
+												   const b3hAabb* __restrict__ sorted,
+												   const int* __restrict__ cellStart, b3hGridParams gp,
+												   const int* __restrict__ counts, const int* __restrict__ offsets,
+												   const int2* __restrict__ stash, b3hInt4* __restrict__ pairs,
+												   int maxPairs, const int* __restrict__ sceneOf)
+{
+	int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= n) return;
+	int c = counts[s], o = offsets[s];
+	if (c <= B3H_GRID_STASH)
+	{
+		for (int j = 0; j < c; j++)
+		{
+			int2 v = stash[(size_t)s * B3H_GRID_STASH + j];
+			if (o + j < maxPairs) pairs[o + j] = b3hInt4{v.x, v.y, -1, -1};
+		}
+	}
+	else
+		grid_scan_one<1>(s, n, keys, sorted, cellStart, gp, nullptr, o, pairs, maxPairs, sceneOf);
 }
 
 __global__ void k_pairs_finish(const int* __restrict__ total, int maxPairs, int* __restrict__ pairCount,
@@ -344,9 +382,8 @@ void b3h_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const i
 		hipLaunchKernelGGL(k_cell_start, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys, sc.cellStart);
 		hipLaunchKernelGGL(k_grid_gather, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.vals, all, smallMap,
 						   sc.sortedAabbs);
-		hipLaunchKernelGGL(k_grid_pairs<false>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys,
-						   (const b3hAabb*)sc.sortedAabbs, sc.cellStart, gp, countsG, (const int*)nullptr,
-						   (b3hInt4*)nullptr, maxPairs, sc.sceneOf);
+		hipLaunchKernelGGL(k_grid_pairs, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys,
+						   (const b3hAabb*)sc.sortedAabbs, sc.cellStart, gp, countsG, sc.stash, sc.sceneOf);
 	}
 	int nTot = 2 * nSmall;
 	b3h_exclusive_scan(sc.counts, sc.offsets, nullptr, nTot, sc.scanTmp, s);
@@ -354,9 +391,9 @@ void b3h_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const i
 	{
 		large_small_pass<true>(nSmall, nLarge, all, smallMap, largeMap, sc, (int*)nullptr, sc.offsets, pairs,
 							   maxPairs, s);
-		hipLaunchKernelGGL(k_grid_pairs<true>, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys,
-						   (const b3hAabb*)sc.sortedAabbs, sc.cellStart, gp, (int*)nullptr, sc.offsets + nSmall, pairs,
-						   maxPairs, sc.sceneOf);
+		hipLaunchKernelGGL(k_grid_emit, dim3(nblk(nSmall)), dim3(256), 0, s, nSmall, sc.keys,
+						   (const b3hAabb*)sc.sortedAabbs, sc.cellStart, gp, (const int*)countsG,
+						   (const int*)sc.offsets + nSmall, (const int2*)sc.stash, pairs, maxPairs, sc.sceneOf);
 	}
 	hipLaunchKernelGGL(k_pairs_finish, dim3(1), dim3(1), 0, s, sc.offsets + nTot, maxPairs, pairCount, pairTotal);
 }

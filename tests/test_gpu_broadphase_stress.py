@@ -90,3 +90,25 @@ def test_million_aabbs_grid_sap_oracle():
     assert np.all(np.diff(key) > 0)
     ng, ns = normalized(grid), normalized(sap)
     assert len(ng) == len(grid) and np.array_equal(ng, ns)
+
+
+def test_grid_proxy_with_many_pairs():
+    """one proxy overlapping far more boxes than the per-proxy stash of the
+    one-pass grid (32 pairs) holds, beside sparse ones: the list equals the
+    oracle's (the two-pass order) bit for bit"""
+    rng = np.random.default_rng(17)
+    n = 400
+    c = rng.uniform(-1.4, 1.4, size=(n, 3)).astype(np.float32)
+    c[200:] += np.float32(6.0)                       # a second, sparse group
+    h = np.full((n, 3), 0.08, np.float32)
+    h[0] = 1.45                                       # box 0 covers the first group
+    h[200] = 1.45
+    c[0] = 0.0
+    mn, mx = np.ascontiguousarray(c - h), np.ascontiguousarray(c + h)
+    grid, gtot, _ = device_pairs(capi.BROADPHASE_GRID, mn, mx, cell=3.0)
+    a = aabb_records(mn, mx)
+    og, _ = po.grid_pairs(a, np.arange(n, dtype=np.int32), np.zeros(0, np.int32), cell=(3.0, 3.0, 3.0),
+                          max_pairs=64 * n)
+    assert len(grid) == gtot
+    assert np.sum(grid[:, 0] == 0) > 150               # box 0 pairs with most of its group
+    assert np.array_equal(grid, og)
