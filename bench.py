@@ -83,7 +83,14 @@ def run(args):
     w.set_solver(sp)
     setup_s = time.time() - t0
     dt = 1.0 / 60.0
-    for _ in range(args.warmup):
+    tuned = None
+    for i in range(args.warmup):
+        if i == args.warmup // 2 and not args.no_autotune:
+            # narrowphase lanes per pair and PGS parts per cell picked by timing
+            # trial steps from a saved state image, which is restored afterwards;
+            # the rest of the warmup captures the step graph again
+            lanes, parts, tms = w.autotune(dt, steps=args.autotune_steps)
+            tuned = {"lanes_per_pair": lanes, "pgs_parts": parts, "trial_ms": round(tms, 4)}
         if args.eager:
             w.step(dt)
         else:
@@ -142,7 +149,7 @@ def run(args):
         "data": "synthetic",
         "config": {"workload": workload_name(args, n_dyn),
                    "bodies": n_dyn, "pairs": n_pairs, "contacts": n_contacts, "pgs_iterations": args.iters,
-                   "setup_s": round(setup_s, 2)},
+                   "setup_s": round(setup_s, 2), "autotune": tuned},
         "stages_ms": {k: round(float(v), 4) for k, v in st.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
@@ -280,6 +287,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start evaluation")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default narrowphase lanes per pair and PGS parts instead of timing the choices")
+    ap.add_argument("--autotune-steps", type=int, default=3)
     ap.add_argument("--eager", action="store_true",
                     help="launch every step's kernels one by one instead of replaying a captured HIP graph")
     ap.add_argument("--learn-collect-steps", type=int, default=40)

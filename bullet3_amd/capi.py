@@ -107,6 +107,7 @@ _SIGS = {
     "b3h_pipeline_get_pgs_levels": ([_vp, _vp], _i),
     "b3h_pipeline_get_pgs_parts": ([_vp, _vp, _vp], _i),
     "b3h_pipeline_set_pgs_max_parts": ([_vp, _i], _i),
+    "b3h_pipeline_autotune": ([_vp, _f, _i, _pi, _pi, _vp], _i),
     "b3h_pipeline_set_pgs_inertia_cache": ([_vp, _i], _i),
     "b3h_pipeline_set_track_residual": ([_vp, _i], _i),
     "b3h_pipeline_get_residuals": ([_vp, _vp, _i], _i),

@@ -1654,6 +1654,66 @@ extern "C" int b3h_pipeline_load_state(b3hPipeline* p, const void* host, size_t 
 	}
 	return B3H_OK;
 }
+
+/* Autotuner: times `steps` eager steps from the current state for every
+ * combination of narrowphase lanes per pair (64, 32, 16) and PGS parts per
+ * cell (8, 4, 1), restoring the state image before each trial, then keeps the
+ * fastest and restores the state once more.  The knobs do not change results,
+ * so the simulation continues exactly as if no trial had run. */
+extern "C" int b3h_pipeline_autotune(b3hPipeline* p, float dt, int steps, int* lanesOut, int* partsOut,
+									 float* msOut)
+{
+	CHECK_ARG(p && steps >= 1 && steps <= 64);
+	if (num_bodies(p) == 0) return B3H_OK;
+	size_t n = 0;
+	int r = b3h_pipeline_save_state(p, nullptr, 0, &n);
+	if (r) return r;
+	std::vector<unsigned char> image(n);
+	if ((r = b3h_pipeline_save_state(p, image.data(), n, &n))) return r;
+	const int lanes[3] = {64, 32, 16}, parts[3] = {B3H_PGS_PARTS, 4, 1};
+	hipEvent_t e0, e1;
+	B3H_CHECK(hipEventCreate(&e0));
+	B3H_CHECK(hipEventCreate(&e1));
+	float best = INFINITY;
+	int bestL = p->np->lanesPerPair, bestP = p->sb.pgsMaxParts > 0 ? p->sb.pgsMaxParts : B3H_PGS_PARTS;
+	for (int li = 0; li < 3 && !r; li++)
+		for (int pi = 0; pi < 3 && !r; pi++)
+		{
+			if ((r = b3h_pipeline_load_state(p, image.data(), n))) break;
+			p->np->lanesPerPair = lanes[li];
+			p->sb.pgsMaxParts = parts[pi];
+			hipEventRecord(e0, p->stream);
+			for (int k = 0; k < steps && !r; k++) r = pl_step(p, dt, false);
+			hipEventRecord(e1, p->stream);
+			if (hipEventSynchronize(e1) != hipSuccess)
+			{
+				b3h_set_error("autotune: %s", hipGetErrorString(hipGetLastError()));
+				r = B3H_ERR_HIP;
+				break;
+			}
+			float ms = 0.f;
+			hipEventElapsedTime(&ms, e0, e1);
+			ms /= (float)steps;
+			if (ms < best)
+			{
+				best = ms;
+				bestL = lanes[li];
+				bestP = parts[pi];
+			}
+		}
+	hipEventDestroy(e0);
+	hipEventDestroy(e1);
+	p->np->lanesPerPair = bestL;
+	p->sb.pgsMaxParts = bestP;
+	int r2 = b3h_pipeline_load_state(p, image.data(), n);
+	drop_graph(p);
+	if (r) return r;
+	if (r2) return r2;
+	if (lanesOut) *lanesOut = bestL;
+	if (partsOut) *partsOut = bestP;
+	if (msOut) *msOut = best;
+	return B3H_OK;
+}
 extern "C" int b3h_pipeline_get_solver_params(b3hPipeline* p, b3hSolverParams* prm)
 {
 	CHECK_ARG(p && prm);

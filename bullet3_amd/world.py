@@ -153,6 +153,14 @@ class World:
         call("b3h_pipeline_get_pgs_parts", self.pl, ptr(parts), ptr(big))
         return parts, big
 
+    def autotune(self, dt=1.0 / 60.0, steps=3):
+        """(lanes per pair, PGS parts cap, ms per step) the autotuner keeps; the
+        state is restored, so the simulation continues unchanged"""
+        lanes, parts, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        call("b3h_pipeline_autotune", self.pl, ctypes.c_float(dt), int(steps), ctypes.byref(lanes),
+             ctypes.byref(parts), ctypes.byref(ms))
+        return lanes.value, parts.value, ms.value
+
     def static0_index(self):
         v = ctypes.c_int()
         call("b3h_np_get_static0_index", self.np_, ctypes.byref(v))

@@ -210,6 +210,14 @@ int b3h_pipeline_get_pgs_parts(b3hPipeline* p, int* parts256, int* largest256);
 /* Upper bound on those parts (1..8, default 8; 1 solves every cell on one
  * work-group, as the reference does).  Results do not depend on it. */
 int b3h_pipeline_set_pgs_max_parts(b3hPipeline* p, int maxParts);
+/* Autotuner for the two schedule knobs above: times `steps` eager steps of
+ * length dt from the current state for each narrowphase lanes-per-pair (64,
+ * 32, 16) and PGS parts cap (8, 4, 1), restoring the state (b3h_pipeline_
+ * save_state image) before each trial and at the end, and keeps the fastest;
+ * writes the choice and its ms per step (host pointers, each may be NULL).
+ * Results do not depend on the knobs, so stepping continues bit for bit as
+ * without the trials.  No counterpart in the reference. */
+int b3h_pipeline_autotune(b3hPipeline* p, float dt, int steps, int* lanesOut, int* partsOut, float* msOut);
 /* Per-sweep residuals of the last solve (host, 2*iterations floats: normal sweeps
  * then friction sweeps); requires b3h_pipeline_set_track_residual(p, 1). */
 int b3h_pipeline_set_track_residual(b3hPipeline* p, int enable);

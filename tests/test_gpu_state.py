@@ -117,3 +117,24 @@ def test_bad_images_are_refused():
     assert lib.b3h_pipeline_save_state(w.pl, capi.ptr(tiny), 16, ctypes.byref(n)) == -2
     assert n.value == len(image)
     w.close()
+
+
+def test_autotune_leaves_the_simulation_unchanged():
+    """the autotuner times trial steps from a state image and restores it, and
+    its knobs do not change results: stepping afterwards equals a twin world
+    that never tuned, bit for bit"""
+    w1, w2 = make_world(1), make_world(1)
+    for _ in range(10):
+        w1.step(DT)
+        w2.step(DT)
+    lanes, parts, ms = w2.autotune(DT, steps=2)
+    print("autotune: lanes %d parts %d %.3f ms" % (lanes, parts, ms), flush=True)
+    assert lanes in (16, 32, 64) and parts in (1, 4, 8) and ms > 0
+    assert bits_equal(w1.bodies(), w2.bodies())
+    for _ in range(6):
+        w1.step(DT)
+        w2.step(DT)
+    a, b = w1.bodies(), w2.bodies()
+    assert bits_equal(a, b), first_diff(a, b)
+    w1.close()
+    w2.close()
