@@ -81,6 +81,8 @@ def run(args):
     if args.friction is not None:
         sp.m_frictionCoeff = args.friction
     w.set_solver(sp)
+    if args.config == "batched":
+        w.set_scenes(sc.meta["scene_offsets"])
     setup_s = time.time() - t0
     dt = 1.0 / 60.0
     tuned = None

@@ -123,41 +123,51 @@ __global__ __launch_bounds__(256) void k_cell_scatter(const int* __restrict__ nP
 
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef __attribute__((address_space(3))) unsigned long long LdsU64;
+typedef __attribute__((address_space(3))) unsigned LdsU32;
 #define LDS_U64(p) ((LdsU64*)(p))
+#define LDS_U32(p) ((LdsU32*)(p))
 #else
 #define LDS_U64(p) (p)
+#define LDS_U32(p) (p)
 #endif
-/* bitonic sort ascending of np2 keys with their payloads (KP: LDS or global) */
-template <typename KP>
-__device__ __forceinline__ void bitonic_sort(KP key, KP pay, int np2, int t)
+/* ascending sort of n keys with their payloads (KP, PP: LDS or global
+ * pointers), with no padding: the bitonic network of the next power of two in
+ * its all-ascending form (the first step of every merge compares mirrored
+ * positions), where the absent positions n.. act as the largest keys and are
+ * never moved, so they are never stored */
+template <typename KP, typename PP>
+__device__ __forceinline__ void bitonic_sort_n(KP key, PP pay, int n, int t)
 {
+	int np2 = 1;
+	while (np2 < n) np2 <<= 1;
 	for (int k = 2; k <= np2; k <<= 1)
 		for (int j = k >> 1; j > 0; j >>= 1)
 		{
-			for (int i = t; i < np2; i += blockDim.x)
+			const int mask = j == (k >> 1) ? k - 1 : j;
+			for (int i = t; i < n; i += blockDim.x)
 			{
-				int ixj = i ^ j;
-				if (ixj > i)
+				int l = i ^ mask;
+				if (l > i && l < n)
 				{
-					unsigned long long ki = key[i], kj = key[ixj];
-					bool up = (i & k) == 0;
-					if ((ki > kj) == up)
+					unsigned long long ki = key[i], kl = key[l];
+					if (ki > kl)
 					{
-						key[i] = kj;
-						key[ixj] = ki;
-						unsigned long long p = pay[i];
-						pay[i] = pay[ixj];
-						pay[ixj] = p;
+						key[i] = kl;
+						key[l] = ki;
+						unsigned p = pay[i];
+						pay[i] = pay[l];
+						pay[l] = p;
 					}
 				}
 			}
 			__syncthreads();
 		}
 }
-/* the greedy pass of k_batch_cells on one wavefront (KP: LDS or global
- * pointers to the keys and payloads); returns the number of batches */
-template <typename KP>
-__device__ __forceinline__ int batch_greedy(KP key, KP pay, int n, int staticIdx)
+/* the greedy pass of k_batch_cells on one wavefront (KP, PP: LDS or global
+ * pointers to the keys and payloads); a contact's batch is stored (batchOut)
+ * when it is accepted, at its final position; returns the number of batches */
+template <typename KP, typename PP>
+__device__ __forceinline__ int batch_greedy(KP key, PP pay, int n, int staticIdx, int* __restrict__ batchOut)
 {
 	const int lane = threadIdx.x & 63;
 	int numValid = 0, batch = 0;
@@ -165,16 +175,21 @@ __device__ __forceinline__ int batch_greedy(KP key, KP pay, int n, int staticIdx
 	{
 		int nCur = 0;
 		unsigned word = 0u; /* bitmap word `lane` */
+		/* the next chunk's keys are read while this one is resolved: the swaps
+		 * write only positions below the end of the current chunk */
+		unsigned long long kkNext = (numValid + lane < n) ? key[numValid + lane] : 0ull;
 		for (int c0 = numValid; c0 < n; c0 += 64)
 		{
 			int i = c0 + lane;
 			bool valid = i < n;
-			unsigned long long kk = valid ? key[i] : 0ull;
+			unsigned long long kk = kkNext;
+			if (i + 64 < n) kkNext = key[i + 64];
 			int aS = (int)(unsigned)(kk >> 32), bS = (int)(unsigned)(kk & 0xffffffffu);
 			int a = iabs(aS), b = iabs(bS);
 			bool aStatic = (aS < 0) || aS == staticIdx;
 			bool bStatic = (bS < 0) || bS == staticIdx;
-			int ia = a % (32 * CHECK_SIZE), ib = b % (32 * CHECK_SIZE);
+			/* body ids are below 2^31, so the remainder is the low 11 bits */
+			int ia = (int)((unsigned)a & (32u * CHECK_SIZE - 1u)), ib = (int)((unsigned)b & (32u * CHECK_SIZE - 1u));
 			int wa = ia >> 5, wb = ib >> 5;
 			unsigned ma = aStatic ? 0u : (1u << (ia & 31));
 			unsigned mb = bStatic ? 0u : (1u << (ib & 31));
@@ -221,12 +236,14 @@ __device__ __forceinline__ int batch_greedy(KP key, KP pay, int n, int staticIdx
 					int m = __popcll(acc & ((1ull << lane) - 1ull));
 					int dst = numValid + m;
 					int ii = c0 + lane;
-					unsigned long long pj = pay[ii];
-					unsigned long long kd = key[dst], pd = pay[dst];
+					unsigned pj = pay[ii];
+					unsigned long long kd = key[dst];
+					unsigned pd = pay[dst];
 					key[ii] = kd;
 					pay[ii] = pd;
 					key[dst] = kk;
-					pay[dst] = (pj & 0xffffffffull) | ((unsigned long long)(unsigned)batch << 32);
+					pay[dst] = pj;
+					batchOut[dst] = batch;
 				}
 				numValid += kAcc;
 			}
@@ -238,20 +255,20 @@ __device__ __forceinline__ int batch_greedy(KP key, KP pay, int n, int staticIdx
 					int j = __ffsll((long long)m) - 1;
 					m &= m - 1;
 					int ii = c0 + j;
-					unsigned long long kj = key[ii];
-					unsigned long long pj = pay[ii];
-					unsigned long long pi = (pj & 0xffffffffull) | ((unsigned long long)(unsigned)batch << 32);
 					if (ii != numValid)
 					{
+						unsigned long long kj = key[ii];
+						unsigned pj = pay[ii];
 						key[ii] = key[numValid];
 						pay[ii] = pay[numValid];
 						key[numValid] = kj;
+						pay[numValid] = pj;
 					}
-					pay[numValid] = pi;
+					batchOut[numValid] = batch;
 					numValid++;
 				}
 			}
-			numValid = __shfl(numValid, 0);
+			numValid = __builtin_amdgcn_readfirstlane(numValid);
 			wave_sync_solver();
 		}
 		batch++;
@@ -260,13 +277,13 @@ __device__ __forceinline__ int batch_greedy(KP key, KP pay, int n, int staticIdx
 }
 
 /* One work-group per cell: sort the cell's (uint bodyA, uint bodyB) keys, then
- * CreateBatchesNew's greedy batching.  Arrays live in LDS when the cell fits
- * (ldsCap entries), otherwise in global scratch (sortKeys, 2 entries per
- * contact so that the power-of-two padding fits). */
+ * CreateBatchesNew's greedy batching.  Keys (8 bytes) and contact slots (4
+ * bytes) live in LDS when the cell fits (ldsBytes), otherwise in the cell's
+ * region of the global scratch (4 keys per contact). */
 __global__ __launch_bounds__(256) void k_batch_cells(const int* __restrict__ cellCounts, const int* __restrict__ cellOffsets,
 													 const int* __restrict__ cellSlots,
 													 const b3hContact4Data* __restrict__ staging,
-													 const int* __restrict__ contactPair, int staticIdx, int ldsCap,
+													 const int* __restrict__ contactPair, int staticIdx, int ldsBytes,
 													 unsigned long long* __restrict__ gKeys, int* __restrict__ perm,
 													 int* __restrict__ batchIdxOut, int* __restrict__ batchSizes)
 {
@@ -280,40 +297,22 @@ __global__ __launch_bounds__(256) void k_batch_cells(const int* __restrict__ cel
 		if (t == 0) batchSizes[cell] = 0;
 		return;
 	}
-	int np2 = 1;
-	while (np2 < n) np2 <<= 1;
-	bool inLds = np2 <= ldsCap;
+	bool inLds = (size_t)n * 12 <= (size_t)ldsBytes;
 	/* key = uint bodyA << 32 | uint bodyB ; payload = contact slot (parallel array) */
-	unsigned long long* key = inLds ? smem : gKeys + 2 * (size_t)off;
-	unsigned long long* pay = inLds ? smem + ldsCap : gKeys + 2 * (size_t)(off + 0) + np2; /* second half */
-	if (!inLds)
+	unsigned long long* key = inLds ? smem : gKeys + 4 * (size_t)off;
+	unsigned* pay = (unsigned*)(key + n);
+	for (int i = t; i < n; i += blockDim.x)
 	{
-		/* global layout: keys at [2*off, 2*off+np2), payload right after; the
-		 * 2*count region of this cell holds 2*np2 > 2*n entries only if np2 <= n,
-		 * so use the padded region of the whole scratch instead */
-		key = gKeys + 4 * (size_t)off;
-		pay = key + np2;
-	}
-	for (int i = t; i < np2; i += blockDim.x)
-	{
-		if (i < n)
-		{
-			int slot = cellSlots[off + i];
-			const b3hContact4Data& c = staging[contactPair[slot]];
-			key[i] = ((unsigned long long)(unsigned)c.m_bodyAPtrAndSignBit << 32) | (unsigned)c.m_bodyBPtrAndSignBit;
-			pay[i] = (unsigned long long)slot;
-		}
-		else
-		{
-			key[i] = ~0ull;
-			pay[i] = ~0ull;
-		}
+		int slot = cellSlots[off + i];
+		const b3hContact4Data& c = staging[contactPair[slot]];
+		key[i] = ((unsigned long long)(unsigned)c.m_bodyAPtrAndSignBit << 32) | (unsigned)c.m_bodyBPtrAndSignBit;
+		pay[i] = (unsigned)slot;
 	}
 	__syncthreads();
 	if (inLds)
-		bitonic_sort(LDS_U64(key), LDS_U64(pay), np2, t);
+		bitonic_sort_n(LDS_U64(key), LDS_U32(pay), n, t);
 	else
-		bitonic_sort(key, pay, np2, t);
+		bitonic_sort_n(key, pay, n, t);
 	/* greedy batching, wave-parallel with the sequential semantics of
 	 * CreateBatchesNew: the 2048-bit bitmap is held one 32-bit word per lane;
 	 * each chunk of 64 candidates is pre-checked against the bitmap at chunk
@@ -324,23 +323,18 @@ __global__ __launch_bounds__(256) void k_batch_cells(const int* __restrict__ cel
 	 * to the front one by one, exactly as the reference moves them (that
 	 * permutes the rejected contacts, and so the next pass's scan order).
 	 * Positions not yet examined are never written by the swaps, so each chunk
-	 * can be read up front.  Batch ids go into the high half of pay[]. */
+	 * can be read up front. */
 	if (t < 64)
 	{
 		int batch;
 		if (inLds)
-			batch = batch_greedy(LDS_U64(key), LDS_U64(pay), n, staticIdx);
+			batch = batch_greedy(LDS_U64(key), LDS_U32(pay), n, staticIdx, batchIdxOut + off);
 		else
-			batch = batch_greedy(key, pay, n, staticIdx);
+			batch = batch_greedy(key, pay, n, staticIdx, batchIdxOut + off);
 		if (t == 0) batchSizes[cell] = batch;
 	}
 	__syncthreads();
-	for (int i = t; i < n; i += blockDim.x)
-	{
-		unsigned long long p = pay[i];
-		perm[off + i] = (int)(unsigned)(p & 0xffffffffull);
-		batchIdxOut[off + i] = (int)(unsigned)(p >> 32);
-	}
+	for (int i = t; i < n; i += blockDim.x) perm[off + i] = (int)pay[i];
 }
 
 /* ---------------- contact -> constraint (setConstraint4, b3ConvertConstraint4.h:62-148) ---------------- */
@@ -910,11 +904,11 @@ __global__ __launch_bounds__(256) void k_lambda_writeback(const int* __restrict_
 	}
 }
 
-/* LDS bytes the level sweep needs for a cell of n constraints and nb bodies */
-__host__ __device__ __forceinline__ size_t pgs_level_lds(int nb, int n, bool cacheInertia)
+/* LDS bytes the level sweep needs for a part of nbp bodies and L levels:
+ * velocities (nbp + 2 slots, linear and angular) and the level starts */
+__host__ __device__ __forceinline__ size_t pgs_part_lds(int nbp, int L)
 {
-	return (size_t)(nb + 2) * 36 + (cacheInertia ? (size_t)(nb + 2) * 48 : 0) + (size_t)(n + 1) * 4 +
-		   (size_t)(n + 2) * 4;
+	return (size_t)(nbp + 2) * 32 + (size_t)(L + 2) * 4;
 }
 
 /* ---------------- per-cell dependency levels and parts ----------------
@@ -932,6 +926,8 @@ __host__ __device__ __forceinline__ size_t pgs_level_lds(int nb, int n, bool cac
  * share no body and their constraints can be solved by different work-groups
  * at once without changing any body's update order.  The tiles are packed into
  * at most B3H_PGS_PARTS parts, largest first onto the least loaded part.
+ * Each part's bodies take part-local velocity slots (cslot is rewritten to
+ * them), so a part needs LDS for its own bodies only.
  *
  * Output per cell: the constraints part by part, each part grouped by level
  * (levelOrder, region [off, off+n)); per part its level starts relative to the
@@ -954,11 +950,16 @@ __device__ __forceinline__ unsigned tile_key(F4 p, float scale)
 	unsigned tz = (unsigned)floor_div_pow2(z, B3H_N_SPLIT_Z) & 1023u;
 	return tx | (ty << 10) | (tz << 20);
 }
-#define PGS_MAX_TILES 64
+/* tile hash table (open addressing) and the sort of the occupied slots */
+#define PGS_TILE_SLOTS 2048
+#define PGS_TILE_LDS (PGS_TILE_SLOTS * (4 + 4 + 4 + 8) + 256)
+/* lev[k]: level in the low 20 bits, tile slot and then part above */
+#define LEV_MASK 0xfffff
+#define LEV_SHIFT 20
 __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cellCounts,
 													 const int* __restrict__ cellOffsets,
 													 const int* __restrict__ batchIdx,
-													 const int* __restrict__ cellBodyCount, const int* __restrict__ cslot,
+													 const int* __restrict__ cellBodyCount, int* __restrict__ cslot,
 													 const b3hContactConstraint4* __restrict__ cs,
 													 const b3hRigidBodyData* __restrict__ bodies, float scale,
 													 int bodyCap, int ldsInts, int pgsLdsBytes, int* __restrict__ levelOrder,
@@ -967,9 +968,10 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 													 int* __restrict__ partBodies, int* __restrict__ posOf, int maxParts)
 {
 	extern __shared__ int sm[];
-	__shared__ int sL, sP, sNumTiles;
-	__shared__ unsigned tKey[PGS_MAX_TILES];
-	__shared__ int tCount[PGS_MAX_TILES], tPart[PGS_MAX_TILES];
+	__shared__ int sL, sP, sNumTiles, sOverflow, sFit;
+	__shared__ unsigned tKey[PGS_TILE_SLOTS];
+	__shared__ int tCount[PGS_TILE_SLOTS], tPart[PGS_TILE_SLOTS];
+	__shared__ unsigned long long tSort[PGS_TILE_SLOTS];
 	__shared__ int pCount[B3H_PGS_PARTS], pbFirst[B3H_PGS_PARTS], pbCursor[B3H_PGS_PARTS];
 	int cell = blockIdx.x;
 	int n = cellCounts[cell];
@@ -984,8 +986,9 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 		return;
 	}
 	int* lsOut = levelStart + off + B3H_PGS_PARTS * cell;
-	/* last[nb] | lev[n] | cnt[max(n + 1, PARTS (L + 1))] */
-	if (nb > bodyCap || nb + 2 * n + 1 > ldsInts || pgs_level_lds(nb, n, false) > (size_t)pgsLdsBytes)
+	/* last[nb] | lev[n] | cnt[P (L + 1)]; a cell whose arrays do not fit
+	 * takes the global walk */
+	if (nb + n + 1 > ldsInts)
 	{
 		if (t == 0)
 		{
@@ -1000,14 +1003,18 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 	int* lev = sm + nb;
 	int* cnt = lev + n;
 	for (int k = t; k < nb; k += blockDim.x) last[k] = 0;
-	for (int k = t; k <= n; k += blockDim.x) cnt[k] = 0;
-	if (t < PGS_MAX_TILES)
+	for (int q = t; q < PGS_TILE_SLOTS; q += blockDim.x)
 	{
-		tKey[t] = 0xffffffffu;
-		tCount[t] = 0;
-		tPart[t] = 0;
+		tKey[q] = 0xffffffffu;
+		tCount[q] = 0;
+		tPart[q] = 0;
 	}
-	if (t == 0) sNumTiles = 0;
+	if (t == 0)
+	{
+		sNumTiles = 0;
+		sOverflow = 0;
+		sFit = 1;
+	}
 	__syncthreads();
 	if (t < 64)
 	{
@@ -1050,6 +1057,17 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 	}
 	__syncthreads();
 	int L = sL;
+	if (nb + n + L + 1 > ldsInts)
+	{
+		if (t == 0)
+		{
+			cellLevels[cell] = -1;
+			meta[0] = b3hInt4{0, n, 0, -1};
+			meta[1] = b3hInt4{0, nb, 0, 0};
+		}
+		for (int k = t; k < n; k += blockDim.x) posOf[off + k] = -1;
+		return;
+	}
 	/* tiles: key of each constraint's first dynamic body, counted in a small
 	 * open-addressing table; an overflowing table leaves the cell in one part */
 	bool split = nb + n + B3H_PGS_PARTS * (L + 1) <= ldsInts;
@@ -1060,53 +1078,92 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 		const b3hContactConstraint4& c = cs[off + k];
 		unsigned body = sA >= 0 ? c.m_bodyA : c.m_bodyB;
 		unsigned key = tile_key(bodies[body].m_pos, scale);
-		int slot = (int)((key * 2654435761u) >> 26); /* 6 bits */
+		int slot = (int)((key * 2654435761u) >> 21); /* 11 bits */
 		int probes = 0;
-		for (;; slot = (slot + 1) & (PGS_MAX_TILES - 1))
+		for (;; slot = (slot + 1) & (PGS_TILE_SLOTS - 1))
 		{
 			unsigned old = atomicCAS(&tKey[slot], 0xffffffffu, key);
 			if (old == 0xffffffffu || old == key) break;
-			if (++probes == PGS_MAX_TILES)
+			if (++probes == PGS_TILE_SLOTS)
 			{
 				slot = -1;
 				break;
 			}
 		}
 		if (slot < 0)
-			sNumTiles = PGS_MAX_TILES + 1;
+			sOverflow = 1;
 		else
 		{
-			atomicAdd(&tCount[slot], 1);
-			lev[k] |= slot << 24;
+			if (atomicAdd(&tCount[slot], 1) == 0) atomicAdd(&sNumTiles, 1);
+			lev[k] |= slot << LEV_SHIFT;
 		}
 	}
 	__syncthreads();
+	/* the occupied slots in assignment order: largest tile first, ties by the
+	 * smaller key (count < 2^20 and key < 2^30 pack with the slot) */
+	const bool tiled = split && !sOverflow && maxParts > 1 && sNumTiles > 1;
+	int nT = 0, np2T = 1;
+	if (tiled)
+	{
+		__shared__ int sCur;
+		if (t == 0) sCur = 0;
+		__syncthreads();
+		for (int q = t; q < PGS_TILE_SLOTS; q += blockDim.x)
+			if (tCount[q] > 0)
+			{
+				int i = atomicAdd(&sCur, 1);
+				tSort[i] = ((unsigned long long)(0xfffffu - (unsigned)tCount[q]) << 41) |
+						   ((unsigned long long)tKey[q] << 11) | (unsigned long long)q;
+			}
+		__syncthreads();
+		nT = sCur;
+		while (np2T < nT) np2T <<= 1;
+		for (int i = nT + t; i < np2T; i += blockDim.x) tSort[i] = ~0ull;
+		__syncthreads();
+		for (int kk = 2; kk <= np2T; kk <<= 1)
+			for (int j = kk >> 1; j > 0; j >>= 1)
+			{
+				for (int i = t; i < np2T; i += blockDim.x)
+				{
+					int ixj = i ^ j;
+					if (ixj > i)
+					{
+						unsigned long long a = tSort[i], b = tSort[ixj];
+						if ((a > b) == ((i & kk) == 0))
+						{
+							tSort[i] = b;
+							tSort[ixj] = a;
+						}
+					}
+				}
+				__syncthreads();
+			}
+	}
 	if (t == 0)
 	{
 		int P = 1;
-		if (split && sNumTiles <= PGS_MAX_TILES)
+		if (tiled)
 		{
-			for (int q = 0; q < B3H_PGS_PARTS; q++) pCount[q] = 0;
-			/* largest tile first (ties: smaller key) onto the least loaded part */
-			for (;;)
+			/* each tile onto the least loaded part (ties: the lower part) */
+			int load[B3H_PGS_PARTS];
+#pragma unroll
+			for (int q = 0; q < B3H_PGS_PARTS; q++) load[q] = 0;
+			for (int i = 0; i < nT; i++)
 			{
-				int best = -1;
-				for (int q = 0; q < PGS_MAX_TILES; q++)
-					if (tCount[q] > 0 && tPart[q] == 0 &&
-						(best < 0 || tCount[q] > tCount[best] || (tCount[q] == tCount[best] && tKey[q] < tKey[best])))
-						best = q;
-				if (best < 0) break;
-				int pm = 0;
-				for (int q = 1; q < maxParts; q++)
-					if (pCount[q] < pCount[pm]) pm = q;
-				pCount[pm] += tCount[best];
-				tPart[best] = pm + 1; /* 1-based while assigning */
+				int q = (int)(tSort[i] & (PGS_TILE_SLOTS - 1));
+				int pm = 0, best = load[0];
+#pragma unroll
+				for (int r = 1; r < B3H_PGS_PARTS; r++)
+					if (r < maxParts && load[r] < best)
+					{
+						best = load[r];
+						pm = r;
+					}
+				load[pm] += tCount[q];
+				tPart[q] = pm;
 				P = P > pm + 1 ? P : pm + 1;
 			}
-			for (int q = 0; q < PGS_MAX_TILES; q++) tPart[q] = tPart[q] > 0 ? tPart[q] - 1 : 0;
 		}
-		else
-			for (int q = 0; q < PGS_MAX_TILES; q++) tPart[q] = 0;
 		sP = P;
 	}
 	__syncthreads();
@@ -1117,7 +1174,7 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 	__syncthreads();
 	for (int k = t; k < n && P > 1; k += blockDim.x)
 	{
-		int part = tPart[(unsigned)lev[k] >> 24];
+		int part = tPart[(unsigned)lev[k] >> LEV_SHIFT];
 		int sA = cslot[2 * (size_t)(off + k)], sB = cslot[2 * (size_t)(off + k) + 1];
 		if (sA >= 0)
 		{
@@ -1137,15 +1194,38 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 	__syncthreads();
 	for (int k = t; k < n; k += blockDim.x)
 	{
-		int part = P > 1 ? tPart[(unsigned)lev[k] >> 24] : 0;
-		int lv = lev[k] & 0xffffff;
-		lev[k] = lv | (part << 24);
+		int part = P > 1 ? tPart[(unsigned)lev[k] >> LEV_SHIFT] : 0;
+		int lv = lev[k] & LEV_MASK;
+		lev[k] = lv | (part << LEV_SHIFT);
 		atomicAdd(&cnt[part * stride + lv], 1);
 	}
 	if (t < B3H_PGS_PARTS) pCount[t] = 0;
 	__syncthreads();
 	for (int k = t; k < nb; k += blockDim.x) atomicAdd(&pCount[P > 1 ? last[k] : 0], 1);
 	__syncthreads();
+	if (t == 0)
+	{
+		/* every part's velocities and level starts must fit the sweep's LDS;
+		 * otherwise the cell takes the global walk */
+		for (int q = 0; q < P; q++)
+		{
+			int Lp = 0;
+			for (int l = 0; l < L; l++) Lp += cnt[q * stride + l] > 0;
+			if (pCount[q] > bodyCap || pgs_part_lds(pCount[q], Lp) > (size_t)pgsLdsBytes) sFit = 0;
+		}
+		if (!sFit)
+		{
+			cellLevels[cell] = -1;
+			meta[0] = b3hInt4{0, n, 0, -1};
+			meta[1] = b3hInt4{0, nb, 0, 0};
+		}
+	}
+	__syncthreads();
+	if (!sFit)
+	{
+		for (int k = t; k < n; k += blockDim.x) posOf[off + k] = -1;
+		return;
+	}
 	if (t == 0)
 	{
 		int runB = 0;
@@ -1183,17 +1263,28 @@ __global__ __launch_bounds__(256) void k_cell_levels(const int* __restrict__ cel
 	__syncthreads();
 	for (int k = t; k < n; k += blockDim.x)
 	{
-		int part = (unsigned)lev[k] >> 24, lv = lev[k] & 0xffffff;
+		int part = (unsigned)lev[k] >> LEV_SHIFT, lv = lev[k] & LEV_MASK;
 		int pos = atomicAdd(&cnt[part * stride + lv], 1);
 		levelOrder[off + pos] = off + k;
 		posOf[off + k] = off + pos;
 	}
+	/* each body's part-local slot: its place in the part's body list */
 	for (int k = t; k < nb; k += blockDim.x)
 	{
-		int pos = atomicAdd(&pbCursor[P > 1 ? last[k] : 0], 1);
+		int part = P > 1 ? last[k] : 0;
+		int pos = atomicAdd(&pbCursor[part], 1);
 		int* e = partBodies + 2 * (2 * (size_t)off + pos);
-		e[0] = k;
+		e[0] = pos - pbFirst[part];
 		e[1] = cellBodies[2 * (size_t)off + k];
+		last[k] = pos - pbFirst[part];
+	}
+	__syncthreads();
+	for (int k = t; k < n; k += blockDim.x)
+	{
+		int* cp = cslot + 2 * (size_t)(off + k);
+		int sA = cp[0], sB = cp[1];
+		cp[0] = sA >= 0 ? last[sA] : -1;
+		cp[1] = sB >= 0 ? last[sB] : -1;
 	}
 }
 
@@ -1608,16 +1699,13 @@ __global__ __launch_bounds__(PGS_THREADS) void k_pgs_level(b3hRigidBodyData* __r
 	const int nbp = pb.y; /* this part's bodies: (slot, body) pairs */
 	const int* own = partBodies + 2 * (2 * (size_t)start + pb.x);
 	const int base = start + pm.x; /* the part's first position */
-	int nb = cellBodyCount[cell];
-	/* velocities, sized for the whole cell (slots are the cell's), filled for
-	 * this part's bodies only; the rows carry their inverse-inertia products,
-	 * so no inertia is read here */
+	/* velocities in part-local slots (own[2 q] == q); the rows carry their
+	 * inverse-inertia products, so no inertia is read here */
+	const int nb = nbp;
 	(void)ldsBytes;
-	F4* sLv = smem4;                   /* nb + 2: slot nb zero (static bodies), nb + 1 padding */
-	F4* sAv = sLv + nb + 2;            /* nb + 2 */
-	int* sList = (int*)(sAv + nb + 2); /* nb + 2 (unused by the sweep) */
-	int* sOrder = sList + nb + 2;      /* n + 1 (unused by the sweep) */
-	int* sLs = sOrder + n + 1;         /* L + 2: level starts relative to start */
+	F4* sLv = smem4;                 /* nb + 2: slot nb zero (static bodies), nb + 1 padding */
+	F4* sAv = sLv + nb + 2;          /* nb + 2 */
+	int* sLs = (int*)(sAv + nb + 2); /* L + 2: level starts relative to start */
 	/* stage the level starts (positions are the part's first position + q;
 	 * records are read by position, so the level order itself is not needed) */
 	{
@@ -1740,7 +1828,8 @@ int b3h_solver_module_init()
 	if (hipFuncSetAttribute((const void*)k_batch_cells, hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess ||
 		hipFuncSetAttribute((const void*)k_cell_bodies, hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess)
 		want = 65536 - 1024;
-	if (hipFuncSetAttribute((const void*)k_cell_levels, hipFuncAttributeMaxDynamicSharedMemorySize, want) != hipSuccess)
+	if (hipFuncSetAttribute((const void*)k_cell_levels, hipFuncAttributeMaxDynamicSharedMemorySize,
+							want - PGS_TILE_LDS) != hipSuccess)
 		want = 65536 - 1024;
 	int pgsBytes = want;
 	if (hipFuncSetAttribute((const void*)k_pgs_level<0>, hipFuncAttributeMaxDynamicSharedMemorySize, pgsBytes) !=
@@ -1772,8 +1861,6 @@ void b3h_solve_setup(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaDa
 					 hipStream_t s)
 {
 	int ldsCapBytes = s_ldsCapBytes > 0 ? s_ldsCapBytes : 32768;
-	int ldsCap = 1;
-	while ((ldsCap * 2) * 16 <= ldsCapBytes) ldsCap *= 2;
 	b3h_fill_i32(sb.cellCounts, 0, B3H_N_CELLS, s);
 	int gb = gblocks(maxContacts);
 	int kb = gb < B3H_N_CELLS ? gb : B3H_N_CELLS;
@@ -1781,8 +1868,8 @@ void b3h_solve_setup(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaDa
 					   1.f / prm.m_batchCellSize, sb.cellOf, sb.cellCounts);
 	hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(B3H_N_CELLS), 0, s, sb.cellCounts, sb.cellOffsets, sb.cellCursor);
 	hipLaunchKernelGGL(k_cell_scatter, dim3(kb), dim3(256), 0, s, numContacts, sb.cellOf, sb.cellCursor, sb.cellSlots);
-	hipLaunchKernelGGL(k_batch_cells, dim3(B3H_N_CELLS), dim3(256), (size_t)ldsCap * 16, s, sb.cellCounts, sb.cellOffsets,
-					   sb.cellSlots, staging, contactPair, staticIdx, ldsCap, sb.sortKeys, sb.perm, sb.batchIdx,
+	hipLaunchKernelGGL(k_batch_cells, dim3(B3H_N_CELLS), dim3(256), (size_t)ldsCapBytes, s, sb.cellCounts, sb.cellOffsets,
+					   sb.cellSlots, staging, contactPair, staticIdx, ldsCapBytes, sb.sortKeys, sb.perm, sb.batchIdx,
 					   sb.batchSizes);
 	hipLaunchKernelGGL(k_to_constraints, dim3(gb), dim3(256), 0, s, numContacts, sb.perm, sb.batchIdx, staging,
 					   contactPair, bodies, inertias, prm, sb.constraints,
@@ -1794,9 +1881,10 @@ void b3h_solve_setup(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaDa
 					   sb.cellBodyCount, sb.cslot, sb.lastClass);
 	int cap = (sb.pgsBodyCap >= 0 && sb.pgsBodyCap < s_pgsBodyCap) ? sb.pgsBodyCap : s_pgsBodyCap;
 	size_t pgsLds = (size_t)s_pgsLdsBytes;
-	hipLaunchKernelGGL(k_cell_levels, dim3(B3H_N_CELLS), dim3(256), (size_t)ldsCapBytes, s, sb.cellCounts,
+	const int levelsLds = ldsCapBytes - PGS_TILE_LDS; /* beside the kernel's static tile tables */
+	hipLaunchKernelGGL(k_cell_levels, dim3(B3H_N_CELLS), dim3(256), (size_t)levelsLds, s, sb.cellCounts,
 					   sb.cellOffsets, sb.batchIdx, sb.cellBodyCount, sb.cslot, (const b3hContactConstraint4*)sb.constraints,
-					   bodies, 1.f / prm.m_batchCellSize, cap, ldsCapBytes / 4, (int)pgsLds, sb.levelOrder,
+					   bodies, 1.f / prm.m_batchCellSize, cap, levelsLds / 4, (int)pgsLds, sb.levelOrder,
 					   sb.levelStart, sb.cellLevels, (const int*)sb.cellBodies, sb.partMeta, sb.partBodies, sb.posOf,
 					   sb.pgsMaxParts > 0 && sb.pgsMaxParts < B3H_PGS_PARTS ? sb.pgsMaxParts : B3H_PGS_PARTS);
 	hipLaunchKernelGGL(k_pgs_prep, dim3(gb), dim3(256), 0, s, numContacts, sb.constraints, sb.cslot, bodies, inertias,

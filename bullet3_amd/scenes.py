@@ -149,10 +149,14 @@ def aabb_cloud(n=1 << 20, mean_overlap=8.0, hmin=0.05, hmax=1.0, seed=3, side=No
     return mn, mx, float(side)
 
 
-def batched_scenes(n_scenes=4096, min_bodies=16, max_bodies=512, spacing=24.0, n_hull_shapes=32, seed=4):
+def batched_scenes(n_scenes=4096, min_bodies=16, max_bodies=512, spacing=27.0, n_hull_shapes=32, seed=4):
     """Many independent scenes packed into one world: scene k sits on its own
     static ground (16 x 1 x 16 m) at a lattice position `spacing` m from its
-    neighbours, so no two scenes interact.  Body counts are log-uniform in
+    neighbours, so no two scenes interact.  The solver's 256 batching cells
+    (6 m, wrapping every 48 m in x and z) serialise the contacts of one cell;
+    at a 27 m pitch the scene centres fall on 16 different offsets within the
+    48 m period and spread over all cells, where a pitch dividing 48 m (24 m)
+    would put every scene's centre on the same few cells.  Body counts are log-uniform in
     [min_bodies, max_bodies]; even scenes are piles (boxes and 8-24-point hulls,
     random orientations, lattice spacing 1.2 m), odd scenes are box stacks
     (columns, spacing 1.02 m).  Returns one Scene; meta["scene_of_body"] maps

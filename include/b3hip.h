@@ -188,10 +188,11 @@ int b3h_pipeline_get_solver_params(b3hPipeline* p, b3hSolverParams* prm);
 int b3h_pipeline_get_solver_debug(b3hPipeline* p, b3hContact4Data** sortedContacts,
 								  b3hContactConstraint4** constraints, int** cellCounts, int** cellOffsets,
 								  int** batchSizes, int* numConstraints);
-/* Upper bound on the bodies per solver cell whose velocities the PGS sweep
- * keeps in LDS (negative: the device's LDS budget; 0: every cell takes the
- * global read-modify-write walk).  No counterpart in the reference (the
- * reference always walks global memory); for tests and tuning. */
+/* Upper bound on the bodies whose velocities one PGS work-group (one part of
+ * a solver cell) keeps in LDS; a cell with a larger part takes the global
+ * read-modify-write walk (negative: the device's LDS budget; 0: every cell
+ * takes the global walk).  No counterpart in the reference (the reference
+ * always walks global memory); for tests and tuning. */
 int b3h_pipeline_set_pgs_body_cache(b3hPipeline* p, int maxBodiesPerCell);
 /* Kept for callers of earlier versions; no effect: the sweep's rows carry
  * their inverse-inertia products (computed once per step with the sweep's own
