@@ -198,16 +198,22 @@ __device__ __forceinline__ int batch_greedy(KP key, PP pay, int n, int staticIdx
 			unsigned long long cand = __ballot(pre);
 			unsigned long long acc = 0ull;
 			const unsigned long long validMask = __ballot(valid);
+			/* bitmap indices for the in-chunk conflict test: two contacts
+			 * conflict when a dynamic body of one has the index of a dynamic
+			 * body of the other; static bodies take sentinels that match
+			 * nothing (4096 on this lane, 8192 as the accepted one) */
+			const unsigned myA = aStatic ? 4096u : (unsigned)ia, myB = bStatic ? 4096u : (unsigned)ib;
+			const unsigned pk = (aStatic ? 8192u : (unsigned)ia) | ((bStatic ? 8192u : (unsigned)ib) << 16);
 			while (cand)
 			{
-				/* j is the same on every lane: read its values with readlane */
+				/* j is the same on every lane: read its indices with one readlane */
 				int j = __ffsll((long long)cand) - 1;
 				acc |= 1ull << j;
-				int jwa = __builtin_amdgcn_readlane(wa, j), jwb = __builtin_amdgcn_readlane(wb, j);
-				unsigned jma = (unsigned)__builtin_amdgcn_readlane((int)ma, j);
-				unsigned jmb = (unsigned)__builtin_amdgcn_readlane((int)mb, j);
-				if (lane == jwa) word |= jma;
-				if (lane == jwb) word |= jmb;
+				const unsigned jp = (unsigned)__builtin_amdgcn_readlane((int)pk, j);
+				const unsigned jA = jp & 0xffffu, jB = jp >> 16;
+				/* a sentinel's word (256) is no lane */
+				if (lane == (int)(jA >> 5)) word |= 1u << (jA & 31u);
+				if (lane == (int)(jB >> 5)) word |= 1u << (jB & 31u);
 				nCur++;
 				unsigned long long later = (j == 63) ? 0ull : (~0ull << (j + 1));
 				if (nCur == SIMD_WIDTH)
@@ -219,8 +225,7 @@ __device__ __forceinline__ int batch_greedy(KP key, PP pay, int n, int staticIdx
 				else
 				{
 					/* bitwise, without short-circuit branches */
-					bool hit = ((wa == jwa) & ((ma & jma) != 0u)) | ((wa == jwb) & ((ma & jmb) != 0u)) |
-							   ((wb == jwa) & ((mb & jma) != 0u)) | ((wb == jwb) & ((mb & jmb) != 0u));
+					bool hit = (myA == jA) | (myA == jB) | (myB == jA) | (myB == jB);
 					cand &= later & ~__ballot(hit);
 				}
 			}
