@@ -37,30 +37,57 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const unsigned* __res
 	hist[t * numTiles + blockIdx.x] = cnt[t];
 }
 
-/* exclusive scan of a small array (<= a few hundred thousand) by one block */
+/* exclusive scan of a small array (<= a few hundred thousand) by one block:
+ * rounds of 8192 consecutive entries, 8 consecutive per thread (a wavefront
+ * reads 2 KB of contiguous memory per round), a block scan of the thread sums
+ * and a running carry between rounds */
 __global__ __launch_bounds__(1024) void k_scan_single(unsigned* __restrict__ data, int n)
 {
-	__shared__ unsigned part[1024];
-	int t = threadIdx.x;
-	int per = (n + 1023) / 1024;
-	int s = t * per, e = min(n, s + per);
-	unsigned sum = 0;
-	for (int i = s; i < e; i++) sum += data[i];
-	part[t] = sum;
-	__syncthreads();
-	for (int off = 1; off < 1024; off <<= 1)
+	__shared__ unsigned wsum[16];
+	const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+	unsigned carry = 0;
+	for (int base = 0; base < n; base += 8 * 1024)
 	{
-		unsigned v = t >= off ? part[t - off] : 0;
+		const int i0 = base + 8 * t;
+		unsigned v[8];
+#pragma unroll
+		for (int u = 0; u < 8; u++) v[u] = (i0 + u < n) ? data[i0 + u] : 0u;
+		unsigned sum = 0;
+#pragma unroll
+		for (int u = 0; u < 8; u++) sum += v[u];
+		/* inclusive scan of the thread sums: within the wavefront, then over
+		 * the 16 wavefront totals */
+		unsigned x = sum;
+#pragma unroll
+		for (int o = 1; o < 64; o <<= 1)
+		{
+			unsigned y = __shfl_up(x, o);
+			if (lane >= o) x += y;
+		}
+		if (lane == 63) wsum[w] = x;
 		__syncthreads();
-		part[t] += v;
+		if (w == 0)
+		{
+			unsigned y0 = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+			for (int o = 1; o < 16; o <<= 1)
+			{
+				unsigned y = __shfl_up(y0, o);
+				if (lane >= o) y0 += y;
+			}
+			if (lane < 16) wsum[lane] = y0;
+		}
 		__syncthreads();
-	}
-	unsigned run = part[t] - sum;
-	for (int i = s; i < e; i++)
-	{
-		unsigned v = data[i];
-		data[i] = run;
-		run += v;
+		unsigned run = carry + (w > 0 ? wsum[w - 1] : 0u) + x - sum;
+		const unsigned total = wsum[15];
+#pragma unroll
+		for (int u = 0; u < 8; u++)
+		{
+			if (i0 + u < n) data[i0 + u] = run;
+			run += v[u];
+		}
+		carry += total;
+		__syncthreads();
 	}
 }
 
