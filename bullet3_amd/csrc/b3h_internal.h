@@ -63,7 +63,7 @@ struct b3hBroadphaseScratch
 	const int* sceneOf; /* per proxy handle: its scene (NULL: one scene); pairs across scenes are dropped */
 	int2* stash;        /* grid pairs: the first B3H_GRID_STASH pairs of each small proxy (count pass) */
 };
-#define B3H_GRID_STASH 32
+#define B3H_GRID_STASH 64
 #define B3H_LARGE_BRUTE_MAX 32
 void b3h_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
 					const b3hGridParams& gp, int numCells, b3hInt4* pairs, int maxPairs, int* pairCount,
