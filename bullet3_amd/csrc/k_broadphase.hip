@@ -281,26 +281,42 @@ __device__ __forceinline__ int grid_scan_one(int s, int n, const unsigned* __res
 				if (st == -1) continue;
 				int end = st + gp.maxBodiesPerCell;
 				if (end > n) end = n;
-				for (int k = st; k < end; k++)
+				if (st >= end) continue;
+				/* the next entry's key and box are read while this one is
+				 * tested (every index read is below end <= n) */
+				unsigned kk = keys[st];
+				b3hAabb b = sorted[st];
+				for (int k = st;; k++)
 				{
-					if ((int)keys[k] != h) break;
-					if (k == s) continue;
-					b3hAabb b = sorted[k];
-					if (!bp_overlap(a, b)) continue;
-					int hb = bp_handle(b);
-					if (ha < hb && bp_same_scene(sceneOf, ha, hb))
+					if ((int)kk != h) break;
+					const bool more = k + 1 < end;
+					unsigned kkN = 0u;
+					b3hAabb bN = b;
+					if (more)
 					{
-						if (MODE == 0)
-						{
-							if (c < B3H_GRID_STASH) stash[(size_t)s * B3H_GRID_STASH + c] = make_int2(ha, hb);
-						}
-						else
-						{
-							int at = o + c;
-							if (at < maxPairs) pairs[at] = b3hInt4{ha, hb, -1, -1};
-						}
-						c++;
+						kkN = keys[k + 1];
+						bN = sorted[k + 1];
 					}
+					if (k != s && bp_overlap(a, b))
+					{
+						int hb = bp_handle(b);
+						if (ha < hb && bp_same_scene(sceneOf, ha, hb))
+						{
+							if (MODE == 0)
+							{
+								if (c < B3H_GRID_STASH) stash[(size_t)s * B3H_GRID_STASH + c] = make_int2(ha, hb);
+							}
+							else
+							{
+								int at = o + c;
+								if (at < maxPairs) pairs[at] = b3hInt4{ha, hb, -1, -1};
+							}
+							c++;
+						}
+					}
+					if (!more) break;
+					kk = kkN;
+					b = bN;
 				}
 			}
 	return c;
