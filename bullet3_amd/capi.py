@@ -39,6 +39,15 @@ class SolverParams(ctypes.Structure):
                 ("m_angularDamping", ctypes.c_float), ("m_warmStartMode", ctypes.c_int)]
 
 
+class ConvexUtility(ctypes.Structure):
+    """b3hConvexUtility: a precomputed polyhedron (pointers to host arrays)."""
+    _fields_ = [("vertices", ctypes.c_void_p), ("numVertices", ctypes.c_int), ("numFaces", ctypes.c_int),
+                ("facePlanes", ctypes.c_void_p), ("faceIndexCounts", ctypes.c_void_p), ("indices", ctypes.c_void_p),
+                ("uniqueEdges", ctypes.c_void_p), ("numUniqueEdges", ctypes.c_int), ("pad", ctypes.c_int),
+                ("mC", ctypes.c_float * 4), ("mE", ctypes.c_float * 4), ("m_extents", ctypes.c_float * 4),
+                ("m_radius", ctypes.c_float), ("localCenterOut", ctypes.c_void_p)]
+
+
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _f = ctypes.c_float
@@ -54,6 +63,16 @@ _SIGS = {
     "b3h_np_create": ([ctypes.POINTER(Config), _vp, _pvp], _i),
     "b3h_np_destroy": ([_vp], _i),
     "b3h_np_register_convex_hull_shape": ([_vp, _vp, _i, _i, _vp, _pi], _i),
+    "b3h_np_register_convex_polyhedron": ([_vp, ctypes.POINTER(ConvexUtility), _pi], _i),
+    "b3h_np_load_shape_tables": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "b3h_np_get_mpr_debug": ([_vp, _pvp], _i),
+    "b3h_bp_calculate_overlapping_pairs_host": ([_vp, _i], _i),
+    "b3h_bp_get_all_aabbs_gpu": ([_vp, _pvp, _pi], _i),
+    "b3h_bp_get_all_aabbs_cpu": ([_vp, _vp, _i, _pi], _i),
+    "b3h_bp_get_small_aabb_indices_gpu": ([_vp, _pvp, _pi], _i),
+    "b3h_bp_get_large_aabb_indices_gpu": ([_vp, _pvp, _pi], _i),
+    "b3h_pipeline_register_convex_polyhedron": ([_vp, ctypes.POINTER(ConvexUtility), _pi], _i),
+    "b3h_compute_body_registration": ([_vp, _vp, _f, _vp, _vp, _vp, _vp, _vp], _i),
     "b3h_np_register_rigid_body": ([_vp, _i, _f, _vp, _vp, _vp, _vp, _i, _pi], _i),
     "b3h_np_write_all_bodies_to_gpu": ([_vp], _i),
     "b3h_np_reset": ([_vp], _i),
@@ -108,7 +127,6 @@ _SIGS = {
     "b3h_pipeline_get_pgs_parts": ([_vp, _vp, _vp], _i),
     "b3h_pipeline_set_pgs_max_parts": ([_vp, _i], _i),
     "b3h_pipeline_autotune": ([_vp, _f, _i, _pi, _pi, _vp], _i),
-    "b3h_pipeline_set_pgs_inertia_cache": ([_vp, _i], _i),
     "b3h_pipeline_set_track_residual": ([_vp, _i], _i),
     "b3h_pipeline_get_residuals": ([_vp, _vp, _i], _i),
     "b3h_pipeline_get_residuals_l2": ([_vp, _vp, _i], _i),
@@ -267,6 +285,45 @@ def polyhedral_features(points, cap=4096):
         faces.append(idx[off:off + fc[f]].copy())
         off += fc[f]
     return dict(vertices=v[:nv, :3].copy(), planes=p[:nf].copy(), faces=faces, edges=e[:ne, :3].copy())
+
+
+def register_convex_polyhedron(np_handle, tables, pipeline=None):
+    """registerConvexHullShape(b3ConvexUtility*) with precomputed tables
+    {vertices (n,3|4), planes (f,4), faces [index arrays], edges (e,3|4)} and
+    optional mC / mE / extents (4 floats) / radius; returns the collidable
+    index (through the pipeline's registerConvexPolyhedron when `pipeline` is
+    given)."""
+    def f4(a):
+        a = np.asarray(a, np.float32).reshape(len(a), -1)
+        out = np.zeros((len(a), 4), np.float32)
+        out[:, :a.shape[1]] = a[:, :4]
+        return np.ascontiguousarray(out)
+
+    v = f4(tables["vertices"])
+    p = np.ascontiguousarray(tables["planes"], np.float32).reshape(-1, 4)
+    fc = np.array([len(f) for f in tables["faces"]], np.int32)
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(f, np.int32) for f in tables["faces"]])
+                               if len(fc) else np.zeros(0, np.int32), np.int32)
+    e = f4(tables["edges"]) if len(tables["edges"]) else np.zeros((0, 4), np.float32)
+    center = np.zeros(4, np.float32)
+    u = ConvexUtility()
+    u.vertices, u.numVertices = v.ctypes.data, len(v)
+    u.numFaces, u.facePlanes, u.faceIndexCounts = len(fc), p.ctypes.data, fc.ctypes.data
+    u.indices = idx.ctypes.data
+    u.uniqueEdges, u.numUniqueEdges = (e.ctypes.data if len(e) else None), len(e)
+    for name in ("mC", "mE", "m_extents"):
+        val = tables.get(name)
+        if val is not None:
+            getattr(u, name)[:] = [float(x) for x in val]
+    u.m_radius = float(tables.get("radius", 0.0))
+    u.localCenterOut = center.ctypes.data
+    out = ctypes.c_int(-1)
+    if pipeline is not None:
+        call("b3h_pipeline_register_convex_polyhedron", pipeline, ctypes.byref(u), ctypes.byref(out))
+    else:
+        call("b3h_np_register_convex_polyhedron", np_handle, ctypes.byref(u), ctypes.byref(out))
+    tables["localCenter"] = center
+    return out.value
 
 
 def radix_sort32(keys, values, sort_bits=32):

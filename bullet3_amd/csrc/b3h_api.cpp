@@ -157,6 +157,9 @@ struct b3hNarrowPhase
 	b3hFloat4* dMpr; /* per pair: MPR direction, point, (depth, hit) */
 	bool contactsValid;
 	int lanesPerPair;
+	/* bumped by every call that can change what a captured step refers to
+	 * (tables, their device copies, bodies, launch choices) */
+	unsigned gen;
 };
 
 static int np_alloc_work(b3hNarrowPhase* np)
@@ -247,16 +250,112 @@ extern "C" int b3h_np_destroy(b3hNarrowPhase* np)
 	return B3H_OK;
 }
 
-extern "C" int b3h_np_register_convex_hull_shape(b3hNarrowPhase* np, const float* vertices, int strideInBytes,
-												  int numVertices, const float* scaling3, int* outCollidable)
+/* One polyhedron into the shape tables (b3GpuNarrowPhase::registerConvexHullShape(b3ConvexUtility*)
+ * and registerConvexHullShapeInternal, b3GpuNarrowPhase.cpp:233-296, 321-368): the record copies the
+ * caller's mC / mE / m_extents / m_radius, the local centre is the vertex mean (written back through
+ * centerOut, as the reference writes it into the utility), and the local AABB bounds the vertices. */
+static int np_add_polyhedron(b3hNarrowPhase* np, int nv, const float* v, int vstride, int nf, const float* planes4,
+							 const int* faceCounts, const int* indices, int ne, const float* e, int estride,
+							 const float* mC4, const float* mE4, const float* extents4, float radius,
+							 float* centerOut4, int* outCollidable)
 {
-	CHECK_ARG(np && vertices && numVertices > 0 && strideInBytes >= 12 && (strideInBytes % 4) == 0 && outCollidable);
+	np->gen++;
 	*outCollidable = -1;
 	if ((int)np->cols.size() >= np->cfg.m_maxConvexShapes)
 	{
 		b3h_set_error("registerConvexHullShape: exceeding the number of shapes (%d)", np->cfg.m_maxConvexShapes);
 		return B3H_ERR_CAPACITY;
 	}
+	if ((int)np->verts.size() + nv > np->cfg.m_maxConvexVertices)
+	{
+		b3h_set_error("registerConvexHullShape: exceeding the number of hull vertices (%d)", np->cfg.m_maxConvexVertices);
+		return B3H_ERR_CAPACITY;
+	}
+	size_t ni = 0;
+	for (int f = 0; f < nf; f++)
+	{
+		if (faceCounts[f] < 0)
+		{
+			b3h_set_error("registerConvexHullShape: negative index count on face %d", f);
+			return B3H_ERR_INVALID;
+		}
+		ni += (size_t)faceCounts[f];
+	}
+	for (size_t i = 0; i < ni; i++)
+		if (indices[i] < 0 || indices[i] >= nv)
+		{
+			b3h_set_error("registerConvexHullShape: face index %d out of range (%d vertices)", indices[i], nv);
+			return B3H_ERR_INVALID;
+		}
+	b3hConvexPolyhedronData cv;
+	memset(&cv, 0, sizeof(cv));
+	if (mC4) cv.mC = f4(mC4[0], mC4[1], mC4[2], mC4[3]);
+	if (mE4) cv.mE = f4(mE4[0], mE4[1], mE4[2], mE4[3]);
+	if (extents4) cv.m_extents = f4(extents4[0], extents4[1], extents4[2], extents4[3]);
+	cv.m_radius = radius;
+	/* local centre = vertex mean, accumulated in order (b3GpuNarrowPhase.cpp:330-336) */
+	float c[3] = {0.f, 0.f, 0.f};
+	for (int i = 0; i < nv; i++)
+		for (int k = 0; k < 3; k++) c[k] += v[(size_t)i * vstride + k];
+	float inv = 1.f / (float)nv;
+	cv.m_localCenter = f4(c[0] * inv, c[1] * inv, c[2] * inv, 0.f);
+	if (centerOut4)
+	{
+		centerOut4[0] = cv.m_localCenter.x;
+		centerOut4[1] = cv.m_localCenter.y;
+		centerOut4[2] = cv.m_localCenter.z;
+		centerOut4[3] = 0.f;
+	}
+	cv.m_uniqueEdgesOffset = (int)np->edges.size();
+	cv.m_numUniqueEdges = ne;
+	for (int i = 0; i < ne; i++)
+		np->edges.push_back(f4(e[(size_t)i * estride], e[(size_t)i * estride + 1], e[(size_t)i * estride + 2], 0.f));
+	cv.m_faceOffset = (int)np->faces.size();
+	cv.m_numFaces = nf;
+	size_t w = 0;
+	for (int f = 0; f < nf; f++)
+	{
+		b3hGpuFace g;
+		memset(&g, 0, sizeof(g));
+		g.m_plane = f4(planes4[4 * f], planes4[4 * f + 1], planes4[4 * f + 2], planes4[4 * f + 3]);
+		g.m_indexOffset = (int)np->indices.size();
+		g.m_numIndices = faceCounts[f];
+		for (int k = 0; k < faceCounts[f]; k++) np->indices.push_back(indices[w++]);
+		np->faces.push_back(g);
+	}
+	cv.m_vertexOffset = (int)np->verts.size();
+	cv.m_numVertices = nv;
+	for (int i = 0; i < nv; i++)
+		np->verts.push_back(f4(v[(size_t)i * vstride], v[(size_t)i * vstride + 1], v[(size_t)i * vstride + 2], 0.f));
+	b3hCollidable col;
+	col.m_numChildShapes = 0;
+	col.m_radius = 0.f;
+	col.m_shapeType = B3H_SHAPE_CONVEX_HULL;
+	col.m_shapeIndex = (int)np->convexes.size();
+	np->convexes.push_back(cv);
+	/* local AABB of the vertices, setMin / setMax (b3GpuNarrowPhase.cpp:341-365) */
+	float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+	for (int i = 0; i < nv; i++)
+		for (int k = 0; k < 3; k++)
+		{
+			float x = v[(size_t)i * vstride + k];
+			if (x < mn[k]) mn[k] = x;
+			if (mx[k] < x) mx[k] = x;
+		}
+	b3hAabb la;
+	la.m_min = f4(mn[0], mn[1], mn[2], 0.f);
+	la.m_max = f4(mx[0], mx[1], mx[2], 0.f);
+	np->localAabbs.push_back(la);
+	*outCollidable = (int)np->cols.size();
+	np->cols.push_back(col);
+	return B3H_OK;
+}
+
+extern "C" int b3h_np_register_convex_hull_shape(b3hNarrowPhase* np, const float* vertices, int strideInBytes,
+												  int numVertices, const float* scaling3, int* outCollidable)
+{
+	CHECK_ARG(np && vertices && numVertices > 0 && strideInBytes >= 12 && (strideInBytes % 4) == 0 && outCollidable);
+	*outCollidable = -1;
 	float sc[3] = {1.f, 1.f, 1.f};
 	if (scaling3)
 		for (int k = 0; k < 3; k++) sc[k] = scaling3[k];
@@ -267,65 +366,86 @@ extern "C" int b3h_np_register_convex_hull_shape(b3hNarrowPhase* np, const float
 	b3hPolyhedron poly;
 	if (b3h_build_polyhedron(pts.data(), numVertices, 3, &poly) != 0)
 	{
-		b3h_set_error("registerConvexHullShape: degenerate point set (need >= 4 non-coplanar points)");
+		b3h_set_error("registerConvexHullShape: degenerate point set (fewer than 3 distinct, non-collinear points)");
 		return B3H_ERR_INVALID;
 	}
-	int nv = (int)poly.vertices.size() / 3;
-	if ((int)np->verts.size() + nv > np->cfg.m_maxConvexVertices)
+	int nv = (int)poly.vertices.size() / 3, nf = (int)poly.faces.size(), ne = (int)poly.uniqueEdges.size() / 3;
+	std::vector<float> planes(4 * (size_t)nf);
+	std::vector<int> counts(nf), idx;
+	for (int f = 0; f < nf; f++)
 	{
-		b3h_set_error("registerConvexHullShape: exceeding the number of hull vertices (%d)", np->cfg.m_maxConvexVertices);
+		for (int k = 0; k < 4; k++) planes[4 * f + k] = poly.faces[f].plane[k];
+		counts[f] = (int)poly.faces[f].indices.size();
+		idx.insert(idx.end(), poly.faces[f].indices.begin(), poly.faces[f].indices.end());
+	}
+	/* b3ConvexUtility leaves mC, mE, m_extents and m_radius unset unless built with
+	 * TEST_INTERNAL_OBJECTS (b3ConvexUtility.cpp:399-470); no kernel reads them: zero */
+	return np_add_polyhedron(np, nv, poly.vertices.data(), 3, nf, planes.data(), counts.data(), idx.data(), ne,
+							 poly.uniqueEdges.data(), 3, nullptr, nullptr, nullptr, 0.f, nullptr, outCollidable);
+}
+
+extern "C" int b3h_np_register_convex_polyhedron(b3hNarrowPhase* np, const b3hConvexUtility* u, int* outCollidable)
+{
+	CHECK_ARG(np && u && outCollidable && u->numVertices > 0 && u->vertices && u->numFaces >= 0 &&
+			  u->numUniqueEdges >= 0);
+	if (u->numFaces > 0) CHECK_ARG(u->facePlanes && u->faceIndexCounts && u->indices);
+	if (u->numUniqueEdges > 0) CHECK_ARG(u->uniqueEdges);
+	return np_add_polyhedron(np, u->numVertices, &u->vertices[0].x, 4, u->numFaces, &u->facePlanes[0].x,
+							 u->faceIndexCounts, u->indices, u->numUniqueEdges,
+							 u->numUniqueEdges ? &u->uniqueEdges[0].x : nullptr, 4, &u->mC.x, &u->mE.x,
+							 &u->m_extents.x, u->m_radius, u->localCenterOut, outCollidable);
+}
+
+extern "C" int b3h_np_load_shape_tables(b3hNarrowPhase* np, const int* counts, const b3hCollidable* cols,
+										const b3hConvexPolyhedronData* cv, const b3hFloat4* verts,
+										const b3hFloat4* edges, const b3hGpuFace* faces, const int* indices,
+										const b3hAabb* la)
+{
+	CHECK_ARG(np && counts);
+	np->gen++;
+	for (int k = 0; k < 7; k++) CHECK_ARG(counts[k] >= 0);
+	CHECK_ARG((counts[0] == 0 || cols) && (counts[1] == 0 || cv) && (counts[2] == 0 || verts) &&
+			  (counts[3] == 0 || edges) && (counts[4] == 0 || faces) && (counts[5] == 0 || indices) &&
+			  (counts[6] == 0 || la) && counts[6] == counts[0]);
+	if (counts[0] > np->cfg.m_maxConvexShapes || counts[2] > np->cfg.m_maxConvexVertices)
+	{
+		b3h_set_error("load_shape_tables: %d shapes / %d vertices exceed the configured capacity", counts[0], counts[2]);
 		return B3H_ERR_CAPACITY;
 	}
-	b3hConvexPolyhedronData cv;
-	memset(&cv, 0, sizeof(cv));
-	/* local centre = vertex mean (b3GpuNarrowPhase.cpp:330-336) */
-	float c[3] = {0.f, 0.f, 0.f};
-	for (int i = 0; i < nv; i++)
-		for (int k = 0; k < 3; k++) c[k] += poly.vertices[3 * i + k];
-	float inv = 1.f / (float)nv;
-	cv.m_localCenter = f4(c[0] * inv, c[1] * inv, c[2] * inv, 0.f);
-	cv.m_radius = 0.f;
-	cv.m_uniqueEdgesOffset = (int)np->edges.size();
-	cv.m_numUniqueEdges = (int)poly.uniqueEdges.size() / 3;
-	for (int i = 0; i < cv.m_numUniqueEdges; i++)
-		np->edges.push_back(f4(poly.uniqueEdges[3 * i], poly.uniqueEdges[3 * i + 1], poly.uniqueEdges[3 * i + 2], 0.f));
-	cv.m_faceOffset = (int)np->faces.size();
-	cv.m_numFaces = (int)poly.faces.size();
-	for (auto& f : poly.faces)
-	{
-		b3hGpuFace g;
-		memset(&g, 0, sizeof(g));
-		g.m_plane = f4(f.plane[0], f.plane[1], f.plane[2], f.plane[3]);
-		g.m_indexOffset = (int)np->indices.size();
-		g.m_numIndices = (int)f.indices.size();
-		for (int id : f.indices) np->indices.push_back(id);
-		np->faces.push_back(g);
-	}
-	cv.m_vertexOffset = (int)np->verts.size();
-	cv.m_numVertices = nv;
-	for (int i = 0; i < nv; i++)
-		np->verts.push_back(f4(poly.vertices[3 * i], poly.vertices[3 * i + 1], poly.vertices[3 * i + 2], 0.f));
-	b3hCollidable col;
-	col.m_numChildShapes = 0;
-	col.m_radius = 0.f;
-	col.m_shapeType = B3H_SHAPE_CONVEX_HULL;
-	col.m_shapeIndex = (int)np->convexes.size();
-	np->convexes.push_back(cv);
-	/* local AABB of the hull vertices (b3GpuNarrowPhase.cpp:341-365) */
-	float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
-	for (int i = 0; i < nv; i++)
-		for (int k = 0; k < 3; k++)
+	/* every offset a kernel follows must stay inside the tables it indexes */
+	for (int i = 0; i < counts[0]; i++)
+		if (cols[i].m_shapeType == B3H_SHAPE_CONVEX_HULL && (cols[i].m_shapeIndex < 0 || cols[i].m_shapeIndex >= counts[1]))
 		{
-			float v = poly.vertices[3 * i + k];
-			mn[k] = v < mn[k] ? v : mn[k];
-			mx[k] = v > mx[k] ? v : mx[k];
+			b3h_set_error("load_shape_tables: collidable %d names hull %d of %d", i, cols[i].m_shapeIndex, counts[1]);
+			return B3H_ERR_INVALID;
 		}
-	b3hAabb la;
-	la.m_min = f4(mn[0], mn[1], mn[2], 0.f);
-	la.m_max = f4(mx[0], mx[1], mx[2], 0.f);
-	np->localAabbs.push_back(la);
-	*outCollidable = (int)np->cols.size();
-	np->cols.push_back(col);
+	for (int i = 0; i < counts[1]; i++)
+	{
+		const b3hConvexPolyhedronData& c = cv[i];
+		bool ok = c.m_numVertices > 0 && c.m_vertexOffset >= 0 && c.m_vertexOffset + c.m_numVertices <= counts[2] &&
+				  c.m_numFaces >= 0 && c.m_faceOffset >= 0 && c.m_faceOffset + c.m_numFaces <= counts[4] &&
+				  c.m_numUniqueEdges >= 0 && c.m_uniqueEdgesOffset >= 0 &&
+				  c.m_uniqueEdgesOffset + c.m_numUniqueEdges <= counts[3];
+		for (int f = 0; ok && f < c.m_numFaces; f++)
+		{
+			const b3hGpuFace& g = faces[c.m_faceOffset + f];
+			ok = g.m_numIndices >= 0 && g.m_indexOffset >= 0 && g.m_indexOffset + g.m_numIndices <= counts[5];
+			for (int k = 0; ok && k < g.m_numIndices; k++)
+				ok = indices[g.m_indexOffset + k] >= 0 && indices[g.m_indexOffset + k] < c.m_numVertices;
+		}
+		if (!ok)
+		{
+			b3h_set_error("load_shape_tables: hull %d has offsets or indices outside the tables", i);
+			return B3H_ERR_INVALID;
+		}
+	}
+	np->cols.assign(cols, cols + counts[0]);
+	np->convexes.assign(cv, cv + counts[1]);
+	np->verts.assign(verts, verts + counts[2]);
+	np->edges.assign(edges, edges + counts[3]);
+	np->faces.assign(faces, faces + counts[4]);
+	np->indices.assign(indices, indices + counts[5]);
+	np->localAabbs.assign(la, la + counts[6]);
 	return B3H_OK;
 }
 
@@ -349,11 +469,41 @@ static void quat_to_basis(const float q[4], float m[3][3])
 	m[2][2] = 1.0f - (xx + yy);
 }
 
+/* inertia record of a registered body: a box of edges 2*(aabbMax-aabbMin)
+ * (the AABB already carries the 0.01 margin), world inverse inertia
+ * R diag(1/I) R^T (b3GpuNarrowPhase.cpp:860-901) */
+static void body_inertia(float mass, const float* aabbMin, const float* aabbMax, const float* orn, b3hInertiaData* in)
+{
+	memset(in, 0, sizeof(*in));
+	if (mass == 0.f) return;
+	float he[3] = {aabbMax[0] - aabbMin[0], aabbMax[1] - aabbMin[1], aabbMax[2] - aabbMin[2]};
+	float lx = 2.f * he[0], ly = 2.f * he[1], lz = 2.f * he[2];
+	float li[3] = {(mass / 12.0f) * (ly * ly + lz * lz), (mass / 12.0f) * (lx * lx + lz * lz),
+				   (mass / 12.0f) * (lx * lx + ly * ly)};
+	float il[3] = {1.f / li[0], 1.f / li[1], 1.f / li[2]};
+	in->m_initInvInertia.m_row[0] = f4(il[0], 0, 0, 0);
+	in->m_initInvInertia.m_row[1] = f4(0, il[1], 0, 0);
+	in->m_initInvInertia.m_row[2] = f4(0, 0, il[2], 0);
+	float m[3][3];
+	quat_to_basis(orn, m);
+	float ms[3][3];
+	for (int r = 0; r < 3; r++)
+		for (int k = 0; k < 3; k++) ms[r][k] = m[r][k] * il[k];
+	/* (m.scaled(il)) * m^T: row r, column k = ms[r] . m[k] (b3Matrix3x3 operator*, via tdotx/y/z of m^T) */
+	for (int r = 0; r < 3; r++)
+	{
+		float w[3];
+		for (int k = 0; k < 3; k++) w[k] = m[k][0] * ms[r][0] + m[k][1] * ms[r][1] + m[k][2] * ms[r][2];
+		in->m_invInertiaWorld.m_row[r] = f4(w[0], w[1], w[2], 0.f);
+	}
+}
+
 extern "C" int b3h_np_register_rigid_body(b3hNarrowPhase* np, int collidableIndex, float mass, const float* pos,
 										  const float* orn, const float* aabbMin, const float* aabbMax, int writeToGpu,
 										  int* outBody)
 {
 	CHECK_ARG(np && pos && orn && aabbMin && aabbMax && outBody);
+	np->gen++;
 	*outBody = -1;
 	int idx = (int)np->bodies.size();
 	if (idx >= np->cfg.m_maxConvexBodies)
@@ -372,39 +522,8 @@ extern "C" int b3h_np_register_rigid_body(b3hNarrowPhase* np, int collidableInde
 	b.m_collidableIdx = collidableIndex;
 	b.m_invMass = mass ? 1.f / mass : 0.f;
 	b3hInertiaData in;
-	memset(&in, 0, sizeof(in));
-	if (mass == 0.f)
-	{
-		if (idx == 0) np->static0Index = 0;
-	}
-	else
-	{
-		/* inertia of a box with edges 2*(aabbMax-aabbMin) (b3GpuNarrowPhase.cpp:876-901) */
-		float he[3] = {aabbMax[0] - aabbMin[0], aabbMax[1] - aabbMin[1], aabbMax[2] - aabbMin[2]};
-		float lx = 2.f * he[0], ly = 2.f * he[1], lz = 2.f * he[2];
-		float li[3] = {(mass / 12.0f) * (ly * ly + lz * lz), (mass / 12.0f) * (lx * lx + lz * lz),
-					   (mass / 12.0f) * (lx * lx + ly * ly)};
-		float il[3] = {1.f / li[0], 1.f / li[1], 1.f / li[2]};
-		in.m_initInvInertia.m_row[0] = f4(il[0], 0, 0, 0);
-		in.m_initInvInertia.m_row[1] = f4(0, il[1], 0, 0);
-		in.m_initInvInertia.m_row[2] = f4(0, 0, il[2], 0);
-		float m[3][3];
-		quat_to_basis(orn, m);
-		float ms[3][3];
-		for (int r = 0; r < 3; r++)
-			for (int k = 0; k < 3; k++) ms[r][k] = m[r][k] * il[k];
-		/* (m.scaled(il)) * m^T, via tdotx/tdoty/tdotz of m^T (b3Matrix3x3.h:1190-1230) */
-		for (int r = 0; r < 3; r++)
-		{
-			float w[3];
-			for (int k = 0; k < 3; k++)
-			{
-				/* (m^T).tdot_k(row) = m^T[0][k]*row.x + m^T[1][k]*row.y + m^T[2][k]*row.z = m[k][0]*x + m[k][1]*y + m[k][2]*z */
-				w[k] = m[k][0] * ms[r][0] + m[k][1] * ms[r][1] + m[k][2] * ms[r][2];
-			}
-			in.m_invInertiaWorld.m_row[r] = f4(w[0], w[1], w[2], 0.f);
-		}
-	}
+	if (mass == 0.f && idx == 0) np->static0Index = 0;
+	body_inertia(mass, aabbMin, aabbMax, orn, &in);
 	np->bodies.push_back(b);
 	np->inertias.push_back(in);
 	if (writeToGpu)
@@ -434,6 +553,7 @@ static int upload_table(T** d, size_t* cap, const std::vector<T>& h, hipStream_t
 extern "C" int b3h_np_write_all_bodies_to_gpu(b3hNarrowPhase* np)
 {
 	CHECK_ARG(np);
+	np->gen++;
 	int r;
 	if ((r = upload_table(&np->dCols, &np->capCols, np->cols, np->stream))) return r;
 	if ((r = upload_table(&np->dLocalAabbs, &np->capLocal, np->localAabbs, np->stream))) return r;
@@ -456,6 +576,7 @@ extern "C" int b3h_np_write_all_bodies_to_gpu(b3hNarrowPhase* np)
 extern "C" int b3h_np_reset(b3hNarrowPhase* np)
 {
 	CHECK_ARG(np);
+	np->gen++;
 	np->cols.clear();
 	np->convexes.clear();
 	np->verts.clear();
@@ -483,6 +604,7 @@ extern "C" int b3h_np_readback_all_bodies_to_cpu(b3hNarrowPhase* np, b3hRigidBod
 extern "C" int b3h_np_set_object_transform_cpu(b3hNarrowPhase* np, const float* p, const float* o, int i)
 {
 	CHECK_ARG(np && p && o && i >= 0 && i < (int)np->bodies.size());
+	np->gen++;
 	np->bodies[i].m_pos = f4(p[0], p[1], p[2], 0.f);
 	np->bodies[i].m_quat = f4(o[0], o[1], o[2], o[3]);
 	return B3H_OK;
@@ -490,6 +612,7 @@ extern "C" int b3h_np_set_object_transform_cpu(b3hNarrowPhase* np, const float* 
 extern "C" int b3h_np_set_object_velocity_cpu(b3hNarrowPhase* np, const float* lv, const float* av, int i)
 {
 	CHECK_ARG(np && lv && av && i >= 0 && i < (int)np->bodies.size());
+	np->gen++;
 	np->bodies[i].m_linVel = f4(lv[0], lv[1], lv[2], 0.f);
 	np->bodies[i].m_angVel = f4(av[0], av[1], av[2], 0.f);
 	return B3H_OK;
@@ -628,7 +751,14 @@ extern "C" int b3h_np_get_inertias_cpu(b3hNarrowPhase* np, b3hInertiaData* dst, 
 extern "C" int b3h_np_set_pairs_per_group(b3hNarrowPhase* np, int lanes)
 {
 	CHECK_ARG(np && lanes >= 1 && lanes <= 64 && (lanes & (lanes - 1)) == 0);
+	np->gen++;
 	np->lanesPerPair = lanes;
+	return B3H_OK;
+}
+extern "C" int b3h_np_get_mpr_debug(b3hNarrowPhase* np, b3hFloat4** mpr)
+{
+	CHECK_ARG(np && mpr);
+	*mpr = np->dMpr;
 	return B3H_OK;
 }
 extern "C" int b3h_np_get_sat_debug(b3hNarrowPhase* np, int** st, b3hFloat4** sn)
@@ -660,6 +790,7 @@ struct b3hBroadphase
 	int scCapSmall;
 	int scCapLarge;
 	int scCells;
+	unsigned gen; /* as b3hNarrowPhase::gen: proxies, grid parameters, scratch reallocation */
 };
 
 static void bp_free_scratch(b3hBroadphase* bp)
@@ -750,6 +881,7 @@ extern "C" int b3h_bp_destroy(b3hBroadphase* bp)
 extern "C" int b3h_bp_set_grid_params(b3hBroadphase* bp, const float* cs, const int* dims, int maxPerCell)
 {
 	CHECK_ARG(bp && cs && dims && maxPerCell > 0);
+	bp->gen++;
 	for (int k = 0; k < 3; k++)
 	{
 		CHECK_ARG(cs[k] > 0.f && dims[k] > 0 && (dims[k] & (dims[k] - 1)) == 0);
@@ -769,6 +901,7 @@ extern "C" int b3h_bp_set_grid_params(b3hBroadphase* bp, const float* cs, const 
 static int bp_proxy(b3hBroadphase* bp, const float* mn, const float* mx, int user, bool large)
 {
 	CHECK_ARG(bp && mn && mx);
+	bp->gen++;
 	if ((int)bp->all.size() >= bp->cfg.m_maxConvexBodies)
 	{
 		b3h_set_error("createProxy: exceeding capacity %d", bp->cfg.m_maxConvexBodies);
@@ -790,6 +923,7 @@ extern "C" int b3h_bp_create_proxy(b3hBroadphase* bp, const float* mn, const flo
 extern "C" int b3h_bp_create_proxies(b3hBroadphase* bp, int n, const float* mn, const float* mx, const int* user)
 {
 	CHECK_ARG(bp && n >= 0 && (n == 0 || (mn && mx)));
+	bp->gen++;
 	if ((long long)bp->all.size() + n > bp->cfg.m_maxConvexBodies)
 	{
 		b3h_set_error("createProxies: %d more proxies exceed capacity %d", n, bp->cfg.m_maxConvexBodies);
@@ -816,6 +950,7 @@ static int bp_ensure_scratch(b3hBroadphase* bp)
 	int ns = (int)bp->smallMap.size();
 	int nl = (int)bp->largeMap.size();
 	if (ns <= bp->scCapSmall && nl <= bp->scCapLarge && bp->scCells == bp->numCells && bp->sc.keys) return B3H_OK;
+	bp->gen++;
 	bp_free_scratch(bp);
 	int cap = ns + 64;
 	int capL = nl + 64;
@@ -841,6 +976,7 @@ static int bp_ensure_scratch(b3hBroadphase* bp)
 extern "C" int b3h_bp_write_aabbs_to_gpu(b3hBroadphase* bp)
 {
 	CHECK_ARG(bp);
+	bp->gen++;
 	int r = bp_ensure_scratch(bp);
 	if (r) return r;
 	if (!bp->all.empty())
@@ -876,6 +1012,101 @@ extern "C" int b3h_bp_calculate_overlapping_pairs(b3hBroadphase* bp, int maxPair
 	int r = bp_calc_pairs(bp, maxPairs, bp->stream);
 	if (r) return r;
 	B3H_CHECK(hipGetLastError());
+	return B3H_OK;
+}
+/* inclusive overlap test on x, y, z (b3TestAabbAgainstAabb2, b3AabbUtil.h; TestAabbAgainstAabb2 in
+ * b3GpuSapBroadphase.cpp) */
+static bool aabb_overlap(const b3hAabb& a, const b3hAabb& b)
+{
+	return !(a.m_min.x > b.m_max.x || a.m_max.x < b.m_min.x) && !(a.m_min.z > b.m_max.z || a.m_max.z < b.m_min.z) &&
+		   !(a.m_min.y > b.m_max.y || a.m_max.y < b.m_min.y);
+}
+
+extern "C" int b3h_bp_calculate_overlapping_pairs_host(b3hBroadphase* bp, int maxPairs)
+{
+	CHECK_ARG(bp && maxPairs >= 0);
+	if (maxPairs > bp->cfg.m_maxBroadphasePairs) maxPairs = bp->cfg.m_maxBroadphasePairs;
+	int n = (int)bp->all.size();
+	std::vector<b3hAabb> a(n);
+	std::vector<int> scene;
+	if (n) B3H_CHECK(hipMemcpyAsync(a.data(), bp->dAll, n * sizeof(b3hAabb), hipMemcpyDeviceToHost, bp->stream));
+	if (bp->sc.sceneOf && n)
+	{
+		scene.resize(n);
+		B3H_CHECK(hipMemcpyAsync(scene.data(), bp->sc.sceneOf, n * sizeof(int), hipMemcpyDeviceToHost, bp->stream));
+	}
+	B3H_CHECK(hipStreamSynchronize(bp->stream));
+	auto user = [&](int h) {
+		int u;
+		memcpy(&u, &a[h].m_min.w, 4);
+		return u;
+	};
+	std::vector<b3hInt4> out;
+	long long total = 0;
+	auto add = [&](int h0, int h1) {
+		if (!scene.empty() && scene[h0] != scene[h1]) return;
+		int x = user(h0), y = user(h1);
+		total++;
+		if ((int)out.size() < maxPairs) out.push_back(x <= y ? b3hInt4{x, y, -1, -1} : b3hInt4{y, x, -1, -1});
+	};
+	if (bp->kind == B3H_BROADPHASE_GRID)
+	{
+		/* every proxy against every later one (b3GpuGridBroadphase.cpp:260-292) */
+		for (int i = 0; i < n; i++)
+			for (int j = i + 1; j < n; j++)
+				if (aabb_overlap(a[i], a[j])) add(j, i);
+	}
+	else
+	{
+		/* small x small, then small x large (b3GpuSapBroadphase.cpp:862-981) */
+		int ns = (int)bp->smallMap.size(), nl = (int)bp->largeMap.size();
+		for (int i = 0; i < ns; i++)
+			for (int j = i + 1; j < ns; j++)
+				if (aabb_overlap(a[bp->smallMap[i]], a[bp->smallMap[j]])) add(bp->smallMap[i], bp->smallMap[j]);
+		for (int i = 0; i < ns; i++)
+			for (int j = 0; j < nl; j++)
+				if (aabb_overlap(a[bp->smallMap[i]], a[bp->largeMap[j]])) add(bp->largeMap[j], bp->smallMap[i]);
+	}
+	int cnt[2] = {(int)out.size(), (int)(total > 0x7fffffff ? 0x7fffffff : total)};
+	if (!out.empty())
+		B3H_CHECK(hipMemcpyAsync(bp->dPairs, out.data(), out.size() * sizeof(b3hInt4), hipMemcpyHostToDevice, bp->stream));
+	B3H_CHECK(hipMemcpyAsync(bp->dPairCount, cnt, sizeof(cnt), hipMemcpyHostToDevice, bp->stream));
+	B3H_CHECK(hipStreamSynchronize(bp->stream));
+	return B3H_OK;
+}
+
+extern "C" int b3h_bp_get_all_aabbs_gpu(b3hBroadphase* bp, b3hAabb** dev, int* n)
+{
+	CHECK_ARG(bp && dev);
+	*dev = bp->dAll;
+	if (n) *n = (int)bp->all.size();
+	return B3H_OK;
+}
+extern "C" int b3h_bp_get_all_aabbs_cpu(b3hBroadphase* bp, b3hAabb* host, int capacity, int* n)
+{
+	CHECK_ARG(bp && n);
+	*n = (int)bp->all.size();
+	if (!host) return B3H_OK;
+	if (capacity < *n)
+	{
+		b3h_set_error("getAllAabbsCPU: capacity %d < %d proxies", capacity, *n);
+		return B3H_ERR_CAPACITY;
+	}
+	if (*n) memcpy(host, bp->all.data(), (size_t)*n * sizeof(b3hAabb));
+	return B3H_OK;
+}
+extern "C" int b3h_bp_get_small_aabb_indices_gpu(b3hBroadphase* bp, int** dev, int* n)
+{
+	CHECK_ARG(bp && dev);
+	*dev = bp->dSmall;
+	if (n) *n = (int)bp->smallMap.size();
+	return B3H_OK;
+}
+extern "C" int b3h_bp_get_large_aabb_indices_gpu(b3hBroadphase* bp, int** dev, int* n)
+{
+	CHECK_ARG(bp && dev);
+	*dev = bp->dLarge;
+	if (n) *n = (int)bp->largeMap.size();
 	return B3H_OK;
 }
 extern "C" int b3h_bp_get_aabb_buffer_ws(b3hBroadphase* bp, b3hAabb** dev, int* n)
@@ -955,6 +1186,7 @@ struct b3hPipeline
 	hipGraphExec_t graphExec;
 	float graphDt;
 	int graphBodies;
+	unsigned graphNpGen, graphBpGen; /* component generations the graph was captured at */
 	float* dWarm;
 	int warmCount;
 	/* warm start and contact logging */
@@ -1295,7 +1527,8 @@ extern "C" int b3h_pipeline_step_graph(b3hPipeline* p, float dt, int count)
 	CHECK_ARG(p && count >= 0);
 	if (num_bodies(p) == 0) return B3H_OK;
 	if (int rs = check_scenes(p)) return rs;
-	if (!p->graphExec || p->graphDt != dt || p->graphBodies != num_bodies(p))
+	if (!p->graphExec || p->graphDt != dt || p->graphBodies != num_bodies(p) || p->graphNpGen != p->np->gen ||
+		p->graphBpGen != p->bp->gen)
 	{
 		if (p->graphExec) hipGraphExecDestroy(p->graphExec);
 		if (p->graph) hipGraphDestroy(p->graph);
@@ -1319,6 +1552,8 @@ extern "C" int b3h_pipeline_step_graph(b3hPipeline* p, float dt, int count)
 		B3H_CHECK(hipGraphInstantiate(&p->graphExec, g, nullptr, nullptr, 0));
 		p->graphDt = dt;
 		p->graphBodies = num_bodies(p);
+		p->graphNpGen = p->np->gen;
+		p->graphBpGen = p->bp->gen;
 	}
 	for (int i = 0; i < count; i++) B3H_CHECK(hipGraphLaunch(p->graphExec, p->stream));
 	return B3H_OK;
@@ -1400,6 +1635,25 @@ extern "C" int b3h_pipeline_register_physics_instance(b3hPipeline* p, float mass
 	return B3H_OK;
 }
 
+extern "C" int b3h_compute_body_registration(const float* localAabbMin3, const float* localAabbMax3, float mass,
+											 const float* pos3, const float* orn4, float* aabbMin3, float* aabbMax3,
+											 b3hInertiaData* inertia)
+{
+	CHECK_ARG(localAabbMin3 && localAabbMax3 && pos3 && orn4 && aabbMin3 && aabbMax3 && inertia);
+	b3hAabb la;
+	la.m_min = f4(localAabbMin3[0], localAabbMin3[1], localAabbMin3[2], 0.f);
+	la.m_max = f4(localAabbMax3[0], localAabbMax3[1], localAabbMax3[2], 0.f);
+	transform_aabb(la, 0.01f, pos3, orn4, aabbMin3, aabbMax3);
+	body_inertia(mass, aabbMin3, aabbMax3, orn4, inertia);
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_register_convex_polyhedron(b3hPipeline* p, const b3hConvexUtility* u, int* outCollidable)
+{
+	CHECK_ARG(p);
+	return b3h_np_register_convex_polyhedron(p->np, u, outCollidable);
+}
+
 extern "C" int b3h_pipeline_write_all_instances_to_gpu(b3hPipeline* p)
 {
 	CHECK_ARG(p);
@@ -1410,6 +1664,7 @@ extern "C" int b3h_pipeline_write_all_instances_to_gpu(b3hPipeline* p)
 extern "C" int b3h_pipeline_set_gravity(b3hPipeline* p, const float* g)
 {
 	CHECK_ARG(p && g);
+	drop_graph(p);
 	p->gravity = f4(g[0], g[1], g[2], 0.f);
 	if (p->graphExec)
 	{
@@ -1421,6 +1676,7 @@ extern "C" int b3h_pipeline_set_gravity(b3hPipeline* p, const float* g)
 extern "C" int b3h_pipeline_reset(b3hPipeline* p)
 {
 	CHECK_ARG(p);
+	drop_graph(p);
 	p->bp->all.clear();
 	p->bp->smallMap.clear();
 	p->bp->largeMap.clear();
@@ -1676,6 +1932,9 @@ extern "C" int b3h_pipeline_autotune(b3hPipeline* p, float dt, int steps, int* l
 	B3H_CHECK(hipEventCreate(&e1));
 	float best = INFINITY;
 	int bestL = p->np->lanesPerPair, bestP = p->sb.pgsMaxParts > 0 ? p->sb.pgsMaxParts : B3H_PGS_PARTS;
+	/* trial steps append nothing to a contact log */
+	bool logSave = p->logRows;
+	p->logRows = false;
 	for (int li = 0; li < 3 && !r; li++)
 		for (int pi = 0; pi < 3 && !r; pi++)
 		{
@@ -1703,7 +1962,9 @@ extern "C" int b3h_pipeline_autotune(b3hPipeline* p, float dt, int steps, int* l
 		}
 	hipEventDestroy(e0);
 	hipEventDestroy(e1);
+	p->logRows = logSave;
 	p->np->lanesPerPair = bestL;
+	p->np->gen++;
 	p->sb.pgsMaxParts = bestP;
 	int r2 = b3h_pipeline_load_state(p, image.data(), n);
 	drop_graph(p);
@@ -1735,6 +1996,7 @@ extern "C" int b3h_pipeline_get_solver_debug(b3hPipeline* p, b3hContact4Data** s
 extern "C" int b3h_pipeline_set_pgs_body_cache(b3hPipeline* p, int maxBodiesPerCell)
 {
 	CHECK_ARG(p);
+	drop_graph(p);
 	p->sb.pgsBodyCap = maxBodiesPerCell;
 	if (p->graphExec)
 	{
@@ -1749,13 +2011,6 @@ extern "C" int b3h_pipeline_set_pgs_body_cache(b3hPipeline* p, int maxBodiesPerC
 	return B3H_OK;
 }
 
-extern "C" int b3h_pipeline_set_pgs_inertia_cache(b3hPipeline* p, int enable)
-{
-	CHECK_ARG(p);
-	p->sb.pgsNoInertiaCache = enable ? 0 : 1;
-	drop_graph(p);
-	return B3H_OK;
-}
 extern "C" int b3h_pipeline_set_pgs_max_parts(b3hPipeline* p, int maxParts)
 {
 	CHECK_ARG(p && maxParts >= 1 && maxParts <= B3H_PGS_PARTS);
@@ -1795,6 +2050,7 @@ extern "C" int b3h_pipeline_get_pgs_parts(b3hPipeline* p, int* parts, int* large
 extern "C" int b3h_pipeline_set_track_residual(b3hPipeline* p, int enable)
 {
 	CHECK_ARG(p);
+	drop_graph(p);
 	p->trackResidual = enable != 0;
 	return B3H_OK;
 }
@@ -1828,6 +2084,7 @@ extern "C" int b3h_pipeline_get_residuals_l2(b3hPipeline* p, float* out, int n)
 extern "C" int b3h_pipeline_set_warm_start_impulses(b3hPipeline* p, const float* dev, int n)
 {
 	CHECK_ARG(p && n >= 0);
+	drop_graph(p);
 	p->dWarm = (float*)dev;
 	p->warmCount = n;
 	if (p->graphExec)

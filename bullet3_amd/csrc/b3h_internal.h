@@ -129,7 +129,6 @@ struct b3hSolverBuffers
 	int* cellBodyCount;   /* 256 */
 	int* cslot;           /* per constraint: local slots of bodyA, bodyB in its cell's list (-1: static) */
 	int pgsBodyCap;       /* < 0: the LDS budget; otherwise a lower cap on cached bodies per cell */
-	int pgsNoInertiaCache; /* set by b3h_pipeline_set_pgs_inertia_cache; no longer read */
 	int pgsMaxParts;       /* work-groups a cell may be split into (0: B3H_PGS_PARTS) */
 	b3hFloat4* recN;      /* 10 fields x maxContacts float4 of normal-row invariants, by level position */
 	b3hFloat4* recF;      /* 8 fields x maxContacts float4 of friction-row invariants, by level position */

@@ -1985,7 +1985,7 @@ void b3h_solve_sweeps(b3hRigidBodyData* bodies, const b3hInertiaData* inertias, 
 		(const int*)sb.partBodies, RecSrc{(const F4*)sb.recN, (const F4*)sb.recF, (F4*)sb.lamN, (F4*)sb.lamF,      \
 										 sb.maxContacts},                                                      \
 		ib,                                                                                                    \
-		prm.m_frictionCoeff, warm_, resid_, (const int*)sb.lastClass, fuse_, ip, sb.pgsNoInertiaCache ? 0 : (int)pgsLds
+		prm.m_frictionCoeff, warm_, resid_, (const int*)sb.lastClass, fuse_, ip, (int)pgsLds
 	if (sb.warmImpulses)
 		for (int ib = 0; ib < B3H_N_CELL_BATCHES; ib++)
 			hipLaunchKernelGGL(k_pgs_level<2>, grid, block, pgsLds, s,

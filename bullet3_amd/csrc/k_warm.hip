@@ -202,21 +202,29 @@ __global__ __launch_bounds__(256) void k_warm_from_net(const int* __restrict__ n
 		float* w = warm + 6 * (size_t)k;
 		float f0 = pr[4], f1 = pr[5];
 		float sf0 = fabsf(c.m_fJacCoeffInv[0]) * velUnit, sf1 = fabsf(c.m_fJacCoeffInv[1]) * velUnit;
+		float v[4];
+		bool finite = isfinite(f0) && isfinite(f1);
 		for (int i = 0; i < 4; i++)
 		{
 			if (i >= np)
 			{
-				w[i] = 0.f;
+				v[i] = 0.f;
 				continue;
 			}
 			const float* y = Y + (size_t)(4 * k + i) * B3H_WARM_OUTPUTS;
 			float si = fabsf(c.m_jacCoeffInv[i]) * velUnit;
-			w[i] = cl_max(pr[i] + si * y[0], 0.f);
+			v[i] = cl_max(pr[i] + si * y[0], 0.f);
 			f0 += sf0 * y[1];
 			f1 += sf1 * y[2];
+			finite = finite && isfinite(v[i]);
 		}
-		w[4] = f0;
-		w[5] = f1;
+		finite = finite && isfinite(f0) && isfinite(f1);
+		/* a non-finite network output falls back to the previous-frame guess
+		 * for the whole constraint (the warm pass then clamps the friction
+		 * rows to the friction bound of the normal impulses, as the sweeps do) */
+		for (int i = 0; i < 4; i++) w[i] = finite ? v[i] : pr[i];
+		w[4] = finite ? f0 : pr[4];
+		w[5] = finite ? f1 : pr[5];
 	}
 }
 

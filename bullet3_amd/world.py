@@ -20,7 +20,11 @@ from . import layout
 
 class World:
     def __init__(self, scene, broadphase="grid", max_bodies=None, solver=None, grid_dims=(128, 128, 128),
-                 max_bodies_per_cell=256, cell_size=None, pairs_per_body=16):
+                 max_bodies_per_cell=256, cell_size=None, pairs_per_body=16, polyhedra=None):
+        """polyhedra: optional list (one per scene shape) of precomputed tables
+        {vertices (n,3), planes (f,4), faces [index arrays], edges (e,3)},
+        registered as given (registerConvexHullShape(b3ConvexUtility*))
+        instead of hulling the scene's point clouds."""
         n = scene.num_bodies
         if max_bodies is None:
             max_bodies = max(128 * 1024, n + 10)
@@ -39,7 +43,10 @@ class World:
         call("b3h_bp_set_grid_params", self.bp, ptr(cell3), ptr(dims3), int(max_bodies_per_cell))
         call("b3h_pipeline_create", self.np_, self.bp, ctypes.byref(self.cfg), None, ctypes.byref(self.pl))
         self.collidables = []
-        for pts in scene.shapes:
+        for k, pts in enumerate(scene.shapes):
+            if polyhedra is not None:
+                self.collidables.append(capi.register_convex_polyhedron(self.np_, polyhedra[k]))
+                continue
             pts = np.ascontiguousarray(pts, np.float32)
             out = ctypes.c_int(-1)
             call("b3h_np_register_convex_hull_shape", self.np_, ptr(pts), 12, len(pts),

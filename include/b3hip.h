@@ -63,6 +63,11 @@ int b3h_np_destroy(b3hNarrowPhase* np);
  * new collidable index. */
 int b3h_np_register_convex_hull_shape(b3hNarrowPhase* np, const float* vertices, int strideInBytes, int numVertices,
 									  const float* scaling3, int* outCollidableIndex);
+/* b3GpuNarrowPhase::registerConvexHullShape(b3ConvexUtility* utilPtr),
+ * b3GpuNarrowPhase.h:39, b3GpuNarrowPhase.cpp:321-368: registers a polyhedron
+ * whose vertices, faces and unique edges the caller computed (the reference
+ * demo's route, GpuConvexScene.cpp:231,243), tables taken as given. */
+int b3h_np_register_convex_polyhedron(b3hNarrowPhase* np, const b3hConvexUtility* convex, int* outCollidableIndex);
 /* b3GpuNarrowPhase::registerRigidBody(collidableIndex, mass, position, orientation,
  * aabbMin, aabbMax, writeToGpu), b3GpuNarrowPhase.cpp:816-908 */
 int b3h_np_register_rigid_body(b3hNarrowPhase* np, int collidableIndex, float mass, const float* position3,
@@ -101,6 +106,16 @@ int b3h_np_get_local_space_aabb(b3hNarrowPhase* np, int collidableIndex, b3hAabb
 int b3h_np_get_shape_tables(b3hNarrowPhase* np, int* counts7, b3hCollidable* collidables,
 							b3hConvexPolyhedronData* convexes, b3hFloat4* vertices, b3hFloat4* uniqueEdges,
 							b3hGpuFace* faces, int* indices, b3hAabb* localAabbs);
+/* Replace the shape tables wholesale (same arrays and counts as
+ * b3h_np_get_shape_tables; counts7[6] must equal counts7[0]).  Every hull's
+ * offsets and face indices are checked against the tables.  No counterpart in
+ * the reference, whose kernel tests hand recorded tables straight to a kernel
+ * (test/OpenCL/AllBullet3Kernels/testExecuteBullet3NarrowphaseKernels.cpp:
+ * 380-424); the tables reach the device with writeAllBodiesToGpu. */
+int b3h_np_load_shape_tables(b3hNarrowPhase* np, const int* counts7, const b3hCollidable* collidables,
+							 const b3hConvexPolyhedronData* convexes, const b3hFloat4* vertices,
+							 const b3hFloat4* uniqueEdges, const b3hGpuFace* faces, const int* indices,
+							 const b3hAabb* localAabbs);
 /* Host copy of the registered inertias. */
 int b3h_np_get_inertias_cpu(b3hNarrowPhase* np, b3hInertiaData* dst, int n);
 /* Narrowphase tuning: lanes cooperating on one pair in the SAT, clipping and
@@ -108,6 +123,11 @@ int b3h_np_get_inertias_cpu(b3hNarrowPhase* np, b3hInertiaData* dst, int n);
  * pairs per wavefront); other powers of two up to 64 select 64.  Results do
  * not depend on it. */
 int b3h_np_set_pairs_per_group(b3hNarrowPhase* np, int lanesPerPair);
+/* Per-pair MPR result of the last computeContacts (device b3hFloat4[3 * numPairs]:
+ * direction, point, (depth, 1 if MPR found penetration else 0, 0, 0)); the
+ * reference's mprPenetrationKernel appends a contact for exactly the pairs
+ * marked 1 (mpr.cl:14-90). */
+int b3h_np_get_mpr_debug(b3hNarrowPhase* np, b3hFloat4** mprDev);
 /* Per-pair separating-axis state and normal of the last computeContacts (device). */
 int b3h_np_get_sat_debug(b3hNarrowPhase* np, int** sepStateDev, b3hFloat4** sepNormalDev);
 
@@ -133,6 +153,20 @@ int b3h_bp_create_proxies(b3hBroadphase* bp, int n, const float* aabbMins3, cons
 int b3h_bp_write_aabbs_to_gpu(b3hBroadphase* bp);
 /* calculateOverlappingPairs(maxPairs), b3GpuBroadphaseInterface.h:24 (no host sync) */
 int b3h_bp_calculate_overlapping_pairs(b3hBroadphase* bp, int maxPairs);
+/* calculateOverlappingPairsHost(maxPairs), b3GpuBroadphaseInterface.h:25: the
+ * same pair set by brute force on the host from the device AABBs
+ * (b3GpuGridBroadphase.cpp:260-292: every proxy against every later one;
+ * b3GpuSapBroadphase.cpp:862-981: small x small, then small x large), pairs
+ * normalised to (min, max), the first maxPairs kept, written to the device
+ * pair buffer (syncs). */
+int b3h_bp_calculate_overlapping_pairs_host(b3hBroadphase* bp, int maxPairs);
+/* getAllAabbsGPU (device b3hAabb[n]) / getAllAabbsCPU (host copy of the proxy
+ * AABBs, capacity entries; NULL host only reports n) / getSmallAabbIndicesGPU /
+ * getLargeAabbIndicesGPU (device int[n]), b3GpuBroadphaseInterface.h:34-39 */
+int b3h_bp_get_all_aabbs_gpu(b3hBroadphase* bp, b3hAabb** dev, int* n);
+int b3h_bp_get_all_aabbs_cpu(b3hBroadphase* bp, b3hAabb* host, int capacity, int* n);
+int b3h_bp_get_small_aabb_indices_gpu(b3hBroadphase* bp, int** dev, int* n);
+int b3h_bp_get_large_aabb_indices_gpu(b3hBroadphase* bp, int** dev, int* n);
 /* getAabbBufferWS (device b3hAabb[numAabbs]), getNumOverlap (syncs),
  * getOverlappingPairBuffer (device b3hInt4[]): b3GpuBroadphaseInterface.h:30-34 */
 int b3h_bp_get_aabb_buffer_ws(b3hBroadphase* bp, b3hAabb** dev, int* numAabbs);
@@ -170,6 +204,10 @@ int b3h_pipeline_solve_contacts(b3hPipeline* p);
 int b3h_pipeline_register_physics_instance(b3hPipeline* p, float mass, const float* position3,
 										   const float* orientation4, int collidableIndex, int userIndex,
 										   int writeInstanceToGpu, int* outBodyIndex);
+/* registerConvexPolyhedron(b3ConvexUtility*), b3GpuRigidBodyPipeline.h:40 (declared
+ * there without a definition): registers through the narrowphase as
+ * b3h_np_register_convex_polyhedron does. */
+int b3h_pipeline_register_convex_polyhedron(b3hPipeline* p, const b3hConvexUtility* convex, int* outCollidableIndex);
 /* writeAllInstancesToGpu, b3GpuRigidBodyPipeline.cpp:597-601 */
 int b3h_pipeline_write_all_instances_to_gpu(b3hPipeline* p);
 /* setGravity, b3GpuRigidBodyPipeline.cpp:589-592 */
@@ -194,11 +232,6 @@ int b3h_pipeline_get_solver_debug(b3hPipeline* p, b3hContact4Data** sortedContac
  * takes the global walk).  No counterpart in the reference (the reference
  * always walks global memory); for tests and tuning. */
 int b3h_pipeline_set_pgs_body_cache(b3hPipeline* p, int maxBodiesPerCell);
-/* Kept for callers of earlier versions; no effect: the sweep's rows carry
- * their inverse-inertia products (computed once per step with the sweep's own
- * operations), so no inertia is read during the sweeps.  No counterpart in
- * the reference. */
-int b3h_pipeline_set_pgs_inertia_cache(b3hPipeline* p, int enable);
 /* Dependency levels per solver cell of the last solve (host int[256]; -1: the
  * cell took the global walk) -- syncs.  No counterpart in the reference. */
 int b3h_pipeline_get_pgs_levels(b3hPipeline* p, int* levels256);
@@ -348,6 +381,15 @@ int b3h_mlp_train(b3hMlp* m, const float* xDev, int ldx, const float* tDev, int 
  * planes and edges as float4, faceIndexCounts and indices as int. */
 int b3h_compute_polyhedral_features(const float* vertices, int strideInBytes, int numVertices, int cap, int* counts4,
 									float* vertices4, float* planes4, int* faceIndexCounts, int* indices, float* edges4);
+
+/* What registerPhysicsInstance computes for one body, on the host (no device
+ * needed): its broadphase AABB -- the local AABB grown by the 0.01 margin and
+ * transformed (b3GpuRigidBodyPipeline.cpp:605-618, b3TransformAabb) -- and its
+ * inertia record from that AABB (b3GpuNarrowPhase::registerRigidBody,
+ * b3GpuNarrowPhase.cpp:860-901).  The registration entry points use it. */
+int b3h_compute_body_registration(const float* localAabbMin3, const float* localAabbMax3, float mass,
+								  const float* position3, const float* orientation4, float* aabbMin3, float* aabbMax3,
+								  b3hInertiaData* inertia);
 
 /* ---------------- device memory helpers for bindings ---------------- */
 int b3h_device_count(int* n);

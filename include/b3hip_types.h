@@ -97,6 +97,33 @@ typedef struct b3hConvexPolyhedronData
 	int m_unused;
 } b3hConvexPolyhedronData;
 
+/* A precomputed convex polyhedron: the members of b3ConvexUtility
+ * (src/Bullet3Collision/NarrowPhaseCollision/b3ConvexUtility.h:28-45) as flat host
+ * arrays, for registerConvexHullShape(b3ConvexUtility*).  vertices and
+ * uniqueEdges are b3Vector3 arrays (xyz, w unused); face f has the plane
+ * facePlanes[f] and the faceIndexCounts[f] next entries of indices.  mC, mE,
+ * m_extents and m_radius are copied into the hull record as the reference
+ * copies them.  localCenterOut (float[4], may be NULL) receives the local
+ * centre the registration computes, as the reference writes it into the
+ * utility's m_localCenter. */
+typedef struct b3hConvexUtility
+{
+	const b3hFloat4* vertices;
+	int numVertices;
+	int numFaces;
+	const b3hFloat4* facePlanes;
+	const int* faceIndexCounts;
+	const int* indices;
+	const b3hFloat4* uniqueEdges;
+	int numUniqueEdges;
+	int pad;
+	b3hFloat4 mC;
+	b3hFloat4 mE;
+	b3hFloat4 m_extents;
+	float m_radius;
+	float* localCenterOut;
+} b3hConvexUtility;
+
 /* min.w carries the body index as int bits (m_minIndices[3]); max.w carries
  * an int (m_signedMaxIndices[3]). */
 typedef struct b3hAabb

@@ -8,6 +8,8 @@
  *   b3ComputeWorldAabb    src/Bullet3Collision/NarrowPhaseCollision/shared/b3UpdateAabbs.h:8-33
  *   setConstraint4        src/Bullet3Dynamics/shared/b3ConvertConstraint4.h:62-148
  *   b3ConvexUtility::initializePolyhedralFeatures  src/Bullet3Collision/NarrowPhaseCollision/b3ConvexUtility.cpp:26-248
+ *   registerRigidBody's inertia  src/Bullet3OpenCL/RigidBody/b3GpuNarrowPhase.cpp:860-901, evaluated with
+ *                         the reference's b3Vector3 / b3Matrix3x3 / b3Quaternion classes
  * The headers are the reference's dual-compile ("shared") headers, built here
  * as the C++ they are written to be.
  */
@@ -23,6 +25,8 @@
 #include "Bullet3Dynamics/shared/b3IntegrateTransforms.h"
 #include "Bullet3Dynamics/shared/b3ConvertConstraint4.h"
 #include "Bullet3Collision/NarrowPhaseCollision/b3ConvexUtility.h"
+#include "Bullet3Common/b3Transform.h"
+#include "Bullet3Geometry/b3AabbUtil.h"
 #include <string.h>
 
 extern "C" {
@@ -118,5 +122,60 @@ int ref_hull(const float* pts3, int n, int cap, int* counts, float* vertsOut, fl
 	for (int i = 0; i < ne; i++)
 		for (int k = 0; k < 4; k++) edgesOut[4 * i + k] = k < 3 ? u.m_uniqueEdges[i][k] : 0.f;
 	return 0;
+}
+
+/* b3InertiaData of a body registered with this mass, orientation and
+ * (margin-inflated) AABB: the statements of b3GpuNarrowPhase::registerRigidBody
+ * (b3GpuNarrowPhase.cpp:860-901) on the reference's own math classes. */
+void ref_inertia(float mass, const float* aabbMinPtr, const float* aabbMaxPtr, const float* orn, void* out)
+{
+	b3InertiaData shapeInfo;
+	memset(&shapeInfo, 0, sizeof(shapeInfo));
+	if (mass == 0.f)
+	{
+		shapeInfo.m_initInvInertia.setValue(0, 0, 0, 0, 0, 0, 0, 0, 0);
+		shapeInfo.m_invInertiaWorld.setValue(0, 0, 0, 0, 0, 0, 0, 0, 0);
+	}
+	else
+	{
+		b3Vector3 aabbMin = b3MakeVector3(aabbMinPtr[0], aabbMinPtr[1], aabbMinPtr[2]);
+		b3Vector3 aabbMax = b3MakeVector3(aabbMaxPtr[0], aabbMaxPtr[1], aabbMaxPtr[2]);
+		b3Vector3 halfExtents = (aabbMax - aabbMin);
+		b3Vector3 localInertia;
+		float lx = 2.f * halfExtents[0];
+		float ly = 2.f * halfExtents[1];
+		float lz = 2.f * halfExtents[2];
+		localInertia.setValue((mass / 12.0f) * (ly * ly + lz * lz), (mass / 12.0f) * (lx * lx + lz * lz),
+							  (mass / 12.0f) * (lx * lx + ly * ly));
+		b3Vector3 invLocalInertia;
+		invLocalInertia[0] = 1.f / localInertia[0];
+		invLocalInertia[1] = 1.f / localInertia[1];
+		invLocalInertia[2] = 1.f / localInertia[2];
+		invLocalInertia[3] = 0.f;
+		shapeInfo.m_initInvInertia.setValue(invLocalInertia[0], 0, 0, 0, invLocalInertia[1], 0, 0, 0, invLocalInertia[2]);
+		b3Quaternion q(orn[0], orn[1], orn[2], orn[3]);
+		b3Matrix3x3 m(q);
+		shapeInfo.m_invInertiaWorld = m.scaled(invLocalInertia) * m.transpose();
+	}
+	memcpy(out, &shapeInfo, sizeof(shapeInfo));
+}
+
+/* broadphase AABB of registerPhysicsInstance (b3GpuRigidBodyPipeline.cpp:605-618) */
+void ref_instance_aabb(const float* lmin, const float* lmax, const float* pos, const float* orn, float* mn, float* mx)
+{
+	b3Vector3 localAabbMin = b3MakeVector3(lmin[0], lmin[1], lmin[2]);
+	b3Vector3 localAabbMax = b3MakeVector3(lmax[0], lmax[1], lmax[2]);
+	b3Scalar margin = 0.01f;
+	b3Transform t;
+	t.setIdentity();
+	t.setOrigin(b3MakeVector3(pos[0], pos[1], pos[2]));
+	t.setRotation(b3Quaternion(orn[0], orn[1], orn[2], orn[3]));
+	b3Vector3 aabbMin, aabbMax;
+	b3TransformAabb(localAabbMin, localAabbMax, margin, t, aabbMin, aabbMax);
+	for (int k = 0; k < 3; k++)
+	{
+		mn[k] = aabbMin[k];
+		mx[k] = aabbMax[k];
+	}
 }
 }

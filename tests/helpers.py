@@ -20,6 +20,22 @@ def ref_lib():
     return None
 
 
+def ref_hull_fixture():
+    """{name: tables} of tests/golden/ref_hulls.npz: the reference's
+    b3ConvexUtility output for each point cloud (points, vertices (n,3), planes
+    (f,4), faces (list of index arrays), edges (e,3))"""
+    d = np.load(os.path.join(ROOT, "tests", "golden", "ref_hulls.npz"))
+    out = {}
+    for name in d["names"]:
+        name = str(name)
+        fc, idx = d["face_counts_" + name], d["indices_" + name]
+        off = np.concatenate([[0], np.cumsum(fc)])
+        out[name] = dict(points=d["pts_" + name], vertices=d["vertices_" + name][:, :3],
+                         planes=d["planes_" + name], faces=[idx[off[k]:off[k + 1]] for k in range(len(fc))],
+                         edges=d["edges_" + name][:, :3])
+    return out
+
+
 def aligned(a, align=64):
     """copy into a 64-byte aligned buffer (the reference's types are 16-byte aligned)"""
     a = np.ascontiguousarray(a)

@@ -4,9 +4,9 @@ Shape tables come from the library's host hull routine (the one behind
 registerConvexHullShape, b3GpuNarrowPhase.cpp:298-368); bodies and inertias
 follow registerRigidBody (b3GpuNarrowPhase.cpp:816-908) and the registration
 AABB of registerPhysicsInstance (b3GpuRigidBodyPipeline.cpp:603-669, margin
-0.01).  The arithmetic here is numpy float32, so records can differ from the
-library's in the last bit; that is irrelevant for the closed-form and
-invariant checks that use it (the GPU tests take the library's own records).
+0.01), computed by the library's host routine b3h_compute_body_registration.
+The local centre is summed in numpy float32 in vertex order, as the library
+does.
 """
 import ctypes
 
@@ -67,10 +67,14 @@ def shape_tables(shapes):
                 localAabbs=np.array(laabbs, layout.AABB))
 
 
-def bodies_and_inertias(scene, tables, margin=0.01):
+def bodies_and_inertias(scene, tables):
+    """body and inertia records as registerPhysicsInstance writes them (the
+    library's host routine b3h_compute_body_registration: margin-grown
+    transformed local AABB, box inertia of twice its extent)"""
     n = scene.num_bodies
     b = np.zeros(n, layout.BODY)
     inr = np.zeros(n, layout.INERTIA)
+    amin, amax = np.zeros(3, np.float32), np.zeros(3, np.float32)
     for i in range(n):
         m = np.float32(scene.mass[i])
         b["pos"][i, :3] = scene.pos[i]
@@ -78,18 +82,14 @@ def bodies_and_inertias(scene, tables, margin=0.01):
         b["collidableIdx"][i] = int(scene.shape[i])
         b["invMass"][i] = np.float32(1.0) / m if m else np.float32(0.0)
         b["friction"][i] = 1.0
-        if not m:
-            continue
         la = tables["localAabbs"][int(scene.shape[i])]
-        R = quat_basis(scene.orn[i])
-        he = 0.5 * (la["max"][:3] - la["min"][:3]) + margin
-        e = np.abs(R) @ he
-        lx, ly, lz = (2.0 * 2.0 * e).astype(np.float32)
-        li = np.array([(m / 12.0) * (ly * ly + lz * lz), (m / 12.0) * (lx * lx + lz * lz),
-                       (m / 12.0) * (lx * lx + ly * ly)], np.float32)
-        il = (np.float32(1.0) / li).astype(np.float32)
-        inr["initInvInertia"][i, :, :3] = np.diag(il)
-        inr["invInertiaWorld"][i, :, :3] = (R * il) @ R.T
+        rec = np.zeros(1, layout.INERTIA)
+        capi.call("b3h_compute_body_registration", capi.ptr(np.ascontiguousarray(la["min"][:3])),
+                  capi.ptr(np.ascontiguousarray(la["max"][:3])), ctypes.c_float(float(m)),
+                  capi.ptr(np.ascontiguousarray(scene.pos[i], np.float32)),
+                  capi.ptr(np.ascontiguousarray(scene.orn[i], np.float32)), capi.ptr(amin), capi.ptr(amax),
+                  capi.ptr(rec))
+        inr[i] = rec[0]
     return b, inr
 
 

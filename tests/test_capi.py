@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from bullet3_amd import capi, scenes
-from helpers import ROOT, ref_lib
+from helpers import ROOT, ref_hull_fixture, ref_lib
 
 
 def declared_symbols():
@@ -97,44 +97,47 @@ def ref_hull(ref, pts, cap=4096):
     return dict(vertices=v[:nv, :3], planes=p[:nf], edges=e[:ne, :3], nfaces=nf, faces=faces)
 
 
-def canon_dirs(e):
-    out = []
-    for x in e:
-        k = np.argmax(np.abs(x))
-        out.append(tuple(np.round(x * np.sign(x[k]), 5)))
-    return sorted(out)
+def tables_equal(a, b):
+    """bit-for-bit equality of two shape tables, order included"""
+    if len(a["vertices"]) != len(b["vertices"]) or len(a["faces"]) != len(b["faces"]):
+        return False
+    if a["vertices"][:, :3].tobytes() != b["vertices"][:, :3].tobytes():
+        return False
+    if any(not np.array_equal(x, y) for x, y in zip(a["faces"], b["faces"])):
+        return False
+    return a["planes"].tobytes() == b["planes"].tobytes() and a["edges"][:, :3].tobytes() == b["edges"][:, :3].tobytes()
 
 
-@pytest.mark.parametrize("name,pts", hull_cases())
-def test_hull_matches_reference_convex_utility(name, pts):
+def test_hull_tables_equal_reference_fixture():
+    """registerConvexHullShape's tables equal the reference's b3ConvexUtility
+    tables byte for byte, in the reference's order (vertices, faces, face
+    indices, planes, unique edges), for every shape the benchmark scenes use and
+    100 seeded clouds (tests/golden/ref_hulls.npz, written by the reference's
+    own code)"""
+    fx = ref_hull_fixture()
+    assert len(fx) == 201
+    for name, r in fx.items():
+        a = capi.polyhedral_features(r["points"])
+        assert tables_equal(a, r), name
+
+
+def test_hull_tables_equal_reference_live():
+    """the same against the reference's code itself on 600 more clouds,
+    degenerate ones (integer grids: coplanar faces, collinear and repeated
+    points) included"""
     ref = ref_lib()
     if ref is None:
         pytest.skip("oracle/_ref not built")
-    a = capi.polyhedral_features(pts)
-    b = ref_hull(ref, pts)
-    assert sorted(map(tuple, np.round(a["vertices"], 6))) == sorted(map(tuple, np.round(b["vertices"], 6)))
-    assert len(a["faces"]) == b["nfaces"]
-    # faces matched by their vertex sets; the enumeration order differs, and a
-    # merged face takes the plane of whichever of its triangles is the merge
-    # reference (b3ConvexUtility.cpp:161-164), so merged planes agree to the
-    # 0.999 weld threshold and triangle planes to float rounding
-    def key(vs, idx):
-        return tuple(sorted(tuple(np.round(vs[i], 6)) for i in idx))
-    ref_faces = {key(b["vertices"], f): b["planes"][k] for k, f in enumerate(b["faces"])}
-    matched = 0
-    for k, f in enumerate(a["faces"]):
-        pk = key(a["vertices"], f)
-        if pk not in ref_faces:
-            continue  # a different greedy merge of near-coplanar triangles
-        matched += 1
-        pr, pm = ref_faces[pk], a["planes"][k]
-        if len(f) == 3:
-            assert np.abs(pr - pm).max() < 2e-5
+    rng = np.random.default_rng(77)
+    for k in range(600):
+        n = int(rng.integers(4, 129))
+        if k % 3 == 0:
+            pts = scenes.sphere_points(rng, n)
+        elif k % 3 == 1:
+            pts = (rng.normal(size=(n, 3)) * rng.uniform(0.01, 10, 3)).astype(np.float32)
         else:
-            assert float(np.dot(pr[:3], pm[:3])) > 0.999
-    assert matched >= 0.9 * len(a["faces"])
-    if matched == len(a["faces"]):
-        assert canon_dirs(a["edges"]) == canon_dirs(b["edges"])
+            pts = (rng.integers(-3, 4, size=(n, 3)) * 0.5).astype(np.float32)
+        assert tables_equal(capi.polyhedral_features(pts), ref_hull(ref, pts)), k
 
 
 def test_public_headers_compile_standalone(tmp_path):

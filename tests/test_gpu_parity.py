@@ -138,19 +138,15 @@ def test_narrowphase_bit_exact(scene, lanes):
     w.close()
 
 
-@pytest.mark.parametrize("cache", [-1, 0, 3, "no-inertia"])
+@pytest.mark.parametrize("cache", [-1, 0, 3])
 def test_solver_bit_exact(scene, cache):
     """cache: bodies per cell kept in LDS (-1: the LDS budget; 0: every cell on
-    the global walk; 3: mixed, small cells cached, larger ones global;
-    no-inertia: velocities in LDS, inverse inertias read from global memory)"""
+    the global walk; 3: mixed, small cells cached, larger ones global)"""
     w = World(scene)
     sp = w.solver_params()
     sp.m_numIterations = 10
     w.set_solver(sp)
-    if cache == "no-inertia":
-        capi.call("b3h_pipeline_set_pgs_inertia_cache", w.pl, 0)
-    else:
-        capi.call("b3h_pipeline_set_pgs_body_cache", w.pl, cache)
+    capi.call("b3h_pipeline_set_pgs_body_cache", w.pl, cache)
     ow = OracleWorld(w)
     run_steps(w, ow, 2)
     capi.call("b3h_pipeline_setup_gpu_aabbs_full", w.pl)

@@ -138,3 +138,54 @@ def test_autotune_leaves_the_simulation_unchanged():
     assert bits_equal(a, b), first_diff(a, b)
     w1.close()
     w2.close()
+
+
+def test_graph_recaptured_after_component_changes():
+    """a captured step is re-captured when a setter changes what it refers to:
+    grid parameters (whose scratch an eager step then reallocates), gravity,
+    residual tracking, narrowphase lanes per pair; replays keep matching an
+    eager twin"""
+    a, b = make_world(0), make_world(0)
+    cs = np.full(3, a.cell_size * 1.25, np.float32)
+    dims = np.array([64, 64, 64], np.int32)
+    for rep, change in enumerate(["grid", "gravity", "residual", "lanes"]):
+        a.step_graph(DT, 3)
+        for _ in range(3):
+            b.step(DT)
+        a.sync()
+        assert bits_equal(a.bodies(), b.bodies()), "before %s" % change
+        for w in (a, b):
+            if change == "grid":
+                capi.call("b3h_bp_set_grid_params", w.bp, capi.ptr(cs), capi.ptr(dims), 256)
+            elif change == "gravity":
+                capi.call("b3h_pipeline_set_gravity", w.pl, capi.ptr(np.array([0.5, -12.0, 0.0], np.float32)))
+            elif change == "residual":
+                w.track_residual(True)
+            else:
+                capi.call("b3h_np_set_pairs_per_group", w.np_, 16)
+        a.step(DT)
+        b.step(DT)
+        a.step_graph(DT, 3)
+        for _ in range(3):
+            b.step(DT)
+        a.sync()
+        assert bits_equal(a.bodies(), b.bodies()), "after %s\n%s" % (change, first_diff(a.bodies(), b.bodies()))
+    a.close()
+    b.close()
+
+
+def test_autotune_leaves_contact_log_alone():
+    w = make_world(1)
+    for _ in range(5):
+        w.step(DT)
+    lx, lt = capi.DeviceBuffer(4 * 24 * 200000), capi.DeviceBuffer(4 * 3 * 200000)
+    w.set_contact_log(lx.ptr, lt.ptr, 200000)
+    w.step(DT)
+    rows = w.contact_log_count()
+    assert rows > 0
+    w.autotune(DT, steps=2)
+    assert w.contact_log_count() == rows
+    w.set_contact_log(None, None, 0)
+    lx.free()
+    lt.free()
+    w.close()

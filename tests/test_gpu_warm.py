@@ -181,3 +181,32 @@ def test_warm_and_cold_share_the_fixed_point():
     assert far > 0
     assert near < 0.1 * far
     w.close()
+
+
+@pytest.mark.parametrize("bad", ["nan", "huge"])
+def test_bad_network_outputs_stay_safe(bad):
+    """a network whose outputs are NaN falls back, constraint by constraint, to
+    the previous-frame guess (so stepping equals the previous-frame mode bit
+    for bit); outputs of 1e30 give finite, bounded motion (the warm pass clamps
+    normal impulses at zero and friction to the friction bound)"""
+    from bullet3_amd.learned import WarmStartNet
+    sc = pile()
+    a, b = World(sc), World(sc)
+    a.set_warm_start(1)
+    net = WarmStartNet(24, 3, max_train_rows=64)
+    prm = np.zeros(net.num_params, np.float32)
+    prm[-3:] = np.nan if bad == "nan" else 1e30  # output biases
+    net.set_params(prm)
+    b.set_warm_start(2, net)
+    for s in range(30):
+        a.step(DT)
+        b.step(DT)
+    bb = b.bodies()
+    if bad == "nan":
+        assert np.array_equal(a.bodies().view(np.float32), bb.view(np.float32))
+    else:
+        for f in ("pos", "linVel", "angVel"):
+            assert np.isfinite(bb[f]).all()
+    a.close()
+    b.close()
+    net.close()

@@ -24,6 +24,7 @@ import pytest
 
 import pyoracle as po
 from bullet3_amd import layout, scenes
+from helpers import aligned
 from host_world import HostWorld
 
 G = 9.8
@@ -352,3 +353,65 @@ def test_sat_normal_minimises_overlap(kinds):
         assert n == 1
         checked += 1
     assert checked >= 20
+
+
+def hand_batches(pairs, n_bodies):
+    """batch ids of hand-built manifolds (bodyA, bodyB) that all fall in one
+    solver cell (every body at (1, 1, 1)); body 0 is the static ground"""
+    bodies = np.zeros(n_bodies, layout.BODY)
+    bodies["pos"][:, :3] = 1.0
+    bodies["quat"][:, 3] = 1.0
+    bodies["invMass"][1:] = 1.0
+    inert = np.zeros(n_bodies, layout.INERTIA)
+    inert["initInvInertia"][1:, 0, 0] = inert["initInvInertia"][1:, 1, 1] = inert["initInvInertia"][1:, 2, 2] = 1.0
+    c = np.zeros(len(pairs), layout.CONTACT)
+    for k, (a, b) in enumerate(pairs):
+        c["bodyA"][k], c["bodyB"][k] = a, b
+        c["childA"][k] = c["childB"][k] = -1
+        c["worldPosB"][k, 0, :3] = 1.0
+        c["worldNormalOnB"][k] = (0, 1, 0, 1)
+    out = po.solve_contacts(aligned(bodies), aligned(inert), aligned(c), po.solver_params(iterations=0), static_idx=0)
+    sc = out["sorted_contacts"].view(layout.CONTACT).reshape(-1)
+    cell = int(np.argmax(out["counts"]))
+    assert out["counts"][cell] == len(pairs)
+    return [(int(x["bodyA"]), int(x["bodyB"]), int(x["batchIdx"])) for x in sc], int(out["batch_sizes"][cell])
+
+
+def test_batching_hashed_bitmap_aliases_bodies_2048_apart():
+    """CreateBatchesNew keeps a 2048-bit map of used bodies indexed by body % 2048
+    (batchingKernelsNew.cl:116-140): body 2049 collides with body 1 although
+    they are different bodies, so its manifold waits for the next batch"""
+    got, nb = hand_batches([(0, 1), (0, 2049), (0, 3)], 2050)
+    assert got == [(0, 1, 0), (0, 3, 0), (0, 2049, 1)]
+    assert nb == 2
+    # without the alias the three are independent: one batch
+    got, nb = hand_batches([(0, 1), (0, 2050), (0, 3)], 2051)
+    assert got == [(0, 1, 0), (0, 3, 0), (0, 2050, 0)]
+    assert nb == 1
+
+
+def test_batching_map_cleared_after_64_accepted():
+    """after every 64 accepted manifolds the map is cleared
+    (batchingKernelsNew.cl:212-220): manifold (1, 65) reuses body 1 of the
+    first accepted manifold yet joins batch 0, as the 65th accepted"""
+    pairs = [(0, k) for k in range(1, 65)] + [(1, 65)]
+    got, nb = hand_batches(pairs, 66)
+    assert got == [(0, k, 0) for k in range(1, 65)] + [(1, 65, 0)]
+    assert nb == 1
+    # with only 63 before it the map still holds body 1: next batch
+    pairs = [(0, k) for k in range(1, 64)] + [(1, 65)]
+    got, nb = hand_batches(pairs, 66)
+    assert got == [(0, k, 0) for k in range(1, 64)] + [(1, 65, 1)]
+    assert nb == 2
+
+
+def test_batching_swaps_accepted_to_front():
+    """an accepted manifold is swapped with the first not yet batched one
+    (batchingKernelsNew.cl:196-206): (2, 3) overtakes the rejected (1, 2)"""
+    got, nb = hand_batches([(0, 1), (1, 2), (2, 3)], 4)
+    assert got == [(0, 1, 0), (2, 3, 0), (1, 2, 1)]
+    assert nb == 2
+    # a longer chain: rejected entries rotate behind each accepted one
+    got, nb = hand_batches([(1, 2), (2, 3), (3, 4), (4, 5)], 6)
+    assert got == [(1, 2, 0), (3, 4, 0), (2, 3, 1), (4, 5, 1)]
+    assert nb == 2

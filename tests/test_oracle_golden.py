@@ -158,3 +158,38 @@ def test_constraint_setup_matches_reference_cpp():
         if npts == 0:
             valid[24:28] = False
         assert np.abs(mine[valid] - out[valid]).max() < 1e-5 * max(1.0, np.abs(out[valid]).max())
+
+
+def test_body_registration_matches_reference_cpp():
+    """registerPhysicsInstance's broadphase AABB and registerRigidBody's inertia
+    record, bit for bit against the reference's own b3Transform / b3Matrix3x3 /
+    b3Quaternion code (oracle/_ref), for boxes, hulls and ground shapes at random
+    poses and masses"""
+    ref = ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref not built")
+    from bullet3_amd import capi
+    rng = np.random.default_rng(8)
+    f = ctypes.c_float
+    for trial in range(2000):
+        half = rng.uniform(0.05, 3.0, 3).astype(np.float32)
+        off = rng.normal(scale=0.1, size=3).astype(np.float32)
+        lmin, lmax = (off - half).astype(np.float32), (off + half).astype(np.float32)
+        pos = rng.normal(scale=20, size=3).astype(np.float32)
+        q = rng.normal(size=4)
+        q = (q / np.linalg.norm(q)).astype(np.float32)
+        if trial % 7 == 0:
+            q = np.array([0, 0, 0, 1], np.float32)
+        mass = 0.0 if trial % 11 == 0 else float(np.exp(rng.uniform(np.log(0.5), np.log(4.0))))
+        amin, amax = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        inert = aligned(np.zeros(24, np.float32))
+        capi.call("b3h_compute_body_registration", po.ptr(lmin), po.ptr(lmax), f(mass), po.ptr(pos), po.ptr(q),
+                  po.ptr(amin), po.ptr(amax), po.ptr(inert))
+        rmin, rmax = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        ref.ref_instance_aabb(po.ptr(lmin), po.ptr(lmax), po.ptr(pos), po.ptr(q), po.ptr(rmin), po.ptr(rmax))
+        assert amin.tobytes() == rmin.tobytes() and amax.tobytes() == rmax.tobytes(), trial
+        rin = aligned(np.zeros(24, np.float32))
+        ref.ref_inertia(f(np.float32(mass)), po.ptr(amin), po.ptr(amax), po.ptr(q), po.ptr(rin))
+        # the w lane of each row is padding the reference leaves to its vector code
+        cols = [c for c in range(24) if c % 4 != 3]
+        assert inert[cols].tobytes() == rin[cols].tobytes(), (trial, inert, rin)
