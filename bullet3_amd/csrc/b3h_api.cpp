@@ -1341,7 +1341,7 @@ extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const 
 		(r = dalloc(&p->sb.cellOffsets, B3H_N_CELLS + 1)) || (r = dalloc(&p->sb.cellCursor, B3H_N_CELLS)) ||
 		(r = dalloc(&p->sb.cellSlots, mc)) || (r = dalloc(&p->sb.perm, mc)) || (r = dalloc(&p->sb.batchIdx, mc)) ||
 		(r = dalloc(&p->sb.batchSizes, B3H_N_CELLS)) || (r = dalloc(&p->sb.sortKeys, 4 * (size_t)mc + 64)) ||
-		(r = dalloc(&p->sb.constraints, mc)) || (r = dalloc(&p->sb.sortedContacts, mc)) || (r = dalloc(&p->sb.resid, 4 * 64)) ||
+		(r = dalloc(&p->sb.constraints, mc)) || (r = dalloc(&p->sb.sortedContacts, mc)) || (r = dalloc(&p->sb.resid, 4 * B3H_MAX_ITERATIONS)) ||
 		(r = dalloc(&p->sb.cellBodies, 2 * (size_t)mc + 64)) || (r = dalloc(&p->sb.cellBodyCount, B3H_N_CELLS)) ||
 		(r = dalloc(&p->sb.cslot, 2 * (size_t)mc + 2)) || (r = dalloc(&p->sb.recN, 16 * (size_t)mc)) ||
 		(r = dalloc(&p->sb.recF, 11 * (size_t)mc)) || (r = dalloc(&p->sb.lamN, mc)) ||
@@ -1697,7 +1697,7 @@ extern "C" int b3h_pipeline_get_num_bodies(b3hPipeline* p, int* n)
 }
 extern "C" int b3h_pipeline_set_solver_params(b3hPipeline* p, const b3hSolverParams* prm)
 {
-	CHECK_ARG(p && prm && prm->m_numIterations >= 0 && prm->m_numIterations <= 64 && prm->m_solverDt > 0.f &&
+	CHECK_ARG(p && prm && prm->m_numIterations >= 0 && prm->m_numIterations <= B3H_MAX_ITERATIONS && prm->m_solverDt > 0.f &&
 			  prm->m_batchCellSize > 0.f);
 	CHECK_ARG(prm->m_warmStartMode >= 0 && prm->m_warmStartMode <= 2);
 	if (prm->m_warmStartMode == 2 && !p->warmNet)
@@ -2057,8 +2057,9 @@ extern "C" int b3h_pipeline_set_track_residual(b3hPipeline* p, int enable)
 extern "C" int b3h_pipeline_get_residuals(b3hPipeline* p, float* out, int n)
 {
 	CHECK_ARG(p && out && n >= 0 && n <= 2 * p->prm.m_numIterations);
-	float tmp[4 * 64];
-	B3H_CHECK(hipMemcpyAsync(tmp, p->sb.resid, sizeof(float) * 4 * p->prm.m_numIterations, hipMemcpyDeviceToHost, p->stream));
+	std::vector<float> tmp(4 * (size_t)p->prm.m_numIterations + 4);
+	B3H_CHECK(hipMemcpyAsync(tmp.data(), p->sb.resid, sizeof(float) * 4 * p->prm.m_numIterations, hipMemcpyDeviceToHost,
+							 p->stream));
 	B3H_CHECK(hipStreamSynchronize(p->stream));
 	for (int i = 0; i < n; i++)
 	{
@@ -2070,8 +2071,8 @@ extern "C" int b3h_pipeline_get_residuals(b3hPipeline* p, float* out, int n)
 extern "C" int b3h_pipeline_get_residuals_l2(b3hPipeline* p, float* out, int n)
 {
 	CHECK_ARG(p && out && n >= 0 && n <= 2 * p->prm.m_numIterations);
-	double tmp[4 * B3H_MAX_ITERATIONS];
-	B3H_CHECK(hipMemcpyAsync(tmp, p->sb.residL2, sizeof(double) * 4 * p->prm.m_numIterations, hipMemcpyDeviceToHost,
+	std::vector<double> tmp(4 * (size_t)p->prm.m_numIterations + 4);
+	B3H_CHECK(hipMemcpyAsync(tmp.data(), p->sb.residL2, sizeof(double) * 4 * p->prm.m_numIterations, hipMemcpyDeviceToHost,
 							 p->stream));
 	B3H_CHECK(hipStreamSynchronize(p->stream));
 	for (int i = 0; i < n; i++)

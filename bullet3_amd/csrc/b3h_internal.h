@@ -146,7 +146,7 @@ struct b3hSolverBuffers
 	double* residL2;      /* per sweep: sum of squared impulse changes, sum of squared impulses */
 	const int* numContacts;
 };
-#define B3H_MAX_ITERATIONS 64
+#define B3H_MAX_ITERATIONS 1024
 int b3h_solver_module_init();
 int b3h_solver_lds_bytes();
 /* set-up (ordering, batching, constraints, PGS records and levels), then the sweeps */

@@ -38,6 +38,24 @@ def collect(world, steps, capacity_rows, dt=1.0 / 60.0, mode=COLD, iterations=20
     return X, T, rows
 
 
+def collect_into(world, X, T, row0, steps, dt=1.0 / 60.0, mode=PREVIOUS, iterations=20):
+    """like collect, appending to existing logs X [cap][24] / T [cap][3] from
+    row `row0`; returns the new row count"""
+    cap = X.shape[0] - row0
+    if cap <= 0 or steps <= 0:
+        return row0
+    sp = world.solver_params()
+    sp.m_numIterations = iterations
+    sp.m_warmStartMode = mode
+    world.set_solver(sp)
+    world.set_contact_log(X.ptr + 4 * 24 * row0, T.ptr + 4 * 3 * row0, cap)
+    for _ in range(steps):
+        world.step(dt)
+    rows = world.contact_log_count()
+    world.set_contact_log(None, None, 0)
+    return row0 + rows
+
+
 def fit_input_normalization(net, X, rows, sample=400_000):
     """standardise every feature: shift = mean, scale = 1 / std over the first
     `sample` logged rows (constant features keep scale 1)"""

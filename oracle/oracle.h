@@ -97,6 +97,11 @@ typedef struct b3oWorld
 extern "C" {
 #endif
 int b3o_step(b3oWorld* w, float dt);
+/* Threads the oracle may use (default 1).  Results do not depend on it: the
+ * narrowphase compacts its per-pair results in pair order, and the solver runs
+ * at once only cells of one parity class, which share no dynamic body. */
+void b3o_set_threads(int n);
+int b3o_threads(void);
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,7 @@ void o_quat_to_abs_rows(F4 q, F4 rows[3])
 extern "C" void b3o_compute_world_aabbs(int n, const b3hRigidBodyData* bodies, const b3hCollidable* cols,
 										  const b3hAabb* localAabbs, b3hAabb* worldAabbs)
 {
+#pragma omp parallel for num_threads(b3o_threads()) schedule(static)
 	for (int i = 0; i < n; i++)
 	{
 		const b3hRigidBodyData& b = bodies[i];
@@ -61,6 +62,7 @@ extern "C" void b3o_compute_world_aabbs(int n, const b3hRigidBodyData* bodies, c
 extern "C" void b3o_integrate(int n, b3hRigidBodyData* bodies, float dt, float angDamp, const float* grav3)
 {
 	F4 g = f4(grav3[0], grav3[1], grav3[2], 0.f);
+#pragma omp parallel for num_threads(b3o_threads()) schedule(static)
 	for (int i = 0; i < n; i++)
 	{
 		b3hRigidBodyData& b = bodies[i];

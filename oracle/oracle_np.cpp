@@ -679,7 +679,114 @@ extern "C" int b3o_mpr_contacts(b3hInt4* pairs, int nPairs, const b3hRigidBodyDa
 	return nc;
 }
 
-/* Full convex-convex narrowphase of one step.  hasSepOut/sepOut (optional)
+/* One pair of the convex-convex narrowphase: MPR, SAT over faces and edges,
+ * clipping and reduction.  Writes the pair's manifold into *c (slot-free: the
+ * header names the pair) and returns whether the pair makes a contact;
+ * state / sep receive the final SAT state and normal. */
+static bool narrow_pair(const OShapes& S, const b3hInt4& pr, int i, b3hContact4Data* c, int* stateOut, F4* sepOut)
+{
+	const b3hRigidBodyData* bodies = S.bodies;
+	int a = pr.x, b = pr.y;
+	int state = 0;
+	F4 sep = f4(0, 0, 0, 0);
+	bool hull = pair_is_hull_hull(S, a, b);
+	/* MPR */
+	bool mprHit = false;
+	float mprDepth = 0.f;
+	F4 mprDir = f4(0, 0, 0, 0), mprPos = f4(0, 0, 0, 0);
+	if (hull) mprHit = mpr_penetration(S, a, b, &mprDepth, &mprDir, &mprPos) == 0;
+	/* the separating-normal slot starts with MPR's -dir when MPR found a
+	 * penetration (b3MprPenetration.h:857); it only survives in the
+	 * degenerate "state 4" path below */
+	if (mprHit) sep = neg4(mprDir);
+	if (hull)
+	{
+		const b3hRigidBodyData& A = bodies[a];
+		const b3hRigidBodyData& B = bodies[b];
+		const b3hConvexPolyhedronData& ha = S.hullOf(a);
+		const b3hConvexPolyhedronData& hb = S.hullOf(b);
+		F4 posA = A.m_pos; posA.w = 0.f;
+		F4 posB = B.m_pos; posB.w = 0.f;
+		F4 c0 = xform(ha.m_localCenter, posA, A.m_quat);
+		F4 c1 = xform(hb.m_localCenter, posB, B.m_quat);
+		F4 dc = sub4(c0, c1);
+		/* findSeparatingAxisVertexFaceKernel (sat.cl:1417-1509) */
+		float dmin = FLT_MAX;
+		F4 sepN = sep;
+		bool sepA = sat_faces(S, ha, hb, posA, A.m_quat, posB, B.m_quat, dc, &sepN, &dmin);
+		float dminSaved = FLT_MAX;
+		if (!sepA)
+			state = 0;
+		else
+		{
+			bool sepB = sat_faces(S, hb, ha, posB, B.m_quat, posA, A.m_quat, dc, &sepN, &dmin);
+			if (sepB)
+			{
+				state = 1;
+				dminSaved = dmin;
+				sep = sepN;
+			}
+			else
+				state = 4; /* sat.cl:1490-1506: flag left at 4, normal slot untouched */
+		}
+		int nEE = ha.m_numUniqueEdges * hb.m_numUniqueEdges;
+		if (state != 0)
+		{
+			float dm = dminSaved;
+			F4 s2 = sep;
+			bool ok;
+			if (nEE <= B3_UNIT_SPHERE_162_COUNT)
+				ok = sat_edges(S, ha, hb, posA, A.m_quat, posB, B.m_quat, dc, &s2, &dm);
+			else
+				ok = sat_sphere(S, ha, hb, posA, A.m_quat, posB, B.m_quat, dc, &s2, &dm);
+			if (!ok)
+				state = 0;
+			else
+			{
+				state = 1;
+				sep = s2;
+			}
+		}
+	}
+	*stateOut = state;
+	*sepOut = sep;
+
+	/* clipHullHullKernel (satClipHullContacts.cl:890-1000) */
+	int nReduced = 0;
+	int idx[4] = {-1, -1, -1, -1};
+	F4 pts[CLIP_CAP];
+	if (hull && state == 1)
+	{
+		const b3hRigidBodyData& A = bodies[a];
+		const b3hRigidBodyData& B = bodies[b];
+		int np = clip_hull_hull(S, sep, S.hullOf(a), S.hullOf(b), A.m_pos, A.m_quat, B.m_pos, B.m_quat, pts);
+		if (np > 0) nReduced = extract_manifold(pts, np, neg4(sep), idx);
+	}
+	if (!mprHit && nReduced == 0) return false;
+	memset(c, 0, sizeof(*c));
+	write_contact_header(c, S, pr, i, 0.7f);
+	if (mprHit)
+	{
+		c->m_worldNormalOnB = neg4(mprDir);
+		F4 p = mprPos;
+		p.w = -mprDepth;
+		c->m_worldPosB[0] = p;
+		c->m_worldNormalOnB.w = 1.f;
+	}
+	if (nReduced > 0)
+	{
+		c->m_worldNormalOnB = sep;
+		for (int k = 0; k < nReduced; k++)
+			if (k > 0 || !mprHit) c->m_worldPosB[k] = pts[idx[k]];
+		c->m_worldNormalOnB.w = (float)nReduced;
+	}
+	return true;
+}
+
+/* Full convex-convex narrowphase of one step: pairs in order, contacts take
+ * slots in pair order.  With more than one oracle thread the pairs are
+ * evaluated in parallel into a per-pair buffer and compacted in pair order
+ * afterwards, so the output is the same bytes.  hasSepOut/sepOut (optional)
  * receive the final SAT state and normal of each pair. */
 extern "C" int b3o_narrowphase(b3hInt4* pairs, int nPairs, const b3hRigidBodyData* bodies,
 								const b3hCollidable* cols, const b3hConvexPolyhedronData* convexes,
@@ -689,107 +796,45 @@ extern "C" int b3o_narrowphase(b3hInt4* pairs, int nPairs, const b3hRigidBodyDat
 {
 	OShapes S = {bodies, cols, convexes, vertices, uniqueEdges, faces, indices};
 	int nc = 0;
+	int threads = b3o_threads();
+	if (threads > 1 && nPairs > 1)
+	{
+		std::vector<b3hContact4Data> tmp(nPairs);
+		std::vector<unsigned char> hit(nPairs);
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 256)
+		for (int i = 0; i < nPairs; i++)
+		{
+			int state;
+			F4 sep;
+			hit[i] = narrow_pair(S, pairs[i], i, &tmp[i], &state, &sep) ? 1 : 0;
+			if (hasSepOut) hasSepOut[i] = state;
+			if (sepOut) sepOut[i] = sep;
+		}
+		for (int i = 0; i < nPairs; i++)
+		{
+			pairs[i].z = -1;
+			if (!hit[i]) continue;
+			int slot = nc++;
+			if (slot >= capacity) continue;
+			pairs[i].z = slot;
+			contacts[slot] = tmp[i];
+		}
+		return nc;
+	}
 	for (int i = 0; i < nPairs; i++)
 	{
 		pairs[i].z = -1;
-		int a = pairs[i].x, b = pairs[i].y;
-		int state = 0;
-		F4 sep = f4(0, 0, 0, 0);
-		bool hull = pair_is_hull_hull(S, a, b);
-		/* MPR */
-		bool mprHit = false;
-		float mprDepth = 0.f;
-		F4 mprDir = f4(0, 0, 0, 0), mprPos = f4(0, 0, 0, 0);
-		if (hull) mprHit = mpr_penetration(S, a, b, &mprDepth, &mprDir, &mprPos) == 0;
-		/* the separating-normal slot starts with MPR's -dir when MPR found a
-		 * penetration (b3MprPenetration.h:857); it only survives in the
-		 * degenerate "state 4" path below */
-		if (mprHit) sep = neg4(mprDir);
-		if (hull)
-		{
-			const b3hRigidBodyData& A = bodies[a];
-			const b3hRigidBodyData& B = bodies[b];
-			const b3hConvexPolyhedronData& ha = S.hullOf(a);
-			const b3hConvexPolyhedronData& hb = S.hullOf(b);
-			F4 posA = A.m_pos; posA.w = 0.f;
-			F4 posB = B.m_pos; posB.w = 0.f;
-			F4 c0 = xform(ha.m_localCenter, posA, A.m_quat);
-			F4 c1 = xform(hb.m_localCenter, posB, B.m_quat);
-			F4 dc = sub4(c0, c1);
-			/* findSeparatingAxisVertexFaceKernel (sat.cl:1417-1509) */
-			float dmin = FLT_MAX;
-			F4 sepN = sep;
-			bool sepA = sat_faces(S, ha, hb, posA, A.m_quat, posB, B.m_quat, dc, &sepN, &dmin);
-			float dminSaved = FLT_MAX;
-			if (!sepA)
-				state = 0;
-			else
-			{
-				bool sepB = sat_faces(S, hb, ha, posB, B.m_quat, posA, A.m_quat, dc, &sepN, &dmin);
-				if (sepB)
-				{
-					state = 1;
-					dminSaved = dmin;
-					sep = sepN;
-				}
-				else
-					state = 4; /* sat.cl:1490-1506: flag left at 4, normal slot untouched */
-			}
-			int nEE = ha.m_numUniqueEdges * hb.m_numUniqueEdges;
-			if (state != 0)
-			{
-				float dm = dminSaved;
-				F4 s2 = sep;
-				bool ok;
-				if (nEE <= B3_UNIT_SPHERE_162_COUNT)
-					ok = sat_edges(S, ha, hb, posA, A.m_quat, posB, B.m_quat, dc, &s2, &dm);
-				else
-					ok = sat_sphere(S, ha, hb, posA, A.m_quat, posB, B.m_quat, dc, &s2, &dm);
-				if (!ok)
-					state = 0;
-				else
-				{
-					state = 1;
-					sep = s2;
-				}
-			}
-		}
+		int state;
+		F4 sep;
+		b3hContact4Data c;
+		bool h = narrow_pair(S, pairs[i], i, &c, &state, &sep);
 		if (hasSepOut) hasSepOut[i] = state;
 		if (sepOut) sepOut[i] = sep;
-
-		/* clipHullHullKernel (satClipHullContacts.cl:890-1000) */
-		int nReduced = 0;
-		int idx[4] = {-1, -1, -1, -1};
-		F4 pts[CLIP_CAP];
-		if (hull && state == 1)
-		{
-			const b3hRigidBodyData& A = bodies[a];
-			const b3hRigidBodyData& B = bodies[b];
-			int np = clip_hull_hull(S, sep, S.hullOf(a), S.hullOf(b), A.m_pos, A.m_quat, B.m_pos, B.m_quat, pts);
-			if (np > 0) nReduced = extract_manifold(pts, np, neg4(sep), idx);
-		}
-		if (!mprHit && nReduced == 0) continue;
+		if (!h) continue;
 		int slot = nc++;
 		if (slot >= capacity) continue;
 		pairs[i].z = slot;
-		b3hContact4Data* c = contacts + slot;
-		memset(c, 0, sizeof(*c));
-		write_contact_header(c, S, pairs[i], i, 0.7f);
-		if (mprHit)
-		{
-			c->m_worldNormalOnB = neg4(mprDir);
-			F4 p = mprPos;
-			p.w = -mprDepth;
-			c->m_worldPosB[0] = p;
-			c->m_worldNormalOnB.w = 1.f;
-		}
-		if (nReduced > 0)
-		{
-			c->m_worldNormalOnB = sep;
-			for (int k = 0; k < nReduced; k++)
-				if (k > 0 || !mprHit) c->m_worldPosB[k] = pts[idx[k]];
-			c->m_worldNormalOnB.w = (float)nReduced;
-		}
+		contacts[slot] = c;
 	}
 	return nc;
 }

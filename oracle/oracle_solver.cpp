@@ -400,9 +400,12 @@ extern "C" int b3o_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const
 	for (int i = 0; i < n; i++) cs[i] = det[kv[i].m_value];
 	/* 3. batching per cell */
 	int batchSizes[N_CELLS];
+	const int threads = b3o_threads();
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
 	for (int c = 0; c < N_CELLS; c++) batchSizes[c] = counts[c] ? create_batches(&cs[offsets[c]], (int)counts[c], staticIdx) : 0;
 	/* 4. constraints (initial local inverse inertia, solverSetup.cl:254,260) */
 	std::vector<b3hContactConstraint4> con(n);
+#pragma omp parallel for num_threads(threads) schedule(static)
 	for (int i = 0; i < n; i++)
 	{
 		int a = abs(cs[i].m_bodyAPtrAndSignBit), b = abs(cs[i].m_bodyBPtrAndSignBit);
@@ -416,8 +419,11 @@ extern "C" int b3o_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const
 	int K = prm->m_numIterations;
 	for (int phase = 0; phase < 2; phase++)
 	{
+		/* the cells of one parity class share no dynamic body (they are a cell
+		 * apart and bodies are smaller than a cell), so they may run at once */
 		if (initialImpulses)
 			for (int ib = 0; ib < N_CELL_BATCHES; ib++)
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
 				for (int wg = 0; wg < N_CELLS / N_CELL_BATCHES; wg++)
 				{
 					int cell = cell_of(wg, ib);
@@ -429,10 +435,12 @@ extern "C" int b3o_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const
 				}
 		for (int it = 0; it < K; it++)
 		{
-			Resid res = {0.f, 0.f};
+			float maxDelta = 0.f, maxLambda = 0.f;
 			for (int ib = 0; ib < N_CELL_BATCHES; ib++)
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) reduction(max : maxDelta, maxLambda)
 				for (int wg = 0; wg < N_CELLS / N_CELL_BATCHES; wg++)
 				{
+					Resid res = {0.f, 0.f};
 					int cell = cell_of(wg, ib);
 					for (unsigned k = 0; k < counts[cell]; k++)
 					{
@@ -449,11 +457,13 @@ extern "C" int b3o_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const
 						write_back(A, va);
 						write_back(B, vb);
 					}
+					maxDelta = res.maxDelta > maxDelta ? res.maxDelta : maxDelta;
+					maxLambda = res.maxLambda > maxLambda ? res.maxLambda : maxLambda;
 				}
 			if (residualsOut)
 			{
-				float denom = res.maxLambda > 1e-9f ? res.maxLambda : 1e-9f;
-				residualsOut[phase * K + it] = res.maxDelta / denom;
+				float denom = maxLambda > 1e-9f ? maxLambda : 1e-9f;
+				residualsOut[phase * K + it] = maxDelta / denom;
 			}
 		}
 	}

@@ -5,6 +5,10 @@
  */
 #include "oracle.h"
 
+static int g_threads = 1;
+extern "C" void b3o_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+extern "C" int b3o_threads(void) { return g_threads; }
+
 extern "C" int b3o_step(b3oWorld* w, float dt)
 {
 	b3o_compute_world_aabbs(w->numBodies, w->bodies, w->collidables, w->localAabbs, w->worldAabbs);

@@ -39,6 +39,11 @@ class SolverParams(ctypes.Structure):
                 ("m_angularDamping", ctypes.c_float), ("m_warmStartMode", ctypes.c_int)]
 
 
+def set_threads(n):
+    """threads the oracle may use (results do not depend on it)"""
+    lib().b3o_set_threads(int(n))
+
+
 def solver_params(dt=1.0 / 60.0, iterations=4, friction=0.7, drift=0.005, coeff=0.2, cell=6.0, damping=0.99):
     return SolverParams(dt, iterations, friction, drift, coeff, cell, damping, 0)
 

@@ -415,3 +415,23 @@ def test_batching_swaps_accepted_to_front():
     got, nb = hand_batches([(1, 2), (2, 3), (3, 4), (4, 5)], 6)
     assert got == [(1, 2, 0), (3, 4, 0), (2, 3, 1), (4, 5, 1)]
     assert nb == 2
+
+
+def test_oracle_threads_give_the_same_bytes():
+    """the all-core oracle (per-pair narrowphase in parallel, compacted in pair
+    order; the cells of one parity class in parallel) steps bit for bit like
+    the one-thread oracle"""
+    sc = scenes.convex_pile(6, 4, 6, spacing=1.0, n_hull_shapes=8, drop=0.0, seed=3)
+    a = HostWorld(sc, solver=po.solver_params(iterations=10))
+    b = HostWorld(sc, solver=po.solver_params(iterations=10))
+    try:
+        for step in range(25):
+            po.set_threads(1)
+            a.step()
+            po.set_threads(4)
+            b.step()
+            assert a.bodies.tobytes() == b.bodies.tobytes(), step
+            assert a.num_contacts == b.num_contacts
+        assert a.num_contacts > 50
+    finally:
+        po.set_threads(1)
