@@ -19,6 +19,8 @@ STAGE_NAMES = ("aabb", "broadphase", "narrowphase_mpr", "narrowphase_sat", "narr
                "pgs_normal", "pgs_friction", "integrate")
 BROADPHASE_GRID = 0
 BROADPHASE_SAP = 1
+BROADPHASE_PLBVH = 2
+BROADPHASE_KINDS = {"grid": BROADPHASE_GRID, "sap": BROADPHASE_SAP, "plbvh": BROADPHASE_PLBVH}
 
 
 class B3hError(RuntimeError):
@@ -44,8 +46,13 @@ class ConvexUtility(ctypes.Structure):
     _fields_ = [("vertices", ctypes.c_void_p), ("numVertices", ctypes.c_int), ("numFaces", ctypes.c_int),
                 ("facePlanes", ctypes.c_void_p), ("faceIndexCounts", ctypes.c_void_p), ("indices", ctypes.c_void_p),
                 ("uniqueEdges", ctypes.c_void_p), ("numUniqueEdges", ctypes.c_int), ("pad", ctypes.c_int),
+                # b3hFloat4 is 16-byte aligned: mC starts at byte 64
+                ("pad16", ctypes.c_int * 2),
                 ("mC", ctypes.c_float * 4), ("mE", ctypes.c_float * 4), ("m_extents", ctypes.c_float * 4),
                 ("m_radius", ctypes.c_float), ("localCenterOut", ctypes.c_void_p)]
+
+
+assert ctypes.sizeof(ConvexUtility) == 128 and ConvexUtility.mC.offset == 64 and ConvexUtility.localCenterOut.offset == 120
 
 
 _vp = ctypes.c_void_p
@@ -104,6 +111,15 @@ _SIGS = {
     "b3h_bp_get_overlapping_pair_buffer": ([_vp, _pvp], _i),
     "b3h_bp_get_total_found": ([_vp, _pi], _i),
     "b3h_bp_get_sap_axis": ([_vp, _pi], _i),
+    "b3h_bp_get_bvh_info": ([_vp, _pi, _pi], _i),
+    "b3h_pipeline_create_point2point_constraint": ([_vp, _i, _i, _vp, _vp, _f, _pi], _i),
+    "b3h_pipeline_create_fixed_constraint": ([_vp, _i, _i, _vp, _vp, _vp, _f, _pi], _i),
+    "b3h_pipeline_remove_constraint_by_uid": ([_vp, _i], _i),
+    "b3h_pipeline_copy_constraints_to_host": ([_vp], _i),
+    "b3h_pipeline_get_constraints": ([_vp, _vp, _i, _pi], _i),
+    "b3h_pipeline_cast_rays": ([_vp, _vp, _vp, _i], _i),
+    "b3h_pipeline_cast_rays_gpu": ([_vp, _vp, _vp, _i], _i),
+    "b3h_pipeline_get_ray_bvh_info": ([_vp, _pi, _pi], _i),
     "b3h_bp_get_proxy_maps": ([_vp, _vp, _vp, _vp], _i),
     "b3h_bp_create_proxies": ([_vp, _i, _vp, _vp, _vp], _i),
     "b3h_bp_get_grid_sort": ([_vp, _pvp, _pvp, _pi], _i),
@@ -326,15 +342,16 @@ def register_convex_polyhedron(np_handle, tables, pipeline=None):
     return out.value
 
 
-def radix_sort32(keys, values, sort_bits=32):
-    """Stable (key, value) sort on the device by the low sort_bits key bits."""
+def radix_sort32(keys, values, sort_bits=32, entry="b3h_radix_sort32"):
+    """Stable (key, value) sort on the device by the low sort_bits key bits
+    (entry: b3h_radix_sort32 or its allocation-free variant)."""
     k = np.ascontiguousarray(keys, np.uint32)
     v = np.ascontiguousarray(values, np.uint32)
     n = len(k)
     dk, dv = DeviceBuffer(4 * max(n, 1)), DeviceBuffer(4 * max(n, 1))
     htod(dk.ptr, k)
     htod(dv.ptr, v)
-    call("b3h_radix_sort32", ctypes.c_void_p(dk.ptr), ctypes.c_void_p(dv.ptr), n, sort_bits)
+    call(entry, ctypes.c_void_p(dk.ptr), ctypes.c_void_p(dv.ptr), n, sort_bits)
     return dtoh(dk.ptr, 4 * n, np.uint32).copy(), dtoh(dv.ptr, 4 * n, np.uint32).copy()
 
 

@@ -815,11 +815,16 @@ static void bp_free_scratch(b3hBroadphase* bp)
 	dfree(bp->sc.lhist);
 	dfree(bp->sc.lmaxExt);
 	dfree(bp->sc.lminx);
+	dfree(bp->sc.bvhMerge);
+	dfree(bp->sc.bvhChild);
+	dfree(bp->sc.bvhBox);
+	dfree(bp->sc.bvhFlag);
+	dfree(bp->sc.bvhMeta);
 }
 
 extern "C" int b3h_bp_create(int kind, const b3hConfig* cfg, void* stream, b3hBroadphase** out)
 {
-	CHECK_ARG(cfg && out && (kind == B3H_BROADPHASE_GRID || kind == B3H_BROADPHASE_SAP));
+	CHECK_ARG(cfg && out && (kind == B3H_BROADPHASE_GRID || kind == B3H_BROADPHASE_SAP || kind == B3H_BROADPHASE_PLBVH));
 	*out = nullptr;
 	int r = ensure_device();
 	if (r) return r;
@@ -967,9 +972,15 @@ static int bp_ensure_scratch(b3hBroadphase* bp)
 		(r = dalloc(&bp->sc.sortedAabbs, cap)) || (r = dalloc(&bp->sc.sapPartials, 6 * (size_t)(b3h_sap_num_partials(cap) + 1))) ||
 		(r = dalloc(&bp->sc.axis, 1)) || (r = dalloc(&bp->sc.stash, (size_t)cap * B3H_GRID_STASH)))
 		return r;
+	if (bp->kind == B3H_BROADPHASE_PLBVH &&
+		((r = dalloc(&bp->sc.bvhMerge, b3h_plbvh_reduce_blocks(cap) + 1)) || (r = dalloc(&bp->sc.bvhChild, cap)) ||
+		 (r = dalloc(&bp->sc.bvhBox, 2 * (size_t)cap)) || (r = dalloc(&bp->sc.bvhFlag, cap)) ||
+		 (r = dalloc(&bp->sc.bvhMeta, 2))))
+		return r;
 	bp->scCapSmall = ns;
 	bp->scCells = bp->numCells;
 	B3H_CHECK(hipMemset(bp->sc.axis, 0, sizeof(int)));
+	if (bp->sc.bvhMeta) B3H_CHECK(hipMemset(bp->sc.bvhMeta, 0, 2 * sizeof(int)));
 	return B3H_OK;
 }
 
@@ -1000,9 +1011,12 @@ static int bp_calc_pairs(b3hBroadphase* bp, int maxPairs, hipStream_t s)
 	if (bp->kind == B3H_BROADPHASE_GRID)
 		b3h_grid_pairs(bp->dAll, bp->dSmall, ns, bp->dLarge, nl, bp->gp, bp->numCells, bp->dPairs, maxPairs, bp->dPairCount,
 					   bp->dPairCount + 1, bp->sc, s);
-	else
+	else if (bp->kind == B3H_BROADPHASE_SAP)
 		b3h_sap_pairs(bp->dAll, bp->dSmall, ns, bp->dLarge, nl, bp->dPairs, maxPairs, bp->dPairCount, bp->dPairCount + 1,
 					  bp->sc, s);
+	else
+		b3h_plbvh_pairs(bp->dAll, bp->dSmall, ns, bp->dLarge, nl, bp->dPairs, maxPairs, bp->dPairCount,
+						bp->dPairCount + 1, bp->sc, s);
 	return B3H_OK;
 }
 
@@ -1058,7 +1072,10 @@ extern "C" int b3h_bp_calculate_overlapping_pairs_host(b3hBroadphase* bp, int ma
 	}
 	else
 	{
-		/* small x small, then small x large (b3GpuSapBroadphase.cpp:862-981) */
+		/* small x small, then small x large (b3GpuSapBroadphase.cpp:862-981;
+		 * b3GpuParallelLinearBvhBroadphase::calculateOverlappingPairsHost,
+		 * b3GpuParallelLinearBvhBroadphase.cpp:59-62, asserts instead: the
+		 * BVH broadphase takes the SAP's host loop) */
 		int ns = (int)bp->smallMap.size(), nl = (int)bp->largeMap.size();
 		for (int i = 0; i < ns; i++)
 			for (int j = i + 1; j < ns; j++)
@@ -1148,6 +1165,21 @@ extern "C" int b3h_bp_get_sap_axis(b3hBroadphase* bp, int* axis)
 	}
 	return read_int(bp->sc.axis, axis, bp->stream);
 }
+extern "C" int b3h_bp_get_bvh_info(b3hBroadphase* bp, int* root, int* overflow)
+{
+	CHECK_ARG(bp && root && overflow);
+	if (bp->kind != B3H_BROADPHASE_PLBVH || !bp->sc.bvhMeta)
+	{
+		b3h_set_error("getBvhInfo: not a BVH broadphase, or no pairs calculated yet");
+		return B3H_ERR_STATE;
+	}
+	int m[2];
+	B3H_CHECK(hipMemcpyAsync(m, bp->sc.bvhMeta, sizeof(m), hipMemcpyDeviceToHost, bp->stream));
+	B3H_CHECK(hipStreamSynchronize(bp->stream));
+	*root = m[0];
+	*overflow = m[1];
+	return B3H_OK;
+}
 extern "C" int b3h_bp_get_grid_sort(b3hBroadphase* bp, const unsigned** k, const unsigned** v, int* n)
 {
 	CHECK_ARG(bp && k && v && n);
@@ -1200,7 +1232,57 @@ struct b3hPipeline
 	b3hMlp* warmNet;
 	bool logRows;
 	float matchTol;
+	/* ray casts: BVH scratch (sized for raySmallCap small proxies) and the
+	 * device copies of the host ray / hit arrays (rayCap rays) */
+	b3hBroadphaseScratch raySc;
+	int raySmallCap;
+	b3hRayInfo* dRays;
+	b3hRayHit* dHits;
+	int rayCap;
+	/* joints: the host list (b3GpuRigidBodyPipeline's m_cpuConstraints), its
+	 * device image and the solver's buffers; jointsDirty: the list changed
+	 * since the last upload (rows, slots and batches are rebuilt then) */
+	std::vector<b3hGpuGenericConstraint> joints;
+	int jointUid;
+	bool jointsDirty;
+	b3hJointBuffers jb;
+	b3hJointParams jp;
 };
+
+static void joint_free(b3hPipeline* p)
+{
+	b3hJointBuffers& jb = p->jb;
+	dfree(jb.constraints);
+	dfree(jb.rowOffset);
+	dfree(jb.rowCount);
+	dfree(jb.slotOf);
+	dfree(jb.rows);
+	dfree(jb.batchStart);
+	dfree(jb.order);
+	dfree(jb.delta);
+	dfree(jb.bodySlot);
+	jb.numConstraints = jb.numBatches = jb.numSlots = jb.numBodies = 0;
+}
+
+static void ray_free(b3hPipeline* p)
+{
+	b3hBroadphaseScratch& sc = p->raySc;
+	dfree(sc.keys);
+	dfree(sc.vals);
+	dfree(sc.tmpKeys);
+	dfree(sc.tmpVals);
+	dfree(sc.hist);
+	dfree(sc.sortedAabbs);
+	dfree(sc.bvhMerge);
+	dfree(sc.bvhChild);
+	dfree(sc.bvhBox);
+	dfree(sc.bvhFlag);
+	dfree(sc.bvhMeta);
+	p->raySmallCap = -1;
+	dfree(p->dRays);
+	dfree(p->dHits);
+	p->rayCap = 0;
+}
 
 static void pl_free(b3hPipeline* p)
 {
@@ -1249,6 +1331,8 @@ static void pl_free(b3hPipeline* p)
 	dfree(w.rowScan);
 	dfree(w.scanTmp);
 	dfree(w.logCount);
+	ray_free(p);
+	joint_free(p);
 }
 
 /* warm-start buffers, allocated on first use (features: the per-point rows of
@@ -1318,6 +1402,21 @@ extern "C" int b3h_pipeline_create(b3hNarrowPhase* np, b3hBroadphase* bp, const 
 	p->warmNet = nullptr;
 	p->logRows = false;
 	p->matchTol = 0.1f;
+	p->jointUid = 0;
+	p->jointsDirty = false;
+	memset((void*)&p->jb, 0, sizeof(p->jb));
+	/* b3GpuPgsConstraintSolver::solveJoints (b3GpuPgsConstraintSolver.cpp:927-937)
+	 * with b3ContactSolverInfo's defaults (b3ContactSolverInfo.h:60-80) */
+	p->jp.timeStep = 1.f / 60.f;
+	p->jp.erp = 0.2f;
+	p->jp.globalCfm = 0.f;
+	p->jp.damping = 1.f;
+	p->jp.iterations = 4;
+	memset((void*)&p->raySc, 0, sizeof(p->raySc));
+	p->raySmallCap = -1;
+	p->dRays = nullptr;
+	p->dHits = nullptr;
+	p->rayCap = 0;
 	int r;
 	/* one stream for all three objects keeps the step in order: while the
 	 * pipeline exists the broadphase issues its stand-alone calls on the
@@ -1378,6 +1477,8 @@ extern "C" int b3h_pipeline_destroy(b3hPipeline* p)
 }
 
 static int num_bodies(b3hPipeline* p) { return (int)p->np->bodies.size(); }
+static int joints_upload(b3hPipeline* p);
+static int joints_pull_flags(b3hPipeline* p);
 
 static void pl_aabbs(b3hPipeline* p)
 {
@@ -1423,6 +1524,8 @@ static int pl_step(b3hPipeline* p, float dt, bool events)
 	np_compute_contacts_dev(p->np, p->bp->dPairs, p->bp->dPairCount, p->cfg.m_maxBroadphasePairs,
 							events ? p->ev[4] : nullptr, events ? p->ev[3] : nullptr);
 	if (events) hipEventRecord(p->ev[5], s);
+	/* joints before the contact solve (b3GpuRigidBodyPipeline.cpp:361-372) */
+	if (p->jb.numConstraints > 0) b3h_solve_joints(p->jb, p->np->dBodies, p->np->dInertias, p->jp, s);
 	/* integration is fused into the last friction sweep for every body a
 	 * contact touches; the rest are integrated by the pass below */
 	b3hIntegrateParams ip = {dt, p->prm.m_angularDamping, p->gravity};
@@ -1516,7 +1619,9 @@ extern "C" int b3h_pipeline_step_simulation(b3hPipeline* p, float dt)
 		for (int i = 0; i <= B3H_NUM_STAGES; i++) B3H_CHECK(hipEventCreate(&p->ev[i]));
 		p->evValid = true;
 	}
-	int r = pl_step(p, dt, p->timing);
+	int r = joints_upload(p);
+	if (r) return r;
+	r = pl_step(p, dt, p->timing);
 	if (r) return r;
 	B3H_CHECK(hipGetLastError());
 	return B3H_OK;
@@ -1527,6 +1632,7 @@ extern "C" int b3h_pipeline_step_graph(b3hPipeline* p, float dt, int count)
 	CHECK_ARG(p && count >= 0);
 	if (num_bodies(p) == 0) return B3H_OK;
 	if (int rs = check_scenes(p)) return rs;
+	if (int ru = joints_upload(p)) return ru;
 	if (!p->graphExec || p->graphDt != dt || p->graphBodies != num_bodies(p) || p->graphNpGen != p->np->gen ||
 		p->graphBpGen != p->bp->gen)
 	{
@@ -1659,7 +1765,9 @@ extern "C" int b3h_pipeline_write_all_instances_to_gpu(b3hPipeline* p)
 	CHECK_ARG(p);
 	int r;
 	if ((r = b3h_np_write_all_bodies_to_gpu(p->np)) || (r = b3h_bp_write_aabbs_to_gpu(p->bp))) return r;
-	return B3H_OK;
+	if ((r = joints_pull_flags(p))) return r;
+	p->jointsDirty = p->jointsDirty || !p->joints.empty();
+	return joints_upload(p);
 }
 extern "C" int b3h_pipeline_set_gravity(b3hPipeline* p, const float* g)
 {
@@ -1680,6 +1788,8 @@ extern "C" int b3h_pipeline_reset(b3hPipeline* p)
 	p->bp->all.clear();
 	p->bp->smallMap.clear();
 	p->bp->largeMap.clear();
+	p->joints.clear();
+	p->jointsDirty = true;
 	if (p->warmAlloc) B3H_CHECK(hipMemsetAsync(p->wb.histValid, 0, sizeof(int), p->stream));
 	return B3H_OK;
 }
@@ -1689,6 +1799,323 @@ extern "C" int b3h_pipeline_get_body_buffer(b3hPipeline* p, b3hRigidBodyData** d
 	*dev = p->np->dBodies;
 	return B3H_OK;
 }
+/* BVH scratch of the ray caster for the broadphase's current small proxies */
+static int ray_ensure(b3hPipeline* p)
+{
+	const int ns = (int)p->bp->smallMap.size();
+	if (ns <= p->raySmallCap && p->raySc.bvhMeta) return B3H_OK;
+	b3hBroadphaseScratch& sc = p->raySc;
+	dfree(sc.keys);
+	dfree(sc.vals);
+	dfree(sc.tmpKeys);
+	dfree(sc.tmpVals);
+	dfree(sc.hist);
+	dfree(sc.sortedAabbs);
+	dfree(sc.bvhMerge);
+	dfree(sc.bvhChild);
+	dfree(sc.bvhBox);
+	dfree(sc.bvhFlag);
+	dfree(sc.bvhMeta);
+	p->raySmallCap = -1;
+	const int cap = ns + 64;
+	int r;
+	if ((r = dalloc(&sc.keys, cap)) || (r = dalloc(&sc.vals, cap)) || (r = dalloc(&sc.tmpKeys, cap)) ||
+		(r = dalloc(&sc.tmpVals, cap)) || (r = dalloc(&sc.hist, b3h_radix_hist_size(cap))) ||
+		(r = dalloc(&sc.sortedAabbs, cap)) || (r = dalloc(&sc.bvhMerge, b3h_plbvh_reduce_blocks(cap) + 1)) ||
+		(r = dalloc(&sc.bvhChild, cap)) || (r = dalloc(&sc.bvhBox, 2 * (size_t)cap)) ||
+		(r = dalloc(&sc.bvhFlag, cap)) || (r = dalloc(&sc.bvhMeta, 2)))
+		return r;
+	B3H_CHECK(hipMemset(sc.bvhMeta, 0, 2 * sizeof(int)));
+	p->raySmallCap = ns;
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_cast_rays_gpu(b3hPipeline* p, const b3hRayInfo* rays, b3hRayHit* hits, int numRays)
+{
+	CHECK_ARG(p && numRays >= 0);
+	if (numRays == 0) return B3H_OK;
+	CHECK_ARG(rays && hits);
+	int r = ray_ensure(p);
+	if (r) return r;
+	b3hBroadphase* bp = p->bp;
+	b3h_cast_rays(numRays, rays, hits, bp->dAll, bp->dSmall, (int)bp->smallMap.size(), bp->dLarge,
+				  (int)bp->largeMap.size(), p->np->dBodies, np_tables(p->np), p->raySc, p->stream);
+	B3H_CHECK(hipGetLastError());
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_cast_rays(b3hPipeline* p, const b3hRayInfo* rays, b3hRayHit* hits, int numRays)
+{
+	CHECK_ARG(p && numRays >= 0);
+	if (numRays == 0) return B3H_OK;
+	CHECK_ARG(rays && hits);
+	if (numRays > p->rayCap)
+	{
+		dfree(p->dRays);
+		dfree(p->dHits);
+		p->rayCap = 0;
+		int r;
+		if ((r = dalloc(&p->dRays, numRays)) || (r = dalloc(&p->dHits, numRays))) return r;
+		p->rayCap = numRays;
+	}
+	B3H_CHECK(hipMemcpyAsync(p->dRays, rays, sizeof(b3hRayInfo) * numRays, hipMemcpyHostToDevice, p->stream));
+	B3H_CHECK(hipMemcpyAsync(p->dHits, hits, sizeof(b3hRayHit) * numRays, hipMemcpyHostToDevice, p->stream));
+	int r = b3h_pipeline_cast_rays_gpu(p, p->dRays, p->dHits, numRays);
+	if (r) return r;
+	B3H_CHECK(hipMemcpyAsync(hits, p->dHits, sizeof(b3hRayHit) * numRays, hipMemcpyDeviceToHost, p->stream));
+	B3H_CHECK(hipStreamSynchronize(p->stream));
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_get_ray_bvh_info(b3hPipeline* p, int* root, int* overflow)
+{
+	CHECK_ARG(p && root && overflow);
+	if (!p->raySc.bvhMeta)
+	{
+		b3h_set_error("no ray cast yet");
+		return B3H_ERR_STATE;
+	}
+	int m[2];
+	B3H_CHECK(hipMemcpyAsync(m, p->raySc.bvhMeta, sizeof(m), hipMemcpyDeviceToHost, p->stream));
+	B3H_CHECK(hipStreamSynchronize(p->stream));
+	*root = m[0];
+	*overflow = m[1];
+	return B3H_OK;
+}
+
+/* ---- joints ---- */
+/* the device's enabled flags (cleared by breaking) into the host list */
+static int joints_pull_flags(b3hPipeline* p)
+{
+	if (p->jointsDirty || p->jb.numConstraints == 0 || (int)p->joints.size() != p->jb.numConstraints) return B3H_OK;
+	std::vector<b3hGpuGenericConstraint> dev((size_t)p->jb.numConstraints);
+	B3H_CHECK(hipMemcpyAsync(dev.data(), p->jb.constraints, dev.size() * sizeof(b3hGpuGenericConstraint),
+							 hipMemcpyDeviceToHost, p->stream));
+	B3H_CHECK(hipStreamSynchronize(p->stream));
+	for (size_t i = 0; i < dev.size(); i++) p->joints[i].m_flags = dev[i].m_flags;
+	return B3H_OK;
+}
+
+/* Greedy joint batches on the host, as b3GpuPgsConstraintSolver::
+ * sortConstraintByBatch3 (b3GpuPgsConstraintSolver.cpp:862-948) builds them
+ * with simdWidth = numConstraints + 1 (never reached, so the used-body map is
+ * cleared only between batches) and staticIdx -1: pass after pass over the
+ * constraints not yet batched, in their current order, a constraint joins
+ * the batch when neither of its non-static bodies is used yet, and is swapped
+ * to the front.  Static means a negative signed index (invMass 0,
+ * initBatchConstraintsKernel, jointSolver.cl:400-416), so body 0 counts as
+ * dynamic even when static (-0 == 0).  order receives the original constraint
+ * index per position, sizes the batch sizes. */
+static void joints_batch(const std::vector<b3hGpuGenericConstraint>& cs, const std::vector<b3hRigidBodyData>& bodies,
+						 std::vector<int>& order, std::vector<int>& sizes)
+{
+	struct BC
+	{
+		int a, b, orig;
+	};
+	const int n = (int)cs.size();
+	std::vector<BC> bc((size_t)n);
+	for (int i = 0; i < n; i++)
+	{
+		int ra = cs[i].m_rbA, rb = cs[i].m_rbB;
+		bc[i].a = bodies[ra].m_invMass != 0.f ? ra : -ra;
+		bc[i].b = bodies[rb].m_invMass != 0.f ? rb : -rb;
+		bc[i].orig = i;
+	}
+	const int staticIdx = -1;
+	std::vector<unsigned> used(bodies.size() / 32 + 1, 0u);
+	std::vector<int> cur;
+	sizes.clear();
+	int valid = 0;
+	while (valid < n)
+	{
+		int inBatch = 0;
+		for (int b : cur) used[b / 32] = 0;
+		cur.clear();
+		for (int i = valid; i < n; i++)
+		{
+			const int as = bc[i].a, bs = bc[i].b;
+			const int a = as < 0 ? -as : as, b = bs < 0 ? -bs : bs;
+			const bool aStatic = as < 0 || as == staticIdx, bStatic = bs < 0 || bs == staticIdx;
+			unsigned aUn = 0, bUn = 0;
+			if (!aStatic) aUn = used[a / 32] & (1u << (a & 31));
+			if (!aUn && !bStatic) bUn = used[b / 32] & (1u << (b & 31));
+			if (aUn == 0 && bUn == 0)
+			{
+				if (!aStatic)
+				{
+					used[a / 32] |= 1u << (a & 31);
+					cur.push_back(a);
+				}
+				if (!bStatic)
+				{
+					used[b / 32] |= 1u << (b & 31);
+					cur.push_back(b);
+				}
+				if (i != valid) std::swap(bc[i], bc[valid]);
+				valid++;
+				inBatch++;
+			}
+		}
+		sizes.push_back(inBatch);
+	}
+	order.resize((size_t)n);
+	for (int i = 0; i < n; i++) order[i] = bc[i].orig;
+}
+
+/* upload the host list and rebuild rows, slots and batches (the reference
+ * uploads at writeAllInstancesToGpu, b3GpuRigidBodyPipeline.cpp:597-601, and
+ * re-batches after recomputeBatches) */
+static int joints_upload(b3hPipeline* p)
+{
+	if (!p->jointsDirty) return B3H_OK;
+	drop_graph(p);
+	joint_free(p);
+	b3hJointBuffers& jb = p->jb;
+	const int n = (int)p->joints.size();
+	const int nb = num_bodies(p);
+	p->jointsDirty = false;
+	if (n == 0) return B3H_OK;
+	std::vector<int> rowOffset(n), rowCount(n), slotOf(2 * (size_t)n), bodySlot((size_t)nb, -1), order, sizes;
+	int rows = 0, slots = 0;
+	for (int i = 0; i < n; i++)
+	{
+		const b3hGpuGenericConstraint& c = p->joints[i];
+		rowCount[i] = c.m_constraintType == B3H_POINT2POINT_CONSTRAINT_TYPE ? 3 : (c.m_constraintType == B3H_FIXED_CONSTRAINT_TYPE ? 6 : 0);
+		rowOffset[i] = rows;
+		rows += rowCount[i];
+		const int ab[2] = {c.m_rbA, c.m_rbB};
+		for (int k = 0; k < 2; k++)
+		{
+			if (bodySlot[ab[k]] < 0) bodySlot[ab[k]] = slots++;
+			slotOf[2 * i + k] = bodySlot[ab[k]];
+		}
+	}
+	joints_batch(p->joints, p->np->bodies, order, sizes);
+	std::vector<int> start(sizes.size() + 1, 0);
+	for (size_t k = 0; k < sizes.size(); k++) start[k + 1] = start[k] + sizes[k];
+	int r;
+	if ((r = dalloc(&jb.constraints, n)) || (r = dalloc(&jb.rowOffset, n)) || (r = dalloc(&jb.rowCount, n)) ||
+		(r = dalloc(&jb.slotOf, 2 * (size_t)n)) || (r = dalloc(&jb.rows, rows > 0 ? rows : 1)) ||
+		(r = dalloc(&jb.batchStart, start.size())) || (r = dalloc(&jb.order, n)) || (r = dalloc(&jb.delta, 2 * (size_t)slots)) ||
+		(r = dalloc(&jb.bodySlot, nb > 0 ? nb : 1)))
+	{
+		joint_free(p);
+		p->jointsDirty = true;
+		return r;
+	}
+	hipStream_t s = p->stream;
+	B3H_CHECK(hipMemcpyAsync(jb.constraints, p->joints.data(), n * sizeof(b3hGpuGenericConstraint), hipMemcpyHostToDevice, s));
+	B3H_CHECK(hipMemcpyAsync(jb.rowOffset, rowOffset.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+	B3H_CHECK(hipMemcpyAsync(jb.rowCount, rowCount.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+	B3H_CHECK(hipMemcpyAsync(jb.slotOf, slotOf.data(), 2 * n * sizeof(int), hipMemcpyHostToDevice, s));
+	B3H_CHECK(hipMemcpyAsync(jb.batchStart, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, s));
+	B3H_CHECK(hipMemcpyAsync(jb.order, order.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+	if (nb > 0) B3H_CHECK(hipMemcpyAsync(jb.bodySlot, bodySlot.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
+	B3H_CHECK(hipStreamSynchronize(s));
+	jb.numConstraints = n;
+	jb.numBatches = (int)sizes.size();
+	jb.numSlots = slots;
+	jb.numBodies = nb;
+	return B3H_OK;
+}
+
+static int joint_check_bodies(b3hPipeline* p, int a, int b)
+{
+	const int nb = num_bodies(p);
+	if (a < 0 || a >= nb || b < 0 || b >= nb)
+	{
+		b3h_set_error("create constraint: body %d or %d out of range (%d bodies)", a, b, nb);
+		return B3H_ERR_INVALID;
+	}
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_create_point2point_constraint(b3hPipeline* p, int bodyA, int bodyB, const float* pivotInA3,
+														   const float* pivotInB3, float breakingThreshold, int* uidOut)
+{
+	CHECK_ARG(p && pivotInA3 && pivotInB3 && uidOut);
+	int r = joint_check_bodies(p, bodyA, bodyB);
+	if (r || (r = joints_pull_flags(p))) return r;
+	b3hGpuGenericConstraint c;
+	memset(&c, 0, sizeof(c));
+	c.m_uid = p->jointUid++;
+	c.m_flags = B3H_CONSTRAINT_FLAG_ENABLED;
+	c.m_rbA = bodyA;
+	c.m_rbB = bodyB;
+	c.m_pivotInA = f4(pivotInA3[0], pivotInA3[1], pivotInA3[2], 0.f);
+	c.m_pivotInB = f4(pivotInB3[0], pivotInB3[1], pivotInB3[2], 0.f);
+	c.m_relTargetAB = f4(0.f, 0.f, 0.f, 1.f);
+	c.m_breakingImpulseThreshold = breakingThreshold;
+	c.m_constraintType = B3H_POINT2POINT_CONSTRAINT_TYPE;
+	p->joints.push_back(c);
+	p->jointsDirty = true;
+	drop_graph(p);
+	*uidOut = c.m_uid;
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_create_fixed_constraint(b3hPipeline* p, int bodyA, int bodyB, const float* pivotInA3,
+													 const float* pivotInB3, const float* relTargetAB4,
+													 float breakingThreshold, int* uidOut)
+{
+	CHECK_ARG(p && pivotInA3 && pivotInB3 && relTargetAB4 && uidOut);
+	int r = joint_check_bodies(p, bodyA, bodyB);
+	if (r || (r = joints_pull_flags(p))) return r;
+	b3hGpuGenericConstraint c;
+	memset(&c, 0, sizeof(c));
+	c.m_uid = p->jointUid++;
+	c.m_flags = B3H_CONSTRAINT_FLAG_ENABLED;
+	c.m_rbA = bodyA;
+	c.m_rbB = bodyB;
+	c.m_pivotInA = f4(pivotInA3[0], pivotInA3[1], pivotInA3[2], 0.f);
+	c.m_pivotInB = f4(pivotInB3[0], pivotInB3[1], pivotInB3[2], 0.f);
+	c.m_relTargetAB = f4(relTargetAB4[0], relTargetAB4[1], relTargetAB4[2], relTargetAB4[3]);
+	c.m_breakingImpulseThreshold = breakingThreshold;
+	c.m_constraintType = B3H_FIXED_CONSTRAINT_TYPE;
+	p->joints.push_back(c);
+	p->jointsDirty = true;
+	drop_graph(p);
+	*uidOut = c.m_uid;
+	return B3H_OK;
+}
+
+extern "C" int b3h_pipeline_remove_constraint_by_uid(b3hPipeline* p, int uid)
+{
+	CHECK_ARG(p);
+	int r = joints_pull_flags(p);
+	if (r) return r;
+	for (size_t i = 0; i < p->joints.size(); i++)
+		if (p->joints[i].m_uid == uid)
+		{
+			std::swap(p->joints[i], p->joints.back());
+			p->joints.pop_back();
+			p->jointsDirty = true;
+			drop_graph(p);
+			return B3H_OK;
+		}
+	b3h_set_error("removeConstraintByUid: no constraint with uid %d", uid);
+	return B3H_ERR_INVALID;
+}
+
+extern "C" int b3h_pipeline_copy_constraints_to_host(b3hPipeline* p) { CHECK_ARG(p); return joints_pull_flags(p); }
+
+extern "C" int b3h_pipeline_get_constraints(b3hPipeline* p, b3hGpuGenericConstraint* host, int capacity, int* n)
+{
+	CHECK_ARG(p && n);
+	int r = joints_pull_flags(p);
+	if (r) return r;
+	*n = (int)p->joints.size();
+	if (!host) return B3H_OK;
+	if (capacity < *n)
+	{
+		b3h_set_error("getConstraints: capacity %d < %d constraints", capacity, *n);
+		return B3H_ERR_CAPACITY;
+	}
+	if (*n) memcpy(host, p->joints.data(), (size_t)*n * sizeof(b3hGpuGenericConstraint));
+	return B3H_OK;
+}
+
 extern "C" int b3h_pipeline_get_num_bodies(b3hPipeline* p, int* n)
 {
 	CHECK_ARG(p && n);

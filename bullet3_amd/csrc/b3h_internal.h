@@ -61,7 +61,13 @@ struct b3hBroadphaseScratch
 	unsigned* lmaxExt; /* float bits of the largest x-extent */
 	float* lminx;      /* min.x in sorted order */
 	const int* sceneOf; /* per proxy handle: its scene (NULL: one scene); pairs across scenes are dropped */
-	int2* stash;        /* grid pairs: the first B3H_GRID_STASH pairs of each small proxy (count pass) */
+	int2* stash;        /* grid and BVH pairs: the first B3H_GRID_STASH pairs of each small proxy (count pass) */
+	/* parallel linear BVH over the small proxies (BVH broadphase only) */
+	b3hAabb* bvhMerge;  /* per-block merged boxes, then the merged box */
+	int2* bvhChild;     /* internal nodes: (left, right), internal children marked by the sign bit */
+	b3hFloat4* bvhBox;  /* internal nodes: (min.xyz, highest leaf), (max.xyz, lowest leaf) */
+	int* bvhFlag;       /* internal nodes: children arrived (build) */
+	int* bvhMeta;       /* root, traversal stack overflow */
 };
 #define B3H_GRID_STASH 64
 #define B3H_LARGE_BRUTE_MAX 32
@@ -72,6 +78,11 @@ void b3h_sap_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const in
 				   b3hInt4* pairs, int maxPairs, int* pairCount, int* pairTotal, b3hBroadphaseScratch& sc,
 				   hipStream_t s);
 int b3h_sap_num_partials(int nSmall);
+int b3h_plbvh_reduce_blocks(int nSmall);
+void b3h_plbvh_build(const b3hAabb* all, const int* smallMap, int nSmall, b3hBroadphaseScratch& sc, hipStream_t s);
+void b3h_plbvh_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
+					 b3hInt4* pairs, int maxPairs, int* pairCount, int* pairTotal, b3hBroadphaseScratch& sc,
+					 hipStream_t s);
 void b3h_scene_contact_counts(const int* nPairs, int maxPairs, const b3hInt4* pairs, const int* sceneOf, int* counts,
 							  int numScenes, hipStream_t s);
 
@@ -92,6 +103,42 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 					 b3hFloat4* mprScratch, hipEvent_t evAfterMpr = nullptr);
 void b3h_gather_contacts(const int* contactPair, const int* numContacts, int maxContacts,
 						 const b3hContact4Data* staging, b3hContact4Data* out, hipStream_t s);
+
+/* ---------------- joints (k_joints.hip) ---------------- */
+struct b3hJointRow
+{
+	F4 r1xn, normal, r2xn, angA, angB;
+	float applied, jacInv, rhs, cfm, lo, hi, invMassA, invMassB;
+	int slotA, slotB, pad0, pad1;
+};
+struct b3hJointParams
+{
+	float timeStep, erp, globalCfm, damping;
+	int iterations;
+};
+struct b3hJointBuffers
+{
+	int numConstraints;
+	b3hGpuGenericConstraint* constraints; /* device copy (flags updated by breaking) */
+	int* rowOffset;  /* per constraint */
+	int* rowCount;   /* per constraint: 3, 6 or 0 */
+	int* slotOf;     /* per constraint: joint-body slots of A and B */
+	b3hJointRow* rows;
+	int numBatches;
+	int* batchStart; /* numBatches + 1 */
+	int* order;      /* constraint index per batched position */
+	F4* delta;       /* 2 per joint body: linear, angular velocity deltas */
+	int numSlots;
+	int* bodySlot;   /* per body: its joint-body slot or -1 */
+	int numBodies;
+};
+void b3h_solve_joints(const b3hJointBuffers& jb, b3hRigidBodyData* bodies, const b3hInertiaData* inertias,
+					  const b3hJointParams& jp, hipStream_t s);
+
+/* ---------------- ray casts (k_raycast.hip) ---------------- */
+void b3h_cast_rays(int numRays, const b3hRayInfo* rays, b3hRayHit* hits, const b3hAabb* all, const int* smallMap,
+				   int nSmall, const int* largeMap, int nLarge, const b3hRigidBodyData* bodies,
+				   const b3hShapeTables& sh, b3hBroadphaseScratch& sc, hipStream_t s);
 
 /* ---------------- solver (k_solver.hip) ---------------- */
 #define B3H_N_SPLIT_X 8

@@ -167,4 +167,30 @@ B3H_HD void plane_space(F4 n, F4* p, F4* q)
 	q->w = 0.f;
 }
 
+
+/* arc cosine of x in [-1, 1], from the arcsine series
+ * asin(y) = sum_n (2n)! / (4^n (n!)^2 (2n+1)) y^(2n+1) on |y| <= 1/2 (terms
+ * shrink at least fourfold, 30 of them reach double precision) with
+ * acos(x) = 2 asin(sqrt((1-x)/2)) for x >= 1/2, pi - 2 asin(sqrt((1+x)/2)) for
+ * x <= -1/2 and pi/2 - asin(x) between; evaluated in double and rounded once,
+ * so host and device give the same float */
+B3H_HD double b3h_asin_series(double y)
+{
+	double y2 = y * y, term = y, sum = y;
+	for (int n = 1; n <= 30; n++)
+	{
+		term = term * (y2 * (double)((2 * n - 1) * (2 * n - 1)) / (double)((2 * n) * (2 * n + 1)));
+		sum = sum + term;
+	}
+	return sum;
+}
+B3H_HD float b3h_acos(float xf)
+{
+	const double pi = 3.141592653589793;
+	double x = (double)xf;
+	if (x >= 0.5) return (float)(2.0 * b3h_asin_series(sqrt((1.0 - x) * 0.5)));
+	if (x <= -0.5) return (float)(pi - 2.0 * b3h_asin_series(sqrt((1.0 + x) * 0.5)));
+	return (float)(pi * 0.5 - b3h_asin_series(x));
+}
+
 #endif

@@ -28,10 +28,17 @@ CONSTRAINT = np.dtype([("linear", F4), ("worldPos", np.float32, (4, 4)), ("cente
                        ("fAppliedRambdaDt", np.float32, (2,)), ("bodyA", np.uint32), ("bodyB", np.uint32),
                        ("batchIdx", np.int32), ("paddings", np.uint32)])
 PAIR = np.dtype([("x", np.int32), ("y", np.int32), ("z", np.int32), ("w", np.int32)])
+CONSTRAINT_JOINT = np.dtype([("constraintType", np.int32), ("rbA", np.int32), ("rbB", np.int32),
+                             ("breakingImpulseThreshold", np.float32), ("pivotInA", F4), ("pivotInB", F4),
+                             ("relTargetAB", F4), ("flags", np.int32), ("uid", np.int32), ("padding", np.int32, (2,))])
+RAYINFO = np.dtype([("from", F4), ("to", F4)])
+RAYHIT = np.dtype([("hitFraction", np.float32), ("hitBody", np.int32), ("hitResult1", np.int32),
+                   ("hitResult2", np.int32), ("hitPoint", F4), ("hitNormal", F4)])
 
 assert BODY.itemsize == 80 and INERTIA.itemsize == 96 and COLLIDABLE.itemsize == 16
 assert CONVEX.itemsize == 96 and FACE.itemsize == 32 and AABB.itemsize == 32
 assert CONTACT.itemsize == 112 and CONSTRAINT.itemsize == 176 and PAIR.itemsize == 16
+assert RAYINFO.itemsize == 32 and RAYHIT.itemsize == 48 and CONSTRAINT_JOINT.itemsize == 80
 
 
 def aabb_handle(aabbs):

@@ -34,7 +34,7 @@ class World:
         self.bp = ctypes.c_void_p()
         self.pl = ctypes.c_void_p()
         call("b3h_np_create", ctypes.byref(self.cfg), None, ctypes.byref(self.np_))
-        kind = capi.BROADPHASE_GRID if broadphase == "grid" else capi.BROADPHASE_SAP
+        kind = capi.BROADPHASE_KINDS[broadphase]
         call("b3h_bp_create", kind, ctypes.byref(self.cfg), None, ctypes.byref(self.bp))
         cs = float(cell_size if cell_size is not None else scene.cell_size)
         self.cell_size = cs
@@ -252,6 +252,48 @@ class World:
         a, b = ctypes.c_int(), ctypes.c_int()
         call("b3h_pipeline_get_overflow", self.pl, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    # -- joints (createPoint2PointConstraint / createFixedConstraint)
+    def add_point2point(self, a, b, pivot_a, pivot_b, breaking=1e30):
+        uid = ctypes.c_int(-1)
+        call("b3h_pipeline_create_point2point_constraint", self.pl, int(a), int(b),
+             ptr(np.ascontiguousarray(pivot_a, np.float32)), ptr(np.ascontiguousarray(pivot_b, np.float32)),
+             ctypes.c_float(breaking), ctypes.byref(uid))
+        return uid.value
+
+    def add_fixed(self, a, b, pivot_a, pivot_b, rel_target=(0.0, 0.0, 0.0, 1.0), breaking=1e30):
+        uid = ctypes.c_int(-1)
+        call("b3h_pipeline_create_fixed_constraint", self.pl, int(a), int(b),
+             ptr(np.ascontiguousarray(pivot_a, np.float32)), ptr(np.ascontiguousarray(pivot_b, np.float32)),
+             ptr(np.ascontiguousarray(rel_target, np.float32)), ctypes.c_float(breaking), ctypes.byref(uid))
+        return uid.value
+
+    def remove_constraint(self, uid):
+        call("b3h_pipeline_remove_constraint_by_uid", self.pl, int(uid))
+
+    def constraints(self):
+        """the pipeline's constraint list, with the device's enabled flags"""
+        n = ctypes.c_int()
+        call("b3h_pipeline_get_constraints", self.pl, None, 0, ctypes.byref(n))
+        out = np.zeros(n.value, layout.CONSTRAINT_JOINT)
+        if n.value:
+            call("b3h_pipeline_get_constraints", self.pl, ptr(out), n.value, ctypes.byref(n))
+        return out
+
+    def aabbs(self):
+        """the broadphase's AABBs (getAllAabbsGPU), as the last step left them"""
+        d, n = ctypes.c_void_p(), ctypes.c_int()
+        call("b3h_bp_get_all_aabbs_gpu", self.bp, ctypes.byref(d), ctypes.byref(n))
+        call("b3h_synchronize")
+        return capi.dtoh(d.value, 32 * n.value, np.uint8).copy().view(layout.AABB)
+
+    def cast_rays(self, rays, hits):
+        """castRays on host arrays (layout.RAYINFO, layout.RAYHIT); returns the
+        updated hit records"""
+        rays = np.ascontiguousarray(rays)
+        out = np.ascontiguousarray(hits).copy()
+        call("b3h_pipeline_cast_rays", self.pl, ptr(rays), ptr(out), len(rays))
+        return out
 
     def close(self):
         if self.pl:

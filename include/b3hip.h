@@ -42,6 +42,7 @@ typedef struct b3hMlp b3hMlp;
 
 #define B3H_BROADPHASE_GRID 0
 #define B3H_BROADPHASE_SAP 1
+#define B3H_BROADPHASE_PLBVH 2
 
 /* Text of the last error on this thread ("" if none). */
 const char* b3h_last_error(void);
@@ -132,8 +133,14 @@ int b3h_np_get_mpr_debug(b3hNarrowPhase* np, b3hFloat4** mprDev);
 int b3h_np_get_sat_debug(b3hNarrowPhase* np, int** sepStateDev, b3hFloat4** sepNormalDev);
 
 /* ---------------- b3GpuBroadphaseInterface ---------------- */
-/* b3GpuGridBroadphase / b3GpuSapBroadphase constructors
- * (b3GpuGridBroadphase.cpp:24-81, b3GpuSapBroadphase.cpp:20-170) */
+/* b3GpuGridBroadphase / b3GpuSapBroadphase / b3GpuParallelLinearBvhBroadphase
+ * constructors (b3GpuGridBroadphase.cpp:24-81, b3GpuSapBroadphase.cpp:20-170,
+ * b3GpuParallelLinearBvhBroadphase.cpp:16-24), kind B3H_BROADPHASE_GRID / _SAP /
+ * _PLBVH.  The BVH broadphase rebuilds a linear BVH over the small proxies on
+ * every calculateOverlappingPairs and pairs each small proxy with the later
+ * ones in Morton order as (earlier, later), then with the large proxies as
+ * (large, small), unnormalised as the reference writes them
+ * (b3GpuParallelLinearBvh.cpp:120-355). */
 int b3h_bp_create(int kind, const b3hConfig* cfg, void* hipStream, b3hBroadphase** out);
 int b3h_bp_destroy(b3hBroadphase* bp);
 /* Grid cell size, grid dimensions (powers of two) and bucket cap; the reference
@@ -176,6 +183,11 @@ int b3h_bp_get_overlapping_pair_buffer(b3hBroadphase* bp, b3hInt4** dev);
 int b3h_bp_get_total_found(b3hBroadphase* bp, int* n);
 /* SAP axis chosen by the last calculateOverlappingPairs (syncs). */
 int b3h_bp_get_sap_axis(b3hBroadphase* bp, int* axis);
+/* BVH broadphase: root node of the last build (internal nodes carry the sign
+ * bit, as m_rootNodeIndex, b3GpuParallelLinearBvh.h:70) and whether a
+ * traversal ran out of stack (0: never; the stack covers the deepest tree 64-bit
+ * keys allow) -- syncs. */
+int b3h_bp_get_bvh_info(b3hBroadphase* bp, int* rootNode, int* stackOverflow);
 /* Host access to the proxy lists (copies). */
 int b3h_bp_get_proxy_maps(b3hBroadphase* bp, int* counts2, int* smallMap, int* largeMap);
 /* Debug view of the grid's last (cell hash, small-proxy index) arrays after the
@@ -217,6 +229,46 @@ int b3h_pipeline_reset(b3hPipeline* p);
 /* getBodyBuffer / getNumBodies, b3GpuRigidBodyPipeline.cpp:577-587 */
 int b3h_pipeline_get_body_buffer(b3hPipeline* p, b3hRigidBodyData** dev);
 int b3h_pipeline_get_num_bodies(b3hPipeline* p, int* n);
+/* castRays(rays, hitResults), b3GpuRigidBodyPipeline.h:63,
+ * b3GpuRigidBodyPipeline.cpp:671-677 -> b3GpuRaycast::castRays, b3GpuRaycast.cpp:
+ * 245-373: host arrays of numRays rays and hit records (in: m_hitResult2 = a body
+ * the ray ignores, -1 for none; out: m_hitFraction (1 if nothing is hit),
+ * m_hitBody, m_hitPoint, m_hitNormal of the nearest hit, the last three left as
+ * given when nothing is hit; a hull's normal is its face normal in the body's
+ * own frame, as the reference leaves it, rayCastKernels.cl:191-193,383-389).  Candidates are the bodies whose broadphase AABB
+ * (as the last step computed it, before its integration) the segment crosses,
+ * found through a linear BVH rebuilt per call; convex hulls and spheres are
+ * tested (rayCastKernels.cl:155-240).  Syncs. */
+int b3h_pipeline_cast_rays(b3hPipeline* p, const b3hRayInfo* rays, b3hRayHit* hitResults, int numRays);
+/* The same on device arrays, queued on the pipeline stream without a host
+ * sync.  No counterpart in the reference, whose castRays copies host arrays. */
+int b3h_pipeline_cast_rays_gpu(b3hPipeline* p, const b3hRayInfo* raysDev, b3hRayHit* hitResultsDev, int numRays);
+/* Root of the ray caster's last BVH and whether a traversal ran out of stack
+ * (0: never) -- syncs.  For tests. */
+int b3h_pipeline_get_ray_bvh_info(b3hPipeline* p, int* rootNode, int* stackOverflow);
+/* Joints.  createPoint2PointConstraint(bodyA, bodyB, pivotInA, pivotInB,
+ * breakingThreshold) / createFixedConstraint(..., relTargetAB, ...),
+ * b3GpuRigidBodyPipeline.h:56-57, .cpp:186-219: append an enabled constraint to
+ * the pipeline's list and write its uid.  removeConstraintByUid (:58,
+ * .cpp:159-184): the last constraint takes the removed one's place.
+ * copyConstraintsToHost (:52, .cpp:594-596): the device's enabled flags (a
+ * constraint is disabled when a row impulse reaches its breaking threshold)
+ * into the host list; get_constraints copies that list out (capacity entries;
+ * host NULL reports the count).  The list reaches the device at
+ * writeAllInstancesToGpu or, when it changed since, at the next step; every
+ * step then solves the joints (b3GpuPgsConstraintSolver::solveJoints: 4
+ * iterations, dt 1/60, erp 0.2, b3GpuPgsConstraintSolver.cpp:927-937) after the
+ * narrowphase and before the contact solve, as stepSimulation does
+ * (b3GpuRigidBodyPipeline.cpp:361-372).  Changes to the list pull the device
+ * flags first, so breaking survives them. */
+int b3h_pipeline_create_point2point_constraint(b3hPipeline* p, int bodyA, int bodyB, const float* pivotInA3,
+											   const float* pivotInB3, float breakingThreshold, int* uidOut);
+int b3h_pipeline_create_fixed_constraint(b3hPipeline* p, int bodyA, int bodyB, const float* pivotInA3,
+										 const float* pivotInB3, const float* relTargetAB4, float breakingThreshold,
+										 int* uidOut);
+int b3h_pipeline_remove_constraint_by_uid(b3hPipeline* p, int uid);
+int b3h_pipeline_copy_constraints_to_host(b3hPipeline* p);
+int b3h_pipeline_get_constraints(b3hPipeline* p, b3hGpuGenericConstraint* host, int capacity, int* n);
 /* Solver parameters (iterations, friction, dt, Baumgarte, drift, cell size,
  * damping, warm start); the reference hard-codes them. */
 int b3h_pipeline_set_solver_params(b3hPipeline* p, const b3hSolverParams* prm);

@@ -59,9 +59,16 @@ typedef struct b3hInertiaData
 	b3hMat3x3 m_initInvInertia;
 } b3hInertiaData;
 
+/* collidable shape types, shared/b3Collidable.h:8-18 (SHAPE_CONVEX_HULL = 3,
+ * SHAPE_PLANE = 4, SHAPE_CONCAVE_TRIMESH = 5, SHAPE_COMPOUND_OF_CONVEX_HULLS = 6,
+ * SHAPE_SPHERE = 7) */
 enum b3hShapeTypes
 {
-	B3H_SHAPE_CONVEX_HULL = 3
+	B3H_SHAPE_CONVEX_HULL = 3,
+	B3H_SHAPE_PLANE = 4,
+	B3H_SHAPE_CONCAVE_TRIMESH = 5,
+	B3H_SHAPE_COMPOUND_OF_CONVEX_HULLS = 6,
+	B3H_SHAPE_SPHERE = 7
 };
 
 typedef struct b3hCollidable
@@ -162,6 +169,43 @@ typedef struct b3hContactConstraint4
 	int m_batchIdx;
 	unsigned int m_paddings;
 } b3hContactConstraint4;
+
+/* b3GpuGenericConstraint, src/Bullet3OpenCL/RigidBody/b3GpuGenericConstraint.h:
+ * 20-84: a joint between bodies m_rbA and m_rbB (point-to-point: type 3, three
+ * rows; fixed: type 4, six rows), enabled while m_flags has bit 1 set (the
+ * solver clears it when a row's impulse reaches m_breakingImpulseThreshold). */
+#define B3H_CONSTRAINT_FLAG_ENABLED 1
+#define B3H_POINT2POINT_CONSTRAINT_TYPE 3
+#define B3H_FIXED_CONSTRAINT_TYPE 4
+typedef struct __attribute__((aligned(16))) b3hGpuGenericConstraint
+{
+	int m_constraintType;
+	int m_rbA;
+	int m_rbB;
+	float m_breakingImpulseThreshold;
+	b3hFloat4 m_pivotInA;
+	b3hFloat4 m_pivotInB;
+	b3hFloat4 m_relTargetAB; /* quaternion */
+	int m_flags;
+	int m_uid;
+	int m_padding[2];
+} b3hGpuGenericConstraint;
+
+/* b3RayInfo / b3RayHit, src/Bullet3Collision/NarrowPhaseCollision/b3RaycastInfo.h:7-23 */
+typedef struct b3hRayInfo
+{
+	b3hFloat4 m_from;
+	b3hFloat4 m_to;
+} b3hRayInfo;
+typedef struct __attribute__((aligned(16))) b3hRayHit
+{
+	float m_hitFraction;
+	int m_hitBody;
+	int m_hitResult1;
+	int m_hitResult2; /* input: a body the ray ignores (-1: none) */
+	b3hFloat4 m_hitPoint;
+	b3hFloat4 m_hitNormal;
+} b3hRayHit;
 
 typedef struct b3hSortData
 {

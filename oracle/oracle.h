@@ -31,6 +31,14 @@ int b3o_grid_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const in
 int b3o_sap_axis(const b3hAabb* all, const int* smallMap, int nSmall);
 int b3o_sap_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
 				  int axisOverride, b3hInt4* pairsOut, int maxPairs, int* totalFound, int* axisOut);
+/* parallel linear BVH (pairs in the device's fixed order, see oracle_bp.cpp);
+ * rootOut (may be NULL): the root node, sign bit set for an internal node */
+int b3o_plbvh_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
+					b3hInt4* pairsOut, int maxPairs, int* totalFound, int* rootOut);
+/* ray casts against the bodies (nearest hit per ray; hits in/out) */
+int b3o_cast_rays(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
+				  const b3hRigidBodyData* bodies, const b3hCollidable* cols, const b3hConvexPolyhedronData* convexes,
+				  const b3hGpuFace* faces, const b3hRayInfo* rays, b3hRayHit* hits, int numRays);
 int b3o_brute_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const int* largeMap, int nLarge,
 					b3hInt4* pairsOut, int maxPairs);
 
@@ -44,6 +52,11 @@ int b3o_narrowphase(b3hInt4* pairs, int nPairs, const b3hRigidBodyData* bodies,
 					const b3hFloat4* vertices, const b3hFloat4* uniqueEdges, const b3hGpuFace* faces,
 					const int* indices, b3hContact4Data* contacts, int capacity,
 					int* hasSepOut, b3hFloat4* sepOut);
+
+/* joint solver (constraints' flags updated by breaking; bodies' velocities in
+ * place); returns the number of batches */
+int b3o_solve_joints(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaData* inertias,
+					 b3hGpuGenericConstraint* constraints, int numConstraints, float timeStep, int iterations);
 
 /* solver front end + PGS (contacts in, bodies updated in place) */
 int b3o_solve_contacts(int numBodies, b3hRigidBodyData* bodies, const b3hInertiaData* inertias,
@@ -74,7 +87,7 @@ typedef struct b3oWorld
 	int nSmall;
 	const int* largeMap;
 	int nLarge;
-	int broadphase; /* 0 uniform grid, 1 SAP */
+	int broadphase; /* 0 uniform grid, 1 SAP, 2 parallel linear BVH */
 	float cellSize[3];
 	int gridDims[3];
 	int maxBodiesPerCell;
@@ -91,6 +104,9 @@ typedef struct b3oWorld
 	int numPairs;
 	int numContacts;
 	float* residuals; /* optional, 2*iterations */
+	/* joints (optional), solved after the narrowphase, before the contacts */
+	b3hGpuGenericConstraint* constraints;
+	int numConstraints;
 } b3oWorld;
 
 #ifdef __cplusplus

@@ -16,8 +16,10 @@ extern "C" int b3o_step(b3oWorld* w, float dt)
 	if (w->broadphase == 0)
 		np = b3o_grid_pairs(w->worldAabbs, w->smallMap, w->nSmall, w->largeMap, w->nLarge, w->cellSize, w->gridDims,
 							w->maxBodiesPerCell, w->pairs, w->maxPairs, 0);
-	else
+	else if (w->broadphase == 1)
 		np = b3o_sap_pairs(w->worldAabbs, w->smallMap, w->nSmall, w->largeMap, w->nLarge, -1, w->pairs, w->maxPairs, 0, 0);
+	else
+		np = b3o_plbvh_pairs(w->worldAabbs, w->smallMap, w->nSmall, w->largeMap, w->nLarge, w->pairs, w->maxPairs, 0, 0);
 	w->numPairs = np;
 	int nc = 0;
 	if (np)
@@ -27,6 +29,9 @@ extern "C" int b3o_step(b3oWorld* w, float dt)
 		if (nc > w->maxContacts) nc = w->maxContacts;
 	}
 	w->numContacts = nc;
+	/* joints before the contacts (b3GpuRigidBodyPipeline.cpp:361-372) */
+	if (w->numConstraints > 0 && w->constraints)
+		b3o_solve_joints(w->numBodies, w->bodies, w->inertias, w->constraints, w->numConstraints, 1.f / 60.f, 4);
 	if (nc)
 		b3o_solve_contacts(w->numBodies, w->bodies, w->inertias, nc, w->contacts, &w->solver, w->staticIdx, 0, 0, 0, 0, 0,
 						   0, w->residuals);

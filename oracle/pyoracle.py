@@ -58,7 +58,8 @@ class World(ctypes.Structure):
                 ("maxPairs", ctypes.c_int), ("maxContacts", ctypes.c_int), ("solver", SolverParams),
                 ("gravity", ctypes.c_float * 3), ("staticIdx", ctypes.c_int),
                 ("worldAabbs", ctypes.c_void_p), ("pairs", ctypes.c_void_p), ("contacts", ctypes.c_void_p),
-                ("numPairs", ctypes.c_int), ("numContacts", ctypes.c_int), ("residuals", ctypes.c_void_p)]
+                ("numPairs", ctypes.c_int), ("numContacts", ctypes.c_int), ("residuals", ctypes.c_void_p),
+                ("constraints", ctypes.c_void_p), ("numConstraints", ctypes.c_int)]
 
 
 def world_aabbs(bodies, collidables, local_aabbs):
@@ -98,6 +99,47 @@ def sap_pairs(aabbs, small, large, axis=-1, max_pairs=None):
     n = lib().b3o_sap_pairs(ptr(aabbs), ptr(small), len(small), ptr(large), len(large), axis, ptr(out), max_pairs,
                             ctypes.byref(total), ctypes.byref(ax))
     return out[:n].copy(), total.value, ax.value
+
+
+def plbvh_pairs(aabbs, small, large, max_pairs=None):
+    """(pairs in the device's order, total found, root node)"""
+    small = np.ascontiguousarray(small, np.int32)
+    large = np.ascontiguousarray(large, np.int32)
+    if max_pairs is None:
+        max_pairs = 64 * max(1, len(small)) + 16
+    out = np.zeros((max_pairs, 4), np.int32)
+    total = ctypes.c_int(0)
+    root = ctypes.c_int(0)
+    n = lib().b3o_plbvh_pairs(ptr(aabbs), ptr(small), len(small), ptr(large), len(large), ptr(out), max_pairs,
+                              ctypes.byref(total), ctypes.byref(root))
+    return out[:n].copy(), total.value, root.value
+
+
+def world_aabbs(bodies, collidables, local_aabbs):
+    """broadphase AABBs of the bodies (b3ComputeWorldAabb, margin 0)"""
+    from bullet3_amd import layout
+    out = np.zeros(len(bodies), layout.AABB)
+    lib().b3o_compute_world_aabbs(len(bodies), ptr(bodies), ptr(collidables), ptr(local_aabbs), ptr(out))
+    return out
+
+
+def cast_rays(aabbs, small, large, bodies, tables, rays, hits):
+    """nearest hit of each ray (hits: RAYHIT records, updated copy returned)"""
+    small = np.ascontiguousarray(small, np.int32)
+    large = np.ascontiguousarray(large, np.int32)
+    rays = np.ascontiguousarray(rays)
+    out = np.ascontiguousarray(hits).copy()
+    lib().b3o_cast_rays(ptr(aabbs), ptr(small), len(small), ptr(large), len(large), ptr(bodies),
+                        ptr(tables["collidables"]), ptr(tables["convexes"]), ptr(tables["faces"]), ptr(rays), ptr(out),
+                        len(rays))
+    return out
+
+
+def solve_joints(bodies, inertias, constraints, time_step=1.0 / 60.0, iterations=4):
+    """one joint solve in place (bodies' velocities, constraints' flags);
+    returns the number of batches"""
+    return lib().b3o_solve_joints(len(bodies), ptr(bodies), ptr(inertias), ptr(constraints), len(constraints),
+                                  ctypes.c_float(time_step), int(iterations))
 
 
 def brute_pairs(aabbs, small, large, max_pairs=None):

@@ -97,6 +97,9 @@ class OracleWorld:
         w.pairs = self.pairs.ctypes.data
         w.contacts = self.contacts.ctypes.data
         w.residuals = self.resid.ctypes.data
+        self.joints = np.ascontiguousarray(world.constraints())
+        w.constraints = self.joints.ctypes.data if len(self.joints) else None
+        w.numConstraints = len(self.joints)
         self.w = w
 
     def step(self, dt=1.0 / 60.0):

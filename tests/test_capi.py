@@ -151,3 +151,35 @@ def test_public_headers_compile_standalone(tmp_path):
     cc.write_text('#include "b3hip.hpp"\nint main(){ b3hConfig c; b3h_config_default(&c); '
                   'b3GpuNarrowPhase np(c); return np.isValid() ? 0 : 1; }\n')
     subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", inc, str(cc)], check=True)
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """the ctypes mirrors of the ABI structs have the header's size and field
+    offsets (a C probe compiled with gcc prints them)"""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    mirrors = {"b3hConfig": capi.Config, "b3hSolverParams": capi.SolverParams,
+               "b3hConvexUtility": capi.ConvexUtility}
+    lines = []
+    for cname, cls in mirrors.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f in cls._fields_:
+            if f[0].startswith("pad"):
+                continue
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f[0], cname, f[0]))
+    src = tmp_path / "probe.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "b3hip_types.h"\nint main(void){\n%s\nreturn 0;}\n'
+                   % "\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run([gcc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                  text=True).stdout.splitlines())
+    for cname, cls in mirrors.items():
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f in cls._fields_:
+            if f[0].startswith("pad"):
+                continue
+            assert int(got["%s.%s" % (cname, f[0])]) == getattr(cls, f[0]).offset, (cname, f[0])

@@ -1,0 +1,115 @@
+"""The BVH broadphase on the device (kind B3H_BROADPHASE_PLBVH; the reference's
+b3GpuParallelLinearBvhBroadphase, src/Bullet3OpenCL/BroadphaseCollision/
+b3GpuParallelLinearBvh.cpp:120-355) against the oracle, which builds the tree
+the reference's way (binary searches for parents, per-depth boxes) where the
+device builds it bottom-up in one pass:
+
+* the pair list equals the oracle's bit for bit, order included (the same root
+  and the same traversal order mean the same tree), on uniform, duplicated,
+  coplanar and clustered clouds, with and without large proxies, and on 0-3
+  proxies; no traversal runs out of stack;
+* at 1,048,576 AABBs (the broadphase stress cloud) the list equals the
+  oracle's, and as a normalised set it equals the uniform grid's;
+* a pile stepped with the BVH broadphase stays bit for bit with the oracle
+  stepping with its BVH broadphase.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from bullet3_amd import capi, scenes
+from bullet3_amd.world import World
+from helpers import OracleWorld, bits_equal, first_diff
+from test_gpu_broadphase_stress import device_pairs, normalized
+from test_oracle_plbvh import cloud, records
+
+pytestmark = pytest.mark.gpu
+
+
+def device_plbvh(mn, mx, large_mn=None, large_mx=None):
+    n = len(mn)
+    nl = 0 if large_mn is None else len(large_mn)
+    cfg = capi.default_config(n + nl + 64, 16)
+    bp = ctypes.c_void_p()
+    capi.call("b3h_bp_create", capi.BROADPHASE_PLBVH, ctypes.byref(cfg), None, ctypes.byref(bp))
+    try:
+        if n:
+            capi.call("b3h_bp_create_proxies", bp, n, capi.ptr(mn), capi.ptr(mx), None)
+        for j in range(nl):
+            capi.call("b3h_bp_create_large_proxy", bp, capi.ptr(large_mn[j]), capi.ptr(large_mx[j]), n + j, 0, 0)
+        capi.call("b3h_bp_write_aabbs_to_gpu", bp)
+        capi.call("b3h_bp_calculate_overlapping_pairs", bp, cfg.m_maxBroadphasePairs)
+        cnt, tot = ctypes.c_int(), ctypes.c_int()
+        capi.call("b3h_bp_get_num_overlap", bp, ctypes.byref(cnt))
+        capi.call("b3h_bp_get_total_found", bp, ctypes.byref(tot))
+        d = ctypes.c_void_p()
+        capi.call("b3h_bp_get_overlapping_pair_buffer", bp, ctypes.byref(d))
+        pairs = capi.dtoh(d.value, cnt.value * 16, np.int32).reshape(-1, 4).copy()
+        root, ovf = ctypes.c_int(), ctypes.c_int()
+        capi.call("b3h_bp_get_bvh_info", bp, ctypes.byref(root), ctypes.byref(ovf))
+        return pairs, tot.value, root.value, ovf.value
+    finally:
+        capi.call("b3h_bp_destroy", bp)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "duplicates", "coplanar", "clustered"])
+@pytest.mark.parametrize("n,nl", [(2000, 0), (1500, 3), (40000, 40)])
+def test_plbvh_pairs_equal_oracle(kind, n, nl):
+    mn, mx = cloud(kind, n + nl, n * 7 + nl)
+    mn, mx = np.ascontiguousarray(mn), np.ascontiguousarray(mx)
+    print("plbvh case", kind, n, nl, flush=True)
+    dp, dtot, droot, ovf = device_plbvh(mn[:n], mx[:n], mn[n:], mx[n:])
+    a = records(mn, mx)
+    op, otot, oroot = po.plbvh_pairs(a, np.arange(n, dtype=np.int32), np.arange(n, n + nl, dtype=np.int32),
+                                      max_pairs=64 * n)
+    assert ovf == 0
+    assert dtot == otot == len(dp)
+    assert droot == oroot
+    assert np.array_equal(dp, op)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3])
+def test_plbvh_tiny(n):
+    mn = np.zeros((n, 3), np.float32)
+    mx = np.ones((n, 3), np.float32)
+    mx[:, 0] += np.arange(n, dtype=np.float32)
+    dp, dtot, droot, ovf = device_plbvh(mn, mx)
+    op, otot, oroot = po.plbvh_pairs(records(mn, mx), np.arange(n, dtype=np.int32), np.zeros(0, np.int32))
+    assert dtot == otot == n * (n - 1) // 2
+    assert np.array_equal(dp, op)
+    if n:
+        assert droot == oroot
+
+
+def test_plbvh_million_aabbs():
+    N = 1 << 20
+    mn, mx, side = scenes.aabb_cloud(N)
+    mn = np.ascontiguousarray(mn, np.float32)
+    mx = np.ascontiguousarray(mx, np.float32)
+    dp, dtot, droot, ovf = device_plbvh(mn, mx)
+    assert ovf == 0 and dtot == len(dp)
+    op, otot, oroot = po.plbvh_pairs(records(mn, mx), np.arange(N, dtype=np.int32), np.zeros(0, np.int32),
+                                      max_pairs=16 * N)
+    assert droot == oroot and otot == dtot
+    assert np.array_equal(dp, op)
+    grid, gtot, _ = device_pairs(capi.BROADPHASE_GRID, mn, mx)
+    assert np.array_equal(normalized(dp), normalized(grid)) and len(normalized(dp)) == len(dp)
+
+
+def test_pile_steps_with_plbvh_bit_exact():
+    sc = scenes.convex_pile(6, 3, 6, seed=5)
+    w = World(sc, broadphase="plbvh")
+    sp = w.solver_params()
+    sp.m_numIterations = 10
+    w.set_solver(sp)
+    ow = OracleWorld(w, broadphase=2)
+    for s in range(40):
+        w.step()
+        ow.step()
+        if s % 10 == 9:
+            gb = w.bodies()
+            assert bits_equal(gb, ow.bodies), "step %d: %s" % (s, first_diff(gb, ow.bodies))
+    assert ow.w.numContacts > 0
+    w.close()
