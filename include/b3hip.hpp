@@ -225,6 +225,16 @@ public:
 	}
 };
 
+/* b3GpuParallelLinearBvhBroadphase, b3GpuParallelLinearBvhBroadphase.h:22-66 */
+class b3GpuParallelLinearBvhBroadphase : public b3GpuBroadphaseInterface
+{
+public:
+	explicit b3GpuParallelLinearBvhBroadphase(const b3hConfig& config, void* hipStream = nullptr)
+		: b3GpuBroadphaseInterface(B3H_BROADPHASE_PLBVH, config, hipStream)
+	{
+	}
+};
+
 class b3GpuRigidBodyPipeline
 {
 	b3hPipeline* m_p = nullptr;
@@ -271,6 +281,34 @@ public:
 		b3hip::ok(b3h_pipeline_get_num_bodies(m_p, &n));
 		return n;
 	}
+	/* castRays(rays, hitResults), b3GpuRigidBodyPipeline.h:63 (host arrays) */
+	void castRays(const std::vector<b3hRayInfo>& rays, std::vector<b3hRayHit>& hitResults)
+	{
+		if (hitResults.size() < rays.size()) hitResults.resize(rays.size());
+		b3hip::ok(b3h_pipeline_cast_rays(m_p, rays.data(), hitResults.data(), (int)rays.size()));
+	}
+	/* createPoint2PointConstraint / createFixedConstraint / removeConstraintByUid /
+	 * copyConstraintsToHost, b3GpuRigidBodyPipeline.h:52-58; return the uid or -1 */
+	int createPoint2PointConstraint(int bodyA, int bodyB, const float* pivotInA, const float* pivotInB,
+									float breakingThreshold)
+	{
+		int uid = -1;
+		return b3hip::ok(b3h_pipeline_create_point2point_constraint(m_p, bodyA, bodyB, pivotInA, pivotInB,
+																	 breakingThreshold, &uid))
+				   ? uid
+				   : -1;
+	}
+	int createFixedConstraint(int bodyA, int bodyB, const float* pivotInA, const float* pivotInB,
+							  const float* relTargetAB, float breakingThreshold)
+	{
+		int uid = -1;
+		return b3hip::ok(b3h_pipeline_create_fixed_constraint(m_p, bodyA, bodyB, pivotInA, pivotInB, relTargetAB,
+															   breakingThreshold, &uid))
+				   ? uid
+				   : -1;
+	}
+	void removeConstraintByUid(int uid) { b3hip::ok(b3h_pipeline_remove_constraint_by_uid(m_p, uid)); }
+	void copyConstraintsToHost() { b3hip::ok(b3h_pipeline_copy_constraints_to_host(m_p)); }
 	/* solver parameters the reference hard-codes (iterations, friction, dt, ...) */
 	void setSolverParams(const b3hSolverParams& prm) { b3hip::ok(b3h_pipeline_set_solver_params(m_p, &prm)); }
 	b3hSolverParams getSolverParams() const

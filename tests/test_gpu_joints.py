@@ -86,7 +86,7 @@ def test_joints_graph_replay_bit_exact():
 
 
 def test_breaking_flags_equal_oracle():
-    sc, w = build(breaking=4.0, fixed_breaking=0.5)
+    sc, w = build(breaking=1.0, fixed_breaking=0.02)
     ow = run_bit_exact(w, 40)
     flags = w.constraints()["flags"]
     assert np.array_equal(flags, ow.joints["flags"])

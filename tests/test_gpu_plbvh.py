@@ -28,10 +28,10 @@ from test_oracle_plbvh import cloud, records
 pytestmark = pytest.mark.gpu
 
 
-def device_plbvh(mn, mx, large_mn=None, large_mx=None):
+def device_plbvh(mn, mx, large_mn=None, large_mx=None, pairs_per_proxy=64):
     n = len(mn)
     nl = 0 if large_mn is None else len(large_mn)
-    cfg = capi.default_config(n + nl + 64, 16)
+    cfg = capi.default_config(n + nl + 64, pairs_per_proxy)
     bp = ctypes.c_void_p()
     capi.call("b3h_bp_create", capi.BROADPHASE_PLBVH, ctypes.byref(cfg), None, ctypes.byref(bp))
     try:
@@ -58,14 +58,19 @@ def device_plbvh(mn, mx, large_mn=None, large_mx=None):
 @pytest.mark.parametrize("n,nl", [(2000, 0), (1500, 3), (40000, 40)])
 def test_plbvh_pairs_equal_oracle(kind, n, nl):
     mn, mx = cloud(kind, n + nl, n * 7 + nl)
-    mn, mx = np.ascontiguousarray(mn), np.ascontiguousarray(mx)
+    # the same density at every size: the cloud's 20 m cube (square for the
+    # coplanar cloud) grows with n
+    k = np.float32((n / 2000.0) ** (1.0 / (2.0 if kind == "coplanar" else 3.0)))
+    kv = np.array([k, 1.0 if kind == "coplanar" else k, k], np.float32)
+    c, h = (mn + mx) * np.float32(0.5), (mx - mn) * np.float32(0.5)
+    mn, mx = np.ascontiguousarray(c * kv - h), np.ascontiguousarray(c * kv + h)
     print("plbvh case", kind, n, nl, flush=True)
     dp, dtot, droot, ovf = device_plbvh(mn[:n], mx[:n], mn[n:], mx[n:])
     a = records(mn, mx)
     op, otot, oroot = po.plbvh_pairs(a, np.arange(n, dtype=np.int32), np.arange(n, n + nl, dtype=np.int32),
                                       max_pairs=64 * n)
     assert ovf == 0
-    assert dtot == otot == len(dp)
+    assert dtot == otot == len(dp) == len(op)
     assert droot == oroot
     assert np.array_equal(dp, op)
 

@@ -44,8 +44,8 @@ def cloud(kind, n, seed):
         c[:, 1] = 3.0
         h[:, 1] = 0.5                         # merged box flat in y
     elif kind == "clustered":
-        # 20 tight clusters of tiny boxes: runs of equal Morton codes
-        k = rng.integers(0, 20, n)
+        # tight clusters of about 100 tiny boxes: runs of equal Morton codes
+        k = rng.integers(0, max(20, n // 100), n)
         c = (c[k] + rng.uniform(-1e-3, 1e-3, size=(n, 3))).astype(np.float32)
         h = np.full((n, 3), 0.01, np.float32)
     return c - h, c + h
