@@ -172,6 +172,7 @@ _SIGS = {
     "b3h_mlp_loss_gradients": ([_vp, _vp, _i, _vp, _i, _i, _vp, _vp], _i),
     "b3h_mlp_train": ([_vp, _vp, _i, _vp, _i, _i, _i, _i, _f, ctypes.c_uint, _vp], _i),
     "b3h_radix_sort32": ([_vp, _vp, _i, _i], _i),
+    "b3h_radix_sort32_noalloc": ([_vp, _vp, _i, _i], _i),
     "b3h_prefix_scan": ([_vp, _vp, _i, _pi], _i),
     "b3h_compute_polyhedral_features": ([_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "b3h_device_count": ([_pi], _i),

@@ -13,6 +13,7 @@
  *           are prefixed through LDS -> stable, deterministic placement.
  * Only as many passes as the key width needs are run.
  */
+#include <mutex>
 #include "b3h_internal.h"
 
 #define RS_THREADS 256
@@ -250,4 +251,73 @@ void b3h_fill_i32(int* p, int v, int n, hipStream_t s)
 {
 	if (n <= 0) return;
 	hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+}
+
+/* ---- allocation-free stable sort ----
+ * The scratch of b3h_radix_sort32 (ping-pong keys and values and the digit
+ * histograms) lives in one file-scope device array instead of hipMalloc'd
+ * buffers, for callers that cannot allocate.  Its device address is fetched
+ * once per device with hipGetSymbolAddress (the array's name is only a host
+ * stand-in) and kept in noalloc_scratch_ptr[device]. */
+#define NOALLOC_SORT_HIST (RS_BUCKETS * (B3H_NOALLOC_SORT_MAX / RS_TILE + 1))
+#define NOALLOC_SORT_WORDS (2 * B3H_NOALLOC_SORT_MAX + NOALLOC_SORT_HIST)
+static __device__ __align__(256) unsigned scratch[NOALLOC_SORT_WORDS];
+
+#define NOALLOC_MAX_DEVICES 64
+static unsigned* noalloc_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex noalloc_mutex;
+
+static int noalloc_scratch(unsigned** out)
+{
+	int dev = -1;
+	B3H_CHECK(hipGetDevice(&dev));
+	if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+	{
+		b3h_set_error("device %d beyond the %d devices the allocation-free sort supports", dev, NOALLOC_MAX_DEVICES);
+		return B3H_ERR_INVALID;
+	}
+	std::lock_guard<std::mutex> lock(noalloc_mutex);
+	if (!noalloc_scratch_ptr[dev])
+	{
+		void* p = nullptr;
+		hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p)
+		{
+			b3h_set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+			return B3H_ERR_HIP;
+		}
+		noalloc_scratch_ptr[dev] = (unsigned*)p;
+	}
+	*out = noalloc_scratch_ptr[dev];
+	return B3H_OK;
+}
+
+extern "C" int b3h_radix_sort32_noalloc(unsigned* keys, unsigned* vals, int n, int bits)
+{
+	if (n < 0 || bits < 0 || bits > 32)
+	{
+		b3h_set_error("invalid argument: n %d, sortBits %d", n, bits);
+		return B3H_ERR_INVALID;
+	}
+	if (n > B3H_NOALLOC_SORT_MAX)
+	{
+		b3h_set_error("b3h_radix_sort32_noalloc: %d entries exceed its scratch of %d entries", n, B3H_NOALLOC_SORT_MAX);
+		return B3H_ERR_CAPACITY;
+	}
+	if (n == 0) return B3H_OK;
+	if (!keys || !vals)
+	{
+		b3h_set_error("invalid argument: keys and values must be device pointers");
+		return B3H_ERR_INVALID;
+	}
+	unsigned* base = nullptr;
+	int r = noalloc_scratch(&base);
+	if (r) return r;
+	/* every scratch word the sort reads is written earlier in the same call
+	 * (histograms by k_radix_hist, ping-pong arrays by k_radix_scatter), so
+	 * the previous call's contents need no clearing */
+	b3h_radix_sort(keys, vals, base, base + B3H_NOALLOC_SORT_MAX, base + 2 * B3H_NOALLOC_SORT_MAX, n, bits, 0);
+	B3H_CHECK(hipGetLastError());
+	B3H_CHECK(hipDeviceSynchronize());
+	return B3H_OK;
 }

@@ -385,6 +385,17 @@ int b3h_pipeline_step_graph(b3hPipeline* p, float deltaTime, int count);
  * src/Bullet3OpenCL/ParallelPrimitives/b3RadixSort32CL.h:84.  Keys and values
  * are taken as separate device arrays of uint32. */
 int b3h_radix_sort32(unsigned* keysDev, unsigned* valuesDev, int n, int sortBits);
+/* Allocation-free variant of b3h_radix_sort32: the same stable sort (same
+ * kernels, same result), with its scratch in a fixed device array inside the
+ * library instead of hipMalloc'd buffers, and no allocating call of any kind.
+ * No counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  n must not exceed B3H_NOALLOC_SORT_MAX (B3H_ERR_CAPACITY
+ * otherwise); each device holds about 8.5 MB of scratch until the process
+ * ends.  Calls on one device share that scratch: calls that overlap on one
+ * device race.  Synchronises the device like b3h_radix_sort32. */
+#define B3H_NOALLOC_SORT_MAX (1 << 20)
+int b3h_radix_sort32_noalloc(unsigned* keysDev, unsigned* valuesDev, int n, int sortBits);
 /* Exclusive prefix sum of n ints (device) into dst (n+1 ints, dst[n] = total);
  * b3PrefixScanCL::execute(src, dst, n, sum), b3PrefixScanCL.h:24. */
 int b3h_prefix_scan(const int* srcDev, int* dstDev, int n, int* sumOut);
