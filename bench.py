@@ -367,8 +367,8 @@ def sub_cubes(args):
 
 def sub_aabbs(args):
     """broadphase stress: 1,048,576 AABBs (half-extents log-uniform 0.05-1 m,
-    about 8 partners each), grid (cell 2 m) and SAP; pairs/s over repeated
-    calculateOverlappingPairs; one-thread oracle grid timed once"""
+    about 8 partners each), grid (cell 2 m), SAP and the linear BVH; pairs/s
+    over repeated calculateOverlappingPairs; one-thread oracle grid timed once"""
     from bullet3_amd import capi, scenes
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -376,7 +376,8 @@ def sub_aabbs(args):
     n = len(mn)
     res = {"workload": "1048576 AABBs, half-extents log-uniform 0.05-1.0 m, cube side %.1f m" % side, "aabbs": n}
     aabbs_host = None
-    for kind, name in ((capi.BROADPHASE_GRID, "grid"), (capi.BROADPHASE_SAP, "sap")):
+    for kind, name in ((capi.BROADPHASE_GRID, "grid"), (capi.BROADPHASE_SAP, "sap"),
+                       (capi.BROADPHASE_PLBVH, "plbvh")):
         cfg = capi.default_config(n + 16, 16)
         bp = ctypes.c_void_p()
         capi.call("b3h_bp_create", kind, ctypes.byref(cfg), None, ctypes.byref(bp))
@@ -409,11 +410,55 @@ def sub_aabbs(args):
     small = np.arange(n, dtype=np.int32)
     large = np.zeros(0, np.int32)
     t0 = time.perf_counter()
-    cp = po.grid_pairs(a, small, large, cell=(2.0, 2.0, 2.0), max_pairs=16 * n)
+    cp, _ = po.grid_pairs(a, small, large, cell=(2.0, 2.0, 2.0), max_pairs=16 * n)
     csec = time.perf_counter() - t0
     res["cpu_port_grid_one_thread"] = {"pairs": len(cp), "ms": round(1000 * csec, 1),
                                        "pairs_per_s": round(len(cp) / csec, 1)}
     return res
+
+
+def sub_rays(w, n_rays=1 << 20, seed=3):
+    """ray casts into the settled pile (castRays on device arrays): n_rays
+    segments from 45 m above the pile straight down to 5 m below the ground
+    top, uniform over the pile's footprint; rays/s over repeated casts"""
+    from bullet3_amd import capi, layout
+    b = w.bodies()
+    dyn = b["invMass"] > 0
+    lo, hi = b["pos"][dyn, :3].min(0), b["pos"][dyn, :3].max(0)
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n_rays, layout.RAYINFO)
+    h = np.zeros(n_rays, layout.RAYHIT)
+    xs = rng.uniform(lo[0], hi[0], n_rays).astype(np.float32)
+    zs = rng.uniform(lo[2], hi[2], n_rays).astype(np.float32)
+    r["from"][:, 0], r["from"][:, 1], r["from"][:, 2] = xs, 45.0, zs
+    r["to"][:, 0], r["to"][:, 1], r["to"][:, 2] = xs, -5.0, zs
+    h["hitBody"], h["hitResult2"] = -1, -1
+    dr, dh, d0 = capi.DeviceBuffer(r.nbytes), capi.DeviceBuffer(h.nbytes), capi.DeviceBuffer(h.nbytes)
+    capi.htod(dr.ptr, r)
+    capi.htod(dh.ptr, h)
+    capi.htod(d0.ptr, h)
+    cast = lambda: capi.call("b3h_pipeline_cast_rays_gpu", w.pl, ctypes.c_void_p(dr.ptr), ctypes.c_void_p(dh.ptr),
+                             n_rays)
+    cast()
+    w.sync()
+    reps = 10
+    sec = 0.0
+    for _ in range(reps):
+        capi.call("b3h_memcpy_dtod", ctypes.c_void_p(dh.ptr), ctypes.c_void_p(d0.ptr), h.nbytes)
+        capi.call("b3h_synchronize")
+        t0 = time.perf_counter()
+        cast()
+        w.sync()
+        sec += time.perf_counter() - t0
+    sec /= reps
+    out = capi.dtoh(dh.ptr, h.nbytes, np.uint8).copy().view(layout.RAYHIT)
+    hits = int((out["hitBody"] >= 0).sum())
+    dr.free()
+    dh.free()
+    d0.free()
+    return {"workload": "%d vertical rays over the settled pile's footprint, 50 m long" % n_rays, "rays": n_rays,
+            "hits": hits, "ms": round(1000 * sec, 4), "rays_per_s": round(n_rays / sec, 1),
+            "note": "host clock around each cast (BVH build and traversal), device synchronised on both sides"}
 
 
 def sub_batched(args):
@@ -470,7 +515,10 @@ def main():
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--ny", type=int, default=32)
     ap.add_argument("--nz", type=int, default=64)
-    ap.add_argument("--lanes", type=int, default=0, help="force narrowphase lanes per pair (64, 32, 16)")
+    ap.add_argument("--lanes", type=int, default=32,
+                    help="narrowphase lanes per pair (64, 32, 16; 0 lets the autotuner pick).  Fixed at 32 by "
+                         "default: the three time within 1%% of each other at the pile, and a fixed choice keeps "
+                         "the SAT kernel's measured traffic (profiles/traffic.json, 32 lanes) applicable")
     ap.add_argument("--scenes", type=int, default=4096, help="scenes of the batched workload")
     ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--cpu-steps", type=int, default=10, help="all-core oracle steps")
@@ -504,6 +552,8 @@ def main():
     if not args.no_configs:
         log("config: cubes")
         out["configs"] = {"cubes_1000": sub_cubes(args)}
+        log("config: rays")
+        out["configs"]["rays_1m"] = sub_rays(w)
         log("config: 1M AABBs")
         out["configs"]["aabbs_1m"] = sub_aabbs(args)
         log("config: batched scenes")

@@ -384,6 +384,64 @@ extern "C" int b3h_np_register_convex_hull_shape(b3hNarrowPhase* np, const float
 							 poly.uniqueEdges.data(), 3, nullptr, nullptr, nullptr, 0.f, nullptr, outCollidable);
 }
 
+/* b3GpuNarrowPhase::registerSphereShape(radius), b3GpuNarrowPhase.cpp:159-192 */
+extern "C" int b3h_np_register_sphere_shape(b3hNarrowPhase* np, float radius, int* outCollidable)
+{
+	CHECK_ARG(np && outCollidable);
+	*outCollidable = -1;
+	if ((int)np->cols.size() >= np->cfg.m_maxConvexShapes)
+	{
+		b3h_set_error("registerSphereShape: exceeding the number of shapes (%d)", np->cfg.m_maxConvexShapes);
+		return B3H_ERR_CAPACITY;
+	}
+	np->gen++;
+	b3hCollidable col;
+	col.m_numChildShapes = 0;
+	col.m_radius = radius;
+	col.m_shapeType = B3H_SHAPE_SPHERE;
+	col.m_shapeIndex = 0;
+	b3hAabb la;
+	la.m_min = f4(-radius, -radius, -radius, 0.f);
+	la.m_max = f4(radius, radius, radius, 0.f);
+	np->localAabbs.push_back(la);
+	*outCollidable = (int)np->cols.size();
+	np->cols.push_back(col);
+	return B3H_OK;
+}
+
+/* b3GpuNarrowPhase::registerPlaneShape(planeNormal, planeConstant) and
+ * registerFace, b3GpuNarrowPhase.cpp:194-232: the plane n.x = constant is
+ * stored as a face (plane (n, constant)); the collidable's radius holds the
+ * constant; its local AABB is +-1e30 */
+extern "C" int b3h_np_register_plane_shape(b3hNarrowPhase* np, const float* normal3, float constant, int* outCollidable)
+{
+	CHECK_ARG(np && normal3 && outCollidable);
+	*outCollidable = -1;
+	if ((int)np->cols.size() >= np->cfg.m_maxConvexShapes)
+	{
+		b3h_set_error("registerPlaneShape: exceeding the number of shapes (%d)", np->cfg.m_maxConvexShapes);
+		return B3H_ERR_CAPACITY;
+	}
+	np->gen++;
+	b3hGpuFace face;
+	memset(&face, 0, sizeof(face));
+	face.m_plane = f4(normal3[0], normal3[1], normal3[2], constant);
+	const int faceIndex = (int)np->faces.size();
+	np->faces.push_back(face);
+	b3hCollidable col;
+	col.m_numChildShapes = 0;
+	col.m_radius = constant;
+	col.m_shapeType = B3H_SHAPE_PLANE;
+	col.m_shapeIndex = faceIndex;
+	b3hAabb la;
+	la.m_min = f4(-1e30f, -1e30f, -1e30f, 0.f);
+	la.m_max = f4(1e30f, 1e30f, 1e30f, 0.f);
+	np->localAabbs.push_back(la);
+	*outCollidable = (int)np->cols.size();
+	np->cols.push_back(col);
+	return B3H_OK;
+}
+
 extern "C" int b3h_np_register_convex_polyhedron(b3hNarrowPhase* np, const b3hConvexUtility* u, int* outCollidable)
 {
 	CHECK_ARG(np && u && outCollidable && u->numVertices > 0 && u->vertices && u->numFaces >= 0 &&
@@ -648,9 +706,11 @@ static b3hShapeTables np_tables(b3hNarrowPhase* np)
 static void np_compute_contacts_dev(b3hNarrowPhase* np, b3hInt4* pairs, const int* dNumPairs, int maxPairs,
 									hipEvent_t evAfterKernel = nullptr, hipEvent_t evAfterMpr = nullptr)
 {
+	bool prim = false;
+	for (const b3hCollidable& c : np->cols) prim = prim || c.m_shapeType != B3H_SHAPE_CONVEX_HULL;
 	b3h_narrowphase(pairs, dNumPairs, maxPairs, np->dBodies, np_tables(np), np->dStaging, np->dFlags, np->dFlagScan,
 					np->dScanTmp, np->dContactPair, np->cfg.m_maxContactCapacity, np->dNumContacts, np->dSepState,
-					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr, evAfterMpr);
+					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr, evAfterMpr, prim);
 	np->contactsValid = false;
 }
 

@@ -100,7 +100,7 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 					 const b3hShapeTables& sh, b3hContact4Data* staging, int* flags, int* flagScan,
 					 unsigned* scanTmp, int* contactPair, int maxContacts, int* numContacts, int* sepState,
 					 b3hFloat4* sepNormal, int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel,
-					 b3hFloat4* mprScratch, hipEvent_t evAfterMpr = nullptr);
+					 b3hFloat4* mprScratch, hipEvent_t evAfterMpr = nullptr, bool hasPrimitives = false);
 void b3h_gather_contacts(const int* contactPair, const int* numContacts, int maxContacts,
 						 const b3hContact4Data* staging, b3hContact4Data* out, hipStream_t s);
 

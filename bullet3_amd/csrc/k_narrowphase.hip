@@ -985,10 +985,16 @@ __global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_narrowphase(const b
 			}
 		}
 		int flag = (mprHit || nReduced > 0) ? 1 : 0;
+		/* a pair with a sphere or a plane is the primitive kernel's */
+		const bool prim = sh.collidables[A.m_collidableIdx].m_shapeType != B3H_SHAPE_CONVEX_HULL ||
+						  sh.collidables[B.m_collidableIdx].m_shapeType != B3H_SHAPE_CONVEX_HULL;
 		if (g.lane == 0)
 		{
 			if (sepStateOut) sepStateOut[i] = state;
 			if (sepOut) sepOut[i] = sep;
+		}
+		if (g.lane == 0 && !prim)
+		{
 			flags[i] = flag;
 			if (flag)
 			{
@@ -1023,6 +1029,345 @@ __global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_narrowphase(const b
 				staging[i] = c;
 			}
 		}
+	}
+}
+
+/* ---------------- primitive pairs ----------------
+ * primitiveContactsKernel (primitiveContacts.cl:793-977): a plane or a sphere
+ * against a hull, a sphere or a plane; one thread per pair.  The reference
+ * appends these contacts with an atomic counter before the hull pairs; here
+ * they take their pair's slot like every contact. */
+__device__ __forceinline__ F4 pc_transform(F4 p, F4 t, F4 q) { return add4(qrot(q, p), t); }
+__device__ __forceinline__ void pc_tr_inverse(F4 t, F4 q, F4* ti, F4* qi)
+{
+	*qi = qinv(q);
+	*ti = qrot(*qi, neg4(t));
+}
+__device__ __forceinline__ void pc_tr_mul(F4 tA, F4 qA, F4 tB, F4 qB, F4* to, F4* qo)
+{
+	*qo = qmul(qA, qB);
+	*to = pc_transform(tB, tA, qA);
+}
+__device__ __forceinline__ void pc_header(b3hContact4Data& c, int pairIndex, int bA, int bB,
+										  const b3hRigidBodyData* bodies)
+{
+	for (int k = 0; k < 4; k++) c.m_worldPosB[k] = f4(0, 0, 0, 0);
+	c.m_restituitionCoeffCmp = (unsigned short)(0.f * 0xffff);
+	c.m_frictionCoeffCmp = (unsigned short)(0.7f * 0xffff);
+	c.m_batchIdx = pairIndex;
+	c.m_bodyAPtrAndSignBit = bodies[bA].m_invMass == 0 ? -bA : bA;
+	c.m_bodyBPtrAndSignBit = bodies[bB].m_invMass == 0 ? -bB : bB;
+	c.m_childIndexA = -1;
+	c.m_childIndexB = -1;
+	c.m_unused1 = 0;
+	c.m_unused2 = 0;
+}
+
+/* extractManifoldSequential (primitiveContacts.cl:489-582): four of n > 4
+ * points (P(i) gives point i); the extreme searches start at FLT_MIN, as the
+ * reference's do */
+template <class PT>
+__device__ int pc_extract_manifold(const PT& P, int n, F4 nearNormal, int* idx)
+{
+	if (n > 64) n = 64;
+	F4 center = f4(0.f, 0.f, 0.f, 0.f);
+	for (int i = 0; i < n; i++) center = add4(center, P(i));
+	const float fn = (float)n;
+	center = f4(center.x / fn, center.y / fn, center.z / fn, center.w / fn);
+	const F4 a = sub4(P(0), center);
+	F4 u = cross3(nearNormal, a);
+	F4 v = cross3(nearNormal, u);
+	u = nrm3(u);
+	v = nrm3(v);
+	float minW = FLT_MAX;
+	int minIndex = -1;
+	float md[4] = {FLT_MIN, FLT_MIN, FLT_MIN, FLT_MIN};
+	const F4 nu = neg4(u), nv = neg4(v);
+	for (int ie = 0; ie < n; ie++)
+	{
+		const F4 pe = P(ie);
+		if (pe.w < minW)
+		{
+			minW = pe.w;
+			minIndex = ie;
+		}
+		const F4 r = sub4(pe, center);
+		const float f0 = dot3(u, r), f1 = dot3(nu, r), f2 = dot3(v, r), f3 = dot3(nv, r);
+		if (f0 < md[0]) md[0] = f0, idx[0] = ie;
+		if (f1 < md[1]) md[1] = f1, idx[1] = ie;
+		if (f2 < md[2]) md[2] = f2, idx[2] = ie;
+		if (f3 < md[3]) md[3] = f3, idx[3] = ie;
+	}
+	if (idx[0] != minIndex && idx[1] != minIndex && idx[2] != minIndex && idx[3] != minIndex) idx[0] = minIndex;
+	return 4;
+}
+
+#define PC_MAX_PLANE_POINTS 64
+#define PC_THREADS 128
+/* computeContactPlaneConvex (:584-727): the hull's vertices below the plane,
+ * reduced to four; contact bodies (plane, hull).  The kept points are held
+ * as vertex numbers (an LDS column of the thread, stride PC_THREADS) and
+ * recomputed where needed, with the same operations. */
+__device__ bool pc_plane_convex(int pairIndex, int bPlane, int bHull, const b3hRigidBodyData* bodies,
+								const b3hShapeTables& sh, int* col, b3hContact4Data& c)
+{
+	const b3hRigidBodyData& A = bodies[bPlane];
+	const b3hRigidBodyData& B = bodies[bHull];
+	const b3hConvexPolyhedronData hullB = sh.convexes[sh.collidables[B.m_collidableIdx].m_shapeIndex];
+	const F4 planeEq = sh.faces[sh.collidables[A.m_collidableIdx].m_shapeIndex].m_plane;
+	const F4 planeNormal = f4(planeEq.x, planeEq.y, planeEq.z, 0.f);
+	const F4 planeNormalWorld = qrot(A.m_quat, planeNormal);
+	const float planeConstant = planeEq.w;
+	F4 invPosA, invOrnA, invPosB, invOrnB, pic, qic;
+	pc_tr_inverse(A.m_pos, A.m_quat, &invPosA, &invOrnA);
+	pc_tr_inverse(B.m_pos, B.m_quat, &invPosB, &invOrnB);
+	pc_tr_mul(invPosB, invOrnB, A.m_pos, A.m_quat, &pic, &qic);
+	const F4 nInConvex = qrot(qic, neg4(planeNormal));
+	const F4* verts = sh.vertices + hullB.m_vertexOffset;
+	/* world vertex with w = its distance below the plane */
+	auto point = [&](int vi, float* distOut) {
+		F4 vw = pc_transform(verts[vi], B.m_pos, B.m_quat);
+		const F4 vp = pc_transform(vw, invPosA, invOrnA);
+		const float dist = dot4(planeNormal, vp) - planeConstant;
+		vw.w = dist;
+		*distOut = dist;
+		return vw;
+	};
+	float maxDot = -1e30f;
+	int numPoints = 0;
+	int idx[4] = {0, 1, 2, 3};
+	for (int i = 0; i < hullB.m_numVertices; i++)
+	{
+		const float curDot = dot4(verts[i], nInConvex);
+		if (curDot > maxDot)
+		{
+			maxDot = curDot;
+			if (numPoints == PC_MAX_PLANE_POINTS) numPoints--;
+		}
+		if (numPoints < PC_MAX_PLANE_POINTS)
+		{
+			float dist;
+			point(i, &dist);
+			if (dist < 0.f) col[PC_THREADS * (numPoints++)] = i;
+		}
+	}
+	auto P = [&](int k) {
+		float d;
+		return point(col[PC_THREADS * k], &d);
+	};
+	int nr = numPoints;
+	if (numPoints > 4) nr = pc_extract_manifold(P, numPoints, nInConvex, idx);
+	if (nr <= 0) return false;
+	pc_header(c, pairIndex, bPlane, bHull, bodies);
+	c.m_worldNormalOnB = neg4(planeNormalWorld);
+	for (int k = 0; k < 4; k++)
+		if (k < nr) c.m_worldPosB[k] = P(idx[k]);
+	c.m_worldNormalOnB.w = (float)nr;
+	return true;
+}
+
+/* computeContactPlaneSphere (:729-791); contact bodies (plane, sphere) */
+__device__ bool pc_plane_sphere(int pairIndex, int bPlane, int bSphere, const b3hRigidBodyData* bodies,
+								const b3hShapeTables& sh, b3hContact4Data& c)
+{
+	const b3hRigidBodyData& A = bodies[bPlane];
+	const b3hRigidBodyData& B = bodies[bSphere];
+	const F4 planeEq = sh.faces[sh.collidables[A.m_collidableIdx].m_shapeIndex].m_plane;
+	const float radius = sh.collidables[B.m_collidableIdx].m_radius;
+	const F4 n1 = f4(planeEq.x, planeEq.y, planeEq.z, 0.f);
+	const float planeConstant = planeEq.w;
+	F4 invPosA, invOrnA, invPosB, invOrnB, cpPos, cpOrn, pcPos, pcOrn;
+	pc_tr_inverse(A.m_pos, A.m_quat, &invPosA, &invOrnA);
+	pc_tr_mul(invPosA, invOrnA, B.m_pos, B.m_quat, &cpPos, &cpOrn);
+	pc_tr_inverse(B.m_pos, B.m_quat, &invPosB, &invOrnB);
+	pc_tr_mul(invPosB, invOrnB, A.m_pos, A.m_quat, &pcPos, &pcOrn);
+	const F4 vtx1 = mul4(qrot(pcOrn, neg4(n1)), radius);
+	const F4 vip = pc_transform(vtx1, cpPos, cpOrn);
+	const float distance = dot3(n1, vip) - planeConstant;
+	if (!(distance < 0.f)) return false;
+	const F4 proj = sub4(vip, smul4(distance, n1));
+	const F4 world = pc_transform(proj, A.m_pos, A.m_quat);
+	const F4 nB = qrot(A.m_quat, n1);
+	F4 p = add4(world, mul4(nB, distance));
+	p.w = distance;
+	pc_header(c, pairIndex, bPlane, bSphere, bodies);
+	c.m_worldNormalOnB = neg4(nB);
+	c.m_worldPosB[0] = p;
+	c.m_worldNormalOnB.w = 1.f;
+	return true;
+}
+
+/* IsPointInPolygon (:280-337): true if p lies inside the face's edges;
+ * otherwise *out is the closest point of the first edge it lies outside (the
+ * reference leaves out.w unset on the edge's interior; 0 here) */
+__device__ bool pc_point_in_polygon(F4 p, const b3hGpuFace& face, const F4* base, const int* indices, F4* out)
+{
+	const F4 plane = f4(face.m_plane.x, face.m_plane.y, face.m_plane.z, 0.f);
+	if (face.m_numIndices < 2) return false;
+	F4 b = base[indices[face.m_indexOffset + face.m_numIndices - 1]];
+	for (int i = 0; i < face.m_numIndices; i++)
+	{
+		const F4 a = b;
+		b = base[indices[face.m_indexOffset + i]];
+		const F4 ab = sub4(b, a), ap = sub4(p, a);
+		const F4 v = cross3(ab, plane);
+		if (dot4(ap, v) > 0.f)
+		{
+			const float abm2 = dot4(ab, ab);
+			const float rt = abm2 != 0.f ? dot4(ab, ap) / abm2 : 0.f;
+			if (rt <= 0.f)
+				*out = a;
+			else if (rt >= 1.f)
+				*out = b;
+			else
+			{
+				const float s = 1.f - rt;
+				*out = f4(s * a.x + rt * b.x, s * a.y + rt * b.y, s * a.z + rt * b.z, 0.f);
+			}
+			return false;
+		}
+	}
+	return true;
+}
+
+/* computeContactSphereConvex (:339-487); contact bodies (sphere, hull) */
+__device__ bool pc_sphere_convex(int pairIndex, int bSphere, int bHull, const b3hRigidBodyData* bodies,
+								 const b3hShapeTables& sh, b3hContact4Data& c)
+{
+	const b3hRigidBodyData& S = bodies[bSphere];
+	const b3hRigidBodyData& H = bodies[bHull];
+	const float radius = sh.collidables[S.m_collidableIdx].m_radius;
+	const F4 pos = H.m_pos, quat = H.m_quat;
+	F4 invPos, invOrn;
+	pc_tr_inverse(pos, quat, &invPos, &invOrn);
+	const F4 sp = pc_transform(S.m_pos, invPos, invOrn);
+	const b3hConvexPolyhedronData hull = sh.convexes[sh.collidables[H.m_collidableIdx].m_shapeIndex];
+	F4 closest = f4(0.f, 0.f, 0.f, 0.f), hitN = f4(0.f, 0.f, 0.f, 0.f);
+	float minDist = -1000000.f;
+	bool collide = true;
+	for (int f = 0; f < hull.m_numFaces; f++)
+	{
+		const b3hGpuFace face = sh.faces[hull.m_faceOffset + f];
+		const F4 planeEqn = face.m_plane;
+		const F4 n = f4(planeEqn.x, planeEqn.y, planeEqn.z, 0.f);
+		const float dist0 = dot3(n, sp) + planeEqn.w;
+		const F4 pnt = sub4(sp, smul4(dist0, n));
+		float dist = dist0;
+		if (dist > radius)
+		{
+			collide = false;
+			break;
+		}
+		if (dist > 0.f)
+		{
+			F4 out = f4(0.f, 0.f, 0.f, 0.f);
+			if (pc_point_in_polygon(sp, face, sh.vertices + hull.m_vertexOffset, sh.indices, &out))
+			{
+				if (dist > minDist)
+				{
+					minDist = dist;
+					closest = pnt;
+					hitN = planeEqn;
+				}
+			}
+			else
+			{
+				const F4 tmp = sub4(sp, out);
+				const float l2 = dot4(tmp, tmp);
+				if (l2 < radius * radius)
+				{
+					dist = sqrtf(l2);
+					if (dist > minDist)
+					{
+						minDist = dist;
+						closest = out;
+						hitN = f4(tmp.x / dist, tmp.y / dist, tmp.z / dist, tmp.w / dist);
+					}
+				}
+				else
+				{
+					collide = false;
+					break;
+				}
+			}
+		}
+		else if (dist > minDist)
+		{
+			minDist = dist;
+			closest = pnt;
+			hitN = f4(planeEqn.x, planeEqn.y, planeEqn.z, hitN.w);
+		}
+	}
+	if (!(collide && minDist > -10000.f)) return false;
+	const F4 nB = qrot(quat, neg4(hitN));
+	F4 p = pc_transform(closest, pos, quat);
+	const float depth = minDist - radius;
+	if (!(depth <= 0.f)) return false;
+	p.w = depth;
+	pc_header(c, pairIndex, bSphere, bHull, bodies);
+	c.m_worldNormalOnB = neg4(nB);
+	c.m_worldPosB[0] = p;
+	c.m_worldNormalOnB.w = 1.f;
+	return true;
+}
+
+/* sphere-sphere (primitiveContactsKernel :905-951); contact bodies (A, B) */
+__device__ bool pc_sphere_sphere(int pairIndex, int a, int b, const b3hRigidBodyData* bodies, const b3hShapeTables& sh,
+								 b3hContact4Data& c)
+{
+	const float rA = sh.collidables[bodies[a].m_collidableIdx].m_radius;
+	const float rB = sh.collidables[bodies[b].m_collidableIdx].m_radius;
+	const F4 posA = bodies[a].m_pos, posB = bodies[b].m_pos;
+	const F4 diff = sub4(posA, posB);
+	const float len = sqrtf(dot4(diff, diff));
+	if (!(len <= rA + rB)) return false;
+	const float dist = len - (rA + rB);
+	F4 n = f4(1.f, 0.f, 0.f, 0.f);
+	if ((double)len > 0.00001) n = f4(diff.x / len, diff.y / len, diff.z / len, diff.w / len);
+	F4 p = add4(posB, mul4(n, rB));
+	p.w = dist;
+	pc_header(c, pairIndex, a, b, bodies);
+	c.m_worldNormalOnB = n;
+	c.m_worldPosB[0] = p;
+	c.m_worldNormalOnB.w = 1.f;
+	return true;
+}
+
+/* every pair that is not hull x hull: its contact (staging) and flag; the
+ * hull x hull pairs are the SAT kernel's */
+__global__ __launch_bounds__(PC_THREADS) void k_np_primitive(const b3hInt4* __restrict__ pairs,
+															 const int* __restrict__ numPairsPtr,
+															 const b3hRigidBodyData* __restrict__ bodies,
+															 b3hShapeTables sh, b3hContact4Data* __restrict__ staging,
+															 int* __restrict__ flags)
+{
+	__shared__ int sIdx[PC_MAX_PLANE_POINTS * PC_THREADS];
+	int* col = sIdx + threadIdx.x;
+	const int numPairs = *numPairsPtr;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < numPairs; i += gridDim.x * blockDim.x)
+	{
+		const b3hInt4 pr = pairs[i];
+		const int ta = sh.collidables[bodies[pr.x].m_collidableIdx].m_shapeType;
+		const int tb = sh.collidables[bodies[pr.y].m_collidableIdx].m_shapeType;
+		if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_CONVEX_HULL) continue;
+		b3hContact4Data c;
+		bool hit = false;
+		if (ta == B3H_SHAPE_PLANE && tb == B3H_SHAPE_CONVEX_HULL)
+			hit = pc_plane_convex(i, pr.x, pr.y, bodies, sh, col, c);
+		else if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_PLANE)
+			hit = pc_plane_convex(i, pr.y, pr.x, bodies, sh, col, c);
+		else if (ta == B3H_SHAPE_PLANE && tb == B3H_SHAPE_SPHERE)
+			hit = pc_plane_sphere(i, pr.x, pr.y, bodies, sh, c);
+		else if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_PLANE)
+			hit = pc_plane_sphere(i, pr.y, pr.x, bodies, sh, c);
+		else if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_CONVEX_HULL)
+			hit = pc_sphere_convex(i, pr.x, pr.y, bodies, sh, c);
+		else if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_SPHERE)
+			hit = pc_sphere_convex(i, pr.y, pr.x, bodies, sh, c);
+		else if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_SPHERE)
+			hit = pc_sphere_sphere(i, pr.x, pr.y, bodies, sh, c);
+		flags[i] = hit ? 1 : 0;
+		if (hit) staging[i] = c;
 	}
 }
 
@@ -1066,7 +1411,7 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 					 const b3hShapeTables& sh, b3hContact4Data* staging, int* flags, int* flagScan, unsigned* scanTmp,
 					 int* contactPair, int maxContacts, int* numContacts, int* sepState, b3hFloat4* sepNormal,
 					 int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel, b3hFloat4* mprScratch,
-					 hipEvent_t evAfterMpr)
+					 hipEvent_t evAfterMpr, bool hasPrimitives)
 {
 	{
 		int mb = (maxPairs + 255) / 256;
@@ -1091,6 +1436,14 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 		launch(k_narrowphase<32, 256>, 32, 256);
 	else
 		launch(k_narrowphase<64, NP_THREADS>, 64, NP_THREADS);
+	if (hasPrimitives)
+	{
+		int pb = (maxPairs + PC_THREADS - 1) / PC_THREADS;
+		if (pb > 16384) pb = 16384;
+		if (pb < 1) pb = 1;
+		hipLaunchKernelGGL(k_np_primitive, dim3(pb), dim3(PC_THREADS), 0, s, pairs, numPairs, bodies, sh, staging,
+						   flags);
+	}
 	if (evAfterKernel) hipEventRecord(evAfterKernel, s);
 	b3h_exclusive_scan(flags, flagScan, numPairs, maxPairs, scanTmp, s);
 	int cblocks = (maxPairs + 255) / 256;

@@ -44,6 +44,17 @@ class World:
         call("b3h_pipeline_create", self.np_, self.bp, ctypes.byref(self.cfg), None, ctypes.byref(self.pl))
         self.collidables = []
         for k, pts in enumerate(scene.shapes):
+            if isinstance(pts, dict):
+                # primitive shapes: {"sphere": radius} or {"plane": (nx, ny, nz, constant)}
+                out = ctypes.c_int(-1)
+                if "sphere" in pts:
+                    call("b3h_np_register_sphere_shape", self.np_, ctypes.c_float(pts["sphere"]), ctypes.byref(out))
+                else:
+                    normal = np.ascontiguousarray(pts["plane"][:3], np.float32)
+                    call("b3h_np_register_plane_shape", self.np_, ptr(normal), ctypes.c_float(pts["plane"][3]),
+                         ctypes.byref(out))
+                self.collidables.append(out.value)
+                continue
             if polyhedra is not None:
                 self.collidables.append(capi.register_convex_polyhedron(self.np_, polyhedra[k]))
                 continue

@@ -69,6 +69,14 @@ int b3h_np_register_convex_hull_shape(b3hNarrowPhase* np, const float* vertices,
  * whose vertices, faces and unique edges the caller computed (the reference
  * demo's route, GpuConvexScene.cpp:231,243), tables taken as given. */
 int b3h_np_register_convex_polyhedron(b3hNarrowPhase* np, const b3hConvexUtility* convex, int* outCollidableIndex);
+/* b3GpuNarrowPhase::registerSphereShape(radius) / registerPlaneShape(normal,
+ * constant), b3GpuNarrowPhase.h:29-30, b3GpuNarrowPhase.cpp:159-232: a sphere
+ * collidable (radius in the collidable), a plane n.x = constant (stored as a
+ * face).  Pairs with a sphere or a plane get the reference's primitive
+ * contacts (primitiveContactsKernel, primitiveContacts.cl:793-977: plane-hull,
+ * plane-sphere, sphere-hull, sphere-sphere). */
+int b3h_np_register_sphere_shape(b3hNarrowPhase* np, float radius, int* outCollidableIndex);
+int b3h_np_register_plane_shape(b3hNarrowPhase* np, const float* normal3, float planeConstant, int* outCollidableIndex);
 /* b3GpuNarrowPhase::registerRigidBody(collidableIndex, mass, position, orientation,
  * aabbMin, aabbMax, writeToGpu), b3GpuNarrowPhase.cpp:816-908 */
 int b3h_np_register_rigid_body(b3hNarrowPhase* np, int collidableIndex, float mass, const float* position3,

@@ -62,6 +62,17 @@ public:
 				   : -1;
 	}
 	/* registerRigidBody(...), b3GpuNarrowPhase.h:42 */
+	/* registerSphereShape / registerPlaneShape, b3GpuNarrowPhase.h:29-30 */
+	int registerSphereShape(float radius)
+	{
+		int idx = -1;
+		return b3hip::ok(b3h_np_register_sphere_shape(m_np, radius, &idx)) ? idx : -1;
+	}
+	int registerPlaneShape(const float* planeNormal, float planeConstant)
+	{
+		int idx = -1;
+		return b3hip::ok(b3h_np_register_plane_shape(m_np, planeNormal, planeConstant, &idx)) ? idx : -1;
+	}
 	int registerRigidBody(int collidableIndex, float mass, const float* position, const float* orientation,
 						  const float* aabbMin, const float* aabbMax, bool writeToGpu)
 	{

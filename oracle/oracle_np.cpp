@@ -679,6 +679,292 @@ extern "C" int b3o_mpr_contacts(b3hInt4* pairs, int nPairs, const b3hRigidBodyDa
 	return nc;
 }
 
+/* ---------------- primitive pairs (primitiveContactsKernel,
+ * primitiveContacts.cl:793-977) ---------------- */
+static F4 p_transform(F4 p, F4 t, F4 q) { return add4(qrot(q, p), t); }                 /* :230-234 */
+static void p_tr_inverse(F4 t, F4 q, F4* to, F4* qo) { *qo = qinv(q); *to = qrot(*qo, neg4(t)); } /* :236-241 */
+static void p_tr_mul(F4 tA, F4 qA, F4 tB, F4 qB, F4* to, F4* qo)                       /* :243-251 */
+{
+	*qo = qmul(qA, qB);
+	*to = p_transform(tB, tA, qA);
+}
+static void p_header(b3hContact4Data* c, const OShapes& S, int pairIndex, int bA, int bB)
+{
+	b3hInt4 pr = {bA, bB, 0, 0};
+	memset(c, 0, sizeof(*c));
+	write_contact_header(c, S, pr, pairIndex, 0.7f);
+}
+/* extractManifoldSequential, :489-582 */
+static int p_extract_manifold(const F4* p, int n, F4 nearNormal, int* idx)
+{
+	if (n > 64) n = 64;
+	F4 center = f4(0, 0, 0, 0);
+	for (int i = 0; i < n; i++) center = add4(center, p[i]);
+	float fn = (float)n;
+	center = f4(center.x / fn, center.y / fn, center.z / fn, center.w / fn);
+	F4 aVector = sub4(p[0], center);
+	F4 u = cross3(nearNormal, aVector);
+	F4 v = cross3(nearNormal, u);
+	u = nrm3(u);
+	v = nrm3(v);
+	float minW = FLT_MAX;
+	int minIndex = -1;
+	float maxDots[4] = {FLT_MIN, FLT_MIN, FLT_MIN, FLT_MIN};
+	for (int ie = 0; ie < n; ie++)
+	{
+		if (p[ie].w < minW)
+		{
+			minW = p[ie].w;
+			minIndex = ie;
+		}
+		F4 r = sub4(p[ie], center);
+		float f = dot3(u, r);
+		if (f < maxDots[0]) { maxDots[0] = f; idx[0] = ie; }
+		f = dot3(neg4(u), r);
+		if (f < maxDots[1]) { maxDots[1] = f; idx[1] = ie; }
+		f = dot3(v, r);
+		if (f < maxDots[2]) { maxDots[2] = f; idx[2] = ie; }
+		f = dot3(neg4(v), r);
+		if (f < maxDots[3]) { maxDots[3] = f; idx[3] = ie; }
+	}
+	if (idx[0] != minIndex && idx[1] != minIndex && idx[2] != minIndex && idx[3] != minIndex) idx[0] = minIndex;
+	return 4;
+}
+/* computeContactPlaneConvex, :584-727 (contact bodies: plane, hull) */
+static bool p_plane_convex(const OShapes& S, int pairIndex, int bA, int bB, b3hContact4Data* c)
+{
+	const b3hRigidBodyData& A = S.bodies[bA];
+	const b3hRigidBodyData& B = S.bodies[bB];
+	const b3hConvexPolyhedronData& hullB = S.hullOf(bB);
+	F4 planeEq = S.faces[S.cols[A.m_collidableIdx].m_shapeIndex].m_plane;
+	F4 planeNormal = f4(planeEq.x, planeEq.y, planeEq.z, 0.f);
+	F4 planeNormalWorld = qrot(A.m_quat, planeNormal);
+	float planeConstant = planeEq.w;
+	F4 invPosA, invOrnA, invPosB, invOrnB, pic, qic;
+	p_tr_inverse(A.m_pos, A.m_quat, &invPosA, &invOrnA);
+	p_tr_inverse(B.m_pos, B.m_quat, &invPosB, &invOrnB);
+	p_tr_mul(invPosB, invOrnB, A.m_pos, A.m_quat, &pic, &qic);
+	F4 nInConvex = qrot(qic, neg4(planeNormal));
+	float maxDot = -1e30f;
+	F4 pts[64];
+	int numPoints = 0;
+	int idx[4] = {0, 1, 2, 3};
+	for (int i = 0; i < hullB.m_numVertices; i++)
+	{
+		F4 vtx = S.verts[hullB.m_vertexOffset + i];
+		float curDot = dot4(vtx, nInConvex);
+		if (curDot > maxDot)
+		{
+			maxDot = curDot;
+			if (numPoints == 64) numPoints--;
+		}
+		if (numPoints < 64)
+		{
+			F4 vtxWorld = p_transform(vtx, B.m_pos, B.m_quat);
+			F4 vtxInPlane = p_transform(vtxWorld, invPosA, invOrnA);
+			float dist = dot4(planeNormal, vtxInPlane) - planeConstant;
+			if (dist < 0.f)
+			{
+				vtxWorld.w = dist;
+				pts[numPoints++] = vtxWorld;
+			}
+		}
+	}
+	int nr = numPoints;
+	if (numPoints > 4) nr = p_extract_manifold(pts, numPoints, nInConvex, idx);
+	if (nr <= 0) return false;
+	p_header(c, S, pairIndex, bA, bB);
+	c->m_worldNormalOnB = neg4(planeNormalWorld);
+	switch (nr)
+	{
+		case 4: c->m_worldPosB[3] = pts[idx[3]]; /* fall through */
+		case 3: c->m_worldPosB[2] = pts[idx[2]]; /* fall through */
+		case 2: c->m_worldPosB[1] = pts[idx[1]]; /* fall through */
+		case 1: c->m_worldPosB[0] = pts[idx[0]];
+	}
+	c->m_worldNormalOnB.w = (float)nr;
+	return true;
+}
+/* computeContactPlaneSphere, :729-791 (contact bodies: plane, sphere) */
+static bool p_plane_sphere(const OShapes& S, int pairIndex, int bA, int bB, b3hContact4Data* c)
+{
+	const b3hRigidBodyData& A = S.bodies[bA];
+	const b3hRigidBodyData& B = S.bodies[bB];
+	F4 planeEq = S.faces[S.cols[A.m_collidableIdx].m_shapeIndex].m_plane;
+	float radius = S.cols[B.m_collidableIdx].m_radius;
+	F4 n1 = f4(planeEq.x, planeEq.y, planeEq.z, 0.f);
+	float planeConstant = planeEq.w;
+	F4 invPosA, invOrnA, cpPos, cpOrn, invPosB, invOrnB, pcPos, pcOrn;
+	p_tr_inverse(A.m_pos, A.m_quat, &invPosA, &invOrnA);
+	p_tr_mul(invPosA, invOrnA, B.m_pos, B.m_quat, &cpPos, &cpOrn);
+	p_tr_inverse(B.m_pos, B.m_quat, &invPosB, &invOrnB);
+	p_tr_mul(invPosB, invOrnB, A.m_pos, A.m_quat, &pcPos, &pcOrn);
+	F4 vtx1 = mul4(qrot(pcOrn, neg4(n1)), radius);
+	F4 vtxInPlane1 = p_transform(vtx1, cpPos, cpOrn);
+	float distance = dot3(n1, vtxInPlane1) - planeConstant;
+	if (!(distance < 0.f)) return false;
+	F4 projected = sub4(vtxInPlane1, smul4(distance, n1));
+	F4 world = p_transform(projected, A.m_pos, A.m_quat);
+	F4 normalOnSurfaceB1 = qrot(A.m_quat, n1);
+	F4 pOnB1 = add4(world, mul4(normalOnSurfaceB1, distance));
+	pOnB1.w = distance;
+	p_header(c, S, pairIndex, bA, bB);
+	c->m_worldNormalOnB = neg4(normalOnSurfaceB1);
+	c->m_worldPosB[0] = pOnB1;
+	c->m_worldNormalOnB.w = 1.f;
+	return true;
+}
+/* IsPointInPolygon, :280-337 (out.w on an edge's interior: 0; the reference leaves it unset) */
+static bool p_point_in_polygon(F4 p, const b3hGpuFace& face, const F4* base, const int* indices, F4* out)
+{
+	F4 plane = f4(face.m_plane.x, face.m_plane.y, face.m_plane.z, 0.f);
+	if (face.m_numIndices < 2) return false;
+	F4 b = base[indices[face.m_indexOffset + face.m_numIndices - 1]];
+	for (int i = 0; i < face.m_numIndices; i++)
+	{
+		F4 a = b;
+		b = base[indices[face.m_indexOffset + i]];
+		F4 ab = sub4(b, a), ap = sub4(p, a);
+		F4 v = cross3(ab, plane);
+		if (dot4(ap, v) > 0.f)
+		{
+			float ab_m2 = dot4(ab, ab);
+			float rt = ab_m2 != 0.f ? dot4(ab, ap) / ab_m2 : 0.f;
+			if (rt <= 0.f)
+				*out = a;
+			else if (rt >= 1.f)
+				*out = b;
+			else
+			{
+				float sv = 1.f - rt;
+				*out = f4(sv * a.x + rt * b.x, sv * a.y + rt * b.y, sv * a.z + rt * b.z, 0.f);
+			}
+			return false;
+		}
+	}
+	return true;
+}
+/* computeContactSphereConvex, :339-487 (contact bodies: sphere, hull) */
+static bool p_sphere_convex(const OShapes& S, int pairIndex, int bA, int bB, b3hContact4Data* c)
+{
+	const b3hRigidBodyData& Sp = S.bodies[bA];
+	const b3hRigidBodyData& H = S.bodies[bB];
+	float radius = S.cols[Sp.m_collidableIdx].m_radius;
+	F4 pos = H.m_pos, quat = H.m_quat;
+	F4 invPos, invOrn;
+	p_tr_inverse(pos, quat, &invPos, &invOrn);
+	F4 spherePos = p_transform(Sp.m_pos, invPos, invOrn);
+	const b3hConvexPolyhedronData& hull = S.hullOf(bB);
+	F4 closestPnt = f4(0, 0, 0, 0), hitNormalWorld = f4(0, 0, 0, 0);
+	float minDist = -1000000.f;
+	bool bCollide = true;
+	for (int f = 0; f < hull.m_numFaces; f++)
+	{
+		const b3hGpuFace& face = S.faces[hull.m_faceOffset + f];
+		F4 planeEqn = face.m_plane;
+		F4 n = f4(planeEqn.x, planeEqn.y, planeEqn.z, 0.f);
+		/* signedDistanceFromPointToPlane, :270-278 */
+		float dist = dot3(n, spherePos) + planeEqn.w;
+		F4 pntReturn = sub4(spherePos, smul4(dist, n));
+		if (dist > radius)
+		{
+			bCollide = false;
+			break;
+		}
+		if (dist > 0)
+		{
+			F4 out = f4(0, 0, 0, 0);
+			if (p_point_in_polygon(spherePos, face, S.verts + hull.m_vertexOffset, S.indices, &out))
+			{
+				if (dist > minDist)
+				{
+					minDist = dist;
+					closestPnt = pntReturn;
+					hitNormalWorld = planeEqn;
+				}
+			}
+			else
+			{
+				F4 tmp = sub4(spherePos, out);
+				float l2 = dot4(tmp, tmp);
+				if (l2 < radius * radius)
+				{
+					dist = sqrtf(l2);
+					if (dist > minDist)
+					{
+						minDist = dist;
+						closestPnt = out;
+						hitNormalWorld = f4(tmp.x / dist, tmp.y / dist, tmp.z / dist, tmp.w / dist);
+					}
+				}
+				else
+				{
+					bCollide = false;
+					break;
+				}
+			}
+		}
+		else
+		{
+			if (dist > minDist)
+			{
+				minDist = dist;
+				closestPnt = pntReturn;
+				hitNormalWorld.x = planeEqn.x;
+				hitNormalWorld.y = planeEqn.y;
+				hitNormalWorld.z = planeEqn.z;
+			}
+		}
+	}
+	if (!(bCollide && minDist > -10000)) return false;
+	F4 normalOnSurfaceB1 = qrot(quat, neg4(hitNormalWorld));
+	F4 pOnB1 = p_transform(closestPnt, pos, quat);
+	float actualDepth = minDist - radius;
+	if (!(actualDepth <= 0.f)) return false;
+	pOnB1.w = actualDepth;
+	p_header(c, S, pairIndex, bA, bB);
+	c->m_worldNormalOnB = neg4(normalOnSurfaceB1);
+	c->m_worldPosB[0] = pOnB1;
+	c->m_worldNormalOnB.w = 1.f;
+	return true;
+}
+/* sphere-sphere, primitiveContactsKernel :905-951 */
+static bool p_sphere_sphere(const OShapes& S, int pairIndex, int bA, int bB, b3hContact4Data* c)
+{
+	float radiusA = S.cols[S.bodies[bA].m_collidableIdx].m_radius;
+	float radiusB = S.cols[S.bodies[bB].m_collidableIdx].m_radius;
+	F4 posA = S.bodies[bA].m_pos, posB = S.bodies[bB].m_pos;
+	F4 diff = sub4(posA, posB);
+	float len = sqrtf(dot4(diff, diff));
+	if (!(len <= radiusA + radiusB)) return false;
+	float dist = len - (radiusA + radiusB);
+	F4 normalOnSurfaceB = f4(1.f, 0.f, 0.f, 0.f);
+	if ((double)len > 0.00001) normalOnSurfaceB = f4(diff.x / len, diff.y / len, diff.z / len, diff.w / len);
+	F4 contactPosB = add4(posB, mul4(normalOnSurfaceB, radiusB));
+	contactPosB.w = dist;
+	p_header(c, S, pairIndex, bA, bB);
+	c->m_worldNormalOnB = normalOnSurfaceB;
+	c->m_worldPosB[0] = contactPosB;
+	c->m_worldNormalOnB.w = 1.f;
+	return true;
+}
+/* the primitive kernel's dispatch by shape types; false for hull x hull */
+static bool p_primitive_pair(const OShapes& S, const b3hInt4& pr, int i, b3hContact4Data* c, bool* handled)
+{
+	int ta = S.cols[S.bodies[pr.x].m_collidableIdx].m_shapeType;
+	int tb = S.cols[S.bodies[pr.y].m_collidableIdx].m_shapeType;
+	*handled = !(ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_CONVEX_HULL);
+	if (!*handled) return false;
+	if (ta == B3H_SHAPE_PLANE && tb == B3H_SHAPE_CONVEX_HULL) return p_plane_convex(S, i, pr.x, pr.y, c);
+	if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_PLANE) return p_plane_convex(S, i, pr.y, pr.x, c);
+	if (ta == B3H_SHAPE_PLANE && tb == B3H_SHAPE_SPHERE) return p_plane_sphere(S, i, pr.x, pr.y, c);
+	if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_PLANE) return p_plane_sphere(S, i, pr.y, pr.x, c);
+	if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_CONVEX_HULL) return p_sphere_convex(S, i, pr.x, pr.y, c);
+	if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_SPHERE) return p_sphere_convex(S, i, pr.y, pr.x, c);
+	if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_SPHERE) return p_sphere_sphere(S, i, pr.x, pr.y, c);
+	return false;
+}
+
 /* One pair of the convex-convex narrowphase: MPR, SAT over faces and edges,
  * clipping and reduction.  Writes the pair's manifold into *c (slot-free: the
  * header names the pair) and returns whether the pair makes a contact;
@@ -689,6 +975,16 @@ static bool narrow_pair(const OShapes& S, const b3hInt4& pr, int i, b3hContact4D
 	int a = pr.x, b = pr.y;
 	int state = 0;
 	F4 sep = f4(0, 0, 0, 0);
+	{
+		bool handled;
+		bool hit = p_primitive_pair(S, pr, i, c, &handled);
+		if (handled)
+		{
+			*stateOut = 0;
+			*sepOut = sep;
+			return hit;
+		}
+	}
 	bool hull = pair_is_hull_hull(S, a, b);
 	/* MPR */
 	bool mprHit = false;

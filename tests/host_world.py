@@ -28,6 +28,24 @@ def quat_basis(q):
 def shape_tables(shapes):
     cols, convs, verts, edges, faces, idx, laabbs = [], [], [], [], [], [], []
     for s, pts in enumerate(shapes):
+        if isinstance(pts, dict):
+            # registerSphereShape / registerPlaneShape (b3GpuNarrowPhase.cpp:159-232)
+            col = np.zeros(1, layout.COLLIDABLE)[0]
+            la = np.zeros(1, layout.AABB)[0]
+            if "sphere" in pts:
+                r = np.float32(pts["sphere"])
+                col["shapeType"], col["shapeIndex"], col["radius"] = 7, 0, r
+                la["min"][:3], la["max"][:3] = -r, r
+            else:
+                fr = np.zeros(1, layout.FACE)[0]
+                fr["plane"] = np.asarray(pts["plane"], np.float32)
+                col["shapeType"], col["shapeIndex"], col["radius"] = 4, len(faces), np.float32(pts["plane"][3])
+                faces.append(fr)
+                la["min"][:3], la["max"][:3] = -1e30, 1e30
+            cols.append(col)
+            convs.append(np.zeros(1, layout.CONVEX)[0])
+            laabbs.append(la)
+            continue
         f = capi.polyhedral_features(np.asarray(pts, np.float32))
         v = f["vertices"].astype(np.float32)
         cv = np.zeros(1, layout.CONVEX)[0]
