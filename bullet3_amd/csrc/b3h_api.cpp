@@ -14,6 +14,7 @@
  */
 #include "b3h_internal.h"
 #include "hull.h"
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -155,6 +156,23 @@ struct b3hNarrowPhase
 	int* dSepState;
 	b3hFloat4* dSepNormal;
 	b3hFloat4* dMpr; /* per pair: MPR direction, point, (depth, hit) */
+	/* per-pair buffers between the SAT passes */
+	float* dSatDmin;
+	int *dSatFlag, *dSatScan, *dSatList, *dSatCount;
+	/* per-body face records for the SAT (b3hSatCache): record ranges follow
+	 * the host mirror's collidable indices and are rebuilt when those change */
+	b3hSatFace* dSatRecs;
+	int *dSatBase, *dSatOwner;
+	size_t satCapRecs, satCapBase;
+	int satBodies, satTotal;
+	float2* dSatDirs;
+	int *dSatDslot, *dSatDowner;
+	size_t satCapDirBodies;
+	int satDirBodies;
+	unsigned satGen;
+	bool satValid;
+	std::vector<int> satCols; /* collidable index per body when the ranges were built */
+	size_t satNumConvex;
 	bool contactsValid;
 	int lanesPerPair;
 	/* bumped by every call that can change what a captured step refers to
@@ -179,6 +197,9 @@ static int np_alloc_work(b3hNarrowPhase* np)
 	if ((r = dalloc(&np->dSepState, mp))) return r;
 	if ((r = dalloc(&np->dSepNormal, mp))) return r;
 	if ((r = dalloc(&np->dMpr, 3 * (size_t)mp))) return r;
+	if ((r = dalloc(&np->dSatDmin, mp)) || (r = dalloc(&np->dSatFlag, mp + 1)) || (r = dalloc(&np->dSatScan, mp + 2)) ||
+		(r = dalloc(&np->dSatList, mp)) || (r = dalloc(&np->dSatCount, 1)))
+		return r;
 	B3H_CHECK(hipMemset(np->dNumContacts, 0, 2 * sizeof(int)));
 	return B3H_OK;
 }
@@ -210,6 +231,19 @@ extern "C" int b3h_np_create(const b3hConfig* cfg, void* stream, b3hNarrowPhase*
 	np->dSepState = nullptr;
 	np->dSepNormal = nullptr;
 	np->dMpr = nullptr;
+	np->dSatDmin = nullptr;
+	np->dSatFlag = np->dSatScan = np->dSatList = np->dSatCount = nullptr;
+	np->dSatRecs = nullptr;
+	np->dSatBase = np->dSatOwner = nullptr;
+	np->satCapRecs = np->satCapBase = 0;
+	np->satBodies = np->satTotal = 0;
+	np->dSatDirs = nullptr;
+	np->dSatDslot = np->dSatDowner = nullptr;
+	np->satCapDirBodies = 0;
+	np->satDirBodies = 0;
+	np->satGen = 0;
+	np->satValid = false;
+	np->satNumConvex = 0;
 	np->contactsValid = true;
 	np->lanesPerPair = 64;
 	if ((r = make_stream(stream, &np->stream, &np->ownStream)) || (r = np_alloc_work(np)) || (r = b3h_np_module_init()))
@@ -245,6 +279,17 @@ extern "C" int b3h_np_destroy(b3hNarrowPhase* np)
 	dfree(np->dSepState);
 	dfree(np->dSepNormal);
 	dfree(np->dMpr);
+	dfree(np->dSatDmin);
+	dfree(np->dSatFlag);
+	dfree(np->dSatScan);
+	dfree(np->dSatList);
+	dfree(np->dSatCount);
+	dfree(np->dSatRecs);
+	dfree(np->dSatBase);
+	dfree(np->dSatOwner);
+	dfree(np->dSatDirs);
+	dfree(np->dSatDslot);
+	dfree(np->dSatDowner);
 	if (np->ownStream) hipStreamDestroy(np->stream);
 	delete np;
 	return B3H_OK;
@@ -702,15 +747,125 @@ static b3hShapeTables np_tables(b3hNarrowPhase* np)
 	return t;
 }
 
+/* Record ranges of the per-body face records (b3hSatCache).  Called outside
+ * graph capture before every narrowphase entry; the ranges are rebuilt only
+ * when a body's collidable or the shape tables changed. */
+static int np_ensure_sat(b3hNarrowPhase* np)
+{
+	if (np->satValid && np->satGen == np->gen) return B3H_OK;
+	int nb = (int)np->bodies.size();
+	bool same = np->satValid && (int)np->satCols.size() == nb && np->satNumConvex == np->convexes.size();
+	for (int b = 0; same && b < nb; b++) same = np->satCols[(size_t)b] == np->bodies[(size_t)b].m_collidableIdx;
+	if (same)
+	{
+		np->satGen = np->gen;
+		return B3H_OK;
+	}
+	np->satValid = false;
+	std::vector<int> base((size_t)nb + 1);
+	long long total = 0;
+	for (int b = 0; b < nb; b++)
+	{
+		base[(size_t)b] = (int)total;
+		int ci = np->bodies[(size_t)b].m_collidableIdx;
+		if (ci < 0 || ci >= (int)np->cols.size() || np->cols[(size_t)ci].m_shapeType != B3H_SHAPE_CONVEX_HULL) continue;
+		int si = np->cols[(size_t)ci].m_shapeIndex;
+		if (si >= 0 && si < (int)np->convexes.size()) total += np->convexes[(size_t)si].m_numFaces;
+		if (total > (1ll << 30)) break;
+	}
+	np->satBodies = 0;
+	np->satTotal = 0;
+	if (nb == 0 || total == 0 || total > (1ll << 30)) /* nothing to cache, or too large: SAT projects directly */
+	{
+		np->satGen = np->gen;
+		np->satValid = true;
+		np->satCols.clear();
+		return B3H_OK;
+	}
+	base[(size_t)nb] = (int)total;
+	std::vector<int> owner((size_t)total);
+	for (int b = 0; b < nb; b++)
+		for (int r = base[(size_t)b]; r < base[(size_t)b + 1]; r++) owner[(size_t)r] = b;
+	/* fixed-direction intervals for every body that can meet a pair whose
+	 * edge-pair count exceeds the 162 directions */
+	auto hullOf = [&](int b) -> const b3hConvexPolyhedronData* {
+		int ci = np->bodies[(size_t)b].m_collidableIdx;
+		if (ci < 0 || ci >= (int)np->cols.size() || np->cols[(size_t)ci].m_shapeType != B3H_SHAPE_CONVEX_HULL)
+			return nullptr;
+		int si = np->cols[(size_t)ci].m_shapeIndex;
+		return (si >= 0 && si < (int)np->convexes.size()) ? &np->convexes[(size_t)si] : nullptr;
+	};
+	long long maxE = 0;
+	for (int b = 0; b < nb; b++)
+		if (const b3hConvexPolyhedronData* h = hullOf(b)) maxE = std::max<long long>(maxE, h->m_numUniqueEdges);
+	std::vector<int> dslot((size_t)nb, -1), downer;
+	for (int b = 0; b < nb; b++)
+	{
+		const b3hConvexPolyhedronData* h = hullOf(b);
+		if (h && (long long)h->m_numUniqueEdges * maxE > B3H_SAT_DIRS)
+		{
+			dslot[(size_t)b] = (int)downer.size();
+			downer.push_back(b);
+		}
+	}
+	B3H_CHECK(hipStreamSynchronize(np->stream)); /* earlier steps may still read the old ranges */
+	int r;
+	if (np->satCapBase < (size_t)nb + 1)
+	{
+		dfree(np->dSatBase);
+		dfree(np->dSatDslot);
+		np->satCapBase = 0;
+		if ((r = dalloc(&np->dSatBase, (size_t)nb + 1)) || (r = dalloc(&np->dSatDslot, (size_t)nb + 1))) return r;
+		np->satCapBase = (size_t)nb + 1;
+	}
+	if (np->satCapDirBodies < downer.size())
+	{
+		dfree(np->dSatDirs);
+		dfree(np->dSatDowner);
+		np->satCapDirBodies = 0;
+		if ((r = dalloc(&np->dSatDirs, downer.size() * B3H_SAT_DIRS)) ||
+			(r = dalloc(&np->dSatDowner, downer.size())))
+			return r;
+		np->satCapDirBodies = downer.size();
+	}
+	B3H_CHECK(hipMemcpy(np->dSatDslot, dslot.data(), sizeof(int) * (size_t)nb, hipMemcpyHostToDevice));
+	if (!downer.empty())
+		B3H_CHECK(hipMemcpy(np->dSatDowner, downer.data(), sizeof(int) * downer.size(), hipMemcpyHostToDevice));
+	np->satDirBodies = (int)downer.size();
+	if (np->satCapRecs < (size_t)total)
+	{
+		dfree(np->dSatRecs);
+		dfree(np->dSatOwner);
+		np->satCapRecs = 0;
+		if ((r = dalloc(&np->dSatRecs, (size_t)total)) || (r = dalloc(&np->dSatOwner, (size_t)total))) return r;
+		np->satCapRecs = (size_t)total;
+	}
+	B3H_CHECK(hipMemcpy(np->dSatBase, base.data(), sizeof(int) * ((size_t)nb + 1), hipMemcpyHostToDevice));
+	B3H_CHECK(hipMemcpy(np->dSatOwner, owner.data(), sizeof(int) * (size_t)total, hipMemcpyHostToDevice));
+	np->satCols.resize((size_t)nb);
+	for (int b = 0; b < nb; b++) np->satCols[(size_t)b] = np->bodies[(size_t)b].m_collidableIdx;
+	np->satNumConvex = np->convexes.size();
+	np->satBodies = nb;
+	np->satTotal = (int)total;
+	np->satGen = np->gen;
+	np->satValid = true;
+	return B3H_OK;
+}
+
 /* device-count form used inside a step */
 static void np_compute_contacts_dev(b3hNarrowPhase* np, b3hInt4* pairs, const int* dNumPairs, int maxPairs,
 									hipEvent_t evAfterKernel = nullptr, hipEvent_t evAfterMpr = nullptr)
 {
 	bool prim = false;
 	for (const b3hCollidable& c : np->cols) prim = prim || c.m_shapeType != B3H_SHAPE_CONVEX_HULL;
+	b3hSatCache sc = {nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, 0};
+	if (np->satValid && np->satGen == np->gen && np->satBodies > 0)
+		sc = {np->dSatRecs, np->dSatBase, np->dSatOwner, np->satBodies, np->satTotal,
+			  np->satDirBodies > 0 ? np->dSatDirs : nullptr, np->dSatDslot, np->dSatDowner, np->satDirBodies};
 	b3h_narrowphase(pairs, dNumPairs, maxPairs, np->dBodies, np_tables(np), np->dStaging, np->dFlags, np->dFlagScan,
 					np->dScanTmp, np->dContactPair, np->cfg.m_maxContactCapacity, np->dNumContacts, np->dSepState,
-					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr, evAfterMpr, prim);
+					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr, evAfterMpr, prim, &sc,
+					b3hSatWork{np->dSatDmin, np->dSatFlag, np->dSatScan, np->dSatList, np->dSatCount});
 	np->contactsValid = false;
 }
 
@@ -720,6 +875,7 @@ extern "C" int b3h_np_compute_contacts(b3hNarrowPhase* np, b3hInt4* pairs, int n
 	(void)aabbs;
 	(void)numObjects;
 	if (numPairs > 0) CHECK_ARG(pairs);
+	if (int re = np_ensure_sat(np)) return re;
 	B3H_CHECK(hipMemcpyAsync(np->dNumPairs, &numPairs, sizeof(int), hipMemcpyHostToDevice, np->stream));
 	np_compute_contacts_dev(np, pairs, np->dNumPairs, numPairs > 0 ? numPairs : 1);
 	B3H_CHECK(hipGetLastError());
@@ -1681,6 +1837,7 @@ extern "C" int b3h_pipeline_step_simulation(b3hPipeline* p, float dt)
 	}
 	int r = joints_upload(p);
 	if (r) return r;
+	if ((r = np_ensure_sat(p->np))) return r;
 	r = pl_step(p, dt, p->timing);
 	if (r) return r;
 	B3H_CHECK(hipGetLastError());
@@ -1703,6 +1860,7 @@ extern "C" int b3h_pipeline_step_graph(b3hPipeline* p, float dt, int count)
 		/* scratch sizes and one-time module setup must exist before capture */
 		int r = bp_ensure_scratch(p->bp);
 		if (r) return r;
+		if ((r = np_ensure_sat(p->np))) return r;
 		B3H_CHECK(hipStreamSynchronize(p->stream));
 		B3H_CHECK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
 		r = pl_step(p, dt, false);
@@ -2425,7 +2583,7 @@ extern "C" int b3h_pipeline_autotune(b3hPipeline* p, float dt, int steps, int* l
 	for (int li = 0; li < 3 && !r; li++)
 		for (int pi = 0; pi < 3 && !r; pi++)
 		{
-			if ((r = b3h_pipeline_load_state(p, image.data(), n))) break;
+			if ((r = b3h_pipeline_load_state(p, image.data(), n)) || (r = np_ensure_sat(p->np))) break;
 			p->np->lanesPerPair = lanes[li];
 			p->sb.pgsMaxParts = parts[pi];
 			hipEventRecord(e0, p->stream);

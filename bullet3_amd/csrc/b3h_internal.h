@@ -86,6 +86,9 @@ void b3h_plbvh_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const 
 void b3h_scene_contact_counts(const int* nPairs, int maxPairs, const b3hInt4* pairs, const int* sceneOf, int* counts,
 							  int numScenes, hipStream_t s);
 
+/* compute units of the current device (cached per device) */
+int b3h_cu_count();
+
 /* ---------------- narrowphase (k_narrowphase.hip) ---------------- */
 struct b3hShapeTables
 {
@@ -96,11 +99,52 @@ struct b3hShapeTables
 	const b3hGpuFace* faces;
 	const int* indices;
 };
+/* Per-body face records, rebuilt every step before the SAT kernel: for face f
+ * of body b's hull, the world face normal w = qrot(orn_b, plane_f) and the
+ * interval [mn, mx] of b's own hull projected on w (the projection the SAT
+ * would otherwise repeat for every pair b is in).  Records of body b sit at
+ * [base[b], base[b+1]); owner[r] is the body of record r.  A body whose range
+ * does not match its hull's face count (device bodies changed behind the host
+ * mirror) is treated as uncached. */
+struct b3hSatFace
+{
+	b3hFloat4 w;
+	float mn, mx, pad0, pad1;
+};
+#define B3H_SAT_DIRS 162 /* fixed directions of the edge-edge approximation */
+struct b3hSatCache
+{
+	b3hSatFace* recs;
+	const int* base;  /* numBodies + 1 */
+	const int* owner; /* total */
+	int numBodies;    /* bodies covered; 0 disables the cache */
+	int total;
+	/* intervals of a body's hull on the 162 fixed directions (the edge-edge
+	 * approximation): body b's 162 (mn, mx) at dirs[162 dslot[b]]; bodies
+	 * that cannot meet a pair taking that branch have dslot -1 */
+	float2* dirs;
+	const int* dslot;  /* numBodies */
+	const int* downer; /* numDirBodies */
+	int numDirBodies;
+};
+/* per-pair buffers between the SAT passes (maxPairs entries each; flag and
+ * scan one and two more) */
+struct b3hSatWork
+{
+	float* dmin; /* the face families' minimum overlap of an open pair */
+	int* flag;   /* pass 1: pair left open; pass 2: pair to clip */
+	int* scan;   /* exclusive prefix sum of flag */
+	int* list;   /* the flagged pairs in pair order */
+	int* count;  /* their number */
+};
+/* sepState and sepNormal (per pair) are required: they carry the SAT state
+ * between the passes */
 void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3hRigidBodyData* bodies,
 					 const b3hShapeTables& sh, b3hContact4Data* staging, int* flags, int* flagScan,
 					 unsigned* scanTmp, int* contactPair, int maxContacts, int* numContacts, int* sepState,
 					 b3hFloat4* sepNormal, int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel,
-					 b3hFloat4* mprScratch, hipEvent_t evAfterMpr = nullptr, bool hasPrimitives = false);
+					 b3hFloat4* mprScratch, hipEvent_t evAfterMpr, bool hasPrimitives, const b3hSatCache* satCache,
+					 const b3hSatWork& work);
 void b3h_gather_contacts(const int* contactPair, const int* numContacts, int maxContacts,
 						 const b3hContact4Data* staging, b3hContact4Data* out, hipStream_t s);
 

@@ -1,7 +1,9 @@
 /*
- * k_narrowphase.hip -- convex-convex contact generation, one pair per lane
- * group (L lanes of a wavefront; L = 64 by default), fused into one kernel,
- * followed by an order-preserving compaction.
+ * k_narrowphase.hip -- convex-convex contact generation: MPR one pair per
+ * thread, then the SAT and clipping in three passes (face families; edge
+ * family of the pairs still open; clipping of the pairs still touching), each
+ * pair handled by a group of L lanes of a wavefront, followed by an
+ * order-preserving compaction of the contacts.
  *
  * Per pair (reference kernels, all in src/Bullet3OpenCL/NarrowphaseCollision/kernels/):
  *   1. MPR penetration + 1-point contact      mpr.cl:14-90 -> b3MprPenetration.h:816-886
@@ -13,22 +15,26 @@
  *      488-541, extractManifoldSequential :720-810); point 0 stays the MPR point
  *      when MPR found one (clipHullHullKernel :890-1000).
  *
- * Work split inside a group:
- *   - both hulls' vertices are staged in LDS (per-group slice);
+ * Work split:
  *   - MPR runs in its own kernel, one pair per thread (the portal iteration
  *     is serial; support queries use the sequential b3MaxDot);
- *   - every SAT family spreads its axes over the lanes; a lane projects both
- *     hulls on its axis in vertex order (so lo/hi are the reference's values),
- *     the family stops at the first round that finds a separating axis, and the
- *     minimum overlap is an arg-min with first-index tie-break, which equals the
- *     sequential "if (d < dmin)" scan;
+ *   - per step, every body's world face normals with its own hull's interval
+ *     on each (k_sat_faces) and its intervals on the 162 fixed directions
+ *     (k_sat_dirs) are computed once, instead of once per pair the body is in;
+ *   - the SAT passes stage both hulls' vertices in LDS (per-group slice); every
+ *     SAT family spreads its axes over the lanes; a lane projects the hull(s)
+ *     not covered by the per-body intervals on its axis in vertex order (so
+ *     lo/hi are the reference's values), the family stops at the first round
+ *     that finds a separating axis, and the minimum overlap is an arg-min with
+ *     first-index tie-break, which equals the sequential "if (d < dmin)" scan;
  *   - each Sutherland-Hodgman pass gives one polygon edge per lane; lanes emit
  *     0, 1 or 2 points and a ballot prefix places them in sequential order.
  * The reduction orders therefore reproduce the sequential kernels exactly.
  * The reference runs these stages as 7 launches with a host sync after each;
- * here one launch writes a pair-indexed contact record and a flag, and a prefix
- * sum over the flags assigns contact slots in pair order (deterministic; the
- * reference uses an atomic counter).
+ * here the passes hand pair lists to each other through prefix sums on the
+ * device, each pair gets a pair-indexed contact record and a flag, and a
+ * prefix sum over the flags assigns contact slots in pair order
+ * (deterministic; the reference uses an atomic counter).
  */
 #include "b3h_internal.h"
 #include "unit_sphere_162.h"
@@ -41,6 +47,7 @@
 #define NP_THREADS 256
 
 __constant__ float c_unitSphere[162][3];
+static_assert(B3_UNIT_SPHERE_162_COUNT == B3H_SAT_DIRS, "fixed-direction count");
 
 /* LDS ordering between lanes of one wavefront (groups never span waves) */
 __device__ __forceinline__ void wave_sync()
@@ -483,6 +490,12 @@ template <typename VP>
 __device__ __forceinline__ void project_loop(VP vs, int nv, F4 d, float& lo, float& hi)
 {
 	int i = 0;
+	for (; i + 4 <= nv; i += 4)
+	{
+		float pa = vdot3(vs, i, d), pb = vdot3(vs, i + 1, d), pc = vdot3(vs, i + 2, d), pd = vdot3(vs, i + 3, d);
+		lo = fminf(fminf(lo, fminf(pa, pb)), fminf(pc, pd));
+		hi = fmaxf(fmaxf(hi, fmaxf(pa, pb)), fmaxf(pc, pd));
+	}
 	for (; i + 2 <= nv; i += 2)
 	{
 		float pa = vdot3(vs, i, d), pb = vdot3(vs, i + 1, d);
@@ -541,29 +554,25 @@ __device__ __forceinline__ bool test_sep_axis(const PairGeom& P, F4 axis, float*
 }
 __device__ __forceinline__ bool almost_zero(F4 v) { return !(fabsf(v.x) > 1e-6f || fabsf(v.y) > 1e-6f || fabsf(v.z) > 1e-6f); }
 
-/* Candidate axis `t` of a family; returns false when the family has no axis t
- * (skipped edge pairs). kind 0: faces of A (tested with A as "A"), kind 1:
- * faces of B (tested with B as "A", sat.cl:1463-1489), kind 2: edge crosses,
- * kind 3: the 162 fixed directions. */
+/* The SAT families: kind 0: faces of A (tested with A as hull 0), kind 1:
+ * faces of B (tested with B as hull 0, sat.cl:1463-1489), kind 2: edge
+ * crosses, kind 3: the 162 fixed directions. */
 struct Family
 {
 	int kind;
 	int count;
 	const b3hGpuFace* faces; /* kind 0/1 */
-	F4 orn;                  /* kind 0/1 */
 	const F4* edgesA;        /* kind 2 */
 	const F4* edgesB;
 	int nEB;
+	const b3hSatFace* cache; /* kind 0/1: the face owner's records, or null */
+	const float2* dA;        /* kind 3: A's and B's intervals on the fixed directions, or null */
+	const float2* dB;
 };
+/* Edge or fixed-direction axis t; false when the family has no axis t
+ * (parallel edges) */
 __device__ __forceinline__ bool family_axis(const Family& F, const PairGeom& P, F4 dc, int t, F4* axis)
 {
-	if (F.kind <= 1)
-	{
-		F4 nw = qrot(F.orn, F.faces[t].m_plane);
-		if (dot3(dc, nw) < 0) nw = mul4(nw, -1.f);
-		*axis = nw;
-		return true;
-	}
 	if (F.kind == 2)
 	{
 		int e0 = t / F.nEB, e1 = t - e0 * F.nEB;
@@ -582,30 +591,108 @@ __device__ __forceinline__ bool family_axis(const Family& F, const PairGeom& P, 
 	return true;
 }
 /* One SAT family over the group: false if some axis separates; otherwise the
- * sequential min-overlap update of (sep, dmin) and the final orientation flip. */
+ * sequential min-overlap update of (sep, dmin) and the final orientation flip.
+ *
+ * A face axis is the owner's world face normal w = qrot(orn, plane), turned
+ * towards dc, and the owner's own interval on it.  Both come from the owner's
+ * face record (k_sat_faces) when there is one, else they are computed here with
+ * the same operations.  Turning the axis negates every product of the
+ * projection, so the owner's interval on -w is (-mx, -mn) of its interval on w
+ * (the same values up to the sign of a zero end, which no comparison or
+ * difference downstream can see).  Only the other hull is projected per pair. */
 template <int L>
 __device__ bool sat_family(const Group<L>& g, const Family& F, const PairGeom& P, F4 dc, F4* sep, float* dmin)
 {
 	float bv = INFINITY;
 	int bi = INT_MAX;
 	F4 bn = f4(0, 0, 0, 0);
+	const bool faceKind = F.kind <= 1;
+	const bool ownA = F.kind == 0; /* the face owner is hull 0 of the overlap test */
 	for (int t0 = 0; t0 < F.count; t0 += L)
 	{
 		int t = t0 + g.lane;
 		bool separated = false;
 		F4 axis;
-		if (t < F.count && family_axis(F, P, dc, t, &axis))
+		if (faceKind)
+		{
+			if (t < F.count)
+			{
+				b3hSatFace r;
+				if (F.cache)
+					r = F.cache[t];
+				else
+				{
+					r.w = qrot(ownA ? P.ornA : P.ornB, F.faces[t].m_plane);
+					project<L == 64>(ownA ? P.vA : P.vB, ownA ? P.ldsA : P.ldsB, ownA ? P.nA : P.nB,
+									 ownA ? P.posA : P.posB, ownA ? P.ornA : P.ornB, r.w, &r.mn, &r.mx);
+				}
+				float mn0, mx0, mn1, mx1;
+				if (dot3(dc, r.w) < 0)
+				{
+					axis = mul4(r.w, -1.f);
+					mn0 = -r.mx;
+					mx0 = -r.mn;
+				}
+				else
+				{
+					axis = r.w;
+					mn0 = r.mn;
+					mx0 = r.mx;
+				}
+				project<L == 64>(ownA ? P.vB : P.vA, ownA ? P.ldsB : P.ldsA, ownA ? P.nB : P.nA, ownA ? P.posB : P.posA,
+								 ownA ? P.ornB : P.ornA, axis, &mn1, &mx1);
+				if (mx0 < mn1 || mx1 < mn0)
+					separated = true;
+				else
+				{
+					float d0 = mx0 - mn1, d1 = mx1 - mn0;
+					float d = d0 < d1 ? d0 : d1;
+					if (d < bv)
+					{
+						bv = d;
+						bi = t;
+						bn = axis;
+					}
+				}
+			}
+		}
+		else if (F.dA)
+		{
+			/* fixed directions with both hulls' intervals precomputed on the
+			 * unturned direction; the turned axis negates and swaps them */
+			if (t < F.count)
+			{
+				F4 c = f4(c_unitSphere[t][0], c_unitSphere[t][1], c_unitSphere[t][2], 0.f);
+				const float2 ia = F.dA[t], ib = F.dB[t];
+				float mn0 = ia.x, mx0 = ia.y, mn1 = ib.x, mx1 = ib.y;
+				if (dot3(dc, c) > 0)
+				{
+					c = mul4(c, -1.f);
+					mn0 = -ia.y;
+					mx0 = -ia.x;
+					mn1 = -ib.y;
+					mx1 = -ib.x;
+				}
+				axis = c;
+				if (mx0 < mn1 || mx1 < mn0)
+					separated = true;
+				else
+				{
+					float d0 = mx0 - mn1, d1 = mx1 - mn0;
+					float d = d0 < d1 ? d0 : d1;
+					if (d < bv)
+					{
+						bv = d;
+						bi = t;
+						bn = axis;
+					}
+				}
+			}
+		}
+		else if (t < F.count && family_axis(F, P, dc, t, &axis))
 		{
 			float d;
-			bool overlap;
-			if (F.kind == 1)
-			{
-				PairGeom Q = {P.vB, P.vA, P.nB, P.nA, P.posB, P.ornB, P.posA, P.ornA, P.ldsB, P.ldsA};
-				overlap = test_sep_axis<L == 64>(Q, axis, &d);
-			}
-			else
-				overlap = test_sep_axis<L == 64>(P, axis, &d);
-			if (!overlap)
+			if (!test_sep_axis<L == 64>(P, axis, &d))
 				separated = true;
 			else if (d < bv)
 			{
@@ -616,10 +703,8 @@ __device__ bool sat_family(const Group<L>& g, const Family& F, const PairGeom& P
 		}
 		if (g.any(separated)) return false;
 	}
-	int own = bi;
 	g.argmin(bv, bi);
 	F4 n = g.bcast(bn, g.src(bi == INT_MAX ? 0 : bi));
-	(void)own;
 	if (bi != INT_MAX && bv < *dmin)
 	{
 		*dmin = bv;
@@ -676,7 +761,8 @@ __device__ int clip_pass(const Group<L>& g, const F4* in, int nIn, F4 n, float e
  * (xyz + depth) in `out` */
 template <int L>
 __device__ int clip_hull_hull(const Group<L>& g, F4 sep, const PairGeom& P, const b3hConvexPolyhedronData& ha,
-							  const b3hConvexPolyhedronData& hb, const b3hShapeTables& sh, F4* bufA, F4* bufB, F4* out)
+							  const b3hConvexPolyhedronData& hb, const b3hShapeTables& sh, F4* bufA, F4* bufB, F4* out,
+							  const b3hSatFace* cA, const b3hSatFace* cB)
 {
 	const float minDist = -1e30f, maxDist = 0.02f;
 	/* incident face of B: largest dot with the separating normal */
@@ -684,7 +770,7 @@ __device__ int clip_hull_hull(const Group<L>& g, F4 sep, const PairGeom& P, cons
 	int bi = INT_MAX;
 	for (int f = g.lane; f < hb.m_numFaces; f += L)
 	{
-		F4 wn = qrot(P.ornB, xyz0(sh.faces[hb.m_faceOffset + f].m_plane));
+		F4 wn = cB ? cB[f].w : qrot(P.ornB, xyz0(sh.faces[hb.m_faceOffset + f].m_plane));
 		float d = dot3(wn, sep);
 		if (d > -FLT_MAX && (bi == INT_MAX || d > bv))
 		{
@@ -704,7 +790,7 @@ __device__ int clip_hull_hull(const Group<L>& g, F4 sep, const PairGeom& P, cons
 	int ai = INT_MAX;
 	for (int f = g.lane; f < ha.m_numFaces; f += L)
 	{
-		F4 wn = qrot(P.ornA, xyz0(sh.faces[ha.m_faceOffset + f].m_plane));
+		F4 wn = cA ? cA[f].w : qrot(P.ornA, xyz0(sh.faces[ha.m_faceOffset + f].m_plane));
 		float d = dot3(wn, sep);
 		if (d < FLT_MAX && (ai == INT_MAX || d < av))
 		{
@@ -717,18 +803,31 @@ __device__ int clip_hull_hull(const Group<L>& g, F4 sep, const PairGeom& P, cons
 	const b3hGpuFace polyA = sh.faces[ha.m_faceOffset + ai];
 	int nvA = polyA.m_numIndices;
 	const int* ia = sh.indices + polyA.m_indexOffset;
-	F4 wpn = qrot(P.ornA, xyz0(polyA.m_plane));
+	F4 wpn = cA ? cA[ai].w : qrot(P.ornA, xyz0(polyA.m_plane));
 	F4* pin = bufA;
 	F4* pout = bufB;
 	int nIn = nB;
+	/* the side planes of the reference face, L at a time, one per lane; pass
+	 * e0 takes plane e0 from its lane */
+	F4 lanePn = f4(0, 0, 0, 0);
+	float lanePeq = 0.f;
 	for (int e0 = 0; e0 < nvA; e0++)
 	{
-		F4 a = P.vA[ia[e0]];
-		F4 b = P.vA[ia[(e0 + 1) % nvA]];
-		F4 we0 = qrot(P.ornA, sub4(a, b));
-		F4 pn = neg4(cross3(we0, wpn));
-		F4 wa = xform(a, P.posA, P.ornA);
-		float peq = -dot3(wa, pn);
+		if ((e0 & (L - 1)) == 0)
+		{
+			int e = e0 + g.lane;
+			if (e < nvA)
+			{
+				F4 a = P.vA[ia[e]];
+				F4 b = P.vA[ia[(e + 1) % nvA]];
+				F4 we0 = qrot(P.ornA, sub4(a, b));
+				lanePn = neg4(cross3(we0, wpn));
+				F4 wa = xform(a, P.posA, P.ornA);
+				lanePeq = -dot3(wa, lanePn);
+			}
+		}
+		F4 pn = g.bcast(lanePn, g.src(e0));
+		float peq = __shfl(lanePeq, g.src(e0));
 		int nOut = clip_pass(g, pin, nIn, pn, peq, pout);
 		F4* t = pout;
 		pout = pin;
@@ -875,21 +974,194 @@ __global__ __launch_bounds__(256, 4) void k_np_mpr(const b3hInt4* __restrict__ p
 	}
 }
 
-/* ---------------- the fused kernel ---------------- */
+/* ---------------- per-body face records ---------------- */
+/* the records of body b when they cover its current hull, else null */
+__device__ __forceinline__ const b3hSatFace* sat_records(const b3hSatCache& sc, int b, int numFaces)
+{
+	if (b >= sc.numBodies) return nullptr;
+	int s0 = sc.base[b], s1 = sc.base[b + 1];
+	if (s1 - s0 != numFaces || s0 < 0 || s1 > sc.total) return nullptr;
+	return sc.recs + s0;
+}
+/* body b's intervals on the 162 fixed directions, or null */
+__device__ __forceinline__ const float2* sat_dirs(const b3hSatCache& sc, int b)
+{
+	if (b >= sc.numBodies || !sc.dirs) return nullptr;
+	int k = sc.dslot[b];
+	if (k < 0 || k >= sc.numDirBodies) return nullptr;
+	return sc.dirs + (size_t)k * B3_UNIT_SPHERE_162_COUNT;
+}
+/* one (body, direction) per thread: body downer[r / 162]'s interval on the
+ * unturned fixed direction r % 162 */
+__global__ __launch_bounds__(256) void k_sat_dirs(const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh,
+												  b3hSatCache sc)
+{
+	int total = sc.numDirBodies * B3_UNIT_SPHERE_162_COUNT;
+	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < total; r += gridDim.x * blockDim.x)
+	{
+		int k = r / B3_UNIT_SPHERE_162_COUNT, t = r - k * B3_UNIT_SPHERE_162_COUNT;
+		const b3hRigidBodyData body = bodies[sc.downer[k]];
+		const b3hCollidable col = sh.collidables[body.m_collidableIdx];
+		if (col.m_shapeType != B3H_SHAPE_CONVEX_HULL) continue;
+		const b3hConvexPolyhedronData h = sh.convexes[col.m_shapeIndex];
+		F4 pos = body.m_pos;
+		pos.w = 0.f;
+		F4 c = f4(c_unitSphere[t][0], c_unitSphere[t][1], c_unitSphere[t][2], 0.f);
+		float mn, mx;
+		project<false>(sh.vertices + h.m_vertexOffset, false, h.m_numVertices, pos, body.m_quat, c, &mn, &mx);
+		sc.dirs[r] = make_float2(mn, mx);
+	}
+}
+/* one record per thread: body owner[r], face r - base[owner[r]] of its hull */
+__global__ __launch_bounds__(256) void k_sat_faces(const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh,
+												   b3hSatCache sc)
+{
+	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < sc.total; r += gridDim.x * blockDim.x)
+	{
+		int b = sc.owner[r];
+		const b3hRigidBodyData body = bodies[b];
+		const b3hCollidable col = sh.collidables[body.m_collidableIdx];
+		if (col.m_shapeType != B3H_SHAPE_CONVEX_HULL) continue;
+		const b3hConvexPolyhedronData h = sh.convexes[col.m_shapeIndex];
+		int f = r - sc.base[b];
+		if (f >= h.m_numFaces) continue;
+		F4 pos = body.m_pos;
+		pos.w = 0.f;
+		b3hSatFace rec;
+		rec.w = qrot(body.m_quat, sh.faces[h.m_faceOffset + f].m_plane);
+		project<false>(sh.vertices + h.m_vertexOffset, false, h.m_numVertices, pos, body.m_quat, rec.w, &rec.mn, &rec.mx);
+		rec.pad0 = 0.f;
+		rec.pad1 = 0.f;
+		sc.recs[r] = rec;
+	}
+}
+
+/* ---------------- the SAT passes ----------------
+ * Three kernels over the pair list, each finishing the pairs it decides:
+ *   faces:  all pairs; the face families of A and B.  A pair separated there,
+ *           or not a hull pair, is finished (its contact is the MPR point
+ *           alone, if MPR hit); the others keep (state, sep, dmin) for
+ *   edges:  one pair per wavefront, pairs still open; the edge-pair or fixed-
+ *           direction family.  Separated pairs are finished, the rest are
+ *           flagged for clipping, and a prefix sum lists them in pair order for
+ *   clip:   the listed pairs, several per wavefront; clipping and reduction.
+ * Every pass stages both hulls' vertices in LDS.  Splitting the stages keeps
+ * each wavefront on one kind of work: pairs that need clipping (about a fifth)
+ * or an edge family (about a tenth) no longer hold the lanes of pairs that are
+ * done after the face families. */
+struct PairHulls
+{
+	b3hRigidBodyData A, B;
+	b3hConvexPolyhedronData ha, hb;
+};
+/* vertices of both hulls into LDS (when they fit) and the pair geometry */
+template <int L>
+__device__ __forceinline__ void stage_pair(const Group<L>& g, const b3hShapeTables& sh, const PairHulls& H, F4* sA, F4* sB,
+										   PairGeom& P)
+{
+	const F4* gA = sh.vertices + H.ha.m_vertexOffset;
+	const F4* gB = sh.vertices + H.hb.m_vertexOffset;
+	bool ldsA = H.ha.m_numVertices <= NP_VCAP, ldsB = H.hb.m_numVertices <= NP_VCAP;
+	wave_sync(); /* the previous pair's LDS reads are done */
+	if (ldsA)
+		for (int k = g.lane; k < H.ha.m_numVertices; k += L) sA[k] = gA[k];
+	if (ldsB)
+		for (int k = g.lane; k < H.hb.m_numVertices; k += L) sB[k] = gB[k];
+	wave_sync();
+	P.vA = ldsA ? sA : gA;
+	P.vB = ldsB ? sB : gB;
+	P.ldsA = ldsA;
+	P.ldsB = ldsB;
+	P.nA = H.ha.m_numVertices;
+	P.nB = H.hb.m_numVertices;
+	P.posA = H.A.m_pos;
+	P.posB = H.B.m_pos;
+	P.ornA = H.A.m_quat;
+	P.ornB = H.B.m_quat;
+}
+__device__ __forceinline__ void load_hulls(const b3hShapeTables& sh, const b3hRigidBodyData* bodies, int a, int b,
+										   PairHulls& H)
+{
+	H.A = bodies[a];
+	H.B = bodies[b];
+	H.ha = sh.convexes[sh.collidables[H.A.m_collidableIdx].m_shapeIndex];
+	H.hb = sh.convexes[sh.collidables[H.B.m_collidableIdx].m_shapeIndex];
+}
+/* the SAT frame: positions with w = 0 and the centre difference dc */
+__device__ __forceinline__ F4 sat_frame(const PairHulls& H, const PairGeom& P, PairGeom& Q)
+{
+	Q = P;
+	Q.posA.w = 0.f;
+	Q.posB.w = 0.f;
+	F4 c0 = xform(H.ha.m_localCenter, Q.posA, Q.ornA);
+	F4 c1 = xform(H.hb.m_localCenter, Q.posB, Q.ornB);
+	return sub4(c0, c1);
+}
+/* the pair's final record (lane 0 of the group): SAT state and axis, and for
+ * a hull pair its flag and contact (the MPR point, then the reduced clip
+ * points); pairs with a sphere or a plane get their contact from k_np_primitive */
+__device__ void sat_finish(int i, int a, int b, const b3hRigidBodyData& A, const b3hRigidBodyData& B, bool prim, int state,
+						   F4 sep, const F4* __restrict__ mprIn, int nReduced, const int* idx, const F4* pts,
+						   int* __restrict__ flags, b3hContact4Data* __restrict__ staging, int* __restrict__ sepStateOut,
+						   b3hFloat4* __restrict__ sepOut)
+{
+	sepStateOut[i] = state;
+	sepOut[i] = sep;
+	if (prim) return;
+	F4 mprDH = mprIn[3 * (size_t)i + 2];
+	bool mprHit = mprDH.y != 0.f;
+	int flag = (mprHit || nReduced > 0) ? 1 : 0;
+	flags[i] = flag;
+	if (!flag) return;
+	b3hContact4Data c;
+	for (int k = 0; k < 4; k++) c.m_worldPosB[k] = f4(0, 0, 0, 0);
+	c.m_restituitionCoeffCmp = (unsigned short)(0.f * 0xffff);
+	c.m_frictionCoeffCmp = (unsigned short)(0.7f * 0xffff);
+	c.m_batchIdx = i;
+	c.m_bodyAPtrAndSignBit = A.m_invMass == 0 ? -a : a;
+	c.m_bodyBPtrAndSignBit = B.m_invMass == 0 ? -b : b;
+	c.m_childIndexA = -1;
+	c.m_childIndexB = -1;
+	c.m_unused1 = 0;
+	c.m_unused2 = 0;
+	c.m_worldNormalOnB = f4(0, 0, 0, 0);
+	if (mprHit)
+	{
+		F4 mprDir = mprIn[3 * (size_t)i], mprPos = mprIn[3 * (size_t)i + 1];
+		c.m_worldNormalOnB = neg4(mprDir);
+		F4 p = mprPos;
+		p.w = -mprDH.x;
+		c.m_worldPosB[0] = p;
+		c.m_worldNormalOnB.w = 1.f;
+	}
+	if (nReduced > 0)
+	{
+		c.m_worldNormalOnB = sep;
+#pragma unroll
+		for (int k = 0; k < 4; k++)
+			if (k < nReduced && (k > 0 || !mprHit)) c.m_worldPosB[k] = pts[idx[k]];
+		c.m_worldNormalOnB.w = (float)nReduced;
+	}
+	staging[i] = c;
+}
+__device__ __forceinline__ bool pair_is_prim(const b3hRigidBodyData& A, const b3hRigidBodyData& B, const b3hShapeTables& sh)
+{
+	return sh.collidables[A.m_collidableIdx].m_shapeType != B3H_SHAPE_CONVEX_HULL ||
+		   sh.collidables[B.m_collidableIdx].m_shapeType != B3H_SHAPE_CONVEX_HULL;
+}
+
+/* pass 1: the face families (findSeparatingAxisKernel's faces of A then B).
+ * launch bounds: 1024 / THREADS blocks per CU keep the register count at 128,
+ * so four waves share each SIMD */
 template <int L, int THREADS>
-/* launch bounds: 1024 / THREADS blocks per CU keep the register count at 128, so
- * four waves share each SIMD (a few values spill to scratch; measured faster
- * than three waves without spills) */
-__global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_narrowphase(const b3hInt4* __restrict__ pairs,
-															 const int* __restrict__ numPairsPtr,
-															 const b3hRigidBodyData* __restrict__ bodies,
-															 b3hShapeTables sh, b3hContact4Data* __restrict__ staging,
-															 int* __restrict__ flags, int* __restrict__ sepStateOut,
-															 b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn)
+__global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_sat_face_pass(
+	const b3hInt4* __restrict__ pairs, const int* __restrict__ numPairsPtr, const b3hRigidBodyData* __restrict__ bodies,
+	b3hShapeTables sh, b3hContact4Data* __restrict__ staging, int* __restrict__ flags, int* __restrict__ sepStateOut,
+	b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn, b3hSatCache sc, float* __restrict__ dminOut,
+	int* __restrict__ openFlag)
 {
 	constexpr int G = THREADS / L;
 	__shared__ F4 sVerts[G][2][NP_VCAP];
-	__shared__ F4 sClip[G][3][CLIP_CAP];
 	const Group<L> g;
 	const int gi = threadIdx.x / L;
 	int numPairs = *numPairsPtr;
@@ -897,138 +1169,172 @@ __global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_narrowphase(const b
 	{
 		b3hInt4 pr = pairs[i];
 		int a = pr.x, b = pr.y;
-		const b3hRigidBodyData A = bodies[a];
-		const b3hRigidBodyData B = bodies[b];
-		bool hull = pair_is_hull_hull(A, B, sh);
-		int state = 0;
-		F4 sep = f4(0, 0, 0, 0);
-		F4 mprDir = mprIn[3 * (size_t)i], mprPos = mprIn[3 * (size_t)i + 1];
+		PairHulls H;
+		H.A = bodies[a];
+		H.B = bodies[b];
 		F4 mprDH = mprIn[3 * (size_t)i + 2];
-		float mprDepth = mprDH.x;
-		bool mprHit = mprDH.y != 0.f;
-		int nReduced = 0;
-		int idx[4] = {0, 0, 0, 0};
-		F4* pts = sClip[gi][2];
-		if (hull)
+		F4 sep = f4(0, 0, 0, 0);
+		if (mprDH.y != 0.f) sep = neg4(mprIn[3 * (size_t)i]);
+		if (!pair_is_hull_hull(H.A, H.B, sh))
 		{
-			const b3hConvexPolyhedronData ha = sh.convexes[sh.collidables[A.m_collidableIdx].m_shapeIndex];
-			const b3hConvexPolyhedronData hb = sh.convexes[sh.collidables[B.m_collidableIdx].m_shapeIndex];
-			const F4* gA = sh.vertices + ha.m_vertexOffset;
-			const F4* gB = sh.vertices + hb.m_vertexOffset;
-			bool ldsA = ha.m_numVertices <= NP_VCAP, ldsB = hb.m_numVertices <= NP_VCAP;
-			wave_sync(); /* the previous pair's LDS reads are done */
-			if (ldsA)
-				for (int k = g.lane; k < ha.m_numVertices; k += L) sVerts[gi][0][k] = gA[k];
-			if (ldsB)
-				for (int k = g.lane; k < hb.m_numVertices; k += L) sVerts[gi][1][k] = gB[k];
-			wave_sync();
-			PairGeom P;
-			P.vA = ldsA ? sVerts[gi][0] : gA;
-			P.vB = ldsB ? sVerts[gi][1] : gB;
-			P.ldsA = ldsA;
-			P.ldsB = ldsB;
-			P.nA = ha.m_numVertices;
-			P.nB = hb.m_numVertices;
-			P.posA = A.m_pos;
-			P.posB = B.m_pos;
-			P.ornA = A.m_quat;
-			P.ornB = B.m_quat;
-			if (mprHit) sep = neg4(mprDir);
-			/* SAT (findSeparatingAxis* kernels) */
-			PairGeom Q = P;
-			Q.posA.w = 0.f;
-			Q.posB.w = 0.f;
-			F4 c0 = xform(ha.m_localCenter, Q.posA, Q.ornA);
-			F4 c1 = xform(hb.m_localCenter, Q.posB, Q.ornB);
-			F4 dc = sub4(c0, c1);
-			float dmin = FLT_MAX;
-			F4 sepN = sep;
-			Family fa = {0, ha.m_numFaces, sh.faces + ha.m_faceOffset, Q.ornA, nullptr, nullptr, 1};
-			bool sepA = sat_family(g, fa, Q, dc, &sepN, &dmin);
-			float dminSaved = FLT_MAX;
-			if (sepA)
+			if (g.lane == 0)
 			{
-				Family fb = {1, hb.m_numFaces, sh.faces + hb.m_faceOffset, Q.ornB, nullptr, nullptr, 1};
-				bool sepB = sat_family(g, fb, Q, dc, &sepN, &dmin);
-				if (sepB)
-				{
-					state = 1;
-					dminSaved = dmin;
-					sep = sepN;
-				}
-				else
-					state = 4;
+				openFlag[i] = 0;
+				sat_finish(i, a, b, H.A, H.B, pair_is_prim(H.A, H.B, sh), 0, f4(0, 0, 0, 0), mprIn, 0, nullptr, nullptr,
+						   flags, staging, sepStateOut, sepOut);
 			}
-			if (state != 0)
-			{
-				float dm = dminSaved;
-				F4 s2 = sep;
-				int nEE = ha.m_numUniqueEdges * hb.m_numUniqueEdges;
-				Family fe;
-				if (nEE <= B3_UNIT_SPHERE_162_COUNT)
-					fe = {2, nEE, nullptr, Q.ornA, sh.uniqueEdges + ha.m_uniqueEdgesOffset,
-						  sh.uniqueEdges + hb.m_uniqueEdgesOffset, hb.m_numUniqueEdges};
-				else
-					fe = {3, B3_UNIT_SPHERE_162_COUNT, nullptr, Q.ornA, nullptr, nullptr, 1};
-				if (!sat_family(g, fe, Q, dc, &s2, &dm))
-					state = 0;
-				else
-				{
-					state = 1;
-					sep = s2;
-				}
-			}
-			if (state == 1)
-			{
-				int np = clip_hull_hull(g, sep, P, ha, hb, sh, sClip[gi][0], sClip[gi][1], pts);
-				if (np > 0) nReduced = extract_manifold(g, pts, np, neg4(sep), idx);
-			}
+			continue;
 		}
-		int flag = (mprHit || nReduced > 0) ? 1 : 0;
-		/* a pair with a sphere or a plane is the primitive kernel's */
-		const bool prim = sh.collidables[A.m_collidableIdx].m_shapeType != B3H_SHAPE_CONVEX_HULL ||
-						  sh.collidables[B.m_collidableIdx].m_shapeType != B3H_SHAPE_CONVEX_HULL;
+		H.ha = sh.convexes[sh.collidables[H.A.m_collidableIdx].m_shapeIndex];
+		H.hb = sh.convexes[sh.collidables[H.B.m_collidableIdx].m_shapeIndex];
+		PairGeom P, Q;
+		stage_pair(g, sh, H, sVerts[gi][0], sVerts[gi][1], P);
+		F4 dc = sat_frame(H, P, Q);
+		float dmin = FLT_MAX;
+		F4 sepN = sep;
+		int state = 0;
+		float dminSaved = FLT_MAX;
+		Family fa = {0, H.ha.m_numFaces, sh.faces + H.ha.m_faceOffset, nullptr, nullptr, 1,
+					 sat_records(sc, a, H.ha.m_numFaces), nullptr, nullptr};
+		if (sat_family(g, fa, Q, dc, &sepN, &dmin))
+		{
+			Family fb = {1, H.hb.m_numFaces, sh.faces + H.hb.m_faceOffset, nullptr, nullptr, 1,
+						 sat_records(sc, b, H.hb.m_numFaces), nullptr, nullptr};
+			if (sat_family(g, fb, Q, dc, &sepN, &dmin))
+			{
+				state = 1;
+				dminSaved = dmin;
+				sep = sepN;
+			}
+			else
+				state = 4;
+		}
 		if (g.lane == 0)
 		{
-			if (sepStateOut) sepStateOut[i] = state;
-			if (sepOut) sepOut[i] = sep;
-		}
-		if (g.lane == 0 && !prim)
-		{
-			flags[i] = flag;
-			if (flag)
+			openFlag[i] = state != 0;
+			if (state == 0)
+				sat_finish(i, a, b, H.A, H.B, false, 0, sep, mprIn, 0, nullptr, nullptr, flags, staging, sepStateOut,
+						   sepOut);
+			else
 			{
-				b3hContact4Data c;
-				for (int k = 0; k < 4; k++) c.m_worldPosB[k] = f4(0, 0, 0, 0);
-				c.m_restituitionCoeffCmp = (unsigned short)(0.f * 0xffff);
-				c.m_frictionCoeffCmp = (unsigned short)(0.7f * 0xffff);
-				c.m_batchIdx = i;
-				c.m_bodyAPtrAndSignBit = A.m_invMass == 0 ? -a : a;
-				c.m_bodyBPtrAndSignBit = B.m_invMass == 0 ? -b : b;
-				c.m_childIndexA = -1;
-				c.m_childIndexB = -1;
-				c.m_unused1 = 0;
-				c.m_unused2 = 0;
-				c.m_worldNormalOnB = f4(0, 0, 0, 0);
-				if (mprHit)
-				{
-					c.m_worldNormalOnB = neg4(mprDir);
-					F4 p = mprPos;
-					p.w = -mprDepth;
-					c.m_worldPosB[0] = p;
-					c.m_worldNormalOnB.w = 1.f;
-				}
-				if (nReduced > 0)
-				{
-					c.m_worldNormalOnB = sep;
-#pragma unroll
-					for (int k = 0; k < 4; k++)
-						if (k < nReduced && (k > 0 || !mprHit)) c.m_worldPosB[k] = pts[idx[k]];
-					c.m_worldNormalOnB.w = (float)nReduced;
-				}
-				staging[i] = c;
+				sepStateOut[i] = state;
+				sepOut[i] = sep;
+				dminOut[i] = dminSaved;
 			}
 		}
+	}
+}
+
+/* pass 2: the edge family (exact edge pairs, sat.cl:719-798, or the 162 fixed
+ * directions, mpr.cl:180-311) of the listed pairs (those the face pass left
+ * open), one pair per wavefront; each sets its clip flag (1: clip it) */
+__global__ __launch_bounds__(256, 4) void k_sat_edge_pass(
+	const b3hInt4* __restrict__ pairs, const b3hRigidBodyData* __restrict__ bodies,
+	b3hShapeTables sh, b3hContact4Data* __restrict__ staging, int* __restrict__ flags, int* __restrict__ sepStateOut,
+	b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn, b3hSatCache sc, const float* __restrict__ dminIn,
+	const int* __restrict__ listCount, const int* __restrict__ list, int* __restrict__ clipFlag)
+{
+	constexpr int L = 64, G = 4;
+	__shared__ F4 sVerts[G][2][NP_VCAP];
+	const Group<L> g;
+	const int gi = threadIdx.x / L;
+	int n = *listCount;
+	for (int k = blockIdx.x * G + gi; k < n; k += gridDim.x * G)
+	{
+		int i = list[k];
+		b3hInt4 pr = pairs[i];
+		int a = pr.x, b = pr.y;
+		PairHulls H;
+		load_hulls(sh, bodies, a, b, H);
+		PairGeom P, Q;
+		stage_pair(g, sh, H, sVerts[gi][0], sVerts[gi][1], P);
+		F4 dc = sat_frame(H, P, Q);
+		F4 sep = sepOut[i];
+		float dm = dminIn[i];
+		F4 s2 = sep;
+		int nEE = H.ha.m_numUniqueEdges * H.hb.m_numUniqueEdges;
+		Family fe;
+		if (nEE <= B3_UNIT_SPHERE_162_COUNT)
+			fe = {2, nEE, nullptr, sh.uniqueEdges + H.ha.m_uniqueEdgesOffset, sh.uniqueEdges + H.hb.m_uniqueEdgesOffset,
+				  H.hb.m_numUniqueEdges, nullptr, nullptr, nullptr};
+		else
+		{
+			const float2* dA = sat_dirs(sc, a);
+			const float2* dB = sat_dirs(sc, b);
+			fe = {3, B3_UNIT_SPHERE_162_COUNT, nullptr, nullptr, nullptr, 1, nullptr, dB ? dA : nullptr, dB};
+		}
+		bool open = sat_family(g, fe, Q, dc, &s2, &dm);
+		if (g.lane == 0)
+		{
+			if (!open)
+			{
+				clipFlag[i] = 0;
+				sat_finish(i, a, b, H.A, H.B, false, 0, sep, mprIn, 0, nullptr, nullptr, flags, staging, sepStateOut,
+						   sepOut);
+			}
+			else
+			{
+				clipFlag[i] = 1;
+				sepStateOut[i] = 1;
+				sepOut[i] = s2;
+			}
+		}
+	}
+}
+
+/* the flagged pairs in pair order, and their number */
+__global__ __launch_bounds__(256) void k_pair_list(const int* __restrict__ numPairsPtr, const int* __restrict__ flag,
+												   const int* __restrict__ scan, int* __restrict__ list,
+												   int* __restrict__ count)
+{
+	int n = *numPairsPtr;
+	if (blockIdx.x == 0 && threadIdx.x == 0) *count = scan[n];
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+		if (flag[i]) list[scan[i]] = i;
+}
+
+/* pass 3: clipping and contact reduction of the listed pairs (clipHullHullKernel,
+ * satClipHullContacts.cl:890-1000), L lanes per pair */
+template <int L, int THREADS>
+__global__ __launch_bounds__(THREADS, 3) void k_sat_clip_pass(
+	const b3hInt4* __restrict__ pairs, const int* __restrict__ clipCount, const int* __restrict__ clipList,
+	const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh, b3hContact4Data* __restrict__ staging,
+	int* __restrict__ flags, int* __restrict__ sepStateOut, b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn,
+	b3hSatCache sc)
+{
+	constexpr int G = THREADS / L;
+	__shared__ F4 sClip[G][3][CLIP_CAP];
+	const Group<L> g;
+	const int gi = threadIdx.x / L;
+	int n = *clipCount;
+	for (int k = blockIdx.x * G + gi; k < n; k += gridDim.x * G)
+	{
+		int i = clipList[k];
+		b3hInt4 pr = pairs[i];
+		int a = pr.x, b = pr.y;
+		PairHulls H;
+		load_hulls(sh, bodies, a, b, H);
+		/* clipping reads a few vertices of two faces: from the (cached) table */
+		PairGeom P;
+		P.vA = sh.vertices + H.ha.m_vertexOffset;
+		P.vB = sh.vertices + H.hb.m_vertexOffset;
+		P.ldsA = P.ldsB = false;
+		P.nA = H.ha.m_numVertices;
+		P.nB = H.hb.m_numVertices;
+		P.posA = H.A.m_pos;
+		P.posB = H.B.m_pos;
+		P.ornA = H.A.m_quat;
+		P.ornB = H.B.m_quat;
+		wave_sync(); /* the previous pair's clip buffers are read */
+		F4 sep = sepOut[i];
+		F4* pts = sClip[gi][2];
+		int idx[4] = {0, 0, 0, 0};
+		int nReduced = 0;
+		int np = clip_hull_hull(g, sep, P, H.ha, H.hb, sh, sClip[gi][0], sClip[gi][1], pts,
+								sat_records(sc, a, H.ha.m_numFaces), sat_records(sc, b, H.hb.m_numFaces));
+		if (np > 0) nReduced = extract_manifold(g, pts, np, neg4(sep), idx);
+		if (g.lane == 0)
+			sat_finish(i, a, b, H.A, H.B, false, 1, sep, mprIn, nReduced, idx, pts, flags, staging, sepStateOut, sepOut);
 	}
 }
 
@@ -1411,7 +1717,7 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 					 const b3hShapeTables& sh, b3hContact4Data* staging, int* flags, int* flagScan, unsigned* scanTmp,
 					 int* contactPair, int maxContacts, int* numContacts, int* sepState, b3hFloat4* sepNormal,
 					 int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel, b3hFloat4* mprScratch,
-					 hipEvent_t evAfterMpr, bool hasPrimitives)
+					 hipEvent_t evAfterMpr, bool hasPrimitives, const b3hSatCache* satCache, const b3hSatWork& work)
 {
 	{
 		int mb = (maxPairs + 255) / 256;
@@ -1420,22 +1726,58 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 		hipLaunchKernelGGL(k_np_mpr, dim3(mb), dim3(256), 0, s, pairs, numPairs, bodies, sh, (F4*)mprScratch);
 	}
 	if (evAfterMpr) hipEventRecord(evAfterMpr, s);
-	/* lanes per pair: 64 (one pair per wavefront), 32 or 16; the block holds
-	 * eight groups' LDS staging (40 KB) for the narrower groups */
-	auto launch = [&](auto kern, int lanes, int threads) {
-		int groups = threads / lanes;
-		int blocks = (maxPairs + groups - 1) / groups;
-		if (blocks > 16384) blocks = 16384;
-		if (blocks < 1) blocks = 1;
-		hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, s, pairs, numPairs, bodies, sh, staging, flags,
-						   sepState, sepNormal, (const F4*)mprScratch);
+	b3hSatCache sc = {nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, 0};
+	if (satCache && satCache->numBodies > 0 && satCache->total > 0)
+	{
+		sc = *satCache;
+		int fb = (sc.total + 255) / 256;
+		if (fb > 16384) fb = 16384;
+		hipLaunchKernelGGL(k_sat_faces, dim3(fb), dim3(256), 0, s, bodies, sh, sc);
+		if (sc.numDirBodies > 0 && sc.dirs)
+		{
+			int db = (sc.numDirBodies * B3_UNIT_SPHERE_162_COUNT + 255) / 256;
+			if (db > 16384) db = 16384;
+			hipLaunchKernelGGL(k_sat_dirs, dim3(db), dim3(256), 0, s, bodies, sh, sc);
+		}
+	}
+	/* grid: four times the blocks the CUs hold at four waves per SIMD.  The
+	 * blocks loop over pairs, so a larger grid only adds block start-ups, each
+	 * of which writes its loop-invariant register spills to fresh scratch
+	 * lines (HBM traffic); the factor four leaves the block scheduler room to
+	 * balance pairs of unequal cost. */
+	const int maxBlocks256 = 16 * b3h_cu_count();
+	auto blocksFor = [&](int items, int perBlock, int threads) {
+		int blocks = (items + perBlock - 1) / perBlock;
+		int cap = maxBlocks256 * (256 / threads);
+		if (blocks > cap) blocks = cap;
+		return blocks < 1 ? 1 : blocks;
+	};
+	/* pass 1, faces: lanes per pair 64 (one pair per wavefront), 32 or 16 */
+	auto facePass = [&](auto kern, int lanes, int threads) {
+		hipLaunchKernelGGL(kern, dim3(blocksFor(maxPairs, threads / lanes, threads)), dim3(threads), 0, s, pairs,
+						   numPairs, bodies, sh, staging, flags, sepState, sepNormal, (const F4*)mprScratch, sc, work.dmin,
+						   work.flag);
 	};
 	if (pairsPerGroup == 16)
-		launch(k_narrowphase<16, 128>, 16, 128);
+		facePass(k_sat_face_pass<16, 128>, 16, 128);
 	else if (pairsPerGroup == 32)
-		launch(k_narrowphase<32, 256>, 32, 256);
+		facePass(k_sat_face_pass<32, 256>, 32, 256);
 	else
-		launch(k_narrowphase<64, NP_THREADS>, 64, NP_THREADS);
+		facePass(k_sat_face_pass<64, NP_THREADS>, 64, NP_THREADS);
+	/* pass 2, edges, over the open pairs */
+	int lb = blocksFor(maxPairs, 256, 256);
+	b3h_exclusive_scan(work.flag, work.scan, numPairs, maxPairs, scanTmp, s);
+	hipLaunchKernelGGL(k_pair_list, dim3(lb), dim3(256), 0, s, numPairs, work.flag, work.scan, work.list, work.count);
+	hipLaunchKernelGGL(k_sat_edge_pass, dim3(blocksFor(maxPairs, 4, 256)), dim3(256), 0, s, pairs, bodies, sh, staging,
+					   flags, sepState, sepNormal, (const F4*)mprScratch, sc, (const float*)work.dmin,
+					   (const int*)work.count, (const int*)work.list, work.flag);
+	/* pass 3, clipping, over the pairs pass 2 flagged (the flags of pairs it
+	 * did not visit are still 0 from pass 1) */
+	b3h_exclusive_scan(work.flag, work.scan, numPairs, maxPairs, scanTmp, s);
+	hipLaunchKernelGGL(k_pair_list, dim3(lb), dim3(256), 0, s, numPairs, work.flag, work.scan, work.list, work.count);
+	hipLaunchKernelGGL((k_sat_clip_pass<16, 256>), dim3(blocksFor(maxPairs, 16, 256)), dim3(256), 0, s, pairs,
+					   (const int*)work.count, (const int*)work.list, bodies, sh, staging, flags, sepState, sepNormal,
+					   (const F4*)mprScratch, sc);
 	if (hasPrimitives)
 	{
 		int pb = (maxPairs + PC_THREADS - 1) / PC_THREADS;

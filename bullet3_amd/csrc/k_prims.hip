@@ -241,6 +241,20 @@ void b3h_exclusive_scan(const int* in, int* out, const int* nDev, int maxN, unsi
 }
 size_t b3h_scan_tmp_size(int maxN) { return (size_t)(maxN + SC_TILE - 1) / SC_TILE + 2; }
 
+int b3h_cu_count()
+{
+	static int cached[64] = {0};
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+	if (cached[dev] <= 0)
+	{
+		int n = 0;
+		if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+		cached[dev] = n;
+	}
+	return cached[dev];
+}
+
 /* fill helpers (used instead of memset nodes so steps can be graph-captured) */
 __global__ void k_fill_i32(int* p, int v, int n)
 {

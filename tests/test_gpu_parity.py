@@ -138,6 +138,42 @@ def test_narrowphase_bit_exact(scene, lanes):
     w.close()
 
 
+@pytest.mark.parametrize("lanes", [64, 32])
+def test_narrowphase_shapes_changed_on_device(lanes):
+    """Bodies whose collidable is rewritten in device memory only (the host
+    mirror, and with it the per-body face-record ranges, still has the old
+    shapes): the SAT projects those bodies' faces and fixed directions itself,
+    the rest come from the records, and the results stay the oracle's."""
+    w = World(small_pile(n=(6, 4, 6), seed=9))
+    capi.call("b3h_np_set_pairs_per_group", w.np_, lanes)
+    ow = OracleWorld(w)
+    run_steps(w, ow, 2)
+    b = w.bodies()
+    dyn = np.nonzero(b["invMass"] > 0)[0]
+    b2 = b.copy()
+    b2["collidableIdx"][dyn] = np.roll(b["collidableIdx"][dyn], 1)
+    assert np.count_nonzero(b2["collidableIdx"] != b["collidableIdx"]) > len(dyn) // 2
+    w.set_bodies(b2)
+    capi.call("b3h_pipeline_setup_gpu_aabbs_full", w.pl)
+    capi.call("b3h_bp_calculate_overlapping_pairs", w.bp, w.cfg.m_maxBroadphasePairs)
+    pairs = w.pairs()
+    assert len(pairs) > 50
+    d = ctypes.c_void_p()
+    capi.call("b3h_bp_get_overlapping_pair_buffer", w.bp, ctypes.byref(d))
+    capi.call("b3h_np_compute_contacts", w.np_, d, len(pairs), None, 0)
+    gpu_c = w.contacts()
+    st, sn = ctypes.c_void_p(), ctypes.c_void_p()
+    capi.call("b3h_np_get_sat_debug", w.np_, ctypes.byref(st), ctypes.byref(sn))
+    g_state = capi.dtoh(st.value, len(pairs) * 4, np.int32)
+    g_sep = capi.dtoh(sn.value, len(pairs) * 16, np.float32).reshape(-1, 4)
+    n, cpu_c, _, has_sep, sep = po.narrowphase(pairs, b2, ow.t, capacity=w.cfg.m_maxContactCapacity, want_sep=True)
+    assert np.array_equal(g_state, has_sep)
+    assert bits_equal(g_sep, sep)
+    assert n == len(gpu_c) and n > 0
+    assert bits_equal(gpu_c, cpu_c.view(layout.CONTACT)), first_diff(gpu_c, cpu_c.view(layout.CONTACT))
+    w.close()
+
+
 @pytest.mark.parametrize("cache", [-1, 0, 3])
 def test_solver_bit_exact(scene, cache):
     """cache: bodies per cell kept in LDS (-1: the LDS budget; 0: every cell on
