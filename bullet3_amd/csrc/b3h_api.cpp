@@ -136,6 +136,10 @@ struct b3hNarrowPhase
 	std::vector<b3hGpuFace> faces;
 	std::vector<int> indices;
 	std::vector<b3hAabb> localAabbs;
+	std::vector<b3hGpuChildShape> children;
+	std::vector<b3hBvhInfo> bvhInfos;
+	std::vector<b3hQuantizedBvhNode> bvhNodes;
+	std::vector<b3hBvhSubtreeInfo> bvhSubtrees;
 	std::vector<b3hRigidBodyData> bodies;
 	std::vector<b3hInertiaData> inertias;
 	int static0Index;
@@ -148,7 +152,12 @@ struct b3hNarrowPhase
 	b3hGpuFace* dFaces;
 	int* dIndices;
 	b3hAabb* dLocalAabbs;
+	b3hGpuChildShape* dChildren;
+	b3hBvhInfo* dBvhInfos;
+	b3hQuantizedBvhNode* dBvhNodes;
+	b3hBvhSubtreeInfo* dBvhSubtrees;
 	size_t capCols, capConvex, capVerts, capEdges, capFaces, capIndices, capLocal;
+	size_t capChildren, capBvhInfos, capBvhNodes, capBvhSubtrees;
 	b3hContact4Data* dStaging;
 	int *dFlags, *dFlagScan, *dContactPair, *dNumContacts, *dNumPairs;
 	unsigned* dScanTmp;
@@ -167,6 +176,7 @@ struct b3hNarrowPhase
 	int satBodies, satTotal;
 	float2* dSatDirs;
 	int *dSatDslot, *dSatDowner;
+	b3hSatBody* dSatDesc;
 	size_t satCapDirBodies;
 	int satDirBodies;
 	unsigned satGen;
@@ -223,7 +233,12 @@ extern "C" int b3h_np_create(const b3hConfig* cfg, void* stream, b3hNarrowPhase*
 	np->dFaces = nullptr;
 	np->dIndices = nullptr;
 	np->dLocalAabbs = nullptr;
+	np->dChildren = nullptr;
+	np->dBvhInfos = nullptr;
+	np->dBvhNodes = nullptr;
+	np->dBvhSubtrees = nullptr;
 	np->capCols = np->capConvex = np->capVerts = np->capEdges = np->capFaces = np->capIndices = np->capLocal = 0;
+	np->capChildren = np->capBvhInfos = np->capBvhNodes = np->capBvhSubtrees = 0;
 	np->dStaging = nullptr;
 	np->dFlags = np->dFlagScan = np->dContactPair = np->dNumContacts = np->dNumPairs = nullptr;
 	np->dScanTmp = nullptr;
@@ -239,6 +254,7 @@ extern "C" int b3h_np_create(const b3hConfig* cfg, void* stream, b3hNarrowPhase*
 	np->satBodies = np->satTotal = 0;
 	np->dSatDirs = nullptr;
 	np->dSatDslot = np->dSatDowner = nullptr;
+	np->dSatDesc = nullptr;
 	np->satCapDirBodies = 0;
 	np->satDirBodies = 0;
 	np->satGen = 0;
@@ -268,6 +284,10 @@ extern "C" int b3h_np_destroy(b3hNarrowPhase* np)
 	dfree(np->dFaces);
 	dfree(np->dIndices);
 	dfree(np->dLocalAabbs);
+	dfree(np->dChildren);
+	dfree(np->dBvhInfos);
+	dfree(np->dBvhNodes);
+	dfree(np->dBvhSubtrees);
 	dfree(np->dStaging);
 	dfree(np->dFlags);
 	dfree(np->dFlagScan);
@@ -290,6 +310,7 @@ extern "C" int b3h_np_destroy(b3hNarrowPhase* np)
 	dfree(np->dSatDirs);
 	dfree(np->dSatDslot);
 	dfree(np->dSatDowner);
+	dfree(np->dSatDesc);
 	if (np->ownStream) hipStreamDestroy(np->stream);
 	delete np;
 	return B3H_OK;
@@ -487,6 +508,79 @@ extern "C" int b3h_np_register_plane_shape(b3hNarrowPhase* np, const float* norm
 	return B3H_OK;
 }
 
+/* b3GpuNarrowPhase::registerCompoundShape, b3GpuNarrowPhase.cpp:370-517: a
+ * compound of registered convex hulls placed by child transforms; its local
+ * AABB bounds the children's transformed local AABBs, and a quantized AABB
+ * tree over the children serves the compound-compound pair search */
+extern "C" int b3h_np_register_compound_shape(b3hNarrowPhase* np, const b3hGpuChildShape* children, int numChildren,
+											  int* outCollidable)
+{
+	CHECK_ARG(np && children && numChildren > 0 && outCollidable);
+	*outCollidable = -1;
+	if ((int)np->cols.size() >= np->cfg.m_maxConvexShapes)
+	{
+		b3h_set_error("registerCompoundShape: exceeding the number of shapes (%d)", np->cfg.m_maxConvexShapes);
+		return B3H_ERR_CAPACITY;
+	}
+	if ((long long)np->children.size() + numChildren >= (long long)np->cfg.m_maxCompoundChildShapes)
+	{
+		b3h_set_error("registerCompoundShape: exceeding the number of child shapes (%d)", np->cfg.m_maxCompoundChildShapes);
+		return B3H_ERR_CAPACITY;
+	}
+	std::vector<b3hAabb> childLocal((size_t)numChildren);
+	for (int i = 0; i < numChildren; i++)
+	{
+		int c = children[i].m_shapeIndex;
+		if (c < 0 || c >= (int)np->cols.size() || np->cols[(size_t)c].m_shapeType != B3H_SHAPE_CONVEX_HULL)
+		{
+			b3h_set_error("registerCompoundShape: child %d refers to collidable %d, which is not a convex hull", i, c);
+			return B3H_ERR_INVALID;
+		}
+		childLocal[(size_t)i] = np->localAabbs[(size_t)c];
+	}
+	b3hAabb aabb;
+	b3hBvhInfo info;
+	std::vector<b3hQuantizedBvhNode> nodes;
+	std::vector<b3hBvhSubtreeInfo> subtrees;
+	int r = b3h_build_compound(numChildren, children, childLocal.data(), &aabb, &info, nodes, subtrees);
+	if (r) return r;
+	np->gen++;
+	info.m_nodeOffset = (int)np->bvhNodes.size();
+	info.m_subTreeOffset = (int)np->bvhSubtrees.size();
+	b3hCollidable col;
+	col.m_numChildShapes = numChildren;
+	col.m_compoundBvhIndex = (int)np->bvhInfos.size();
+	col.m_shapeType = B3H_SHAPE_COMPOUND_OF_CONVEX_HULLS;
+	col.m_shapeIndex = (int)np->children.size();
+	np->children.insert(np->children.end(), children, children + numChildren);
+	np->bvhInfos.push_back(info);
+	np->bvhNodes.insert(np->bvhNodes.end(), nodes.begin(), nodes.end());
+	np->bvhSubtrees.insert(np->bvhSubtrees.end(), subtrees.begin(), subtrees.end());
+	np->localAabbs.push_back(aabb);
+	*outCollidable = (int)np->cols.size();
+	np->cols.push_back(col);
+	return B3H_OK;
+}
+
+/* the host tables of a compound without registering it (tests and tools) */
+extern "C" int b3h_compute_compound_tables(const b3hGpuChildShape* children, int numChildren, const b3hAabb* childLocal,
+										   b3hAabb* aabbOut, b3hBvhInfo* infoOut, b3hQuantizedBvhNode* nodesOut,
+										   int nodesCap, int* numNodesOut, b3hBvhSubtreeInfo* subtreesOut,
+										   int subtreesCap, int* numSubtreesOut)
+{
+	CHECK_ARG(children && numChildren > 0 && childLocal && aabbOut && infoOut && numNodesOut && numSubtreesOut);
+	std::vector<b3hQuantizedBvhNode> nodes;
+	std::vector<b3hBvhSubtreeInfo> subtrees;
+	int r = b3h_build_compound(numChildren, children, childLocal, aabbOut, infoOut, nodes, subtrees);
+	if (r) return r;
+	*numNodesOut = (int)nodes.size();
+	*numSubtreesOut = (int)subtrees.size();
+	CHECK_ARG((int)nodes.size() <= nodesCap && (int)subtrees.size() <= subtreesCap && nodesOut && subtreesOut);
+	memcpy(nodesOut, nodes.data(), nodes.size() * sizeof(b3hQuantizedBvhNode));
+	memcpy(subtreesOut, subtrees.data(), subtrees.size() * sizeof(b3hBvhSubtreeInfo));
+	return B3H_OK;
+}
+
 extern "C" int b3h_np_register_convex_polyhedron(b3hNarrowPhase* np, const b3hConvexUtility* u, int* outCollidable)
 {
 	CHECK_ARG(np && u && outCollidable && u->numVertices > 0 && u->vertices && u->numFaces >= 0 &&
@@ -553,7 +647,7 @@ extern "C" int b3h_np_load_shape_tables(b3hNarrowPhase* np, const int* counts, c
 }
 
 /* b3Matrix3x3::setRotation (scalar path, b3Matrix3x3.h:201-268) */
-static void quat_to_basis(const float q[4], float m[3][3])
+void b3h_quat_to_basis(const float q[4], float m[3][3])
 {
 	float d = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
 	float s = 2.0f / d;
@@ -588,7 +682,7 @@ static void body_inertia(float mass, const float* aabbMin, const float* aabbMax,
 	in->m_initInvInertia.m_row[1] = f4(0, il[1], 0, 0);
 	in->m_initInvInertia.m_row[2] = f4(0, 0, il[2], 0);
 	float m[3][3];
-	quat_to_basis(orn, m);
+	b3h_quat_to_basis(orn, m);
 	float ms[3][3];
 	for (int r = 0; r < 3; r++)
 		for (int k = 0; k < 3; k++) ms[r][k] = m[r][k] * il[k];
@@ -665,6 +759,10 @@ extern "C" int b3h_np_write_all_bodies_to_gpu(b3hNarrowPhase* np)
 	if ((r = upload_table(&np->dEdges, &np->capEdges, np->edges, np->stream))) return r;
 	if ((r = upload_table(&np->dFaces, &np->capFaces, np->faces, np->stream))) return r;
 	if ((r = upload_table(&np->dIndices, &np->capIndices, np->indices, np->stream))) return r;
+	if ((r = upload_table(&np->dChildren, &np->capChildren, np->children, np->stream))) return r;
+	if ((r = upload_table(&np->dBvhInfos, &np->capBvhInfos, np->bvhInfos, np->stream))) return r;
+	if ((r = upload_table(&np->dBvhNodes, &np->capBvhNodes, np->bvhNodes, np->stream))) return r;
+	if ((r = upload_table(&np->dBvhSubtrees, &np->capBvhSubtrees, np->bvhSubtrees, np->stream))) return r;
 	if (!np->bodies.empty())
 	{
 		B3H_CHECK(hipMemcpyAsync(np->dBodies, np->bodies.data(), np->bodies.size() * sizeof(b3hRigidBodyData),
@@ -687,6 +785,10 @@ extern "C" int b3h_np_reset(b3hNarrowPhase* np)
 	np->faces.clear();
 	np->indices.clear();
 	np->localAabbs.clear();
+	np->children.clear();
+	np->bvhInfos.clear();
+	np->bvhNodes.clear();
+	np->bvhSubtrees.clear();
 	np->bodies.clear();
 	np->inertias.clear();
 	np->static0Index = -1;
@@ -744,6 +846,10 @@ static b3hShapeTables np_tables(b3hNarrowPhase* np)
 	t.uniqueEdges = np->dEdges;
 	t.faces = np->dFaces;
 	t.indices = np->dIndices;
+	t.children = np->dChildren;
+	t.bvhInfos = np->dBvhInfos;
+	t.bvhNodes = np->dBvhNodes;
+	t.bvhSubtrees = np->dBvhSubtrees;
 	return t;
 }
 
@@ -762,32 +868,6 @@ static int np_ensure_sat(b3hNarrowPhase* np)
 		return B3H_OK;
 	}
 	np->satValid = false;
-	std::vector<int> base((size_t)nb + 1);
-	long long total = 0;
-	for (int b = 0; b < nb; b++)
-	{
-		base[(size_t)b] = (int)total;
-		int ci = np->bodies[(size_t)b].m_collidableIdx;
-		if (ci < 0 || ci >= (int)np->cols.size() || np->cols[(size_t)ci].m_shapeType != B3H_SHAPE_CONVEX_HULL) continue;
-		int si = np->cols[(size_t)ci].m_shapeIndex;
-		if (si >= 0 && si < (int)np->convexes.size()) total += np->convexes[(size_t)si].m_numFaces;
-		if (total > (1ll << 30)) break;
-	}
-	np->satBodies = 0;
-	np->satTotal = 0;
-	if (nb == 0 || total == 0 || total > (1ll << 30)) /* nothing to cache, or too large: SAT projects directly */
-	{
-		np->satGen = np->gen;
-		np->satValid = true;
-		np->satCols.clear();
-		return B3H_OK;
-	}
-	base[(size_t)nb] = (int)total;
-	std::vector<int> owner((size_t)total);
-	for (int b = 0; b < nb; b++)
-		for (int r = base[(size_t)b]; r < base[(size_t)b + 1]; r++) owner[(size_t)r] = b;
-	/* fixed-direction intervals for every body that can meet a pair whose
-	 * edge-pair count exceeds the 162 directions */
 	auto hullOf = [&](int b) -> const b3hConvexPolyhedronData* {
 		int ci = np->bodies[(size_t)b].m_collidableIdx;
 		if (ci < 0 || ci >= (int)np->cols.size() || np->cols[(size_t)ci].m_shapeType != B3H_SHAPE_CONVEX_HULL)
@@ -795,6 +875,21 @@ static int np_ensure_sat(b3hNarrowPhase* np)
 		int si = np->cols[(size_t)ci].m_shapeIndex;
 		return (si >= 0 && si < (int)np->convexes.size()) ? &np->convexes[(size_t)si] : nullptr;
 	};
+	/* face-record ranges */
+	std::vector<int> base((size_t)nb + 1);
+	long long total = 0;
+	for (int b = 0; b < nb; b++)
+	{
+		base[(size_t)b] = (int)std::min<long long>(total, 1ll << 30);
+		if (const b3hConvexPolyhedronData* h = hullOf(b)) total += h->m_numFaces;
+	}
+	if (total > (1ll << 30)) total = 0; /* too large: the SAT projects the faces itself */
+	base[(size_t)nb] = (int)total;
+	std::vector<int> owner((size_t)total);
+	for (int b = 0; total > 0 && b < nb; b++)
+		for (int r = base[(size_t)b]; r < base[(size_t)b + 1]; r++) owner[(size_t)r] = b;
+	/* fixed-direction intervals for every body that can meet a pair whose
+	 * edge-pair count exceeds the 162 directions */
 	long long maxE = 0;
 	for (int b = 0; b < nb; b++)
 		if (const b3hConvexPolyhedronData* h = hullOf(b)) maxE = std::max<long long>(maxE, h->m_numUniqueEdges);
@@ -810,28 +905,27 @@ static int np_ensure_sat(b3hNarrowPhase* np)
 	}
 	B3H_CHECK(hipStreamSynchronize(np->stream)); /* earlier steps may still read the old ranges */
 	int r;
-	if (np->satCapBase < (size_t)nb + 1)
+	size_t nbAlloc = (size_t)(nb > 0 ? nb : 1);
+	if (np->satCapBase < nbAlloc + 1)
 	{
 		dfree(np->dSatBase);
 		dfree(np->dSatDslot);
+		dfree(np->dSatDesc);
 		np->satCapBase = 0;
-		if ((r = dalloc(&np->dSatBase, (size_t)nb + 1)) || (r = dalloc(&np->dSatDslot, (size_t)nb + 1))) return r;
-		np->satCapBase = (size_t)nb + 1;
+		if ((r = dalloc(&np->dSatBase, nbAlloc + 1)) || (r = dalloc(&np->dSatDslot, nbAlloc + 1)) ||
+			(r = dalloc(&np->dSatDesc, nbAlloc)))
+			return r;
+		np->satCapBase = nbAlloc + 1;
 	}
 	if (np->satCapDirBodies < downer.size())
 	{
 		dfree(np->dSatDirs);
 		dfree(np->dSatDowner);
 		np->satCapDirBodies = 0;
-		if ((r = dalloc(&np->dSatDirs, downer.size() * B3H_SAT_DIRS)) ||
-			(r = dalloc(&np->dSatDowner, downer.size())))
+		if ((r = dalloc(&np->dSatDirs, downer.size() * B3H_SAT_DIRS)) || (r = dalloc(&np->dSatDowner, downer.size())))
 			return r;
 		np->satCapDirBodies = downer.size();
 	}
-	B3H_CHECK(hipMemcpy(np->dSatDslot, dslot.data(), sizeof(int) * (size_t)nb, hipMemcpyHostToDevice));
-	if (!downer.empty())
-		B3H_CHECK(hipMemcpy(np->dSatDowner, downer.data(), sizeof(int) * downer.size(), hipMemcpyHostToDevice));
-	np->satDirBodies = (int)downer.size();
 	if (np->satCapRecs < (size_t)total)
 	{
 		dfree(np->dSatRecs);
@@ -840,8 +934,16 @@ static int np_ensure_sat(b3hNarrowPhase* np)
 		if ((r = dalloc(&np->dSatRecs, (size_t)total)) || (r = dalloc(&np->dSatOwner, (size_t)total))) return r;
 		np->satCapRecs = (size_t)total;
 	}
-	B3H_CHECK(hipMemcpy(np->dSatBase, base.data(), sizeof(int) * ((size_t)nb + 1), hipMemcpyHostToDevice));
-	B3H_CHECK(hipMemcpy(np->dSatOwner, owner.data(), sizeof(int) * (size_t)total, hipMemcpyHostToDevice));
+	if (nb > 0)
+	{
+		B3H_CHECK(hipMemcpy(np->dSatDslot, dslot.data(), sizeof(int) * (size_t)nb, hipMemcpyHostToDevice));
+		B3H_CHECK(hipMemcpy(np->dSatBase, base.data(), sizeof(int) * ((size_t)nb + 1), hipMemcpyHostToDevice));
+	}
+	if (!downer.empty())
+		B3H_CHECK(hipMemcpy(np->dSatDowner, downer.data(), sizeof(int) * downer.size(), hipMemcpyHostToDevice));
+	if (total > 0)
+		B3H_CHECK(hipMemcpy(np->dSatOwner, owner.data(), sizeof(int) * (size_t)total, hipMemcpyHostToDevice));
+	np->satDirBodies = (int)downer.size();
 	np->satCols.resize((size_t)nb);
 	for (int b = 0; b < nb; b++) np->satCols[(size_t)b] = np->bodies[(size_t)b].m_collidableIdx;
 	np->satNumConvex = np->convexes.size();
@@ -852,21 +954,34 @@ static int np_ensure_sat(b3hNarrowPhase* np)
 	return B3H_OK;
 }
 
-/* device-count form used inside a step */
-static void np_compute_contacts_dev(b3hNarrowPhase* np, b3hInt4* pairs, const int* dNumPairs, int maxPairs,
-									hipEvent_t evAfterKernel = nullptr, hipEvent_t evAfterMpr = nullptr)
+/* device-count form used inside a step; the per-body SAT data must be set up
+ * (np_ensure_sat) before, outside any graph capture */
+static int np_compute_contacts_dev(b3hNarrowPhase* np, b3hInt4* pairs, const int* dNumPairs, int maxPairs,
+								   hipEvent_t evAfterKernel = nullptr, hipEvent_t evAfterMpr = nullptr)
 {
+	if (!np->satValid || np->satGen != np->gen || np->satBodies != (int)np->bodies.size())
+	{
+		b3h_set_error("narrowphase: per-body SAT data not set up for the current bodies");
+		return B3H_ERR_STATE;
+	}
 	bool prim = false;
 	for (const b3hCollidable& c : np->cols) prim = prim || c.m_shapeType != B3H_SHAPE_CONVEX_HULL;
-	b3hSatCache sc = {nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, 0};
-	if (np->satValid && np->satGen == np->gen && np->satBodies > 0)
-		sc = {np->dSatRecs, np->dSatBase, np->dSatOwner, np->satBodies, np->satTotal,
-			  np->satDirBodies > 0 ? np->dSatDirs : nullptr, np->dSatDslot, np->dSatDowner, np->satDirBodies};
+	b3hSatCache sc = {np->satTotal > 0 ? np->dSatRecs : nullptr,
+					  np->dSatBase,
+					  np->dSatOwner,
+					  np->satBodies,
+					  np->satTotal,
+					  np->satDirBodies > 0 ? np->dSatDirs : nullptr,
+					  np->dSatDslot,
+					  np->dSatDowner,
+					  np->satDirBodies,
+					  np->dSatDesc};
 	b3h_narrowphase(pairs, dNumPairs, maxPairs, np->dBodies, np_tables(np), np->dStaging, np->dFlags, np->dFlagScan,
 					np->dScanTmp, np->dContactPair, np->cfg.m_maxContactCapacity, np->dNumContacts, np->dSepState,
 					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr, evAfterMpr, prim, &sc,
 					b3hSatWork{np->dSatDmin, np->dSatFlag, np->dSatScan, np->dSatList, np->dSatCount});
 	np->contactsValid = false;
+	return B3H_OK;
 }
 
 extern "C" int b3h_np_compute_contacts(b3hNarrowPhase* np, b3hInt4* pairs, int numPairs, const b3hAabb* aabbs, int numObjects)
@@ -877,7 +992,7 @@ extern "C" int b3h_np_compute_contacts(b3hNarrowPhase* np, b3hInt4* pairs, int n
 	if (numPairs > 0) CHECK_ARG(pairs);
 	if (int re = np_ensure_sat(np)) return re;
 	B3H_CHECK(hipMemcpyAsync(np->dNumPairs, &numPairs, sizeof(int), hipMemcpyHostToDevice, np->stream));
-	np_compute_contacts_dev(np, pairs, np->dNumPairs, numPairs > 0 ? numPairs : 1);
+	if (int rc = np_compute_contacts_dev(np, pairs, np->dNumPairs, numPairs > 0 ? numPairs : 1)) return rc;
 	B3H_CHECK(hipGetLastError());
 	return B3H_OK;
 }
@@ -956,6 +1071,21 @@ extern "C" int b3h_np_get_shape_tables(b3hNarrowPhase* np, int* counts, b3hColli
 	if (faces && !np->faces.empty()) memcpy(faces, np->faces.data(), np->faces.size() * sizeof(*faces));
 	if (indices && !np->indices.empty()) memcpy(indices, np->indices.data(), np->indices.size() * sizeof(*indices));
 	if (la && !np->localAabbs.empty()) memcpy(la, np->localAabbs.data(), np->localAabbs.size() * sizeof(*la));
+	return B3H_OK;
+}
+/* the compound tables (counts: children, tree infos, tree nodes, subtrees) */
+extern "C" int b3h_np_get_compound_tables(b3hNarrowPhase* np, int* counts, b3hGpuChildShape* children,
+										  b3hBvhInfo* infos, b3hQuantizedBvhNode* nodes, b3hBvhSubtreeInfo* subtrees)
+{
+	CHECK_ARG(np && counts);
+	counts[0] = (int)np->children.size();
+	counts[1] = (int)np->bvhInfos.size();
+	counts[2] = (int)np->bvhNodes.size();
+	counts[3] = (int)np->bvhSubtrees.size();
+	if (children && counts[0]) memcpy(children, np->children.data(), np->children.size() * sizeof(*children));
+	if (infos && counts[1]) memcpy(infos, np->bvhInfos.data(), np->bvhInfos.size() * sizeof(*infos));
+	if (nodes && counts[2]) memcpy(nodes, np->bvhNodes.data(), np->bvhNodes.size() * sizeof(*nodes));
+	if (subtrees && counts[3]) memcpy(subtrees, np->bvhSubtrees.data(), np->bvhSubtrees.size() * sizeof(*subtrees));
 	return B3H_OK;
 }
 extern "C" int b3h_np_get_inertias_cpu(b3hNarrowPhase* np, b3hInertiaData* dst, int n)
@@ -1737,8 +1867,9 @@ static int pl_step(b3hPipeline* p, float dt, bool events)
 	int r = bp_calc_pairs(p->bp, p->cfg.m_maxBroadphasePairs, s);
 	if (r) return r;
 	if (events) hipEventRecord(p->ev[2], s);
-	np_compute_contacts_dev(p->np, p->bp->dPairs, p->bp->dPairCount, p->cfg.m_maxBroadphasePairs,
-							events ? p->ev[4] : nullptr, events ? p->ev[3] : nullptr);
+	if ((r = np_compute_contacts_dev(p->np, p->bp->dPairs, p->bp->dPairCount, p->cfg.m_maxBroadphasePairs,
+									 events ? p->ev[4] : nullptr, events ? p->ev[3] : nullptr)))
+		return r;
 	if (events) hipEventRecord(p->ev[5], s);
 	/* joints before the contact solve (b3GpuRigidBodyPipeline.cpp:361-372) */
 	if (p->jb.numConstraints > 0) b3h_solve_joints(p->jb, p->np->dBodies, p->np->dInertias, p->jp, s);
@@ -1910,7 +2041,7 @@ extern "C" int b3h_pipeline_solve_contacts(b3hPipeline* p)
 static void transform_aabb(const b3hAabb& la, float margin, const float* pos, const float* orn, float* mn, float* mx)
 {
 	float m[3][3];
-	quat_to_basis(orn, m);
+	b3h_quat_to_basis(orn, m);
 	float he[3], lc[3];
 	const float lmin[3] = {la.m_min.x, la.m_min.y, la.m_min.z};
 	const float lmax[3] = {la.m_max.x, la.m_max.y, la.m_max.z};

@@ -86,6 +86,14 @@ void b3h_plbvh_pairs(const b3hAabb* all, const int* smallMap, int nSmall, const 
 void b3h_scene_contact_counts(const int* nPairs, int maxPairs, const b3hInt4* pairs, const int* sceneOf, int* counts,
 							  int numScenes, hipStream_t s);
 
+/* ---------------- host geometry (b3h_api.cpp, compound.cpp) ---------------- */
+/* b3Matrix3x3::setRotation of (x, y, z, w) */
+void b3h_quat_to_basis(const float q[4], float m[3][3]);
+/* a compound's local AABB and quantized child tree (offsets in info left 0) */
+int b3h_build_compound(int n, const b3hGpuChildShape* children, const b3hAabb* childLocal, b3hAabb* compoundAabb,
+					   b3hBvhInfo* info, std::vector<b3hQuantizedBvhNode>& nodes,
+					   std::vector<b3hBvhSubtreeInfo>& subtrees);
+
 /* compute units of the current device (cached per device) */
 int b3h_cu_count();
 
@@ -98,6 +106,11 @@ struct b3hShapeTables
 	const b3hFloat4* uniqueEdges;
 	const b3hGpuFace* faces;
 	const int* indices;
+	/* compounds: children, and the quantized child trees */
+	const b3hGpuChildShape* children;
+	const b3hBvhInfo* bvhInfos;
+	const b3hQuantizedBvhNode* bvhNodes;
+	const b3hBvhSubtreeInfo* bvhSubtrees;
 };
 /* Per-body face records, rebuilt every step before the SAT kernel: for face f
  * of body b's hull, the world face normal w = qrot(orn_b, plane_f) and the
@@ -112,6 +125,19 @@ struct b3hSatFace
 	float mn, mx, pad0, pad1;
 };
 #define B3H_SAT_DIRS 162 /* fixed directions of the edge-edge approximation */
+/* Per-body SAT descriptor, rebuilt every step: the pose, the hull centre in
+ * the SAT frame (position with w = 0) and the table offsets the SAT passes
+ * read, so a pair reaches both hulls with one load per body. */
+#define B3H_SB_HULL 1   /* the body's collidable is a convex hull */
+#define B3H_SB_STATIC 2 /* inverse mass 0 */
+struct b3hSatBody
+{
+	b3hFloat4 pos, quat;
+	b3hFloat4 center;
+	int vOff, nV, fOff, nF;
+	int eOff, nE, rec, dir; /* face-record base and fixed-direction slot, -1 when absent */
+	int flags, col, pad0, pad1;
+};
 struct b3hSatCache
 {
 	b3hSatFace* recs;
@@ -126,6 +152,7 @@ struct b3hSatCache
 	const int* dslot;  /* numBodies */
 	const int* downer; /* numDirBodies */
 	int numDirBodies;
+	b3hSatBody* desc; /* numBodies */
 };
 /* per-pair buffers between the SAT passes (maxPairs entries each; flag and
  * scan one and two more) */

@@ -760,17 +760,17 @@ __device__ int clip_pass(const Group<L>& g, const F4* in, int nIn, F4 n, float e
 /* clipHullAgainstHull (satClipHullContacts.cl:488-541): returns the kept points
  * (xyz + depth) in `out` */
 template <int L>
-__device__ int clip_hull_hull(const Group<L>& g, F4 sep, const PairGeom& P, const b3hConvexPolyhedronData& ha,
-							  const b3hConvexPolyhedronData& hb, const b3hShapeTables& sh, F4* bufA, F4* bufB, F4* out,
-							  const b3hSatFace* cA, const b3hSatFace* cB)
+__device__ int clip_hull_hull(const Group<L>& g, F4 sep, const PairGeom& P, const b3hSatBody& ha, const b3hSatBody& hb,
+							  const b3hShapeTables& sh, F4* bufA, F4* bufB, F4* out, const b3hSatFace* cA,
+							  const b3hSatFace* cB)
 {
 	const float minDist = -1e30f, maxDist = 0.02f;
 	/* incident face of B: largest dot with the separating normal */
 	float bv = -INFINITY;
 	int bi = INT_MAX;
-	for (int f = g.lane; f < hb.m_numFaces; f += L)
+	for (int f = g.lane; f < hb.nF; f += L)
 	{
-		F4 wn = cB ? cB[f].w : qrot(P.ornB, xyz0(sh.faces[hb.m_faceOffset + f].m_plane));
+		F4 wn = cB ? cB[f].w : qrot(P.ornB, xyz0(sh.faces[hb.fOff + f].m_plane));
 		float d = dot3(wn, sep);
 		if (d > -FLT_MAX && (bi == INT_MAX || d > bv))
 		{
@@ -780,7 +780,7 @@ __device__ int clip_hull_hull(const Group<L>& g, F4 sep, const PairGeom& P, cons
 	}
 	g.argmax(bv, bi);
 	if (bi == INT_MAX) return 0;
-	const b3hGpuFace polyB = sh.faces[hb.m_faceOffset + bi];
+	const b3hGpuFace polyB = sh.faces[hb.fOff + bi];
 	int nB = polyB.m_numIndices < CLIP_CAP ? polyB.m_numIndices : CLIP_CAP;
 	for (int e = g.lane; e < nB; e += L)
 		bufA[e] = xform(P.vB[sh.indices[polyB.m_indexOffset + e]], P.posB, P.ornB);
@@ -788,9 +788,9 @@ __device__ int clip_hull_hull(const Group<L>& g, F4 sep, const PairGeom& P, cons
 	/* reference face of A: smallest dot with the separating normal */
 	float av = INFINITY;
 	int ai = INT_MAX;
-	for (int f = g.lane; f < ha.m_numFaces; f += L)
+	for (int f = g.lane; f < ha.nF; f += L)
 	{
-		F4 wn = cA ? cA[f].w : qrot(P.ornA, xyz0(sh.faces[ha.m_faceOffset + f].m_plane));
+		F4 wn = cA ? cA[f].w : qrot(P.ornA, xyz0(sh.faces[ha.fOff + f].m_plane));
 		float d = dot3(wn, sep);
 		if (d < FLT_MAX && (ai == INT_MAX || d < av))
 		{
@@ -800,7 +800,7 @@ __device__ int clip_hull_hull(const Group<L>& g, F4 sep, const PairGeom& P, cons
 	}
 	g.argmin(av, ai);
 	if (ai == INT_MAX) return 0;
-	const b3hGpuFace polyA = sh.faces[ha.m_faceOffset + ai];
+	const b3hGpuFace polyA = sh.faces[ha.fOff + ai];
 	int nvA = polyA.m_numIndices;
 	const int* ia = sh.indices + polyA.m_indexOffset;
 	F4 wpn = cA ? cA[ai].w : qrot(P.ornA, xyz0(polyA.m_plane));
@@ -974,65 +974,113 @@ __global__ __launch_bounds__(256, 4) void k_np_mpr(const b3hInt4* __restrict__ p
 	}
 }
 
-/* ---------------- per-body face records ---------------- */
-/* the records of body b when they cover its current hull, else null */
-__device__ __forceinline__ const b3hSatFace* sat_records(const b3hSatCache& sc, int b, int numFaces)
+/* ---------------- per-body data of the SAT ---------------- */
+/* The descriptor of every body (thread per body).  The face-record base and
+ * the fixed-direction slot are kept only when they cover the body's current
+ * hull. */
+__global__ __launch_bounds__(256) void k_sat_bodies(const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh,
+													b3hSatCache sc)
 {
-	if (b >= sc.numBodies) return nullptr;
-	int s0 = sc.base[b], s1 = sc.base[b + 1];
-	if (s1 - s0 != numFaces || s0 < 0 || s1 > sc.total) return nullptr;
-	return sc.recs + s0;
-}
-/* body b's intervals on the 162 fixed directions, or null */
-__device__ __forceinline__ const float2* sat_dirs(const b3hSatCache& sc, int b)
-{
-	if (b >= sc.numBodies || !sc.dirs) return nullptr;
-	int k = sc.dslot[b];
-	if (k < 0 || k >= sc.numDirBodies) return nullptr;
-	return sc.dirs + (size_t)k * B3_UNIT_SPHERE_162_COUNT;
-}
-/* one (body, direction) per thread: body downer[r / 162]'s interval on the
- * unturned fixed direction r % 162 */
-__global__ __launch_bounds__(256) void k_sat_dirs(const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh,
-												  b3hSatCache sc)
-{
-	int total = sc.numDirBodies * B3_UNIT_SPHERE_162_COUNT;
-	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < total; r += gridDim.x * blockDim.x)
+	for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < sc.numBodies; b += gridDim.x * blockDim.x)
 	{
-		int k = r / B3_UNIT_SPHERE_162_COUNT, t = r - k * B3_UNIT_SPHERE_162_COUNT;
-		const b3hRigidBodyData body = bodies[sc.downer[k]];
+		const b3hRigidBodyData body = bodies[b];
 		const b3hCollidable col = sh.collidables[body.m_collidableIdx];
-		if (col.m_shapeType != B3H_SHAPE_CONVEX_HULL) continue;
-		const b3hConvexPolyhedronData h = sh.convexes[col.m_shapeIndex];
-		F4 pos = body.m_pos;
-		pos.w = 0.f;
-		F4 c = f4(c_unitSphere[t][0], c_unitSphere[t][1], c_unitSphere[t][2], 0.f);
-		float mn, mx;
-		project<false>(sh.vertices + h.m_vertexOffset, false, h.m_numVertices, pos, body.m_quat, c, &mn, &mx);
-		sc.dirs[r] = make_float2(mn, mx);
+		b3hSatBody d;
+		d.pos = body.m_pos;
+		d.quat = body.m_quat;
+		d.center = f4(0, 0, 0, 0);
+		d.vOff = d.nV = d.fOff = d.nF = d.eOff = d.nE = 0;
+		d.rec = d.dir = -1;
+		d.flags = body.m_invMass == 0 ? B3H_SB_STATIC : 0;
+		d.col = body.m_collidableIdx;
+		d.pad0 = d.pad1 = 0;
+		if (col.m_shapeType == B3H_SHAPE_CONVEX_HULL)
+		{
+			const b3hConvexPolyhedronData h = sh.convexes[col.m_shapeIndex];
+			d.flags |= B3H_SB_HULL;
+			d.vOff = h.m_vertexOffset;
+			d.nV = h.m_numVertices;
+			d.fOff = h.m_faceOffset;
+			d.nF = h.m_numFaces;
+			d.eOff = h.m_uniqueEdgesOffset;
+			d.nE = h.m_numUniqueEdges;
+			F4 p = body.m_pos;
+			p.w = 0.f;
+			d.center = xform(h.m_localCenter, p, body.m_quat);
+			if (sc.recs)
+			{
+				int s0 = sc.base[b], s1 = sc.base[b + 1];
+				if (s1 - s0 == h.m_numFaces && s0 >= 0 && s1 <= sc.total) d.rec = s0;
+			}
+			if (sc.dirs)
+			{
+				int k = sc.dslot[b];
+				if (k >= 0 && k < sc.numDirBodies) d.dir = k;
+			}
+		}
+		sc.desc[b] = d;
 	}
 }
-/* one record per thread: body owner[r], face r - base[owner[r]] of its hull */
-__global__ __launch_bounds__(256) void k_sat_faces(const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh,
-												   b3hSatCache sc)
+/* One record per thread: body owner[r], face r - base[owner[r]] of its hull:
+ * the world face normal and the body's own interval on it.  Records of a body
+ * whose hull changed on the device are not used (its descriptor has rec -1). */
+__global__ __launch_bounds__(256) void k_sat_faces(b3hShapeTables sh, b3hSatCache sc)
 {
 	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < sc.total; r += gridDim.x * blockDim.x)
 	{
-		int b = sc.owner[r];
-		const b3hRigidBodyData body = bodies[b];
-		const b3hCollidable col = sh.collidables[body.m_collidableIdx];
-		if (col.m_shapeType != B3H_SHAPE_CONVEX_HULL) continue;
-		const b3hConvexPolyhedronData h = sh.convexes[col.m_shapeIndex];
-		int f = r - sc.base[b];
-		if (f >= h.m_numFaces) continue;
-		F4 pos = body.m_pos;
+		const b3hSatBody d = sc.desc[sc.owner[r]];
+		if (d.rec < 0) continue;
+		int f = r - d.rec;
+		F4 pos = d.pos;
 		pos.w = 0.f;
 		b3hSatFace rec;
-		rec.w = qrot(body.m_quat, sh.faces[h.m_faceOffset + f].m_plane);
-		project<false>(sh.vertices + h.m_vertexOffset, false, h.m_numVertices, pos, body.m_quat, rec.w, &rec.mn, &rec.mx);
+		rec.w = qrot(d.quat, sh.faces[d.fOff + f].m_plane);
+		project<false>(sh.vertices + d.vOff, false, d.nV, pos, d.quat, rec.w, &rec.mn, &rec.mx);
 		rec.pad0 = 0.f;
 		rec.pad1 = 0.f;
 		sc.recs[r] = rec;
+	}
+}
+/* Intervals on the 162 fixed directions: items (body k, direction t) in
+ * tiles of 256; the few bodies a tile touches have their vertices staged in
+ * LDS first, and each thread projects its body on its direction. */
+#define SD_TILE 256
+#define SD_BODIES 3 /* bodies a tile of 256 items can touch (162 items each) */
+__global__ __launch_bounds__(SD_TILE) void k_sat_dirs(b3hShapeTables sh, b3hSatCache sc)
+{
+	__shared__ F4 sV[SD_BODIES][NP_VCAP];
+	const int total = sc.numDirBodies * B3_UNIT_SPHERE_162_COUNT;
+	const int t = threadIdx.x;
+	for (int base = blockIdx.x * SD_TILE; base < total; base += gridDim.x * SD_TILE)
+	{
+		const int k0 = base / B3_UNIT_SPHERE_162_COUNT;
+		/* stage: thread t loads vertex t % 64 of the tile's body t / 64 */
+		{
+			int j = t / NP_VCAP, v = t % NP_VCAP, k = k0 + j;
+			if (j < SD_BODIES && k < sc.numDirBodies)
+			{
+				const b3hSatBody d = sc.desc[sc.downer[k]];
+				if (d.nV <= NP_VCAP && v < d.nV) sV[j][v] = sh.vertices[d.vOff + v];
+			}
+		}
+		__syncthreads();
+		int r = base + t;
+		if (r < total)
+		{
+			int k = r / B3_UNIT_SPHERE_162_COUNT, dirIdx = r - k * B3_UNIT_SPHERE_162_COUNT;
+			const b3hSatBody d = sc.desc[sc.downer[k]];
+			if (d.flags & B3H_SB_HULL)
+			{
+				F4 pos = d.pos;
+				pos.w = 0.f;
+				F4 c = f4(c_unitSphere[dirIdx][0], c_unitSphere[dirIdx][1], c_unitSphere[dirIdx][2], 0.f);
+				float mn, mx;
+				bool inLds = d.nV <= NP_VCAP;
+				project<false>(inLds ? sV[k - k0] : sh.vertices + d.vOff, inLds, d.nV, pos, d.quat, c, &mn, &mx);
+				sc.dirs[r] = make_float2(mn, mx);
+			}
+		}
+		__syncthreads();
 	}
 }
 
@@ -1049,59 +1097,51 @@ __global__ __launch_bounds__(256) void k_sat_faces(const b3hRigidBodyData* __res
  * each wavefront on one kind of work: pairs that need clipping (about a fifth)
  * or an edge family (about a tenth) no longer hold the lanes of pairs that are
  * done after the face families. */
-struct PairHulls
-{
-	b3hRigidBodyData A, B;
-	b3hConvexPolyhedronData ha, hb;
-};
 /* vertices of both hulls into LDS (when they fit) and the pair geometry */
 template <int L>
-__device__ __forceinline__ void stage_pair(const Group<L>& g, const b3hShapeTables& sh, const PairHulls& H, F4* sA, F4* sB,
-										   PairGeom& P)
+__device__ __forceinline__ void stage_pair(const Group<L>& g, const b3hShapeTables& sh, const b3hSatBody& A,
+										   const b3hSatBody& B, F4* sA, F4* sB, PairGeom& P)
 {
-	const F4* gA = sh.vertices + H.ha.m_vertexOffset;
-	const F4* gB = sh.vertices + H.hb.m_vertexOffset;
-	bool ldsA = H.ha.m_numVertices <= NP_VCAP, ldsB = H.hb.m_numVertices <= NP_VCAP;
+	const F4* gA = sh.vertices + A.vOff;
+	const F4* gB = sh.vertices + B.vOff;
+	bool ldsA = A.nV <= NP_VCAP, ldsB = B.nV <= NP_VCAP;
 	wave_sync(); /* the previous pair's LDS reads are done */
 	if (ldsA)
-		for (int k = g.lane; k < H.ha.m_numVertices; k += L) sA[k] = gA[k];
+		for (int k = g.lane; k < A.nV; k += L) sA[k] = gA[k];
 	if (ldsB)
-		for (int k = g.lane; k < H.hb.m_numVertices; k += L) sB[k] = gB[k];
+		for (int k = g.lane; k < B.nV; k += L) sB[k] = gB[k];
 	wave_sync();
 	P.vA = ldsA ? sA : gA;
 	P.vB = ldsB ? sB : gB;
 	P.ldsA = ldsA;
 	P.ldsB = ldsB;
-	P.nA = H.ha.m_numVertices;
-	P.nB = H.hb.m_numVertices;
-	P.posA = H.A.m_pos;
-	P.posB = H.B.m_pos;
-	P.ornA = H.A.m_quat;
-	P.ornB = H.B.m_quat;
+	P.nA = A.nV;
+	P.nB = B.nV;
+	P.posA = A.pos;
+	P.posB = B.pos;
+	P.ornA = A.quat;
+	P.ornB = B.quat;
 }
-__device__ __forceinline__ void load_hulls(const b3hShapeTables& sh, const b3hRigidBodyData* bodies, int a, int b,
-										   PairHulls& H)
-{
-	H.A = bodies[a];
-	H.B = bodies[b];
-	H.ha = sh.convexes[sh.collidables[H.A.m_collidableIdx].m_shapeIndex];
-	H.hb = sh.convexes[sh.collidables[H.B.m_collidableIdx].m_shapeIndex];
-}
-/* the SAT frame: positions with w = 0 and the centre difference dc */
-__device__ __forceinline__ F4 sat_frame(const PairHulls& H, const PairGeom& P, PairGeom& Q)
+/* the SAT frame: positions with w = 0 */
+__device__ __forceinline__ void sat_frame(const PairGeom& P, PairGeom& Q)
 {
 	Q = P;
 	Q.posA.w = 0.f;
 	Q.posB.w = 0.f;
-	F4 c0 = xform(H.ha.m_localCenter, Q.posA, Q.ornA);
-	F4 c1 = xform(H.hb.m_localCenter, Q.posB, Q.ornB);
-	return sub4(c0, c1);
+}
+__device__ __forceinline__ const b3hSatFace* body_records(const b3hSatCache& sc, const b3hSatBody& d)
+{
+	return d.rec >= 0 ? sc.recs + d.rec : nullptr;
+}
+__device__ __forceinline__ const float2* body_dirs(const b3hSatCache& sc, const b3hSatBody& d)
+{
+	return d.dir >= 0 ? sc.dirs + (size_t)d.dir * B3_UNIT_SPHERE_162_COUNT : nullptr;
 }
 /* the pair's final record (lane 0 of the group): SAT state and axis, and for
  * a hull pair its flag and contact (the MPR point, then the reduced clip
  * points); pairs with a sphere or a plane get their contact from k_np_primitive */
-__device__ void sat_finish(int i, int a, int b, const b3hRigidBodyData& A, const b3hRigidBodyData& B, bool prim, int state,
-						   F4 sep, const F4* __restrict__ mprIn, int nReduced, const int* idx, const F4* pts,
+__device__ void sat_finish(int i, int a, int b, bool aStatic, bool bStatic, bool prim, int state, F4 sep,
+						   const F4* __restrict__ mprIn, int nReduced, const int* idx, const F4* pts,
 						   int* __restrict__ flags, b3hContact4Data* __restrict__ staging, int* __restrict__ sepStateOut,
 						   b3hFloat4* __restrict__ sepOut)
 {
@@ -1118,8 +1158,8 @@ __device__ void sat_finish(int i, int a, int b, const b3hRigidBodyData& A, const
 	c.m_restituitionCoeffCmp = (unsigned short)(0.f * 0xffff);
 	c.m_frictionCoeffCmp = (unsigned short)(0.7f * 0xffff);
 	c.m_batchIdx = i;
-	c.m_bodyAPtrAndSignBit = A.m_invMass == 0 ? -a : a;
-	c.m_bodyBPtrAndSignBit = B.m_invMass == 0 ? -b : b;
+	c.m_bodyAPtrAndSignBit = aStatic ? -a : a;
+	c.m_bodyBPtrAndSignBit = bStatic ? -b : b;
 	c.m_childIndexA = -1;
 	c.m_childIndexB = -1;
 	c.m_unused1 = 0;
@@ -1144,19 +1184,14 @@ __device__ void sat_finish(int i, int a, int b, const b3hRigidBodyData& A, const
 	}
 	staging[i] = c;
 }
-__device__ __forceinline__ bool pair_is_prim(const b3hRigidBodyData& A, const b3hRigidBodyData& B, const b3hShapeTables& sh)
-{
-	return sh.collidables[A.m_collidableIdx].m_shapeType != B3H_SHAPE_CONVEX_HULL ||
-		   sh.collidables[B.m_collidableIdx].m_shapeType != B3H_SHAPE_CONVEX_HULL;
-}
 
 /* pass 1: the face families (findSeparatingAxisKernel's faces of A then B).
  * launch bounds: 1024 / THREADS blocks per CU keep the register count at 128,
  * so four waves share each SIMD */
 template <int L, int THREADS>
 __global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_sat_face_pass(
-	const b3hInt4* __restrict__ pairs, const int* __restrict__ numPairsPtr, const b3hRigidBodyData* __restrict__ bodies,
-	b3hShapeTables sh, b3hContact4Data* __restrict__ staging, int* __restrict__ flags, int* __restrict__ sepStateOut,
+	const b3hInt4* __restrict__ pairs, const int* __restrict__ numPairsPtr, b3hShapeTables sh,
+	b3hContact4Data* __restrict__ staging, int* __restrict__ flags, int* __restrict__ sepStateOut,
 	b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn, b3hSatCache sc, float* __restrict__ dminOut,
 	int* __restrict__ openFlag)
 {
@@ -1169,37 +1204,34 @@ __global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_sat_face_pass(
 	{
 		b3hInt4 pr = pairs[i];
 		int a = pr.x, b = pr.y;
-		PairHulls H;
-		H.A = bodies[a];
-		H.B = bodies[b];
+		const b3hSatBody A = sc.desc[a], B = sc.desc[b];
+		const bool hullA = A.flags & B3H_SB_HULL, hullB = B.flags & B3H_SB_HULL;
+		const bool aStatic = A.flags & B3H_SB_STATIC, bStatic = B.flags & B3H_SB_STATIC;
 		F4 mprDH = mprIn[3 * (size_t)i + 2];
 		F4 sep = f4(0, 0, 0, 0);
 		if (mprDH.y != 0.f) sep = neg4(mprIn[3 * (size_t)i]);
-		if (!pair_is_hull_hull(H.A, H.B, sh))
+		if (!(hullA && hullB) || (aStatic && bStatic))
 		{
 			if (g.lane == 0)
 			{
 				openFlag[i] = 0;
-				sat_finish(i, a, b, H.A, H.B, pair_is_prim(H.A, H.B, sh), 0, f4(0, 0, 0, 0), mprIn, 0, nullptr, nullptr,
-						   flags, staging, sepStateOut, sepOut);
+				sat_finish(i, a, b, aStatic, bStatic, !(hullA && hullB), 0, f4(0, 0, 0, 0), mprIn, 0, nullptr,
+						   nullptr, flags, staging, sepStateOut, sepOut);
 			}
 			continue;
 		}
-		H.ha = sh.convexes[sh.collidables[H.A.m_collidableIdx].m_shapeIndex];
-		H.hb = sh.convexes[sh.collidables[H.B.m_collidableIdx].m_shapeIndex];
 		PairGeom P, Q;
-		stage_pair(g, sh, H, sVerts[gi][0], sVerts[gi][1], P);
-		F4 dc = sat_frame(H, P, Q);
+		stage_pair(g, sh, A, B, sVerts[gi][0], sVerts[gi][1], P);
+		sat_frame(P, Q);
+		F4 dc = sub4(A.center, B.center);
 		float dmin = FLT_MAX;
 		F4 sepN = sep;
 		int state = 0;
 		float dminSaved = FLT_MAX;
-		Family fa = {0, H.ha.m_numFaces, sh.faces + H.ha.m_faceOffset, nullptr, nullptr, 1,
-					 sat_records(sc, a, H.ha.m_numFaces), nullptr, nullptr};
+		Family fa = {0, A.nF, sh.faces + A.fOff, nullptr, nullptr, 1, body_records(sc, A), nullptr, nullptr};
 		if (sat_family(g, fa, Q, dc, &sepN, &dmin))
 		{
-			Family fb = {1, H.hb.m_numFaces, sh.faces + H.hb.m_faceOffset, nullptr, nullptr, 1,
-						 sat_records(sc, b, H.hb.m_numFaces), nullptr, nullptr};
+			Family fb = {1, B.nF, sh.faces + B.fOff, nullptr, nullptr, 1, body_records(sc, B), nullptr, nullptr};
 			if (sat_family(g, fb, Q, dc, &sepN, &dmin))
 			{
 				state = 1;
@@ -1213,8 +1245,8 @@ __global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_sat_face_pass(
 		{
 			openFlag[i] = state != 0;
 			if (state == 0)
-				sat_finish(i, a, b, H.A, H.B, false, 0, sep, mprIn, 0, nullptr, nullptr, flags, staging, sepStateOut,
-						   sepOut);
+				sat_finish(i, a, b, aStatic, bStatic, false, 0, sep, mprIn, 0, nullptr, nullptr, flags, staging,
+						   sepStateOut, sepOut);
 			else
 			{
 				sepStateOut[i] = state;
@@ -1229,10 +1261,10 @@ __global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_sat_face_pass(
  * directions, mpr.cl:180-311) of the listed pairs (those the face pass left
  * open), one pair per wavefront; each sets its clip flag (1: clip it) */
 __global__ __launch_bounds__(256, 4) void k_sat_edge_pass(
-	const b3hInt4* __restrict__ pairs, const b3hRigidBodyData* __restrict__ bodies,
-	b3hShapeTables sh, b3hContact4Data* __restrict__ staging, int* __restrict__ flags, int* __restrict__ sepStateOut,
-	b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn, b3hSatCache sc, const float* __restrict__ dminIn,
-	const int* __restrict__ listCount, const int* __restrict__ list, int* __restrict__ clipFlag)
+	const b3hInt4* __restrict__ pairs, b3hShapeTables sh, b3hContact4Data* __restrict__ staging, int* __restrict__ flags,
+	int* __restrict__ sepStateOut, b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn, b3hSatCache sc,
+	const float* __restrict__ dminIn, const int* __restrict__ listCount, const int* __restrict__ list,
+	int* __restrict__ clipFlag)
 {
 	constexpr int L = 64, G = 4;
 	__shared__ F4 sVerts[G][2][NP_VCAP];
@@ -1244,23 +1276,22 @@ __global__ __launch_bounds__(256, 4) void k_sat_edge_pass(
 		int i = list[k];
 		b3hInt4 pr = pairs[i];
 		int a = pr.x, b = pr.y;
-		PairHulls H;
-		load_hulls(sh, bodies, a, b, H);
+		const b3hSatBody A = sc.desc[a], B = sc.desc[b];
 		PairGeom P, Q;
-		stage_pair(g, sh, H, sVerts[gi][0], sVerts[gi][1], P);
-		F4 dc = sat_frame(H, P, Q);
+		stage_pair(g, sh, A, B, sVerts[gi][0], sVerts[gi][1], P);
+		sat_frame(P, Q);
+		F4 dc = sub4(A.center, B.center);
 		F4 sep = sepOut[i];
 		float dm = dminIn[i];
 		F4 s2 = sep;
-		int nEE = H.ha.m_numUniqueEdges * H.hb.m_numUniqueEdges;
+		int nEE = A.nE * B.nE;
 		Family fe;
 		if (nEE <= B3_UNIT_SPHERE_162_COUNT)
-			fe = {2, nEE, nullptr, sh.uniqueEdges + H.ha.m_uniqueEdgesOffset, sh.uniqueEdges + H.hb.m_uniqueEdgesOffset,
-				  H.hb.m_numUniqueEdges, nullptr, nullptr, nullptr};
+			fe = {2, nEE, nullptr, sh.uniqueEdges + A.eOff, sh.uniqueEdges + B.eOff, B.nE, nullptr, nullptr, nullptr};
 		else
 		{
-			const float2* dA = sat_dirs(sc, a);
-			const float2* dB = sat_dirs(sc, b);
+			const float2* dA = body_dirs(sc, A);
+			const float2* dB = body_dirs(sc, B);
 			fe = {3, B3_UNIT_SPHERE_162_COUNT, nullptr, nullptr, nullptr, 1, nullptr, dB ? dA : nullptr, dB};
 		}
 		bool open = sat_family(g, fe, Q, dc, &s2, &dm);
@@ -1269,8 +1300,8 @@ __global__ __launch_bounds__(256, 4) void k_sat_edge_pass(
 			if (!open)
 			{
 				clipFlag[i] = 0;
-				sat_finish(i, a, b, H.A, H.B, false, 0, sep, mprIn, 0, nullptr, nullptr, flags, staging, sepStateOut,
-						   sepOut);
+				sat_finish(i, a, b, A.flags & B3H_SB_STATIC, B.flags & B3H_SB_STATIC, false, 0, sep, mprIn, 0, nullptr,
+						   nullptr, flags, staging, sepStateOut, sepOut);
 			}
 			else
 			{
@@ -1298,9 +1329,8 @@ __global__ __launch_bounds__(256) void k_pair_list(const int* __restrict__ numPa
 template <int L, int THREADS>
 __global__ __launch_bounds__(THREADS, 3) void k_sat_clip_pass(
 	const b3hInt4* __restrict__ pairs, const int* __restrict__ clipCount, const int* __restrict__ clipList,
-	const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh, b3hContact4Data* __restrict__ staging,
-	int* __restrict__ flags, int* __restrict__ sepStateOut, b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn,
-	b3hSatCache sc)
+	b3hShapeTables sh, b3hContact4Data* __restrict__ staging, int* __restrict__ flags, int* __restrict__ sepStateOut,
+	b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn, b3hSatCache sc)
 {
 	constexpr int G = THREADS / L;
 	__shared__ F4 sClip[G][3][CLIP_CAP];
@@ -1312,29 +1342,29 @@ __global__ __launch_bounds__(THREADS, 3) void k_sat_clip_pass(
 		int i = clipList[k];
 		b3hInt4 pr = pairs[i];
 		int a = pr.x, b = pr.y;
-		PairHulls H;
-		load_hulls(sh, bodies, a, b, H);
+		const b3hSatBody A = sc.desc[a], B = sc.desc[b];
 		/* clipping reads a few vertices of two faces: from the (cached) table */
 		PairGeom P;
-		P.vA = sh.vertices + H.ha.m_vertexOffset;
-		P.vB = sh.vertices + H.hb.m_vertexOffset;
+		P.vA = sh.vertices + A.vOff;
+		P.vB = sh.vertices + B.vOff;
 		P.ldsA = P.ldsB = false;
-		P.nA = H.ha.m_numVertices;
-		P.nB = H.hb.m_numVertices;
-		P.posA = H.A.m_pos;
-		P.posB = H.B.m_pos;
-		P.ornA = H.A.m_quat;
-		P.ornB = H.B.m_quat;
+		P.nA = A.nV;
+		P.nB = B.nV;
+		P.posA = A.pos;
+		P.posB = B.pos;
+		P.ornA = A.quat;
+		P.ornB = B.quat;
 		wave_sync(); /* the previous pair's clip buffers are read */
 		F4 sep = sepOut[i];
 		F4* pts = sClip[gi][2];
 		int idx[4] = {0, 0, 0, 0};
 		int nReduced = 0;
-		int np = clip_hull_hull(g, sep, P, H.ha, H.hb, sh, sClip[gi][0], sClip[gi][1], pts,
-								sat_records(sc, a, H.ha.m_numFaces), sat_records(sc, b, H.hb.m_numFaces));
+		int np = clip_hull_hull(g, sep, P, A, B, sh, sClip[gi][0], sClip[gi][1], pts, body_records(sc, A),
+								body_records(sc, B));
 		if (np > 0) nReduced = extract_manifold(g, pts, np, neg4(sep), idx);
 		if (g.lane == 0)
-			sat_finish(i, a, b, H.A, H.B, false, 1, sep, mprIn, nReduced, idx, pts, flags, staging, sepStateOut, sepOut);
+			sat_finish(i, a, b, A.flags & B3H_SB_STATIC, B.flags & B3H_SB_STATIC, false, 1, sep, mprIn, nReduced, idx,
+					   pts, flags, staging, sepStateOut, sepOut);
 	}
 }
 
@@ -1726,19 +1756,24 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 		hipLaunchKernelGGL(k_np_mpr, dim3(mb), dim3(256), 0, s, pairs, numPairs, bodies, sh, (F4*)mprScratch);
 	}
 	if (evAfterMpr) hipEventRecord(evAfterMpr, s);
-	b3hSatCache sc = {nullptr, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, 0};
-	if (satCache && satCache->numBodies > 0 && satCache->total > 0)
+	b3hSatCache sc = *satCache;
+	if (sc.numBodies > 0)
 	{
-		sc = *satCache;
+		int nb = (sc.numBodies + 255) / 256;
+		if (nb > 16384) nb = 16384;
+		hipLaunchKernelGGL(k_sat_bodies, dim3(nb), dim3(256), 0, s, bodies, sh, sc);
+	}
+	if (sc.recs && sc.total > 0)
+	{
 		int fb = (sc.total + 255) / 256;
 		if (fb > 16384) fb = 16384;
-		hipLaunchKernelGGL(k_sat_faces, dim3(fb), dim3(256), 0, s, bodies, sh, sc);
-		if (sc.numDirBodies > 0 && sc.dirs)
-		{
-			int db = (sc.numDirBodies * B3_UNIT_SPHERE_162_COUNT + 255) / 256;
-			if (db > 16384) db = 16384;
-			hipLaunchKernelGGL(k_sat_dirs, dim3(db), dim3(256), 0, s, bodies, sh, sc);
-		}
+		hipLaunchKernelGGL(k_sat_faces, dim3(fb), dim3(256), 0, s, sh, sc);
+	}
+	if (sc.dirs && sc.numDirBodies > 0)
+	{
+		int db = (sc.numDirBodies * B3_UNIT_SPHERE_162_COUNT + SD_TILE - 1) / SD_TILE;
+		if (db > 16384) db = 16384;
+		hipLaunchKernelGGL(k_sat_dirs, dim3(db), dim3(SD_TILE), 0, s, sh, sc);
 	}
 	/* grid: four times the blocks the CUs hold at four waves per SIMD.  The
 	 * blocks loop over pairs, so a larger grid only adds block start-ups, each
@@ -1755,7 +1790,7 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 	/* pass 1, faces: lanes per pair 64 (one pair per wavefront), 32 or 16 */
 	auto facePass = [&](auto kern, int lanes, int threads) {
 		hipLaunchKernelGGL(kern, dim3(blocksFor(maxPairs, threads / lanes, threads)), dim3(threads), 0, s, pairs,
-						   numPairs, bodies, sh, staging, flags, sepState, sepNormal, (const F4*)mprScratch, sc, work.dmin,
+						   numPairs, sh, staging, flags, sepState, sepNormal, (const F4*)mprScratch, sc, work.dmin,
 						   work.flag);
 	};
 	if (pairsPerGroup == 16)
@@ -1768,15 +1803,15 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 	int lb = blocksFor(maxPairs, 256, 256);
 	b3h_exclusive_scan(work.flag, work.scan, numPairs, maxPairs, scanTmp, s);
 	hipLaunchKernelGGL(k_pair_list, dim3(lb), dim3(256), 0, s, numPairs, work.flag, work.scan, work.list, work.count);
-	hipLaunchKernelGGL(k_sat_edge_pass, dim3(blocksFor(maxPairs, 4, 256)), dim3(256), 0, s, pairs, bodies, sh, staging,
-					   flags, sepState, sepNormal, (const F4*)mprScratch, sc, (const float*)work.dmin,
-					   (const int*)work.count, (const int*)work.list, work.flag);
+	hipLaunchKernelGGL(k_sat_edge_pass, dim3(blocksFor(maxPairs, 4, 256)), dim3(256), 0, s, pairs, sh, staging, flags,
+					   sepState, sepNormal, (const F4*)mprScratch, sc, (const float*)work.dmin, (const int*)work.count,
+					   (const int*)work.list, work.flag);
 	/* pass 3, clipping, over the pairs pass 2 flagged (the flags of pairs it
 	 * did not visit are still 0 from pass 1) */
 	b3h_exclusive_scan(work.flag, work.scan, numPairs, maxPairs, scanTmp, s);
 	hipLaunchKernelGGL(k_pair_list, dim3(lb), dim3(256), 0, s, numPairs, work.flag, work.scan, work.list, work.count);
 	hipLaunchKernelGGL((k_sat_clip_pass<16, 256>), dim3(blocksFor(maxPairs, 16, 256)), dim3(256), 0, s, pairs,
-					   (const int*)work.count, (const int*)work.list, bodies, sh, staging, flags, sepState, sepNormal,
+					   (const int*)work.count, (const int*)work.list, sh, staging, flags, sepState, sepNormal,
 					   (const F4*)mprScratch, sc);
 	if (hasPrimitives)
 	{
