@@ -72,6 +72,9 @@ _SIGS = {
     "b3h_np_register_convex_hull_shape": ([_vp, _vp, _i, _i, _vp, _pi], _i),
     "b3h_np_register_sphere_shape": ([_vp, _f, _pi], _i),
     "b3h_np_register_plane_shape": ([_vp, _vp, _f, _pi], _i),
+    "b3h_np_register_compound_shape": ([_vp, _vp, _i, _pi], _i),
+    "b3h_np_get_compound_tables": ([_vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "b3h_compute_compound_tables": ([_vp, _i, _vp, _vp, _vp, _vp, _i, _pi, _vp, _i, _pi], _i),
     "b3h_np_register_convex_polyhedron": ([_vp, ctypes.POINTER(ConvexUtility), _pi], _i),
     "b3h_np_load_shape_tables": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "b3h_np_get_mpr_debug": ([_vp, _pvp], _i),

@@ -34,11 +34,19 @@ CONSTRAINT_JOINT = np.dtype([("constraintType", np.int32), ("rbA", np.int32), ("
 RAYINFO = np.dtype([("from", F4), ("to", F4)])
 RAYHIT = np.dtype([("hitFraction", np.float32), ("hitBody", np.int32), ("hitResult1", np.int32),
                    ("hitResult2", np.int32), ("hitPoint", F4), ("hitNormal", F4)])
+CHILD_SHAPE = np.dtype([("childPosition", F4), ("childOrientation", F4), ("shapeIndex", np.int32),
+                        ("numChildShapes", np.int32), ("collidableShapeIndex", np.int32), ("shapeType", np.int32)])
+BVH_NODE = np.dtype([("qmin", np.uint16, (3,)), ("qmax", np.uint16, (3,)), ("escapeOrIndex", np.int32)])
+BVH_SUBTREE = np.dtype([("qmin", np.uint16, (3,)), ("qmax", np.uint16, (3,)), ("rootNodeIndex", np.int32),
+                        ("subtreeSize", np.int32), ("padding", np.int32, (3,))])
+BVH_INFO = np.dtype([("aabbMin", F4), ("aabbMax", F4), ("quantization", F4), ("numNodes", np.int32),
+                     ("numSubTrees", np.int32), ("nodeOffset", np.int32), ("subTreeOffset", np.int32)])
 
 assert BODY.itemsize == 80 and INERTIA.itemsize == 96 and COLLIDABLE.itemsize == 16
 assert CONVEX.itemsize == 96 and FACE.itemsize == 32 and AABB.itemsize == 32
 assert CONTACT.itemsize == 112 and CONSTRAINT.itemsize == 176 and PAIR.itemsize == 16
 assert RAYINFO.itemsize == 32 and RAYHIT.itemsize == 48 and CONSTRAINT_JOINT.itemsize == 80
+assert CHILD_SHAPE.itemsize == 48 and BVH_NODE.itemsize == 16 and BVH_SUBTREE.itemsize == 32 and BVH_INFO.itemsize == 64
 
 
 def aabb_handle(aabbs):

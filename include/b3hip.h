@@ -77,6 +77,18 @@ int b3h_np_register_convex_polyhedron(b3hNarrowPhase* np, const b3hConvexUtility
  * plane-sphere, sphere-hull, sphere-sphere). */
 int b3h_np_register_sphere_shape(b3hNarrowPhase* np, float radius, int* outCollidableIndex);
 int b3h_np_register_plane_shape(b3hNarrowPhase* np, const float* normal3, float planeConstant, int* outCollidableIndex);
+/* b3GpuNarrowPhase::registerCompoundShape(b3AlignedObjectArray<b3GpuChildShape>*),
+ * b3GpuNarrowPhase.cpp:370-517 (b3GpuNarrowPhase.h:33): a compound of
+ * registered convex hulls (children[i].m_shapeIndex: the child's collidable)
+ * placed by child transforms.  Its local AABB bounds the transformed child
+ * AABBs; a quantized AABB tree over the children serves the compound-compound
+ * pair search (findCompoundPairsKernel, sat.cl:945-1130).  Compound pairs get
+ * their contacts from the compound SAT and clipping (processCompoundPairsKernel,
+ * sat.cl:802-925; clipCompoundsHullHullKernel, satClipHullContacts.cl:1003-1130)
+ * and, against a plane or a sphere, from the primitive routines
+ * (processCompoundPairsPrimitivesKernel, primitiveContacts.cl:980-1090). */
+int b3h_np_register_compound_shape(b3hNarrowPhase* np, const b3hGpuChildShape* children, int numChildren,
+								   int* outCollidableIndex);
 /* b3GpuNarrowPhase::registerRigidBody(collidableIndex, mass, position, orientation,
  * aabbMin, aabbMax, writeToGpu), b3GpuNarrowPhase.cpp:816-908 */
 int b3h_np_register_rigid_body(b3hNarrowPhase* np, int collidableIndex, float mass, const float* position3,
@@ -115,6 +127,11 @@ int b3h_np_get_local_space_aabb(b3hNarrowPhase* np, int collidableIndex, b3hAabb
 int b3h_np_get_shape_tables(b3hNarrowPhase* np, int* counts7, b3hCollidable* collidables,
 							b3hConvexPolyhedronData* convexes, b3hFloat4* vertices, b3hFloat4* uniqueEdges,
 							b3hGpuFace* faces, int* indices, b3hAabb* localAabbs);
+/* Host copies of the compound tables (children, child-tree records, tree
+ * nodes, subtree headers): counts[4] are written; arrays are filled when
+ * non-NULL.  The tables of registerCompoundShape, b3GpuNarrowPhase.cpp:381-513. */
+int b3h_np_get_compound_tables(b3hNarrowPhase* np, int* counts4, b3hGpuChildShape* children, b3hBvhInfo* infos,
+							   b3hQuantizedBvhNode* nodes, b3hBvhSubtreeInfo* subtrees);
 /* Replace the shape tables wholesale (same arrays and counts as
  * b3h_np_get_shape_tables; counts7[6] must equal counts7[0]).  Every hull's
  * offsets and face indices are checked against the tables.  No counterpart in
@@ -461,6 +478,16 @@ int b3h_compute_polyhedral_features(const float* vertices, int strideInBytes, in
 int b3h_compute_body_registration(const float* localAabbMin3, const float* localAabbMax3, float mass,
 								  const float* position3, const float* orientation4, float* aabbMin3, float* aabbMax3,
 								  b3hInertiaData* inertia);
+/* The tables registerCompoundShape computes for a compound (b3GpuNarrowPhase.cpp:
+ * 391-513: the local AABB over the transformed child AABBs, and the quantized
+ * child tree of b3QuantizedBvh), on the host, without registering it:
+ * childLocalAabbs are the children's local AABBs; the tree record's node and
+ * subtree offsets are 0; 2 * numChildren nodes are written (the last one is
+ * unused).  The registration entry uses it. */
+int b3h_compute_compound_tables(const b3hGpuChildShape* children, int numChildren, const b3hAabb* childLocalAabbs,
+								b3hAabb* aabbOut, b3hBvhInfo* infoOut, b3hQuantizedBvhNode* nodesOut, int nodesCapacity,
+								int* numNodesOut, b3hBvhSubtreeInfo* subtreesOut, int subtreesCapacity,
+								int* numSubtreesOut);
 
 /* ---------------- device memory helpers for bindings ---------------- */
 int b3h_device_count(int* n);

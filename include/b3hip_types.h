@@ -73,11 +73,56 @@ enum b3hShapeTypes
 
 typedef struct b3hCollidable
 {
-	int m_numChildShapes;
-	float m_radius;
+	int m_numChildShapes; /* compound: its children */
+	union
+	{
+		float m_radius;         /* sphere radius, plane constant */
+		int m_compoundBvhIndex; /* compound: its b3hBvhInfo */
+	};
 	int m_shapeType;
-	int m_shapeIndex;
+	int m_shapeIndex; /* hull: convex index; plane: face; compound: first child */
 } b3hCollidable;
+
+/* b3GpuChildShape, shared/b3Collidable.h:40-58: a compound's child (a
+ * registered convex-hull collidable placed by a child transform) */
+typedef struct b3hGpuChildShape
+{
+	b3hFloat4 m_childPosition;
+	b3hFloat4 m_childOrientation;
+	int m_shapeIndex; /* the child's collidable */
+	int m_numChildShapes;
+	int m_collidableShapeIndex;
+	int m_shapeType;
+} b3hGpuChildShape;
+
+/* the quantized AABB tree over a compound's children (b3QuantizedBvhNodeData,
+ * b3BvhSubtreeInfoData, b3BvhInfo: src/Bullet3Collision/NarrowPhaseCollision/
+ * shared/b3QuantizedBvhNodeData.h:14-21, shared/b3BvhSubtreeInfoData.h:7-17,
+ * src/Bullet3OpenCL/NarrowphaseCollision/b3BvhInfo.h:6-15) */
+typedef struct b3hQuantizedBvhNode
+{
+	unsigned short m_quantizedAabbMin[3];
+	unsigned short m_quantizedAabbMax[3];
+	int m_escapeIndexOrTriangleIndex; /* leaf: child index (>= 0); internal: -(subtree size) */
+} b3hQuantizedBvhNode;
+typedef struct b3hBvhSubtreeInfo
+{
+	unsigned short m_quantizedAabbMin[3];
+	unsigned short m_quantizedAabbMax[3];
+	int m_rootNodeIndex;
+	int m_subtreeSize;
+	int m_padding[3];
+} b3hBvhSubtreeInfo;
+typedef struct b3hBvhInfo
+{
+	b3hFloat4 m_aabbMin;
+	b3hFloat4 m_aabbMax;
+	b3hFloat4 m_quantization;
+	int m_numNodes;
+	int m_numSubTrees;
+	int m_nodeOffset;
+	int m_subTreeOffset;
+} b3hBvhInfo;
 
 typedef struct b3hGpuFace
 {

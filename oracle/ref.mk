@@ -1,6 +1,6 @@
 # Builds oracle/_ref/libb3ref.so from the reference's own sources where they
 # lie under /root/reference (nothing is copied): Bullet3Common, the hull
-# computer, b3ConvexUtility and the dual-compile shared headers, driven by
+# computer, b3ConvexUtility, b3QuantizedBvh and the dual-compile shared headers, driven by
 # oracle/ref_harness.cpp.  Test infrastructure only.
 REF ?= /root/reference/src
 CXX ?= g++
@@ -10,7 +10,8 @@ SRCS = ref_harness.cpp \
 	$(REF)/Bullet3Common/b3Logging.cpp \
 	$(REF)/Bullet3Common/b3Vector3.cpp \
 	$(REF)/Bullet3Geometry/b3ConvexHullComputer.cpp \
-	$(REF)/Bullet3Collision/NarrowPhaseCollision/b3ConvexUtility.cpp
+	$(REF)/Bullet3Collision/NarrowPhaseCollision/b3ConvexUtility.cpp \
+	$(REF)/Bullet3OpenCL/NarrowphaseCollision/b3QuantizedBvh.cpp
 
 all: _ref/libb3ref.so _ref/libb3refsolver.so
 

@@ -10,6 +10,7 @@
  *   b3ConvexUtility::initializePolyhedralFeatures  src/Bullet3Collision/NarrowPhaseCollision/b3ConvexUtility.cpp:26-248
  *   registerRigidBody's inertia  src/Bullet3OpenCL/RigidBody/b3GpuNarrowPhase.cpp:860-901, evaluated with
  *                         the reference's b3Vector3 / b3Matrix3x3 / b3Quaternion classes
+ *   registerCompoundShape's local AABB and b3QuantizedBvh  src/Bullet3OpenCL/RigidBody/b3GpuNarrowPhase.cpp:370-517
  * The headers are the reference's dual-compile ("shared") headers, built here
  * as the C++ they are written to be.
  */
@@ -27,6 +28,7 @@
 #include "Bullet3Collision/NarrowPhaseCollision/b3ConvexUtility.h"
 #include "Bullet3Common/b3Transform.h"
 #include "Bullet3Geometry/b3AabbUtil.h"
+#include "Bullet3OpenCL/NarrowphaseCollision/b3QuantizedBvh.h"
 #include <string.h>
 
 extern "C" {
@@ -177,5 +179,71 @@ void ref_instance_aabb(const float* lmin, const float* lmax, const float* pos, c
 		mn[k] = aabbMin[k];
 		mx[k] = aabbMax[k];
 	}
+}
+
+/* registerCompoundShape (b3GpuNarrowPhase.cpp:370-517) for n children:
+ * childPos/childOrn (4 floats each), childMin/childMax the children's local
+ * AABBs (4 floats each).  Out: the compound AABB (min, max: 3 floats each),
+ * the tree's aabbMin, aabbMax, quantization (3 floats each), its 2n nodes (16
+ * bytes each; the last one is never written by the build and is zeroed here)
+ * and its subtree headers (32 bytes each, at most subCap). */
+int ref_compound_bvh(int n, const float* childPos, const float* childOrn, const float* childMin, const float* childMax,
+					 float* aabbMin, float* aabbMax, float* bvhMin, float* bvhMax, float* bvhQuant, void* nodesOut,
+					 void* subtreesOut, int subCap)
+{
+	b3Vector3 myAabbMin = b3MakeVector3(1e30f, 1e30f, 1e30f);
+	b3Vector3 myAabbMax = b3MakeVector3(-1e30f, -1e30f, -1e30f);
+	b3AlignedObjectArray<b3Vector3> cmin, cmax;
+	cmin.resize(n);
+	cmax.resize(n);
+	for (int i = 0; i < n; i++)
+	{
+		b3Vector3 lmin = b3MakeVector3(childMin[4 * i], childMin[4 * i + 1], childMin[4 * i + 2]);
+		b3Vector3 lmax = b3MakeVector3(childMax[4 * i], childMax[4 * i + 1], childMax[4 * i + 2]);
+		b3Transform childTr;
+		childTr.setIdentity();
+		childTr.setOrigin(b3MakeVector3(childPos[4 * i], childPos[4 * i + 1], childPos[4 * i + 2]));
+		childTr.setRotation(b3Quaternion(childOrn[4 * i], childOrn[4 * i + 1], childOrn[4 * i + 2], childOrn[4 * i + 3]));
+		b3Vector3 aMin, aMax;
+		b3Scalar margin(0.f);
+		b3TransformAabb(lmin, lmax, margin, childTr, aMin, aMax);
+		myAabbMin.setMin(aMin);
+		myAabbMax.setMax(aMax);
+		cmin[i] = aMin;
+		cmax[i] = aMax;
+	}
+	for (int k = 0; k < 3; k++)
+	{
+		aabbMin[k] = myAabbMin[k];
+		aabbMax[k] = myAabbMax[k];
+	}
+	b3QuantizedBvh* bvh = new b3QuantizedBvh;
+	bvh->setQuantizationValues(myAabbMin, myAabbMax);
+	QuantizedNodeArray& nodes = bvh->getLeafNodeArray();
+	for (int i = 0; i < n; i++)
+	{
+		b3QuantizedBvhNode node;
+		bvh->quantize(&node.m_quantizedAabbMin[0], cmin[i], 0);
+		bvh->quantize(&node.m_quantizedAabbMax[0], cmax[i], 1);
+		int partId = 0;
+		node.m_escapeIndexOrTriangleIndex = (partId << (31 - MAX_NUM_PARTS_IN_BITS)) | i;
+		nodes.push_back(node);
+	}
+	int numLeaves = n;
+	bvh->buildInternal();
+	for (int k = 0; k < 3; k++)
+	{
+		bvhMin[k] = bvh->m_bvhAabbMin[k];
+		bvhMax[k] = bvh->m_bvhAabbMax[k];
+		bvhQuant[k] = bvh->m_bvhQuantization[k];
+	}
+	const QuantizedNodeArray& all = bvh->getQuantizedNodeArray();
+	memset(nodesOut, 0, sizeof(b3QuantizedBvhNode) * 2 * (size_t)numLeaves);
+	memcpy(nodesOut, &all[0], sizeof(b3QuantizedBvhNode) * (2 * (size_t)numLeaves - 1));
+	const BvhSubtreeInfoArray& st = bvh->getSubtreeInfoArray();
+	int ns = st.size();
+	for (int i = 0; i < ns && i < subCap; i++) memcpy((char*)subtreesOut + 32 * i, &st[i], 32);
+	delete bvh;
+	return ns;
 }
 }
