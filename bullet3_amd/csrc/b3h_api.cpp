@@ -165,6 +165,8 @@ struct b3hNarrowPhase
 	int* dSepState;
 	b3hFloat4* dSepNormal;
 	b3hFloat4* dMpr; /* per pair: MPR direction, point, (depth, hit) */
+	/* compound child pairs (allocated once a compound is registered) */
+	b3hCompoundWork cw;
 	/* per-pair buffers between the SAT passes */
 	float* dSatDmin;
 	int *dSatFlag, *dSatScan, *dSatList, *dSatCount;
@@ -246,6 +248,7 @@ extern "C" int b3h_np_create(const b3hConfig* cfg, void* stream, b3hNarrowPhase*
 	np->dSepState = nullptr;
 	np->dSepNormal = nullptr;
 	np->dMpr = nullptr;
+	memset(&np->cw, 0, sizeof(np->cw));
 	np->dSatDmin = nullptr;
 	np->dSatFlag = np->dSatScan = np->dSatList = np->dSatCount = nullptr;
 	np->dSatRecs = nullptr;
@@ -299,6 +302,13 @@ extern "C" int b3h_np_destroy(b3hNarrowPhase* np)
 	dfree(np->dSepState);
 	dfree(np->dSepNormal);
 	dfree(np->dMpr);
+	dfree(np->cw.counts);
+	dfree(np->cw.offsets);
+	dfree(np->cw.cpairs);
+	dfree(np->cw.num);
+	dfree(np->cw.flags);
+	dfree(np->cw.scan);
+	dfree(np->cw.scanTmp);
 	dfree(np->dSatDmin);
 	dfree(np->dSatFlag);
 	dfree(np->dSatScan);
@@ -856,8 +866,36 @@ static b3hShapeTables np_tables(b3hNarrowPhase* np)
 /* Record ranges of the per-body face records (b3hSatCache).  Called outside
  * graph capture before every narrowphase entry; the ranges are rebuilt only
  * when a body's collidable or the shape tables changed. */
+/* the compound buffers, once a compound shape exists: child-pair lists and
+ * a staging area for their contacts after the pairs' records */
+static int np_ensure_compound(b3hNarrowPhase* np)
+{
+	if (np->cw.cap > 0) return B3H_OK;
+	bool any = false;
+	for (const b3hCollidable& c : np->cols) any = any || c.m_shapeType == B3H_SHAPE_COMPOUND_OF_CONVEX_HULLS;
+	if (!any) return B3H_OK;
+	int mp = np->cfg.m_maxBroadphasePairs;
+	int cap = np->cfg.m_compoundPairCapacity > 0 ? np->cfg.m_compoundPairCapacity : 1;
+	B3H_CHECK(hipStreamSynchronize(np->stream));
+	b3hCompoundWork& w = np->cw;
+	int r;
+	size_t tmp = b3h_scan_tmp_size(mp > cap ? mp : cap);
+	if ((r = dalloc(&w.counts, (size_t)mp + 1)) || (r = dalloc(&w.offsets, (size_t)mp + 2)) ||
+		(r = dalloc(&w.cpairs, (size_t)cap)) || (r = dalloc(&w.num, 2)) || (r = dalloc(&w.flags, (size_t)cap + 1)) ||
+		(r = dalloc(&w.scan, (size_t)cap + 2)) || (r = dalloc(&w.scanTmp, tmp)))
+		return r;
+	B3H_CHECK(hipMemset(w.num, 0, 2 * sizeof(int)));
+	dfree(np->dStaging);
+	if ((r = dalloc(&np->dStaging, (size_t)mp + (size_t)cap))) return r;
+	w.stagingBase = mp;
+	w.cap = cap;
+	np->gen++;
+	return B3H_OK;
+}
+
 static int np_ensure_sat(b3hNarrowPhase* np)
 {
+	if (int rc = np_ensure_compound(np)) return rc;
 	if (np->satValid && np->satGen == np->gen) return B3H_OK;
 	int nb = (int)np->bodies.size();
 	bool same = np->satValid && (int)np->satCols.size() == nb && np->satNumConvex == np->convexes.size();
@@ -979,7 +1017,7 @@ static int np_compute_contacts_dev(b3hNarrowPhase* np, b3hInt4* pairs, const int
 	b3h_narrowphase(pairs, dNumPairs, maxPairs, np->dBodies, np_tables(np), np->dStaging, np->dFlags, np->dFlagScan,
 					np->dScanTmp, np->dContactPair, np->cfg.m_maxContactCapacity, np->dNumContacts, np->dSepState,
 					np->dSepNormal, np->lanesPerPair, np->stream, evAfterKernel, np->dMpr, evAfterMpr, prim, &sc,
-					b3hSatWork{np->dSatDmin, np->dSatFlag, np->dSatScan, np->dSatList, np->dSatCount});
+					b3hSatWork{np->dSatDmin, np->dSatFlag, np->dSatScan, np->dSatList, np->dSatCount}, np->cw);
 	np->contactsValid = false;
 	return B3H_OK;
 }

@@ -164,6 +164,20 @@ struct b3hSatWork
 	int* list;   /* the flagged pairs in pair order */
 	int* count;  /* their number */
 };
+/* compound child pairs (enabled when a compound shape is registered): the
+ * child pairs go to cpairs, their contacts to staging[stagingBase + k] */
+struct b3hCompoundWork
+{
+	int cap;             /* child pairs kept (0: no compounds) */
+	int stagingBase;     /* first staging record of the child pairs */
+	int* counts;         /* per pair: its child pairs (maxPairs + 1) */
+	int* offsets;        /* exclusive prefix sum of counts (maxPairs + 2) */
+	b3hInt4* cpairs;     /* cap */
+	int* num;            /* [0] child pairs kept, [1] found */
+	int* flags;          /* cap + 1 */
+	int* scan;           /* cap + 2 */
+	unsigned* scanTmp;   /* b3h_scan_tmp_size(max(maxPairs, cap)) */
+};
 /* sepState and sepNormal (per pair) are required: they carry the SAT state
  * between the passes */
 void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3hRigidBodyData* bodies,
@@ -171,7 +185,7 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 					 unsigned* scanTmp, int* contactPair, int maxContacts, int* numContacts, int* sepState,
 					 b3hFloat4* sepNormal, int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel,
 					 b3hFloat4* mprScratch, hipEvent_t evAfterMpr, bool hasPrimitives, const b3hSatCache* satCache,
-					 const b3hSatWork& work);
+					 const b3hSatWork& work, const b3hCompoundWork& cw);
 void b3h_gather_contacts(const int* contactPair, const int* numContacts, int maxContacts,
 						 const b3hContact4Data* staging, b3hContact4Data* out, hipStream_t s);
 

@@ -61,6 +61,41 @@ B3H_HD F4 qinv(F4 q) { return f4(-q.x, -q.y, -q.z, q.w); }
 B3H_HD F4 qrot(F4 q, F4 v) { return qmul(qmul(q, xyz0(v)), qinv(q)); }
 B3H_HD F4 qinvrot(F4 q, F4 v) { return qrot(qinv(q), v); }
 B3H_HD F4 xform(F4 p, F4 pos, F4 q) { return add4(qrot(q, p), pos); }
+/* b3TransformAabb2 (shared/b3Aabb.h:24-43) with margin 0 and the rotation
+ * matrix of b3QuatGetRotationMatrix (shared/b3Mat3x3.h:41-61) */
+B3H_HD void transform_aabb2(F4 lmin, F4 lmax, F4 pos, F4 q, F4* mn, F4* mx)
+{
+	F4 half = smul4(0.5f, sub4(lmax, lmin));
+	half = add4(half, f4(0.f, 0.f, 0.f, 0.f));
+	F4 center = smul4(0.5f, add4(lmax, lmin));
+	F4 q2 = f4(q.x * q.x, q.y * q.y, q.z * q.z, 0.f);
+	F4 r0, r1, r2;
+	r0.x = 1 - 2 * q2.y - 2 * q2.z;
+	r0.y = 2 * q.x * q.y - 2 * q.w * q.z;
+	r0.z = 2 * q.x * q.z + 2 * q.w * q.y;
+	r1.x = 2 * q.x * q.y + 2 * q.w * q.z;
+	r1.y = 1 - 2 * q2.x - 2 * q2.z;
+	r1.z = 2 * q.y * q.z - 2 * q.w * q.x;
+	r2.x = 2 * q.x * q.z - 2 * q.w * q.y;
+	r2.y = 2 * q.y * q.z + 2 * q.w * q.x;
+	r2.z = 1 - 2 * q2.x - 2 * q2.y;
+	F4 a0 = f4(fabsf(r0.x), fabsf(r0.y), fabsf(r0.z), 0.f);
+	F4 a1 = f4(fabsf(r1.x), fabsf(r1.y), fabsf(r1.z), 0.f);
+	F4 a2 = f4(fabsf(r2.x), fabsf(r2.y), fabsf(r2.z), 0.f);
+	F4 c = xform(center, pos, q);
+	F4 ext = f4(dot3(half, a0), dot3(half, a1), dot3(half, a2), 0.f);
+	*mn = sub4(c, ext);
+	*mx = add4(c, ext);
+}
+/* b3TestAabbAgainstAabb (shared/b3Aabb.h:46-54) */
+B3H_HD bool aabb_overlap(F4 mn1, F4 mx1, F4 mn2, F4 mx2)
+{
+	bool overlap = true;
+	overlap = (mn1.x > mx2.x || mx1.x < mn2.x) ? false : overlap;
+	overlap = (mn1.z > mx2.z || mx1.z < mn2.z) ? false : overlap;
+	overlap = (mn1.y > mx2.y || mx1.y < mn2.y) ? false : overlap;
+	return overlap;
+}
 B3H_HD F4 nrm3(F4 v)
 {
 	float l2 = dot3(v, v);

@@ -27,28 +27,11 @@ __global__ __launch_bounds__(256) void k_world_aabbs(int n, const b3hRigidBodyDa
 	int ci = b.m_collidableIdx;
 	if (cols[ci].m_shapeIndex < 0) return;
 	b3hAabb la = localAabbs[ci];
-	F4 half = smul4(0.5f, sub4(la.m_max, la.m_min));
-	half = add4(half, f4(0.f, 0.f, 0.f, 0.f));
-	F4 center = smul4(0.5f, add4(la.m_max, la.m_min));
-	F4 q2 = f4(q.x * q.x, q.y * q.y, q.z * q.z, 0.f);
-	F4 r0, r1, r2;
-	r0.x = 1 - 2 * q2.y - 2 * q2.z;
-	r0.y = 2 * q.x * q.y - 2 * q.w * q.z;
-	r0.z = 2 * q.x * q.z + 2 * q.w * q.y;
-	r1.x = 2 * q.x * q.y + 2 * q.w * q.z;
-	r1.y = 1 - 2 * q2.x - 2 * q2.z;
-	r1.z = 2 * q.y * q.z - 2 * q.w * q.x;
-	r2.x = 2 * q.x * q.z - 2 * q.w * q.y;
-	r2.y = 2 * q.y * q.z + 2 * q.w * q.x;
-	r2.z = 1 - 2 * q2.x - 2 * q2.y;
-	F4 a0 = f4(fabsf(r0.x), fabsf(r0.y), fabsf(r0.z), 0.f);
-	F4 a1 = f4(fabsf(r1.x), fabsf(r1.y), fabsf(r1.z), 0.f);
-	F4 a2 = f4(fabsf(r2.x), fabsf(r2.y), fabsf(r2.z), 0.f);
-	F4 c = xform(center, pos, q);
-	F4 ext = f4(dot3(half, a0), dot3(half, a1), dot3(half, a2), 0.f);
+	F4 mn, mx;
+	transform_aabb2(la.m_min, la.m_max, pos, q, &mn, &mx);
 	b3hAabb w;
-	w.m_min = sub4(c, ext);
-	w.m_max = add4(c, ext);
+	w.m_min = mn;
+	w.m_max = mx;
 	w.m_min.w = i2f(i);
 	w.m_max.w = i2f(b.m_invMass == 0.f ? 0 : 1);
 	world[i] = w;

@@ -1444,12 +1444,18 @@ __device__ int pc_extract_manifold(const PT& P, int n, F4 nearNormal, int* idx)
  * reduced to four; contact bodies (plane, hull).  The kept points are held
  * as vertex numbers (an LDS column of the thread, stride PC_THREADS) and
  * recomputed where needed, with the same operations. */
+/* hullCol, hullPos, hullOrn: the hull's collidable and pose (the body's own,
+ * or a compound child's, processCompoundPairsPrimitivesKernel :980-1090) */
 __device__ bool pc_plane_convex(int pairIndex, int bPlane, int bHull, const b3hRigidBodyData* bodies,
-								const b3hShapeTables& sh, int* col, b3hContact4Data& c)
+								const b3hShapeTables& sh, int* col, b3hContact4Data& c, int hullCol, F4 hullPos,
+								F4 hullOrn)
 {
 	const b3hRigidBodyData& A = bodies[bPlane];
-	const b3hRigidBodyData& B = bodies[bHull];
-	const b3hConvexPolyhedronData hullB = sh.convexes[sh.collidables[B.m_collidableIdx].m_shapeIndex];
+	struct
+	{
+		F4 m_pos, m_quat;
+	} B = {hullPos, hullOrn};
+	const b3hConvexPolyhedronData hullB = sh.convexes[sh.collidables[hullCol].m_shapeIndex];
 	const F4 planeEq = sh.faces[sh.collidables[A.m_collidableIdx].m_shapeIndex].m_plane;
 	const F4 planeNormal = f4(planeEq.x, planeEq.y, planeEq.z, 0.f);
 	const F4 planeNormalWorld = qrot(A.m_quat, planeNormal);
@@ -1568,16 +1574,15 @@ __device__ bool pc_point_in_polygon(F4 p, const b3hGpuFace& face, const F4* base
 
 /* computeContactSphereConvex (:339-487); contact bodies (sphere, hull) */
 __device__ bool pc_sphere_convex(int pairIndex, int bSphere, int bHull, const b3hRigidBodyData* bodies,
-								 const b3hShapeTables& sh, b3hContact4Data& c)
+								 const b3hShapeTables& sh, b3hContact4Data& c, int hullCol, F4 hullPos, F4 hullOrn)
 {
 	const b3hRigidBodyData& S = bodies[bSphere];
-	const b3hRigidBodyData& H = bodies[bHull];
 	const float radius = sh.collidables[S.m_collidableIdx].m_radius;
-	const F4 pos = H.m_pos, quat = H.m_quat;
+	const F4 pos = hullPos, quat = hullOrn;
 	F4 invPos, invOrn;
 	pc_tr_inverse(pos, quat, &invPos, &invOrn);
 	const F4 sp = pc_transform(S.m_pos, invPos, invOrn);
-	const b3hConvexPolyhedronData hull = sh.convexes[sh.collidables[H.m_collidableIdx].m_shapeIndex];
+	const b3hConvexPolyhedronData hull = sh.convexes[sh.collidables[hullCol].m_shapeIndex];
 	F4 closest = f4(0.f, 0.f, 0.f, 0.f), hitN = f4(0.f, 0.f, 0.f, 0.f);
 	float minDist = -1000000.f;
 	bool collide = true;
@@ -1688,23 +1693,361 @@ __global__ __launch_bounds__(PC_THREADS) void k_np_primitive(const b3hInt4* __re
 		if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_CONVEX_HULL) continue;
 		b3hContact4Data c;
 		bool hit = false;
+		const b3hRigidBodyData& A = bodies[pr.x];
+		const b3hRigidBodyData& B = bodies[pr.y];
 		if (ta == B3H_SHAPE_PLANE && tb == B3H_SHAPE_CONVEX_HULL)
-			hit = pc_plane_convex(i, pr.x, pr.y, bodies, sh, col, c);
+			hit = pc_plane_convex(i, pr.x, pr.y, bodies, sh, col, c, B.m_collidableIdx, B.m_pos, B.m_quat);
 		else if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_PLANE)
-			hit = pc_plane_convex(i, pr.y, pr.x, bodies, sh, col, c);
+			hit = pc_plane_convex(i, pr.y, pr.x, bodies, sh, col, c, A.m_collidableIdx, A.m_pos, A.m_quat);
 		else if (ta == B3H_SHAPE_PLANE && tb == B3H_SHAPE_SPHERE)
 			hit = pc_plane_sphere(i, pr.x, pr.y, bodies, sh, c);
 		else if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_PLANE)
 			hit = pc_plane_sphere(i, pr.y, pr.x, bodies, sh, c);
 		else if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_CONVEX_HULL)
-			hit = pc_sphere_convex(i, pr.x, pr.y, bodies, sh, c);
+			hit = pc_sphere_convex(i, pr.x, pr.y, bodies, sh, c, B.m_collidableIdx, B.m_pos, B.m_quat);
 		else if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_SPHERE)
-			hit = pc_sphere_convex(i, pr.y, pr.x, bodies, sh, c);
+			hit = pc_sphere_convex(i, pr.y, pr.x, bodies, sh, c, A.m_collidableIdx, A.m_pos, A.m_quat);
 		else if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_SPHERE)
 			hit = pc_sphere_sphere(i, pr.x, pr.y, bodies, sh, c);
 		flags[i] = hit ? 1 : 0;
 		if (hit) staging[i] = c;
 	}
+}
+
+/* ---------------- compound pairs ----------------
+ * findCompoundPairsKernel (sat.cl:945-1130): a body pair with a compound
+ * becomes child pairs (bodyA, bodyB, childA, childB), -1 for a side that is
+ * not a compound.  Compound x compound: the children whose quantized boxes
+ * (transformed to world) overlap, found by walking both child trees from every
+ * overlapping pair of subtrees (the leaf's child number is the child index,
+ * used as an index into the child table as the reference does).  Compound x
+ * other: every child.  Here the list is built in two passes (count, then emit
+ * at prefix-sum offsets), in pair order and walk order, so it is the same on
+ * every run; the reference appends with an atomic counter. */
+#define CP_STACK 256 /* B3_MAX_STACK_DEPTH, sat.cl:10 */
+__device__ __forceinline__ F4 bvh_unquantize(const unsigned short* q, F4 quant, F4 bmin)
+{
+	F4 v = f4((float)q[0] / quant.x, (float)q[1] / quant.y, (float)q[2] / quant.z, 0.f);
+	return add4(v, bmin);
+}
+__device__ __forceinline__ int bvh_child_index(int escapeOrIndex)
+{
+	unsigned y = (~0u) << (31 - 10);
+	return escapeOrIndex & ~(int)y;
+}
+/* walks the two child trees of a compound x compound pair; emit(ca, cb) per
+ * overlapping leaf pair, in the reference's order */
+template <class EMIT>
+__device__ void compound_walk(const b3hShapeTables& sh, const b3hBvhInfo& iA, F4 posA, F4 ornA, const b3hBvhInfo& iB,
+							  F4 posB, F4 ornB, EMIT emit)
+{
+	int2 stack[CP_STACK];
+	for (int p = 0; p < iA.m_numSubTrees; p++)
+	{
+		const b3hBvhSubtreeInfo sa = sh.bvhSubtrees[iA.m_subTreeOffset + p];
+		F4 aMn, aMx;
+		transform_aabb2(bvh_unquantize(sa.m_quantizedAabbMin, iA.m_quantization, iA.m_aabbMin),
+						bvh_unquantize(sa.m_quantizedAabbMax, iA.m_quantization, iA.m_aabbMin), posA, ornA, &aMn, &aMx);
+		for (int q = 0; q < iB.m_numSubTrees; q++)
+		{
+			const b3hBvhSubtreeInfo sb = sh.bvhSubtrees[iB.m_subTreeOffset + q];
+			F4 bMn, bMx;
+			transform_aabb2(bvh_unquantize(sb.m_quantizedAabbMin, iB.m_quantization, iB.m_aabbMin),
+							bvh_unquantize(sb.m_quantizedAabbMax, iB.m_quantization, iB.m_aabbMin), posB, ornB, &bMn,
+							&bMx);
+			if (!aabb_overlap(aMn, aMx, bMn, bMx)) continue;
+			int depth = 0;
+			stack[depth++] = make_int2(sa.m_rootNodeIndex + iA.m_nodeOffset, sb.m_rootNodeIndex + iB.m_nodeOffset);
+			do
+			{
+				int2 node = stack[--depth];
+				const b3hQuantizedBvhNode na = sh.bvhNodes[node.x], nb = sh.bvhNodes[node.y];
+				F4 mnA, mxA, mnB, mxB;
+				transform_aabb2(bvh_unquantize(na.m_quantizedAabbMin, iA.m_quantization, iA.m_aabbMin),
+								bvh_unquantize(na.m_quantizedAabbMax, iA.m_quantization, iA.m_aabbMin), posA, ornA,
+								&mnA, &mxA);
+				transform_aabb2(bvh_unquantize(nb.m_quantizedAabbMin, iB.m_quantization, iB.m_aabbMin),
+								bvh_unquantize(nb.m_quantizedAabbMax, iB.m_quantization, iB.m_aabbMin), posB, ornB,
+								&mnB, &mxB);
+				if (!aabb_overlap(mnA, mxA, mnB, mxB)) continue;
+				bool leafA = na.m_escapeIndexOrTriangleIndex >= 0, leafB = nb.m_escapeIndexOrTriangleIndex >= 0;
+				if (depth + 4 > CP_STACK && !(leafA && leafB)) continue;
+				if (!leafA)
+				{
+					int la = node.x + 1;
+					int ra = sh.bvhNodes[la].m_escapeIndexOrTriangleIndex >= 0
+								 ? node.x + 2
+								 : node.x + 1 - sh.bvhNodes[la].m_escapeIndexOrTriangleIndex;
+					if (!leafB)
+					{
+						int lb = node.y + 1;
+						int rb = sh.bvhNodes[lb].m_escapeIndexOrTriangleIndex >= 0
+									 ? node.y + 2
+									 : node.y + 1 - sh.bvhNodes[lb].m_escapeIndexOrTriangleIndex;
+						stack[depth++] = make_int2(la, lb);
+						stack[depth++] = make_int2(ra, lb);
+						stack[depth++] = make_int2(la, rb);
+						stack[depth++] = make_int2(ra, rb);
+					}
+					else
+					{
+						stack[depth++] = make_int2(la, node.y);
+						stack[depth++] = make_int2(ra, node.y);
+					}
+				}
+				else if (!leafB)
+				{
+					int lb = node.y + 1;
+					int rb = sh.bvhNodes[lb].m_escapeIndexOrTriangleIndex >= 0
+								 ? node.y + 2
+								 : node.y + 1 - sh.bvhNodes[lb].m_escapeIndexOrTriangleIndex;
+					stack[depth++] = make_int2(node.x, lb);
+					stack[depth++] = make_int2(node.x, rb);
+				}
+				else
+					emit(bvh_child_index(na.m_escapeIndexOrTriangleIndex), bvh_child_index(nb.m_escapeIndexOrTriangleIndex));
+			} while (depth);
+		}
+	}
+}
+/* the child pairs of body pair (a, b): emit(ca, cb) for each, in order */
+template <class EMIT>
+__device__ void compound_children(const b3hShapeTables& sh, const b3hRigidBodyData* bodies, int a, int b, EMIT emit)
+{
+	const b3hRigidBodyData A = bodies[a], B = bodies[b];
+	if (A.m_invMass == 0 && B.m_invMass == 0) return;
+	const b3hCollidable cA = sh.collidables[A.m_collidableIdx], cB = sh.collidables[B.m_collidableIdx];
+	const bool compA = cA.m_shapeType == B3H_SHAPE_COMPOUND_OF_CONVEX_HULLS;
+	const bool compB = cB.m_shapeType == B3H_SHAPE_COMPOUND_OF_CONVEX_HULLS;
+	if (compA && compB)
+		compound_walk(sh, sh.bvhInfos[cA.m_compoundBvhIndex], A.m_pos, A.m_quat, sh.bvhInfos[cB.m_compoundBvhIndex],
+					  B.m_pos, B.m_quat, emit);
+	else if (compA)
+		for (int c = 0; c < cA.m_numChildShapes; c++) emit(cA.m_shapeIndex + c, -1);
+	else if (cA.m_shapeType != B3H_SHAPE_CONCAVE_TRIMESH && compB)
+		for (int c = 0; c < cB.m_numChildShapes; c++) emit(-1, cB.m_shapeIndex + c);
+}
+__global__ __launch_bounds__(256) void k_compound_count(const b3hInt4* __restrict__ pairs, const int* __restrict__ numPairsPtr,
+														const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh,
+														int* __restrict__ counts)
+{
+	int n = *numPairsPtr;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		int cnt = 0;
+		compound_children(sh, bodies, pairs[i].x, pairs[i].y, [&](int, int) { cnt++; });
+		counts[i] = cnt;
+	}
+}
+/* the child pairs at their offsets (capped at cap); *numOut = min(total, cap),
+ * numOut[1] = total */
+__global__ __launch_bounds__(256) void k_compound_emit(const b3hInt4* __restrict__ pairs, const int* __restrict__ numPairsPtr,
+													   const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh,
+													   const int* __restrict__ offsets, int cap, b3hInt4* __restrict__ out,
+													   int* __restrict__ numOut)
+{
+	int n = *numPairsPtr;
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+	{
+		int total = offsets[n];
+		numOut[0] = total < cap ? total : cap;
+		numOut[1] = total;
+	}
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		int a = pairs[i].x, b = pairs[i].y;
+		const int k0 = offsets[i];
+		int k = k0;
+		compound_children(sh, bodies, a, b, [&](int ca, int cb) {
+			if (k < cap) out[k] = b3hInt4{a, b, ca, cb};
+			k++;
+		});
+		/* the body pair's child pairs ordered by (childA, childB) as unsigned
+		 * (the solver's contact order; the walk order of two trees is not it) */
+		int k1 = k < cap ? k : cap;
+		for (int x = k0 + 1; x < k1; x++)
+		{
+			b3hInt4 v = out[x];
+			int y = x - 1;
+			while (y >= k0 && ((unsigned)out[y].z > (unsigned)v.z ||
+							   ((unsigned)out[y].z == (unsigned)v.z && (unsigned)out[y].w > (unsigned)v.w)))
+			{
+				out[y + 1] = out[y];
+				y--;
+			}
+			out[y + 1] = v;
+		}
+	}
+}
+/* a side of a child pair: the child's collidable and pose, or the body's
+ * (processCompoundPairsKernel, sat.cl:830-857) */
+__device__ __forceinline__ void compound_side(const b3hShapeTables& sh, const b3hRigidBodyData& body, int child, int* col,
+											  F4* pos, F4* orn)
+{
+	*pos = body.m_pos;
+	*orn = body.m_quat;
+	*col = body.m_collidableIdx;
+	if (child >= 0)
+	{
+		const b3hGpuChildShape c = sh.children[child];
+		*col = c.m_shapeIndex;
+		*pos = add4(qrot(body.m_quat, c.m_childPosition), body.m_pos);
+		*orn = qmul(body.m_quat, c.m_childOrientation);
+	}
+}
+/* processCompoundPairsKernel (sat.cl:802-925) and clipCompoundsHullHullKernel
+ * (satClipHullContacts.cl:1003-1130), one child pair per wavefront: faces of
+ * A, faces of B, then every edge pair (the exact edge test whatever the edge
+ * count), then clipping and reduction; the contact carries the child indices
+ * and the child pair's index.  Child pairs that are not hull x hull get their
+ * flag here (0) and their contact, if any, from k_compound_primitive. */
+__global__ __launch_bounds__(256, 4) void k_compound_sat(const b3hInt4* __restrict__ cpairs, const int* __restrict__ numPtr,
+														 const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh,
+														 b3hContact4Data* __restrict__ staging, int* __restrict__ flags)
+{
+	constexpr int L = 64, G = 4;
+	__shared__ F4 sVerts[G][2][NP_VCAP];
+	__shared__ F4 sClip[G][3][CLIP_CAP];
+	const Group<L> g;
+	const int gi = threadIdx.x / L;
+	int n = *numPtr;
+	for (int k = blockIdx.x * G + gi; k < n; k += gridDim.x * G)
+	{
+		const b3hInt4 cp = cpairs[k];
+		const b3hRigidBodyData BA = bodies[cp.x], BB = bodies[cp.y];
+		int colA, colB;
+		b3hSatBody A, B;
+		compound_side(sh, BA, cp.z, &colA, &A.pos, &A.quat);
+		compound_side(sh, BB, cp.w, &colB, &B.pos, &B.quat);
+		const b3hCollidable cA = sh.collidables[colA], cB = sh.collidables[colB];
+		int hit = 0;
+		int nReduced = 0;
+		int idx[4] = {0, 0, 0, 0};
+		F4 sep = f4(1.f, 0.f, 0.f, 0.f);
+		F4* pts = sClip[gi][2];
+		if (cA.m_shapeType == B3H_SHAPE_CONVEX_HULL && cB.m_shapeType == B3H_SHAPE_CONVEX_HULL)
+		{
+			const b3hConvexPolyhedronData ha = sh.convexes[cA.m_shapeIndex], hb = sh.convexes[cB.m_shapeIndex];
+			A.vOff = ha.m_vertexOffset;
+			A.nV = ha.m_numVertices;
+			A.fOff = ha.m_faceOffset;
+			A.nF = ha.m_numFaces;
+			B.vOff = hb.m_vertexOffset;
+			B.nV = hb.m_numVertices;
+			B.fOff = hb.m_faceOffset;
+			B.nF = hb.m_numFaces;
+			PairGeom P, Q;
+			stage_pair(g, sh, A, B, sVerts[gi][0], sVerts[gi][1], P);
+			sat_frame(P, Q);
+			F4 dc = sub4(xform(ha.m_localCenter, Q.posA, Q.ornA), xform(hb.m_localCenter, Q.posB, Q.ornB));
+			float dmin = FLT_MAX;
+			Family fa = {0, ha.m_numFaces, sh.faces + ha.m_faceOffset, nullptr, nullptr, 1, nullptr, nullptr, nullptr};
+			Family fb = {1, hb.m_numFaces, sh.faces + hb.m_faceOffset, nullptr, nullptr, 1, nullptr, nullptr, nullptr};
+			Family fe = {2,
+						 ha.m_numUniqueEdges * hb.m_numUniqueEdges,
+						 nullptr,
+						 sh.uniqueEdges + ha.m_uniqueEdgesOffset,
+						 sh.uniqueEdges + hb.m_uniqueEdgesOffset,
+						 hb.m_numUniqueEdges,
+						 nullptr,
+						 nullptr,
+						 nullptr};
+			if (sat_family(g, fa, Q, dc, &sep, &dmin) && sat_family(g, fb, Q, dc, &sep, &dmin) &&
+				sat_family(g, fe, Q, dc, &sep, &dmin))
+			{
+				wave_sync();
+				int np = clip_hull_hull(g, sep, P, A, B, sh, sClip[gi][0], sClip[gi][1], pts, nullptr, nullptr);
+				if (np > 0)
+				{
+					nReduced = extract_manifold(g, pts, np, neg4(sep), idx);
+					hit = 1;
+				}
+			}
+		}
+		const bool prim = !(cA.m_shapeType == B3H_SHAPE_CONVEX_HULL && cB.m_shapeType == B3H_SHAPE_CONVEX_HULL);
+		if (g.lane == 0 && !prim)
+		{
+			flags[k] = hit;
+			if (hit)
+			{
+				b3hContact4Data c;
+				for (int q = 0; q < 4; q++) c.m_worldPosB[q] = q < nReduced ? pts[idx[q]] : f4(0, 0, 0, 0);
+				c.m_worldNormalOnB = sep;
+				c.m_worldNormalOnB.w = (float)nReduced;
+				c.m_restituitionCoeffCmp = (unsigned short)(0.f * 0xffff);
+				c.m_frictionCoeffCmp = (unsigned short)(0.7f * 0xffff);
+				c.m_batchIdx = k;
+				c.m_bodyAPtrAndSignBit = BA.m_invMass == 0 ? -cp.x : cp.x;
+				c.m_bodyBPtrAndSignBit = BB.m_invMass == 0 ? -cp.y : cp.y;
+				c.m_childIndexA = cp.z;
+				c.m_childIndexB = cp.w;
+				c.m_unused1 = 0;
+				c.m_unused2 = 0;
+				staging[k] = c;
+			}
+		}
+		if (g.lane == 0 && prim) flags[k] = 0;
+	}
+}
+/* processCompoundPairsPrimitivesKernel (primitiveContacts.cl:980-1090): a
+ * child hull against a plane or a sphere body, one child pair per thread; the
+ * contact carries the child pair's index and child indices -1 */
+__global__ __launch_bounds__(PC_THREADS) void k_compound_primitive(const b3hInt4* __restrict__ cpairs,
+																   const int* __restrict__ numPtr,
+																   const b3hRigidBodyData* __restrict__ bodies,
+																   b3hShapeTables sh, b3hContact4Data* __restrict__ staging,
+																   int* __restrict__ flags)
+{
+	__shared__ int sIdx[PC_MAX_PLANE_POINTS * PC_THREADS];
+	int* col = sIdx + threadIdx.x;
+	const int n = *numPtr;
+	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+	{
+		const b3hInt4 cp = cpairs[k];
+		int colA, colB;
+		F4 posA, ornA, posB, ornB;
+		compound_side(sh, bodies[cp.x], cp.z, &colA, &posA, &ornA);
+		compound_side(sh, bodies[cp.y], cp.w, &colB, &posB, &ornB);
+		const int ta = sh.collidables[colA].m_shapeType, tb = sh.collidables[colB].m_shapeType;
+		b3hContact4Data c;
+		bool hit = false, mine = true;
+		if (ta == B3H_SHAPE_PLANE && tb == B3H_SHAPE_CONVEX_HULL)
+			hit = pc_plane_convex(k, cp.x, cp.y, bodies, sh, col, c, colB, posB, ornB);
+		else if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_PLANE)
+			hit = pc_plane_convex(k, cp.y, cp.x, bodies, sh, col, c, colA, posA, ornA);
+		else if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_SPHERE)
+			hit = pc_sphere_convex(k, cp.y, cp.x, bodies, sh, c, colA, posA, ornA);
+		else if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_CONVEX_HULL)
+			hit = pc_sphere_convex(k, cp.x, cp.y, bodies, sh, c, colB, posB, ornB);
+		else
+			mine = false;
+		if (!mine) continue;
+		flags[k] = hit ? 1 : 0;
+		if (hit) staging[k] = c;
+	}
+}
+/* compound contacts after the pair contacts: slot base + scan[k] for flagged
+ * child pair k (staging index stagingBase + k); counts[0] grows by their
+ * number (capped at maxContacts), counts[1] (contacts produced) likewise */
+__global__ __launch_bounds__(256) void k_compound_compact(const int* __restrict__ numPtr, const int* __restrict__ flags,
+														  const int* __restrict__ scan, int* __restrict__ contactPair,
+														  int stagingBase, int maxContacts, int* __restrict__ numContacts)
+{
+	int n = *numPtr;
+	int base = numContacts[1];
+	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+	{
+		if (!flags[k]) continue;
+		int slot = base + scan[k];
+		if (slot < maxContacts) contactPair[slot] = stagingBase + k;
+	}
+}
+__global__ void k_compound_count_contacts(const int* __restrict__ numPtr, const int* __restrict__ scan, int maxContacts,
+										  int* __restrict__ numContacts)
+{
+	int total = numContacts[1] + scan[*numPtr];
+	numContacts[1] = total;
+	numContacts[0] = total < maxContacts ? total : maxContacts;
 }
 
 /* slots in pair order: pairs[i].z = slot or -1, contactPair[slot] = i */
@@ -1747,7 +2090,8 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 					 const b3hShapeTables& sh, b3hContact4Data* staging, int* flags, int* flagScan, unsigned* scanTmp,
 					 int* contactPair, int maxContacts, int* numContacts, int* sepState, b3hFloat4* sepNormal,
 					 int pairsPerGroup, hipStream_t s, hipEvent_t evAfterKernel, b3hFloat4* mprScratch,
-					 hipEvent_t evAfterMpr, bool hasPrimitives, const b3hSatCache* satCache, const b3hSatWork& work)
+					 hipEvent_t evAfterMpr, bool hasPrimitives, const b3hSatCache* satCache, const b3hSatWork& work,
+					 const b3hCompoundWork& cw)
 {
 	{
 		int mb = (maxPairs + 255) / 256;
@@ -1821,6 +2165,24 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 		hipLaunchKernelGGL(k_np_primitive, dim3(pb), dim3(PC_THREADS), 0, s, pairs, numPairs, bodies, sh, staging,
 						   flags);
 	}
+	if (cw.cap > 0)
+	{
+		/* compound child pairs: list, SAT and clipping, primitives */
+		int pb = blocksFor(maxPairs, 256, 256);
+		hipLaunchKernelGGL(k_compound_count, dim3(pb), dim3(256), 0, s, pairs, numPairs, bodies, sh, cw.counts);
+		b3h_exclusive_scan(cw.counts, cw.offsets, numPairs, maxPairs, cw.scanTmp, s);
+		hipLaunchKernelGGL(k_compound_emit, dim3(pb), dim3(256), 0, s, pairs, numPairs, bodies, sh,
+						   (const int*)cw.offsets, cw.cap, cw.cpairs, cw.num);
+		hipLaunchKernelGGL(k_compound_sat, dim3(blocksFor(cw.cap, 4, 256)), dim3(256), 0, s, (const b3hInt4*)cw.cpairs,
+						   (const int*)cw.num, bodies, sh, staging + cw.stagingBase, cw.flags);
+		if (hasPrimitives)
+		{
+			int cb = (cw.cap + PC_THREADS - 1) / PC_THREADS;
+			if (cb > 16384) cb = 16384;
+			hipLaunchKernelGGL(k_compound_primitive, dim3(cb), dim3(PC_THREADS), 0, s, (const b3hInt4*)cw.cpairs,
+							   (const int*)cw.num, bodies, sh, staging + cw.stagingBase, cw.flags);
+		}
+	}
 	if (evAfterKernel) hipEventRecord(evAfterKernel, s);
 	b3h_exclusive_scan(flags, flagScan, numPairs, maxPairs, scanTmp, s);
 	int cblocks = (maxPairs + 255) / 256;
@@ -1829,6 +2191,17 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 	hipLaunchKernelGGL(k_np_compact, dim3(cblocks), dim3(256), 0, s, pairs, numPairs, flags, flagScan, contactPair,
 					   maxContacts);
 	hipLaunchKernelGGL(k_np_count, dim3(1), dim3(1), 0, s, numPairs, flagScan, maxContacts, numContacts, numContacts + 1);
+	if (cw.cap > 0)
+	{
+		/* the child pairs' contacts after the pair contacts, in child-pair order */
+		b3h_exclusive_scan(cw.flags, cw.scan, cw.num, cw.cap, cw.scanTmp, s);
+		int cb = (cw.cap + 255) / 256;
+		if (cb > 16384) cb = 16384;
+		hipLaunchKernelGGL(k_compound_compact, dim3(cb), dim3(256), 0, s, (const int*)cw.num, (const int*)cw.flags,
+						   (const int*)cw.scan, contactPair, cw.stagingBase, maxContacts, numContacts);
+		hipLaunchKernelGGL(k_compound_count_contacts, dim3(1), dim3(1), 0, s, (const int*)cw.num, (const int*)cw.scan,
+						   maxContacts, numContacts);
+	}
 }
 
 __global__ __launch_bounds__(256) void k_gather_contacts(const int* __restrict__ contactPair, const int* __restrict__ nPtr,

@@ -150,7 +150,8 @@ __device__ __forceinline__ void bitonic_sort_n(KP key, PP pay, int n, int t)
 				if (l > i && l < n)
 				{
 					unsigned long long ki = key[i], kl = key[l];
-					if (ki > kl)
+					/* equal keys (a compound's child contacts) keep slot order */
+					if (ki > kl || (ki == kl && pay[i] > pay[l]))
 					{
 						key[i] = kl;
 						key[l] = ki;

@@ -129,6 +129,49 @@ def convex_pile(nx=64, ny=32, nz=64, spacing=1.2, n_hull_shapes=64, drop=1.0, se
                         cell_size=2.0 * math.sqrt(3.0))
 
 
+def compound_scene(n=64, seed=7, plane=True, sphere=True, ground_half=(12.0, 0.5, 12.0)):
+    """Compounds among hulls: a ground box (body 0), optionally a static plane
+    at y = 0 and spheres, and n dynamic bodies cycling through a unit box, a
+    16-point hull and three compounds: a dumbbell (two small boxes), a three-
+    part cluster (a hull, a small box and a unit box under child rotations) and
+    a one-child compound whose child sits at the body origin.  Shapes are
+    {"compound": [(shape k, position, orientation), ...]} entries after the
+    hulls they use."""
+    rng = np.random.default_rng(seed)
+    qa, qb, qc = (random_quats(rng, 1)[0].astype(np.float32) for _ in range(3))
+    shapes = [box_points(*ground_half), box_points(0.5, 0.5, 0.5), sphere_points(rng, 16),
+              box_points(0.3, 0.15, 0.15),
+              {"compound": [(3, (-0.45, 0.0, 0.0), (0, 0, 0, 1)), (3, (0.45, 0.0, 0.0), (0, 0, 0, 1))]},
+              {"compound": [(2, (0.0, 0.35, 0.0), tuple(qa)), (3, (0.4, -0.2, 0.0), tuple(qb)),
+                            (1, (-0.4, -0.2, 0.1), tuple(qc))]},
+              {"compound": [(1, (0.0, 0.0, 0.0), (0, 0, 0, 1))]}]
+    kinds = [1, 2, 4, 5, 6]
+    plane_shape = -1
+    if plane:
+        plane_shape = len(shapes)
+        shapes.append({"plane": (0.0, 1.0, 0.0, 0.0)})
+    if sphere:
+        shapes.append({"sphere": 0.35})
+        kinds.append(len(shapes) - 1)
+    mass, pos, shape = [], [], []
+    side = max(2, int(math.ceil(n ** (1.0 / 3.0))))
+    for k in range(n):
+        x, z, y = k % side, (k // side) % side, k // (side * side)
+        pos.append((1.4 * (x - (side - 1) / 2.0) + rng.uniform(-0.05, 0.05), 0.9 + 1.3 * y,
+                    1.4 * (z - (side - 1) / 2.0) + rng.uniform(-0.05, 0.05)))
+        shape.append(kinds[k % len(kinds)])
+        mass.append(float(np.exp(rng.uniform(np.log(0.5), np.log(2.0)))))
+    orn = random_quats(rng, n).astype(np.float32)
+    if plane:
+        # the plane as the second static body
+        pos.insert(0, (0.0, 0.0, 0.0))
+        shape.insert(0, plane_shape)
+        mass.insert(0, 0.0)
+        orn = np.concatenate([[[0, 0, 0, 1.0]], orn]).astype(np.float32)
+    return _with_ground("compounds", shapes, np.array(mass, np.float32), np.array(pos, np.float32), orn,
+                        np.array(shape, np.int32), ground_half, cell_size=4.5)
+
+
 def aabb_cloud(n=1 << 20, mean_overlap=8.0, hmin=0.05, hmax=1.0, seed=3, side=None):
     """Broadphase stress: n cubic AABBs, half-extents log-uniform [hmin,hmax],
     centres uniform in a cube whose side gives about `mean_overlap` partners

@@ -45,9 +45,18 @@ class World:
         self.collidables = []
         for k, pts in enumerate(scene.shapes):
             if isinstance(pts, dict):
-                # primitive shapes: {"sphere": radius} or {"plane": (nx, ny, nz, constant)}
+                # primitive shapes: {"sphere": radius} or {"plane": (nx, ny, nz, constant)};
+                # compounds: {"compound": [(shape k of an earlier hull, (x, y, z), (qx, qy, qz, qw)), ...]}
                 out = ctypes.c_int(-1)
-                if "sphere" in pts:
+                if "compound" in pts:
+                    kids = pts["compound"]
+                    ch = np.zeros(len(kids), layout.CHILD_SHAPE)
+                    for j, (sk, cpos, corn) in enumerate(kids):
+                        ch["childPosition"][j, :3] = cpos
+                        ch["childOrientation"][j] = corn
+                        ch["shapeIndex"][j] = self.collidables[sk]
+                    call("b3h_np_register_compound_shape", self.np_, ptr(ch), len(ch), ctypes.byref(out))
+                elif "sphere" in pts:
                     call("b3h_np_register_sphere_shape", self.np_, ctypes.c_float(pts["sphere"]), ctypes.byref(out))
                 else:
                     normal = np.ascontiguousarray(pts["plane"][:3], np.float32)
@@ -110,6 +119,20 @@ class World:
         d = ctypes.c_void_p()
         call("b3h_np_get_bodies_gpu", self.np_, ctypes.byref(d), None)
         capi.htod(d.value, np.ascontiguousarray(bodies))
+
+    def compound_tables(self):
+        """host copies of the compound tables: children, infos, nodes, subtrees"""
+        counts = np.zeros(4, np.int32)
+        call("b3h_np_get_compound_tables", self.np_, ptr(counts), None, None, None, None)
+        t = dict(children=np.zeros(max(1, counts[0]), layout.CHILD_SHAPE),
+                 infos=np.zeros(max(1, counts[1]), layout.BVH_INFO),
+                 nodes=np.zeros(max(1, counts[2]), layout.BVH_NODE),
+                 subtrees=np.zeros(max(1, counts[3]), layout.BVH_SUBTREE))
+        call("b3h_np_get_compound_tables", self.np_, ptr(counts), ptr(t["children"]), ptr(t["infos"]),
+             ptr(t["nodes"]), ptr(t["subtrees"]))
+        if counts[0] == 0:
+            t["children"] = t["children"][:0]
+        return t
 
     def inertias(self):
         out = np.zeros(self.num_bodies, layout.INERTIA)

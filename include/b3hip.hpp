@@ -73,6 +73,13 @@ public:
 		int idx = -1;
 		return b3hip::ok(b3h_np_register_plane_shape(m_np, planeNormal, planeConstant, &idx)) ? idx : -1;
 	}
+	/* registerCompoundShape(b3AlignedObjectArray<b3GpuChildShape>*), b3GpuNarrowPhase.h:33 */
+	int registerCompoundShape(const std::vector<b3hGpuChildShape>& childShapes)
+	{
+		int idx = -1;
+		return b3hip::ok(b3h_np_register_compound_shape(m_np, childShapes.data(), (int)childShapes.size(), &idx)) ? idx
+																												   : -1;
+	}
 	int registerRigidBody(int collidableIndex, float mass, const float* position, const float* orientation,
 						  const float* aabbMin, const float* aabbMax, bool writeToGpu)
 	{

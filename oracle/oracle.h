@@ -52,6 +52,25 @@ int b3o_narrowphase(b3hInt4* pairs, int nPairs, const b3hRigidBodyData* bodies,
 					const b3hFloat4* vertices, const b3hFloat4* uniqueEdges, const b3hGpuFace* faces,
 					const int* indices, b3hContact4Data* contacts, int capacity,
 					int* hasSepOut, b3hFloat4* sepOut);
+/* compound tables (registerCompoundShape); capacity 0: no compounds */
+typedef struct b3oCompoundTables
+{
+	const b3hGpuChildShape* children;
+	const b3hBvhInfo* infos;
+	const b3hQuantizedBvhNode* nodes;
+	const b3hBvhSubtreeInfo* subtrees;
+	int capacity; /* child pairs kept */
+} b3oCompoundTables;
+/* as b3o_narrowphase, then the compound child pairs: their contacts follow the
+ * pair contacts, in child-pair order (pairs in order, each pair's child pairs
+ * sorted by (childA, childB) as unsigned); childPairsOut (optional, capacity
+ * entries) receives the child pairs and *numChildPairs their number (all
+ * found, also beyond the capacity) */
+int b3o_narrowphase_ex(b3hInt4* pairs, int nPairs, const b3hRigidBodyData* bodies, const b3hCollidable* cols,
+					   const b3hConvexPolyhedronData* convexes, const b3hFloat4* vertices, const b3hFloat4* uniqueEdges,
+					   const b3hGpuFace* faces, const int* indices, b3hContact4Data* contacts, int capacity,
+					   int* hasSepOut, b3hFloat4* sepOut, const b3oCompoundTables* compound, b3hInt4* childPairsOut,
+					   int* numChildPairs);
 
 /* joint solver (constraints' flags updated by breaking; bodies' velocities in
  * place); returns the number of batches */
@@ -107,6 +126,8 @@ typedef struct b3oWorld
 	/* joints (optional), solved after the narrowphase, before the contacts */
 	b3hGpuGenericConstraint* constraints;
 	int numConstraints;
+	/* compound tables (capacity 0: none) */
+	b3oCompoundTables compound;
 } b3oWorld;
 
 #ifdef __cplusplus

@@ -13,6 +13,8 @@
 #include "oracle.h"
 #include "unit_sphere_162.h"
 #include <vector>
+#include <algorithm>
+#include <cstring>
 
 #define MPR_TOL 1E-6f
 #define MPR_MAX_ITER 1000
@@ -730,12 +732,22 @@ static int p_extract_manifold(const F4* p, int n, F4 nearNormal, int* idx)
 	if (idx[0] != minIndex && idx[1] != minIndex && idx[2] != minIndex && idx[3] != minIndex) idx[0] = minIndex;
 	return 4;
 }
+/* the hull side of a primitive contact: its collidable and pose (the body's
+ * own, or a compound child's: processCompoundPairsPrimitivesKernel, :980-1090) */
+struct HullPose
+{
+	int col;
+	F4 m_pos, m_quat;
+};
+static HullPose body_hull(const OShapes& S, int b)
+{
+	return HullPose{S.bodies[b].m_collidableIdx, S.bodies[b].m_pos, S.bodies[b].m_quat};
+}
 /* computeContactPlaneConvex, :584-727 (contact bodies: plane, hull) */
-static bool p_plane_convex(const OShapes& S, int pairIndex, int bA, int bB, b3hContact4Data* c)
+static bool p_plane_convex(const OShapes& S, int pairIndex, int bA, int bB, b3hContact4Data* c, const HullPose& B)
 {
 	const b3hRigidBodyData& A = S.bodies[bA];
-	const b3hRigidBodyData& B = S.bodies[bB];
-	const b3hConvexPolyhedronData& hullB = S.hullOf(bB);
+	const b3hConvexPolyhedronData& hullB = S.convexes[S.cols[B.col].m_shapeIndex];
 	F4 planeEq = S.faces[S.cols[A.m_collidableIdx].m_shapeIndex].m_plane;
 	F4 planeNormal = f4(planeEq.x, planeEq.y, planeEq.z, 0.f);
 	F4 planeNormalWorld = qrot(A.m_quat, planeNormal);
@@ -845,16 +857,15 @@ static bool p_point_in_polygon(F4 p, const b3hGpuFace& face, const F4* base, con
 	return true;
 }
 /* computeContactSphereConvex, :339-487 (contact bodies: sphere, hull) */
-static bool p_sphere_convex(const OShapes& S, int pairIndex, int bA, int bB, b3hContact4Data* c)
+static bool p_sphere_convex(const OShapes& S, int pairIndex, int bA, int bB, b3hContact4Data* c, const HullPose& H)
 {
 	const b3hRigidBodyData& Sp = S.bodies[bA];
-	const b3hRigidBodyData& H = S.bodies[bB];
 	float radius = S.cols[Sp.m_collidableIdx].m_radius;
 	F4 pos = H.m_pos, quat = H.m_quat;
 	F4 invPos, invOrn;
 	p_tr_inverse(pos, quat, &invPos, &invOrn);
 	F4 spherePos = p_transform(Sp.m_pos, invPos, invOrn);
-	const b3hConvexPolyhedronData& hull = S.hullOf(bB);
+	const b3hConvexPolyhedronData& hull = S.convexes[S.cols[H.col].m_shapeIndex];
 	F4 closestPnt = f4(0, 0, 0, 0), hitNormalWorld = f4(0, 0, 0, 0);
 	float minDist = -1000000.f;
 	bool bCollide = true;
@@ -955,12 +966,12 @@ static bool p_primitive_pair(const OShapes& S, const b3hInt4& pr, int i, b3hCont
 	int tb = S.cols[S.bodies[pr.y].m_collidableIdx].m_shapeType;
 	*handled = !(ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_CONVEX_HULL);
 	if (!*handled) return false;
-	if (ta == B3H_SHAPE_PLANE && tb == B3H_SHAPE_CONVEX_HULL) return p_plane_convex(S, i, pr.x, pr.y, c);
-	if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_PLANE) return p_plane_convex(S, i, pr.y, pr.x, c);
+	if (ta == B3H_SHAPE_PLANE && tb == B3H_SHAPE_CONVEX_HULL) return p_plane_convex(S, i, pr.x, pr.y, c, body_hull(S, pr.y));
+	if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_PLANE) return p_plane_convex(S, i, pr.y, pr.x, c, body_hull(S, pr.x));
 	if (ta == B3H_SHAPE_PLANE && tb == B3H_SHAPE_SPHERE) return p_plane_sphere(S, i, pr.x, pr.y, c);
 	if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_PLANE) return p_plane_sphere(S, i, pr.y, pr.x, c);
-	if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_CONVEX_HULL) return p_sphere_convex(S, i, pr.x, pr.y, c);
-	if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_SPHERE) return p_sphere_convex(S, i, pr.y, pr.x, c);
+	if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_CONVEX_HULL) return p_sphere_convex(S, i, pr.x, pr.y, c, body_hull(S, pr.y));
+	if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_SPHERE) return p_sphere_convex(S, i, pr.y, pr.x, c, body_hull(S, pr.x));
 	if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_SPHERE) return p_sphere_sphere(S, i, pr.x, pr.y, c);
 	return false;
 }
@@ -1084,6 +1095,219 @@ static bool narrow_pair(const OShapes& S, const b3hInt4& pr, int i, b3hContact4D
  * evaluated in parallel into a per-pair buffer and compacted in pair order
  * afterwards, so the output is the same bytes.  hasSepOut/sepOut (optional)
  * receive the final SAT state and normal of each pair. */
+/* ---------------- compounds ---------------- */
+/* b3TransformAabb2 (shared/b3Aabb.h:24-43), margin 0 */
+static void o_transform_aabb2(F4 lmin, F4 lmax, F4 pos, F4 q, F4* mn, F4* mx)
+{
+	F4 half = smul4(0.5f, sub4(lmax, lmin));
+	half = add4(half, f4(0.f, 0.f, 0.f, 0.f));
+	F4 center = smul4(0.5f, add4(lmax, lmin));
+	F4 q2 = f4(q.x * q.x, q.y * q.y, q.z * q.z, 0.f);
+	F4 r0, r1, r2;
+	r0 = f4(1 - 2 * q2.y - 2 * q2.z, 2 * q.x * q.y - 2 * q.w * q.z, 2 * q.x * q.z + 2 * q.w * q.y, 0.f);
+	r1 = f4(2 * q.x * q.y + 2 * q.w * q.z, 1 - 2 * q2.x - 2 * q2.z, 2 * q.y * q.z - 2 * q.w * q.x, 0.f);
+	r2 = f4(2 * q.x * q.z - 2 * q.w * q.y, 2 * q.y * q.z + 2 * q.w * q.x, 1 - 2 * q2.x - 2 * q2.y, 0.f);
+	F4 a0 = f4(fabsf(r0.x), fabsf(r0.y), fabsf(r0.z), 0.f);
+	F4 a1 = f4(fabsf(r1.x), fabsf(r1.y), fabsf(r1.z), 0.f);
+	F4 a2 = f4(fabsf(r2.x), fabsf(r2.y), fabsf(r2.z), 0.f);
+	F4 c = xform(center, pos, q);
+	F4 ext = f4(dot3(half, a0), dot3(half, a1), dot3(half, a2), 0.f);
+	*mn = sub4(c, ext);
+	*mx = add4(c, ext);
+}
+/* b3TestAabbAgainstAabb (shared/b3Aabb.h:46-54) */
+static bool o_aabb_overlap(F4 mn1, F4 mx1, F4 mn2, F4 mx2)
+{
+	bool overlap = true;
+	overlap = (mn1.x > mx2.x || mx1.x < mn2.x) ? false : overlap;
+	overlap = (mn1.z > mx2.z || mx1.z < mn2.z) ? false : overlap;
+	overlap = (mn1.y > mx2.y || mx1.y < mn2.y) ? false : overlap;
+	return overlap;
+}
+/* MyUnQuantize (sat.cl:926-940) */
+static F4 o_unquantize(const unsigned short* q, F4 quant, F4 bmin)
+{
+	return add4(f4((float)q[0] / quant.x, (float)q[1] / quant.y, (float)q[2] / quant.z, 0.f), bmin);
+}
+static void o_node_box(const b3hBvhInfo& info, const unsigned short* qmn, const unsigned short* qmx, F4 pos, F4 orn,
+					   F4* mn, F4* mx)
+{
+	o_transform_aabb2(o_unquantize(qmn, info.m_quantization, info.m_aabbMin),
+					  o_unquantize(qmx, info.m_quantization, info.m_aabbMin), pos, orn, mn, mx);
+}
+/* findCompoundPairsKernel (sat.cl:945-1130): the child pairs of body pair
+ * (a, b) appended to out, in the walk order of the reference */
+static void compound_children(const OShapes& S, const b3oCompoundTables& T, int a, int b, std::vector<b3hInt4>& out)
+{
+	const b3hRigidBodyData& A = S.bodies[a];
+	const b3hRigidBodyData& B = S.bodies[b];
+	if (A.m_invMass == 0 && B.m_invMass == 0) return;
+	const b3hCollidable& cA = S.cols[A.m_collidableIdx];
+	const b3hCollidable& cB = S.cols[B.m_collidableIdx];
+	const bool compA = cA.m_shapeType == B3H_SHAPE_COMPOUND_OF_CONVEX_HULLS;
+	const bool compB = cB.m_shapeType == B3H_SHAPE_COMPOUND_OF_CONVEX_HULLS;
+	if (compA && compB)
+	{
+		const b3hBvhInfo& iA = T.infos[cA.m_compoundBvhIndex];
+		const b3hBvhInfo& iB = T.infos[cB.m_compoundBvhIndex];
+		const int maxDepth = 256; /* B3_MAX_STACK_DEPTH, sat.cl:10 */
+		for (int p = 0; p < iA.m_numSubTrees; p++)
+		{
+			const b3hBvhSubtreeInfo& sa = T.subtrees[iA.m_subTreeOffset + p];
+			F4 aMn, aMx;
+			o_node_box(iA, sa.m_quantizedAabbMin, sa.m_quantizedAabbMax, A.m_pos, A.m_quat, &aMn, &aMx);
+			for (int q = 0; q < iB.m_numSubTrees; q++)
+			{
+				const b3hBvhSubtreeInfo& sb = T.subtrees[iB.m_subTreeOffset + q];
+				F4 bMn, bMx;
+				o_node_box(iB, sb.m_quantizedAabbMin, sb.m_quantizedAabbMax, B.m_pos, B.m_quat, &bMn, &bMx);
+				if (!o_aabb_overlap(aMn, aMx, bMn, bMx)) continue;
+				std::vector<std::pair<int, int>> stack;
+				stack.push_back({sa.m_rootNodeIndex + iA.m_nodeOffset, sb.m_rootNodeIndex + iB.m_nodeOffset});
+				while (!stack.empty())
+				{
+					std::pair<int, int> node = stack.back();
+					stack.pop_back();
+					const b3hQuantizedBvhNode& na = T.nodes[node.first];
+					const b3hQuantizedBvhNode& nb = T.nodes[node.second];
+					F4 mnA, mxA, mnB, mxB;
+					o_node_box(iA, na.m_quantizedAabbMin, na.m_quantizedAabbMax, A.m_pos, A.m_quat, &mnA, &mxA);
+					o_node_box(iB, nb.m_quantizedAabbMin, nb.m_quantizedAabbMax, B.m_pos, B.m_quat, &mnB, &mxB);
+					if (!o_aabb_overlap(mnA, mxA, mnB, mxB)) continue;
+					bool leafA = na.m_escapeIndexOrTriangleIndex >= 0, leafB = nb.m_escapeIndexOrTriangleIndex >= 0;
+					if ((int)stack.size() + 4 > maxDepth && !(leafA && leafB)) continue;
+					auto right = [&](int n) {
+						const b3hQuantizedBvhNode& l = T.nodes[n + 1];
+						return l.m_escapeIndexOrTriangleIndex >= 0 ? n + 2 : n + 1 - l.m_escapeIndexOrTriangleIndex;
+					};
+					if (!leafA)
+					{
+						int la = node.first + 1, ra = right(node.first);
+						if (!leafB)
+						{
+							int lb = node.second + 1, rb = right(node.second);
+							stack.push_back({la, lb});
+							stack.push_back({ra, lb});
+							stack.push_back({la, rb});
+							stack.push_back({ra, rb});
+						}
+						else
+						{
+							stack.push_back({la, node.second});
+							stack.push_back({ra, node.second});
+						}
+					}
+					else if (!leafB)
+					{
+						int lb = node.second + 1, rb = right(node.second);
+						stack.push_back({node.first, lb});
+						stack.push_back({node.first, rb});
+					}
+					else
+					{
+						/* the leaf's child number (getTriangleIndexGlobal, sat.cl:67-73) */
+						unsigned y = (~0u) << (31 - 10);
+						out.push_back(b3hInt4{a, b, na.m_escapeIndexOrTriangleIndex & ~(int)y,
+											  nb.m_escapeIndexOrTriangleIndex & ~(int)y});
+					}
+				}
+			}
+		}
+	}
+	else if (compA)
+		for (int c = 0; c < cA.m_numChildShapes; c++) out.push_back(b3hInt4{a, b, cA.m_shapeIndex + c, -1});
+	else if (cA.m_shapeType != B3H_SHAPE_CONCAVE_TRIMESH && compB)
+		for (int c = 0; c < cB.m_numChildShapes; c++) out.push_back(b3hInt4{a, b, -1, cB.m_shapeIndex + c});
+}
+/* a child pair's side: the child's collidable and pose, or the body's
+ * (processCompoundPairsKernel, sat.cl:830-857) */
+static HullPose compound_side(const OShapes& S, const b3oCompoundTables& T, int body, int child)
+{
+	const b3hRigidBodyData& B = S.bodies[body];
+	if (child < 0) return HullPose{B.m_collidableIdx, B.m_pos, B.m_quat};
+	const b3hGpuChildShape& c = T.children[child];
+	return HullPose{c.m_shapeIndex, add4(qrot(B.m_quat, c.m_childPosition), B.m_pos), qmul(B.m_quat, c.m_childOrientation)};
+}
+/* the contact of child pair k: processCompoundPairsKernel (sat.cl:802-925) with
+ * clipCompoundsHullHullKernel (satClipHullContacts.cl:1003-1130) for two hulls,
+ * processCompoundPairsPrimitivesKernel (primitiveContacts.cl:980-1090) for a
+ * hull against a plane or a sphere */
+static bool compound_pair(const OShapes& S, const b3oCompoundTables& T, const b3hInt4& cp, int k, b3hContact4Data* c)
+{
+	HullPose A = compound_side(S, T, cp.x, cp.z), B = compound_side(S, T, cp.y, cp.w);
+	const int ta = S.cols[A.col].m_shapeType, tb = S.cols[B.col].m_shapeType;
+	memset(c, 0, sizeof(*c));
+	if (ta == B3H_SHAPE_PLANE && tb == B3H_SHAPE_CONVEX_HULL) return p_plane_convex(S, k, cp.x, cp.y, c, B);
+	if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_PLANE) return p_plane_convex(S, k, cp.y, cp.x, c, A);
+	if (ta == B3H_SHAPE_CONVEX_HULL && tb == B3H_SHAPE_SPHERE) return p_sphere_convex(S, k, cp.y, cp.x, c, A);
+	if (ta == B3H_SHAPE_SPHERE && tb == B3H_SHAPE_CONVEX_HULL) return p_sphere_convex(S, k, cp.x, cp.y, c, B);
+	if (ta != B3H_SHAPE_CONVEX_HULL || tb != B3H_SHAPE_CONVEX_HULL) return false;
+	const b3hConvexPolyhedronData& ha = S.convexes[S.cols[A.col].m_shapeIndex];
+	const b3hConvexPolyhedronData& hb = S.convexes[S.cols[B.col].m_shapeIndex];
+	float dmin = FLT_MAX;
+	F4 posA = A.m_pos, posB = B.m_pos;
+	posA.w = 0.f;
+	posB.w = 0.f;
+	F4 c0 = xform(ha.m_localCenter, posA, A.m_quat);
+	F4 c1 = xform(hb.m_localCenter, posB, B.m_quat);
+	F4 dc = sub4(c0, c1);
+	F4 sep = f4(1, 0, 0, 0);
+	if (!sat_faces(S, ha, hb, A.m_pos, A.m_quat, B.m_pos, B.m_quat, dc, &sep, &dmin)) return false;
+	if (!sat_faces(S, hb, ha, B.m_pos, B.m_quat, A.m_pos, A.m_quat, dc, &sep, &dmin)) return false;
+	if (!sat_edges(S, ha, hb, A.m_pos, A.m_quat, B.m_pos, B.m_quat, dc, &sep, &dmin)) return false;
+	F4 pts[CLIP_CAP];
+	int np = clip_hull_hull(S, sep, ha, hb, A.m_pos, A.m_quat, B.m_pos, B.m_quat, pts);
+	if (np <= 0) return false;
+	int idx[4] = {-1, -1, -1, -1};
+	int nr = extract_manifold(pts, np, neg4(sep), idx);
+	c->m_worldNormalOnB = sep;
+	c->m_restituitionCoeffCmp = (unsigned short)(0.f * 0xffff);
+	c->m_frictionCoeffCmp = (unsigned short)(0.7f * 0xffff);
+	c->m_batchIdx = k;
+	c->m_bodyAPtrAndSignBit = S.bodies[cp.x].m_invMass == 0 ? -cp.x : cp.x;
+	c->m_bodyBPtrAndSignBit = S.bodies[cp.y].m_invMass == 0 ? -cp.y : cp.y;
+	c->m_childIndexA = cp.z;
+	c->m_childIndexB = cp.w;
+	for (int i = 0; i < nr; i++) c->m_worldPosB[i] = pts[idx[i]];
+	c->m_worldNormalOnB.w = (float)nr;
+	return true;
+}
+
+extern "C" int b3o_narrowphase_ex(b3hInt4* pairs, int nPairs, const b3hRigidBodyData* bodies, const b3hCollidable* cols,
+								   const b3hConvexPolyhedronData* convexes, const b3hFloat4* vertices,
+								   const b3hFloat4* uniqueEdges, const b3hGpuFace* faces, const int* indices,
+								   b3hContact4Data* contacts, int capacity, int* hasSepOut, b3hFloat4* sepOut,
+								   const b3oCompoundTables* compound, b3hInt4* childPairsOut, int* numChildPairs)
+{
+	int nc = b3o_narrowphase(pairs, nPairs, bodies, cols, convexes, vertices, uniqueEdges, faces, indices, contacts,
+							 capacity, hasSepOut, sepOut);
+	if (numChildPairs) *numChildPairs = 0;
+	if (!compound || compound->capacity <= 0) return nc;
+	OShapes S = {bodies, cols, convexes, vertices, uniqueEdges, faces, indices};
+	const b3oCompoundTables& T = *compound;
+	std::vector<b3hInt4> cps;
+	for (int i = 0; i < nPairs; i++)
+	{
+		size_t k0 = cps.size();
+		compound_children(S, T, pairs[i].x, pairs[i].y, cps);
+		std::stable_sort(cps.begin() + (long)k0, cps.end(), [](const b3hInt4& u, const b3hInt4& v) {
+			if ((unsigned)u.z != (unsigned)v.z) return (unsigned)u.z < (unsigned)v.z;
+			return (unsigned)u.w < (unsigned)v.w;
+		});
+	}
+	if (numChildPairs) *numChildPairs = (int)cps.size();
+	int kept = (int)cps.size() < T.capacity ? (int)cps.size() : T.capacity;
+	for (int k = 0; k < kept; k++)
+	{
+		if (childPairsOut) childPairsOut[k] = cps[(size_t)k];
+		b3hContact4Data c;
+		if (!compound_pair(S, T, cps[(size_t)k], k, &c)) continue;
+		int slot = nc++;
+		if (slot < capacity) contacts[slot] = c;
+	}
+	return nc;
+}
+
 extern "C" int b3o_narrowphase(b3hInt4* pairs, int nPairs, const b3hRigidBodyData* bodies,
 								const b3hCollidable* cols, const b3hConvexPolyhedronData* convexes,
 								const b3hFloat4* vertices, const b3hFloat4* uniqueEdges, const b3hGpuFace* faces,

@@ -24,8 +24,8 @@ extern "C" int b3o_step(b3oWorld* w, float dt)
 	int nc = 0;
 	if (np)
 	{
-		nc = b3o_narrowphase(w->pairs, np, w->bodies, w->collidables, w->convexes, w->vertices, w->uniqueEdges,
-							 w->faces, w->indices, w->contacts, w->maxContacts, 0, 0);
+		nc = b3o_narrowphase_ex(w->pairs, np, w->bodies, w->collidables, w->convexes, w->vertices, w->uniqueEdges,
+								w->faces, w->indices, w->contacts, w->maxContacts, 0, 0, &w->compound, 0, 0);
 		if (nc > w->maxContacts) nc = w->maxContacts;
 	}
 	w->numContacts = nc;

@@ -48,6 +48,25 @@ def solver_params(dt=1.0 / 60.0, iterations=4, friction=0.7, drift=0.005, coeff=
     return SolverParams(dt, iterations, friction, drift, coeff, cell, damping, 0)
 
 
+class CompoundTables(ctypes.Structure):
+    _fields_ = [("children", ctypes.c_void_p), ("infos", ctypes.c_void_p), ("nodes", ctypes.c_void_p),
+                ("subtrees", ctypes.c_void_p), ("capacity", ctypes.c_int)]
+
+
+def compound_tables(tables, capacity):
+    """CompoundTables over the arrays of tables (keys children, infos, nodes,
+    subtrees; the arrays must outlive the struct)"""
+    t = CompoundTables()
+    if tables is None or capacity <= 0 or len(tables.get("children", ())) == 0:
+        return t
+    t.children = tables["children"].ctypes.data
+    t.infos = tables["infos"].ctypes.data
+    t.nodes = tables["nodes"].ctypes.data
+    t.subtrees = tables["subtrees"].ctypes.data
+    t.capacity = capacity
+    return t
+
+
 class World(ctypes.Structure):
     _fields_ = [("numBodies", ctypes.c_int), ("bodies", ctypes.c_void_p), ("inertias", ctypes.c_void_p),
                 ("collidables", ctypes.c_void_p), ("convexes", ctypes.c_void_p), ("vertices", ctypes.c_void_p),
@@ -59,7 +78,7 @@ class World(ctypes.Structure):
                 ("gravity", ctypes.c_float * 3), ("staticIdx", ctypes.c_int),
                 ("worldAabbs", ctypes.c_void_p), ("pairs", ctypes.c_void_p), ("contacts", ctypes.c_void_p),
                 ("numPairs", ctypes.c_int), ("numContacts", ctypes.c_int), ("residuals", ctypes.c_void_p),
-                ("constraints", ctypes.c_void_p), ("numConstraints", ctypes.c_int)]
+                ("constraints", ctypes.c_void_p), ("numConstraints", ctypes.c_int), ("compound", CompoundTables)]
 
 
 def world_aabbs(bodies, collidables, local_aabbs):
@@ -162,20 +181,29 @@ def mpr_contacts(pairs, bodies, collidables, convexes, vertices, capacity=None):
     return n, contacts[:min(n, capacity)].copy(), pairs
 
 
-def narrowphase(pairs, bodies, shapes, capacity=None, want_sep=False):
+def narrowphase(pairs, bodies, shapes, capacity=None, want_sep=False, compound=None, compound_capacity=0,
+                want_child_pairs=False):
+    """compound: dict of the compound tables (children, infos, nodes,
+    subtrees) with compound_capacity child pairs kept; want_child_pairs adds
+    the child-pair list to the result"""
     pairs = np.ascontiguousarray(pairs, np.int32).copy()
     if capacity is None:
         capacity = max(1, len(pairs))
     contacts = np.zeros((capacity, 112), np.uint8)
     has_sep = np.zeros(max(1, len(pairs)), np.int32) if want_sep else None
     sep = np.zeros((max(1, len(pairs)), 4), np.float32) if want_sep else None
-    n = lib().b3o_narrowphase(ptr(pairs), len(pairs), ptr(bodies), ptr(shapes["collidables"]),
-                              ptr(shapes["convexes"]), ptr(shapes["vertices"]), ptr(shapes["uniqueEdges"]),
-                              ptr(shapes["faces"]), ptr(shapes["indices"]), ptr(contacts), capacity,
-                              ptr(has_sep), ptr(sep))
+    ct = compound_tables(compound, compound_capacity)
+    child = np.zeros((max(1, compound_capacity), 4), np.int32)
+    nchild = ctypes.c_int(0)
+    n = lib().b3o_narrowphase_ex(ptr(pairs), len(pairs), ptr(bodies), ptr(shapes["collidables"]),
+                                 ptr(shapes["convexes"]), ptr(shapes["vertices"]), ptr(shapes["uniqueEdges"]),
+                                 ptr(shapes["faces"]), ptr(shapes["indices"]), ptr(contacts), capacity,
+                                 ptr(has_sep), ptr(sep), ctypes.byref(ct), ptr(child), ctypes.byref(nchild))
     res = (n, contacts[:min(n, capacity)].copy(), pairs)
     if want_sep:
         res = res + (has_sep[:len(pairs)], sep[:len(pairs)])
+    if want_child_pairs:
+        res = res + (child[:min(nchild.value, compound_capacity)].copy(), nchild.value)
     return res
 
 

@@ -100,6 +100,9 @@ class OracleWorld:
         self.joints = np.ascontiguousarray(world.constraints())
         w.constraints = self.joints.ctypes.data if len(self.joints) else None
         w.numConstraints = len(self.joints)
+        self.compound = world.compound_tables()
+        self.compound_capacity = cfg.m_compoundPairCapacity
+        w.compound = po.compound_tables(self.compound, self.compound_capacity)
         self.w = w
 
     def step(self, dt=1.0 / 60.0):

@@ -25,9 +25,48 @@ def quat_basis(q):
                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
 
 
+def compound_tables_host(kids, laabbs, children, infos, nodes, subtrees):
+    """registerCompoundShape's tables (b3GpuNarrowPhase.cpp:370-517) through the
+    library's host routine; appends to the table lists, returns (collidable,
+    local AABB)"""
+    n = len(kids)
+    ch = np.zeros(n, layout.CHILD_SHAPE)
+    for j, (sk, cpos, corn) in enumerate(kids):
+        ch["childPosition"][j, :3] = cpos
+        ch["childOrientation"][j] = corn
+        ch["shapeIndex"][j] = sk
+    child_local = np.array([laabbs[int(sk)] for sk, _, _ in kids], layout.AABB)
+    aabb = np.zeros(1, layout.AABB)
+    info = np.zeros(1, layout.BVH_INFO)
+    nd = np.zeros(2 * n, layout.BVH_NODE)
+    st = np.zeros(4 * n + 4, layout.BVH_SUBTREE)
+    nn, ns = ctypes.c_int(), ctypes.c_int()
+    capi.call("b3h_compute_compound_tables", capi.ptr(ch), n, capi.ptr(child_local), capi.ptr(aabb), capi.ptr(info),
+              capi.ptr(nd), len(nd), ctypes.byref(nn), capi.ptr(st), len(st), ctypes.byref(ns))
+    info["nodeOffset"] = len(nodes)
+    info["subTreeOffset"] = len(subtrees)
+    col = np.zeros(1, layout.COLLIDABLE)
+    col["numChildShapes"] = n
+    col["radius"] = np.array([len(infos)], np.int32).view(np.float32)
+    col["shapeType"] = 6
+    col["shapeIndex"] = len(children)
+    children.extend(ch)
+    infos.extend(info)
+    nodes.extend(nd[:nn.value])
+    subtrees.extend(st[:ns.value])
+    return col[0], aabb[0]
+
+
 def shape_tables(shapes):
     cols, convs, verts, edges, faces, idx, laabbs = [], [], [], [], [], [], []
+    children, infos, nodes, subtrees = [], [], [], []
     for s, pts in enumerate(shapes):
+        if isinstance(pts, dict) and "compound" in pts:
+            col, la = compound_tables_host(pts["compound"], laabbs, children, infos, nodes, subtrees)
+            cols.append(col)
+            convs.append(np.zeros(1, layout.CONVEX)[0])
+            laabbs.append(la)
+            continue
         if isinstance(pts, dict):
             # registerSphereShape / registerPlaneShape (b3GpuNarrowPhase.cpp:159-232)
             col = np.zeros(1, layout.COLLIDABLE)[0]
@@ -82,7 +121,9 @@ def shape_tables(shapes):
                 vertices=np.array(verts, np.float32).reshape(-1, 4),
                 uniqueEdges=np.array(edges, np.float32).reshape(-1, 4) if edges else np.zeros((1, 4), np.float32),
                 faces=np.array(faces, layout.FACE), indices=np.array(idx, np.int32),
-                localAabbs=np.array(laabbs, layout.AABB))
+                localAabbs=np.array(laabbs, layout.AABB),
+                compound=dict(children=np.array(children, layout.CHILD_SHAPE), infos=np.array(infos, layout.BVH_INFO),
+                              nodes=np.array(nodes, layout.BVH_NODE), subtrees=np.array(subtrees, layout.BVH_SUBTREE)))
 
 
 def bodies_and_inertias(scene, tables):
@@ -153,6 +194,8 @@ class HostWorld:
         w.pairs = self.pairs.ctypes.data
         w.contacts = self.contacts.ctypes.data
         w.residuals = self.resid.ctypes.data
+        self.compound_capacity = 1024 * 1024
+        w.compound = po.compound_tables(self.t["compound"], self.compound_capacity)
         self.w = w
 
     def step(self, dt=1.0 / 60.0):
