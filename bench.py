@@ -30,7 +30,9 @@ Also in the one JSON line:
   cpu_baseline  the oracle (the same algorithm in scalar C++) stepped from the
                 GPU's settled state: one thread (value) and all host threads.
   configs       the other workloads at their sizes: 1000 cubes, 1,048,576
-                AABBs (grid and SAP pairs/s), 4096 batched scenes.
+                ray casts into the pile, 1,048,576 AABBs (grid, SAP and BVH
+                pairs/s), 32768 bodies of mixed shapes (compounds, spheres, a
+                plane) with 2048 joints, 4096 batched scenes.
 """
 import argparse
 import ctypes
@@ -365,6 +367,64 @@ def sub_cubes(args):
             "cpu_port_one_thread": round(1000 * steps / csec, 1), "gpu_equals_oracle_after_200_steps": same}
 
 
+def sub_shapes(args, n=32768, joints=2048, settle=100, steps=100):
+    """the shapes beyond hulls in one scene: n dynamic bodies cycling through a
+    unit box, a 16-point hull, three compounds (two small boxes; a hull, a box
+    and a small box under child rotations; one box at the origin) and spheres,
+    on a ground box and a static plane, with point-to-point joints between
+    x-neighbours of the lattice; 10 PGS iterations.  settle graph-launched
+    steps, then body-steps/s over steps more; stage times on 5 eager steps;
+    the one-thread oracle (which restates the compound, primitive and joint
+    paths as well) on 1 step from the same state"""
+    from bullet3_amd import scenes
+    from bullet3_amd.capi import STAGE_NAMES
+    from bullet3_amd.world import World
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import pyoracle as po
+    from helpers import OracleWorld
+    sc = scenes.compound_scene(n=n, seed=11, ground_half=(40.0, 0.5, 40.0))
+    w = World(sc, broadphase="grid")
+    sp = w.solver_params()
+    sp.m_numIterations = 10
+    w.set_solver(sp)
+    # bodies 0, 1 are the ground box and the plane
+    jl = scenes.lattice_joints(n, joints)
+    for a, b, pa, pb in jl:
+        w.add_point2point(a, b, pa, pb)
+    made = len(jl)
+    w.step_graph(DT, settle)
+    w.sync()
+    t0 = time.perf_counter()
+    w.step_graph(DT, steps)
+    w.sync()
+    sec = time.perf_counter() - t0
+    w.enable_timing(True)
+    acc = np.zeros(len(STAGE_NAMES))
+    for _ in range(5):
+        w.step(DT)
+        acc += w.stage_times()
+    w.enable_timing(False)
+    c = w.contacts()
+    child = int(np.count_nonzero((c["childA"] >= 0) | (c["childB"] >= 0)))
+    out = {"workload": "%d dynamic bodies (unit boxes, 16-point hulls, 3 compound shapes, spheres) on a ground box "
+                       "and a plane, %d point-to-point joints, grid broadphase, 10 PGS iterations, settled %d steps"
+                       % (n, made, settle),
+           "bodies": n, "joints": made, "steps": steps, "value": round(n * steps / sec, 1), "unit": "body-steps/s",
+           "ms_per_step": round(1000 * sec / steps, 4), "pairs": len(w.pairs()), "contacts": len(c),
+           "compound_child_contacts": child,
+           "stages_ms": {k: round(float(v) / 5, 4) for k, v in zip(STAGE_NAMES, acc)}}
+    if not args.no_cpu:
+        ow = OracleWorld(w)
+        po.set_threads(1)
+        t0 = time.perf_counter()
+        ow.step(DT)
+        csec = time.perf_counter() - t0
+        out["cpu_port_one_thread"] = round(n / csec, 1)
+    w.close()
+    return out
+
+
 def sub_aabbs(args):
     """broadphase stress: 1,048,576 AABBs (half-extents log-uniform 0.05-1 m,
     about 8 partners each), grid (cell 2 m), SAP and the linear BVH; pairs/s
@@ -556,6 +616,8 @@ def main():
         out["configs"]["rays_1m"] = sub_rays(w)
         log("config: 1M AABBs")
         out["configs"]["aabbs_1m"] = sub_aabbs(args)
+        log("config: shapes")
+        out["configs"]["shapes_32k"] = sub_shapes(args)
         log("config: batched scenes")
         out["configs"]["batched_%d" % args.scenes], bw = sub_batched(args)
         extra.append(bw)

@@ -1896,16 +1896,20 @@ __device__ __forceinline__ void compound_side(const b3hShapeTables& sh, const b3
 	}
 }
 /* processCompoundPairsKernel (sat.cl:802-925) and clipCompoundsHullHullKernel
- * (satClipHullContacts.cl:1003-1130), one child pair per wavefront: faces of
+ * (satClipHullContacts.cl:1003-1130), one child pair per L lanes: faces of
  * A, faces of B, then every edge pair (the exact edge test whatever the edge
  * count), then clipping and reduction; the contact carries the child indices
  * and the child pair's index.  Child pairs that are not hull x hull get their
  * flag here (0) and their contact, if any, from k_compound_primitive. */
-__global__ __launch_bounds__(256, 4) void k_compound_sat(const b3hInt4* __restrict__ cpairs, const int* __restrict__ numPtr,
-														 const b3hRigidBodyData* __restrict__ bodies, b3hShapeTables sh,
-														 b3hContact4Data* __restrict__ staging, int* __restrict__ flags)
+template <int L, int THREADS>
+__global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_compound_sat(const b3hInt4* __restrict__ cpairs,
+																		  const int* __restrict__ numPtr,
+																		  const b3hRigidBodyData* __restrict__ bodies,
+																		  b3hShapeTables sh,
+																		  b3hContact4Data* __restrict__ staging,
+																		  int* __restrict__ flags)
 {
-	constexpr int L = 64, G = 4;
+	constexpr int G = THREADS / L;
 	__shared__ F4 sVerts[G][2][NP_VCAP];
 	__shared__ F4 sClip[G][3][CLIP_CAP];
 	const Group<L> g;
@@ -2173,8 +2177,11 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 		b3h_exclusive_scan(cw.counts, cw.offsets, numPairs, maxPairs, cw.scanTmp, s);
 		hipLaunchKernelGGL(k_compound_emit, dim3(pb), dim3(256), 0, s, pairs, numPairs, bodies, sh,
 						   (const int*)cw.offsets, cw.cap, cw.cpairs, cw.num);
-		hipLaunchKernelGGL(k_compound_sat, dim3(blocksFor(cw.cap, 4, 256)), dim3(256), 0, s, (const b3hInt4*)cw.cpairs,
-						   (const int*)cw.num, bodies, sh, staging + cw.stagingBase, cw.flags);
+		/* two child pairs per wavefront: most children are boxes, whose
+		 * families fill a fraction of 64 lanes */
+		hipLaunchKernelGGL((k_compound_sat<32, 256>), dim3(blocksFor(cw.cap, 8, 256)), dim3(256), 0, s,
+						   (const b3hInt4*)cw.cpairs, (const int*)cw.num, bodies, sh, staging + cw.stagingBase,
+						   cw.flags);
 		if (hasPrimitives)
 		{
 			int cb = (cw.cap + PC_THREADS - 1) / PC_THREADS;

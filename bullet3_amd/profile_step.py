@@ -1,4 +1,5 @@
-"""Steps the 131072-body pile for hardware-counter runs (rocprofv3 --pmc).
+"""Steps the 131072-body pile (or, with --scene shapes, the mixed-shape scene)
+for profiler runs (rocprofv3 --kernel-trace, --pmc).
 
     python3 -m bullet3_amd.profile_step --warmup 300 --save-state /tmp/pile300.npy
     rocprofv3 --pmc FETCH_SIZE --kernel-include-regex "k_pgs_level|k_narrowphase|k_np_mpr|copyBuffer" \\
@@ -33,8 +34,16 @@ def main():
     ap.add_argument("--steps", type=int, default=3, help="eager steps (after --state)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=64, help="narrowphase lanes per pair (64, 32, 16)")
+    ap.add_argument("--scene", default="pile", choices=["pile", "shapes"],
+                    help="pile: the convex pile of nx x ny x nz bodies; shapes: 32768 bodies of boxes, hulls, "
+                         "compounds and spheres on a ground box and a plane, with 2048 point-to-point joints")
     args = ap.parse_args()
-    w = World(scenes.convex_pile(args.nx, args.ny, args.nz), broadphase="grid")
+    if args.scene == "shapes":
+        w = World(scenes.compound_scene(n=32768, seed=11, ground_half=(40.0, 0.5, 40.0)), broadphase="grid")
+        for a, b, pa, pb in scenes.lattice_joints(32768, 2048):
+            w.add_point2point(a, b, pa, pb)
+    else:
+        w = World(scenes.convex_pile(args.nx, args.ny, args.nz), broadphase="grid")
     capi.call("b3h_np_set_pairs_per_group", w.np_, args.lanes)
     sp = w.solver_params()
     sp.m_numIterations = args.iters

@@ -172,6 +172,21 @@ def compound_scene(n=64, seed=7, plane=True, sphere=True, ground_half=(12.0, 0.5
                         np.array(shape, np.int32), ground_half, cell_size=4.5)
 
 
+def lattice_joints(n, count, first=2, spacing=1.4):
+    """point-to-point joints between x-neighbours of compound_scene's lattice
+    (dynamic body k is body first + k; neighbours sit spacing apart in x):
+    up to count (a, b, pivot in a, pivot in b) tuples, pivots at the midpoint"""
+    side = max(2, int(math.ceil(n ** (1.0 / 3.0))))
+    out = []
+    h = 0.5 * spacing
+    for k in range(n - 1):
+        if len(out) >= count:
+            break
+        if k % side != side - 1:
+            out.append((first + k, first + k + 1, (h, 0.0, 0.0), (-h, 0.0, 0.0)))
+    return out
+
+
 def aabb_cloud(n=1 << 20, mean_overlap=8.0, hmin=0.05, hmax=1.0, seed=3, side=None):
     """Broadphase stress: n cubic AABBs, half-extents log-uniform [hmin,hmax],
     centres uniform in a cube whose side gives about `mean_overlap` partners
