@@ -1260,13 +1260,14 @@ __global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_sat_face_pass(
 /* pass 2: the edge family (exact edge pairs, sat.cl:719-798, or the 162 fixed
  * directions, mpr.cl:180-311) of the listed pairs (those the face pass left
  * open), one pair per wavefront; each sets its clip flag (1: clip it) */
-__global__ __launch_bounds__(256, 4) void k_sat_edge_pass(
+template <int L, int THREADS>
+__global__ __launch_bounds__(THREADS, 1024 / THREADS) void k_sat_edge_pass(
 	const b3hInt4* __restrict__ pairs, b3hShapeTables sh, b3hContact4Data* __restrict__ staging, int* __restrict__ flags,
 	int* __restrict__ sepStateOut, b3hFloat4* __restrict__ sepOut, const F4* __restrict__ mprIn, b3hSatCache sc,
 	const float* __restrict__ dminIn, const int* __restrict__ listCount, const int* __restrict__ list,
 	int* __restrict__ clipFlag)
 {
-	constexpr int L = 64, G = 4;
+	constexpr int G = THREADS / L;
 	__shared__ F4 sVerts[G][2][NP_VCAP];
 	const Group<L> g;
 	const int gi = threadIdx.x / L;
@@ -1277,15 +1278,13 @@ __global__ __launch_bounds__(256, 4) void k_sat_edge_pass(
 		b3hInt4 pr = pairs[i];
 		int a = pr.x, b = pr.y;
 		const b3hSatBody A = sc.desc[a], B = sc.desc[b];
-		PairGeom P, Q;
-		stage_pair(g, sh, A, B, sVerts[gi][0], sVerts[gi][1], P);
-		sat_frame(P, Q);
 		F4 dc = sub4(A.center, B.center);
 		F4 sep = sepOut[i];
 		float dm = dminIn[i];
 		F4 s2 = sep;
 		int nEE = A.nE * B.nE;
 		Family fe;
+		bool cached = false;
 		if (nEE <= B3_UNIT_SPHERE_162_COUNT)
 			fe = {2, nEE, nullptr, sh.uniqueEdges + A.eOff, sh.uniqueEdges + B.eOff, B.nE, nullptr, nullptr, nullptr};
 		else
@@ -1293,7 +1292,26 @@ __global__ __launch_bounds__(256, 4) void k_sat_edge_pass(
 			const float2* dA = body_dirs(sc, A);
 			const float2* dB = body_dirs(sc, B);
 			fe = {3, B3_UNIT_SPHERE_162_COUNT, nullptr, nullptr, nullptr, 1, nullptr, dB ? dA : nullptr, dB};
+			cached = dA && dB;
 		}
+		/* the vertices are staged only when an axis needs projecting: the
+		 * fixed directions with both hulls' intervals cached read none */
+		PairGeom P, Q;
+		if (cached)
+		{
+			P.vA = sh.vertices + A.vOff;
+			P.vB = sh.vertices + B.vOff;
+			P.ldsA = P.ldsB = false;
+			P.nA = A.nV;
+			P.nB = B.nV;
+			P.posA = A.pos;
+			P.posB = B.pos;
+			P.ornA = A.quat;
+			P.ornB = B.quat;
+		}
+		else
+			stage_pair(g, sh, A, B, sVerts[gi][0], sVerts[gi][1], P);
+		sat_frame(P, Q);
 		bool open = sat_family(g, fe, Q, dc, &s2, &dm);
 		if (g.lane == 0)
 		{
@@ -2151,7 +2169,7 @@ void b3h_narrowphase(b3hInt4* pairs, const int* numPairs, int maxPairs, const b3
 	int lb = blocksFor(maxPairs, 256, 256);
 	b3h_exclusive_scan(work.flag, work.scan, numPairs, maxPairs, scanTmp, s);
 	hipLaunchKernelGGL(k_pair_list, dim3(lb), dim3(256), 0, s, numPairs, work.flag, work.scan, work.list, work.count);
-	hipLaunchKernelGGL(k_sat_edge_pass, dim3(blocksFor(maxPairs, 4, 256)), dim3(256), 0, s, pairs, sh, staging, flags,
+	hipLaunchKernelGGL((k_sat_edge_pass<32, 256>), dim3(blocksFor(maxPairs, 8, 256)), dim3(256), 0, s, pairs, sh, staging, flags,
 					   sepState, sepNormal, (const F4*)mprScratch, sc, (const float*)work.dmin, (const int*)work.count,
 					   (const int*)work.list, work.flag);
 	/* pass 3, clipping, over the pairs pass 2 flagged (the flags of pairs it
