@@ -23,7 +23,8 @@ Also in the one JSON line:
                 steps right after the timed region.  traffic: measured HBM
                 bytes per launch of that kernel from profiles/traffic.json,
                 used only when that file was measured at this run's state
-                (bodies, settle steps, lanes per pair), else null.
+                (bodies, graph-launched steps before the stage timing, lanes
+                per pair: 620 for the defaults), else null.
   stages_ms     per-stage HIP-event times of the same eager steps.
   warm_start    PGS iterations to reach a residual, learned warm start
                 against plain PGS (cold, as the reference, and previous-frame).
@@ -159,7 +160,7 @@ def run_pile(args):
     achieved = kb[dom] / (avg_ms * 1e-3) / 1e9
     total_bytes = step_bytes(sc.num_bodies, n_pairs, n_contacts, args.iters)
     lanes_now = tuned["lanes_per_pair"] if tuned else (args.lanes or 64)
-    traffic, traffic_note = measured_traffic(dom, n_dyn, args.settle, lanes_now)
+    traffic, traffic_note = measured_traffic(dom, n_dyn, args.settle + args.warmup + args.steps, lanes_now)
     out = {
         "metric": "body-steps/s at 128k convexes; PGS iters to residual 1e-4, learned vs plain",
         "value": round(value, 1),
@@ -197,16 +198,17 @@ def run_pile(args):
     return out, w, sc
 
 
-def measured_traffic(kernel, bodies, settle, lanes):
+def measured_traffic(kernel, bodies, steps_before, lanes):
     """HBM bytes per launch from profiles/traffic.json when it was measured at
-    this state and launch shape, else (None, reason)"""
+    this state (the pile after the same number of graph-launched steps) and
+    launch shape, else (None, reason)"""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None, "no measurement"
     with open(path) as f:
         t = json.load(f)
     s = t.get("state", {})
-    if s.get("bodies") != bodies or s.get("settle_steps") != settle or s.get("lanes_per_pair") != lanes:
+    if s.get("bodies") != bodies or s.get("steps_before") != steps_before or s.get("lanes_per_pair") != lanes:
         return None, "profiles/traffic.json was measured at another state or lane count (%s)" % s
     k = t.get("kernels", {}).get(kernel)
     if not k:
