@@ -264,7 +264,7 @@ extern "C" int b3h_np_create(const b3hConfig* cfg, void* stream, b3hNarrowPhase*
 	np->satValid = false;
 	np->satNumConvex = 0;
 	np->contactsValid = true;
-	np->lanesPerPair = 64;
+	np->lanesPerPair = 32;
 	if ((r = make_stream(stream, &np->stream, &np->ownStream)) || (r = np_alloc_work(np)) || (r = b3h_np_module_init()))
 	{
 		b3h_np_destroy(np);

@@ -33,7 +33,7 @@ def main():
     ap.add_argument("--state", default=None, help="bodies saved by --save-state")
     ap.add_argument("--steps", type=int, default=3, help="eager steps (after --state)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0)
-    ap.add_argument("--lanes", type=int, default=64, help="narrowphase lanes per pair (64, 32, 16)")
+    ap.add_argument("--lanes", type=int, default=32, help="narrowphase lanes per pair (64, 32, 16)")
     ap.add_argument("--scene", default="pile", choices=["pile", "shapes"],
                     help="pile: the convex pile of nx x ny x nz bodies; shapes: 32768 bodies of boxes, hulls, "
                          "compounds and spheres on a ground box and a plane, with 2048 point-to-point joints")

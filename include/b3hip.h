@@ -144,10 +144,10 @@ int b3h_np_load_shape_tables(b3hNarrowPhase* np, const int* counts7, const b3hCo
 							 const b3hAabb* localAabbs);
 /* Host copy of the registered inertias. */
 int b3h_np_get_inertias_cpu(b3hNarrowPhase* np, b3hInertiaData* dst, int n);
-/* Narrowphase tuning: lanes cooperating on one pair in the SAT, clipping and
- * reduction kernel: 64 (default, one pair per wavefront), 32 or 16 (two or four
- * pairs per wavefront); other powers of two up to 64 select 64.  Results do
- * not depend on it. */
+/* Narrowphase tuning: lanes cooperating on one pair in the SAT face pass: 32
+ * (default, two pairs per wavefront), 64 or 16 (one or four pairs per
+ * wavefront); other powers of two up to 64 select 64.  Results do not depend
+ * on it. */
 int b3h_np_set_pairs_per_group(b3hNarrowPhase* np, int lanesPerPair);
 /* Per-pair MPR result of the last computeContacts (device b3hFloat4[3 * numPairs]:
  * direction, point, (depth, 1 if MPR found penetration else 0, 0, 0)); the
